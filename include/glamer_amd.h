@@ -1,0 +1,557 @@
+// glamer_amd.h -- drop-in C++ front end of the MI355X particle ray shooter.
+//
+// Header-only layer over the C ABI in slsim_amd.h that keeps the names,
+// signatures, argument meaning and error behaviour of GLAMER's particle
+// ray-shooting API, so that user code written against
+//   ParticleType<T>            include/particle_types.h:46-58
+//   Point_2d, Matrix2x2, Point, LinkedPoint, RAY   include/point.h:48-621
+//   COSMOLOGY (flat LCDM part)  CosmoLib, used at MultiPlane/lens.cpp:776,2489
+//   LensHaloParticles<PType>   include/particle_halo.h:58-113, 341-434
+//   Lens                       include/lens.h:75-349, MultiPlane/lens.cpp
+//   GridMap                    include/gridmap.h:31-180, TreeCode_link/gridmap.cpp
+// compiles against this header and runs the rays on the GPU.  Link with
+// libslsim_amd.so.  Reference behaviours kept on purpose are marked
+// "(reference)".  Deliberate differences: a non-flat cosmology throws
+// std::invalid_argument instead of exit(1); moveinMainHalo always adds the
+// halo as addMainHaloToPlane does; LensHaloParticles takes an optional
+// trailing opening angle (the reference fixes 0.1).
+#pragma once
+
+#include <cmath>
+#include <cstddef>
+#include <memory>
+#include <type_traits>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "slsim_amd.h"
+
+typedef double PosType;
+typedef double KappaType;
+
+#ifndef arcsecTOradians
+#define arcsecTOradians 0.000004848136811   // include/standard.h:51 (reference)
+#endif
+
+namespace slsim_cpp {
+
+inline void check(int rc) {
+  if (rc == SLSIM_OK) return;
+  std::string msg = slsim_last_error();
+  if (rc == SLSIM_ERR_ARG || rc == SLSIM_ERR_SAME_PLACE || rc == SLSIM_ERR_NOT_FLAT)
+    throw std::invalid_argument(msg);
+  throw std::runtime_error(msg);
+}
+
+// One library context per process, on the device chosen before first use.
+inline int& device_index() {
+  static int d = 0;
+  return d;
+}
+inline slsim_ctx context() {
+  static slsim_ctx ctx = [] {
+    slsim_ctx c = nullptr;
+    check(slsim_ctx_create(device_index(), 0, &c));
+    return c;
+  }();
+  return ctx;
+}
+
+// Precision of the tree walk used by Lens::rayshooterInternal and GridMap.
+inline int& precision() {
+  static int p = SLSIM_FP64;
+  return p;
+}
+
+}  // namespace slsim_cpp
+
+// ------------------------------------------------------------------- points
+struct Point_2d {
+  Point_2d() { x[0] = x[1] = 0.0; }
+  Point_2d(double x1, double x2) { x[0] = x1; x[1] = x2; }
+  Point_2d(const PosType* p) { x[0] = p[0]; x[1] = p[1]; }
+  double& operator[](int i) { return x[i]; }
+  double operator[](int i) const { return x[i]; }
+  Point_2d operator-(const Point_2d& p) const { return Point_2d(x[0] - p.x[0], x[1] - p.x[1]); }
+  Point_2d operator+(const Point_2d& p) const { return Point_2d(x[0] + p.x[0], x[1] + p.x[1]); }
+  double length() const { return std::sqrt(x[0] * x[0] + x[1] * x[1]); }
+  double x[2];
+};
+
+template <typename T = double>
+struct Matrix2x2 {
+  T a[4];
+  T& operator[](int i) { return a[i]; }
+  T operator[](int i) const { return a[i]; }
+  void setToI() { a[0] = a[3] = 1; a[1] = a[2] = 0; }
+  static Matrix2x2<T> I() { Matrix2x2<T> m; m.setToI(); return m; }
+  T det() const { return a[0] * a[3] - a[1] * a[2]; }
+  // point.h:377-381 (reference, "GLAMER II" convention)
+  T kappa() const { return 1 - (a[0] + a[3]) / 2; }
+  T gamma1() const { return (a[0] - a[3]) / 2; }
+  T gamma2() const { return (a[1] + a[2]) / 2; }
+  T gamma3() const { return (a[2] - a[1]) / 2; }
+  Matrix2x2<T> operator*(const Matrix2x2<T>& F) const {
+    Matrix2x2<T> m;
+    m.a[0] = a[0] * F.a[0] + a[1] * F.a[2];
+    m.a[1] = a[0] * F.a[1] + a[1] * F.a[3];
+    m.a[2] = a[2] * F.a[0] + a[3] * F.a[2];
+    m.a[3] = a[2] * F.a[1] + a[3] * F.a[3];
+    return m;
+  }
+  Matrix2x2<T> operator*(T f) const {
+    Matrix2x2<T> m = *this;
+    for (auto& v : m.a) v *= f;
+    return m;
+  }
+};
+
+enum Boo { NO, YES, MAYBE };
+
+struct Point : public Point_2d {
+  Point() { A.setToI(); }
+  Point(PosType x1, PosType x2) : Point_2d(x1, x2) { A.setToI(); }
+  Point* next = nullptr;
+  Point* prev = nullptr;
+  Point* image = nullptr;
+  unsigned long id = 0;
+  unsigned long head = 0;
+  Boo in_image = NO;
+  double dt = 0;
+  Matrix2x2<KappaType> A;
+  double gridsize = 0;
+  float surface_brightness = 0;
+  bool flag = false;
+  PosType* ptr_y() { return image->x; }
+  KappaType invmag() const { return A.det(); }
+  KappaType kappa() const { return A.kappa(); }
+  KappaType gamma1() const { return A.gamma1(); }
+  KappaType gamma2() const { return A.gamma2(); }
+  KappaType gamma3() const { return A.gamma3(); }
+};
+
+struct LinkedPoint : public Point {
+  LinkedPoint() { image = &im; im.image = this; }
+  LinkedPoint(const LinkedPoint& o) : Point(o), im(o.im) { image = &im; im.image = this; }
+  Point_2d& y() { return im; }
+ private:
+  Point im;
+};
+
+struct RAY {
+  RAY() { A.setToI(); }
+  RAY(const Point& p, double zs = -1) : x(p.x[0], p.x[1]), y(p.image->x[0], p.image->x[1]),
+                                         A(p.A), dt(p.dt), z(zs) {}
+  Point_2d x;
+  Point_2d y;
+  Matrix2x2<KappaType> A;
+  KappaType dt = 0;
+  KappaType z = -1;
+  PosType* ptr_y() { return y.x; }
+  KappaType invmag() const { return A.det(); }
+  KappaType kappa() const { return A.kappa(); }
+  KappaType gamma1() const { return A.gamma1(); }
+  KappaType gamma2() const { return A.gamma2(); }
+  KappaType gamma3() const { return A.gamma3(); }
+  Point_2d alpha() const { return x - y; }
+};
+
+template <typename T = float>
+struct ParticleType {
+  T& operator[](int i) { return x[i]; }
+  T* operator*() { return x; }
+  T x[3];
+  float Mass;
+  float Size;
+  int type;
+  float size() { return Size; }
+  float mass() { return Mass; }
+};
+
+enum class LensingVariable { DELAYT, ALPHA, ALPHA1, ALPHA2, KAPPA, GAMMA, GAMMA1, GAMMA2,
+                             GAMMA3, INVMAG, PHI, SurfBrightness };
+
+// --------------------------------------------------------------- cosmology
+// The flat-LCDM distances the particle path uses (CosmoLib's COSMOLOGY).
+class COSMOLOGY {
+ public:
+  COSMOLOGY(double Omega_matter = 0.3, double Omega_lambda = 0.7, double h = 0.7)
+      : Om(Omega_matter), OL(Omega_lambda), hh(h) {}
+  double getOmega_matter() const { return Om; }
+  double getOmega_lambda() const { return OL; }
+  double gethubble() const { return hh; }
+  double coorDist(double z1, double z2) const { return slsim_coordist(Om, OL, hh, z1, z2); }
+  double coorDist(double z) const { return coorDist(0, z); }
+  double angDist(double z) const { return coorDist(0, z) / (1 + z); }
+  double invCoorDist(double D) const { return slsim_inv_coordist(Om, OL, hh, D); }
+ private:
+  double Om, OL, hh;
+};
+
+// ------------------------------------------------------ particle lens halo
+template <typename PType>
+class LensHaloParticles {
+ public:
+  // particle_halo.h:58-75: takes possession of the particles and builds the
+  // device quadtree (bucket 20, opening angle theta_force).
+  LensHaloParticles(std::vector<PType>& pvector, float redshift, const COSMOLOGY& cosmo,
+                    Point_2d theta_rotate, bool recenter, double my_inv_area,
+                    float MinPSize = 0, double max_range = -1, bool verbose = false,
+                    double theta_force = 0.1)
+      : zlens(redshift), inv_area(my_inv_area), min_size(MinPSize) {
+    (void)cosmo; (void)max_range; (void)verbose;
+    std::swap(pvector, trash_collector);
+    set_up(theta_rotate, recenter, theta_force);
+  }
+  LensHaloParticles(LensHaloParticles&& h) noexcept
+      : zlens(h.zlens), inv_area(h.inv_area), min_size(h.min_size), mass(h.mass),
+        trash_collector(std::move(h.trash_collector)), plane(h.plane) {
+    for (int i = 0; i < 3; ++i) mcenter[i] = h.mcenter[i];
+    h.plane = nullptr;
+  }
+  LensHaloParticles(const LensHaloParticles&) = delete;
+  ~LensHaloParticles() {
+    if (plane) slsim_plane_destroy(plane);
+  }
+  size_t getN() const { return trash_collector.size(); }
+  float getZlens() const { return zlens; }
+  double get_mass() const { return mass; }
+  PType& operator[](size_t i) { return trash_collector[i]; }
+  slsim_plane release_plane() {
+    slsim_plane p = plane;
+    plane = nullptr;
+    return p;
+  }
+
+ private:
+  // set_up, particle_halo.h:341-417
+  void set_up(Point_2d theta_rotate, bool recenter, double theta_force) {
+    std::vector<PType>& pp = trash_collector;
+    const size_t n = pp.size();
+    mcenter[0] = mcenter[1] = mcenter[2] = 0;
+    mass = 0;
+    for (size_t i = 0; i < n; ++i) {
+      for (int j = 0; j < 3; ++j) mcenter[j] += pp[i][j] * pp[i].mass();
+      mass += pp[i].mass();
+    }
+    for (int j = 0; j < 3; ++j) mcenter[j] /= mass;
+    if (recenter) {
+      // (reference) the centre is zeroed inside the loop, so only the first
+      // particle moves (particle_halo.h:395-411)
+      for (size_t i = 0; i < n; ++i) {
+        for (int j = 0; j < 3; ++j) pp[i][j] -= mcenter[j];
+        mcenter[0] = mcenter[1] = mcenter[2] = 0;
+      }
+    }
+    rotate_particles(theta_rotate[0], theta_rotate[1]);
+    std::vector<float> m(n), h(n);
+    for (size_t i = 0; i < n; ++i) {
+      m[i] = pp[i].mass();
+      h[i] = pp[i].size();
+    }
+    using T = typename std::remove_reference<decltype(pp[0][0])>::type;
+    std::vector<T> x(n), y(n);
+    for (size_t i = 0; i < n; ++i) { x[i] = pp[i][0]; y[i] = pp[i][1]; }
+    build<T>(x, y, m, h, theta_force);
+  }
+  template <typename T>
+  void build(const std::vector<T>& x, const std::vector<T>& y, const std::vector<float>& m,
+             const std::vector<float>& h, double theta_force) {
+    if constexpr (sizeof(T) == sizeof(float))
+      slsim_cpp::check(slsim_plane_create_f32(slsim_cpp::context(), (const float*)x.data(),
+                                              (const float*)y.data(), m.data(), h.data(),
+                                              x.size(), theta_force, 20, inv_area, &plane));
+    else
+      slsim_cpp::check(slsim_plane_create_f64(slsim_cpp::context(), (const double*)x.data(),
+                                              (const double*)y.data(), m.data(), h.data(),
+                                              x.size(), theta_force, 20, inv_area, &plane));
+  }
+  // rotate_particles, particle_halo.h:634-680
+  void rotate_particles(PosType tx, PosType ty) {
+    if (tx == 0.0 && ty == 0.0) return;
+    const double cx = std::cos(tx), sx = std::sin(tx), cy = std::cos(ty), sy = std::sin(ty);
+    const double c[3][3] = {{cy, 0, -sy}, {-sy * sx, cx, -cy * sx}, {cx, sx, cy * cx}};
+    for (auto& p : trash_collector) {
+      double t[3] = {0, 0, 0};
+      for (int j = 0; j < 3; ++j)
+        for (int r = 0; r < 3; ++r) t[r] += c[r][j] * p[j];
+      for (int j = 0; j < 3; ++j) p[j] = t[j];
+    }
+  }
+
+  float zlens;
+  double inv_area;
+  float min_size;
+  double mass = 0;
+  double mcenter[3] = {0, 0, 0};
+  std::vector<PType> trash_collector;
+  slsim_plane plane = nullptr;
+};
+
+// --------------------------------------------------------------------- lens
+class Lens {
+ public:
+  Lens(long* my_seed, PosType z_source, const COSMOLOGY& cosmo, bool verbose = false)
+      : seed(my_seed), cosmo_(cosmo) {
+    (void)verbose;
+    slsim_cpp::check(slsim_lens_create(slsim_cpp::context(), z_source, cosmo.getOmega_matter(),
+                                       cosmo.getOmega_lambda(), cosmo.gethubble(), &lens));
+  }
+  ~Lens() {
+    if (lens) slsim_lens_destroy(lens);
+  }
+  Lens(const Lens&) = delete;
+
+  // include/lens.h:172-185: the lens takes possession of the halo.
+  template <typename T>
+  void moveinMainHalo(T& halo_in, bool addplanes, bool verbose = false) {
+    (void)addplanes; (void)verbose;
+    T* halo = new T(std::move(halo_in));
+    owned.emplace_back(halo, [](void* p) { delete static_cast<T*>(p); });
+    slsim_cpp::check(slsim_lens_add_halo(lens, halo->release_plane(), halo->getZlens()));
+  }
+
+  int getNplanes() const {
+    int n = 0;
+    slsim_cpp::check(slsim_lens_tables(lens, &n, nullptr, nullptr, nullptr, nullptr));
+    return n;
+  }
+  PosType getSourceZ() const {
+    int n = getNplanes();
+    std::vector<double> z(n + 1);
+    slsim_cpp::check(slsim_lens_tables(lens, &n, z.data(), nullptr, nullptr, nullptr));
+    return toggle ? zs_implant : z.back();
+  }
+  // MultiPlane/lens.cpp:2609-2657 (nearest = false)
+  short ResetSourcePlane(PosType z, bool nearest = false, bool verbose = false) {
+    (void)nearest; (void)verbose;
+    if (z <= 0.0) return 0;
+    slsim_cpp::check(slsim_lens_reset_source(lens, z));
+    toggle = true;
+    zs_implant = z;
+    int n = getNplanes();
+    std::vector<double> zp(n + 1);
+    slsim_cpp::check(slsim_lens_tables(lens, &n, zp.data(), nullptr, nullptr, nullptr));
+    short j = 0;
+    while (j < n && zp[j] < z) ++j;
+    return j;
+  }
+  void RevertSourcePlane() {
+    slsim_cpp::check(slsim_lens_reset_source(lens, -1));
+    toggle = false;
+  }
+
+  // FullRange/internal_rayshooter_multi.cpp:53-118
+  void rayshooterInternal(unsigned long Npoints, Point* i_points, bool RSIverbose = false) {
+    (void)RSIverbose;
+    std::vector<size_t> idx;
+    std::vector<double> x;
+    for (unsigned long i = 0; i < Npoints; ++i) {
+      if (i_points[i].in_image == MAYBE) continue;
+      idx.push_back(i);
+      x.push_back(i_points[i].x[0]);
+      x.push_back(i_points[i].x[1]);
+    }
+    std::vector<double> y(x.size()), A(2 * x.size());
+    shoot(idx.size(), x.data(), nullptr, y.data(), A.data());
+    for (size_t k = 0; k < idx.size(); ++k) {
+      Point& p = i_points[idx[k]];
+      p.image->x[0] = y[2 * k];
+      p.image->x[1] = y[2 * k + 1];
+      for (int c = 0; c < 4; ++c) p.A.a[c] = A[4 * k + c];
+      p.dt = 0;
+      p.image->A = p.A;
+      p.image->dt = 0;
+    }
+  }
+  // internal_rayshooter_multi.cpp:120-182
+  void rayshooterInternal(unsigned long Npoints, LinkedPoint* i_points,
+                          std::vector<double>& source_zs, bool RSIverbose = false) {
+    (void)RSIverbose;
+    if (source_zs.size() != Npoints) throw std::invalid_argument("missing redshifts");
+    std::vector<double> x(2 * Npoints), y(2 * Npoints), A(4 * Npoints);
+    for (unsigned long i = 0; i < Npoints; ++i) {
+      x[2 * i] = i_points[i].x[0];
+      x[2 * i + 1] = i_points[i].x[1];
+    }
+    shoot(Npoints, x.data(), source_zs.data(), y.data(), A.data());
+    for (unsigned long i = 0; i < Npoints; ++i) {
+      LinkedPoint& p = i_points[i];
+      p.image->x[0] = y[2 * i];
+      p.image->x[1] = y[2 * i + 1];
+      for (int c = 0; c < 4; ++c) p.A.a[c] = A[4 * i + c];
+      p.dt = 0;
+      p.image->A = p.A;
+      p.image->dt = 0;
+    }
+  }
+  // internal_rayshooter_multi.cpp:184-235: each ray carries its source z
+  void rayshooterInternal(unsigned long Npoints, RAY* rays) {
+    std::vector<double> x(2 * Npoints), zs(Npoints), y(2 * Npoints), A(4 * Npoints);
+    for (unsigned long i = 0; i < Npoints; ++i) {
+      x[2 * i] = rays[i].x[0];
+      x[2 * i + 1] = rays[i].x[1];
+      zs[i] = rays[i].z;
+    }
+    shoot(Npoints, x.data(), zs.data(), y.data(), A.data());
+    for (unsigned long i = 0; i < Npoints; ++i) {
+      rays[i].y[0] = y[2 * i];
+      rays[i].y[1] = y[2 * i + 1];
+      for (int c = 0; c < 4; ++c) rays[i].A.a[c] = A[4 * i + c];
+      rays[i].dt = 0;
+    }
+  }
+  void rayshooterInternal(RAY& ray) { rayshooterInternal(1, &ray); }
+  // internal_rayshooter_multi.cpp:40-50: a single ray to the lens's source
+  void rayshooter(RAY& ray) {
+    LinkedPoint p;
+    p.x[0] = ray.x[0];
+    p.x[1] = ray.x[1];
+    rayshooterInternal(1, static_cast<Point*>(&p));
+    ray = RAY(p, -1);
+  }
+
+  slsim_lens handle() const { return lens; }
+  const COSMOLOGY& getCosmo() const { return cosmo_; }
+
+ private:
+  void shoot(size_t n, const double* x, const double* zs, double* y, double* A) {
+    if (n == 0) return;
+    slsim_cpp::check(slsim_shoot(lens, n, x, zs, y, A, slsim_cpp::precision(), nullptr));
+  }
+  long* seed;
+  COSMOLOGY cosmo_;
+  slsim_lens lens = nullptr;
+  bool toggle = false;
+  double zs_implant = 0;
+  std::vector<std::unique_ptr<void, void (*)(void*)>> owned;
+};
+typedef Lens* LensHndl;
+
+// ---------------------------------------------------------------- pixel map
+template <typename T>
+class PixelMap {
+ public:
+  PixelMap(const double center[2], size_t Nx, size_t Ny, double resolution)
+      : Nx_(Nx), Ny_(Ny), res(resolution), map(Nx * Ny) {
+    c[0] = center[0];
+    c[1] = center[1];
+  }
+  T& operator[](size_t i) { return map[i]; }
+  size_t size() const { return map.size(); }
+  size_t getNx() const { return Nx_; }
+  size_t getNy() const { return Ny_; }
+  double getResolution() const { return res; }
+  const std::vector<T>& data() const { return map; }
+ private:
+  size_t Nx_, Ny_;
+  double res;
+  double c[2];
+  std::vector<T> map;
+};
+
+// ------------------------------------------------------------------ gridmap
+struct GridMap {
+  // TreeCode_link/gridmap.cpp:93-120: N1d x N1d rays over `range`, x fastest;
+  // the grid is generated and shot on the device.
+  GridMap(LensHndl lens, unsigned long N1d, const double center_[2], double range)
+      : Ngrid_init(N1d), Ngrid_init2(N1d), x_range(range) {
+    if (N1d < 1) throw std::runtime_error("GridMap size is < 1!");
+    if (range <= 0) throw std::runtime_error("GridMap range is <= 0!");
+    center[0] = center_[0];
+    center[1] = center_[1];
+    const size_t n = (size_t)N1d * N1d;
+    i_points.resize(n);
+    s_points.resize(n);
+    i_points[0].head = n;
+    s_points[0].head = n;
+    for (size_t i = 0; i < n; ++i) {
+      i_points[i].id = i;
+      i_points[i].gridsize = N1d > 1 ? range / (N1d - 1) : range;
+      if (N1d == 1) {
+        i_points[i].x[0] = center[0];
+        i_points[i].x[1] = center[1];
+      } else {
+        i_points[i].x[0] = center[0] + range * (1.0 * (i % N1d) / (N1d - 1) - 0.5);
+        i_points[i].x[1] = center[1] + range * (1.0 * (i / N1d) / (N1d - 1) - 0.5);
+      }
+      s_points[i].id = i;
+      i_points[i].image = &s_points[i];
+      s_points[i].image = &i_points[i];
+      s_points[i].gridsize = i_points[i].gridsize;
+    }
+    std::lock_guard<std::mutex> hold(grid_mutex());
+    slsim_ctx ctx = slsim_cpp::context();
+    void *yd = nullptr, *Ad = nullptr;
+    slsim_cpp::check(slsim_device_alloc(ctx, 16 * n, &yd));
+    slsim_cpp::check(slsim_device_alloc(ctx, 32 * n, &Ad));
+    int rc = slsim_gridmap_shoot(lens->handle(), N1d, center[0], center[1], range, 0.0,
+                                 (double*)yd, (double*)Ad, slsim_cpp::precision(), nullptr);
+    std::vector<double> y(2 * n), A(4 * n);
+    if (rc == SLSIM_OK) rc = slsim_memcpy_d2h(ctx, y.data(), yd, 16 * n);
+    if (rc == SLSIM_OK) rc = slsim_memcpy_d2h(ctx, A.data(), Ad, 32 * n);
+    slsim_device_free(ctx, yd);
+    slsim_device_free(ctx, Ad);
+    slsim_cpp::check(rc);
+    for (size_t i = 0; i < n; ++i) {
+      s_points[i].x[0] = y[2 * i];
+      s_points[i].x[1] = y[2 * i + 1];
+      for (int c = 0; c < 4; ++c) i_points[i].A.a[c] = A[4 * i + c];
+      s_points[i].A = i_points[i].A;
+    }
+  }
+
+  size_t getNumberOfPoints() const { return (size_t)Ngrid_init * Ngrid_init2; }
+  int getInitNgrid() const { return Ngrid_init; }
+  double getXRange() const { return x_range; }
+  double getResolution() const { return x_range / (Ngrid_init - 1); }
+  Point* operator[](size_t i) { return i_points.data() + i; }
+  Point_2d image_point(size_t i) { return Point_2d(i_points[i].x[0], i_points[i].x[1]); }
+  Point_2d source_point(size_t i) { return Point_2d(s_points[i].x[0], s_points[i].x[1]); }
+  RAY ray(size_t i) { return RAY(i_points[i]); }
+
+  // gridmap.h:533-603
+  template <typename T>
+  PixelMap<T> writePixelMap(LensingVariable lensvar) {
+    PixelMap<T> map(center, Ngrid_init, Ngrid_init2, x_range / (Ngrid_init - 1));
+    const size_t N = map.size();
+    for (size_t i = 0; i < N; ++i) {
+      const Point& p = i_points[i];
+      const double a0 = p.x[0] - p.image->x[0], a1 = p.x[1] - p.image->x[1];
+      switch (lensvar) {
+        case LensingVariable::ALPHA: map[i] = std::sqrt(a0 * a0 + a1 * a1); break;
+        case LensingVariable::ALPHA1: map[i] = a0; break;
+        case LensingVariable::ALPHA2: map[i] = a1; break;
+        case LensingVariable::KAPPA: map[i] = p.kappa(); break;
+        case LensingVariable::GAMMA:
+          map[i] = std::sqrt(p.gamma1() * p.gamma1() + p.gamma2() * p.gamma2());
+          break;
+        case LensingVariable::GAMMA1: map[i] = p.gamma1(); break;
+        case LensingVariable::GAMMA2: map[i] = p.gamma2(); break;
+        case LensingVariable::GAMMA3: map[i] = p.gamma3(); break;
+        case LensingVariable::INVMAG: map[i] = p.invmag(); break;
+        case LensingVariable::DELAYT: map[i] = p.dt; break;
+        case LensingVariable::SurfBrightness: map[i] = p.surface_brightness; break;
+        default:
+          throw std::runtime_error(
+              "GridMap::writePixelMapUniform() does not work for the input LensingVariable");
+      }
+    }
+    return map;
+  }
+
+ private:
+  // gridmap.cpp:15: one process-wide lock around the shoot (reference)
+  static std::mutex& grid_mutex() {
+    static std::mutex m;
+    return m;
+  }
+  int Ngrid_init, Ngrid_init2;
+  double x_range;
+  double center[2];
+  std::vector<Point> i_points;
+  std::vector<Point> s_points;
+};

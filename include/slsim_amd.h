@@ -1,0 +1,199 @@
+/*
+ * slsim_amd.h -- C ABI of the MI355X multi-plane particle ray shooter.
+ *
+ * Plain C types only: opaque handles, pointers and sizes, int status codes.
+ * No exception and no allocation crosses this boundary.  Device memory is
+ * owned by the library and counted against the context's HBM budget; host
+ * buffers are owned by the caller.  Calls on distinct contexts may run
+ * concurrently; calls on one context are serialised on its HIP stream.
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the GLAMER source tree).
+ */
+#ifndef SLSIM_AMD_H
+#define SLSIM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+enum {
+  SLSIM_OK = 0,
+  SLSIM_ERR_ARG = 1,          /* bad argument (reference: std::invalid_argument) */
+  SLSIM_ERR_HIP = 2,          /* a HIP runtime call failed                        */
+  SLSIM_ERR_BUDGET = 3,       /* request exceeds the context's HBM budget         */
+  SLSIM_ERR_SAME_PLACE = 4,   /* "particles in same place." (quadTree.h:327-329)  */
+  SLSIM_ERR_TOO_DEEP = 5,     /* a cell below level 32 still holds > bucket       */
+  SLSIM_ERR_NOT_FLAT = 6,     /* Om + OL != 1 (MultiPlane/lens.cpp:43-47)         */
+  SLSIM_ERR_STATE = 7         /* call out of order (e.g. serving not started)     */
+};
+
+/* Arithmetic of the tree walk.  The plane-to-plane recursion is FP64 in both. */
+enum {
+  SLSIM_FP64 = 0,             /* plain method: FP64 walk, reference arithmetic    */
+  SLSIM_FP32C = 1,            /* FP32 walk with compensated (TwoSum) accumulation */
+  SLSIM_DIRECT64 = 2          /* FP64 direct sum over every particle of a plane, no
+                                 tree: the error-column oracle (O(N) per ray)      */
+};
+
+typedef struct slsim_ctx_s* slsim_ctx;
+typedef struct slsim_plane_s* slsim_plane;
+typedef struct slsim_lens_s* slsim_lens;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* slsim_last_error(void);
+
+/* ---------------------------------------------------------------- context */
+/* Opens device `device` with an HBM budget in bytes (0 = 90 % of free HBM). */
+int slsim_ctx_create(int device, size_t hbm_budget_bytes, slsim_ctx* out);
+int slsim_ctx_destroy(slsim_ctx ctx);
+/* Bytes currently allocated by the context and its budget. */
+int slsim_ctx_memory(slsim_ctx ctx, size_t* used, size_t* budget);
+/* Blocks until all work of the context's stream is done. */
+int slsim_ctx_synchronize(slsim_ctx ctx);
+
+/* ------------------------------------------------------------ particle tree */
+/*
+ * Builds the device quadtree of one particle halo: replaces the
+ * LensHaloParticles(std::vector<PType>&, float z, ...) constructor and the
+ * TreeQuadParticles build it runs (include/particle_halo.h:58-75, 341-417;
+ * include/quadTree.h:247-628).  Positions in physical Mpc, masses in Msun and
+ * sizes (softening) in Mpc; the arrays may be freed on return.
+ * theta: opening angle (reference default 0.1); bucket: leaf size (20);
+ * inv_area: uniform compensating sheet (0 = none).
+ */
+int slsim_plane_create_f64(slsim_ctx ctx, const double* x, const double* y, const float* m,
+                           const float* h, size_t n, double theta, int bucket, double inv_area,
+                           slsim_plane* out);
+/* Same for ParticleType<float> positions. */
+int slsim_plane_create_f32(slsim_ctx ctx, const float* x, const float* y, const float* m,
+                           const float* h, size_t n, double theta, int bucket, double inv_area,
+                           slsim_plane* out);
+int slsim_plane_destroy(slsim_plane plane);
+
+typedef struct {
+  int64_t n_particles;
+  int64_t n_nodes;
+  int64_t n_leaves;
+  int64_t n_big_internal;   /* particles registered in internal-node big lists */
+  int32_t depth;            /* deepest level */
+  int32_t pad;
+  int64_t device_bytes;
+  double build_ms;          /* device build time (HIP events)               */
+  double root_p1[2], root_p2[2];
+} slsim_plane_info;
+int slsim_plane_get_info(slsim_plane plane, slsim_plane_info* info);
+
+/*
+ * Exports the node set in depth-first pre-order (child0 first, the order of
+ * walkTree_recur, include/quadTree.h:1079-1086).  Per node:
+ *   dbl[13 i + 0..12] = p1x, p1y, p2x, p2y, cx, cy, mass, q0, q1, q2, r2crit, 0, 0
+ *   ints[6 i + 0..5]  = level, nparticles, nbig, is_leaf, 0, 0
+ * nbig counts the reference's big_particles list of the node.
+ */
+int slsim_plane_export_nodes(slsim_plane plane, double* dbl, int64_t* ints);
+
+/* ------------------------------------------------------------------- lens */
+/* Lens(long* seed, PosType zs, const COSMOLOGY&) with a flat LCDM cosmology
+ * (MultiPlane/lens.cpp:59-77).  h is H0/100. */
+int slsim_lens_create(slsim_ctx ctx, double zsource, double Om, double OL, double h,
+                      slsim_lens* out);
+int slsim_lens_destroy(slsim_lens lens);
+/* Lens::moveinMainHalo(halo, addplanes=true) (include/lens.h:172-185): the
+ * lens takes ownership of `plane`, placed at float redshift z. */
+int slsim_lens_add_halo(slsim_lens lens, slsim_plane plane, float z);
+/* Number of lens planes, and the plane tables plane_redshifts, Dl, dDl (each
+ * nplanes + 1 long, the source pseudo-plane last) and charge = 4 pi G. */
+int slsim_lens_tables(slsim_lens lens, int* nplanes, double* z, double* Dl, double* dDl,
+                      double* charge);
+/* Lens::ResetSourcePlane(z, nearest=false) (MultiPlane/lens.cpp:2609-2657);
+ * z <= 0 reverts to the lens source. */
+int slsim_lens_reset_source(slsim_lens lens, double z);
+
+/* ------------------------------------------------------------- ray shooting */
+typedef struct {
+  double total_ms;          /* wall time of the call                         */
+  double walk_ms;           /* sum of walk-kernel times (HIP events)         */
+  double walk_ms_max_plane; /* slowest single walk launch                    */
+  int32_t n_walk_launches;
+  int32_t pad;
+} slsim_shoot_stats;
+
+/*
+ * Lens::rayshooterInternal(unsigned long N, Point*) and the per-ray-redshift
+ * overloads (LinkedPoint* + source_zs, RAY*)
+ * (FullRange/internal_rayshooter_multi.cpp:53-251; include/lens.h:913-1103).
+ * Host buffers: x (2n, x0 x1 interleaved, radians), zs (n source redshifts or
+ * NULL = lens source), y (2n out), A (4n out, row-major a0 a1 a2 a3).
+ * Rays are Morton-ordered on the device so that each wavefront walks a
+ * spatially coherent bundle; results come back in input order.
+ */
+int slsim_shoot(slsim_lens lens, size_t n, const double* x, const double* zs, double* y,
+                double* A, int precision, slsim_shoot_stats* stats);
+/* Same with device pointers (no host transfer); synchronous. */
+int slsim_shoot_device(slsim_lens lens, size_t n, const double* x_dev, const double* zs_dev,
+                       double* y_dev, double* A_dev, int precision, slsim_shoot_stats* stats);
+
+/*
+ * GridMap(LensHndl, N, center, range) (TreeCode_link/gridmap.cpp:93-120): the
+ * N x N ray grid x_i = center + range*((i%N, i/N)/(N-1) - 0.5) is generated on
+ * the device and shot; y_dev (2 N^2) and A_dev (4 N^2) are device buffers in
+ * grid order (x fastest).  zs <= 0 uses the lens source.
+ */
+int slsim_gridmap_shoot(slsim_lens lens, size_t N, double c0, double c1, double range,
+                        double zs, double* y_dev, double* A_dev, int precision,
+                        slsim_shoot_stats* stats);
+
+/*
+ * Work counters of the reference's per-ray criterion over a grid shoot, per
+ * ray summed over planes: cnt[5 i + k] = visits, far cells, leaf particles,
+ * big-particle tests, B-spline corrections (host buffer, 5 N^2).
+ */
+int slsim_gridmap_count(slsim_lens lens, size_t N, double c0, double c1, double range,
+                        double zs, uint64_t* cnt_host);
+
+/* FP64 direct sum of the smoothed-particle kernel of one plane (no tree), in
+ * the tree's sign convention: out[5 i] = alpha0, alpha1, kappa, gamma0, gamma1
+ * at physical positions rays[2 i] (host buffers).  Error column oracle. */
+int slsim_plane_direct_sum(slsim_plane plane, size_t n, const double* rays, double* out);
+/* The tree walk of one plane at physical positions (host buffers), same layout. */
+int slsim_plane_force(slsim_plane plane, size_t n, const double* rays, double* out,
+                      int precision);
+
+/* ------------------------------------------------------- device buffers */
+int slsim_device_alloc(slsim_ctx ctx, size_t bytes, void** dev);
+int slsim_device_free(slsim_ctx ctx, void* dev);
+int slsim_memcpy_d2h(slsim_ctx ctx, void* host, const void* dev, size_t bytes);
+int slsim_memcpy_h2d(slsim_ctx ctx, void* dev, const void* host, size_t bytes);
+
+/* ---------------------------------------------------------- cosmology */
+/* Flat LCDM comoving distance between z1 and z2 (Mpc), the role of
+ * COSMOLOGY::coorDist in CosmoLib (MultiPlane/lens.cpp:776, 2489, 2661). */
+double slsim_coordist(double Om, double OL, double h, double z1, double z2);
+/* Inverse of coorDist(0, z): the redshift at comoving distance D. */
+double slsim_inv_coordist(double Om, double OL, double h, double D);
+
+/* ------------------------------------------------- synthetic particles */
+/* r200 (physical Mpc) of mass M at redshift z: M = 800/3 pi r^3 rho_crit(z). */
+double slsim_nfw_r200(double mass, double z, double Om, double OL, double h);
+/* n particle positions of one projected NFW halo (concentration c, truncated
+ * at r200) centred on (xc, yc), physical Mpc, from a seeded mt19937_64. */
+int slsim_gen_nfw(uint64_t seed, size_t n, double mass, double conc, double z, double Om,
+                  double OL, double h, double xc, double yc, double* x, double* y);
+/* Superposition of n_halos projected NFW halos with masses log-uniform in
+ * [Mmin, Mmax], c = 10 (M/1e12)^-0.1 10^(0.1 N(0,1)), centres uniform in a
+ * square of side field_arcsec at angular distance D_A(z); n equal-mass
+ * particles split by mass with largest remainders; sizes = softening. */
+int slsim_gen_halo_field(uint64_t seed, size_t n, int n_halos, double Mmin, double Mmax,
+                         double field_arcsec, double z, double Om, double OL, double h,
+                         float softening, double* x, double* y, float* m, float* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLSIM_AMD_H */

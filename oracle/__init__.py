@@ -1,0 +1,167 @@
+"""CPU oracle of GLAMER's particle ray-shooting path (ctypes view of
+oracle/liboracle.so, built from oracle/glamer_oracle.cpp by oracle/Makefile).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never uses it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+    return LIB_PATH
+
+
+if not os.path.exists(LIB_PATH):
+    build()
+
+_lib = ctypes.CDLL(LIB_PATH)
+P = ctypes.POINTER
+c_d, c_f, c_sz, c_int, c_vp = ctypes.c_double, ctypes.c_float, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [("visits", ctypes.c_uint64), ("far", ctypes.c_uint64), ("leafp", ctypes.c_uint64),
+                ("bigtest", ctypes.c_uint64), ("bspline", ctypes.c_uint64)]
+
+
+_sigs = {
+    "orc_coordist": (c_d, [c_d, c_d, c_d, c_d, c_d]),
+    "orc_tree_build": (c_vp, [P(c_d), P(c_d), P(c_f), P(c_f), c_sz, c_d, c_int, c_d, P(c_int)]),
+    "orc_tree_free": (None, [c_vp]),
+    "orc_tree_nnodes": (c_sz, [c_vp]),
+    "orc_tree_export": (None, [c_vp, P(c_d), P(ctypes.c_int64)]),
+    "orc_tree_force": (None, [c_vp, P(c_d), c_sz, P(c_d), c_vp, c_int]),
+    "orc_direct_plane": (None, [P(c_d), P(c_d), P(c_f), P(c_f), c_sz, c_d, P(c_d), c_sz, P(c_d),
+                                c_int]),
+    "orc_bspline": (None, [P(c_d), c_d, c_d, c_d, P(c_d)]),
+    "orc_lens_create": (c_vp, [c_d, c_d, c_d, c_d]),
+    "orc_lens_free": (None, [c_vp]),
+    "orc_lens_add_halo": (None, [c_vp, c_vp, c_f]),
+    "orc_lens_nplanes": (c_int, [c_vp]),
+    "orc_lens_tables": (None, [c_vp, P(c_d), P(c_d), P(c_d), P(c_d)]),
+    "orc_lens_shoot": (None, [c_vp, c_sz, P(c_d), P(c_d), P(c_d), P(c_d), c_int, c_vp]),
+    "orc_gridmap_positions": (None, [c_sz, c_d, c_d, c_d, P(c_d)]),
+}
+for _n, (_r, _a) in _sigs.items():
+    getattr(_lib, _n).restype = _r
+    getattr(_lib, _n).argtypes = _a
+
+
+def _p(a, t=c_d):
+    return a.ctypes.data_as(P(t))
+
+
+def coorDist(z1, z2, Om=0.3, OL=0.7, h=0.7):
+    return _lib.orc_coordist(Om, OL, h, z1, z2)
+
+
+def _counts(arr):
+    return np.array([[c.visits, c.far, c.leafp, c.bigtest, c.bspline] for c in arr], np.uint64)
+
+
+class Tree:
+    """TreeQuadParticles over (x, y, m, h); positions promote to float64."""
+
+    def __init__(self, x, y, m, h, theta=0.1, bucket=20, inv_area=0.0):
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.m = np.ascontiguousarray(m, dtype=np.float32)
+        self.hh = np.ascontiguousarray(h, dtype=np.float32)
+        self.inv_area = inv_area
+        err = c_int(0)
+        self.t = _lib.orc_tree_build(_p(self.x), _p(self.y), _p(self.m, c_f), _p(self.hh, c_f),
+                                     self.m.size, inv_area, bucket, theta, ctypes.byref(err))
+        if not self.t:
+            raise ValueError("particles in same place.")
+        self.owned = True
+
+    def nnodes(self):
+        return _lib.orc_tree_nnodes(self.t)
+
+    def export_nodes(self):
+        nn = self.nnodes()
+        d = np.zeros((nn, 13))
+        q = np.zeros((nn, 6), np.int64)
+        _lib.orc_tree_export(self.t, _p(d), q.ctypes.data_as(P(ctypes.c_int64)))
+        return d, q
+
+    def force(self, rays, nthreads=8, counts=False):
+        r = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 2)
+        out = np.zeros((r.shape[0], 5))
+        cnt = (Counters * r.shape[0])() if counts else None
+        _lib.orc_tree_force(self.t, _p(r), r.shape[0], _p(out),
+                            ctypes.cast(cnt, c_vp) if counts else None, nthreads)
+        return (out, _counts(cnt)) if counts else out
+
+    def __del__(self):
+        if getattr(self, "owned", False) and self.t:
+            _lib.orc_tree_free(self.t)
+
+
+def direct_sum(x, y, m, h, rays, inv_area=0.0, nthreads=8):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    m = np.ascontiguousarray(m, dtype=np.float32)
+    h = np.ascontiguousarray(h, dtype=np.float32)
+    r = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros((r.shape[0], 5))
+    _lib.orc_direct_plane(_p(x), _p(y), _p(m, c_f), _p(h, c_f), m.size, inv_area, _p(r),
+                          r.shape[0], _p(out), nthreads)
+    return out
+
+
+def bspline(xcm, r, mass, size):
+    xcm = np.ascontiguousarray(xcm, dtype=np.float64)
+    out = np.zeros(5)
+    _lib.orc_bspline(_p(xcm), r, mass, size, _p(out))
+    return out
+
+
+class Lens:
+    """Lens with moveinMainHalo'd particle halos; shoot = rayshooterInternal."""
+
+    def __init__(self, zsource, Om=0.3, OL=0.7, h=0.7):
+        self.L = _lib.orc_lens_create(zsource, Om, OL, h)
+
+    def moveinMainHalo(self, tree, z):
+        tree.owned = False          # the lens owns it now
+        _lib.orc_lens_add_halo(self.L, tree.t, np.float32(z))
+
+    def tables(self):
+        n = _lib.orc_lens_nplanes(self.L)
+        z, Dl, dDl = (np.zeros(n + 1) for _ in range(3))
+        ch = c_d()
+        _lib.orc_lens_tables(self.L, _p(z), _p(Dl), _p(dDl), ctypes.byref(ch))
+        return {"nplanes": n, "z": z, "Dl": Dl, "dDl": dDl, "charge": ch.value}
+
+    def shoot(self, x, zs=None, nthreads=8, counts=False):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 2)
+        n = x.shape[0]
+        y = np.zeros((n, 2))
+        A = np.zeros((n, 4))
+        zp = None
+        if zs is not None:
+            zs = np.ascontiguousarray(zs, dtype=np.float64)
+            zp = _p(zs)
+        cnt = (Counters * n)() if counts else None
+        _lib.orc_lens_shoot(self.L, n, _p(x), zp, _p(y), _p(A), nthreads,
+                            ctypes.cast(cnt, c_vp) if counts else None)
+        return (y, A, _counts(cnt)) if counts else (y, A)
+
+    def __del__(self):
+        if getattr(self, "L", None):
+            _lib.orc_lens_free(self.L)
+
+
+def gridmap_positions(N, center, range_):
+    x = np.zeros((N * N, 2))
+    _lib.orc_gridmap_positions(N, center[0], center[1], range_, _p(x))
+    return x

@@ -1,0 +1,841 @@
+// glamer_oracle.cpp -- CPU restatement of GLAMER's particle ray-shooting path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle and the timed
+// CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load it; the product library never links or calls it.
+//
+// It restates, in FP64 with float32 particle masses and sizes, the reference
+// functions listed below (paths relative to the reference root):
+//   TreeQuadParticles ctor / BuildQTreeNB / _BuildQTreeNB / CalcMoments
+//                                        include/quadTree.h:247-628
+//   TreeQuadParticles::b_spline_profile  include/quadTree.h:147-190
+//   TreeQuadParticles::force2D_recur     include/quadTree.h:932-987
+//   TreeQuadParticles::walkTree_recur    include/quadTree.h:990-1128
+//   Utilities::quickPartition            include/Tree.h:396-411
+//   LensHaloParticles::force_halo        include/particle_halo.h:426-434
+//   LensPlaneSingular::force             MultiPlane/planes.cpp:114-161
+//   Lens::addMainHaloToPlane             MultiPlane/lens.cpp:772-815
+//   Lens::combinePlanes                  MultiPlane/lens.cpp:2397-2515
+//   Lens::FindSourcePlane                MultiPlane/lens.cpp:2659-2681
+//   Lens::compute_points_parallel        include/lens.h:913-1103
+//   Lens::rayshooterInternal (threads)   FullRange/internal_rayshooter_multi.cpp:53-118
+//   Utilities::PositionFromIndex         TreeCode_link/utilities.cpp:149-158
+//   constants PI, Grav, arcsecTOradians  include/standard.h:38-51
+//
+// Cosmological distances come from CosmoLib in the reference, which is not
+// part of the reference tree.  Here they are the flat-LCDM comoving distance
+// D_C = (c/H0) * int dz / sqrt(Om (1+z)^3 + OL), integrated by adaptive
+// Gauss-Kronrod (7/15) to 1e-14 relative.
+//
+// A second oracle, orc_direct_plane, sums the same smoothed-particle kernel
+// (point mass plus the reference's B-spline correction for r < 2h) over all
+// particles with no tree.
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace orc {
+
+static const double PI = M_PI;               // standard.h:38-40 (M_PI)
+static const double Grav = 4.7788e-20;       // standard.h:43, G/c^2 in Mpc/Msun
+static const double MIN_PLANE_DIST = 1e-6;   // MultiPlane/lens.cpp:15
+
+// Interaction counters of one walk (per ray per plane).
+struct Counters {
+  uint64_t visits = 0;      // nodes with nparticles > 0 given the open test
+  uint64_t far = 0;         // far-cell (multipole) evaluations
+  uint64_t leafp = 0;       // leaf particle point-mass interactions
+  uint64_t bigtest = 0;     // big-particle candidate tests
+  uint64_t bspline = 0;     // B-spline corrections applied
+  void add(const Counters& o) {
+    visits += o.visits; far += o.far; leafp += o.leafp;
+    bigtest += o.bigtest; bspline += o.bspline;
+  }
+};
+
+// ---------------------------------------------------------------- cosmology
+struct Cosmo {
+  double Om, OL, h;
+  double DH() const { return 2997.92458 / h; }   // c/H0 in Mpc
+  double invE(double z) const {
+    double a = 1 + z;
+    return 1.0 / std::sqrt(Om * a * a * a + OL);
+  }
+  double gk15(double a, double b, double* err) const {
+    static const double xgk[8] = {0.991455371120812639206854697526329,
+                                  0.949107912342758524526189684047851,
+                                  0.864864423359769072789712788640926,
+                                  0.741531185599394439863864773280788,
+                                  0.586087235467691130294144845693013,
+                                  0.405845151377397166906606412076961,
+                                  0.207784955007898467600689403773245,
+                                  0.000000000000000000000000000000000};
+    static const double wgk[8] = {0.022935322010529224963732008058970,
+                                  0.063092092629978553290700663189204,
+                                  0.104790010322250183839876322541518,
+                                  0.140653259715525918745189590510238,
+                                  0.169004726639267902826583426598550,
+                                  0.190350578064785409913256402421014,
+                                  0.204432940075298892414161999234649,
+                                  0.209482141084727828012999174891714};
+    static const double wg[4] = {0.129484966168869693270611432679082,
+                                 0.279705391489276667901467771423780,
+                                 0.381830050505118944950369775488975,
+                                 0.417959183673469387755102040816327};
+    double c = 0.5 * (a + b), hl = 0.5 * (b - a);
+    double fc = invE(c);
+    double rk = fc * wgk[7], rg = fc * wg[3];
+    for (int j = 0; j < 7; ++j) {
+      double f1 = invE(c - hl * xgk[j]), f2 = invE(c + hl * xgk[j]);
+      rk += wgk[j] * (f1 + f2);
+      if (j % 2 == 1) rg += wg[j / 2] * (f1 + f2);
+    }
+    *err = std::fabs((rk - rg) * hl);
+    return rk * hl;
+  }
+  double integ(double a, double b, int depth = 0) const {
+    double err;
+    double r = gk15(a, b, &err);
+    if (depth > 40 || err <= 1e-15 * std::fabs(r) + 1e-300) return r;
+    double m = 0.5 * (a + b);
+    return integ(a, m, depth + 1) + integ(m, b, depth + 1);
+  }
+  // comoving distance between z1 and z2 (Mpc)
+  double coorDist(double z1, double z2) const {
+    if (z2 == z1) return 0.0;
+    if (z2 < z1) return -coorDist(z2, z1);
+    return DH() * integ(z1, z2);
+  }
+  double angDist(double z) const { return coorDist(0, z) / (1 + z); }
+};
+
+// ---------------------------------------------------------------- quadtree
+struct Node {
+  double p1[2], p2[2];
+  double center[2];
+  double mass = 0, quad[3] = {0, 0, 0}, r2crit = 0;
+  size_t* particles = nullptr;   // points into TreeQuad::index
+  size_t nparticles = 0;
+  std::vector<size_t> big;      // big_particles
+  int child[4] = {-1, -1, -1, -1};
+  int level = 0;
+  bool leaf() const {
+    return child[0] < 0 && child[1] < 0 && child[2] < 0 && child[3] < 0;
+  }
+};
+
+struct TreeQuad {
+  // particles (positions promoted to double exactly as PType::operator[] -> PosType)
+  std::vector<double> px, py;
+  std::vector<float> pm, ph;
+  size_t N = 0;
+  double inv_area = 0;
+  int Nbucket = 20;
+  double force_theta = 0.1;
+  std::vector<size_t> index;
+  std::vector<Node> nodes;   // nodes[0] = root
+  std::vector<double> workspace;
+
+  // Utilities::quickPartition, include/Tree.h:396-411 (Lomuto, x <= pivot first)
+  static void quickPartition(double pivot, size_t* pivotindex, size_t* particles,
+                             double* arr, size_t n) {
+    *pivotindex = 0;
+    for (size_t i = 0; i < n; ++i) {
+      if (arr[i] <= pivot) {
+        std::swap(arr[*pivotindex], arr[i]);
+        std::swap(particles[*pivotindex], particles[i]);
+        ++(*pivotindex);
+      }
+    }
+  }
+
+  // BuildQTreeNB, include/quadTree.h:295-350
+  void build() {
+    double p1[2] = {px[0], py[0]}, p2[2] = {px[0], py[0]};
+    for (size_t i = 0; i < N; ++i) {
+      if (px[i] < p1[0]) p1[0] = px[i];
+      if (px[i] > p2[0]) p2[0] = px[i];
+      if (py[i] < p1[1]) p1[1] = py[i];
+      if (py[i] > p2[1]) p2[1] = py[i];
+    }
+    if (N == 1) { p1[0] = -0.25; p1[1] = -0.25; p2[0] = 0.25; p2[1] = 0.25; }
+    double lengths[2] = {p2[0] - p1[0], p2[1] - p1[1]};
+    if (lengths[0] == 0 || lengths[1] == 0)
+      throw std::invalid_argument("particles in same place.");
+    int j = lengths[0] > lengths[1] ? 1 : 0;
+    p2[j] = p1[j] + lengths[!j];
+
+    nodes.clear();
+    nodes.reserve(N / 4 + 16);
+    Node root;
+    root.p1[0] = p1[0]; root.p1[1] = p1[1];
+    root.p2[0] = p2[0]; root.p2[1] = p2[1];
+    root.particles = index.data();
+    root.nparticles = N;
+    root.level = 0;
+    nodes.push_back(std::move(root));
+    workspace.resize(N);
+    build_rec(0);
+    workspace.clear();
+    workspace.shrink_to_fit();
+  }
+
+  // _BuildQTreeNB, include/quadTree.h:358-565 (max_range = -1, MultiRadius = MultiMass = true)
+  void build_rec(int ni) {
+    Node* cb = &nodes[ni];
+    cb->center[0] = (cb->p1[0] + cb->p2[0]) / 2;
+    cb->center[1] = (cb->p1[1] + cb->p2[1]) / 2;
+    cb->quad[0] = cb->quad[1] = cb->quad[2] = 0;
+    cb->mass = 0.0;
+    double theta_range = 2 * force_theta;
+    const double box = cb->p2[0] - cb->p1[0];
+    size_t* particles = cb->particles;
+    const size_t n = cb->nparticles;
+
+    if (n <= (size_t)Nbucket || force_theta > theta_range) {
+      cb->big.clear();
+      for (size_t i = 0; i < n; ++i) {
+        double r = ph[particles[i]];
+        if (r < box) cb->big.push_back(particles[i]);
+      }
+      return;
+    }
+
+    double* x = workspace.data();
+    size_t cut, cut2;
+    for (size_t i = 0; i < n; ++i) x[i] = ph[particles[i]];
+    double maxsize = box / 2;
+    quickPartition(maxsize, &cut, particles, x, n);
+    if (cut < n) {
+      if (ni == 0) {
+        cut2 = n - cut;
+      } else {
+        quickPartition(2 * maxsize, &cut2, &particles[cut], &x[cut], n - cut);
+      }
+      cb->big.assign(particles + cut, particles + cut + cut2);
+    }
+
+    const size_t NpInChildren = n;
+    if (NpInChildren == 0) return;
+
+    // children: 0 = (x>xc, y>yc), 1 = (x<=xc, y>yc), 2 = (x>xc, y<=yc), 3 = (x<=xc, y<=yc)
+    int first = (int)nodes.size();
+    for (int c = 0; c < 4; ++c) {
+      Node ch;
+      ch.level = nodes[ni].level + 1;
+      nodes.push_back(std::move(ch));
+    }
+    cb = &nodes[ni];   // re-fetch after possible reallocation
+    for (int c = 0; c < 4; ++c) cb->child[c] = first + c;
+    Node* c0 = &nodes[first + 0];
+    Node* c1 = &nodes[first + 1];
+    Node* c2 = &nodes[first + 2];
+    Node* c3 = &nodes[first + 3];
+    c0->p1[0] = cb->center[0]; c0->p1[1] = cb->center[1];
+    c0->p2[0] = cb->p2[0];     c0->p2[1] = cb->p2[1];
+    c1->p1[0] = cb->p1[0];     c1->p1[1] = cb->center[1];
+    c1->p2[0] = cb->center[0]; c1->p2[1] = cb->p2[1];
+    c2->p1[0] = cb->center[0]; c2->p1[1] = cb->p1[1];
+    c2->p2[0] = cb->p2[0];     c2->p2[1] = cb->center[1];
+    c3->p1[0] = cb->p1[0];     c3->p1[1] = cb->p1[1];
+    c3->p2[0] = cb->center[0]; c3->p2[1] = cb->center[1];
+
+    const double xcut = cb->center[0], ycut = cb->center[1];
+    size_t cutx, cuty;
+    for (size_t i = 0; i < NpInChildren; ++i) x[i] = py[particles[i]];
+    quickPartition(ycut, &cuty, particles, x, NpInChildren);
+    if (cuty > 0) {
+      for (size_t i = 0; i < cuty; ++i) x[i] = px[particles[i]];
+      quickPartition(xcut, &cutx, particles, x, cuty);
+      c3->nparticles = cutx;
+      c3->particles = cutx > 0 ? particles : nullptr;
+      c2->nparticles = cuty - cutx;
+      c2->particles = (cuty - cutx) > 0 ? &particles[cutx] : nullptr;
+    } else {
+      c3->nparticles = 0; c3->particles = nullptr;
+      c2->nparticles = 0; c2->particles = nullptr;
+    }
+    if (cuty < NpInChildren) {
+      for (size_t i = cuty; i < NpInChildren; ++i) x[i - cuty] = px[particles[i]];
+      quickPartition(xcut, &cutx, &particles[cuty], x, NpInChildren - cuty);
+      c1->nparticles = cutx;
+      c1->particles = cutx > 0 ? &particles[cuty] : nullptr;
+      c0->nparticles = NpInChildren - cuty - cutx;
+      c0->particles = c0->nparticles > 0 ? &particles[cuty + cutx] : nullptr;
+    } else {
+      c1->nparticles = 0; c1->particles = nullptr;
+      c0->nparticles = 0; c0->particles = nullptr;
+    }
+    for (int c = 0; c < 4; ++c) build_rec(first + c);
+  }
+
+  // CalcMoments, include/quadTree.h:567-628
+  void calc_moments() {
+    for (auto& b : nodes) {
+      double absmass = 0;
+      b.mass = 0;
+      for (size_t i = 0; i < b.nparticles; ++i) {
+        b.mass += (double)pm[b.particles[i]];
+        absmass += std::fabs((double)pm[b.particles[i]]);
+      }
+      b.center[0] = b.center[1] = 0;
+      for (size_t i = 0; i < b.nparticles; ++i) {
+        double tmp = std::fabs((double)pm[b.particles[i]]);
+        b.center[0] += tmp * px[b.particles[i]] / absmass;
+        b.center[1] += tmp * py[b.particles[i]] / absmass;
+      }
+      b.quad[0] = b.quad[1] = b.quad[2] = 0;
+      for (size_t i = 0; i < b.nparticles; ++i) {
+        double xcm0 = px[b.particles[i]] - b.center[0];
+        double xcm1 = py[b.particles[i]] - b.center[1];
+        double xcut = xcm0 * xcm0 + xcm1 * xcm1;
+        double tmp = pm[b.particles[i]];
+        b.quad[0] += (xcut - 2 * xcm0 * xcm0) * tmp;
+        b.quad[1] += (xcut - 2 * xcm1 * xcm1) * tmp;
+        b.quad[2] += -2 * xcm0 * xcm1 * tmp;
+      }
+      double rcom = 0.0;
+      for (int i = 0; i < 2; ++i) {
+        double a = (b.center[i] - b.p1[i]) * (b.center[i] - b.p1[i]);
+        double c = (b.center[i] - b.p2[i]) * (b.center[i] - b.p2[i]);
+        rcom += a > c ? a : c;
+      }
+      rcom = std::sqrt(rcom);
+      if (force_theta > 0.0) {
+        b.r2crit = 1.15470 * rcom / force_theta;
+        b.r2crit *= b.r2crit;
+      } else {
+        b.r2crit = 1.0e100;
+      }
+    }
+  }
+
+  // b_spline_profile, include/quadTree.h:147-190 (phi omitted: dead downstream)
+  static void b_spline(const double* xcm, double r, double Mass, double size,
+                       double* alpha, double* kappa, double* gamma) {
+    double q = r / size;
+    double M, sigma;
+    if (q > 2) {
+      sigma = 0;
+      M = 1;
+    } else {
+      double q2 = q * q, q3 = q2 * q;
+      sigma = (8 - 12 * q + 6 * q2 - q3) / 4;
+      if (q > 1) {
+        sigma *= 10 / size / size / 7 / PI;
+        M = (-1 + 20 * q2 * (1 - q + 3 * q2 / 8 - q3 / 20)) / 7;
+      } else {
+        sigma = 10 * (sigma - 1 + 3 * q - 3 * q2 + q3) / size / size / 7 / PI;
+        M = 10 * q2 * (1 - 3 * q / 4 + 3 * q3 / 10) / 7;
+      }
+    }
+    double alpha_r = (M - 1) / PI / r;
+    double gt = alpha_r / r - sigma;
+    alpha[0] -= Mass * alpha_r * xcm[0] / r;
+    alpha[1] -= Mass * alpha_r * xcm[1] / r;
+    gamma[0] -= gt * Mass * (xcm[0] * xcm[0] - xcm[1] * xcm[1]) / r / r;
+    gamma[1] -= gt * Mass * 2 * xcm[0] * xcm[1] / r / r;
+    *kappa += Mass * sigma;
+  }
+
+  // walkTree_recur, include/quadTree.h:990-1128 (screening = 0, phi omitted)
+  void walk(int ni, const double* ray, double* alpha, double* kappa, double* gamma,
+            Counters* cnt) const {
+    const Node& b = nodes[ni];
+    if (b.nparticles == 0) return;
+    double xcm[2] = {b.center[0] - ray[0], b.center[1] - ray[1]};
+    double rcm2cell = xcm[0] * xcm[0] + xcm[1] * xcm[1];
+    double boxsize2 = (b.p2[0] - b.p1[0]) * (b.p2[0] - b.p1[0]);
+    if (cnt) cnt->visits++;
+    if (rcm2cell < b.r2crit || rcm2cell < 5.83 * boxsize2) {
+      if (b.leaf()) {
+        for (size_t i = 0; i < b.nparticles; ++i) {
+          size_t p = b.particles[i];
+          double d[2] = {px[p] - ray[0], py[p] - ray[1]};
+          double rcm2 = d[0] * d[0] + d[1] * d[1];
+          if (rcm2 < 1e-20) rcm2 = 1e-20;
+          double mass = pm[p];
+          double prefac = mass / rcm2 / PI;
+          alpha[0] += (inv_area * mass - prefac) * d[0];
+          alpha[1] += (inv_area * mass - prefac) * d[1];
+          double tmp = -2.0 * prefac / rcm2;
+          gamma[0] += 0.5 * (d[0] * d[0] - d[1] * d[1]) * tmp;
+          gamma[1] += d[0] * d[1] * tmp;
+          *kappa -= mass * inv_area;
+        }
+        if (cnt) cnt->leafp += b.nparticles;
+      }
+      if (rcm2cell < 5.83 * boxsize2) {
+        for (size_t p : b.big) {
+          double d[2] = {px[p] - ray[0], py[p] - ray[1]};
+          double rcm2 = d[0] * d[0] + d[1] * d[1];
+          if (rcm2 < 1e-20) rcm2 = 1e-20;
+          double size = ph[p];
+          if (cnt) cnt->bigtest++;
+          if (rcm2 < 4 * size * size) {
+            b_spline(d, std::sqrt(rcm2), (double)pm[p], size, alpha, kappa, gamma);
+            if (cnt) cnt->bspline++;
+          }
+        }
+      }
+      for (int c = 0; c < 4; ++c)
+        if (b.child[c] >= 0) walk(b.child[c], ray, alpha, kappa, gamma, cnt);
+    } else {
+      if (cnt) cnt->far++;
+      double tmp = b.mass * inv_area - b.mass / rcm2cell / PI;
+      alpha[0] += tmp * xcm[0];
+      alpha[1] += tmp * xcm[1];
+      tmp = -2.0 * b.mass / PI / rcm2cell / rcm2cell;
+      gamma[0] += 0.5 * (xcm[0] * xcm[0] - xcm[1] * xcm[1]) * tmp;
+      gamma[1] += xcm[0] * xcm[1] * tmp;
+      alpha[0] -= (b.quad[0] * xcm[0] + b.quad[2] * xcm[1]) / (rcm2cell * rcm2cell) / PI;
+      alpha[1] -= (b.quad[1] * xcm[1] + b.quad[2] * xcm[0]) / (rcm2cell * rcm2cell) / PI;
+      tmp = 4 * (b.quad[0] * xcm[0] * xcm[0] + b.quad[1] * xcm[1] * xcm[1] +
+                 2 * b.quad[2] * xcm[0] * xcm[1]) /
+            (rcm2cell * rcm2cell * rcm2cell) / PI;
+      alpha[0] += tmp * xcm[0];
+      alpha[1] += tmp * xcm[1];
+      *kappa -= inv_area * b.mass;
+    }
+  }
+
+  // force2D_recur, include/quadTree.h:932-987 (periodic_buffer = false)
+  void force2D(const double* ray, double* alpha, double* kappa, double* gamma,
+               Counters* cnt) const {
+    alpha[0] = alpha[1] = gamma[0] = gamma[1] = gamma[2] = 0.0;
+    *kappa = 0.0;
+    if (N == 0) return;
+    walk(0, ray, alpha, kappa, gamma, cnt);
+  }
+};
+
+// LensHaloParticles with posHalo = (0,0): force_halo negates the tree's alpha
+// (include/particle_halo.h:426-434).
+struct Halo {
+  std::unique_ptr<TreeQuad> tree;
+  float z;
+  void force_halo(const double* x, double* alpha, double* kappa, double* gamma,
+                  Counters* cnt) const {
+    tree->force2D(x, alpha, kappa, gamma, cnt);
+    alpha[0] *= -1;
+    alpha[1] *= -1;
+  }
+};
+
+// LensPlaneSingular::force, MultiPlane/planes.cpp:114-161
+struct Plane {
+  std::vector<Halo*> halos;
+  void force(double* alpha, double* kappa, double* gamma, const double* xx,
+             Counters* cnt) const {
+    alpha[0] = alpha[1] = 0.0;
+    *kappa = 0.0;
+    gamma[0] = gamma[1] = gamma[2] = 0.0;
+    for (const Halo* h : halos) {
+      double at[2] = {0, 0}, kt = 0, gt[3] = {0, 0, 0};
+      double xt[2] = {xx[0] - 0.0, xx[1] - 0.0};   // getX() = (0,0) for particles
+      h->force_halo(xt, at, &kt, gt, cnt);
+      alpha[0] -= at[0];
+      alpha[1] -= at[1];
+      *kappa += kt;
+      gamma[0] += gt[0];
+      gamma[1] += gt[1];
+      gamma[2] += gt[2];
+    }
+  }
+};
+
+struct Lens {
+  Cosmo cosmo;
+  double zsource;
+  double charge;
+  std::vector<std::unique_ptr<Halo>> halos;
+  std::vector<std::unique_ptr<Plane>> main_planes;
+  std::vector<double> main_Dl, main_z;
+  // combined
+  std::vector<Plane*> planes;
+  std::vector<double> plane_redshifts, Dl, dDl;
+
+  Lens(double zs, double Om, double OL, double h) : zsource(zs) {
+    cosmo = Cosmo{Om, OL, h};
+    charge = 4 * PI * Grav;   // MultiPlane/lens.cpp:93
+    combinePlanes();
+  }
+
+  // addMainHaloToPlane, MultiPlane/lens.cpp:772-815
+  void addMainHaloToPlane(Halo* halo) {
+    double halo_z = halo->z;
+    double halo_Dl = cosmo.coorDist(0, halo_z);
+    size_t i = std::upper_bound(main_Dl.begin(), main_Dl.end(), halo_Dl) - main_Dl.begin();
+    if (i > 0 && (halo_Dl - main_Dl[i - 1]) < MIN_PLANE_DIST) {
+      main_planes[i - 1]->halos.push_back(halo);
+    } else if (i == main_Dl.size()) {
+      auto p = std::make_unique<Plane>();
+      p->halos.push_back(halo);
+      main_planes.push_back(std::move(p));
+      main_z.push_back(halo_z);
+      main_Dl.push_back(halo_Dl);
+    } else if ((main_Dl[i] - halo_Dl) < MIN_PLANE_DIST) {
+      main_planes[i]->halos.push_back(halo);
+    } else {
+      auto p = std::make_unique<Plane>();
+      p->halos.push_back(halo);
+      main_planes.insert(main_planes.begin() + i, std::move(p));
+      main_z.insert(main_z.begin() + i, halo_z);
+      main_Dl.insert(main_Dl.begin() + i, halo_Dl);
+    }
+  }
+
+  // combinePlanes, MultiPlane/lens.cpp:2397-2515 (no field planes)
+  void combinePlanes() {
+    planes.clear();
+    plane_redshifts.clear();
+    Dl = main_Dl;
+    dDl.clear();
+    std::vector<size_t> idx(Dl.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return Dl[a] < Dl[b]; });
+    std::sort(Dl.begin(), Dl.end());
+    for (size_t i = 1; i < Dl.size(); ++i)
+      if ((Dl[i] - Dl[i - 1]) < MIN_PLANE_DIST) Dl[i] = Dl[i - 1] + MIN_PLANE_DIST;
+    for (size_t i : idx) {
+      plane_redshifts.push_back(main_z[i]);
+      planes.push_back(main_planes[i].get());
+    }
+    if (!plane_redshifts.empty() && zsource <= plane_redshifts.back())
+      zsource = plane_redshifts.back() + 0.1;
+    plane_redshifts.push_back(zsource);
+    Dl.push_back(cosmo.coorDist(0, zsource));
+    dDl.push_back(Dl[0]);
+    for (size_t i = 1; i < Dl.size(); ++i) dDl.push_back(Dl[i] - Dl[i - 1]);
+  }
+
+  // moveinMainHalo(addplanes = true), include/lens.h:172-185
+  void moveinMainHalo(std::unique_ptr<Halo> h) {
+    Halo* raw = h.get();
+    halos.push_back(std::move(h));
+    addMainHaloToPlane(raw);
+    combinePlanes();
+  }
+
+  // FindSourcePlane, MultiPlane/lens.cpp:2659-2681
+  void FindSourcePlane(double zs, long& jmax, double& Dls, double& Ds) const {
+    Ds = cosmo.coorDist(0, zs);
+    if (zs <= plane_redshifts[0]) { jmax = 0; Dls = Ds; return; }
+    jmax = (long)plane_redshifts.size() - 1;
+    if (jmax <= 0) { Dls = Ds; return; }
+    while (plane_redshifts[jmax - 1] > zs) --jmax;
+    Dls = cosmo.coorDist(plane_redshifts[jmax - 1], zs);
+  }
+
+  // compute_points_parallel, include/lens.h:913-1103, on SoA rays.
+  // x: 2n (x0,x1 interleaved), y: 2n out, A: 4n out (a0..a3), zs: per-ray or null.
+  void compute_points(size_t start, size_t end, const double* x, const double* zs_ray,
+                      double source_z, double* y, double* A, Counters* cnt_ray) const {
+    long jmax = (long)planes.size();
+    double Dls_Ds = 0, D_Ds = 0;
+    if (!zs_ray) {
+      if (source_z == plane_redshifts.back()) {
+        Dls_Ds = dDl.back() / Dl.back();
+        D_Ds = Dl[Dl.size() - 2] / Dl.back();
+      } else {
+        double Dls, Ds;
+        FindSourcePlane(source_z, jmax, Dls, Ds);
+        Dls_Ds = Dls / Ds;
+        if (jmax > 0) D_Ds = Dl[jmax - 1] / Ds;
+      }
+    }
+    for (size_t i = start; i < end; ++i) {
+      double theta[2] = {x[2 * i], x[2 * i + 1]};
+      double SPA[2] = {theta[0], theta[1]};
+      double SAG[4] = {1, 0, 0, 1};
+      double Ai[4] = {1, 0, 0, 1};
+      Counters c;
+      if (zs_ray) {
+        double Dls, Ds;
+        FindSourcePlane(zs_ray[i], jmax, Dls, Ds);
+        Dls_Ds = Dls / Ds;
+        if (jmax > 0) D_Ds = Dl[jmax - 1] / Ds;
+      }
+      for (long j = 0; j < jmax; ++j) {
+        double Dphysical = Dl[j] / (1 + plane_redshifts[j]);
+        double xx[2] = {theta[0] * Dphysical, theta[1] * Dphysical};
+        double alpha[2], kappa, gamma[3];
+        planes[j]->force(alpha, &kappa, gamma, xx, cnt_ray ? &c : nullptr);
+        double G[4] = {kappa + gamma[0], gamma[1], gamma[1], kappa - gamma[0]};
+        double f = charge * Dl[j] / (1 + plane_redshifts[j]);
+        for (int k = 0; k < 4; ++k) G[k] *= f;
+        SPA[0] -= charge * alpha[0];
+        SPA[1] -= charge * alpha[1];
+        double aa, bb;
+        if (j < jmax - 1) {
+          aa = dDl[j + 1] / Dl[j + 1];
+          bb = Dl[j] / Dl[j + 1];
+        } else {
+          aa = Dls_Ds;
+          bb = D_Ds;
+        }
+        theta[0] = bb * theta[0] + aa * SPA[0];
+        theta[1] = bb * theta[1] + aa * SPA[1];
+        // SumPrevAG -= G * A  (Matrix2x2::operator*, include/point.h:258-268)
+        double GA[4] = {G[0] * Ai[0] + G[1] * Ai[2], G[0] * Ai[1] + G[1] * Ai[3],
+                        G[2] * Ai[0] + G[3] * Ai[2], G[2] * Ai[1] + G[3] * Ai[3]};
+        for (int k = 0; k < 4; ++k) SAG[k] -= GA[k];
+        // A = A * bb + SumPrevAG * aa
+        for (int k = 0; k < 4; ++k) {
+          double t1 = Ai[k] * bb;
+          double t2 = SAG[k] * aa;
+          Ai[k] = t1 + t2;
+        }
+      }
+      y[2 * i] = theta[0];
+      y[2 * i + 1] = theta[1];
+      for (int k = 0; k < 4; ++k) A[4 * i + k] = Ai[k];
+      if (cnt_ray) cnt_ray[i] = c;
+    }
+  }
+
+  // rayshooterInternal, FullRange/internal_rayshooter_multi.cpp:53-118
+  void shoot(size_t n, const double* x, const double* zs_ray, double* y, double* A,
+             int nthreads, Counters* cnt_ray) const {
+    if (plane_redshifts.size() == 1) {
+      for (size_t i = 0; i < n; ++i) {
+        y[2 * i] = x[2 * i];
+        y[2 * i + 1] = x[2 * i + 1];
+        A[4 * i] = 1; A[4 * i + 1] = 0; A[4 * i + 2] = 0; A[4 * i + 3] = 1;
+      }
+      return;
+    }
+    if (n == 0) return;
+    double source_z = plane_redshifts.back();
+    if (nthreads < 1) nthreads = 1;
+    size_t chunk;
+    do {
+      chunk = n / (size_t)nthreads;
+      if (chunk == 0) nthreads /= 2;
+    } while (chunk == 0);
+    std::vector<std::thread> thr;
+    for (int t = 0; t < nthreads; ++t) {
+      size_t size = chunk;
+      if (t == nthreads - 1) size = n - (size_t)(nthreads - 1) * chunk;
+      size_t start = (size_t)t * chunk;
+      thr.emplace_back([=]() {
+        compute_points(start, start + size, x, zs_ray, source_z, y, A, cnt_ray);
+      });
+    }
+    for (auto& t : thr) t.join();
+  }
+};
+
+}  // namespace orc
+
+// ====================================================================== C ABI
+extern "C" {
+
+struct orc_counters {
+  uint64_t visits, far, leafp, bigtest, bspline;
+};
+
+double orc_coordist(double Om, double OL, double h, double z1, double z2) {
+  orc::Cosmo c{Om, OL, h};
+  return c.coorDist(z1, z2);
+}
+
+// Builds a TreeQuadParticles over n particles. Positions are given as double
+// (float positions promote exactly). Returns null and sets *err on failure.
+void* orc_tree_build(const double* x, const double* y, const float* m, const float* h,
+                     size_t n, double inv_area, int bucket, double theta, int* err) {
+  auto t = new orc::TreeQuad();
+  t->px.assign(x, x + n);
+  t->py.assign(y, y + n);
+  t->pm.assign(m, m + n);
+  t->ph.assign(h, h + n);
+  t->N = n;
+  t->inv_area = inv_area;
+  t->Nbucket = bucket;
+  t->force_theta = theta;
+  t->index.resize(n);
+  for (size_t i = 0; i < n; ++i) t->index[i] = i;
+  if (err) *err = 0;
+  try {
+    if (n > 0) {
+      t->build();
+      t->calc_moments();
+    }
+  } catch (const std::exception&) {
+    if (err) *err = 1;
+    delete t;
+    return nullptr;
+  }
+  return t;
+}
+
+void orc_tree_free(void* t) { delete static_cast<orc::TreeQuad*>(t); }
+
+size_t orc_tree_nnodes(void* tp) { return static_cast<orc::TreeQuad*>(tp)->nodes.size(); }
+
+// Node export in depth-first pre-order. Per node:
+//   dbl[13*i + ...] = p1x, p1y, p2x, p2y, cx, cy, mass, q0, q1, q2, r2crit, 0, 0
+//   ints[6*i + ...] = level, nparticles, nbig, is_leaf, first_child, parent-less flag
+void orc_tree_export(void* tp, double* dbl, int64_t* ints) {
+  auto* t = static_cast<orc::TreeQuad*>(tp);
+  // depth-first pre-order, child0 first (walkTree_recur's visiting order)
+  std::vector<int> order, stack{0};
+  while (!stack.empty()) {
+    int k = stack.back();
+    stack.pop_back();
+    order.push_back(k);
+    const auto& b = t->nodes[k];
+    for (int c = 3; c >= 0; --c)
+      if (b.child[c] >= 0) stack.push_back(b.child[c]);
+  }
+  for (size_t i = 0; i < order.size(); ++i) {
+    const auto& b = t->nodes[order[i]];
+    double* d = dbl + 13 * i;
+    d[0] = b.p1[0]; d[1] = b.p1[1]; d[2] = b.p2[0]; d[3] = b.p2[1];
+    d[4] = b.center[0]; d[5] = b.center[1]; d[6] = b.mass;
+    d[7] = b.quad[0]; d[8] = b.quad[1]; d[9] = b.quad[2]; d[10] = b.r2crit;
+    d[11] = 0; d[12] = 0;
+    int64_t* q = ints + 6 * i;
+    q[0] = b.level; q[1] = (int64_t)b.nparticles; q[2] = (int64_t)b.big.size();
+    q[3] = b.leaf() ? 1 : 0; q[4] = b.child[0]; q[5] = 0;
+  }
+}
+
+// Raw force2D_recur outputs (tree sign convention) at n rays (x interleaved).
+// out: 5 per ray = alpha0, alpha1, kappa, gamma0, gamma1.
+void orc_tree_force(void* tp, const double* rays, size_t n, double* out,
+                    orc_counters* cnt, int nthreads) {
+  auto* t = static_cast<orc::TreeQuad*>(tp);
+  auto work = [&](size_t s, size_t e) {
+    for (size_t i = s; i < e; ++i) {
+      double a[2], k, g[3];
+      orc::Counters c;
+      t->force2D(rays + 2 * i, a, &k, g, cnt ? &c : nullptr);
+      out[5 * i] = a[0]; out[5 * i + 1] = a[1]; out[5 * i + 2] = k;
+      out[5 * i + 3] = g[0]; out[5 * i + 4] = g[1];
+      if (cnt) {
+        cnt[i].visits = c.visits; cnt[i].far = c.far; cnt[i].leafp = c.leafp;
+        cnt[i].bigtest = c.bigtest; cnt[i].bspline = c.bspline;
+      }
+    }
+  };
+  if (nthreads <= 1 || n < 64) { work(0, n); return; }
+  std::vector<std::thread> th;
+  size_t chunk = (n + nthreads - 1) / nthreads;
+  for (int k = 0; k < nthreads; ++k) {
+    size_t s = k * chunk, e = std::min(n, s + chunk);
+    if (s >= e) break;
+    th.emplace_back(work, s, e);
+  }
+  for (auto& x : th) x.join();
+}
+
+// The same smoothed-particle kernel summed over every particle (no tree), in
+// the tree's sign convention. out: 5 per ray.
+void orc_direct_plane(const double* px, const double* py, const float* pm, const float* ph,
+                      size_t np, double inv_area, const double* rays, size_t n, double* out,
+                      int nthreads) {
+  auto work = [&](size_t s, size_t e) {
+    for (size_t i = s; i < e; ++i) {
+      double alpha[2] = {0, 0}, kappa = 0, gamma[2] = {0, 0};
+      const double* ray = rays + 2 * i;
+      for (size_t p = 0; p < np; ++p) {
+        double d[2] = {px[p] - ray[0], py[p] - ray[1]};
+        double rcm2 = d[0] * d[0] + d[1] * d[1];
+        if (rcm2 < 1e-20) rcm2 = 1e-20;
+        double mass = pm[p];
+        double prefac = mass / rcm2 / orc::PI;
+        alpha[0] += (inv_area * mass - prefac) * d[0];
+        alpha[1] += (inv_area * mass - prefac) * d[1];
+        double tmp = -2.0 * prefac / rcm2;
+        gamma[0] += 0.5 * (d[0] * d[0] - d[1] * d[1]) * tmp;
+        gamma[1] += d[0] * d[1] * tmp;
+        kappa -= mass * inv_area;
+        double size = ph[p];
+        if (rcm2 < 4 * size * size)
+          orc::TreeQuad::b_spline(d, std::sqrt(rcm2), mass, size, alpha, &kappa, gamma);
+      }
+      out[5 * i] = alpha[0]; out[5 * i + 1] = alpha[1]; out[5 * i + 2] = kappa;
+      out[5 * i + 3] = gamma[0]; out[5 * i + 4] = gamma[1];
+    }
+  };
+  if (nthreads <= 1 || n < 16) { work(0, n); return; }
+  std::vector<std::thread> th;
+  size_t chunk = (n + nthreads - 1) / nthreads;
+  for (int k = 0; k < nthreads; ++k) {
+    size_t s = k * chunk, e = std::min(n, s + chunk);
+    if (s >= e) break;
+    th.emplace_back(work, s, e);
+  }
+  for (auto& x : th) x.join();
+}
+
+// B-spline correction alone (for closed-form checks). out: alpha0, alpha1, kappa, gamma0, gamma1
+void orc_bspline(const double* xcm, double r, double mass, double size, double* out) {
+  double a[2] = {0, 0}, k = 0, g[2] = {0, 0};
+  orc::TreeQuad::b_spline(xcm, r, mass, size, a, &k, g);
+  out[0] = a[0]; out[1] = a[1]; out[2] = k; out[3] = g[0]; out[4] = g[1];
+}
+
+void* orc_lens_create(double zsource, double Om, double OL, double h) {
+  return new orc::Lens(zsource, Om, OL, h);
+}
+void orc_lens_free(void* l) { delete static_cast<orc::Lens*>(l); }
+
+// Moves a built tree into the lens as one LensHaloParticles at float redshift z.
+void orc_lens_add_halo(void* lp, void* tree, float z) {
+  auto* L = static_cast<orc::Lens*>(lp);
+  auto h = std::make_unique<orc::Halo>();
+  h->tree.reset(static_cast<orc::TreeQuad*>(tree));
+  h->z = z;
+  L->moveinMainHalo(std::move(h));
+}
+
+int orc_lens_nplanes(void* lp) { return (int)static_cast<orc::Lens*>(lp)->planes.size(); }
+
+// Copies plane_redshifts, Dl and dDl (each nplanes + 1 long).
+void orc_lens_tables(void* lp, double* z, double* Dl, double* dDl, double* charge) {
+  auto* L = static_cast<orc::Lens*>(lp);
+  for (size_t i = 0; i < L->plane_redshifts.size(); ++i) {
+    z[i] = L->plane_redshifts[i];
+    Dl[i] = L->Dl[i];
+    dDl[i] = L->dDl[i];
+  }
+  *charge = L->charge;
+}
+
+// rayshooterInternal over n rays: x (2n) -> y (2n), A (4n). zs: per-ray source
+// redshifts (LinkedPoint/RAY overloads) or null (Point overload).
+void orc_lens_shoot(void* lp, size_t n, const double* x, const double* zs, double* y,
+                    double* A, int nthreads, orc_counters* cnt) {
+  auto* L = static_cast<orc::Lens*>(lp);
+  std::vector<orc::Counters> c;
+  if (cnt) c.resize(n);
+  L->shoot(n, x, zs, y, A, nthreads, cnt ? c.data() : nullptr);
+  if (cnt)
+    for (size_t i = 0; i < n; ++i) {
+      cnt[i].visits = c[i].visits; cnt[i].far = c[i].far; cnt[i].leafp = c[i].leafp;
+      cnt[i].bigtest = c[i].bigtest; cnt[i].bspline = c[i].bspline;
+    }
+}
+
+// GridMap square-grid ray positions (TreeCode_link/gridmap.cpp:293-321 via
+// Utilities::PositionFromIndex, TreeCode_link/utilities.cpp:149-158).
+void orc_gridmap_positions(size_t N, double c0, double c1, double range, double* x) {
+  const double center[2] = {c0, c1};
+  for (size_t i = 0; i < N * N; ++i) {
+    if (N == 1) { x[2 * i] = center[0]; x[2 * i + 1] = center[1]; continue; }
+    x[2 * i] = center[0] + range * (1.0 * (i % N) / (N - 1) - 0.5);
+    x[2 * i + 1] = center[1] + range * (1.0 * (i / N) / (N - 1) - 0.5);
+  }
+}
+
+}  // extern "C"

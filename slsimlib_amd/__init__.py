@@ -1,0 +1,346 @@
+"""MI355X multi-plane particle ray shooter (GLAMER's Lens / GridMap /
+LensHaloParticles particle path), Python view over the C ABI in
+include/slsim_amd.h.
+
+The compute path is the HIP library libslsim_amd.so built in-tree; importing
+this package fails loudly if it is missing (there is no CPU fallback).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import SLSIM_DIRECT64, SLSIM_FP32C, SLSIM_FP64, SlsimError, lib
+
+arcsecTOradians = 0.000004848136811   # include/standard.h:51
+Grav = 4.7788e-20                      # include/standard.h:43
+
+__all__ = [
+    "Context", "LensHaloParticles", "Lens", "GridMap", "coorDist", "invCoorDist",
+    "nfw_r200", "gen_nfw", "gen_halo_field", "SLSIM_FP64", "SLSIM_FP32C", "SLSIM_DIRECT64",
+    "SlsimError",
+    "arcsecTOradians", "Grav", "lib",
+]
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _check(rc):
+    _abi.check(rc)
+
+
+class Context:
+    """Device context with an HBM budget (0 = 90 % of free HBM)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        self.h = _abi.slsim_ctx()
+        _check(lib.slsim_ctx_create(int(device), ctypes.c_size_t(int(hbm_budget)),
+                                    ctypes.byref(self.h)))
+
+    def memory(self):
+        used, bud = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(lib.slsim_ctx_memory(self.h, ctypes.byref(used), ctypes.byref(bud)))
+        return used.value, bud.value
+
+    def synchronize(self):
+        _check(lib.slsim_ctx_synchronize(self.h))
+
+    def alloc(self, nbytes):
+        p = ctypes.c_void_p()
+        _check(lib.slsim_device_alloc(self.h, ctypes.c_size_t(int(nbytes)), ctypes.byref(p)))
+        return p
+
+    def free(self, p):
+        _check(lib.slsim_device_free(self.h, p))
+
+    def d2h(self, host, dev):
+        _check(lib.slsim_memcpy_d2h(self.h, host.ctypes.data_as(ctypes.c_void_p), dev,
+                                    ctypes.c_size_t(host.nbytes)))
+
+    def h2d(self, dev, host):
+        _check(lib.slsim_memcpy_h2d(self.h, dev, host.ctypes.data_as(ctypes.c_void_p),
+                                    ctypes.c_size_t(host.nbytes)))
+
+    def close(self):
+        if self.h:
+            lib.slsim_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LensHaloParticles:
+    """Particle halo with its device quadtree (particle_halo.h:58-75).
+
+    x, y: positions in physical Mpc (float64 -> ParticleType<double>,
+    float32 -> ParticleType<float>); m: masses (Msun, stored float32);
+    h: sizes (Mpc, float32)."""
+
+    def __init__(self, ctx, x, y, m, h, z, theta=0.1, bucket=20, inv_area=0.0):
+        self.ctx = ctx
+        self.z = np.float32(z)
+        self.owned_by_lens = False
+        m = np.ascontiguousarray(m, dtype=np.float32)
+        h = np.ascontiguousarray(h, dtype=np.float32)
+        n = m.size
+        self.h = _abi.slsim_plane()
+        if np.asarray(x).dtype == np.float32:
+            xs = np.ascontiguousarray(x, dtype=np.float32)
+            ys = np.ascontiguousarray(y, dtype=np.float32)
+            fn = lib.slsim_plane_create_f32
+            ct = ctypes.c_float
+        else:
+            xs = np.ascontiguousarray(x, dtype=np.float64)
+            ys = np.ascontiguousarray(y, dtype=np.float64)
+            fn = lib.slsim_plane_create_f64
+            ct = ctypes.c_double
+        _check(fn(ctx.h, _ptr(xs, ct), _ptr(ys, ct), _ptr(m, ctypes.c_float),
+                  _ptr(h, ctypes.c_float), ctypes.c_size_t(n), float(theta), int(bucket),
+                  float(inv_area), ctypes.byref(self.h)))
+
+    def info(self):
+        i = _abi.slsim_plane_info()
+        _check(lib.slsim_plane_get_info(self.h, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in i._fields_ if k != "pad"}
+
+    def export_nodes(self):
+        nn = self.info()["n_nodes"]
+        d = np.zeros((nn, 13), np.float64)
+        q = np.zeros((nn, 6), np.int64)
+        _check(lib.slsim_plane_export_nodes(self.h, _ptr(d, ctypes.c_double),
+                                            _ptr(q, ctypes.c_int64)))
+        return d, q
+
+    def force(self, rays, precision=SLSIM_FP64):
+        """Tree walk at physical positions rays (n, 2): raw force2D_recur
+        outputs (alpha0, alpha1, kappa, gamma0, gamma1)."""
+        r = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 2)
+        out = np.zeros((r.shape[0], 5), np.float64)
+        _check(lib.slsim_plane_force(self.h, ctypes.c_size_t(r.shape[0]),
+                                     _ptr(r, ctypes.c_double), _ptr(out, ctypes.c_double),
+                                     int(precision)))
+        return out
+
+    def direct_sum(self, rays):
+        r = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 2)
+        out = np.zeros((r.shape[0], 5), np.float64)
+        _check(lib.slsim_plane_direct_sum(self.h, ctypes.c_size_t(r.shape[0]),
+                                          _ptr(r, ctypes.c_double), _ptr(out, ctypes.c_double)))
+        return out
+
+    def close(self):
+        if self.h and not self.owned_by_lens:
+            lib.slsim_plane_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Lens:
+    """Lens(seed, zs, cosmo) with flat LCDM (MultiPlane/lens.cpp:59-77)."""
+
+    def __init__(self, ctx, zsource, Om=0.3, OL=0.7, h=0.7):
+        self.ctx = ctx
+        self.cosmo = (Om, OL, h)
+        self.h = _abi.slsim_lens()
+        self.halos = []
+        _check(lib.slsim_lens_create(ctx.h, float(zsource), float(Om), float(OL), float(h),
+                                     ctypes.byref(self.h)))
+
+    def moveinMainHalo(self, halo):
+        _check(lib.slsim_lens_add_halo(self.h, halo.h, ctypes.c_float(float(halo.z))))
+        halo.owned_by_lens = True
+        self.halos.append(halo)
+
+    def ResetSourcePlane(self, z):
+        _check(lib.slsim_lens_reset_source(self.h, float(z)))
+
+    def tables(self):
+        n = ctypes.c_int()
+        _check(lib.slsim_lens_tables(self.h, ctypes.byref(n), None, None, None, None))
+        k = n.value + 1
+        z, Dl, dDl = (np.zeros(k) for _ in range(3))
+        ch = ctypes.c_double()
+        _check(lib.slsim_lens_tables(self.h, ctypes.byref(n), _ptr(z, ctypes.c_double),
+                                     _ptr(Dl, ctypes.c_double), _ptr(dDl, ctypes.c_double),
+                                     ctypes.byref(ch)))
+        return {"nplanes": n.value, "z": z, "Dl": Dl, "dDl": dDl, "charge": ch.value}
+
+    def rayshooterInternal(self, x, zs=None, precision=SLSIM_FP64, stats=None):
+        """x: (n, 2) image positions (radians) -> (y (n, 2), A (n, 4))."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 2)
+        n = x.shape[0]
+        y = np.zeros((n, 2))
+        A = np.zeros((n, 4))
+        zp = None
+        if zs is not None:
+            zs = np.ascontiguousarray(zs, dtype=np.float64).reshape(-1)
+            if zs.size != n:
+                raise ValueError("missing redshifts")
+            zp = _ptr(zs, ctypes.c_double)
+        st = _abi.slsim_shoot_stats()
+        _check(lib.slsim_shoot(self.h, ctypes.c_size_t(n), _ptr(x, ctypes.c_double), zp,
+                               _ptr(y, ctypes.c_double), _ptr(A, ctypes.c_double),
+                               int(precision), ctypes.byref(st)))
+        if stats is not None:
+            stats.update({k: getattr(st, k) for k, _ in st._fields_ if k != "pad"})
+        return y, A
+
+    def close(self):
+        if self.h:
+            lib.slsim_lens_destroy(self.h)
+            for hl in self.halos:
+                hl.h = None
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GridMap:
+    """GridMap(lens, N, center, range) (TreeCode_link/gridmap.cpp:93-120):
+    shoots the N x N grid on the device; outputs stay in HBM until read."""
+
+    def __init__(self, lens, N, center, range_, precision=SLSIM_FP64, zs=0.0, shoot=True):
+        self.lens = lens
+        self.ctx = lens.ctx
+        self.N = int(N)
+        self.center = (float(center[0]), float(center[1]))
+        self.range = float(range_)
+        self.precision = precision
+        self.zs = float(zs)
+        n = self.N * self.N
+        self.y_dev = self.ctx.alloc(16 * n)
+        self.A_dev = self.ctx.alloc(32 * n)
+        self.stats = {}
+        if shoot:
+            self.shoot()
+
+    def shoot(self):
+        st = _abi.slsim_shoot_stats()
+        _check(lib.slsim_gridmap_shoot(self.lens.h, ctypes.c_size_t(self.N), self.center[0],
+                                       self.center[1], self.range, self.zs,
+                                       ctypes.cast(self.y_dev, ctypes.POINTER(ctypes.c_double)),
+                                       ctypes.cast(self.A_dev, ctypes.POINTER(ctypes.c_double)),
+                                       int(self.precision), ctypes.byref(st)))
+        self.stats = {k: getattr(st, k) for k, _ in st._fields_ if k != "pad"}
+        return self.stats
+
+    def positions(self):
+        """Image positions of the grid (x fastest), PositionFromIndex."""
+        N = self.N
+        i = np.arange(N * N)
+        if N == 1:
+            return np.array([[self.center[0], self.center[1]]])
+        x0 = self.center[0] + self.range * (1.0 * (i % N) / (N - 1) - 0.5)
+        x1 = self.center[1] + self.range * (1.0 * (i // N) / (N - 1) - 0.5)
+        return np.stack([x0, x1], axis=1)
+
+    def y(self):
+        out = np.zeros((self.N * self.N, 2))
+        self.ctx.d2h(out, self.y_dev)
+        return out
+
+    def A(self):
+        out = np.zeros((self.N * self.N, 4))
+        self.ctx.d2h(out, self.A_dev)
+        return out
+
+    def counts(self):
+        n = self.N * self.N
+        out = np.zeros((n, 5), np.uint64)
+        _check(lib.slsim_gridmap_count(self.lens.h, ctypes.c_size_t(self.N), self.center[0],
+                                       self.center[1], self.range, self.zs,
+                                       _ptr(out, ctypes.c_uint64)))
+        return out
+
+    def writePixelMap(self, var):
+        """LensingVariable maps of gridmap.h:533-603 as (N, N) arrays."""
+        A = self.A()
+        y = self.y()
+        x = self.positions()
+        v = var.upper()
+        if v == "ALPHA1":
+            r = x[:, 0] - y[:, 0]
+        elif v == "ALPHA2":
+            r = x[:, 1] - y[:, 1]
+        elif v == "ALPHA":
+            r = np.hypot(x[:, 0] - y[:, 0], x[:, 1] - y[:, 1])
+        elif v == "KAPPA":
+            r = 1 - (A[:, 0] + A[:, 3]) / 2
+        elif v == "GAMMA1":
+            r = (A[:, 0] - A[:, 3]) / 2
+        elif v == "GAMMA2":
+            r = (A[:, 1] + A[:, 2]) / 2
+        elif v == "GAMMA3":
+            r = (A[:, 2] - A[:, 1]) / 2
+        elif v == "GAMMA":
+            r = np.hypot((A[:, 0] - A[:, 3]) / 2, (A[:, 1] + A[:, 2]) / 2)
+        elif v == "INVMAG":
+            r = A[:, 0] * A[:, 3] - A[:, 1] * A[:, 2]
+        elif v == "DELAYT":
+            r = np.zeros(self.N * self.N)
+        else:
+            raise ValueError("GridMap::writePixelMap() does not work for the input LensingVariable")
+        return r.reshape(self.N, self.N)
+
+    def close(self):
+        if self.y_dev:
+            self.ctx.free(self.y_dev)
+            self.ctx.free(self.A_dev)
+            self.y_dev = self.A_dev = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def coorDist(z1, z2, Om=0.3, OL=0.7, h=0.7):
+    return lib.slsim_coordist(float(Om), float(OL), float(h), float(z1), float(z2))
+
+
+def invCoorDist(D, Om=0.3, OL=0.7, h=0.7):
+    return lib.slsim_inv_coordist(float(Om), float(OL), float(h), float(D))
+
+
+def nfw_r200(mass, z, Om=0.3, OL=0.7, h=0.7):
+    return lib.slsim_nfw_r200(float(mass), float(z), float(Om), float(OL), float(h))
+
+
+def gen_nfw(seed, n, mass, conc, z, Om=0.3, OL=0.7, h=0.7, center=(0.0, 0.0)):
+    x = np.zeros(n)
+    y = np.zeros(n)
+    _check(lib.slsim_gen_nfw(ctypes.c_uint64(seed), ctypes.c_size_t(n), float(mass), float(conc),
+                             float(z), float(Om), float(OL), float(h), float(center[0]),
+                             float(center[1]), _ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double)))
+    return x, y
+
+
+def gen_halo_field(seed, n, z, n_halos=50, Mmin=1e11, Mmax=1e14, field_arcsec=300.0,
+                   softening=5e-4, Om=0.3, OL=0.7, h=0.7):
+    x = np.zeros(n)
+    y = np.zeros(n)
+    m = np.zeros(n, np.float32)
+    s = np.zeros(n, np.float32)
+    _check(lib.slsim_gen_halo_field(ctypes.c_uint64(seed), ctypes.c_size_t(n), int(n_halos),
+                                    float(Mmin), float(Mmax), float(field_arcsec), float(z),
+                                    float(Om), float(OL), float(h), ctypes.c_float(softening),
+                                    _ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double),
+                                    _ptr(m, ctypes.c_float), _ptr(s, ctypes.c_float)))
+    return x, y, m, s

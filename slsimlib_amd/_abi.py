@@ -1,0 +1,114 @@
+"""ctypes declarations of include/slsim_amd.h and loading of libslsim_amd.so."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libslsim_amd.so")
+
+SLSIM_OK = 0
+SLSIM_ERR_ARG = 1
+SLSIM_ERR_HIP = 2
+SLSIM_ERR_BUDGET = 3
+SLSIM_ERR_SAME_PLACE = 4
+SLSIM_ERR_TOO_DEEP = 5
+SLSIM_ERR_NOT_FLAT = 6
+SLSIM_ERR_STATE = 7
+
+SLSIM_FP64 = 0
+SLSIM_FP32C = 1
+SLSIM_DIRECT64 = 2
+
+ERROR_NAMES = {
+    1: "SLSIM_ERR_ARG", 2: "SLSIM_ERR_HIP", 3: "SLSIM_ERR_BUDGET", 4: "SLSIM_ERR_SAME_PLACE",
+    5: "SLSIM_ERR_TOO_DEEP", 6: "SLSIM_ERR_NOT_FLAT", 7: "SLSIM_ERR_STATE",
+}
+
+
+class SlsimError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (ERROR_NAMES.get(code, str(code)), msg))
+        self.code = code
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        "libslsim_amd.so is not built (%s); run `python -m slsimlib_amd.build` or "
+        "__graft_entry__.build()" % LIB_PATH)
+
+lib = ctypes.CDLL(LIB_PATH)
+
+slsim_ctx = ctypes.c_void_p
+slsim_plane = ctypes.c_void_p
+slsim_lens = ctypes.c_void_p
+
+
+class slsim_plane_info(ctypes.Structure):
+    _fields_ = [("n_particles", ctypes.c_int64), ("n_nodes", ctypes.c_int64),
+                ("n_leaves", ctypes.c_int64), ("n_big_internal", ctypes.c_int64),
+                ("depth", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("device_bytes", ctypes.c_int64), ("build_ms", ctypes.c_double),
+                ("root_p1", ctypes.c_double * 2), ("root_p2", ctypes.c_double * 2)]
+
+
+class slsim_shoot_stats(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("walk_ms", ctypes.c_double),
+                ("walk_ms_max_plane", ctypes.c_double), ("n_walk_launches", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
+P = ctypes.POINTER
+c_int, c_double, c_float, c_size_t = ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_size_t
+c_u64, c_i64, c_vp = ctypes.c_uint64, ctypes.c_int64, ctypes.c_void_p
+
+SIGNATURES = {
+    "slsim_last_error": (ctypes.c_char_p, []),
+    "slsim_ctx_create": (c_int, [c_int, c_size_t, P(slsim_ctx)]),
+    "slsim_ctx_destroy": (c_int, [slsim_ctx]),
+    "slsim_ctx_memory": (c_int, [slsim_ctx, P(c_size_t), P(c_size_t)]),
+    "slsim_ctx_synchronize": (c_int, [slsim_ctx]),
+    "slsim_plane_create_f64": (c_int, [slsim_ctx, P(c_double), P(c_double), P(c_float), P(c_float),
+                                       c_size_t, c_double, c_int, c_double, P(slsim_plane)]),
+    "slsim_plane_create_f32": (c_int, [slsim_ctx, P(c_float), P(c_float), P(c_float), P(c_float),
+                                       c_size_t, c_double, c_int, c_double, P(slsim_plane)]),
+    "slsim_plane_destroy": (c_int, [slsim_plane]),
+    "slsim_plane_get_info": (c_int, [slsim_plane, P(slsim_plane_info)]),
+    "slsim_plane_export_nodes": (c_int, [slsim_plane, P(c_double), P(c_i64)]),
+    "slsim_lens_create": (c_int, [slsim_ctx, c_double, c_double, c_double, c_double, P(slsim_lens)]),
+    "slsim_lens_destroy": (c_int, [slsim_lens]),
+    "slsim_lens_add_halo": (c_int, [slsim_lens, slsim_plane, c_float]),
+    "slsim_lens_tables": (c_int, [slsim_lens, P(c_int), P(c_double), P(c_double), P(c_double),
+                                  P(c_double)]),
+    "slsim_lens_reset_source": (c_int, [slsim_lens, c_double]),
+    "slsim_shoot": (c_int, [slsim_lens, c_size_t, P(c_double), P(c_double), P(c_double),
+                            P(c_double), c_int, P(slsim_shoot_stats)]),
+    "slsim_shoot_device": (c_int, [slsim_lens, c_size_t, P(c_double), P(c_double), P(c_double),
+                                   P(c_double), c_int, P(slsim_shoot_stats)]),
+    "slsim_gridmap_shoot": (c_int, [slsim_lens, c_size_t, c_double, c_double, c_double, c_double,
+                                    P(c_double), P(c_double), c_int, P(slsim_shoot_stats)]),
+    "slsim_gridmap_count": (c_int, [slsim_lens, c_size_t, c_double, c_double, c_double, c_double,
+                                    P(c_u64)]),
+    "slsim_plane_direct_sum": (c_int, [slsim_plane, c_size_t, P(c_double), P(c_double)]),
+    "slsim_plane_force": (c_int, [slsim_plane, c_size_t, P(c_double), P(c_double), c_int]),
+    "slsim_device_alloc": (c_int, [slsim_ctx, c_size_t, P(c_vp)]),
+    "slsim_device_free": (c_int, [slsim_ctx, c_vp]),
+    "slsim_memcpy_d2h": (c_int, [slsim_ctx, c_vp, c_vp, c_size_t]),
+    "slsim_memcpy_h2d": (c_int, [slsim_ctx, c_vp, c_vp, c_size_t]),
+    "slsim_coordist": (c_double, [c_double, c_double, c_double, c_double, c_double]),
+    "slsim_inv_coordist": (c_double, [c_double, c_double, c_double, c_double]),
+    "slsim_nfw_r200": (c_double, [c_double, c_double, c_double, c_double, c_double]),
+    "slsim_gen_nfw": (c_int, [c_u64, c_size_t, c_double, c_double, c_double, c_double, c_double,
+                              c_double, c_double, c_double, P(c_double), P(c_double)]),
+    "slsim_gen_halo_field": (c_int, [c_u64, c_size_t, c_int, c_double, c_double, c_double,
+                                     c_double, c_double, c_double, c_double, c_float, P(c_double),
+                                     P(c_double), P(c_float), P(c_float)]),
+}
+
+for _name, (_res, _args) in SIGNATURES.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def check(rc):
+    if rc != SLSIM_OK:
+        raise SlsimError(rc, lib.slsim_last_error().decode(errors="replace"))

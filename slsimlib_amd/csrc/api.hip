@@ -1,0 +1,784 @@
+// api.hip -- C ABI of the ray shooter: contexts, planes, lenses, shooting.
+//
+// The lens bookkeeping restates the reference's plane handling:
+//   Lens::addMainHaloToPlane   MultiPlane/lens.cpp:772-815
+//   Lens::combinePlanes        MultiPlane/lens.cpp:2397-2515
+//   Lens::FindSourcePlane      MultiPlane/lens.cpp:2659-2681
+//   Lens::ResetSourcePlane     MultiPlane/lens.cpp:2609-2657 (nearest = false)
+//   Lens::rayshooterInternal   FullRange/internal_rayshooter_multi.cpp:53-251
+// Shooting runs one kernel launch per lens plane over all 64-ray bundles; the
+// per-ray FP64 state lives in HBM between planes.
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "cosmo.h"
+#include "slsim_internal.h"
+
+namespace slsim {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+
+int arg_error(const char* msg) {
+  set_error(msg);
+  return SLSIM_ERR_ARG;
+}
+
+// Device scratch for one shoot call (ray state, jobs, permutations).
+struct Scratch {
+  slsim_ctx ctx;
+  std::vector<void*> ptrs;
+  explicit Scratch(slsim_ctx c) : ctx(c) {}
+  ~Scratch() {
+    for (void* p : ptrs) ctx->scratch_put(p);
+  }
+  template <typename T>
+  int get(T** p, size_t count) {
+    void* v = nullptr;
+    int rc = ctx->scratch_get(&v, std::max<size_t>(count, 1) * sizeof(T));
+    if (rc) return rc;
+    ptrs.push_back(v);
+    *p = static_cast<T*>(v);
+    return SLSIM_OK;
+  }
+};
+
+int alloc_state(Scratch& s, int64_t nslots, RayState* st) {
+  double* d;
+  int rc = s.get(&d, 14 * (size_t)nslots);
+  if (rc) return rc;
+  double** f[14] = {&st->th0, &st->th1, &st->sa0, &st->sa1, &st->A0, &st->A1, &st->A2,
+                    &st->A3,  &st->G0,  &st->G1,  &st->G2,  &st->G3, &st->aal, &st->bbl};
+  for (int k = 0; k < 14; ++k) *f[k] = d + (size_t)k * nslots;
+  return s.get(&st->jmax, nslots);
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+}  // namespace
+}  // namespace slsim
+
+using namespace slsim;
+
+// ------------------------------------------------------------------ context
+int slsim_ctx_s::alloc(void** p, size_t bytes) {
+  std::lock_guard<std::mutex> g(mu);
+  if (used + bytes > budget) {
+    set_error("HBM budget exceeded: need " + std::to_string(bytes) + " B, " +
+              std::to_string(budget - used) + " B left");
+    return SLSIM_ERR_BUDGET;
+  }
+  hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return SLSIM_ERR_HIP;
+  }
+  used += bytes;
+  allocs.push_back({*p, bytes});
+  return SLSIM_OK;
+}
+
+int slsim_ctx_s::release(void* p) {
+  if (!p) return SLSIM_OK;
+  std::lock_guard<std::mutex> g(mu);
+  for (size_t i = 0; i < allocs.size(); ++i) {
+    if (allocs[i].first == p) {
+      used -= allocs[i].second;
+      allocs[i] = allocs.back();
+      allocs.pop_back();
+      (void)hipFree(p);
+      return SLSIM_OK;
+    }
+  }
+  set_error("release of an unknown device pointer");
+  return SLSIM_ERR_ARG;
+}
+
+// Scratch blocks: best fit among cached blocks no more than twice the
+// request, else a fresh allocation (dropping the cache if the budget is short).
+int slsim_ctx_s::scratch_get(void** p, size_t bytes) {
+  {
+    std::lock_guard<std::mutex> g(mu);
+    size_t best = (size_t)-1, best_sz = (size_t)-1;
+    for (size_t i = 0; i < cache.size(); ++i) {
+      const size_t sz = cache[i].second;
+      if (sz >= bytes && sz <= 2 * bytes + 4096 && sz < best_sz) {
+        best = i;
+        best_sz = sz;
+      }
+    }
+    if (best != (size_t)-1) {
+      *p = cache[best].first;
+      cache[best] = cache.back();
+      cache.pop_back();
+      return SLSIM_OK;
+    }
+  }
+  int rc = alloc(p, bytes);
+  if (rc == SLSIM_ERR_BUDGET && !cache.empty()) {
+    trim_cache();
+    rc = alloc(p, bytes);
+  }
+  return rc;
+}
+
+void slsim_ctx_s::scratch_put(void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(mu);
+  for (auto& a : allocs)
+    if (a.first == p) {
+      cache.push_back(a);
+      return;
+    }
+}
+
+void slsim_ctx_s::trim_cache() {
+  std::vector<std::pair<void*, size_t>> c;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    c.swap(cache);
+  }
+  for (auto& a : c) release(a.first);
+}
+
+TreeDesc slsim_plane_s::desc() const {
+  TreeDesc t;
+  t.nodesD = nodesD;
+  t.nodesF = nodesF;
+  t.partsD = partsD;
+  t.bigD = bigD;
+  t.inv_area = inv_area;
+  t.nnodes = (int32_t)nnodes;
+  t.nparts = (int32_t)n;
+  return t;
+}
+
+extern "C" {
+
+const char* slsim_last_error(void) { return g_last_error.c_str(); }
+
+int slsim_ctx_create(int device, size_t hbm_budget_bytes, slsim_ctx* out) {
+  if (!out) return arg_error("null output handle");
+  int ndev = 0;
+  SLSIM_HIP_CHECK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return arg_error("no such device");
+  SLSIM_HIP_CHECK(hipSetDevice(device));
+  auto c = std::make_unique<slsim_ctx_s>();
+  c->device = device;
+  size_t fr = 0, tot = 0;
+  SLSIM_HIP_CHECK(hipMemGetInfo(&fr, &tot));
+  c->budget = hbm_budget_bytes ? hbm_budget_bytes : (size_t)(0.9 * (double)fr);
+  SLSIM_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  *out = c.release();
+  return SLSIM_OK;
+}
+
+int slsim_ctx_destroy(slsim_ctx ctx) {
+  if (!ctx) return SLSIM_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& a : ctx->allocs) (void)hipFree(a.first);
+  ctx->allocs.clear();
+  ctx->cache.clear();
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SLSIM_OK;
+}
+
+int slsim_ctx_memory(slsim_ctx ctx, size_t* used, size_t* budget) {
+  if (!ctx) return arg_error("null context");
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->trim_cache();
+  if (used) *used = ctx->used;
+  if (budget) *budget = ctx->budget;
+  return SLSIM_OK;
+}
+
+int slsim_ctx_synchronize(slsim_ctx ctx) {
+  if (!ctx) return arg_error("null context");
+  SLSIM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return SLSIM_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ planes
+static int plane_free_device(slsim_plane p) {
+  slsim_ctx c = p->ctx;
+  void* ptrs[] = {p->nodesD, p->nodesF, p->partsD, p->bigD,
+                  p->box_p1, p->box_p2, p->nbig,   p->npart};
+  for (void* q : ptrs)
+    if (q) c->release(q);
+  return SLSIM_OK;
+}
+
+template <typename T>
+static int plane_create(slsim_ctx ctx, const T* x, const T* y, const float* m, const float* h,
+                        size_t n, double theta, int bucket, double inv_area, slsim_plane* out) {
+  if (!ctx || !out) return arg_error("null handle");
+  if (n == 0 || !x || !y || !m || !h) return arg_error("empty particle set");
+  if (n >= (size_t)1 << 31) return arg_error("more than 2^31 particles in one plane");
+  if (bucket < 1) return arg_error("bucket must be >= 1");
+  SLSIM_HIP_CHECK(hipSetDevice(ctx->device));
+  auto p = std::make_unique<slsim_plane_s>();
+  p->ctx = ctx;
+  p->n = (int64_t)n;
+  p->theta = theta;
+  p->bucket = bucket;
+  p->inv_area = inv_area;
+  int rc = build_plane<T>(ctx, x, y, m, h, n, p.get());
+  if (rc) {
+    plane_free_device(p.get());
+    return rc;
+  }
+  *out = p.release();
+  return SLSIM_OK;
+}
+
+extern "C" {
+
+int slsim_plane_create_f64(slsim_ctx ctx, const double* x, const double* y, const float* m,
+                           const float* h, size_t n, double theta, int bucket, double inv_area,
+                           slsim_plane* out) {
+  return plane_create<double>(ctx, x, y, m, h, n, theta, bucket, inv_area, out);
+}
+
+int slsim_plane_create_f32(slsim_ctx ctx, const float* x, const float* y, const float* m,
+                           const float* h, size_t n, double theta, int bucket, double inv_area,
+                           slsim_plane* out) {
+  return plane_create<float>(ctx, x, y, m, h, n, theta, bucket, inv_area, out);
+}
+
+int slsim_plane_destroy(slsim_plane plane) {
+  if (!plane) return SLSIM_OK;
+  (void)hipStreamSynchronize(plane->ctx->stream);
+  plane_free_device(plane);
+  delete plane;
+  return SLSIM_OK;
+}
+
+int slsim_plane_get_info(slsim_plane p, slsim_plane_info* info) {
+  if (!p || !info) return arg_error("null handle");
+  std::memset(info, 0, sizeof(*info));
+  info->n_particles = p->n;
+  info->n_nodes = p->nnodes;
+  info->n_leaves = p->nleaves;
+  info->n_big_internal = p->nbig_internal;
+  info->depth = p->depth;
+  info->device_bytes = (int64_t)p->bytes;
+  info->build_ms = p->build_ms;
+  info->root_p1[0] = p->p1[0]; info->root_p1[1] = p->p1[1];
+  info->root_p2[0] = p->p2[0]; info->root_p2[1] = p->p2[1];
+  return SLSIM_OK;
+}
+
+int slsim_plane_export_nodes(slsim_plane p, double* dbl, int64_t* ints) {
+  if (!p || !dbl || !ints) return arg_error("null argument");
+  SLSIM_HIP_CHECK(hipSetDevice(p->ctx->device));
+  return export_nodes(p, dbl, ints);
+}
+
+int slsim_plane_force(slsim_plane p, size_t n, const double* rays, double* out, int precision) {
+  if (!p || (n && (!rays || !out))) return arg_error("null argument");
+  if (n == 0) return SLSIM_OK;
+  slsim_ctx c = p->ctx;
+  SLSIM_HIP_CHECK(hipSetDevice(c->device));
+  Scratch s(c);
+  TreeDesc* dt;
+  double *dr, *dout;
+  uint32_t* err;
+  int rc;
+  if ((rc = s.get(&dt, 1)) || (rc = s.get(&dr, 2 * n)) || (rc = s.get(&dout, 5 * n)) ||
+      (rc = s.get(&err, 1)))
+    return rc;
+  TreeDesc t = p->desc();
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dt, &t, sizeof(t), hipMemcpyHostToDevice, c->stream));
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dr, rays, 16 * n, hipMemcpyHostToDevice, c->stream));
+  SLSIM_HIP_CHECK(hipMemsetAsync(err, 0, 4, c->stream));
+  launch_force(c->stream, precision, dt, dr, (int64_t)n, dout, err);
+  SLSIM_HIP_CHECK(hipGetLastError());
+  uint32_t herr = 0;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(out, dout, 40 * n, hipMemcpyDeviceToHost, c->stream));
+  SLSIM_HIP_CHECK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (herr) { set_error("traversal stack overflow"); return SLSIM_ERR_TOO_DEEP; }
+  return SLSIM_OK;
+}
+
+int slsim_plane_direct_sum(slsim_plane p, size_t n, const double* rays, double* out) {
+  if (!p || (n && (!rays || !out))) return arg_error("null argument");
+  if (n == 0) return SLSIM_OK;
+  slsim_ctx c = p->ctx;
+  SLSIM_HIP_CHECK(hipSetDevice(c->device));
+  Scratch s(c);
+  double *dr, *dout;
+  int rc;
+  if ((rc = s.get(&dr, 2 * n)) || (rc = s.get(&dout, 5 * n))) return rc;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dr, rays, 16 * n, hipMemcpyHostToDevice, c->stream));
+  launch_direct(c->stream, p->partsD, p->n, p->inv_area, dr, (int64_t)n, dout);
+  SLSIM_HIP_CHECK(hipGetLastError());
+  SLSIM_HIP_CHECK(hipMemcpyAsync(out, dout, 40 * n, hipMemcpyDeviceToHost, c->stream));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return SLSIM_OK;
+}
+
+}  // extern "C"
+
+// -------------------------------------------------------------------- lens
+double slsim_lens_s::coorDist(double z1, double z2) const {
+  FlatLCDM c{Om, OL, h};
+  return c.coorDist(z1, z2);
+}
+
+void slsim_lens_s::combinePlanes() {
+  plane_redshifts.clear();
+  Dl = main_Dl;
+  dDl.clear();
+  order.resize(Dl.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return Dl[a] < Dl[b]; });
+  std::sort(Dl.begin(), Dl.end());
+  for (size_t i = 1; i < Dl.size(); ++i)
+    if ((Dl[i] - Dl[i - 1]) < kMinPlaneDist) Dl[i] = Dl[i - 1] + kMinPlaneDist;
+  for (int i : order) plane_redshifts.push_back(main_z[i]);
+  if (!plane_redshifts.empty() && zsource <= plane_redshifts.back())
+    zsource = plane_redshifts.back() + 0.1;
+  plane_redshifts.push_back(zsource);
+  Dl.push_back(coorDist(0, zsource));
+  dDl.push_back(Dl[0]);
+  for (size_t i = 1; i < Dl.size(); ++i) dDl.push_back(Dl[i] - Dl[i - 1]);
+  trees_dirty = true;
+}
+
+void slsim_lens_s::FindSourcePlane(double zs, long& jmax, double& Dls, double& Ds) const {
+  Ds = coorDist(0, zs);
+  if (zs <= plane_redshifts[0]) { jmax = 0; Dls = Ds; return; }
+  jmax = (long)plane_redshifts.size() - 1;
+  if (jmax <= 0) { Dls = Ds; return; }
+  while (plane_redshifts[jmax - 1] > zs) --jmax;
+  Dls = coorDist(plane_redshifts[jmax - 1], zs);
+}
+
+int slsim_lens_s::upload_trees() {
+  if (!trees_dirty) return SLSIM_OK;
+  std::vector<TreeDesc> all;
+  tree_off.clear();
+  tree_cnt.clear();
+  for (int mi : order) {
+    tree_off.push_back((int)all.size());
+    tree_cnt.push_back((int)main_planes[mi].size());
+    for (slsim_plane p : main_planes[mi]) all.push_back(p->desc());
+  }
+  if (trees_dev) ctx->release(trees_dev);
+  trees_dev = nullptr;
+  if (!all.empty()) {
+    int rc = ctx->alloc(reinterpret_cast<void**>(&trees_dev), sizeof(TreeDesc) * all.size());
+    if (rc) return rc;
+    SLSIM_HIP_CHECK(hipMemcpyAsync(trees_dev, all.data(), sizeof(TreeDesc) * all.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    SLSIM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  }
+  trees_dirty = false;
+  return SLSIM_OK;
+}
+
+namespace {
+
+// Source selection of compute_points_parallel for a uniform source redshift
+// (include/lens.h:940-953): jmax and the last plane's (aa, bb).
+void uniform_source(const slsim_lens_s* L, double zs, int32_t* jmax, double* aal, double* bbl) {
+  double source_z = zs;
+  if (!(zs > 0)) source_z = L->toggle_source_plane ? L->zs_implant : L->plane_redshifts.back();
+  const int nplanes = (int)L->plane_redshifts.size() - 1;
+  if (source_z == L->plane_redshifts.back()) {
+    *jmax = nplanes;
+    *aal = L->dDl.back() / L->Dl.back();
+    *bbl = L->Dl[L->Dl.size() - 2] / L->Dl.back();
+  } else {
+    long jm;
+    double Dls, Ds;
+    L->FindSourcePlane(source_z, jm, Dls, Ds);
+    *jmax = (int32_t)jm;
+    *aal = Dls / Ds;
+    *bbl = jm > 0 ? L->Dl[jm - 1] / Ds : 0.0;
+  }
+}
+
+// Per-plane step descriptors for one ray job.
+void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint32_t* counters,
+                std::vector<StepJob>& steps) {
+  const int nplanes = (int)L->plane_redshifts.size() - 1;
+  steps.resize(nplanes);
+  for (int j = 0; j < nplanes; ++j) {
+    StepJob& s = steps[j];
+    std::memset(&s, 0, sizeof(s));
+    s.st = st;
+    s.nslots = (int32_t)nslots;
+    s.j = j;
+    s.ntrees = L->tree_cnt[j];
+    s.trees = L->trees_dev + L->tree_off[j];
+    s.Dphys = L->Dl[j] / (1 + L->plane_redshifts[j]);
+    s.Gfac = L->charge * L->Dl[j] / (1 + L->plane_redshifts[j]);
+    s.aa = L->dDl[j + 1] / L->Dl[j + 1];
+    s.bb = L->Dl[j] / L->Dl[j + 1];
+    s.charge = L->charge;
+    s.counters = counters;
+  }
+}
+
+// Runs the plane steps [0, jmax) of one job on the context stream.
+int run_steps(slsim_ctx c, const StepJob* djobs, int jmax, uint32_t nbundles, int precision,
+              bool count, uint32_t* err, slsim_shoot_stats* stats) {
+  std::vector<hipEvent_t> ev(2 * (size_t)jmax);
+  const bool timed = stats != nullptr;
+  if (timed)
+    for (auto& e : ev) SLSIM_HIP_CHECK(hipEventCreate(&e));
+  for (int j = 0; j < jmax; ++j) {
+    if (timed) SLSIM_HIP_CHECK(hipEventRecord(ev[2 * j], c->stream));
+    launch_plane_step(precision, count, nbundles, c->stream, djobs + j, nullptr, err);
+    if (timed) SLSIM_HIP_CHECK(hipEventRecord(ev[2 * j + 1], c->stream));
+  }
+  SLSIM_HIP_CHECK(hipGetLastError());
+  SLSIM_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (timed) {
+    double tot = 0, mx = 0;
+    for (int j = 0; j < jmax; ++j) {
+      float ms = 0;
+      SLSIM_HIP_CHECK(hipEventElapsedTime(&ms, ev[2 * j], ev[2 * j + 1]));
+      tot += ms;
+      mx = std::max(mx, (double)ms);
+    }
+    stats->walk_ms = tot;
+    stats->walk_ms_max_plane = mx;
+    stats->n_walk_launches = jmax;
+    for (auto& e : ev) (void)hipEventDestroy(e);
+  }
+  return SLSIM_OK;
+}
+
+int check_err(slsim_ctx c, uint32_t* err) {
+  uint32_t herr = 0;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (herr) {
+    set_error("traversal stack overflow");
+    return SLSIM_ERR_TOO_DEEP;
+  }
+  return SLSIM_OK;
+}
+
+int gridmap_run(slsim_lens L, size_t N, double c0, double c1, double range, double zs,
+                double* y_dev, double* A_dev, int precision, uint64_t* cnt_host,
+                slsim_shoot_stats* stats) {
+  const double t0 = now_ms();
+  if (!L) return arg_error("null lens");
+  if (N < 1) return arg_error("GridMap size is < 1!");
+  if (!(range > 0)) return arg_error("GridMap range is <= 0!");
+  slsim_ctx c = L->ctx;
+  SLSIM_HIP_CHECK(hipSetDevice(c->device));
+  int rc = L->upload_trees();
+  if (rc) return rc;
+  const int64_t tiles_x = ((int64_t)N + 7) / 8;
+  const int64_t nbundles = tiles_x * tiles_x;
+  const int64_t nslots = nbundles * kWave;
+  if (nbundles > 0xffffffffll) return arg_error("grid too large");
+  Scratch s(c);
+  RayState st;
+  if ((rc = alloc_state(s, nslots, &st))) return rc;
+  uint32_t* err;
+  if ((rc = s.get(&err, 1))) return rc;
+  SLSIM_HIP_CHECK(hipMemsetAsync(err, 0, 4, c->stream));
+  uint32_t* counters = nullptr;
+  if (cnt_host) {
+    if ((rc = s.get(&counters, 5 * (size_t)nslots))) return rc;
+    SLSIM_HIP_CHECK(hipMemsetAsync(counters, 0, 20 * (size_t)nslots, c->stream));
+  }
+  int32_t jmax = 0;
+  double aal = 0, bbl = 0;
+  const int nplanes = (int)L->plane_redshifts.size() - 1;
+  if (nplanes > 0) uniform_source(L, zs, &jmax, &aal, &bbl);
+  launch_init_grid(c->stream, st, nslots, (int64_t)N, tiles_x, c0, c1, range, jmax, aal, bbl);
+  if (jmax > 0) {
+    std::vector<StepJob> steps;
+    make_steps(L, st, nslots, counters, steps);
+    StepJob* djobs;
+    if ((rc = s.get(&djobs, steps.size()))) return rc;
+    SLSIM_HIP_CHECK(hipMemcpyAsync(djobs, steps.data(), sizeof(StepJob) * steps.size(),
+                                   hipMemcpyHostToDevice, c->stream));
+    rc = run_steps(c, djobs, jmax, (uint32_t)nbundles, cnt_host ? SLSIM_FP64 : precision,
+                   cnt_host != nullptr, err, stats);
+    if (rc) return rc;
+  }
+  if (y_dev && A_dev) launch_finalize_grid(c->stream, st, nslots, (int64_t)N, tiles_x, y_dev, A_dev);
+  if (cnt_host) {
+    uint64_t* dcnt;
+    if ((rc = s.get(&dcnt, 5 * N * N))) return rc;
+    launch_counts_grid(c->stream, counters, nslots, (int64_t)N, tiles_x, dcnt);
+    SLSIM_HIP_CHECK(hipMemcpyAsync(cnt_host, dcnt, 40 * N * N, hipMemcpyDeviceToHost, c->stream));
+  }
+  SLSIM_HIP_CHECK(hipGetLastError());
+  if ((rc = check_err(c, err))) return rc;
+  if (stats) stats->total_ms = now_ms() - t0;
+  return SLSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int slsim_lens_create(slsim_ctx ctx, double zsource, double Om, double OL, double h,
+                      slsim_lens* out) {
+  if (!ctx || !out) return arg_error("null handle");
+  if ((Om + OL) != 1.0) {
+    set_error("Lens can only handle flat universes at present.");
+    return SLSIM_ERR_NOT_FLAT;
+  }
+  if (!(h > 0) || !(zsource > 0)) return arg_error("h and zsource must be positive");
+  auto L = std::make_unique<slsim_lens_s>();
+  L->ctx = ctx;
+  L->Om = Om;
+  L->OL = OL;
+  L->h = h;
+  L->zsource = zsource;
+  L->charge = 4 * kPI * kGrav;
+  L->combinePlanes();
+  *out = L.release();
+  return SLSIM_OK;
+}
+
+int slsim_lens_destroy(slsim_lens L) {
+  if (!L) return SLSIM_OK;
+  (void)hipStreamSynchronize(L->ctx->stream);
+  if (L->trees_dev) L->ctx->release(L->trees_dev);
+  for (slsim_plane p : L->owned) slsim_plane_destroy(p);
+  delete L;
+  return SLSIM_OK;
+}
+
+int slsim_lens_add_halo(slsim_lens L, slsim_plane plane, float z) {
+  if (!L || !plane) return arg_error("null handle");
+  if (plane->ctx != L->ctx) return arg_error("plane and lens belong to different contexts");
+  for (slsim_plane p : L->owned)
+    if (p == plane) return arg_error("halo already in the lens");
+  L->owned.push_back(plane);
+  const double halo_z = z;
+  const double halo_Dl = L->coorDist(0, halo_z);
+  auto& mD = L->main_Dl;
+  const size_t i = std::upper_bound(mD.begin(), mD.end(), halo_Dl) - mD.begin();
+  if (i > 0 && (halo_Dl - mD[i - 1]) < kMinPlaneDist) {
+    L->main_planes[i - 1].push_back(plane);
+  } else if (i == mD.size()) {
+    L->main_planes.push_back({plane});
+    L->main_z.push_back(halo_z);
+    mD.push_back(halo_Dl);
+  } else if ((mD[i] - halo_Dl) < kMinPlaneDist) {
+    L->main_planes[i].push_back(plane);
+  } else {
+    L->main_planes.insert(L->main_planes.begin() + i, {plane});
+    L->main_z.insert(L->main_z.begin() + i, halo_z);
+    mD.insert(mD.begin() + i, halo_Dl);
+  }
+  L->combinePlanes();
+  return SLSIM_OK;
+}
+
+int slsim_lens_tables(slsim_lens L, int* nplanes, double* z, double* Dl, double* dDl,
+                      double* charge) {
+  if (!L) return arg_error("null lens");
+  if (nplanes) *nplanes = (int)L->plane_redshifts.size() - 1;
+  for (size_t i = 0; i < L->plane_redshifts.size(); ++i) {
+    if (z) z[i] = L->plane_redshifts[i];
+    if (Dl) Dl[i] = L->Dl[i];
+    if (dDl) dDl[i] = L->dDl[i];
+  }
+  if (charge) *charge = L->charge;
+  return SLSIM_OK;
+}
+
+int slsim_lens_reset_source(slsim_lens L, double z) {
+  if (!L) return arg_error("null lens");
+  if (z <= 0) {
+    L->toggle_source_plane = false;
+    return SLSIM_OK;
+  }
+  L->toggle_source_plane = true;
+  L->zs_implant = z;
+  return SLSIM_OK;
+}
+
+int slsim_gridmap_shoot(slsim_lens L, size_t N, double c0, double c1, double range, double zs,
+                        double* y_dev, double* A_dev, int precision, slsim_shoot_stats* stats) {
+  if (!y_dev || !A_dev) return arg_error("null output buffer");
+  if (precision < SLSIM_FP64 || precision > SLSIM_DIRECT64) return arg_error("bad precision");
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  return gridmap_run(L, N, c0, c1, range, zs, y_dev, A_dev, precision, nullptr, stats);
+}
+
+int slsim_gridmap_count(slsim_lens L, size_t N, double c0, double c1, double range, double zs,
+                        uint64_t* cnt_host) {
+  if (!cnt_host) return arg_error("null counter buffer");
+  return gridmap_run(L, N, c0, c1, range, zs, nullptr, nullptr, SLSIM_FP64, cnt_host, nullptr);
+}
+
+int slsim_shoot_device(slsim_lens L, size_t n, const double* x_dev, const double* zs_dev,
+                       double* y_dev, double* A_dev, int precision, slsim_shoot_stats* stats) {
+  const double t0 = now_ms();
+  if (!L) return arg_error("null lens");
+  if (precision < SLSIM_FP64 || precision > SLSIM_DIRECT64) return arg_error("bad precision");
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (n == 0) return SLSIM_OK;
+  if (!x_dev || !y_dev || !A_dev) return arg_error("null buffer");
+  if (n >= (size_t)1 << 32) return arg_error("too many rays in one call");
+  slsim_ctx c = L->ctx;
+  SLSIM_HIP_CHECK(hipSetDevice(c->device));
+  int rc = L->upload_trees();
+  if (rc) return rc;
+  const int nplanes = (int)L->plane_redshifts.size() - 1;
+  const int64_t nbundles = ((int64_t)n + kWave - 1) / kWave;
+  const int64_t nslots = nbundles * kWave;
+  Scratch s(c);
+  RayState st;
+  if ((rc = alloc_state(s, nslots, &st))) return rc;
+  uint32_t* err;
+  if ((rc = s.get(&err, 1))) return rc;
+  SLSIM_HIP_CHECK(hipMemsetAsync(err, 0, 4, c->stream));
+  // Morton order of the rays: each wavefront gets a compact bundle
+  const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
+  double* part;
+  if ((rc = s.get(&part, 4 * (size_t)nblk))) return rc;
+  launch_minmax_rays(c->stream, x_dev, (int64_t)n, part, nblk);
+  std::vector<double> hp(4 * (size_t)nblk);
+  SLSIM_HIP_CHECK(hipMemcpyAsync(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(c->stream));
+  double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+  for (int b = 0; b < nblk; ++b) {
+    lo[0] = std::min(lo[0], hp[4 * b]); lo[1] = std::min(lo[1], hp[4 * b + 1]);
+    hi[0] = std::max(hi[0], hp[4 * b + 2]); hi[1] = std::max(hi[1], hp[4 * b + 3]);
+  }
+  const double w = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), 1e-300);
+  uint64_t *code, *code_s;
+  uint32_t *idx, *perm;
+  if ((rc = s.get(&code, n)) || (rc = s.get(&code_s, n)) || (rc = s.get(&idx, n)) ||
+      (rc = s.get(&perm, n)))
+    return rc;
+  launch_ray_morton(c->stream, x_dev, (int64_t)n, lo[0], lo[1], 1.0 / w, code, idx);
+  size_t sb = 0;
+  SLSIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sb, code, code_s, idx, perm, (int)n,
+                                                     0, 30, c->stream));
+  char* stmp;
+  if ((rc = s.get(&stmp, sb))) return rc;
+  SLSIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(stmp, sb, code, code_s, idx, perm, (int)n, 0,
+                                                     30, c->stream));
+  int32_t jmax = 0;
+  double aal = 0, bbl = 0;
+  if (nplanes > 0 && !zs_dev) uniform_source(L, 0.0, &jmax, &aal, &bbl);
+  double *zpl, *dl;
+  if ((rc = s.get(&zpl, L->plane_redshifts.size())) || (rc = s.get(&dl, L->Dl.size()))) return rc;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(zpl, L->plane_redshifts.data(), 8 * L->plane_redshifts.size(),
+                                 hipMemcpyHostToDevice, c->stream));
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dl, L->Dl.data(), 8 * L->Dl.size(), hipMemcpyHostToDevice, c->stream));
+  FlatLCDM cosmo{L->Om, L->OL, L->h};
+  launch_init_array(c->stream, st, nslots, (int64_t)n, perm, x_dev, nplanes > 0 ? zs_dev : nullptr,
+                    jmax, aal, bbl, zpl, dl, nplanes, cosmo);
+  const int steps_to_run = zs_dev ? nplanes : jmax;
+  if (steps_to_run > 0) {
+    std::vector<StepJob> steps;
+    make_steps(L, st, nslots, nullptr, steps);
+    StepJob* djobs;
+    if ((rc = s.get(&djobs, steps.size()))) return rc;
+    SLSIM_HIP_CHECK(hipMemcpyAsync(djobs, steps.data(), sizeof(StepJob) * steps.size(),
+                                   hipMemcpyHostToDevice, c->stream));
+    rc = run_steps(c, djobs, steps_to_run, (uint32_t)nbundles, precision, false, err, stats);
+    if (rc) return rc;
+  }
+  launch_finalize_array(c->stream, st, (int64_t)n, perm, y_dev, A_dev);
+  SLSIM_HIP_CHECK(hipGetLastError());
+  if ((rc = check_err(c, err))) return rc;
+  if (stats) stats->total_ms = now_ms() - t0;
+  return SLSIM_OK;
+}
+
+int slsim_shoot(slsim_lens L, size_t n, const double* x, const double* zs, double* y, double* A,
+                int precision, slsim_shoot_stats* stats) {
+  if (!L) return arg_error("null lens");
+  if (n == 0) return SLSIM_OK;
+  if (!x || !y || !A) return arg_error("null buffer");
+  slsim_ctx c = L->ctx;
+  SLSIM_HIP_CHECK(hipSetDevice(c->device));
+  const double t0 = now_ms();
+  Scratch s(c);
+  double *dx, *dzs = nullptr, *dy, *dA;
+  int rc;
+  if ((rc = s.get(&dx, 2 * n)) || (rc = s.get(&dy, 2 * n)) || (rc = s.get(&dA, 4 * n))) return rc;
+  if (zs && (rc = s.get(&dzs, n))) return rc;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dx, x, 16 * n, hipMemcpyHostToDevice, c->stream));
+  if (zs) SLSIM_HIP_CHECK(hipMemcpyAsync(dzs, zs, 8 * n, hipMemcpyHostToDevice, c->stream));
+  rc = slsim_shoot_device(L, n, dx, dzs, dy, dA, precision, stats);
+  if (rc) return rc;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(y, dy, 16 * n, hipMemcpyDeviceToHost, c->stream));
+  SLSIM_HIP_CHECK(hipMemcpyAsync(A, dA, 32 * n, hipMemcpyDeviceToHost, c->stream));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (stats) stats->total_ms = now_ms() - t0;
+  return SLSIM_OK;
+}
+
+// ------------------------------------------------------------ device buffers
+int slsim_device_alloc(slsim_ctx ctx, size_t bytes, void** dev) {
+  if (!ctx || !dev) return arg_error("null argument");
+  SLSIM_HIP_CHECK(hipSetDevice(ctx->device));
+  return ctx->alloc(dev, bytes);
+}
+
+int slsim_device_free(slsim_ctx ctx, void* dev) {
+  if (!ctx) return arg_error("null context");
+  (void)hipStreamSynchronize(ctx->stream);
+  return ctx->release(dev);
+}
+
+int slsim_memcpy_d2h(slsim_ctx ctx, void* host, const void* dev, size_t bytes) {
+  if (!ctx) return arg_error("null context");
+  SLSIM_HIP_CHECK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return SLSIM_OK;
+}
+
+int slsim_memcpy_h2d(slsim_ctx ctx, void* dev, const void* host, size_t bytes) {
+  if (!ctx) return arg_error("null context");
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return SLSIM_OK;
+}
+
+// --------------------------------------------------------------- cosmology
+double slsim_coordist(double Om, double OL, double h, double z1, double z2) {
+  FlatLCDM c{Om, OL, h};
+  return c.coorDist(z1, z2);
+}
+
+double slsim_inv_coordist(double Om, double OL, double h, double D) {
+  FlatLCDM c{Om, OL, h};
+  double lo = 0.0, hi = 1.0;
+  while (c.coorDist(0, hi) < D && hi < 1e4) hi *= 2;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid == lo || mid == hi) break;
+    if (c.coorDist(0, mid) < D) lo = mid; else hi = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+}  // extern "C"

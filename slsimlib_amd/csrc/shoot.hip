@@ -1,0 +1,833 @@
+// shoot.hip -- multi-plane ray shooting kernels.
+//
+// One wavefront carries a bundle of 64 spatially coherent rays through one
+// lens plane: the bundle walks the plane's quadtree depth first with a
+// shared LDS stack whose entries hold a 64-bit live mask.  Every lane
+// applies the reference's own per-ray opening criterion
+// (TreeQuadParticles::walkTree_recur, include/quadTree.h:990-1128): a lane
+// for which a cell is far takes its multipole and drops out of that
+// subtree, a lane that opens it takes the leaf's particles and the
+// big-particle B-spline corrections.  The bundle descends while any lane
+// is live, so each ray sees exactly the cells it would see alone, in the
+// same depth-first order (results do not depend on bundle mates).
+// Cell records and particles are wave-uniform and are read with scalar
+// loads.  The plane-to-plane recursion (Lens::compute_points_parallel,
+// include/lens.h:913-1103) runs in FP64 after each plane.
+
+#include <cstdint>
+
+#include "cosmo.h"
+#include "slsim_internal.h"
+
+namespace slsim {
+
+namespace {
+
+__device__ inline int lane_id() { return (int)__lane_id(); }
+
+__device__ inline uint64_t ballot(bool p) { return (uint64_t)__ballot((int)p); }
+
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ inline uint64_t uni64(uint64_t v) {
+  uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// 1/x in FP64: hardware estimate refined by two Newton steps.
+__device__ inline double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+__device__ inline float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Wave-uniform reads (cell records, particles of a cell, job descriptors) go
+// through the constant address space so that they compile to scalar loads.
+// (The host pass of this single-source file has no constant address space.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+#else
+template <typename T>
+using cptr = const T*;
+#endif
+template <typename T>
+__device__ inline cptr<T> as_const(const T* p) {
+  return (cptr<T>)(uintptr_t)p;
+}
+
+constexpr double kInvPI = 1.0 / kPI;
+constexpr float kInvPIf = (float)(1.0 / kPI);
+
+// XCD-aware block order: hardware dispatches block b to XCD b % 8; give each
+// XCD a contiguous range of bundles so neighbouring tiles share its L2.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t xcd = b & 7u, idx = b >> 3;
+  const uint32_t per = nb >> 3, rem = nb & 7u;
+  return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
+struct Counts {
+  uint32_t v = 0, c = 0, p = 0, b = 0, s = 0;
+};
+
+// b_spline_profile (include/quadTree.h:147-190), FP64, phi omitted.
+__device__ inline void bspline64(double d0, double d1, double r, double Mass, double size,
+                                 double& ax, double& ay, double& kap, double& g0, double& g1) {
+  const double q = r / size;
+  double M, sigma;
+  if (q > 2) {
+    sigma = 0;
+    M = 1;
+  } else {
+    const double q2 = q * q, q3 = q2 * q;
+    sigma = (8 - 12 * q + 6 * q2 - q3) / 4;
+    if (q > 1) {
+      sigma *= 10 / size / size / 7 * kInvPI;
+      M = (-1 + 20 * q2 * (1 - q + 3 * q2 / 8 - q3 / 20)) / 7;
+    } else {
+      sigma = 10 * (sigma - 1 + 3 * q - 3 * q2 + q3) / size / size / 7 * kInvPI;
+      M = 10 * q2 * (1 - 3 * q / 4 + 3 * q3 / 10) / 7;
+    }
+  }
+  const double alpha_r = (M - 1) * kInvPI / r;
+  const double gt = alpha_r / r - sigma;
+  ax -= Mass * alpha_r * d0 / r;
+  ay -= Mass * alpha_r * d1 / r;
+  g0 -= gt * Mass * (d0 * d0 - d1 * d1) / r / r;
+  g1 -= gt * Mass * 2 * d0 * d1 / r / r;
+  kap += Mass * sigma;
+}
+
+struct Acc64 {
+  double ax = 0, ay = 0, k = 0, g0 = 0, g1 = 0;
+};
+
+// Walk of one tree, FP64 (plain method).  Must be called by all 64 lanes.
+template <bool COUNT>
+__device__ void walk64(const TreeDesc& t, double rx, double ry, bool active, Acc64& a,
+                       Counts& cnt, int32_t* s_node, uint64_t* s_mask, uint32_t* err) {
+  const int lane = lane_id();
+  const uint64_t amask = ballot(active);
+  if (amask == 0) return;
+  const cptr<NodeD> nodes = as_const(t.nodesD);
+  const cptr<PartD> parts = as_const(t.partsD);
+  const cptr<PartD> bigs = as_const(t.bigD);
+  const double ia = t.inv_area;
+  if (lane == 0) { s_node[0] = 0; s_mask[0] = amask; }
+  int sp = 1;
+  while (sp > 0) {
+    --sp;
+    __builtin_amdgcn_wave_barrier();
+    const int ni = uni(s_node[sp]);
+    const uint64_t live = uni64(s_mask[sp]);
+    const NodeD nd = nodes[ni];
+    const bool lv = (live >> lane) & 1ull;
+    const double dx = nd.cx - rx, dy = nd.cy - ry;
+    const double r2 = dx * dx + dy * dy;
+    const bool nearc = lv && (r2 < nd.near2);
+    const bool open = lv && ((r2 < nd.r2crit) || nearc);
+    if (COUNT && lv) cnt.v++;
+    if (lv && !open) {
+      // far cell: monopole + quadrupole (include/quadTree.h:1089-1123)
+      if (COUNT) cnt.c++;
+      const double inv = rcp64(r2);
+      const double mi = nd.mass * inv * kInvPI;
+      double tmp = nd.mass * ia - mi;
+      a.ax += tmp * dx;
+      a.ay += tmp * dy;
+      tmp = -2.0 * mi * inv;
+      a.g0 += 0.5 * (dx * dx - dy * dy) * tmp;
+      a.g1 += dx * dy * tmp;
+      const double i2p = inv * inv * kInvPI;
+      a.ax -= (nd.q0 * dx + nd.q2 * dy) * i2p;
+      a.ay -= (nd.q1 * dy + nd.q2 * dx) * i2p;
+      tmp = 4 * (nd.q0 * dx * dx + nd.q1 * dy * dy + 2 * nd.q2 * dx * dy) * inv * i2p;
+      a.ax += tmp * dx;
+      a.ay += tmp * dy;
+      a.k -= ia * nd.mass;
+    }
+    const uint64_t om = ballot(open);
+    if (om == 0) continue;
+    if (nd.child < 0) {
+      // leaf: every particle as a point mass (include/quadTree.h:1014-1048)
+      for (int k = 0; k < nd.count; ++k) {
+        const PartD p = parts[nd.start + k];
+        if (open) {
+          const double d0 = p.x - rx, d1 = p.y - ry;
+          double rr = d0 * d0 + d1 * d1;
+          if (rr < 1e-20) rr = 1e-20;
+          const double m = p.m;
+          const double inv = rcp64(rr);
+          const double pref = m * inv * kInvPI;
+          a.ax += (ia * m - pref) * d0;
+          a.ay += (ia * m - pref) * d1;
+          const double tmp = -2.0 * pref * inv;
+          a.g0 += 0.5 * (d0 * d0 - d1 * d1) * tmp;
+          a.g1 += d0 * d1 * tmp;
+          a.k -= m * ia;
+        }
+      }
+      if (COUNT && open) cnt.p += nd.count;
+      if (ballot(nearc)) {
+        // the leaf's big particles: h < box (include/quadTree.h:379-396, 1051-1077)
+        for (int k = 0; k < nd.count; ++k) {
+          const PartD p = parts[nd.start + k];
+          if (nearc && !signbit(p.h)) {
+            if (COUNT) cnt.b++;
+            const double d0 = p.x - rx, d1 = p.y - ry;
+            double rr = d0 * d0 + d1 * d1;
+            if (rr < 1e-20) rr = 1e-20;
+            const double size = p.h;
+            if (rr < 4 * size * size) {
+              if (COUNT) cnt.s++;
+              bspline64(d0, d1, sqrt(rr), (double)p.m, size, a.ax, a.ay, a.k, a.g0, a.g1);
+            }
+          }
+        }
+      }
+    } else {
+      if (nd.count > 0 && ballot(nearc)) {
+        for (int k = 0; k < nd.count; ++k) {
+          const PartD p = bigs[nd.start + k];
+          if (nearc) {
+            if (COUNT) cnt.b++;
+            const double d0 = p.x - rx, d1 = p.y - ry;
+            double rr = d0 * d0 + d1 * d1;
+            if (rr < 1e-20) rr = 1e-20;
+            const double size = p.h;
+            if (rr < 4 * size * size) {
+              if (COUNT) cnt.s++;
+              bspline64(d0, d1, sqrt(rr), (double)p.m, size, a.ax, a.ay, a.k, a.g0, a.g1);
+            }
+          }
+        }
+      }
+      if (sp + 4 > kStackDepth) {
+        if (lane == 0) atomicOr(err, 1u);
+        break;
+      }
+      __builtin_amdgcn_wave_barrier();
+      for (int c = 3; c >= 0; --c) {
+        if ((nd.cmask >> c) & 1) {
+          if (lane == 0) { s_node[sp] = nd.child + c; s_mask[sp] = om; }
+          ++sp;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- FP32 walk
+struct CSum {   // compensated FP32 sum (TwoSum)
+  float s = 0.f, c = 0.f;
+  __device__ inline void add(float x) {
+    const float t = s + x;
+    const float z = t - s;
+    c += (s - (t - z)) + (x - z);
+    s = t;
+  }
+  __device__ inline double value() const { return (double)s + (double)c; }
+};
+
+struct Acc32 {
+  CSum ax, ay, k, g0, g1;
+  float pax = 0.f, pay = 0.f, pk = 0.f, pg0 = 0.f, pg1 = 0.f;   // plain partials
+  __device__ inline void flush() {
+    ax.add(pax); ay.add(pay); k.add(pk); g0.add(pg0); g1.add(pg1);
+    pax = pay = pk = pg0 = pg1 = 0.f;
+  }
+};
+
+// Smoothed particle (point mass with the reference's B-spline correction
+// folded in, algebraically identical to the two-step FP64 form):
+//   alpha += (ia m - s) d,  gamma -= (s - m sigma) (c2, s2),  kappa += m sigma - m ia
+// with s = m M(q) / (pi r^2).
+__device__ inline void bspline_fused32(float r2, float inv, float h, float& Mr2, float& sig) {
+  const float r = sqrtf(r2);
+  const float ih = rcp32(h);
+  const float q = r * ih;
+  const float ih2 = ih * ih;
+  if (q > 2.f) {
+    Mr2 = inv;
+    sig = 0.f;
+  } else if (q > 1.f) {
+    const float q2 = q * q, q3 = q2 * q;
+    const float M = (-1.f + 20.f * q2 * (1.f - q + 0.375f * q2 - 0.05f * q3)) * (1.f / 7.f);
+    Mr2 = M * inv;
+    const float tq = 2.f - q;
+    sig = (10.f / 28.f) * kInvPIf * tq * tq * tq * ih2;
+  } else {
+    const float q2 = q * q, q3 = q2 * q;
+    Mr2 = (10.f / 7.f) * (1.f - 0.75f * q + 0.3f * q3) * ih2;
+    sig = (10.f / 7.f) * kInvPIf * (1.f - 1.5f * q2 + 0.75f * q3) * ih2;
+  }
+}
+
+// Walk of one tree, FP32 arithmetic with compensated accumulation.  Cell
+// centres relative to the ray and the opening test are FP64.
+__device__ void walk32(const TreeDesc& t, double rx, double ry, bool active, Acc32& a,
+                       int32_t* s_node, uint64_t* s_mask, uint32_t* err) {
+  const int lane = lane_id();
+  const uint64_t amask = ballot(active);
+  if (amask == 0) return;
+  const cptr<NodeF> nodes = as_const(t.nodesF);
+  const cptr<PartD> parts = as_const(t.partsD);
+  const cptr<PartD> bigs = as_const(t.bigD);
+  const float ia = (float)t.inv_area;
+  if (lane == 0) { s_node[0] = 0; s_mask[0] = amask; }
+  int sp = 1;
+  while (sp > 0) {
+    --sp;
+    __builtin_amdgcn_wave_barrier();
+    const int ni = uni(s_node[sp]);
+    const uint64_t live = uni64(s_mask[sp]);
+    const NodeF nd = nodes[ni];
+    const bool lv = (live >> lane) & 1ull;
+    const double dx = nd.cx - rx, dy = nd.cy - ry;
+    const double r2 = dx * dx + dy * dy;
+    const bool nearc = lv && (r2 < nd.near2);
+    const bool open = lv && ((r2 < nd.r2crit) || nearc);
+    const int count = nd.count_cmask & 0x0fffffff;
+    const int cmask = (nd.count_cmask >> 28) & 0xf;
+    if (lv && !open) {
+      // far cell: monopole + quadrupole, one compensated add per component
+      const float ex = (float)dx, ey = (float)dy;
+      const float inv = rcp32((float)r2);
+      const float s = nd.mass * inv * kInvPIf;
+      const float c2 = (ex * ex - ey * ey) * inv;
+      const float s2 = 2.f * ex * ey * inv;
+      const float qx0 = nd.q0 * ex + nd.q2 * ey;
+      const float qx1 = nd.q1 * ey + nd.q2 * ex;
+      const float qq = ex * qx0 + ey * qx1;
+      const float ip = inv * inv * kInvPIf;
+      const float t4 = 4.f * qq * inv;
+      const float mono = nd.mass * ia - s;
+      a.ax.add(mono * ex + ip * (t4 * ex - qx0));
+      a.ay.add(mono * ey + ip * (t4 * ey - qx1));
+      a.g0.add(-s * c2);
+      a.g1.add(-s * s2);
+      a.pk -= ia * nd.mass;
+    }
+    const uint64_t om = ballot(open);
+    if (om == 0) continue;
+    if (nd.child < 0) {
+      // leaf particles; particle - ray offsets are formed in FP64
+      for (int k = 0; k < count; ++k) {
+        const PartD p = parts[nd.start + k];
+        if (open) {
+          const float d0 = (float)(p.x - rx), d1 = (float)(p.y - ry);
+          float rr = d0 * d0 + d1 * d1;
+          rr = fmaxf(rr, 1e-20f);
+          const float inv = rcp32(rr);
+          const float m = p.m;
+          const float hs = fabsf(p.h);
+          float s = m * inv * kInvPIf;
+          float sm = 0.f;
+          if (nearc && !signbit(p.h) && rr < 4.f * hs * hs) {
+            float Mr2, sig;
+            bspline_fused32(rr, inv, hs, Mr2, sig);
+            s = m * Mr2 * kInvPIf;
+            sm = m * sig;
+          }
+          const float c2 = (d0 * d0 - d1 * d1) * inv;
+          const float s2 = 2.f * d0 * d1 * inv;
+          const float mono = ia * m - s;
+          a.pax += mono * d0;
+          a.pay += mono * d1;
+          a.pg0 -= (s - sm) * c2;
+          a.pg1 -= (s - sm) * s2;
+          a.pk += sm - m * ia;
+        }
+      }
+      a.flush();
+    } else {
+      if (count > 0 && ballot(nearc)) {
+        for (int k = 0; k < count; ++k) {
+          const PartD p = bigs[nd.start + k];
+          if (nearc) {
+            const float d0 = (float)(p.x - rx), d1 = (float)(p.y - ry);
+            float rr = d0 * d0 + d1 * d1;
+            rr = fmaxf(rr, 1e-20f);
+            const float hh = p.h;
+            if (rr < 4.f * hh * hh) {
+              // separate B-spline correction (the point mass was taken at the leaf)
+              const float r = sqrtf(rr);
+              const float q = r / hh;
+              float M, sig;
+              const float q2 = q * q, q3 = q2 * q;
+              if (q > 2.f) {
+                M = 1.f; sig = 0.f;
+              } else if (q > 1.f) {
+                const float tq = 2.f - q;
+                sig = (10.f / 28.f) * kInvPIf * tq * tq * tq / (hh * hh);
+                M = (-1.f + 20.f * q2 * (1.f - q + 0.375f * q2 - 0.05f * q3)) / 7.f;
+              } else {
+                sig = (10.f / 7.f) * kInvPIf * (1.f - 1.5f * q2 + 0.75f * q3) / (hh * hh);
+                M = 10.f * q2 * (1.f - 0.75f * q + 0.3f * q3) / 7.f;
+              }
+              const float m = p.m;
+              const float ar = (M - 1.f) * kInvPIf / r;
+              const float gt = ar / r - sig;
+              const float inv = rcp32(rr);
+              a.pax -= m * ar * d0 / r;
+              a.pay -= m * ar * d1 / r;
+              a.pg0 -= gt * m * (d0 * d0 - d1 * d1) * inv;
+              a.pg1 -= gt * m * 2.f * d0 * d1 * inv;
+              a.pk += m * sig;
+            }
+          }
+        }
+        a.flush();
+      }
+      if (sp + 4 > kStackDepth) {
+        if (lane == 0) atomicOr(err, 1u);
+        break;
+      }
+      __builtin_amdgcn_wave_barrier();
+      for (int c = 3; c >= 0; --c) {
+        if ((cmask >> c) & 1) {
+          if (lane == 0) { s_node[sp] = nd.child + c; s_mask[sp] = om; }
+          ++sp;
+        }
+      }
+    }
+  }
+  a.flush();
+}
+
+// FP64 direct sum of the smoothed-particle kernel over every particle of a
+// tree (no cells): the error-column oracle.  Particles are wave-uniform reads.
+__device__ void direct64(const TreeDesc& t, double rx, double ry, bool active, Acc64& a) {
+  if (ballot(active) == 0) return;
+  const cptr<PartD> parts = as_const(t.partsD);
+  const double ia = t.inv_area;
+  const int np = t.nparts;
+  for (int k = 0; k < np; ++k) {
+    const PartD p = parts[k];
+    if (!active) continue;
+    const double d0 = p.x - rx, d1 = p.y - ry;
+    double rr = d0 * d0 + d1 * d1;
+    if (rr < 1e-20) rr = 1e-20;
+    const double m = p.m;
+    const double inv = rcp64(rr);
+    const double pref = m * inv * kInvPI;
+    a.ax += (ia * m - pref) * d0;
+    a.ay += (ia * m - pref) * d1;
+    const double tmp = -2.0 * pref * inv;
+    a.g0 += 0.5 * (d0 * d0 - d1 * d1) * tmp;
+    a.g1 += d0 * d1 * tmp;
+    a.k -= m * ia;
+    const double size = fabs((double)p.h);
+    if (rr < 4 * size * size) bspline64(d0, d1, sqrt(rr), m, size, a.ax, a.ay, a.k, a.g0, a.g1);
+  }
+}
+
+// FP64 plane recursion step (include/lens.h:1003-1066), without FMA
+// contraction so that it performs the reference's roundings.
+__device__ inline void recursion_step(const StepJob& jb, int slot, double alpha0, double alpha1,
+                                      double kappa, double gamma0, double gamma1) {
+#pragma clang fp contract(off)
+  const RayState& st = jb.st;
+  double G[4] = {kappa + gamma0, gamma1, gamma1, kappa - gamma0};
+  for (int k = 0; k < 4; ++k) G[k] *= jb.Gfac;
+  double sa0 = st.sa0[slot], sa1 = st.sa1[slot];
+  sa0 -= jb.charge * alpha0;
+  sa1 -= jb.charge * alpha1;
+  double aa, bb;
+  if (jb.j < st.jmax[slot] - 1) {
+    aa = jb.aa;
+    bb = jb.bb;
+  } else {
+    aa = st.aal[slot];
+    bb = st.bbl[slot];
+  }
+  const double th0 = bb * st.th0[slot] + aa * sa0;
+  const double th1 = bb * st.th1[slot] + aa * sa1;
+  const double A[4] = {st.A0[slot], st.A1[slot], st.A2[slot], st.A3[slot]};
+  double S[4] = {st.G0[slot], st.G1[slot], st.G2[slot], st.G3[slot]};
+  const double GA[4] = {G[0] * A[0] + G[1] * A[2], G[0] * A[1] + G[1] * A[3],
+                        G[2] * A[0] + G[3] * A[2], G[2] * A[1] + G[3] * A[3]};
+  for (int k = 0; k < 4; ++k) S[k] -= GA[k];
+  double An[4];
+  for (int k = 0; k < 4; ++k) {
+    const double t1 = A[k] * bb;
+    const double t2 = S[k] * aa;
+    An[k] = t1 + t2;
+  }
+  st.sa0[slot] = sa0; st.sa1[slot] = sa1;
+  st.th0[slot] = th0; st.th1[slot] = th1;
+  st.A0[slot] = An[0]; st.A1[slot] = An[1]; st.A2[slot] = An[2]; st.A3[slot] = An[3];
+  st.G0[slot] = S[0]; st.G1[slot] = S[1]; st.G2[slot] = S[2]; st.G3[slot] = S[3];
+}
+
+__device__ inline double phys_coord(double th, double Dphys) {
+#pragma clang fp contract(off)
+  return th * Dphys;
+}
+
+}  // namespace
+
+// One plane step for every bundle of the listed jobs.  Grid: one wavefront
+// per work item, 4 per block.
+template <int MODE, bool COUNT>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
+    const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t nitems,
+    uint32_t* __restrict__ err) {
+  __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
+  __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
+  const int w = uni((int)(threadIdx.x >> 6));
+  const int lane = lane_id();
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t wi = blk * kWavesPerBlock + (uint32_t)w;
+  if (wi >= nitems) return;
+  WorkItem it{0u, wi};
+  if (items) {
+    it.job = as_const(items)[wi].job;
+    it.bundle = as_const(items)[wi].bundle;
+  }
+  const StepJob jb = as_const(jobs)[it.job];
+  const int slot = (int)it.bundle * kWave + lane;
+  const bool valid = slot < jb.nslots;
+  const int jm = valid ? jb.st.jmax[slot] : 0;
+  const bool active = jb.j < jm;
+  if (ballot(active) == 0) return;
+  double th0 = 0, th1 = 0;
+  if (active) {
+    th0 = jb.st.th0[slot];
+    th1 = jb.st.th1[slot];
+  }
+  const double rx = phys_coord(th0, jb.Dphys), ry = phys_coord(th1, jb.Dphys);
+  double alpha0 = 0, alpha1 = 0, kappa = 0, gamma0 = 0, gamma1 = 0;
+  Counts cnt;
+  const int ntrees = uni(jb.ntrees);
+  for (int h = 0; h < ntrees; ++h) {
+    const TreeDesc t = as_const(jb.trees)[h];
+    // LensPlaneSingular::force sums the halos; force_halo and the plane each
+    // flip alpha, so the plane's alpha is the tree's raw alpha
+    // (MultiPlane/planes.cpp:151-152, include/particle_halo.h:432-433).
+    if (MODE == SLSIM_FP32C) {
+      Acc32 a;
+      walk32(t, rx, ry, active, a, s_node[w], s_mask[w], err);
+      alpha0 += a.ax.value(); alpha1 += a.ay.value(); kappa += a.k.value();
+      gamma0 += a.g0.value(); gamma1 += a.g1.value();
+    } else if (MODE == SLSIM_DIRECT64) {
+      Acc64 a;
+      direct64(t, rx, ry, active, a);
+      alpha0 += a.ax; alpha1 += a.ay; kappa += a.k; gamma0 += a.g0; gamma1 += a.g1;
+    } else {
+      Acc64 a;
+      walk64<COUNT>(t, rx, ry, active, a, cnt, s_node[w], s_mask[w], err);
+      alpha0 += a.ax; alpha1 += a.ay; kappa += a.k; gamma0 += a.g0; gamma1 += a.g1;
+    }
+  }
+  if (!active) return;
+  if (COUNT && jb.counters) {
+    uint32_t* c = jb.counters + 5 * (size_t)slot;
+    c[0] += cnt.v; c[1] += cnt.c; c[2] += cnt.p; c[3] += cnt.b; c[4] += cnt.s;
+  }
+  recursion_step(jb, slot, alpha0, alpha1, kappa, gamma0, gamma1);
+}
+
+template __global__ void k_plane_step<SLSIM_FP64, false>(const StepJob*, const WorkItem*, uint32_t, uint32_t*);
+template __global__ void k_plane_step<SLSIM_FP32C, false>(const StepJob*, const WorkItem*, uint32_t, uint32_t*);
+template __global__ void k_plane_step<SLSIM_DIRECT64, false>(const StepJob*, const WorkItem*, uint32_t, uint32_t*);
+template __global__ void k_plane_step<SLSIM_FP64, true>(const StepJob*, const WorkItem*, uint32_t, uint32_t*);
+
+// GridMap ray layout (TreeCode_link/gridmap.cpp:293-321 via
+// Utilities::PositionFromIndex, TreeCode_link/utilities.cpp:149-158) in 8x8
+// tiles: bundle b covers tile (b % tiles_x, b / tiles_x).
+__device__ inline bool grid_slot(int64_t slot, int64_t N, int64_t tiles_x, int64_t* out) {
+  const int64_t b = slot >> 6;
+  const int lane = (int)(slot & 63);
+  const int64_t ix = (b % tiles_x) * 8 + (lane & 7);
+  const int64_t iy = (b / tiles_x) * 8 + (lane >> 3);
+  if (ix >= N || iy >= N) return false;
+  *out = ix + N * iy;
+  return true;
+}
+
+__global__ void k_init_grid(RayState st, int64_t nslots, int64_t N, int64_t tiles_x, double c0,
+                            double c1, double range, int32_t jmax, double aal, double bbl) {
+#pragma clang fp contract(off)
+  const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (slot >= nslots) return;
+  int64_t i;
+  const bool valid = grid_slot(slot, N, tiles_x, &i);
+  double x0 = c0, x1 = c1;
+  if (valid && N > 1) {
+    x0 = c0 + range * (1.0 * (double)(i % N) / (double)(N - 1) - 0.5);
+    x1 = c1 + range * (1.0 * (double)(i / N) / (double)(N - 1) - 0.5);
+  }
+  st.th0[slot] = x0; st.th1[slot] = x1;
+  st.sa0[slot] = x0; st.sa1[slot] = x1;
+  st.A0[slot] = 1; st.A1[slot] = 0; st.A2[slot] = 0; st.A3[slot] = 1;
+  st.G0[slot] = 1; st.G1[slot] = 0; st.G2[slot] = 0; st.G3[slot] = 1;
+  st.jmax[slot] = valid ? jmax : 0;
+  st.aal[slot] = aal;
+  st.bbl[slot] = bbl;
+}
+
+__global__ void k_finalize_grid(RayState st, int64_t nslots, int64_t N, int64_t tiles_x,
+                                double* __restrict__ y, double* __restrict__ A) {
+  const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (slot >= nslots) return;
+  int64_t i;
+  if (!grid_slot(slot, N, tiles_x, &i)) return;
+  y[2 * i] = st.th0[slot];
+  y[2 * i + 1] = st.th1[slot];
+  A[4 * i] = st.A0[slot];
+  A[4 * i + 1] = st.A1[slot];
+  A[4 * i + 2] = st.A2[slot];
+  A[4 * i + 3] = st.A3[slot];
+}
+
+__global__ void k_counts_grid(const uint32_t* __restrict__ c, int64_t nslots, int64_t N,
+                              int64_t tiles_x, uint64_t* __restrict__ out) {
+  const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (slot >= nslots) return;
+  int64_t i;
+  if (!grid_slot(slot, N, tiles_x, &i)) return;
+  for (int k = 0; k < 5; ++k) out[5 * i + k] = c[5 * slot + k];
+}
+
+// Array rays: slot s < n holds ray perm[s] (Morton order).  Per-ray source
+// redshifts resolve jmax, Dls and Ds as Lens::FindSourcePlane does
+// (MultiPlane/lens.cpp:2659-2681).
+__global__ void k_init_array(RayState st, int64_t nslots, int64_t n, const uint32_t* __restrict__ perm,
+                             const double* __restrict__ x, const double* __restrict__ zs,
+                             int32_t jmax_u, double aal_u, double bbl_u,
+                             const double* __restrict__ zpl, const double* __restrict__ Dl,
+                             int32_t nplanes, FlatLCDM cosmo) {
+  const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (slot >= nslots) return;
+  const bool valid = slot < n;
+  const int64_t i = valid ? (int64_t)perm[slot] : 0;
+  const double x0 = valid ? x[2 * i] : 0.0, x1 = valid ? x[2 * i + 1] : 0.0;
+  st.th0[slot] = x0; st.th1[slot] = x1;
+  st.sa0[slot] = x0; st.sa1[slot] = x1;
+  st.A0[slot] = 1; st.A1[slot] = 0; st.A2[slot] = 0; st.A3[slot] = 1;
+  st.G0[slot] = 1; st.G1[slot] = 0; st.G2[slot] = 0; st.G3[slot] = 1;
+  int32_t jm = jmax_u;
+  double aal = aal_u, bbl = bbl_u;
+  if (valid && zs) {
+    const double z = zs[i];
+    const double Ds = cosmo.coorDist(0.0, z);
+    double Dls;
+    if (z <= zpl[0]) {
+      jm = 0;
+      Dls = Ds;
+    } else {
+      jm = nplanes;
+      while (jm > 0 && zpl[jm - 1] > z) --jm;
+      Dls = cosmo.coorDist(zpl[jm - 1], z);
+    }
+    aal = Dls / Ds;
+    bbl = jm > 0 ? Dl[jm - 1] / Ds : 0.0;
+  }
+  st.jmax[slot] = valid ? jm : 0;
+  st.aal[slot] = aal;
+  st.bbl[slot] = bbl;
+}
+
+__global__ void k_finalize_array(RayState st, int64_t n, const uint32_t* __restrict__ perm,
+                                 double* __restrict__ y, double* __restrict__ A) {
+  const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (slot >= n) return;
+  const int64_t i = perm[slot];
+  y[2 * i] = st.th0[slot];
+  y[2 * i + 1] = st.th1[slot];
+  A[4 * i] = st.A0[slot];
+  A[4 * i + 1] = st.A1[slot];
+  A[4 * i + 2] = st.A2[slot];
+  A[4 * i + 3] = st.A3[slot];
+}
+
+// 30-bit Morton code of each ray over the rays' bounding box.
+__global__ void k_ray_morton(const double* __restrict__ x, int64_t n, double x0, double y0,
+                             double inv_w, uint64_t* __restrict__ code, uint32_t* __restrict__ idx) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double fx = (x[2 * i] - x0) * inv_w, fy = (x[2 * i + 1] - y0) * inv_w;
+  uint32_t ux = (uint32_t)fmin(fmax(fx * 32767.0, 0.0), 32767.0);
+  uint32_t uy = (uint32_t)fmin(fmax(fy * 32767.0, 0.0), 32767.0);
+  uint64_t c = 0;
+  for (int b = 14; b >= 0; --b) c = (c << 2) | (((uy >> b) & 1u) << 1) | ((ux >> b) & 1u);
+  code[i] = c;
+  idx[i] = (uint32_t)i;
+}
+
+__global__ void k_minmax_rays(const double* __restrict__ x, int64_t n, double* __restrict__ part) {
+  __shared__ double s[4][256];
+  double a0 = 1e300, a1 = 1e300, b0 = -1e300, b1 = -1e300;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    a0 = fmin(a0, x[2 * i]); b0 = fmax(b0, x[2 * i]);
+    a1 = fmin(a1, x[2 * i + 1]); b1 = fmax(b1, x[2 * i + 1]);
+  }
+  s[0][threadIdx.x] = a0; s[1][threadIdx.x] = a1; s[2][threadIdx.x] = b0; s[3][threadIdx.x] = b1;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      s[0][threadIdx.x] = fmin(s[0][threadIdx.x], s[0][threadIdx.x + o]);
+      s[1][threadIdx.x] = fmin(s[1][threadIdx.x], s[1][threadIdx.x + o]);
+      s[2][threadIdx.x] = fmax(s[2][threadIdx.x], s[2][threadIdx.x + o]);
+      s[3][threadIdx.x] = fmax(s[3][threadIdx.x], s[3][threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; ++k) part[4 * blockIdx.x + k] = s[k][0];
+}
+
+// Tree walk alone at physical positions (one plane, no recursion): the
+// force2D_recur outputs in the tree's sign convention, 5 per ray.
+template <bool F32>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_force(
+    const TreeDesc* __restrict__ tree, const double* __restrict__ rays, int64_t n,
+    double* __restrict__ out, uint32_t* __restrict__ err) {
+  __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
+  __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
+  const int w = uni((int)(threadIdx.x >> 6));
+  const int64_t i = (blockIdx.x * (int64_t)kWavesPerBlock + w) * kWave + lane_id();
+  const bool active = i < n;
+  const double rx = active ? rays[2 * i] : 0.0, ry = active ? rays[2 * i + 1] : 0.0;
+  const TreeDesc t = *as_const(tree);
+  double r[5];
+  if (F32) {
+    Acc32 a;
+    walk32(t, rx, ry, active, a, s_node[w], s_mask[w], err);
+    r[0] = a.ax.value(); r[1] = a.ay.value(); r[2] = a.k.value();
+    r[3] = a.g0.value(); r[4] = a.g1.value();
+  } else {
+    Acc64 a;
+    Counts c;
+    walk64<false>(t, rx, ry, active, a, c, s_node[w], s_mask[w], err);
+    r[0] = a.ax; r[1] = a.ay; r[2] = a.k; r[3] = a.g0; r[4] = a.g1;
+  }
+  if (active)
+    for (int k = 0; k < 5; ++k) out[5 * i + k] = r[k];
+}
+
+template __global__ void k_force<false>(const TreeDesc*, const double*, int64_t, double*, uint32_t*);
+template __global__ void k_force<true>(const TreeDesc*, const double*, int64_t, double*, uint32_t*);
+
+// FP64 direct sum of the smoothed-particle kernel over all particles
+// (no tree); particles staged through LDS in tiles of 256.
+__global__ __launch_bounds__(256) void k_direct(const PartD* __restrict__ parts, int64_t np,
+                                                double ia, const double* __restrict__ rays,
+                                                int64_t n, double* __restrict__ out) {
+  __shared__ PartD tile[256];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool active = i < n;
+  const double rx = active ? rays[2 * i] : 0.0, ry = active ? rays[2 * i + 1] : 0.0;
+  double ax = 0, ay = 0, k = 0, g0 = 0, g1 = 0;
+  for (int64_t base = 0; base < np; base += 256) {
+    __syncthreads();
+    if (base + threadIdx.x < np) tile[threadIdx.x] = parts[base + threadIdx.x];
+    __syncthreads();
+    const int cnt = (int)((np - base) < 256 ? (np - base) : 256);
+    for (int q = 0; q < cnt; ++q) {
+      const PartD p = tile[q];
+      const double d0 = p.x - rx, d1 = p.y - ry;
+      double rr = d0 * d0 + d1 * d1;
+      if (rr < 1e-20) rr = 1e-20;
+      const double m = p.m;
+      const double inv = rcp64(rr);
+      const double pref = m * inv * kInvPI;
+      ax += (ia * m - pref) * d0;
+      ay += (ia * m - pref) * d1;
+      const double tmp = -2.0 * pref * inv;
+      g0 += 0.5 * (d0 * d0 - d1 * d1) * tmp;
+      g1 += d0 * d1 * tmp;
+      k -= m * ia;
+      const double size = fabs((double)p.h);
+      if (rr < 4 * size * size) bspline64(d0, d1, sqrt(rr), m, size, ax, ay, k, g0, g1);
+    }
+  }
+  if (active) {
+    out[5 * i] = ax; out[5 * i + 1] = ay; out[5 * i + 2] = k;
+    out[5 * i + 3] = g0; out[5 * i + 4] = g1;
+  }
+}
+
+// ------------------------------------------------------------ host launchers
+namespace {
+inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+}
+
+void launch_plane_step(int precision, bool count, uint32_t nitems, hipStream_t st,
+                       const StepJob* jobs, const WorkItem* items, uint32_t* err) {
+  const unsigned nb = blocks_for(nitems, kWavesPerBlock);
+  if (nb == 0) return;
+  const dim3 blk(kWave * kWavesPerBlock);
+  if (count)
+    k_plane_step<SLSIM_FP64, true><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+  else if (precision == SLSIM_FP32C)
+    k_plane_step<SLSIM_FP32C, false><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+  else if (precision == SLSIM_DIRECT64)
+    k_plane_step<SLSIM_DIRECT64, false><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+  else
+    k_plane_step<SLSIM_FP64, false><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+}
+
+void launch_init_grid(hipStream_t st, const RayState& s, int64_t nslots, int64_t N,
+                      int64_t tiles_x, double c0, double c1, double range, int32_t jmax,
+                      double aal, double bbl) {
+  k_init_grid<<<blocks_for(nslots, 256), 256, 0, st>>>(s, nslots, N, tiles_x, c0, c1, range, jmax,
+                                                       aal, bbl);
+}
+
+void launch_finalize_grid(hipStream_t st, const RayState& s, int64_t nslots, int64_t N,
+                          int64_t tiles_x, double* y, double* A) {
+  k_finalize_grid<<<blocks_for(nslots, 256), 256, 0, st>>>(s, nslots, N, tiles_x, y, A);
+}
+
+void launch_counts_grid(hipStream_t st, const uint32_t* c, int64_t nslots, int64_t N,
+                        int64_t tiles_x, uint64_t* out) {
+  k_counts_grid<<<blocks_for(nslots, 256), 256, 0, st>>>(c, nslots, N, tiles_x, out);
+}
+
+void launch_init_array(hipStream_t st, const RayState& s, int64_t nslots, int64_t n,
+                       const uint32_t* perm, const double* x, const double* zs, int32_t jmax,
+                       double aal, double bbl, const double* zpl, const double* Dl,
+                       int32_t nplanes, const FlatLCDM& cosmo) {
+  k_init_array<<<blocks_for(nslots, 256), 256, 0, st>>>(s, nslots, n, perm, x, zs, jmax, aal, bbl,
+                                                        zpl, Dl, nplanes, cosmo);
+}
+
+void launch_finalize_array(hipStream_t st, const RayState& s, int64_t n, const uint32_t* perm,
+                           double* y, double* A) {
+  k_finalize_array<<<blocks_for(n, 256), 256, 0, st>>>(s, n, perm, y, A);
+}
+
+void launch_ray_morton(hipStream_t st, const double* x, int64_t n, double x0, double y0,
+                       double inv_w, uint64_t* code, uint32_t* idx) {
+  k_ray_morton<<<blocks_for(n, 256), 256, 0, st>>>(x, n, x0, y0, inv_w, code, idx);
+}
+
+void launch_minmax_rays(hipStream_t st, const double* x, int64_t n, double* part, int nblk) {
+  k_minmax_rays<<<nblk, 256, 0, st>>>(x, n, part);
+}
+
+void launch_force(hipStream_t st, int precision, const TreeDesc* tree, const double* rays,
+                  int64_t n, double* out, uint32_t* err) {
+  const unsigned nb = blocks_for(n, kWave * kWavesPerBlock);
+  if (nb == 0) return;
+  if (precision == SLSIM_FP32C)
+    k_force<true><<<nb, kWave * kWavesPerBlock, 0, st>>>(tree, rays, n, out, err);
+  else
+    k_force<false><<<nb, kWave * kWavesPerBlock, 0, st>>>(tree, rays, n, out, err);
+}
+
+void launch_direct(hipStream_t st, const PartD* parts, int64_t np, double ia, const double* rays,
+                   int64_t n, double* out) {
+  k_direct<<<blocks_for(n, 256), 256, 0, st>>>(parts, np, ia, rays, n, out);
+}
+
+}  // namespace slsim

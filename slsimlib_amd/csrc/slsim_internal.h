@@ -1,0 +1,217 @@
+// slsim_internal.h -- device data layout and host-side objects of the ray shooter.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/slsim_amd.h"
+
+namespace slsim {
+
+// Tree levels encoded in the 64-bit Morton path key (2 bits per level).
+constexpr int kMaxLevels = 32;
+// Per-wavefront traversal stack entries: depth-first with four children per
+// opened cell needs at most 3 * (kMaxLevels + 1) + 1.
+constexpr int kStackDepth = 104;
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+
+constexpr double kPI = 3.14159265358979323846;   // M_PI (include/standard.h:38-40)
+constexpr double kGrav = 4.7788e-20;              // G/c^2 in Mpc/Msun (standard.h:43)
+constexpr double kMinPlaneDist = 1e-6;            // MultiPlane/lens.cpp:15
+
+// Cell record of the FP64 walk (96 B, read with scalar loads).
+//  leaf:     start/count = particle range in the tree-ordered particle array
+//  internal: start/count = range of the cell's internal big-particle list,
+//            child = index of the first of four contiguous children
+//            (child0..child3 in the reference's order), cmask = non-empty children
+struct alignas(16) NodeD {
+  double cx, cy;        // centre of mass
+  double mass;
+  double q0, q1, q2;    // quadrupole about the centre of mass
+  double r2crit;        // (1.15470 rcom / theta)^2
+  double near2;         // 5.83 * box^2
+  double box;           // cell width
+  int32_t child;        // -1 for a leaf
+  int32_t start;
+  int32_t count;
+  int32_t cmask;
+  double pad;
+};
+static_assert(sizeof(NodeD) == 96, "NodeD layout");
+
+// Cell record of the FP32 walk (64 B).  The opening test stays FP64 so that
+// both walks open exactly the same cells for a ray.
+struct alignas(16) NodeF {
+  double cx, cy;
+  double r2crit, near2;
+  float mass, q0, q1, q2;
+  float box;
+  int32_t child;
+  int32_t start;
+  int32_t count_cmask;  // count | cmask << 28
+};
+static_assert(sizeof(NodeF) == 64, "NodeF layout");
+
+// Particle in tree order, read by both walks.  In the leaf particle array
+// the sign bit of h carries the leaf's big-particle flag (h < box,
+// include/quadTree.h:379-396): set = not listed; |h| is the size.  Internal
+// big-particle lists store h as is.
+struct PartD {
+  double x, y;
+  float m, h;
+};
+static_assert(sizeof(PartD) == 24, "PartD layout");
+
+struct TreeDesc {
+  const NodeD* nodesD;
+  const NodeF* nodesF;
+  const PartD* partsD;
+  const PartD* bigD;
+  double inv_area;
+  int32_t nnodes;
+  int32_t nparts;
+};
+
+// Per-slot ray state, structure of arrays (slot = bundle * 64 + lane).
+struct RayState {
+  double* th0; double* th1;     // theta (radians) on the current plane
+  double* sa0; double* sa1;     // SumPrevAlphas
+  double* A0; double* A1; double* A2; double* A3;
+  double* G0; double* G1; double* G2; double* G3;   // SumPrevAG
+  double* aal; double* bbl;     // (aa, bb) of the last plane before the source
+  int32_t* jmax;                // planes in front of the source; 0 = inactive slot
+};
+constexpr size_t kStateBytesPerSlot = 14 * sizeof(double) + sizeof(int32_t);
+
+// One plane step of one ray job.
+struct StepJob {
+  RayState st;
+  int32_t nslots;
+  int32_t j;                    // plane index of this step
+  int32_t ntrees;
+  int32_t pad;
+  const TreeDesc* trees;        // the plane's halos (device array)
+  double Dphys;                 // Dl[j] / (1 + z_j)
+  double Gfac;                  // charge * Dl[j] / (1 + z_j)
+  double aa, bb;                // dDl[j+1]/Dl[j+1], Dl[j]/Dl[j+1]
+  double charge;
+  uint32_t* counters;           // 5 per slot or null
+};
+
+struct WorkItem {
+  uint32_t job;
+  uint32_t bundle;
+};
+
+// ------------------------------------------------------------- host objects
+struct Plane;
+
+// Error helpers
+void set_error(const std::string& msg);
+#define SLSIM_HIP_CHECK(expr)                                                   \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      ::slsim::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));   \
+      return SLSIM_ERR_HIP;                                                     \
+    }                                                                           \
+  } while (0)
+
+}  // namespace slsim
+
+struct slsim_ctx_s {
+  int device = 0;
+  size_t budget = 0;
+  size_t used = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::vector<std::pair<void*, size_t>> allocs;
+  // freed scratch blocks kept for reuse (still counted in `used`)
+  std::vector<std::pair<void*, size_t>> cache;
+  int alloc(void** p, size_t bytes);
+  int release(void* p);
+  int scratch_get(void** p, size_t bytes);
+  void scratch_put(void* p);
+  void trim_cache();
+};
+
+struct slsim_plane_s {
+  slsim_ctx ctx = nullptr;
+  int64_t n = 0, nnodes = 0, nleaves = 0, nbig_internal = 0;
+  int32_t depth = 0;
+  double p1[2] = {0, 0}, p2[2] = {0, 0};
+  double theta = 0.1, inv_area = 0.0;
+  int bucket = 20;
+  double build_ms = 0;
+  size_t bytes = 0;
+  std::vector<int64_t> level_offset;   // BFS level starts (host copy)
+  slsim::NodeD* nodesD = nullptr;
+  slsim::NodeF* nodesF = nullptr;
+  slsim::PartD* partsD = nullptr;
+  slsim::PartD* bigD = nullptr;
+  // build products kept for export and direct sums
+  double* box_p1 = nullptr;           // 2 per node (p1x, p1y)
+  double* box_p2 = nullptr;           // 2 per node
+  int32_t* nbig = nullptr;            // reference big-list length per node
+  int32_t* npart = nullptr;           // particles per node
+  slsim::TreeDesc desc() const;
+};
+
+struct slsim_lens_s {
+  slsim_ctx ctx = nullptr;
+  double Om = 0.3, OL = 0.7, h = 0.7;
+  double zsource = 0;
+  double charge = 0;
+  bool toggle_source_plane = false;
+  double zs_implant = 0;
+  std::vector<std::vector<slsim_plane>> main_planes;   // halos per main plane
+  std::vector<double> main_Dl, main_z;
+  std::vector<slsim_plane> owned;
+  // combined (combinePlanes)
+  std::vector<int> order;                              // plane j -> main plane index
+  std::vector<double> plane_redshifts, Dl, dDl;
+  slsim::TreeDesc* trees_dev = nullptr;                // per plane: halos' descs
+  std::vector<int> tree_off, tree_cnt;
+  bool trees_dirty = true;
+  double coorDist(double z1, double z2) const;
+  void combinePlanes();
+  void FindSourcePlane(double zs, long& jmax, double& Dls, double& Ds) const;
+  int upload_trees();
+};
+
+namespace slsim {
+struct FlatLCDM;
+// kernel launchers (shoot.hip)
+void launch_plane_step(int precision, bool count, uint32_t nitems, hipStream_t st,
+                       const StepJob* jobs, const WorkItem* items, uint32_t* err);
+void launch_init_grid(hipStream_t st, const RayState& s, int64_t nslots, int64_t N,
+                      int64_t tiles_x, double c0, double c1, double range, int32_t jmax,
+                      double aal, double bbl);
+void launch_finalize_grid(hipStream_t st, const RayState& s, int64_t nslots, int64_t N,
+                          int64_t tiles_x, double* y, double* A);
+void launch_counts_grid(hipStream_t st, const uint32_t* c, int64_t nslots, int64_t N,
+                        int64_t tiles_x, uint64_t* out);
+void launch_init_array(hipStream_t st, const RayState& s, int64_t nslots, int64_t n,
+                       const uint32_t* perm, const double* x, const double* zs, int32_t jmax,
+                       double aal, double bbl, const double* zpl, const double* Dl,
+                       int32_t nplanes, const FlatLCDM& cosmo);
+void launch_finalize_array(hipStream_t st, const RayState& s, int64_t n, const uint32_t* perm,
+                           double* y, double* A);
+void launch_ray_morton(hipStream_t st, const double* x, int64_t n, double x0, double y0,
+                       double inv_w, uint64_t* code, uint32_t* idx);
+void launch_minmax_rays(hipStream_t st, const double* x, int64_t n, double* part, int nblk);
+void launch_force(hipStream_t st, int precision, const TreeDesc* tree, const double* rays,
+                  int64_t n, double* out, uint32_t* err);
+void launch_direct(hipStream_t st, const PartD* parts, int64_t np, double ia, const double* rays,
+                   int64_t n, double* out);
+// tree build (tree_build.hip)
+template <typename T>
+int build_plane(slsim_ctx ctx, const T* x, const T* y, const float* m, const float* h, size_t n,
+                slsim_plane pl);
+int export_nodes(slsim_plane pl, double* dbl, int64_t* ints);
+}  // namespace slsim

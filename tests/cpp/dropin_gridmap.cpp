@@ -1,0 +1,60 @@
+// Reference-style user program against the drop-in header: builds a
+// projected-NFW particle halo, moves it into a Lens, shoots a GridMap and a
+// list of RAYs with per-ray source redshifts, and writes the results as raw
+// float64 to the file named by argv[1] (read back by the parity test).
+#include <cstdio>
+#include <vector>
+
+#include "glamer_amd.h"
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: %s out.bin\n", argv[0]);
+    return 2;
+  }
+  const size_t n = 16384;
+  COSMOLOGY cosmo(0.3, 0.7, 0.7);
+  std::vector<double> x(n), y(n);
+  if (slsim_gen_nfw(1001, n, 1e13, 8.0, 0.5, 0.3, 0.7, 0.7, 0.0, 0.0, x.data(), y.data()))
+    return 1;
+  std::vector<ParticleType<double>> particles(n);
+  for (size_t i = 0; i < n; ++i) {
+    particles[i].x[0] = x[i];
+    particles[i].x[1] = y[i];
+    particles[i].x[2] = 0.0;
+    particles[i].Mass = (float)(1e13 / n);
+    particles[i].Size = 1e-4f;
+    particles[i].type = 0;
+  }
+  long seed = -1;
+  Lens lens(&seed, 2.0, cosmo);
+  LensHaloParticles<ParticleType<double>> halo(particles, 0.5, cosmo, Point_2d(0, 0), false, 0);
+  lens.moveinMainHalo(halo, true);
+
+  const double center[2] = {0.0, 0.0};
+  const unsigned long N = 64;
+  GridMap grid(&lens, N, center, 30 * arcsecTOradians);
+  std::vector<double> out;
+  for (size_t i = 0; i < grid.getNumberOfPoints(); ++i) {
+    RAY r = grid.ray(i);
+    out.insert(out.end(), {r.x[0], r.x[1], r.y[0], r.y[1], r.A[0], r.A[1], r.A[2], r.A[3]});
+  }
+  PixelMap<double> kap = grid.writePixelMap<double>(LensingVariable::KAPPA);
+  std::vector<RAY> rays(200);
+  for (size_t i = 0; i < rays.size(); ++i) {
+    rays[i].x[0] = (i % 20 - 9.5) * 1e-6;
+    rays[i].x[1] = (i / 20 - 4.5) * 2e-6;
+    rays[i].z = 1.0 + 0.01 * i;
+  }
+  lens.rayshooterInternal(rays.size(), rays.data());
+  for (auto& r : rays)
+    out.insert(out.end(), {r.x[0], r.x[1], r.y[0], r.y[1], r.A[0], r.A[1], r.A[2], r.A[3]});
+  out.push_back(kap[0]);
+  FILE* f = std::fopen(argv[1], "wb");
+  if (!f) return 1;
+  std::fwrite(out.data(), sizeof(double), out.size(), f);
+  std::fclose(f);
+  std::printf("dropin_gridmap: %zu rays, %d planes, kappa[0]=%.6e\n", out.size() / 8,
+              lens.getNplanes(), kap[0]);
+  return 0;
+}

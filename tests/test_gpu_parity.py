@@ -1,0 +1,296 @@
+"""GPU parity of the HIP ray shooter against the CPU oracle, through the C ABI.
+
+Tolerances (stated, per ray):
+  FP64 path vs oracle:   relative |d alpha| <= 1e-4, |d kappa|, |d gamma| <= 1e-3
+                         (same algorithm in the same precision; observed ~1e-12)
+  FP32C path vs FP64:    relative |d alpha| <= 1e-5, |d kappa|, |d gamma| <= 1e-3
+Relative error of a vector quantity v: |dv| / max(|v_ref|, 1e-3 rms|v_ref|).
+Tree structure and interaction counts must match the oracle exactly.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import slsimlib_amd as S
+from slsimlib_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+ALPHA_TOL = 1e-4
+KG_TOL = 1e-3
+FP32C_ALPHA_TOL = 1e-5
+
+
+def rel_err(d, ref):
+    """|d| / max(|ref|, 1e-3 rms(|ref|)) per row."""
+    d = np.asarray(d).reshape(len(d), -1)
+    ref = np.asarray(ref).reshape(len(ref), -1)
+    nr = np.linalg.norm(ref, axis=1)
+    nd = np.linalg.norm(d, axis=1)
+    floor = 1e-3 * np.sqrt(np.mean(nr ** 2))
+    return nd / np.maximum(nr, floor if floor > 0 else 1e-300)
+
+
+def lensing_fields(x, y, A):
+    alpha = x - y
+    kappa = 1 - 0.5 * (A[:, 0] + A[:, 3])
+    gamma = np.stack([0.5 * (A[:, 0] - A[:, 3]), 0.5 * (A[:, 1] + A[:, 2]),
+                      0.5 * (A[:, 2] - A[:, 1])], 1)
+    return alpha, kappa, gamma
+
+
+def assert_fields_close(x, y, A, y_ref, A_ref, atol_alpha, atol_kg):
+    a, k, g = lensing_fields(x, y, A)
+    ar, kr, gr = lensing_fields(x, y_ref, A_ref)
+    ea = rel_err(a - ar, ar).max()
+    ek = rel_err(k - kr, kr).max()
+    eg = rel_err(g - gr, gr).max()
+    assert ea <= atol_alpha, ("alpha", ea)
+    assert ek <= atol_kg, ("kappa", ek)
+    assert eg <= atol_kg, ("gamma", eg)
+    return ea, ek, eg
+
+
+# ----------------------------------------------------------------- tree build
+def _compare_trees(dev, orc):
+    d, q = dev.export_nodes()
+    od, oq = orc.export_nodes()
+    assert d.shape == od.shape, (d.shape, od.shape)
+    assert np.array_equal(q[:, :4], oq[:, :4])          # level, nparticles, nbig, leaf
+    assert np.array_equal(d[:, :4], od[:, :4])          # boxes, bit-exact
+    occ = q[:, 1] > 0
+    box = (d[:, 2] - d[:, 0])
+    assert np.all(np.abs(d[occ, 6] - od[occ, 6]) <= 1e-13 * np.abs(od[occ, 6]))
+    assert np.all(np.abs(d[occ, 4:6] - od[occ, 4:6]) <= 1e-13 * box[occ, None])
+    qs = np.abs(od[occ, 6]) * box[occ] ** 2
+    assert np.all(np.abs(d[occ, 7:10] - od[occ, 7:10]) <= 1e-11 * qs[:, None])
+    assert np.all(np.abs(d[occ, 10] - od[occ, 10]) <= 1e-11 * od[occ, 10])
+
+
+def test_tree_matches_oracle_c1(ctx):
+    x, y, m, h = W.c1_particles()
+    dev = S.LensHaloParticles(ctx, x, y, m, h, 0.5)
+    _compare_trees(dev, O.Tree(x, y, m, h))
+    info = dev.info()
+    assert info["n_particles"] == x.size and info["n_big_internal"] == 0
+
+
+@pytest.mark.parametrize("case", ["single", "bucket", "ties", "bands", "float32", "dupes"])
+def test_tree_edge_cases(ctx, case):
+    rng = np.random.default_rng(5)
+    if case == "single":
+        x, y = np.array([0.3]), np.array([-0.1])
+    elif case == "bucket":
+        x, y = rng.random(20), rng.random(20)
+    elif case == "ties":                       # particles on every split line
+        g = np.linspace(0, 1, 17)
+        X, Y = np.meshgrid(g, g)
+        x, y = X.ravel(), Y.ravel()
+    elif case == "dupes":
+        x, y = rng.random(3000), rng.random(3000)
+        x[100:110] = x[5]
+        y[100:110] = y[5]
+    else:
+        x, y = rng.random(4000) ** 3, rng.random(4000) ** 3
+    n = x.size
+    m = (1e9 * (0.5 + rng.random(n))).astype(np.float32)
+    h = np.full(n, 1e-6, np.float32)
+    if case == "bands":
+        h = (10.0 ** rng.uniform(-6, 0.2, n)).astype(np.float32)
+    if case == "float32":
+        x, y = x.astype(np.float32), y.astype(np.float32)
+    dev = S.LensHaloParticles(ctx, x, y, m, h, 0.5)
+    _compare_trees(dev, O.Tree(x, y, m, h))
+    if case == "bands":
+        assert dev.info()["n_big_internal"] > 0
+
+
+def test_tree_same_place_error(ctx):
+    with pytest.raises(S.SlsimError) as e:
+        S.LensHaloParticles(ctx, np.zeros(10), np.arange(10.0), np.ones(10, np.float32),
+                            np.ones(10, np.float32), 0.5)
+    assert e.value.code == 4
+
+
+# ----------------------------------------------------------------- one plane
+def _c1_rays(N=256):
+    z = W.C1["z_lens"]
+    DA = S.coorDist(0, np.float32(z)) / (1 + np.float32(z))
+    pos = O.gridmap_positions(N, (0.0, 0.0), W.grid_range(W.C1["range_arcsec"]))
+    return pos * DA
+
+
+def test_plane_force_fp64_matches_oracle(ctx):
+    x, y, m, h = W.c1_particles()
+    dev = S.LensHaloParticles(ctx, x, y, m, h, 0.5)
+    T = O.Tree(x, y, m, h)
+    rays = _c1_rays(128)
+    f = dev.force(rays, S.SLSIM_FP64)
+    o = T.force(rays)
+    assert rel_err(f[:, :2] - o[:, :2], o[:, :2]).max() < 1e-11
+    assert rel_err(f[:, 2:] - o[:, 2:], o[:, 2:]).max() < 1e-10
+
+
+def test_plane_force_fp32c_close_to_fp64(ctx):
+    x, y, m, h = W.c1_particles()
+    dev = S.LensHaloParticles(ctx, x, y, m, h, 0.5)
+    rays = _c1_rays(256)
+    f64 = dev.force(rays, S.SLSIM_FP64)
+    f32 = dev.force(rays, S.SLSIM_FP32C)
+    assert rel_err(f32[:, :2] - f64[:, :2], f64[:, :2]).max() <= FP32C_ALPHA_TOL
+    assert rel_err(f32[:, 2:] - f64[:, 2:], f64[:, 2:]).max() <= KG_TOL
+
+
+def test_direct_sum_matches_oracle(ctx):
+    x, y, m, h = W.c1_particles()
+    dev = S.LensHaloParticles(ctx, x, y, m, h, 0.5)
+    rays = _c1_rays(64)[::7]
+    g = dev.direct_sum(rays)
+    o = O.direct_sum(x, y, m, h, rays)
+    assert rel_err(g[:, :2] - o[:, :2], o[:, :2]).max() < 1e-11
+    assert rel_err(g[:, 2:] - o[:, 2:], o[:, 2:]).max() < 1e-9
+
+
+# ----------------------------------------------------------- whole ray shoot
+@pytest.fixture(scope="module")
+def c1(ctx):
+    lens, parts = W.build_c1_lens(ctx)
+    x, y, m, h = parts
+    L = O.Lens(W.C1["z_source"])
+    L.moveinMainHalo(O.Tree(x, y, m, h), W.C1["z_lens"])
+    N = W.C1["N"]
+    rg = W.grid_range(W.C1["range_arcsec"])
+    pos = O.gridmap_positions(N, (0.0, 0.0), rg)
+    y_ref, A_ref, cnt = L.shoot(pos, counts=True)
+    return {"lens": lens, "oracle": L, "pos": pos, "y": y_ref, "A": A_ref, "cnt": cnt,
+            "N": N, "range": rg}
+
+
+def test_lens_tables_match_oracle(c1):
+    a, b = c1["lens"].tables(), c1["oracle"].tables()
+    assert a["nplanes"] == b["nplanes"] == 1
+    assert np.array_equal(a["z"], b["z"])
+    assert np.allclose(a["Dl"], b["Dl"], rtol=1e-13) and a["charge"] == b["charge"]
+
+
+def test_gridmap_fp64_matches_oracle_c1(c1):
+    g = S.GridMap(c1["lens"], c1["N"], (0.0, 0.0), c1["range"], S.SLSIM_FP64)
+    assert np.array_equal(g.positions(), c1["pos"])
+    ea, ek, eg = assert_fields_close(c1["pos"], g.y(), g.A(), c1["y"], c1["A"], ALPHA_TOL, KG_TOL)
+    assert ea < 1e-10 and ek < 1e-8 and eg < 1e-8     # same arithmetic: rounding only
+
+
+def test_interaction_counts_match_oracle(c1):
+    g = S.GridMap(c1["lens"], c1["N"], (0.0, 0.0), c1["range"], S.SLSIM_FP64, shoot=False)
+    cnt = g.counts()
+    assert np.array_equal(cnt, c1["cnt"])
+
+
+def test_gridmap_fp32c_vs_fp64_c1(c1):
+    g64 = S.GridMap(c1["lens"], c1["N"], (0.0, 0.0), c1["range"], S.SLSIM_FP64)
+    g32 = S.GridMap(c1["lens"], c1["N"], (0.0, 0.0), c1["range"], S.SLSIM_FP32C)
+    ea, ek, eg = assert_fields_close(c1["pos"], g32.y(), g32.A(), g64.y(), g64.A(),
+                                     FP32C_ALPHA_TOL, KG_TOL)
+    print("fp32c vs fp64: alpha %.2e kappa %.2e gamma %.2e" % (ea, ek, eg))
+
+
+def test_array_rays_bitwise_equal_grid_rays(c1):
+    # the array path Morton-sorts rays into different bundles: each ray's
+    # result must not depend on its bundle mates
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(c1["pos"].shape[0])
+    for prec in (S.SLSIM_FP64, S.SLSIM_FP32C):
+        g = S.GridMap(c1["lens"], c1["N"], (0.0, 0.0), c1["range"], prec)
+        y, A = c1["lens"].rayshooterInternal(c1["pos"][perm], precision=prec)
+        assert np.array_equal(y, g.y()[perm]) and np.array_equal(A, g.A()[perm])
+
+
+def test_gridmap_repeatable(c1):
+    g1 = S.GridMap(c1["lens"], c1["N"], (0.0, 0.0), c1["range"], S.SLSIM_FP32C)
+    g2 = S.GridMap(c1["lens"], c1["N"], (0.0, 0.0), c1["range"], S.SLSIM_FP32C)
+    assert np.array_equal(g1.y(), g2.y()) and np.array_equal(g1.A(), g2.A())
+
+
+# ------------------------------------------------------------- multi-plane
+@pytest.fixture(scope="module")
+def field(ctx):
+    n = 1 << 14
+    lens, parts, zl = W.build_field_lens(ctx, config=2, nplanes=10, n=n, keep_particles=True)
+    L = O.Lens(3.0)
+    for (x, y, m, h, z) in parts:
+        L.moveinMainHalo(O.Tree(x, y, m, h), z)
+    return {"lens": lens, "oracle": L}
+
+
+def test_multiplane_tables(field):
+    a, b = field["lens"].tables(), field["oracle"].tables()
+    assert a["nplanes"] == b["nplanes"] == 10
+    assert np.array_equal(a["z"], b["z"])
+    assert np.allclose(a["Dl"], b["Dl"], rtol=1e-13)
+
+
+def test_multiplane_gridmap_matches_oracle(field):
+    N, rg = 128, W.grid_range(300.0)
+    pos = O.gridmap_positions(N, (0.0, 0.0), rg)
+    y_ref, A_ref, cnt = field["oracle"].shoot(pos, counts=True)
+    g = S.GridMap(field["lens"], N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    ea, ek, eg = assert_fields_close(pos, g.y(), g.A(), y_ref, A_ref, ALPHA_TOL, KG_TOL)
+    assert ea < 1e-9
+    assert np.array_equal(g.counts(), cnt)
+    g32 = S.GridMap(field["lens"], N, (0.0, 0.0), rg, S.SLSIM_FP32C)
+    assert_fields_close(pos, g32.y(), g32.A(), g.y(), g.A(), FP32C_ALPHA_TOL, KG_TOL)
+
+
+def test_per_ray_source_redshifts_match_oracle(field):
+    rng = np.random.default_rng(11)
+    n = 3000
+    x = (rng.random((n, 2)) - 0.5) * W.grid_range(300.0)
+    zs = rng.uniform(0.05, 4.0, n)          # includes sources in front of planes
+    y, A = field["lens"].rayshooterInternal(x, zs=zs, precision=S.SLSIM_FP64)
+    y_ref, A_ref = field["oracle"].shoot(x, zs=zs)
+    assert_fields_close(x, y, A, y_ref, A_ref, ALPHA_TOL, KG_TOL)
+    front = zs <= np.float32(0.1)
+    assert np.array_equal(y[front], x[front])
+
+
+def test_reset_source_plane(field):
+    lens = field["lens"]
+    N, rg = 64, W.grid_range(300.0)
+    pos = O.gridmap_positions(N, (0.0, 0.0), rg)
+    lens.ResetSourcePlane(1.2)
+    g = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    lens.ResetSourcePlane(-1)
+    y_ref, A_ref = field["oracle"].shoot(pos, zs=np.full(pos.shape[0], 1.2))
+    assert_fields_close(pos, g.y(), g.A(), y_ref, A_ref, 1e-9, 1e-8)
+
+
+# ------------------------------------------------------------ small cases
+def test_no_plane_lens_is_identity(ctx):
+    lens = S.Lens(ctx, 1.0)
+    g = S.GridMap(lens, 16, (0.01, -0.02), 1e-3)
+    assert np.array_equal(g.y(), g.positions())
+    assert np.array_equal(g.A(), np.tile([1.0, 0, 0, 1.0], (256, 1)))
+
+
+def test_ragged_grid_sizes(c1):
+    for N in (1, 7, 63, 65):
+        pos = O.gridmap_positions(N, (1e-5, 0.0), c1["range"])
+        y_ref, A_ref = c1["oracle"].shoot(pos)
+        g = S.GridMap(c1["lens"], N, (1e-5, 0.0), c1["range"], S.SLSIM_FP64)
+        assert_fields_close(pos, g.y(), g.A(), y_ref, A_ref, 1e-9, 1e-8)
+
+
+def test_error_paths(ctx):
+    with pytest.raises(S.SlsimError) as e:
+        S.Lens(ctx, 2.0, 0.3, 0.6)
+    assert e.value.code == 6
+    small = S.Context(0, hbm_budget=1 << 20)
+    x, y, m, h = W.c1_particles()
+    with pytest.raises(S.SlsimError) as e:
+        S.LensHaloParticles(small, x, y, m, h, 0.5)
+    assert e.value.code == 3
+    assert small.memory()[0] == 0          # nothing leaked on the failed build
+    small.close()
+    lens = S.Lens(ctx, 2.0)
+    with pytest.raises(S.SlsimError):
+        S.GridMap(lens, 0, (0, 0), 1e-3)
