@@ -164,6 +164,73 @@ int slsim_plane_direct_sum(slsim_plane plane, size_t n, const double* rays, doub
 int slsim_plane_force(slsim_plane plane, size_t n, const double* rays, double* out,
                       int precision);
 
+/* --------------------------------------------------- batched GridMaps */
+/* Batching and serving have no counterpart in the reference, which shoots
+ * one GridMap at a time under a process-wide mutex (gridmap.cpp:15,116-119);
+ * per-request results are those of slsim_gridmap_shoot. */
+typedef struct slsim_batch_s* slsim_batch;
+typedef struct {
+  int32_t tile_log2;      /* bundle tile 2^t x 2^(6-t) rays: 3 = 8x8, 2 = 4x16, 4 = 16x4 */
+  int32_t queue;          /* 0: one wavefront per work item; 1: persistent grid, work queue */
+  int32_t order;          /* 0: request order; 1: requests interleaved bundle by bundle;
+                             2: requests on the largest planes first                      */
+  int32_t graph;          /* 1: replay the whole launch sequence as a captured HIP graph  */
+  int32_t blocks_per_cu;  /* persistent grid: blocks per compute unit (queue = 1)         */
+  int32_t pad;
+} slsim_schedule;
+
+/* n independent GridMap requests (lens_i, N_i, center_i (2 per request),
+ * range_i, source z_i (<= 0: the lens source)) shot together: one launch per
+ * plane step covers every request that still has a plane in front of its
+ * source.  Outputs stay in device buffers owned by the batch. */
+int slsim_batch_create(slsim_ctx ctx, int n, const slsim_lens* lenses, const int64_t* N,
+                       const double* center, const double* range, const double* zs,
+                       int precision, slsim_batch* out);
+int slsim_batch_set_schedule(slsim_batch b, const slsim_schedule* s);
+int slsim_batch_get_schedule(slsim_batch b, slsim_schedule* s);
+int slsim_batch_run(slsim_batch b, slsim_shoot_stats* stats);
+/* Device pointers of request i's y (2 N^2) and A (4 N^2), grid order. */
+int slsim_batch_output(slsim_batch b, int i, double** y_dev, double** A_dev);
+/* Autotuner: times schedules (greedy over tile, queue, order, queue grid
+ * size; graphs on) and keeps the fastest.  base_ms: the fixed schedule
+ * {8x8, one wave per item, request order, no graph}; best_ms: the chosen. */
+int slsim_batch_autotune(slsim_batch b, int repeats, slsim_schedule* best, double* base_ms,
+                         double* best_ms);
+int slsim_batch_destroy(slsim_batch b);
+
+/* ----------------------------------------------------------- serving */
+typedef struct slsim_server_s* slsim_server;
+typedef struct {
+  double t_submit, t_admit, t_done, t_collect;   /* slsim_now() seconds            */
+  int64_t rays;
+  int32_t iterations;                            /* plane steps the request took   */
+  int32_t status;                                /* SLSIM_OK or the failure code   */
+} slsim_request_info;
+typedef struct {
+  int64_t submitted, admitted, rejected, completed, iterations, max_active;
+  int64_t in_use_bytes, budget_bytes;
+  double busy_s;                                 /* time with requests in flight   */
+} slsim_server_stats;
+
+/* Request loop with continuous batching: a scheduler thread keeps the
+ * lenses' trees resident, admits queued GridMap requests while their ray
+ * state and outputs fit in hbm_budget_bytes, and runs one plane step of
+ * every admitted request per launch, so requests that arrive while others
+ * are on the GPU join at the next step.  Each result equals the request's
+ * slsim_gridmap_shoot result bit for bit. */
+int slsim_server_create(slsim_ctx ctx, size_t hbm_budget_bytes, slsim_server* out);
+/* Queues a request; SLSIM_ERR_BUDGET if it can never fit the budget. */
+int slsim_server_submit(slsim_server s, slsim_lens lens, int64_t N, double c0, double c1,
+                        double range, double zs, int precision, uint64_t* id);
+/* Blocks until request `id` is done, copies y (2 N^2) and A (4 N^2) to the
+ * host buffers (either may be NULL) and releases its device memory. */
+int slsim_server_wait(slsim_server s, uint64_t id, double* y_host, double* A_host,
+                      slsim_request_info* info);
+int slsim_server_get_stats(slsim_server s, slsim_server_stats* st);
+int slsim_server_destroy(slsim_server s);
+/* Monotonic clock (seconds) of the request timestamps. */
+double slsim_now(void);
+
 /* ------------------------------------------------------- device buffers */
 int slsim_device_alloc(slsim_ctx ctx, size_t bytes, void** dev);
 int slsim_device_free(slsim_ctx ctx, void* dev);

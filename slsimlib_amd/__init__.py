@@ -17,7 +17,8 @@ arcsecTOradians = 0.000004848136811   # include/standard.h:51
 Grav = 4.7788e-20                      # include/standard.h:43
 
 __all__ = [
-    "Context", "LensHaloParticles", "Lens", "GridMap", "coorDist", "invCoorDist",
+    "Context", "LensHaloParticles", "Lens", "GridMap", "Batch", "Server", "coorDist",
+    "invCoorDist",
     "nfw_r200", "gen_nfw", "gen_halo_field", "SLSIM_FP64", "SLSIM_FP32C", "SLSIM_DIRECT64",
     "SlsimError",
     "arcsecTOradians", "Grav", "lib",
@@ -309,6 +310,123 @@ class GridMap:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Many GridMap requests shot together (one launch per plane step).
+
+    requests: list of (lens, N, (c0, c1), range, zs) with zs <= 0 for the
+    lens source."""
+
+    def __init__(self, ctx, requests, precision=SLSIM_FP64):
+        self.ctx = ctx
+        self.requests = list(requests)
+        n = len(self.requests)
+        lenses = (_abi.slsim_lens * n)(*[r[0].h for r in self.requests])
+        N = np.array([r[1] for r in self.requests], np.int64)
+        cen = np.array([c for r in self.requests for c in r[2]], np.float64)
+        rg = np.array([r[3] for r in self.requests], np.float64)
+        zs = np.array([r[4] for r in self.requests], np.float64)
+        self._keep = (lenses, N, cen, rg, zs)
+        self.h = _abi.slsim_batch()
+        _check(lib.slsim_batch_create(ctx.h, n, lenses, _ptr(N, ctypes.c_int64),
+                                      _ptr(cen, ctypes.c_double), _ptr(rg, ctypes.c_double),
+                                      _ptr(zs, ctypes.c_double), int(precision),
+                                      ctypes.byref(self.h)))
+
+    def set_schedule(self, tile_log2=3, queue=0, order=0, graph=0, blocks_per_cu=4):
+        s = _abi.slsim_schedule(tile_log2, queue, order, graph, blocks_per_cu, 0)
+        _check(lib.slsim_batch_set_schedule(self.h, ctypes.byref(s)))
+
+    def schedule(self):
+        s = _abi.slsim_schedule()
+        _check(lib.slsim_batch_get_schedule(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_ if k != "pad"}
+
+    def run(self):
+        st = _abi.slsim_shoot_stats()
+        _check(lib.slsim_batch_run(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_ if k != "pad"}
+
+    def autotune(self, repeats=1):
+        s = _abi.slsim_schedule()
+        b, t = ctypes.c_double(), ctypes.c_double()
+        _check(lib.slsim_batch_autotune(self.h, int(repeats), ctypes.byref(s), ctypes.byref(b),
+                                        ctypes.byref(t)))
+        return {k: getattr(s, k) for k, _ in s._fields_ if k != "pad"}, b.value, t.value
+
+    def output(self, i):
+        y = ctypes.POINTER(ctypes.c_double)()
+        A = ctypes.POINTER(ctypes.c_double)()
+        _check(lib.slsim_batch_output(self.h, int(i), ctypes.byref(y), ctypes.byref(A)))
+        N = self.requests[i][1]
+        yh = np.zeros((N * N, 2))
+        Ah = np.zeros((N * N, 4))
+        self.ctx.d2h(yh, ctypes.cast(y, ctypes.c_void_p))
+        self.ctx.d2h(Ah, ctypes.cast(A, ctypes.c_void_p))
+        return yh, Ah
+
+    def close(self):
+        if self.h:
+            lib.slsim_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop with continuous batching over resident lenses."""
+
+    def __init__(self, ctx, hbm_budget=0):
+        self.ctx = ctx
+        self.h = _abi.slsim_server()
+        _check(lib.slsim_server_create(ctx.h, ctypes.c_size_t(int(hbm_budget)),
+                                       ctypes.byref(self.h)))
+
+    def submit(self, lens, N, center, range_, zs=0.0, precision=SLSIM_FP32C):
+        rid = ctypes.c_uint64()
+        _check(lib.slsim_server_submit(self.h, lens.h, int(N), float(center[0]), float(center[1]),
+                                       float(range_), float(zs), int(precision),
+                                       ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid, N=None, fetch=True):
+        info = _abi.slsim_request_info()
+        y = A = None
+        if fetch:
+            y = np.zeros((N * N, 2))
+            A = np.zeros((N * N, 4))
+            rc = lib.slsim_server_wait(self.h, ctypes.c_uint64(rid), _ptr(y, ctypes.c_double),
+                                       _ptr(A, ctypes.c_double), ctypes.byref(info))
+        else:
+            rc = lib.slsim_server_wait(self.h, ctypes.c_uint64(rid), None, None,
+                                       ctypes.byref(info))
+        _check(rc)
+        return y, A, {k: getattr(info, k) for k, _ in info._fields_}
+
+    def stats(self):
+        st = _abi.slsim_server_stats()
+        _check(lib.slsim_server_get_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self.h:
+            lib.slsim_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def now():
+    return lib.slsim_now()
 
 
 def coorDist(z1, z2, Om=0.3, OL=0.7, h=0.7):

@@ -56,6 +56,30 @@ class slsim_shoot_stats(ctypes.Structure):
                 ("pad", ctypes.c_int32)]
 
 
+class slsim_schedule(ctypes.Structure):
+    _fields_ = [("tile_log2", ctypes.c_int32), ("queue", ctypes.c_int32),
+                ("order", ctypes.c_int32), ("graph", ctypes.c_int32),
+                ("blocks_per_cu", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class slsim_request_info(ctypes.Structure):
+    _fields_ = [("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double), ("t_collect", ctypes.c_double),
+                ("rays", ctypes.c_int64), ("iterations", ctypes.c_int32),
+                ("status", ctypes.c_int32)]
+
+
+class slsim_server_stats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("admitted", ctypes.c_int64),
+                ("rejected", ctypes.c_int64), ("completed", ctypes.c_int64),
+                ("iterations", ctypes.c_int64), ("max_active", ctypes.c_int64),
+                ("in_use_bytes", ctypes.c_int64), ("budget_bytes", ctypes.c_int64),
+                ("busy_s", ctypes.c_double)]
+
+
+slsim_batch = ctypes.c_void_p
+slsim_server = ctypes.c_void_p
+
 P = ctypes.POINTER
 c_int, c_double, c_float, c_size_t = ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_size_t
 c_u64, c_i64, c_vp = ctypes.c_uint64, ctypes.c_int64, ctypes.c_void_p
@@ -89,6 +113,23 @@ SIGNATURES = {
                                     P(c_u64)]),
     "slsim_plane_direct_sum": (c_int, [slsim_plane, c_size_t, P(c_double), P(c_double)]),
     "slsim_plane_force": (c_int, [slsim_plane, c_size_t, P(c_double), P(c_double), c_int]),
+    "slsim_batch_create": (c_int, [slsim_ctx, c_int, P(slsim_lens), P(c_i64), P(c_double),
+                                   P(c_double), P(c_double), c_int, P(slsim_batch)]),
+    "slsim_batch_set_schedule": (c_int, [slsim_batch, P(slsim_schedule)]),
+    "slsim_batch_get_schedule": (c_int, [slsim_batch, P(slsim_schedule)]),
+    "slsim_batch_run": (c_int, [slsim_batch, P(slsim_shoot_stats)]),
+    "slsim_batch_output": (c_int, [slsim_batch, c_int, P(P(c_double)), P(P(c_double))]),
+    "slsim_batch_autotune": (c_int, [slsim_batch, c_int, P(slsim_schedule), P(c_double),
+                                     P(c_double)]),
+    "slsim_batch_destroy": (c_int, [slsim_batch]),
+    "slsim_server_create": (c_int, [slsim_ctx, c_size_t, P(slsim_server)]),
+    "slsim_server_submit": (c_int, [slsim_server, slsim_lens, c_i64, c_double, c_double,
+                                    c_double, c_double, c_int, P(c_u64)]),
+    "slsim_server_wait": (c_int, [slsim_server, c_u64, P(c_double), P(c_double),
+                                  P(slsim_request_info)]),
+    "slsim_server_get_stats": (c_int, [slsim_server, P(slsim_server_stats)]),
+    "slsim_server_destroy": (c_int, [slsim_server]),
+    "slsim_now": (c_double, []),
     "slsim_device_alloc": (c_int, [slsim_ctx, c_size_t, P(c_vp)]),
     "slsim_device_free": (c_int, [slsim_ctx, c_vp]),
     "slsim_memcpy_d2h": (c_int, [slsim_ctx, c_vp, c_vp, c_size_t]),

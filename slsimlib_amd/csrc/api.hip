@@ -396,7 +396,7 @@ int slsim_lens_s::upload_trees() {
   return SLSIM_OK;
 }
 
-namespace {
+namespace slsim {
 
 // Source selection of compute_points_parallel for a uniform source redshift
 // (include/lens.h:940-953): jmax and the last plane's (aa, bb).
@@ -439,6 +439,10 @@ void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint3
     s.counters = counters;
   }
 }
+
+}  // namespace slsim
+
+namespace {
 
 // Runs the plane steps [0, jmax) of one job on the context stream.
 int run_steps(slsim_ctx c, const StepJob* djobs, int jmax, uint32_t nbundles, int precision,
@@ -511,7 +515,7 @@ int gridmap_run(slsim_lens L, size_t N, double c0, double c1, double range, doub
   double aal = 0, bbl = 0;
   const int nplanes = (int)L->plane_redshifts.size() - 1;
   if (nplanes > 0) uniform_source(L, zs, &jmax, &aal, &bbl);
-  launch_init_grid(c->stream, st, nslots, (int64_t)N, tiles_x, c0, c1, range, jmax, aal, bbl);
+  launch_init_grid(c->stream, st, nslots, (int64_t)N, tiles_x, 3, c0, c1, range, jmax, aal, bbl);
   if (jmax > 0) {
     std::vector<StepJob> steps;
     make_steps(L, st, nslots, counters, steps);
@@ -523,11 +527,12 @@ int gridmap_run(slsim_lens L, size_t N, double c0, double c1, double range, doub
                    cnt_host != nullptr, err, stats);
     if (rc) return rc;
   }
-  if (y_dev && A_dev) launch_finalize_grid(c->stream, st, nslots, (int64_t)N, tiles_x, y_dev, A_dev);
+  if (y_dev && A_dev)
+    launch_finalize_grid(c->stream, st, nslots, (int64_t)N, tiles_x, 3, y_dev, A_dev);
   if (cnt_host) {
     uint64_t* dcnt;
     if ((rc = s.get(&dcnt, 5 * N * N))) return rc;
-    launch_counts_grid(c->stream, counters, nslots, (int64_t)N, tiles_x, dcnt);
+    launch_counts_grid(c->stream, counters, nslots, (int64_t)N, tiles_x, 3, dcnt);
     SLSIM_HIP_CHECK(hipMemcpyAsync(cnt_host, dcnt, 40 * N * N, hipMemcpyDeviceToHost, c->stream));
   }
   SLSIM_HIP_CHECK(hipGetLastError());

@@ -9,6 +9,7 @@
 // seed and the halo index, independent of the thread count.
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <numeric>
@@ -66,37 +67,51 @@ uint64_t splitmix(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-// n particles in chunks of 65536, chunk k drawn from its own stream
-// splitmix(seed + k), generated on up to 16 threads.
-void sample_nfw(uint64_t seed, size_t n, double rs, double c, double xc, double yc, double* x,
-                double* y) {
-  const double mc = nfw_m(c);
-  const size_t chunk = 65536;
-  const size_t nchunks = (n + chunk - 1) / chunk;
-  auto work = [&](size_t k0, size_t k1) {
-    for (size_t k = k0; k < k1; ++k) {
-      Rng rng(splitmix(seed + k));
-      const size_t e = std::min(n, (k + 1) * chunk);
-      for (size_t i = k * chunk; i < e; ++i) {
-        const double u = rng.uniform();
-        const double r = rs * nfw_invert(u * mc, c);
-        const double ct = 2 * rng.uniform() - 1;
-        const double ph = 2 * kPI * rng.uniform();
-        const double st = std::sqrt(std::max(0.0, 1 - ct * ct));
-        x[i] = xc + r * st * std::cos(ph);
-        y[i] = yc + r * st * std::sin(ph);
-      }
-    }
-  };
-  const size_t nt = std::min<size_t>(std::max(1u, std::thread::hardware_concurrency()),
-                                     std::min<size_t>(16, nchunks));
-  if (nt <= 1) { work(0, nchunks); return; }
-  std::vector<std::thread> th;
-  const size_t per = (nchunks + nt - 1) / nt;
-  for (size_t t = 0; t < nt; ++t) {
-    const size_t a = t * per, b = std::min(nchunks, a + per);
-    if (a < b) th.emplace_back(work, a, b);
+// One chunk of up to 65536 particles of one halo, drawn from its own stream.
+struct Chunk {
+  uint64_t seed;
+  size_t off, n;
+  double rs, c, xc, yc;
+};
+
+void run_chunk(const Chunk& ch, double* x, double* y) {
+  Rng rng(ch.seed);
+  const double mc = nfw_m(ch.c);
+  for (size_t i = ch.off; i < ch.off + ch.n; ++i) {
+    const double u = rng.uniform();
+    const double r = ch.rs * nfw_invert(u * mc, ch.c);
+    const double ct = 2 * rng.uniform() - 1;
+    const double ph = 2 * kPI * rng.uniform();
+    const double st = std::sqrt(std::max(0.0, 1 - ct * ct));
+    x[i] = ch.xc + r * st * std::cos(ph);
+    y[i] = ch.yc + r * st * std::sin(ph);
   }
+}
+
+// Halo of n particles at offset off, split into chunks; chunk k of a halo
+// uses the stream splitmix(seed + k).
+void add_halo_chunks(std::vector<Chunk>& out, uint64_t seed, size_t off, size_t n, double rs,
+                     double c, double xc, double yc) {
+  const size_t chunk = 65536;
+  for (size_t k = 0; k * chunk < n; ++k)
+    out.push_back({splitmix(seed + k), off + k * chunk, std::min(chunk, n - k * chunk), rs, c,
+                   xc, yc});
+}
+
+// Runs the chunks on up to 16 threads (results do not depend on the count).
+void run_chunks(const std::vector<Chunk>& chunks, double* x, double* y) {
+  const size_t nt = std::min<size_t>(std::max(1u, std::thread::hardware_concurrency()),
+                                     std::min<size_t>(16, chunks.size()));
+  if (nt <= 1) {
+    for (const auto& ch : chunks) run_chunk(ch, x, y);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::thread> th;
+  for (size_t t = 0; t < nt; ++t)
+    th.emplace_back([&]() {
+      for (size_t k = next++; k < chunks.size(); k = next++) run_chunk(chunks[k], x, y);
+    });
   for (auto& t : th) t.join();
 }
 
@@ -114,7 +129,9 @@ int slsim_gen_nfw(uint64_t seed, size_t n, double mass, double conc, double z, d
                   double OL, double h, double xc, double yc, double* x, double* y) {
   if (!x || !y || !(conc > 0) || !(mass > 0)) return SLSIM_ERR_ARG;
   const double r200 = slsim_nfw_r200(mass, z, Om, OL, h);
-  sample_nfw(splitmix(seed), n, r200 / conc, conc, xc, yc, x, y);
+  std::vector<Chunk> chunks;
+  add_halo_chunks(chunks, splitmix(seed), 0, n, r200 / conc, conc, xc, yc);
+  run_chunks(chunks, x, y);
   return SLSIM_OK;
 }
 
@@ -150,12 +167,14 @@ int slsim_gen_halo_field(uint64_t seed, size_t n, int n_halos, double Mmin, doub
   for (size_t k = 0; used < n; ++k, ++used) cnt[rem[k % n_halos].second]++;
   const float mp = (float)(Mtot / (double)n);
   size_t off = 0;
+  std::vector<Chunk> chunks;
   for (int k = 0; k < n_halos; ++k) {
     const double r200 = slsim_nfw_r200(M[k], z, Om, OL, h);
-    sample_nfw(splitmix(seed ^ (0x5bd1e995ull * (k + 1))), cnt[k], r200 / c[k], c[k], cx[k],
-               cy[k], x + off, y + off);
+    add_halo_chunks(chunks, splitmix(seed ^ (0x5bd1e995ull * (k + 1))), off, cnt[k], r200 / c[k],
+                    c[k], cx[k], cy[k]);
     off += cnt[k];
   }
+  run_chunks(chunks, x, y);
   for (size_t i = 0; i < n; ++i) {
     m[i] = mp;
     s[i] = softening;

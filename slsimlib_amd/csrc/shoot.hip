@@ -473,24 +473,13 @@ __device__ inline double phys_coord(double th, double Dphys) {
 
 }  // namespace
 
-// One plane step for every bundle of the listed jobs.  Grid: one wavefront
-// per work item, 4 per block.
+// One plane step of one 64-ray bundle (one work item); called by a whole
+// wavefront.
 template <int MODE, bool COUNT>
-__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
-    const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t nitems,
-    uint32_t* __restrict__ err) {
-  __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
-  __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
-  const int w = uni((int)(threadIdx.x >> 6));
+__device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, WorkItem it,
+                                             int32_t* s_node, uint64_t* s_mask,
+                                             uint32_t* __restrict__ err) {
   const int lane = lane_id();
-  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const uint32_t wi = blk * kWavesPerBlock + (uint32_t)w;
-  if (wi >= nitems) return;
-  WorkItem it{0u, wi};
-  if (items) {
-    it.job = as_const(items)[wi].job;
-    it.bundle = as_const(items)[wi].bundle;
-  }
   const StepJob jb = as_const(jobs)[it.job];
   const int slot = (int)it.bundle * kWave + lane;
   const bool valid = slot < jb.nslots;
@@ -513,7 +502,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
     // (MultiPlane/planes.cpp:151-152, include/particle_halo.h:432-433).
     if (MODE == SLSIM_FP32C) {
       Acc32 a;
-      walk32(t, rx, ry, active, a, s_node[w], s_mask[w], err);
+      walk32(t, rx, ry, active, a, s_node, s_mask, err);
       alpha0 += a.ax.value(); alpha1 += a.ay.value(); kappa += a.k.value();
       gamma0 += a.g0.value(); gamma1 += a.g1.value();
     } else if (MODE == SLSIM_DIRECT64) {
@@ -522,7 +511,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
       alpha0 += a.ax; alpha1 += a.ay; kappa += a.k; gamma0 += a.g0; gamma1 += a.g1;
     } else {
       Acc64 a;
-      walk64<COUNT>(t, rx, ry, active, a, cnt, s_node[w], s_mask[w], err);
+      walk64<COUNT>(t, rx, ry, active, a, cnt, s_node, s_mask, err);
       alpha0 += a.ax; alpha1 += a.ay; kappa += a.k; gamma0 += a.g0; gamma1 += a.g1;
     }
   }
@@ -534,36 +523,73 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
   recursion_step(jb, slot, alpha0, alpha1, kappa, gamma0, gamma1);
 }
 
+// One plane step for every bundle of the listed jobs.  Grid: one wavefront
+// per work item, 4 per block (items == null: job 0, bundle = item index).
+template <int MODE, bool COUNT>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
+    const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t nitems,
+    uint32_t* __restrict__ err) {
+  __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
+  __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
+  const int w = uni((int)(threadIdx.x >> 6));
+  const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t wi = blk * kWavesPerBlock + (uint32_t)w;
+  if (wi >= nitems) return;
+  WorkItem it{0u, wi};
+  if (items) {
+    it.job = as_const(items)[wi].job;
+    it.bundle = as_const(items)[wi].bundle;
+  }
+  process_item<MODE, COUNT>(jobs, it, s_node[w], s_mask[w], err);
+}
+
 template __global__ void k_plane_step<SLSIM_FP64, false>(const StepJob*, const WorkItem*, uint32_t, uint32_t*);
 template __global__ void k_plane_step<SLSIM_FP32C, false>(const StepJob*, const WorkItem*, uint32_t, uint32_t*);
 template __global__ void k_plane_step<SLSIM_DIRECT64, false>(const StepJob*, const WorkItem*, uint32_t, uint32_t*);
 template __global__ void k_plane_step<SLSIM_FP64, true>(const StepJob*, const WorkItem*, uint32_t, uint32_t*);
 
+// Persistent variant: a fixed grid whose wavefronts take work items from a
+// queue counter until the list is exhausted (every wave reaches the exit).
+template <int MODE>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step_queue(
+    const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t nitems,
+    uint32_t* __restrict__ counter, uint32_t* __restrict__ err) {
+  __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
+  __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
+  const int w = uni((int)(threadIdx.x >> 6));
+  const int lane = lane_id();
+  for (;;) {
+    uint32_t idx = 0;
+    if (lane == 0) idx = atomicAdd(counter, 1u);
+    idx = (uint32_t)uni((int)idx);
+    if (idx >= nitems) break;
+    WorkItem it;
+    it.job = as_const(items)[idx].job;
+    it.bundle = as_const(items)[idx].bundle;
+    process_item<MODE, false>(jobs, it, s_node[w], s_mask[w], err);
+  }
+}
+
+__global__ void k_zero_u32(uint32_t* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 // GridMap ray layout (TreeCode_link/gridmap.cpp:293-321 via
-// Utilities::PositionFromIndex, TreeCode_link/utilities.cpp:149-158) in 8x8
-// tiles: bundle b covers tile (b % tiles_x, b / tiles_x).
-__device__ inline bool grid_slot(int64_t slot, int64_t N, int64_t tiles_x, int64_t* out) {
+// Utilities::PositionFromIndex, TreeCode_link/utilities.cpp:149-158) in
+// tiles of 2^tw x 2^(6-tw) rays: bundle b covers tile (b % tiles_x, b / tiles_x).
+__device__ inline bool grid_slot(int64_t slot, int64_t N, int64_t tiles_x, int tw, int64_t* out) {
   const int64_t b = slot >> 6;
   const int lane = (int)(slot & 63);
-  const int64_t ix = (b % tiles_x) * 8 + (lane & 7);
-  const int64_t iy = (b / tiles_x) * 8 + (lane >> 3);
+  const int64_t ix = (b % tiles_x) * (1 << tw) + (lane & ((1 << tw) - 1));
+  const int64_t iy = (b / tiles_x) * (1 << (6 - tw)) + (lane >> tw);
   if (ix >= N || iy >= N) return false;
   *out = ix + N * iy;
   return true;
 }
 
-__global__ void k_init_grid(RayState st, int64_t nslots, int64_t N, int64_t tiles_x, double c0,
-                            double c1, double range, int32_t jmax, double aal, double bbl) {
-#pragma clang fp contract(off)
-  const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (slot >= nslots) return;
-  int64_t i;
-  const bool valid = grid_slot(slot, N, tiles_x, &i);
-  double x0 = c0, x1 = c1;
-  if (valid && N > 1) {
-    x0 = c0 + range * (1.0 * (double)(i % N) / (double)(N - 1) - 0.5);
-    x1 = c1 + range * (1.0 * (double)(i / N) / (double)(N - 1) - 0.5);
-  }
+__device__ inline void init_slot(const RayState& st, int64_t slot, bool valid, double x0,
+                                 double x1, int32_t jmax, double aal, double bbl) {
   st.th0[slot] = x0; st.th1[slot] = x1;
   st.sa0[slot] = x0; st.sa1[slot] = x1;
   st.A0[slot] = 1; st.A1[slot] = 0; st.A2[slot] = 0; st.A3[slot] = 1;
@@ -573,12 +599,19 @@ __global__ void k_init_grid(RayState st, int64_t nslots, int64_t N, int64_t tile
   st.bbl[slot] = bbl;
 }
 
-__global__ void k_finalize_grid(RayState st, int64_t nslots, int64_t N, int64_t tiles_x,
-                                double* __restrict__ y, double* __restrict__ A) {
-  const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (slot >= nslots) return;
-  int64_t i;
-  if (!grid_slot(slot, N, tiles_x, &i)) return;
+__device__ inline void grid_position(int64_t i, int64_t N, double c0, double c1, double range,
+                                     double* x0, double* x1) {
+#pragma clang fp contract(off)
+  *x0 = c0;
+  *x1 = c1;
+  if (N > 1) {
+    *x0 = c0 + range * (1.0 * (double)(i % N) / (double)(N - 1) - 0.5);
+    *x1 = c1 + range * (1.0 * (double)(i / N) / (double)(N - 1) - 0.5);
+  }
+}
+
+__device__ inline void finalize_slot(const RayState& st, int64_t slot, int64_t i,
+                                     double* __restrict__ y, double* __restrict__ A) {
   y[2 * i] = st.th0[slot];
   y[2 * i + 1] = st.th1[slot];
   A[4 * i] = st.A0[slot];
@@ -587,13 +620,67 @@ __global__ void k_finalize_grid(RayState st, int64_t nslots, int64_t N, int64_t 
   A[4 * i + 3] = st.A3[slot];
 }
 
-__global__ void k_counts_grid(const uint32_t* __restrict__ c, int64_t nslots, int64_t N,
-                              int64_t tiles_x, uint64_t* __restrict__ out) {
+__global__ void k_init_grid(RayState st, int64_t nslots, int64_t N, int64_t tiles_x, int tw,
+                            double c0, double c1, double range, int32_t jmax, double aal,
+                            double bbl) {
+  const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (slot >= nslots) return;
+  int64_t i = 0;
+  const bool valid = grid_slot(slot, N, tiles_x, tw, &i);
+  double x0, x1;
+  grid_position(i, N, c0, c1, range, &x0, &x1);
+  init_slot(st, slot, valid, x0, x1, jmax, aal, bbl);
+}
+
+__global__ void k_finalize_grid(RayState st, int64_t nslots, int64_t N, int64_t tiles_x, int tw,
+                                double* __restrict__ y, double* __restrict__ A) {
   const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (slot >= nslots) return;
   int64_t i;
-  if (!grid_slot(slot, N, tiles_x, &i)) return;
+  if (!grid_slot(slot, N, tiles_x, tw, &i)) return;
+  finalize_slot(st, slot, i, y, A);
+}
+
+__global__ void k_counts_grid(const uint32_t* __restrict__ c, int64_t nslots, int64_t N,
+                              int64_t tiles_x, int tw, uint64_t* __restrict__ out) {
+  const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (slot >= nslots) return;
+  int64_t i;
+  if (!grid_slot(slot, N, tiles_x, tw, &i)) return;
   for (int k = 0; k < 5; ++k) out[5 * i + k] = c[5 * slot + k];
+}
+
+// Many GridMap requests at once: thread per slot of the concatenated slot
+// range, the request found by binary search over the slot offsets.
+__device__ inline int find_req(const GridReq* __restrict__ reqs, int nreq, int64_t g) {
+  int lo = 0, hi = nreq - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (reqs[mid].slot_off <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_init_batch(const GridReq* __restrict__ reqs, int nreq, int64_t total) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const GridReq& r = reqs[find_req(reqs, nreq, g)];
+  const int64_t slot = g - r.slot_off;
+  int64_t i = 0;
+  const bool valid = grid_slot(slot, r.N, r.tiles_x, r.tw, &i);
+  double x0, x1;
+  grid_position(i, r.N, r.c0, r.c1, r.range, &x0, &x1);
+  init_slot(r.st, slot, valid, x0, x1, r.jmax, r.aal, r.bbl);
+}
+
+__global__ void k_finalize_batch(const GridReq* __restrict__ reqs, int nreq, int64_t total) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const GridReq& r = reqs[find_req(reqs, nreq, g)];
+  const int64_t slot = g - r.slot_off;
+  int64_t i;
+  if (!grid_slot(slot, r.N, r.tiles_x, r.tw, &i)) return;
+  finalize_slot(r.st, slot, i, r.y, r.A);
 }
 
 // Array rays: slot s < n holds ray perm[s] (Morton order).  Per-ray source
@@ -776,21 +863,46 @@ void launch_plane_step(int precision, bool count, uint32_t nitems, hipStream_t s
     k_plane_step<SLSIM_FP64, false><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
 }
 
+void launch_zero_u32(hipStream_t st, uint32_t* p, int n) {
+  k_zero_u32<<<(n + 255) / 256, 256, 0, st>>>(p, n);
+}
+
+void launch_plane_step_queue(int precision, unsigned nblocks, uint32_t nitems, hipStream_t st,
+                             const StepJob* jobs, const WorkItem* items, uint32_t* counter,
+                             uint32_t* err) {
+  launch_zero_u32(st, counter, 1);
+  const dim3 blk(kWave * kWavesPerBlock);
+  if (precision == SLSIM_FP32C)
+    k_plane_step_queue<SLSIM_FP32C><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter, err);
+  else if (precision == SLSIM_DIRECT64)
+    k_plane_step_queue<SLSIM_DIRECT64><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter, err);
+  else
+    k_plane_step_queue<SLSIM_FP64><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter, err);
+}
+
 void launch_init_grid(hipStream_t st, const RayState& s, int64_t nslots, int64_t N,
-                      int64_t tiles_x, double c0, double c1, double range, int32_t jmax,
+                      int64_t tiles_x, int tw, double c0, double c1, double range, int32_t jmax,
                       double aal, double bbl) {
-  k_init_grid<<<blocks_for(nslots, 256), 256, 0, st>>>(s, nslots, N, tiles_x, c0, c1, range, jmax,
-                                                       aal, bbl);
+  k_init_grid<<<blocks_for(nslots, 256), 256, 0, st>>>(s, nslots, N, tiles_x, tw, c0, c1, range,
+                                                       jmax, aal, bbl);
 }
 
 void launch_finalize_grid(hipStream_t st, const RayState& s, int64_t nslots, int64_t N,
-                          int64_t tiles_x, double* y, double* A) {
-  k_finalize_grid<<<blocks_for(nslots, 256), 256, 0, st>>>(s, nslots, N, tiles_x, y, A);
+                          int64_t tiles_x, int tw, double* y, double* A) {
+  k_finalize_grid<<<blocks_for(nslots, 256), 256, 0, st>>>(s, nslots, N, tiles_x, tw, y, A);
 }
 
 void launch_counts_grid(hipStream_t st, const uint32_t* c, int64_t nslots, int64_t N,
-                        int64_t tiles_x, uint64_t* out) {
-  k_counts_grid<<<blocks_for(nslots, 256), 256, 0, st>>>(c, nslots, N, tiles_x, out);
+                        int64_t tiles_x, int tw, uint64_t* out) {
+  k_counts_grid<<<blocks_for(nslots, 256), 256, 0, st>>>(c, nslots, N, tiles_x, tw, out);
+}
+
+void launch_init_batch(hipStream_t st, const GridReq* reqs, int nreq, int64_t total) {
+  if (total > 0) k_init_batch<<<blocks_for(total, 256), 256, 0, st>>>(reqs, nreq, total);
+}
+
+void launch_finalize_batch(hipStream_t st, const GridReq* reqs, int nreq, int64_t total) {
+  if (total > 0) k_finalize_batch<<<blocks_for(total, 256), 256, 0, st>>>(reqs, nreq, total);
 }
 
 void launch_init_array(hipStream_t st, const RayState& s, int64_t nslots, int64_t n,

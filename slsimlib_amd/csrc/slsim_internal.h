@@ -108,6 +108,17 @@ struct WorkItem {
   uint32_t bundle;
 };
 
+// One GridMap request of a batch: its grid, source and state.
+struct GridReq {
+  int64_t N, tiles_x, slot_off, nslots;
+  double c0, c1, range;
+  double aal, bbl;
+  int32_t jmax, tw;             // tw: log2 of the bundle tile width (tile 2^tw x 2^(6-tw))
+  double* y;
+  double* A;
+  RayState st;
+};
+
 // ------------------------------------------------------------- host objects
 struct Plane;
 
@@ -189,13 +200,19 @@ struct FlatLCDM;
 // kernel launchers (shoot.hip)
 void launch_plane_step(int precision, bool count, uint32_t nitems, hipStream_t st,
                        const StepJob* jobs, const WorkItem* items, uint32_t* err);
+void launch_zero_u32(hipStream_t st, uint32_t* p, int n);
+void launch_plane_step_queue(int precision, unsigned nblocks, uint32_t nitems, hipStream_t st,
+                             const StepJob* jobs, const WorkItem* items, uint32_t* counter,
+                             uint32_t* err);
 void launch_init_grid(hipStream_t st, const RayState& s, int64_t nslots, int64_t N,
-                      int64_t tiles_x, double c0, double c1, double range, int32_t jmax,
+                      int64_t tiles_x, int tw, double c0, double c1, double range, int32_t jmax,
                       double aal, double bbl);
 void launch_finalize_grid(hipStream_t st, const RayState& s, int64_t nslots, int64_t N,
-                          int64_t tiles_x, double* y, double* A);
+                          int64_t tiles_x, int tw, double* y, double* A);
 void launch_counts_grid(hipStream_t st, const uint32_t* c, int64_t nslots, int64_t N,
-                        int64_t tiles_x, uint64_t* out);
+                        int64_t tiles_x, int tw, uint64_t* out);
+void launch_init_batch(hipStream_t st, const GridReq* reqs, int nreq, int64_t total);
+void launch_finalize_batch(hipStream_t st, const GridReq* reqs, int nreq, int64_t total);
 void launch_init_array(hipStream_t st, const RayState& s, int64_t nslots, int64_t n,
                        const uint32_t* perm, const double* x, const double* zs, int32_t jmax,
                        double aal, double bbl, const double* zpl, const double* Dl,
@@ -214,4 +231,8 @@ template <typename T>
 int build_plane(slsim_ctx ctx, const T* x, const T* y, const float* m, const float* h, size_t n,
                 slsim_plane pl);
 int export_nodes(slsim_plane pl, double* dbl, int64_t* ints);
+// lens helpers (api.hip)
+void uniform_source(const slsim_lens_s* L, double zs, int32_t* jmax, double* aal, double* bbl);
+void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint32_t* counters,
+                std::vector<StepJob>& steps);
 }  // namespace slsim
