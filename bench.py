@@ -44,7 +44,7 @@ def rel_err(d, ref):
     nr = np.linalg.norm(ref, axis=1)
     floor = 1e-3 * np.sqrt(np.mean(nr ** 2))
     e = np.linalg.norm(d, axis=1) / np.maximum(nr, floor if floor > 0 else 1e-300)
-    return float(e.max()), float(np.sqrt(np.mean(e ** 2)))
+    return float(e.max()), float(np.sqrt(np.mean(e ** 2))), float(np.quantile(e, 0.999))
 
 
 def fields(x, y, A):
@@ -52,14 +52,164 @@ def fields(x, y, A):
             np.stack([0.5 * (A[:, 0] - A[:, 3]), 0.5 * (A[:, 1] + A[:, 2])], 1))
 
 
-def compare(x, y, A, yr, Ar):
+def compare(x, y, A, yr, Ar, xr=None):
+    """Relative differences per ray (|d| / max(|ref|, 1e-3 rms|ref|)): max, rms
+    and 99.9th percentile; xr = the reference rays' image positions if they differ."""
     a, k, g = fields(x, y, A)
-    ar, kr, gr = fields(x, yr, Ar)
-    ea, ra = rel_err(a - ar, ar)
-    ek, rk = rel_err(k - kr, kr)
-    eg, rg = rel_err(g - gr, gr)
-    return {"alpha_max": ea, "alpha_rms": ra, "kappa_max": ek, "kappa_rms": rk,
-            "gamma_max": eg, "gamma_rms": rg}
+    ar, kr, gr = fields(x if xr is None else xr, yr, Ar)
+    out = {}
+    for name, d, r in (("alpha", a - ar, ar), ("kappa", k - kr, kr), ("gamma", g - gr, gr)):
+        mx, rms, p999 = rel_err(d, r)
+        out.update({name + "_max": mx, name + "_rms": rms, name + "_p999": p999})
+    return out
+
+
+def c4_requests(S, W, ctx, n_real, seed0=4000, log=print):
+    """Batched form: n_real independent realizations, each 1-12 planes evenly in
+    D_C over [0.1, min(2, zs - 0.1)], 1e4-1e6 particles per plane (log-uniform,
+    float32 positions, NFW-field generator), grid side from {64..1024}, zs in [1, 4]."""
+    rng = np.random.default_rng(seed0)
+    reqs, nparts = [], 0
+    for r in range(n_real):
+        k = int(rng.integers(1, 13))
+        zs = float(rng.uniform(1.0, 4.0))
+        side = int(rng.choice([64, 128, 256, 512, 1024]))
+        zl = W.planes_evenly_in_Dc(0.1, min(2.0, zs - 0.1), k)
+        lens = S.Lens(ctx, zs)
+        for j, z in enumerate(zl):
+            n = int(np.exp(rng.uniform(np.log(1e4), np.log(1e6))))
+            x, y, m, h = W.field_plane(4, j, z, n, soft=5e-4, realization=r, dtype=np.float32)
+            lens.moveinMainHalo(S.LensHaloParticles(ctx, x, y, m, h, z, theta=0.1))
+            nparts += n
+        reqs.append((lens, side, (0.0, 0.0), W.grid_range(300.0), zs))
+    return reqs, nparts
+
+
+def run_batched(S, W, ctx, n_real, precision):
+    t0 = time.perf_counter()
+    reqs, nparts = c4_requests(S, W, ctx, n_real, log=log)
+    build_s = time.perf_counter() - t0
+    nrays = sum(r[1] ** 2 for r in reqs)
+    b = S.Batch(ctx, reqs, precision)
+    b.set_schedule(tile_log2=3, queue=0, order=0, graph=0)
+    b.run()
+    fixed = min(b.run()["walk_ms"] for _ in range(2))
+    best, base_ms, best_ms = b.autotune(repeats=1)
+    tuned = min(b.run()["walk_ms"] for _ in range(2))
+    out = {"realizations": n_real, "rays": nrays, "particles": nparts,
+           "build_wall_s": build_s, "fixed_schedule_ms": fixed,
+           "fixed_rays_per_s": nrays / (fixed * 1e-3), "autotuned_schedule": best,
+           "autotuned_ms": tuned, "autotuned_rays_per_s": nrays / (tuned * 1e-3),
+           "rays_per_s_incl_build": nrays / (build_s + tuned * 1e-3)}
+    log("batched:", out)
+    return out
+
+
+def run_serving(S, W, ctx, n_models, n_requests, load, precision, seed=5999):
+    """Serving form: resident 10-plane x 2^20-particle lenses; Poisson arrivals
+    of GridMap requests (lens, zs in [1,4], centre in the 5' field, side from
+    {128,...,2048} with p = {.4,.3,.15,.1,.05}, 300"/2048 per pixel)."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    t0 = time.perf_counter()
+    lenses = []
+    for r in range(n_models):
+        lens, _, _ = W.build_field_lens(ctx, config=5, nplanes=10, n=1 << 20, zs=3.0,
+                                        realization=r)
+        lenses.append(lens)
+    build_s = time.perf_counter() - t0
+    res = 300.0 / 2048 * S.arcsecTOradians
+    rng = np.random.default_rng(seed)
+    sides = rng.choice([128, 256, 512, 1024, 2048], size=n_requests,
+                       p=[0.4, 0.3, 0.15, 0.1, 0.05])
+    lid = rng.integers(0, n_models, n_requests)
+    zs = rng.uniform(1.0, 4.0, n_requests)
+    cen = (rng.random((n_requests, 2)) - 0.5) * 300 * S.arcsecTOradians
+    # capacity from a short closed loop, then Poisson arrivals at `load` of it
+    srv = S.Server(ctx, hbm_budget=int(200e9))
+    tc = time.perf_counter()
+    probe = [srv.submit(lenses[int(lid[i])], int(sides[i]), cen[i], res * (sides[i] - 1),
+                        float(zs[i]), precision) for i in range(min(24, n_requests))]
+    for rid in probe:
+        srv.wait(rid, fetch=False)
+    cap_rays = sum(int(sides[i]) ** 2 for i in range(len(probe))) / (time.perf_counter() - tc)
+    mean_rays = float(np.mean(sides.astype(np.float64) ** 2))
+    rate = load * cap_rays / mean_rays
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    lat, infos = [], []
+    lock = threading.Lock()
+
+    def collect(rid, N):
+        _, _, info = srv.wait(rid, N)
+        with lock:
+            infos.append(info)
+
+    pool = ThreadPoolExecutor(max_workers=32)
+    start = time.perf_counter()
+    futs = []
+    for i in range(n_requests):
+        delay = start + arrivals[i] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        N = int(sides[i])
+        rid = srv.submit(lenses[int(lid[i])], N, cen[i], res * (N - 1), float(zs[i]), precision)
+        futs.append(pool.submit(collect, rid, N))
+    for f in futs:
+        f.result()
+    pool.shutdown()
+    lat = np.array([i["t_collect"] - i["t_submit"] for i in infos])
+    t_first = min(i["t_submit"] for i in infos)
+    t_last = max(i["t_collect"] for i in infos)
+    rays = sum(i["rays"] for i in infos)
+    st = srv.stats()
+    srv.close()
+    out = {"models": n_models, "requests": n_requests, "build_wall_s": build_s,
+           "capacity_rays_per_s": cap_rays, "load": load, "rate_per_s": rate,
+           "throughput_rays_per_s": rays / (t_last - t_first),
+           "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+           "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+           "mean_ms": 1e3 * float(lat.mean()), "admitted": st["admitted"],
+           "rejected": st["rejected"], "max_active": st["max_active"],
+           "iterations": st["iterations"], "hbm_budget_bytes": st["budget_bytes"]}
+    log("serving:", out)
+    return out
+
+
+def run_precision_form(S, W, ctx, n=1 << 22, nplanes=20, N=4096):
+    """Precision form: 20 planes x 4,194,304 float32-position particles,
+    opening angle 0.2, 4096^2 rays; FP32C and FP64 trees against the FP64
+    direct sum on a 65,536-ray subsample (every 256th ray in raster order)."""
+    t0 = time.perf_counter()
+    lens, _, _ = W.build_field_lens(ctx, config=3, nplanes=nplanes, n=n, zs=3.0, theta=0.2,
+                                    dtype=np.float32)
+    build_s = time.perf_counter() - t0
+    rg = W.grid_range(300.0)
+    out = {"planes": nplanes, "particles_per_plane": n, "grid": N, "theta": 0.2,
+           "build_wall_s": build_s}
+    res = {}
+    for name, prec in (("fp32c", S.SLSIM_FP32C), ("fp64", S.SLSIM_FP64)):
+        g = S.GridMap(lens, N, (0.0, 0.0), rg, prec, shoot=False)
+        g.shoot()
+        t1 = time.perf_counter()
+        st = g.shoot()
+        ctx.synchronize()
+        dt = time.perf_counter() - t1
+        out[name + "_rays_per_s"] = N * N / dt
+        out[name + "_walk_launch_ms"] = st["walk_ms"] / max(st["n_walk_launches"], 1)
+        res[name] = (g.y(), g.A())
+        g.close()
+    x = S.GridMap(lens, N, (0.0, 0.0), rg, shoot=False).positions()
+    out["fp32c_vs_fp64"] = compare(x, res["fp32c"][0], res["fp32c"][1], res["fp64"][0],
+                                   res["fp64"][1])
+    sub = np.arange(0, N * N, 256)
+    t1 = time.perf_counter()
+    yd, Ad = lens.rayshooterInternal(x[sub], precision=S.SLSIM_DIRECT64)
+    out["direct_sum_seconds"] = time.perf_counter() - t1
+    out["vs_fp64_direct_sum"] = {
+        "fp32c": compare(x[sub], res["fp32c"][0][sub], res["fp32c"][1][sub], yd, Ad),
+        "fp64_tree": compare(x[sub], res["fp64"][0][sub], res["fp64"][1][sub], yd, Ad)}
+    log("precision form:", out)
+    return out
 
 
 def main():
@@ -72,6 +222,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip FP64, direct-sum, counts")
+    ap.add_argument("--precision-form", action="store_true",
+                    help="also run the 20-plane x 2^22-particle, 4096^2-ray precision form")
+    ap.add_argument("--batched", type=int, default=0,
+                    help="also run the batched form with this many realizations")
+    ap.add_argument("--serving-models", type=int, default=0,
+                    help="also run the serving form with this many resident lenses")
+    ap.add_argument("--serving-requests", type=int, default=300)
+    ap.add_argument("--serving-load", type=float, default=0.8)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build's ray shooter runs on a single "
@@ -123,8 +281,8 @@ def main():
                                "zs=3, theta=0.1, bucket 20" % (args.n, N, N),
                    "planes": 10, "particles_per_plane": args.n, "grid": N,
                    "field_arcsec": 300, "z_source": 3.0, "theta": 0.1,
-                   "walk": "FP32 with TwoSum-compensated accumulation; FP64 opening test, "
-                           "cell offsets and plane recursion",
+                   "walk": "FP32 far field with Kahan-compensated accumulation; FP64 opening "
+                           "test, cell offsets, near-field terms and plane recursion",
                    "parallelism": "single GPU"},
         "tree_build": {"device_ms_total": build_ms_dev, "cells_total": nodes},
     }
@@ -185,6 +343,17 @@ def main():
             "fp64_tree": compare(x[sub], y64[sub], A64[sub], yd, Ad)}
         log("vs direct sum:", out["error_vs_fp64_direct_sum"])
 
+        # conditioning of kappa/gamma: the FP64 tree's own response when the
+        # input rays move by 1e-7 of the rms deflection (the size of one FP32
+        # rounding of a plane's deflection), to read fp32c_vs_fp64 against
+        xs = x[sub]
+        amp = 1e-7 * np.sqrt(np.mean(np.sum((xs - y64[sub]) ** 2, 1)))
+        phi = np.random.default_rng(7).uniform(0, 2 * np.pi, xs.shape[0])
+        xn = xs + amp * np.stack([np.cos(phi), np.sin(phi)], 1)
+        yn, An = lens.rayshooterInternal(xn, precision=S.SLSIM_FP64)
+        out["fp64_response_to_1e-7_ray_nudge"] = compare(xn, yn, An, y64[sub], A64[sub], xr=xs)
+        log("fp64 nudge response:", out["fp64_response_to_1e-7_ray_nudge"])
+
     if not args.no_cpu:
         import oracle as O
         nthreads = min(16, os.cpu_count() or 1)
@@ -220,6 +389,14 @@ def main():
         if not args.no_extras:
             out["fp64_vs_cpu_oracle"] = compare(pos[idx], y64[idx], A64[idx], res[0], res[1])
         log("cpu oracle: %.3e rays/s on %d threads" % (cpu_rate, nthreads))
+
+    if args.precision_form:
+        out["precision_form"] = run_precision_form(S, W, ctx)
+    if args.batched:
+        out["batched"] = run_batched(S, W, ctx, args.batched, S.SLSIM_FP32C)
+    if args.serving_models:
+        out["serving"] = run_serving(S, W, ctx, args.serving_models, args.serving_requests,
+                                     args.serving_load, S.SLSIM_FP32C)
 
     print(json.dumps(out), flush=True)
 

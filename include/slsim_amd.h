@@ -156,6 +156,13 @@ int slsim_gridmap_shoot(slsim_lens lens, size_t N, double c0, double c1, double 
 int slsim_gridmap_count(slsim_lens lens, size_t N, double c0, double c1, double range,
                         double zs, uint64_t* cnt_host);
 
+/* Bundle utilisation of a grid shoot (64-ray wavefront bundles, 8x8 tiles):
+ * out4 = { lane cell evaluations the rays need, wavefront cell evaluations,
+ *          leaf-particle interactions the rays need, wavefront leaf-particle
+ *          iterations }; utilisation = need / (64 * wavefront). */
+int slsim_gridmap_bundle_stats(slsim_lens lens, size_t N, double c0, double c1, double range,
+                               double zs, uint64_t* out4);
+
 /* FP64 direct sum of the smoothed-particle kernel of one plane (no tree), in
  * the tree's sign convention: out[5 i] = alpha0, alpha1, kappa, gamma0, gamma1
  * at physical positions rays[2 i] (host buffers).  Error column oracle. */

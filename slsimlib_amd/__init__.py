@@ -269,6 +269,17 @@ class GridMap:
                                        _ptr(out, ctypes.c_uint64)))
         return out
 
+    def bundle_stats(self):
+        """Lane utilisation of the 64-ray bundles (visits, leaf particles)."""
+        out = np.zeros(4, np.uint64)
+        _check(lib.slsim_gridmap_bundle_stats(self.lens.h, ctypes.c_size_t(self.N), self.center[0],
+                                              self.center[1], self.range, self.zs,
+                                              _ptr(out, ctypes.c_uint64)))
+        v, wv, p, wp = (float(t) for t in out)
+        return {"visits": v, "wave_visits": wv, "visit_utilisation": v / max(64 * wv, 1),
+                "leaf_particles": p, "wave_leaf_iterations": wp,
+                "leaf_utilisation": p / max(64 * wp, 1)}
+
     def writePixelMap(self, var):
         """LensingVariable maps of gridmap.h:533-603 as (N, N) arrays."""
         A = self.A()

@@ -32,7 +32,7 @@ class SlsimError(RuntimeError):
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
-        "libslsim_amd.so is not built (%s); run `python -m slsimlib_amd.build` or "
+        "libslsim_amd.so is not built (%s); run `python slsimlib_amd/build.py` or "
         "__graft_entry__.build()" % LIB_PATH)
 
 lib = ctypes.CDLL(LIB_PATH)
@@ -111,6 +111,8 @@ SIGNATURES = {
                                     P(c_double), P(c_double), c_int, P(slsim_shoot_stats)]),
     "slsim_gridmap_count": (c_int, [slsim_lens, c_size_t, c_double, c_double, c_double, c_double,
                                     P(c_u64)]),
+    "slsim_gridmap_bundle_stats": (c_int, [slsim_lens, c_size_t, c_double, c_double, c_double,
+                                           c_double, P(c_u64)]),
     "slsim_plane_direct_sum": (c_int, [slsim_plane, c_size_t, P(c_double), P(c_double)]),
     "slsim_plane_force": (c_int, [slsim_plane, c_size_t, P(c_double), P(c_double), c_int]),
     "slsim_batch_create": (c_int, [slsim_ctx, c_int, P(slsim_lens), P(c_i64), P(c_double),

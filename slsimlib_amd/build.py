@@ -1,6 +1,7 @@
 """Builds libslsim_amd.so in-tree with hipcc for gfx950.
 
-Usage: python -m slsimlib_amd.build   (or slsimlib_amd.build.build())
+Usage: python slsimlib_amd/build.py  (run by path: importing the package
+first would load the library that is being rebuilt)
 """
 import os
 import subprocess
@@ -59,6 +60,22 @@ def build(verbose=False, jobs=4):
     return LIB
 
 
+def build_cpp_example(verbose=False):
+    """Compiles tests/cpp/dropin_gridmap.cpp, a reference-style C++ program
+    against include/glamer_amd.h linked with the library."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "dropin_gridmap.cpp")
+    exe = os.path.join(root, "tests", "cpp", "dropin_gridmap")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "include"), src,
+           "-L", HERE, "-lslsim_amd", "-Wl,-rpath,$ORIGIN/../../slsimlib_amd", "-o", exe]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if out.returncode != 0:
+        raise RuntimeError("C++ example build failed:\n" + out.stdout.decode(errors="replace"))
+    return exe
+
+
 def _wait(item):
     src, p = item
     out, _ = p.communicate()
@@ -68,3 +85,4 @@ def _wait(item):
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_cpp_example(verbose="-v" in sys.argv))

@@ -54,13 +54,11 @@ struct Scratch {
 };
 
 int alloc_state(Scratch& s, int64_t nslots, RayState* st) {
-  double* d;
-  int rc = s.get(&d, 14 * (size_t)nslots);
+  char* d;
+  int rc = s.get(&d, kStateBytesPerSlot * (size_t)nslots);
   if (rc) return rc;
-  double** f[14] = {&st->th0, &st->th1, &st->sa0, &st->sa1, &st->A0, &st->A1, &st->A2,
-                    &st->A3,  &st->G0,  &st->G1,  &st->G2,  &st->G3, &st->aal, &st->bbl};
-  for (int k = 0; k < 14; ++k) *f[k] = d + (size_t)k * nslots;
-  return s.get(&st->jmax, nslots);
+  *st = state_view(reinterpret_cast<double*>(d), nslots);
+  return SLSIM_OK;
 }
 
 double now_ms() {
@@ -426,7 +424,7 @@ void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint3
   for (int j = 0; j < nplanes; ++j) {
     StepJob& s = steps[j];
     std::memset(&s, 0, sizeof(s));
-    s.st = st;
+    s.sbase = st.th0;
     s.nslots = (int32_t)nslots;
     s.j = j;
     s.ntrees = L->tree_cnt[j];
@@ -487,7 +485,7 @@ int check_err(slsim_ctx c, uint32_t* err) {
 
 int gridmap_run(slsim_lens L, size_t N, double c0, double c1, double range, double zs,
                 double* y_dev, double* A_dev, int precision, uint64_t* cnt_host,
-                slsim_shoot_stats* stats) {
+                slsim_shoot_stats* stats, uint64_t* bundle_stats = nullptr) {
   const double t0 = now_ms();
   if (!L) return arg_error("null lens");
   if (N < 1) return arg_error("GridMap size is < 1!");
@@ -508,8 +506,8 @@ int gridmap_run(slsim_lens L, size_t N, double c0, double c1, double range, doub
   SLSIM_HIP_CHECK(hipMemsetAsync(err, 0, 4, c->stream));
   uint32_t* counters = nullptr;
   if (cnt_host) {
-    if ((rc = s.get(&counters, 5 * (size_t)nslots))) return rc;
-    SLSIM_HIP_CHECK(hipMemsetAsync(counters, 0, 20 * (size_t)nslots, c->stream));
+    if ((rc = s.get(&counters, kCountStride * (size_t)nslots))) return rc;
+    SLSIM_HIP_CHECK(hipMemsetAsync(counters, 0, 4 * kCountStride * (size_t)nslots, c->stream));
   }
   int32_t jmax = 0;
   double aal = 0, bbl = 0;
@@ -534,6 +532,21 @@ int gridmap_run(slsim_lens L, size_t N, double c0, double c1, double range, doub
     if ((rc = s.get(&dcnt, 5 * N * N))) return rc;
     launch_counts_grid(c->stream, counters, nslots, (int64_t)N, tiles_x, 3, dcnt);
     SLSIM_HIP_CHECK(hipMemcpyAsync(cnt_host, dcnt, 40 * N * N, hipMemcpyDeviceToHost, c->stream));
+    if (bundle_stats) {
+      std::vector<uint32_t> hc(kCountStride * (size_t)nslots);
+      SLSIM_HIP_CHECK(hipMemcpyAsync(hc.data(), counters, 4 * hc.size(), hipMemcpyDeviceToHost,
+                                     c->stream));
+      SLSIM_HIP_CHECK(hipStreamSynchronize(c->stream));
+      uint64_t t[4] = {0, 0, 0, 0};
+      for (int64_t k = 0; k < nslots; ++k) {
+        const uint32_t* q = hc.data() + kCountStride * (size_t)k;
+        t[0] += q[0];
+        t[1] += q[5];
+        t[2] += q[2];
+        t[3] += q[6];
+      }
+      for (int k = 0; k < 4; ++k) bundle_stats[k] = t[k];
+    }
   }
   SLSIM_HIP_CHECK(hipGetLastError());
   if ((rc = check_err(c, err))) return rc;
@@ -637,6 +650,14 @@ int slsim_gridmap_count(slsim_lens L, size_t N, double c0, double c1, double ran
                         uint64_t* cnt_host) {
   if (!cnt_host) return arg_error("null counter buffer");
   return gridmap_run(L, N, c0, c1, range, zs, nullptr, nullptr, SLSIM_FP64, cnt_host, nullptr);
+}
+
+int slsim_gridmap_bundle_stats(slsim_lens L, size_t N, double c0, double c1, double range,
+                               double zs, uint64_t* out4) {
+  if (!out4) return arg_error("null output");
+  std::vector<uint64_t> cnt(5 * N * N);
+  return gridmap_run(L, N, c0, c1, range, zs, nullptr, nullptr, SLSIM_FP64, cnt.data(), nullptr,
+                     out4);
 }
 
 int slsim_shoot_device(slsim_lens L, size_t n, const double* x_dev, const double* zs_dev,

@@ -57,12 +57,7 @@ struct DevBlock {
   ~DevBlock() { release(); }
 };
 
-void carve_state(double* d, int64_t nslots, RayState* st) {
-  double** f[14] = {&st->th0, &st->th1, &st->sa0, &st->sa1, &st->A0, &st->A1, &st->A2,
-                    &st->A3,  &st->G0,  &st->G1,  &st->G2,  &st->G3, &st->aal, &st->bbl};
-  for (int k = 0; k < 14; ++k) *f[k] = d + (size_t)k * nslots;
-  st->jmax = reinterpret_cast<int32_t*>(d + 14 * (size_t)nslots);
-}
+void carve_state(double* d, int64_t nslots, RayState* st) { *st = state_view(d, nslots); }
 
 size_t request_bytes(int64_t nslots, int64_t N) {
   return kStateBytesPerSlot * (size_t)nslots + 48 * (size_t)N * (size_t)N + 256;

@@ -73,7 +73,9 @@ __device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
 }
 
 struct Counts {
-  uint32_t v = 0, c = 0, p = 0, b = 0, s = 0;
+  uint32_t v = 0, c = 0, p = 0, b = 0, s = 0;   // this ray's work (reference criterion)
+  uint32_t wv = 0, wp = 0;                       // the wavefront's cell evaluations and
+                                                 // leaf-particle iterations
 };
 
 // b_spline_profile (include/quadTree.h:147-190), FP64, phi omitted.
@@ -108,146 +110,201 @@ struct Acc64 {
   double ax = 0, ay = 0, k = 0, g0 = 0, g1 = 0;
 };
 
+// Traversal shared by both walks.  The stack holds opened internal cells as
+// (first child | non-empty-children mask << 28, live mask of the lanes that
+// opened it).  Popping an entry evaluates its four contiguous children for
+// those lanes: a lane for which a child is far takes its multipole, a lane
+// that opens a leaf takes its particles, and internal children opened by
+// any lane are pushed (child3 first, so child0's subtree is walked first).
+// Each lane therefore sees the cells of its own per-ray criterion, in an
+// order fixed by its own open set.
+__device__ inline void push_entry(int32_t* s_node, uint64_t* s_mask, int& sp, int lane,
+                                  int32_t entry, uint64_t mask, uint32_t* err) {
+  if (sp >= kStackDepth) {
+    if (lane == 0) atomicOr(err, 1u);
+    return;
+  }
+  if (lane == 0) {
+    s_node[sp] = entry;
+    s_mask[sp] = mask;
+  }
+  ++sp;
+}
+
+// ---------------------------------------------------------------- FP64 walk
+// Evaluates one cell for the lanes with lv set (include/quadTree.h:990-1128):
+// returns the mask of lanes that opened it when it is internal, 0 otherwise.
+template <bool COUNT>
+__device__ __forceinline__ uint64_t eval64(const NodeD& nd, bool lv, double rx, double ry,
+                                           double ia, cptr<PartD> parts, cptr<PartD> bigs,
+                                           Acc64& a, Counts& cnt) {
+  const double dx = nd.cx - rx, dy = nd.cy - ry;
+  const double r2 = dx * dx + dy * dy;
+  const bool nearc = lv && (r2 < nd.near2);
+  const bool open = lv && ((r2 < nd.r2crit) || nearc);
+  if (COUNT) cnt.wv++;
+  if (COUNT && lv) cnt.v++;
+  if (lv && !open) {
+    // far cell: monopole + quadrupole (include/quadTree.h:1089-1123)
+    if (COUNT) cnt.c++;
+    const double inv = rcp64(r2);
+    const double mi = nd.mass * inv * kInvPI;
+    double tmp = nd.mass * ia - mi;
+    a.ax += tmp * dx;
+    a.ay += tmp * dy;
+    tmp = -2.0 * mi * inv;
+    a.g0 += 0.5 * (dx * dx - dy * dy) * tmp;
+    a.g1 += dx * dy * tmp;
+    const double i2p = inv * inv * kInvPI;
+    a.ax -= (nd.q0 * dx + nd.q2 * dy) * i2p;
+    a.ay -= (nd.q1 * dy + nd.q2 * dx) * i2p;
+    tmp = 4 * (nd.q0 * dx * dx + nd.q1 * dy * dy + 2 * nd.q2 * dx * dy) * inv * i2p;
+    a.ax += tmp * dx;
+    a.ay += tmp * dy;
+    a.k -= ia * nd.mass;
+  }
+  const uint64_t om = ballot(open);
+  if (om == 0) return 0;
+  if (nd.child < 0) {
+    // leaf: every particle as a point mass (include/quadTree.h:1014-1048)
+    for (int k = 0; k < nd.count; ++k) {
+      const PartD p = parts[nd.start + k];
+      if (open) {
+        const double d0 = p.x - rx, d1 = p.y - ry;
+        double rr = d0 * d0 + d1 * d1;
+        if (rr < 1e-20) rr = 1e-20;
+        const double m = p.m;
+        const double inv = rcp64(rr);
+        const double pref = m * inv * kInvPI;
+        a.ax += (ia * m - pref) * d0;
+        a.ay += (ia * m - pref) * d1;
+        const double tmp = -2.0 * pref * inv;
+        a.g0 += 0.5 * (d0 * d0 - d1 * d1) * tmp;
+        a.g1 += d0 * d1 * tmp;
+        a.k -= m * ia;
+      }
+    }
+    if (COUNT && open) cnt.p += nd.count;
+    if (COUNT) cnt.wp += nd.count;
+    if (ballot(nearc)) {
+      // the leaf's big particles: h < box (include/quadTree.h:379-396, 1051-1077)
+      for (int k = 0; k < nd.count; ++k) {
+        const PartD p = parts[nd.start + k];
+        if (nearc && !signbit(p.h)) {
+          if (COUNT) cnt.b++;
+          const double d0 = p.x - rx, d1 = p.y - ry;
+          double rr = d0 * d0 + d1 * d1;
+          if (rr < 1e-20) rr = 1e-20;
+          const double size = p.h;
+          if (rr < 4 * size * size) {
+            if (COUNT) cnt.s++;
+            bspline64(d0, d1, sqrt(rr), (double)p.m, size, a.ax, a.ay, a.k, a.g0, a.g1);
+          }
+        }
+      }
+    }
+    return 0;
+  }
+  if (nd.count > 0 && ballot(nearc)) {
+    for (int k = 0; k < nd.count; ++k) {
+      const PartD p = bigs[nd.start + k];
+      if (nearc) {
+        if (COUNT) cnt.b++;
+        const double d0 = p.x - rx, d1 = p.y - ry;
+        double rr = d0 * d0 + d1 * d1;
+        if (rr < 1e-20) rr = 1e-20;
+        const double size = p.h;
+        if (rr < 4 * size * size) {
+          if (COUNT) cnt.s++;
+          bspline64(d0, d1, sqrt(rr), (double)p.m, size, a.ax, a.ay, a.k, a.g0, a.g1);
+        }
+      }
+    }
+  }
+  return om;
+}
+
 // Walk of one tree, FP64 (plain method).  Must be called by all 64 lanes.
 template <bool COUNT>
 __device__ void walk64(const TreeDesc& t, double rx, double ry, bool active, Acc64& a,
                        Counts& cnt, int32_t* s_node, uint64_t* s_mask, uint32_t* err) {
   const int lane = lane_id();
-  const uint64_t amask = ballot(active);
-  if (amask == 0) return;
+  if (ballot(active) == 0) return;
   const cptr<NodeD> nodes = as_const(t.nodesD);
   const cptr<PartD> parts = as_const(t.partsD);
   const cptr<PartD> bigs = as_const(t.bigD);
   const double ia = t.inv_area;
-  if (lane == 0) { s_node[0] = 0; s_mask[0] = amask; }
-  int sp = 1;
+  int sp = 0;
+  {
+    const NodeD root = nodes[0];
+    const uint64_t om = eval64<COUNT>(root, active, rx, ry, ia, parts, bigs, a, cnt);
+    if (om) push_entry(s_node, s_mask, sp, lane, root.child | (root.cmask << 28), om, err);
+  }
   while (sp > 0) {
     --sp;
     __builtin_amdgcn_wave_barrier();
-    const int ni = uni(s_node[sp]);
+    const int entry = uni(s_node[sp]);
     const uint64_t live = uni64(s_mask[sp]);
-    const NodeD nd = nodes[ni];
+    __builtin_amdgcn_wave_barrier();
     const bool lv = (live >> lane) & 1ull;
-    const double dx = nd.cx - rx, dy = nd.cy - ry;
-    const double r2 = dx * dx + dy * dy;
-    const bool nearc = lv && (r2 < nd.near2);
-    const bool open = lv && ((r2 < nd.r2crit) || nearc);
-    if (COUNT && lv) cnt.v++;
-    if (lv && !open) {
-      // far cell: monopole + quadrupole (include/quadTree.h:1089-1123)
-      if (COUNT) cnt.c++;
-      const double inv = rcp64(r2);
-      const double mi = nd.mass * inv * kInvPI;
-      double tmp = nd.mass * ia - mi;
-      a.ax += tmp * dx;
-      a.ay += tmp * dy;
-      tmp = -2.0 * mi * inv;
-      a.g0 += 0.5 * (dx * dx - dy * dy) * tmp;
-      a.g1 += dx * dy * tmp;
-      const double i2p = inv * inv * kInvPI;
-      a.ax -= (nd.q0 * dx + nd.q2 * dy) * i2p;
-      a.ay -= (nd.q1 * dy + nd.q2 * dx) * i2p;
-      tmp = 4 * (nd.q0 * dx * dx + nd.q1 * dy * dy + 2 * nd.q2 * dx * dy) * inv * i2p;
-      a.ax += tmp * dx;
-      a.ay += tmp * dy;
-      a.k -= ia * nd.mass;
-    }
-    const uint64_t om = ballot(open);
-    if (om == 0) continue;
-    if (nd.child < 0) {
-      // leaf: every particle as a point mass (include/quadTree.h:1014-1048)
-      for (int k = 0; k < nd.count; ++k) {
-        const PartD p = parts[nd.start + k];
-        if (open) {
-          const double d0 = p.x - rx, d1 = p.y - ry;
-          double rr = d0 * d0 + d1 * d1;
-          if (rr < 1e-20) rr = 1e-20;
-          const double m = p.m;
-          const double inv = rcp64(rr);
-          const double pref = m * inv * kInvPI;
-          a.ax += (ia * m - pref) * d0;
-          a.ay += (ia * m - pref) * d1;
-          const double tmp = -2.0 * pref * inv;
-          a.g0 += 0.5 * (d0 * d0 - d1 * d1) * tmp;
-          a.g1 += d0 * d1 * tmp;
-          a.k -= m * ia;
-        }
-      }
-      if (COUNT && open) cnt.p += nd.count;
-      if (ballot(nearc)) {
-        // the leaf's big particles: h < box (include/quadTree.h:379-396, 1051-1077)
-        for (int k = 0; k < nd.count; ++k) {
-          const PartD p = parts[nd.start + k];
-          if (nearc && !signbit(p.h)) {
-            if (COUNT) cnt.b++;
-            const double d0 = p.x - rx, d1 = p.y - ry;
-            double rr = d0 * d0 + d1 * d1;
-            if (rr < 1e-20) rr = 1e-20;
-            const double size = p.h;
-            if (rr < 4 * size * size) {
-              if (COUNT) cnt.s++;
-              bspline64(d0, d1, sqrt(rr), (double)p.m, size, a.ax, a.ay, a.k, a.g0, a.g1);
-            }
-          }
-        }
-      }
-    } else {
-      if (nd.count > 0 && ballot(nearc)) {
-        for (int k = 0; k < nd.count; ++k) {
-          const PartD p = bigs[nd.start + k];
-          if (nearc) {
-            if (COUNT) cnt.b++;
-            const double d0 = p.x - rx, d1 = p.y - ry;
-            double rr = d0 * d0 + d1 * d1;
-            if (rr < 1e-20) rr = 1e-20;
-            const double size = p.h;
-            if (rr < 4 * size * size) {
-              if (COUNT) cnt.s++;
-              bspline64(d0, d1, sqrt(rr), (double)p.m, size, a.ax, a.ay, a.k, a.g0, a.g1);
-            }
-          }
-        }
-      }
-      if (sp + 4 > kStackDepth) {
-        if (lane == 0) atomicOr(err, 1u);
-        break;
-      }
-      __builtin_amdgcn_wave_barrier();
-      for (int c = 3; c >= 0; --c) {
-        if ((nd.cmask >> c) & 1) {
-          if (lane == 0) { s_node[sp] = nd.child + c; s_mask[sp] = om; }
-          ++sp;
-        }
+    const int base = entry & 0x0fffffff;
+    const int cm = (entry >> 28) & 0xf;
+    int32_t pe[4];
+    uint64_t pm[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      pm[k] = 0;
+      pe[k] = 0;
+      if ((cm >> k) & 1) {
+        const NodeD nd = nodes[base + k];
+        pm[k] = eval64<COUNT>(nd, lv, rx, ry, ia, parts, bigs, a, cnt);
+        pe[k] = nd.child | (nd.cmask << 28);
       }
     }
+#pragma unroll
+    for (int k = 3; k >= 0; --k)
+      if (pm[k]) push_entry(s_node, s_mask, sp, lane, pe[k], pm[k], err);
   }
 }
 
 // ---------------------------------------------------------------- FP32 walk
-struct CSum {   // compensated FP32 sum (TwoSum)
-  float s = 0.f, c = 0.f;
+struct CSum {   // Kahan-compensated FP32 sum
+  float s = 0.f, c = 0.f;   // c: the running compensation (lost low part, negated)
   __device__ inline void add(float x) {
-    const float t = s + x;
-    const float z = t - s;
-    c += (s - (t - z)) + (x - z);
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
     s = t;
   }
-  __device__ inline double value() const { return (double)s + (double)c; }
+  __device__ inline double value() const { return (double)s - (double)c; }
 };
+
+// Near-field interactions deferred to FP64 (per lane, in LDS): a particle
+// index, with the top bit set for an internal big-particle list entry.
+constexpr int kNearQueue = 8;
+constexpr uint32_t kBigListBit = 0x80000000u;
 
 struct Acc32 {
   CSum ax, ay, k, g0, g1;
-  float pax = 0.f, pay = 0.f, pk = 0.f, pg0 = 0.f, pg1 = 0.f;   // plain partials
+  float pax = 0.f, pay = 0.f, pk = 0.f, pg0 = 0.f, pg1 = 0.f;   // plain partials of a leaf
+  uint32_t* q = nullptr;   // this lane's queue: q[0], q[64], ... q[64 * (kNearQueue - 1)]
+  int nq = 0;
+  double fx = 0, fy = 0, fk = 0, fg0 = 0, fg1 = 0;   // FP64 sums of the queued terms
   __device__ inline void flush() {
     ax.add(pax); ay.add(pay); k.add(pk); g0.add(pg0); g1.add(pg1);
     pax = pay = pk = pg0 = pg1 = 0.f;
   }
+  __device__ inline bool push(uint32_t idx) {
+    if (nq >= kNearQueue) return false;
+    q[64 * nq] = idx;
+    ++nq;
+    return true;
+  }
 };
 
-// Smoothed particle (point mass with the reference's B-spline correction
-// folded in, algebraically identical to the two-step FP64 form):
-//   alpha += (ia m - s) d,  gamma -= (s - m sigma) (c2, s2),  kappa += m sigma - m ia
-// with s = m M(q) / (pi r^2).
+// M(q) / r^2 and sigma(q) of the reference's cubic B-spline
+// (include/quadTree.h:147-190, M in the reference's form), written so that
+// nothing cancels at small q.
 __device__ inline void bspline_fused32(float r2, float inv, float h, float& Mr2, float& sig) {
   const float r = sqrtf(r2);
   const float ih = rcp32(h);
@@ -269,136 +326,243 @@ __device__ inline void bspline_fused32(float r2, float inv, float h, float& Mr2,
   }
 }
 
-// Walk of one tree, FP32 arithmetic with compensated accumulation.  Cell
-// centres relative to the ray and the opening test are FP64.
+// Smoothed particle in FP32: the point mass with the reference's B-spline
+// correction folded in (algebraically the reference's two-step form),
+//   alpha += (ia m - s) d,  gamma -= (s - m sigma) (c2, s2),  kappa += m sigma - m ia,
+// s = m M(q) / (pi r^2): of order m / h^2, nothing large cancels.
+__device__ inline void smooth32(float d0, float d1, float rr, float m, float h, float ia,
+                                Acc32& a) {
+  const float inv = rcp32(rr);
+  float Mr2, sig;
+  bspline_fused32(rr, inv, h, Mr2, sig);
+  const float s = m * Mr2 * kInvPIf;
+  const float sm = m * sig;
+  const float c2 = (d0 * d0 - d1 * d1) * inv;
+  const float s2 = 2.f * d0 * d1 * inv;
+  const float mono = ia * m - s;
+  a.pax += mono * d0;
+  a.pay += mono * d1;
+  a.pg0 -= (s - sm) * c2;
+  a.pg1 -= (s - sm) * s2;
+  a.pk += sm - m * ia;
+}
+
+// Point mass in FP32 (include/quadTree.h:1019-1046) for a particle at least
+// 2h from the ray: alpha += (ia m - s) d, gamma -= s (c2, s2), kappa -= ia m,
+// s = m / (pi r^2).
+__device__ inline void point32(float d0, float d1, float rr, float m, float ia, Acc32& a) {
+  const float inv = rcp32(rr);
+  const float s = m * inv * kInvPIf;
+  const float c2 = (d0 * d0 - d1 * d1) * inv;
+  const float s2 = 2.f * d0 * d1 * inv;
+  const float mono = ia * m - s;
+  a.pax += mono * d0;
+  a.pay += mono * d1;
+  a.pg0 -= s * c2;
+  a.pg1 -= s * s2;
+  a.pk -= m * ia;
+}
+
+// The queued near-field terms, after the walk (its registers are free then),
+// in FP64 with the reference's arithmetic: a bare point mass within 2h of the
+// ray (its cell is not near, or does not list it as big, so the reference
+// applies no B-spline) and the B-spline corrections of internal-cell lists
+// (which cancel a point mass taken at the particle's leaf).  These terms grow
+// as 1/r^2 near a particle; FP64 keeps them and their cancellation exact.
+__device__ void near_terms64(const cptr<PartD> parts, const cptr<PartD> bigs, double rx,
+                             double ry, double ia, Acc32& a) {
+  for (int i = 0; i < a.nq; ++i) {
+    const uint32_t idx = a.q[64 * i];
+    const bool big = (idx & kBigListBit) != 0;
+    const PartD p = big ? bigs[idx & ~kBigListBit] : parts[idx];
+    const double d0 = p.x - rx, d1 = p.y - ry;
+    double rr = d0 * d0 + d1 * d1;
+    if (rr < 1e-20) rr = 1e-20;
+    const double m = p.m;
+    if (!big) {
+      const double inv = rcp64(rr);
+      const double pref = m * inv * kInvPI;
+      const double tmp = -2.0 * pref * inv;
+      a.fx += (ia * m - pref) * d0;
+      a.fy += (ia * m - pref) * d1;
+      a.fk -= m * ia;
+      a.fg0 += 0.5 * (d0 * d0 - d1 * d1) * tmp;
+      a.fg1 += d0 * d1 * tmp;
+    } else {
+      const double size = p.h;
+      if (rr < 4 * size * size)
+        bspline64(d0, d1, sqrt(rr), m, size, a.fx, a.fy, a.fk, a.fg0, a.fg1);
+    }
+  }
+  a.nq = 0;
+}
+
+// B-spline correction in FP32 (the reference's two-step form), used only
+// when a lane's near-field queue is full.
+__device__ inline void correction32(float d0, float d1, float rr, float m, float hh, Acc32& a) {
+  const float r = sqrtf(rr);
+  const float q = r / hh;
+  float M, sig;
+  const float q2 = q * q, q3 = q2 * q;
+  if (q > 2.f) {
+    M = 1.f; sig = 0.f;
+  } else if (q > 1.f) {
+    const float tq = 2.f - q;
+    sig = (10.f / 28.f) * kInvPIf * tq * tq * tq / (hh * hh);
+    M = (-1.f + 20.f * q2 * (1.f - q + 0.375f * q2 - 0.05f * q3)) / 7.f;
+  } else {
+    sig = (10.f / 7.f) * kInvPIf * (1.f - 1.5f * q2 + 0.75f * q3) / (hh * hh);
+    M = 10.f * q2 * (1.f - 0.75f * q + 0.3f * q3) / 7.f;
+  }
+  const float ar = (M - 1.f) * kInvPIf / r;
+  const float gt = ar / r - sig;
+  const float inv = rcp32(rr);
+  a.pax -= m * ar * d0 / r;
+  a.pay -= m * ar * d1 / r;
+  a.pg0 -= gt * m * (d0 * d0 - d1 * d1) * inv;
+  a.pg1 -= gt * m * 2.f * d0 * d1 * inv;
+  a.pk += m * sig;
+}
+
+// The rest of a cell evaluation once the opening test is known.
+__device__ __forceinline__ uint64_t eval32_rest(const NodeF& nd, bool lv, bool nearc, bool open,
+                                                float ex, float ey, float r2f, double rx,
+                                                double ry, float ia, cptr<PartD> parts,
+                                                cptr<PartD> bigs, Acc32& a) {
+  const int count = nd.count_cmask & 0x0fffffff;
+  if (lv && !open) {
+    // far cell: monopole + quadrupole, one compensated add per component
+    const float inv = rcp32(r2f);
+    const float s = nd.mass * inv * kInvPIf;
+    const float c2 = (ex * ex - ey * ey) * inv;
+    const float s2 = 2.f * ex * ey * inv;
+    const float qx0 = nd.q0 * ex + nd.q2 * ey;
+    const float qx1 = nd.q1 * ey + nd.q2 * ex;
+    const float qq = ex * qx0 + ey * qx1;
+    const float ip = inv * inv * kInvPIf;
+    const float t4 = 4.f * qq * inv;
+    const float mono = nd.mass * ia - s;
+    a.ax.add(mono * ex + ip * (t4 * ex - qx0));
+    a.ay.add(mono * ey + ip * (t4 * ey - qx1));
+    a.g0.add(-s * c2);
+    a.g1.add(-s * s2);
+    a.pk -= ia * nd.mass;
+  }
+  const uint64_t om = ballot(open);
+  if (om == 0) return 0;
+  if (nd.child < 0) {
+    // leaf particles; particle - ray offsets formed in FP64
+    for (int k = 0; k < count; ++k) {
+      const PartD p = parts[nd.start + k];
+      if (open) {
+        const float d0 = (float)(p.x - rx), d1 = (float)(p.y - ry);
+        const float rr = fmaxf(d0 * d0 + d1 * d1, 1e-20f);
+        const float hs = fabsf(p.h);
+        if (rr < 4.f * hs * hs && nearc && !signbit(p.h))
+          smooth32(d0, d1, rr, p.m, hs, ia, a);
+        else if (!(rr < 4.f * hs * hs) || !a.push((uint32_t)(nd.start + k)))
+          point32(d0, d1, rr, p.m, ia, a);
+      }
+    }
+    // flushed only by lanes that took the leaf: a lane's partial sums must
+    // not depend on which cells its bundle mates open
+    if (open) a.flush();
+    return 0;
+  }
+  if (count > 0 && ballot(nearc)) {
+    // the cell's own big particles: B-spline corrections (near field, FP64)
+    for (int k = 0; k < count; ++k) {
+      const PartD p = bigs[nd.start + k];
+      if (nearc) {
+        const float d0 = (float)(p.x - rx), d1 = (float)(p.y - ry);
+        const float rr = fmaxf(d0 * d0 + d1 * d1, 1e-20f);
+        if (rr < 4.0001f * p.h * p.h && !a.push((uint32_t)(nd.start + k) | kBigListBit) &&
+            rr < 4.f * p.h * p.h)
+          correction32(d0, d1, rr, p.m, p.h, a);
+      }
+    }
+    if (nearc) a.flush();
+  }
+  return om;
+}
+
+// FP32 cell evaluation: cell centres relative to the ray and the opening
+// test are FP64, so both walks open the same cells; multipoles and distant
+// particles are FP32 with compensated sums; near-field terms are queued for
+// FP64 (near_terms64).
+__device__ __forceinline__ uint64_t eval32(const NodeF& nd, bool lv, double rx, double ry,
+                                           float ia, cptr<PartD> parts, cptr<PartD> bigs,
+                                           Acc32& a) {
+  const double dx = nd.cx - rx, dy = nd.cy - ry;
+  const double r2 = dx * dx + dy * dy;
+  const bool nearc = lv && (r2 < nd.near2);
+  const bool open = lv && ((r2 < nd.r2crit) || nearc);
+  return eval32_rest(nd, lv, nearc, open, (float)dx, (float)dy, (float)r2, rx, ry, ia, parts,
+                     bigs, a);
+}
+
+// Walk of one tree, FP32 arithmetic with compensated accumulation.
 __device__ void walk32(const TreeDesc& t, double rx, double ry, bool active, Acc32& a,
-                       int32_t* s_node, uint64_t* s_mask, uint32_t* err) {
+                       int32_t* s_node, uint64_t* s_mask, uint32_t* s_near, uint32_t* err) {
   const int lane = lane_id();
-  const uint64_t amask = ballot(active);
-  if (amask == 0) return;
+  a.q = s_near + lane;
+  a.nq = 0;
+  if (ballot(active) == 0) return;
   const cptr<NodeF> nodes = as_const(t.nodesF);
   const cptr<PartD> parts = as_const(t.partsD);
   const cptr<PartD> bigs = as_const(t.bigD);
   const float ia = (float)t.inv_area;
-  if (lane == 0) { s_node[0] = 0; s_mask[0] = amask; }
-  int sp = 1;
+  int sp = 0;
+  {
+    const NodeF root = nodes[0];
+    const uint64_t om = eval32(root, active, rx, ry, ia, parts, bigs, a);
+    if (om) push_entry(s_node, s_mask, sp, lane, root.child | (root.count_cmask & 0xf0000000),
+                       om, err);
+  }
   while (sp > 0) {
     --sp;
     __builtin_amdgcn_wave_barrier();
-    const int ni = uni(s_node[sp]);
+    const int entry = uni(s_node[sp]);
     const uint64_t live = uni64(s_mask[sp]);
-    const NodeF nd = nodes[ni];
+    __builtin_amdgcn_wave_barrier();
     const bool lv = (live >> lane) & 1ull;
-    const double dx = nd.cx - rx, dy = nd.cy - ry;
-    const double r2 = dx * dx + dy * dy;
-    const bool nearc = lv && (r2 < nd.near2);
-    const bool open = lv && ((r2 < nd.r2crit) || nearc);
-    const int count = nd.count_cmask & 0x0fffffff;
-    const int cmask = (nd.count_cmask >> 28) & 0xf;
-    if (lv && !open) {
-      // far cell: monopole + quadrupole, one compensated add per component
-      const float ex = (float)dx, ey = (float)dy;
-      const float inv = rcp32((float)r2);
-      const float s = nd.mass * inv * kInvPIf;
-      const float c2 = (ex * ex - ey * ey) * inv;
-      const float s2 = 2.f * ex * ey * inv;
-      const float qx0 = nd.q0 * ex + nd.q2 * ey;
-      const float qx1 = nd.q1 * ey + nd.q2 * ex;
-      const float qq = ex * qx0 + ey * qx1;
-      const float ip = inv * inv * kInvPIf;
-      const float t4 = 4.f * qq * inv;
-      const float mono = nd.mass * ia - s;
-      a.ax.add(mono * ex + ip * (t4 * ex - qx0));
-      a.ay.add(mono * ey + ip * (t4 * ey - qx1));
-      a.g0.add(-s * c2);
-      a.g1.add(-s * s2);
-      a.pk -= ia * nd.mass;
+    const int base = entry & 0x0fffffff;
+    const int cm = (entry >> 28) & 0xf;
+    // opening tests of the four children first: their loads are issued together
+    float exk[4], eyk[4], r2k[4];
+    bool nk[4], ok[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double cx = nodes[base + k].cx, cy = nodes[base + k].cy;
+      const double rc = nodes[base + k].r2crit, ne = nodes[base + k].near2;
+      const double dx = cx - rx, dy = cy - ry;
+      const double r2 = dx * dx + dy * dy;
+      const bool lk = lv && ((cm >> k) & 1);
+      nk[k] = lk && (r2 < ne);
+      ok[k] = lk && ((r2 < rc) || nk[k]);
+      exk[k] = (float)dx;
+      eyk[k] = (float)dy;
+      r2k[k] = (float)r2;
     }
-    const uint64_t om = ballot(open);
-    if (om == 0) continue;
-    if (nd.child < 0) {
-      // leaf particles; particle - ray offsets are formed in FP64
-      for (int k = 0; k < count; ++k) {
-        const PartD p = parts[nd.start + k];
-        if (open) {
-          const float d0 = (float)(p.x - rx), d1 = (float)(p.y - ry);
-          float rr = d0 * d0 + d1 * d1;
-          rr = fmaxf(rr, 1e-20f);
-          const float inv = rcp32(rr);
-          const float m = p.m;
-          const float hs = fabsf(p.h);
-          float s = m * inv * kInvPIf;
-          float sm = 0.f;
-          if (nearc && !signbit(p.h) && rr < 4.f * hs * hs) {
-            float Mr2, sig;
-            bspline_fused32(rr, inv, hs, Mr2, sig);
-            s = m * Mr2 * kInvPIf;
-            sm = m * sig;
-          }
-          const float c2 = (d0 * d0 - d1 * d1) * inv;
-          const float s2 = 2.f * d0 * d1 * inv;
-          const float mono = ia * m - s;
-          a.pax += mono * d0;
-          a.pay += mono * d1;
-          a.pg0 -= (s - sm) * c2;
-          a.pg1 -= (s - sm) * s2;
-          a.pk += sm - m * ia;
-        }
-      }
-      a.flush();
-    } else {
-      if (count > 0 && ballot(nearc)) {
-        for (int k = 0; k < count; ++k) {
-          const PartD p = bigs[nd.start + k];
-          if (nearc) {
-            const float d0 = (float)(p.x - rx), d1 = (float)(p.y - ry);
-            float rr = d0 * d0 + d1 * d1;
-            rr = fmaxf(rr, 1e-20f);
-            const float hh = p.h;
-            if (rr < 4.f * hh * hh) {
-              // separate B-spline correction (the point mass was taken at the leaf)
-              const float r = sqrtf(rr);
-              const float q = r / hh;
-              float M, sig;
-              const float q2 = q * q, q3 = q2 * q;
-              if (q > 2.f) {
-                M = 1.f; sig = 0.f;
-              } else if (q > 1.f) {
-                const float tq = 2.f - q;
-                sig = (10.f / 28.f) * kInvPIf * tq * tq * tq / (hh * hh);
-                M = (-1.f + 20.f * q2 * (1.f - q + 0.375f * q2 - 0.05f * q3)) / 7.f;
-              } else {
-                sig = (10.f / 7.f) * kInvPIf * (1.f - 1.5f * q2 + 0.75f * q3) / (hh * hh);
-                M = 10.f * q2 * (1.f - 0.75f * q + 0.3f * q3) / 7.f;
-              }
-              const float m = p.m;
-              const float ar = (M - 1.f) * kInvPIf / r;
-              const float gt = ar / r - sig;
-              const float inv = rcp32(rr);
-              a.pax -= m * ar * d0 / r;
-              a.pay -= m * ar * d1 / r;
-              a.pg0 -= gt * m * (d0 * d0 - d1 * d1) * inv;
-              a.pg1 -= gt * m * 2.f * d0 * d1 * inv;
-              a.pk += m * sig;
-            }
-          }
-        }
-        a.flush();
-      }
-      if (sp + 4 > kStackDepth) {
-        if (lane == 0) atomicOr(err, 1u);
-        break;
-      }
-      __builtin_amdgcn_wave_barrier();
-      for (int c = 3; c >= 0; --c) {
-        if ((cmask >> c) & 1) {
-          if (lane == 0) { s_node[sp] = nd.child + c; s_mask[sp] = om; }
-          ++sp;
-        }
+    int32_t pe[4];
+    uint64_t pm[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      pm[k] = 0;
+      pe[k] = 0;
+      if ((cm >> k) & 1) {
+        const NodeF nd = nodes[base + k];
+        pm[k] = eval32_rest(nd, lv, nk[k], ok[k], exk[k], eyk[k], r2k[k], rx, ry, ia, parts,
+                            bigs, a);
+        pe[k] = nd.child | (nd.count_cmask & 0xf0000000);
       }
     }
+#pragma unroll
+    for (int k = 3; k >= 0; --k)
+      if (pm[k]) push_entry(s_node, s_mask, sp, lane, pe[k], pm[k], err);
   }
   a.flush();
+  near_terms64(parts, bigs, rx, ry, t.inv_area, a);
 }
 
 // FP64 direct sum of the smoothed-particle kernel over every particle of a
@@ -433,7 +597,7 @@ __device__ void direct64(const TreeDesc& t, double rx, double ry, bool active, A
 __device__ inline void recursion_step(const StepJob& jb, int slot, double alpha0, double alpha1,
                                       double kappa, double gamma0, double gamma1) {
 #pragma clang fp contract(off)
-  const RayState& st = jb.st;
+  const RayState st = state_view(jb.sbase, jb.nslots);
   double G[4] = {kappa + gamma0, gamma1, gamma1, kappa - gamma0};
   for (int k = 0; k < 4; ++k) G[k] *= jb.Gfac;
   double sa0 = st.sa0[slot], sa1 = st.sa1[slot];
@@ -477,19 +641,20 @@ __device__ inline double phys_coord(double th, double Dphys) {
 // wavefront.
 template <int MODE, bool COUNT>
 __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, WorkItem it,
-                                             int32_t* s_node, uint64_t* s_mask,
+                                             int32_t* s_node, uint64_t* s_mask, uint32_t* s_near,
                                              uint32_t* __restrict__ err) {
   const int lane = lane_id();
   const StepJob jb = as_const(jobs)[it.job];
   const int slot = (int)it.bundle * kWave + lane;
   const bool valid = slot < jb.nslots;
-  const int jm = valid ? jb.st.jmax[slot] : 0;
+  const RayState st = state_view(jb.sbase, jb.nslots);
+  const int jm = valid ? st.jmax[slot] : 0;
   const bool active = jb.j < jm;
   if (ballot(active) == 0) return;
   double th0 = 0, th1 = 0;
   if (active) {
-    th0 = jb.st.th0[slot];
-    th1 = jb.st.th1[slot];
+    th0 = st.th0[slot];
+    th1 = st.th1[slot];
   }
   const double rx = phys_coord(th0, jb.Dphys), ry = phys_coord(th1, jb.Dphys);
   double alpha0 = 0, alpha1 = 0, kappa = 0, gamma0 = 0, gamma1 = 0;
@@ -502,9 +667,10 @@ __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, W
     // (MultiPlane/planes.cpp:151-152, include/particle_halo.h:432-433).
     if (MODE == SLSIM_FP32C) {
       Acc32 a;
-      walk32(t, rx, ry, active, a, s_node, s_mask, err);
-      alpha0 += a.ax.value(); alpha1 += a.ay.value(); kappa += a.k.value();
-      gamma0 += a.g0.value(); gamma1 += a.g1.value();
+      walk32(t, rx, ry, active, a, s_node, s_mask, s_near, err);
+      alpha0 += a.ax.value() + a.fx; alpha1 += a.ay.value() + a.fy;
+      kappa += a.k.value() + a.fk;
+      gamma0 += a.g0.value() + a.fg0; gamma1 += a.g1.value() + a.fg1;
     } else if (MODE == SLSIM_DIRECT64) {
       Acc64 a;
       direct64(t, rx, ry, active, a);
@@ -515,9 +681,13 @@ __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, W
       alpha0 += a.ax; alpha1 += a.ay; kappa += a.k; gamma0 += a.g0; gamma1 += a.g1;
     }
   }
+  if (COUNT && jb.counters && lane == 0) {
+    uint32_t* c = jb.counters + kCountStride * (size_t)slot;
+    c[5] += cnt.wv; c[6] += cnt.wp;
+  }
   if (!active) return;
   if (COUNT && jb.counters) {
-    uint32_t* c = jb.counters + 5 * (size_t)slot;
+    uint32_t* c = jb.counters + kCountStride * (size_t)slot;
     c[0] += cnt.v; c[1] += cnt.c; c[2] += cnt.p; c[3] += cnt.b; c[4] += cnt.s;
   }
   recursion_step(jb, slot, alpha0, alpha1, kappa, gamma0, gamma1);
@@ -531,6 +701,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
     uint32_t* __restrict__ err) {
   __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
   __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
+  __shared__ uint32_t s_near[kWavesPerBlock][kNearQueue * kWave];
   const int w = uni((int)(threadIdx.x >> 6));
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t wi = blk * kWavesPerBlock + (uint32_t)w;
@@ -540,7 +711,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
     it.job = as_const(items)[wi].job;
     it.bundle = as_const(items)[wi].bundle;
   }
-  process_item<MODE, COUNT>(jobs, it, s_node[w], s_mask[w], err);
+  process_item<MODE, COUNT>(jobs, it, s_node[w], s_mask[w], s_near[w], err);
 }
 
 template __global__ void k_plane_step<SLSIM_FP64, false>(const StepJob*, const WorkItem*, uint32_t, uint32_t*);
@@ -556,6 +727,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step_queue(
     uint32_t* __restrict__ counter, uint32_t* __restrict__ err) {
   __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
   __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
+  __shared__ uint32_t s_near[kWavesPerBlock][kNearQueue * kWave];
   const int w = uni((int)(threadIdx.x >> 6));
   const int lane = lane_id();
   for (;;) {
@@ -566,7 +738,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step_queue(
     WorkItem it;
     it.job = as_const(items)[idx].job;
     it.bundle = as_const(items)[idx].bundle;
-    process_item<MODE, false>(jobs, it, s_node[w], s_mask[w], err);
+    process_item<MODE, false>(jobs, it, s_node[w], s_mask[w], s_near[w], err);
   }
 }
 
@@ -647,7 +819,7 @@ __global__ void k_counts_grid(const uint32_t* __restrict__ c, int64_t nslots, in
   if (slot >= nslots) return;
   int64_t i;
   if (!grid_slot(slot, N, tiles_x, tw, &i)) return;
-  for (int k = 0; k < 5; ++k) out[5 * i + k] = c[5 * slot + k];
+  for (int k = 0; k < 5; ++k) out[5 * i + k] = c[kCountStride * slot + k];
 }
 
 // Many GridMap requests at once: thread per slot of the concatenated slot
@@ -780,6 +952,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_force(
     double* __restrict__ out, uint32_t* __restrict__ err) {
   __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
   __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
+  __shared__ uint32_t s_near[kWavesPerBlock][kNearQueue * kWave];
   const int w = uni((int)(threadIdx.x >> 6));
   const int64_t i = (blockIdx.x * (int64_t)kWavesPerBlock + w) * kWave + lane_id();
   const bool active = i < n;
@@ -788,9 +961,9 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_force(
   double r[5];
   if (F32) {
     Acc32 a;
-    walk32(t, rx, ry, active, a, s_node[w], s_mask[w], err);
-    r[0] = a.ax.value(); r[1] = a.ay.value(); r[2] = a.k.value();
-    r[3] = a.g0.value(); r[4] = a.g1.value();
+    walk32(t, rx, ry, active, a, s_node[w], s_mask[w], s_near[w], err);
+    r[0] = a.ax.value() + a.fx; r[1] = a.ay.value() + a.fy; r[2] = a.k.value() + a.fk;
+    r[3] = a.g0.value() + a.fg0; r[4] = a.g1.value() + a.fg1;
   } else {
     Acc64 a;
     Counts c;

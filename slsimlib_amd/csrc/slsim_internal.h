@@ -19,6 +19,9 @@ constexpr int kMaxLevels = 32;
 constexpr int kStackDepth = 104;
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
+// Counters per slot in counting runs: 5 per-ray work counts, then the
+// bundle's wave-level cell evaluations and leaf-particle iterations (lane 0).
+constexpr int kCountStride = 7;
 
 constexpr double kPI = 3.14159265358979323846;   // M_PI (include/standard.h:38-40)
 constexpr double kGrav = 4.7788e-20;              // G/c^2 in Mpc/Msun (standard.h:43)
@@ -88,9 +91,21 @@ struct RayState {
 };
 constexpr size_t kStateBytesPerSlot = 14 * sizeof(double) + sizeof(int32_t);
 
+// The 15 arrays of a ray state laid out back to back from `d` (n slots each).
+__host__ __device__ inline RayState state_view(double* d, int64_t n) {
+  RayState s;
+  s.th0 = d;         s.th1 = d + n;
+  s.sa0 = d + 2 * n; s.sa1 = d + 3 * n;
+  s.A0 = d + 4 * n;  s.A1 = d + 5 * n;  s.A2 = d + 6 * n;  s.A3 = d + 7 * n;
+  s.G0 = d + 8 * n;  s.G1 = d + 9 * n;  s.G2 = d + 10 * n; s.G3 = d + 11 * n;
+  s.aal = d + 12 * n; s.bbl = d + 13 * n;
+  s.jmax = reinterpret_cast<int32_t*>(d + 14 * n);
+  return s;
+}
+
 // One plane step of one ray job.
 struct StepJob {
-  RayState st;
+  double* sbase;                // ray state, state_view(sbase, nslots)
   int32_t nslots;
   int32_t j;                    // plane index of this step
   int32_t ntrees;
@@ -100,7 +115,7 @@ struct StepJob {
   double Gfac;                  // charge * Dl[j] / (1 + z_j)
   double aa, bb;                // dDl[j+1]/Dl[j+1], Dl[j]/Dl[j+1]
   double charge;
-  uint32_t* counters;           // 5 per slot or null
+  uint32_t* counters;           // kCountStride per slot or null
 };
 
 struct WorkItem {

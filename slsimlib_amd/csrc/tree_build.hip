@@ -28,6 +28,7 @@ namespace slsim {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int32_t kBigCell = 4096;   // cells above this are reduced by a block
 
 template <typename T>
 __global__ void k_bbox(const T* __restrict__ x, const T* __restrict__ y, size_t n,
@@ -176,6 +177,7 @@ __global__ void k_moments(int64_t nnodes, const int32_t* __restrict__ st,
   const int64_t node = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   if (node >= nnodes) return;
   const int32_t s = st[node], c = cnt[node];
+  if (c > kBigCell) return;   // reduced by k_moments_big
   double m = 0, am = 0;
   for (int32_t k = lane; k < c; k += 64) {
     double w = ms[s + k];
@@ -206,6 +208,81 @@ __global__ void k_moments(int64_t nnodes, const int32_t* __restrict__ st,
   q1 = wave_sum(q1);
   q2 = wave_sum(q2);
   if (lane == 0) {
+    com[2 * node] = sx;
+    com[2 * node + 1] = sy;
+    mass[node] = m;
+    quad[3 * node] = q0;
+    quad[3 * node + 1] = q1;
+    quad[3 * node + 2] = q2;
+    double rcom = 0;
+    const double c2[2] = {sx, sy};
+    for (int d = 0; d < 2; ++d) {
+      double a = (c2[d] - p1[2 * node + d]) * (c2[d] - p1[2 * node + d]);
+      double b = (c2[d] - p2[2 * node + d]) * (c2[d] - p2[2 * node + d]);
+      rcom += a > b ? a : b;
+    }
+    rcom = sqrt(rcom);
+    if (theta > 0.0) {
+      double r = 1.15470 * rcom / theta;
+      r2crit[node] = r * r;
+    } else {
+      r2crit[node] = 1.0e100;
+    }
+  }
+}
+
+__device__ inline double block_sum(double v, double* sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0;
+  const int nw = blockDim.x >> 6;
+  for (int k = 0; k < nw; ++k) t += sh[k];
+  return t;
+}
+
+// CalcMoments for cells of more than kBigCell particles: one 1024-thread
+// block per cell (same arithmetic as k_moments, wider reduction).
+__global__ __launch_bounds__(1024) void k_moments_big(
+    const int32_t* __restrict__ list, const int32_t* __restrict__ st,
+    const int32_t* __restrict__ cnt, const double* __restrict__ p1,
+    const double* __restrict__ p2, const double* __restrict__ xs, const double* __restrict__ ys,
+    const float* __restrict__ ms, double theta, double* __restrict__ com,
+    double* __restrict__ mass, double* __restrict__ quad, double* __restrict__ r2crit) {
+  __shared__ double sh[16];
+  const int64_t node = list[blockIdx.x];
+  const int32_t s = st[node], c = cnt[node];
+  double m = 0, am = 0;
+  for (int32_t k = threadIdx.x; k < c; k += blockDim.x) {
+    double w = ms[s + k];
+    m += w;
+    am += fabs(w);
+  }
+  m = block_sum(m, sh);
+  am = block_sum(am, sh);
+  double sx = 0, sy = 0;
+  for (int32_t k = threadIdx.x; k < c; k += blockDim.x) {
+    double w = fabs((double)ms[s + k]);
+    sx += w * xs[s + k] / am;
+    sy += w * ys[s + k] / am;
+  }
+  sx = block_sum(sx, sh);
+  sy = block_sum(sy, sh);
+  double q0 = 0, q1 = 0, q2 = 0;
+  for (int32_t k = threadIdx.x; k < c; k += blockDim.x) {
+    double dx = xs[s + k] - sx, dy = ys[s + k] - sy;
+    double r2 = dx * dx + dy * dy;
+    double w = ms[s + k];
+    q0 += (r2 - 2 * dx * dx) * w;
+    q1 += (r2 - 2 * dy * dy) * w;
+    q2 += -2 * dx * dy * w;
+  }
+  q0 = block_sum(q0, sh);
+  q1 = block_sum(q1, sh);
+  q2 = block_sum(q2, sh);
+  if (threadIdx.x == 0) {
     com[2 * node] = sx;
     com[2 * node + 1] = sy;
     mass[node] = m;
@@ -568,6 +645,21 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
   if ((rc = tmp.get(&com, 2 * nnodes)) || (rc = tmp.get(&mass, nnodes)) ||
       (rc = tmp.get(&quad, 3 * nnodes)) || (rc = tmp.get(&r2c, nnodes)))
     return rc;
+  {
+    std::vector<int32_t> hcnt(nnodes), big;
+    SLSIM_HIP_CHECK(hipMemcpyAsync(hcnt.data(), gcnt, 4 * nnodes, hipMemcpyDeviceToHost, st));
+    SLSIM_HIP_CHECK(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < nnodes; ++i)
+      if (hcnt[i] > kBigCell) big.push_back((int32_t)i);
+    if (!big.empty()) {
+      int32_t* dbig;
+      if ((rc = tmp.get(&dbig, big.size()))) return rc;
+      SLSIM_HIP_CHECK(hipMemcpyAsync(dbig, big.data(), 4 * big.size(), hipMemcpyHostToDevice, st));
+      k_moments_big<<<(unsigned)big.size(), 1024, 0, st>>>(dbig, gst, gcnt, gp1, gp2, xs, ys, ms,
+                                                          pl->theta, com, mass, quad, r2c);
+      SLSIM_HIP_CHECK(hipStreamSynchronize(st));   // `big` is pageable host memory
+    }
+  }
   k_moments<<<nblocks(64 * (size_t)nnodes), kBlock, 0, st>>>(nnodes, gst, gcnt, gp1, gp2, xs, ys,
                                                              ms, pl->theta, com, mass, quad, r2c);
 

@@ -34,9 +34,10 @@ def planes_evenly_in_Dc(z_lo, z_hi, k):
     return [invCoorDist(d0 + (d1 - d0) * j / (k - 1), OM, OL, H) for j in range(k)]
 
 
-def field_plane(config, j, z, n, soft=5e-4, realization=0, dtype=np.float64):
-    x, y, m, h = gen_halo_field(seed(config, realization, j), n, z, n_halos=50, Mmin=1e11,
-                                Mmax=1e14, field_arcsec=300.0, softening=soft, Om=OM, OL=OL, h=H)
+def field_plane(config, j, z, n, soft=5e-4, realization=0, dtype=np.float64, mass_scale=1.0):
+    x, y, m, h = gen_halo_field(seed(config, realization, j), n, z, n_halos=50,
+                                Mmin=1e11 * mass_scale, Mmax=1e14 * mass_scale,
+                                field_arcsec=300.0, softening=soft, Om=OM, OL=OL, h=H)
     if dtype == np.float32:
         x, y = x.astype(np.float32), y.astype(np.float32)
     return x, y, m, h
@@ -55,12 +56,16 @@ def build_c1_lens(ctx, theta=0.1, n=65536):
 
 
 def build_field_lens(ctx, config=2, nplanes=10, n=1 << 20, z_lo=0.1, z_hi=2.0, zs=3.0,
-                     theta=0.1, soft=5e-4, dtype=np.float64, keep_particles=False):
+                     theta=0.1, soft=5e-4, dtype=np.float64, keep_particles=False,
+                     mass_scale=1.0, realization=0):
+    """Multi-plane field lens; mass_scale < 1 with fewer particles keeps the
+    particle mass of the full-size configuration."""
     zl = planes_evenly_in_Dc(z_lo, z_hi, nplanes)
     lens = Lens(ctx, zs, OM, OL, H)
     parts = []
     for j, z in enumerate(zl):
-        x, y, m, h = field_plane(config, j, z, n, soft=soft, dtype=dtype)
+        x, y, m, h = field_plane(config, j, z, n, soft=soft, dtype=dtype, mass_scale=mass_scale,
+                                 realization=realization)
         halo = LensHaloParticles(ctx, x, y, m, h, z, theta=theta)
         lens.moveinMainHalo(halo)
         if keep_particles:

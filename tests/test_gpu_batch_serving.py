@@ -1,0 +1,139 @@
+"""Batched GridMaps and the serving loop: every request's result equals its
+single GridMap shoot bit for bit, whatever the schedule or the mix."""
+import os
+import subprocess
+import threading
+
+import numpy as np
+import pytest
+
+import oracle as O
+import slsimlib_amd as S
+from slsimlib_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def lenses(ctx):
+    c1, _ = W.build_c1_lens(ctx, n=16384)
+    f1, _, _ = W.build_field_lens(ctx, config=4, nplanes=4, n=1 << 15, mass_scale=1 / 32)
+    f2, _, _ = W.build_field_lens(ctx, config=4, nplanes=7, n=1 << 14, mass_scale=1 / 64,
+                                  realization=1)
+    return {"c1": c1, "f1": f1, "f2": f2}
+
+
+def _requests(lenses):
+    r30, r300 = W.grid_range(30.0), W.grid_range(300.0)
+    return [(lenses["c1"], 64, (0.0, 0.0), r30, 0.0),
+            (lenses["f1"], 100, (1e-5, -2e-5), r300, 2.5),
+            (lenses["f2"], 37, (0.0, 3e-5), r300, 1.3),
+            (lenses["f1"], 128, (0.0, 0.0), r300, 0.0),
+            (lenses["f2"], 1, (2e-5, 2e-5), r300, 3.9),
+            (lenses["c1"], 65, (1e-6, 0.0), r30, 1.5)]
+
+
+def _single(req, prec):
+    lens, N, c, rg, zs = req
+    g = S.GridMap(lens, N, c, rg, prec, zs=zs)
+    return g.y(), g.A()
+
+
+@pytest.mark.parametrize("prec", [S.SLSIM_FP64, S.SLSIM_FP32C])
+def test_batch_equals_single_gridmaps(ctx, lenses, prec):
+    reqs = _requests(lenses)
+    ref = [_single(r, prec) for r in reqs]
+    b = S.Batch(ctx, reqs, prec)
+    for sched in ({}, {"tile_log2": 2, "queue": 1, "order": 1, "graph": 1},
+                  {"tile_log2": 4, "queue": 0, "order": 2, "graph": 1}):
+        b.set_schedule(**sched)
+        for _ in range(2):            # graph replays too
+            b.run()
+            for i in range(len(reqs)):
+                y, A = b.output(i)
+                assert np.array_equal(y, ref[i][0]) and np.array_equal(A, ref[i][1]), (sched, i)
+
+
+def test_batch_autotune_keeps_results(ctx, lenses):
+    reqs = _requests(lenses)
+    b = S.Batch(ctx, reqs, S.SLSIM_FP32C)
+    best, base_ms, best_ms = b.autotune(repeats=1)
+    assert best_ms <= base_ms * 1.5
+    b.run()
+    for i, r in enumerate(reqs):
+        y, A = b.output(i)
+        ys, As = _single(r, S.SLSIM_FP32C)
+        assert np.array_equal(y, ys) and np.array_equal(A, As)
+
+
+def test_server_results_alone_and_in_mix(ctx, lenses):
+    reqs = _requests(lenses)
+    srv = S.Server(ctx, hbm_budget=8 << 30)
+    alone = []
+    for r in reqs:
+        rid = srv.submit(r[0], r[1], r[2], r[3], r[4], S.SLSIM_FP32C)
+        y, A, info = srv.wait(rid, r[1])
+        assert info["status"] == 0
+        alone.append((y, A))
+    # the same requests submitted together, from several threads, twice over
+    out = {}
+
+    def client(k):
+        r = reqs[k % len(reqs)]
+        rid = srv.submit(r[0], r[1], r[2], r[3], r[4], S.SLSIM_FP32C)
+        out[k] = srv.wait(rid, r[1])
+
+    th = [threading.Thread(target=client, args=(k,)) for k in range(2 * len(reqs))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for k, (y, A, info) in out.items():
+        i = k % len(reqs)
+        assert np.array_equal(y, alone[i][0]) and np.array_equal(A, alone[i][1]), k
+        ys, As = _single(reqs[i], S.SLSIM_FP32C)
+        assert np.array_equal(y, ys) and np.array_equal(A, As)
+    st = srv.stats()
+    assert st["completed"] == 3 * len(reqs) and st["max_active"] >= 2
+    srv.close()
+
+
+def test_server_budget(ctx, lenses):
+    srv = S.Server(ctx, hbm_budget=64 << 20)
+    with pytest.raises(S.SlsimError) as e:
+        srv.submit(lenses["c1"], 2048, (0, 0), W.grid_range(30.0))
+    assert e.value.code == 3
+    # requests that fit only one at a time still all complete
+    ids = [srv.submit(lenses["c1"], 512, (0, 0), W.grid_range(30.0)) for _ in range(3)]
+    for rid in ids:
+        _, _, info = srv.wait(rid, 512, fetch=False)
+        assert info["status"] == 0
+    assert srv.stats()["max_active"] == 1
+    srv.close()
+
+
+def test_cpp_dropin_program(ctx, tmp_path):
+    exe = os.path.join(ROOT, "tests", "cpp", "dropin_gridmap")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/dropin_gridmap is not built (run __graft_entry__.build())")
+    out = tmp_path / "dropin.bin"
+    r = subprocess.run([exe, str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    v = np.fromfile(out, dtype=np.float64)
+    grid = v[:64 * 64 * 8].reshape(-1, 8)
+    rays = v[64 * 64 * 8:-1].reshape(-1, 8)
+    # the same lens through the oracle
+    n = 16384
+    x, y = S.gen_nfw(1001, n, 1e13, 8.0, 0.5)
+    m = np.full(n, np.float32(1e13 / n), np.float32)
+    h = np.full(n, np.float32(1e-4), np.float32)
+    L = O.Lens(2.0)
+    L.moveinMainHalo(O.Tree(x, y, m, h), 0.5)
+    yr, Ar = L.shoot(grid[:, :2])
+    assert np.abs(grid[:, 2:4] - yr).max() <= 1e-12 * np.abs(yr).max()
+    assert np.abs(grid[:, 4:8] - Ar).max() <= 1e-8
+    zs = 1.0 + 0.01 * np.arange(rays.shape[0])
+    yr, Ar = L.shoot(rays[:, :2], zs=zs)
+    assert np.abs(rays[:, 2:4] - yr).max() <= 1e-12 * np.abs(yr).max()
+    assert np.abs(rays[:, 4:8] - Ar).max() <= 1e-8
