@@ -222,6 +222,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip FP64, direct-sum, counts")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one plain device-to-device copy of MIB MiB (HIP runtime), the "
+                         "known-size kernel that scales HBM counters under rocprofv3 --pmc")
     ap.add_argument("--precision-form", action="store_true",
                     help="also run the 20-plane x 2^22-particle, 4096^2-ray precision form")
     ap.add_argument("--batched", type=int, default=0,
@@ -331,6 +334,8 @@ def main():
                                "leaf_particles": tot[2] / nrays, "big_tests": tot[3] / nrays,
                                "bsplines": tot[4] / nrays, "flops": flops / nrays}
         log("roofline:", out["roofline"])
+        out["bundle_utilisation"] = g64.bundle_stats()
+        log("bundles:", out["bundle_utilisation"])
 
         # FP64 direct-sum oracle on a 65,536-ray subsample (every 64th ray)
         sub = np.arange(0, nrays, max(1, nrays // 65536))
@@ -389,6 +394,14 @@ def main():
         if not args.no_extras:
             out["fp64_vs_cpu_oracle"] = compare(pos[idx], y64[idx], A64[idx], res[0], res[1])
         log("cpu oracle: %.3e rays/s on %d threads" % (cpu_rate, nthreads))
+
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        src, dst = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.d2d(dst, src, nb)
+        out["copy_calibration_bytes"] = nb
+        ctx.free(src)
+        ctx.free(dst)
 
     if args.precision_form:
         out["precision_form"] = run_precision_form(S, W, ctx)

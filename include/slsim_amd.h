@@ -243,6 +243,9 @@ int slsim_device_alloc(slsim_ctx ctx, size_t bytes, void** dev);
 int slsim_device_free(slsim_ctx ctx, void* dev);
 int slsim_memcpy_d2h(slsim_ctx ctx, void* host, const void* dev, size_t bytes);
 int slsim_memcpy_h2d(slsim_ctx ctx, void* dev, const void* host, size_t bytes);
+/* Device-to-device copy on the context's stream (the HIP runtime's copy
+ * kernel: a known-size reference for HBM traffic counters). */
+int slsim_memcpy_d2d(slsim_ctx ctx, void* dst, const void* src, size_t bytes);
 
 /* ---------------------------------------------------------- cosmology */
 /* Flat LCDM comoving distance between z1 and z2 (Mpc), the role of

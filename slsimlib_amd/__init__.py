@@ -65,6 +65,9 @@ class Context:
         _check(lib.slsim_memcpy_h2d(self.h, dev, host.ctypes.data_as(ctypes.c_void_p),
                                     ctypes.c_size_t(host.nbytes)))
 
+    def d2d(self, dst, src, nbytes):
+        _check(lib.slsim_memcpy_d2d(self.h, dst, src, ctypes.c_size_t(nbytes)))
+
     def close(self):
         if self.h:
             lib.slsim_ctx_destroy(self.h)

@@ -136,6 +136,7 @@ SIGNATURES = {
     "slsim_device_free": (c_int, [slsim_ctx, c_vp]),
     "slsim_memcpy_d2h": (c_int, [slsim_ctx, c_vp, c_vp, c_size_t]),
     "slsim_memcpy_h2d": (c_int, [slsim_ctx, c_vp, c_vp, c_size_t]),
+    "slsim_memcpy_d2d": (c_int, [slsim_ctx, c_vp, c_vp, c_size_t]),
     "slsim_coordist": (c_double, [c_double, c_double, c_double, c_double, c_double]),
     "slsim_inv_coordist": (c_double, [c_double, c_double, c_double, c_double]),
     "slsim_nfw_r200": (c_double, [c_double, c_double, c_double, c_double, c_double]),

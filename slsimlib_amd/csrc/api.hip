@@ -160,6 +160,9 @@ TreeDesc slsim_plane_s::desc() const {
   t.partsD = partsD;
   t.bigD = bigD;
   t.inv_area = inv_area;
+  t.mass = mass;
+  t.comx = com[0];
+  t.comy = com[1];
   t.nnodes = (int32_t)nnodes;
   t.nparts = (int32_t)n;
   return t;
@@ -785,6 +788,13 @@ int slsim_memcpy_d2h(slsim_ctx ctx, void* host, const void* dev, size_t bytes) {
 int slsim_memcpy_h2d(slsim_ctx ctx, void* dev, const void* host, size_t bytes) {
   if (!ctx) return arg_error("null context");
   SLSIM_HIP_CHECK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return SLSIM_OK;
+}
+
+int slsim_memcpy_d2d(slsim_ctx ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx) return arg_error("null context");
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
   SLSIM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return SLSIM_OK;
 }

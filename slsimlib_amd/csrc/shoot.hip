@@ -268,6 +268,17 @@ __device__ void walk64(const TreeDesc& t, double rx, double ry, bool active, Acc
 }
 
 // ---------------------------------------------------------------- FP32 walk
+// The FP32 walk sums pi times the reference's terms, with 1/pi, the signs
+// and the factor 2 of gamma1 taken out of the per-term arithmetic:
+//   AX, AY = pi sum alpha,  K = pi sum kappa (B-spline terms),
+//   G0 = -pi sum gamma0,    G1 = -(pi/2) sum gamma1,
+// restored once per walk in FP64 (finish32).  Cell quadrupoles are stored
+// as 4 q.  The mass sheet (inv_area: quadTree.h:861-888, 1035-1044) adds
+// inv_area m (c - x) to alpha and -inv_area m to kappa for every cell
+// monopole and point mass; every particle enters exactly once as one or the
+// other, so the sheet's total is inv_area M (c - x) and -inv_area M with M, c
+// the tree's mass and centre of mass, added in FP64 at the end.
+
 struct CSum {   // Kahan-compensated FP32 sum
   float s = 0.f, c = 0.f;   // c: the running compensation (lost low part, negated)
   __device__ inline void add(float x) {
@@ -279,88 +290,103 @@ struct CSum {   // Kahan-compensated FP32 sum
   __device__ inline double value() const { return (double)s - (double)c; }
 };
 
-// Near-field interactions deferred to FP64 (per lane, in LDS): a particle
-// index, with the top bit set for an internal big-particle list entry.
+// Near-field interactions deferred to FP64 (per lane, in LDS): the tree
+// index in the high word, the particle index in the low word with its top
+// bit set for an internal big-particle list entry.
 constexpr int kNearQueue = 8;
 constexpr uint32_t kBigListBit = 0x80000000u;
 
 struct Acc32 {
   CSum ax, ay, k, g0, g1;
-  float pax = 0.f, pay = 0.f, pk = 0.f, pg0 = 0.f, pg1 = 0.f;   // plain partials of a leaf
-  uint32_t* q = nullptr;   // this lane's queue: q[0], q[64], ... q[64 * (kNearQueue - 1)]
+  // plain partial sums of one popped stack entry (four cells), then one
+  // compensated add per entry
+  float pax = 0.f, pay = 0.f, pk = 0.f, pg0 = 0.f, pg1 = 0.f;
+  uint64_t* q = nullptr;   // this lane's queue: q[0], q[64], ... q[64 * (kNearQueue - 1)]
   int nq = 0;
-  double fx = 0, fy = 0, fk = 0, fg0 = 0, fg1 = 0;   // FP64 sums of the queued terms
+  uint32_t tree = 0;       // index of the tree being walked (uniform)
+  // FP64 sums of the queued terms, in the reference's scale and signs
+  double fx = 0, fy = 0, fk = 0, fg0 = 0, fg1 = 0;
   __device__ inline void flush() {
     ax.add(pax); ay.add(pay); k.add(pk); g0.add(pg0); g1.add(pg1);
     pax = pay = pk = pg0 = pg1 = 0.f;
   }
   __device__ inline bool push(uint32_t idx) {
     if (nq >= kNearQueue) return false;
-    q[64 * nq] = idx;
+    q[64 * nq] = ((uint64_t)tree << 32) | idx;
     ++nq;
     return true;
   }
 };
 
-// M(q) / r^2 and sigma(q) of the reference's cubic B-spline
+// M(q) / r^2 and pi sigma(q) of the reference's cubic B-spline
 // (include/quadTree.h:147-190, M in the reference's form), written so that
-// nothing cancels at small q.
-__device__ inline void bspline_fused32(float r2, float inv, float h, float& Mr2, float& sig) {
+// nothing cancels at small q.  inv = 1 / r^2.
+__device__ inline void bspline_fused32(float r2, float inv, float h, float& Mr2, float& sigpi) {
   const float r = sqrtf(r2);
   const float ih = rcp32(h);
   const float q = r * ih;
   const float ih2 = ih * ih;
   if (q > 2.f) {
     Mr2 = inv;
-    sig = 0.f;
+    sigpi = 0.f;
   } else if (q > 1.f) {
     const float q2 = q * q, q3 = q2 * q;
     const float M = (-1.f + 20.f * q2 * (1.f - q + 0.375f * q2 - 0.05f * q3)) * (1.f / 7.f);
     Mr2 = M * inv;
     const float tq = 2.f - q;
-    sig = (10.f / 28.f) * kInvPIf * tq * tq * tq * ih2;
+    sigpi = (10.f / 28.f) * tq * tq * tq * ih2;
   } else {
     const float q2 = q * q, q3 = q2 * q;
     Mr2 = (10.f / 7.f) * (1.f - 0.75f * q + 0.3f * q3) * ih2;
-    sig = (10.f / 7.f) * kInvPIf * (1.f - 1.5f * q2 + 0.75f * q3) * ih2;
+    sigpi = (10.f / 7.f) * (1.f - 1.5f * q2 + 0.75f * q3) * ih2;
   }
 }
 
-// Smoothed particle in FP32: the point mass with the reference's B-spline
-// correction folded in (algebraically the reference's two-step form),
-//   alpha += (ia m - s) d,  gamma -= (s - m sigma) (c2, s2),  kappa += m sigma - m ia,
-// s = m M(q) / (pi r^2): of order m / h^2, nothing large cancels.
-__device__ inline void smooth32(float d0, float d1, float rr, float m, float h, float ia,
-                                Acc32& a) {
+// Point mass (include/quadTree.h:1019-1046) at offset d, |d|^2 = rr, for a
+// particle at least 2h from the ray: s = m / r^2,
+//   AX -= s d0, AY -= s d1, G0 += s/r^2 (d0^2 - d1^2), G1 += s/r^2 d0 d1.
+__device__ inline void point32(float d0, float d1, float d1s, float rr, float m, Acc32& a) {
   const float inv = rcp32(rr);
-  float Mr2, sig;
-  bspline_fused32(rr, inv, h, Mr2, sig);
-  const float s = m * Mr2 * kInvPIf;
-  const float sm = m * sig;
-  const float c2 = (d0 * d0 - d1 * d1) * inv;
-  const float s2 = 2.f * d0 * d1 * inv;
-  const float mono = ia * m - s;
-  a.pax += mono * d0;
-  a.pay += mono * d1;
-  a.pg0 -= (s - sm) * c2;
-  a.pg1 -= (s - sm) * s2;
-  a.pk += sm - m * ia;
+  const float s = m * inv;
+  const float w = s * inv;
+  a.pax = fmaf(-s, d0, a.pax);
+  a.pay = fmaf(-s, d1, a.pay);
+  a.pg0 = fmaf(w, fmaf(d0, d0, -d1s), a.pg0);
+  a.pg1 = fmaf(w * d0, d1, a.pg1);
 }
 
-// Point mass in FP32 (include/quadTree.h:1019-1046) for a particle at least
-// 2h from the ray: alpha += (ia m - s) d, gamma -= s (c2, s2), kappa -= ia m,
-// s = m / (pi r^2).
-__device__ inline void point32(float d0, float d1, float rr, float m, float ia, Acc32& a) {
+// Smoothed particle: the point mass with the reference's B-spline
+// correction folded in (algebraically the reference's two-step form),
+// s = m M(q) / r^2, sm = pi m sigma(q); of order m / h^2, nothing large cancels.
+__device__ inline void smooth32(float d0, float d1, float d1s, float rr, float m, float h,
+                                Acc32& a) {
   const float inv = rcp32(rr);
-  const float s = m * inv * kInvPIf;
-  const float c2 = (d0 * d0 - d1 * d1) * inv;
-  const float s2 = 2.f * d0 * d1 * inv;
-  const float mono = ia * m - s;
-  a.pax += mono * d0;
-  a.pay += mono * d1;
-  a.pg0 -= s * c2;
-  a.pg1 -= s * s2;
-  a.pk -= m * ia;
+  float Mr2, sigpi;
+  bspline_fused32(rr, inv, h, Mr2, sigpi);
+  const float s = m * Mr2;
+  const float sm = m * sigpi;
+  const float w = (s - sm) * inv;
+  a.pax = fmaf(-s, d0, a.pax);
+  a.pay = fmaf(-s, d1, a.pay);
+  a.pg0 = fmaf(w, fmaf(d0, d0, -d1s), a.pg0);
+  a.pg1 = fmaf(w * d0, d1, a.pg1);
+  a.pk += sm;
+}
+
+// B-spline correction alone (the reference's second step), used only when
+// a lane's near-field queue is full: (M - 1) / r^2 = Mr2 - 1/r^2.
+__device__ inline void correction32(float d0, float d1, float rr, float m, float h, Acc32& a) {
+  const float inv = rcp32(rr);
+  float Mr2, sigpi;
+  bspline_fused32(rr, inv, h, Mr2, sigpi);
+  const float am = m * (Mr2 - inv);
+  const float gt = am - m * sigpi;
+  const float w = gt * inv;
+  a.pax = fmaf(-am, d0, a.pax);
+  a.pay = fmaf(-am, d1, a.pay);
+  a.pg0 = fmaf(w, fmaf(d0, d0, -d1 * d1), a.pg0);
+  a.pg1 = fmaf(w * d0, d1, a.pg1);
+  a.pk = fmaf(m, sigpi, a.pk);
 }
 
 // The queued near-field terms, after the walk (its registers are free then),
@@ -369,12 +395,15 @@ __device__ inline void point32(float d0, float d1, float rr, float m, float ia, 
 // applies no B-spline) and the B-spline corrections of internal-cell lists
 // (which cancel a point mass taken at the particle's leaf).  These terms grow
 // as 1/r^2 near a particle; FP64 keeps them and their cancellation exact.
-__device__ void near_terms64(const cptr<PartD> parts, const cptr<PartD> bigs, double rx,
-                             double ry, double ia, Acc32& a) {
+// The mass sheet of the point masses is in finish32.
+__device__ void near_terms64(const TreeDesc* __restrict__ trees, double rx, double ry,
+                             Acc32& a) {
   for (int i = 0; i < a.nq; ++i) {
-    const uint32_t idx = a.q[64 * i];
+    const uint64_t e = a.q[64 * i];
+    const uint32_t idx = (uint32_t)e;
+    const TreeDesc* t = trees + (uint32_t)(e >> 32);
     const bool big = (idx & kBigListBit) != 0;
-    const PartD p = big ? bigs[idx & ~kBigListBit] : parts[idx];
+    const PartD p = big ? t->bigD[idx & ~kBigListBit] : t->partsD[idx];
     const double d0 = p.x - rx, d1 = p.y - ry;
     double rr = d0 * d0 + d1 * d1;
     if (rr < 1e-20) rr = 1e-20;
@@ -383,9 +412,8 @@ __device__ void near_terms64(const cptr<PartD> parts, const cptr<PartD> bigs, do
       const double inv = rcp64(rr);
       const double pref = m * inv * kInvPI;
       const double tmp = -2.0 * pref * inv;
-      a.fx += (ia * m - pref) * d0;
-      a.fy += (ia * m - pref) * d1;
-      a.fk -= m * ia;
+      a.fx -= pref * d0;
+      a.fy -= pref * d1;
       a.fg0 += 0.5 * (d0 * d0 - d1 * d1) * tmp;
       a.fg1 += d0 * d1 * tmp;
     } else {
@@ -397,56 +425,27 @@ __device__ void near_terms64(const cptr<PartD> parts, const cptr<PartD> bigs, do
   a.nq = 0;
 }
 
-// B-spline correction in FP32 (the reference's two-step form), used only
-// when a lane's near-field queue is full.
-__device__ inline void correction32(float d0, float d1, float rr, float m, float hh, Acc32& a) {
-  const float r = sqrtf(rr);
-  const float q = r / hh;
-  float M, sig;
-  const float q2 = q * q, q3 = q2 * q;
-  if (q > 2.f) {
-    M = 1.f; sig = 0.f;
-  } else if (q > 1.f) {
-    const float tq = 2.f - q;
-    sig = (10.f / 28.f) * kInvPIf * tq * tq * tq / (hh * hh);
-    M = (-1.f + 20.f * q2 * (1.f - q + 0.375f * q2 - 0.05f * q3)) / 7.f;
-  } else {
-    sig = (10.f / 7.f) * kInvPIf * (1.f - 1.5f * q2 + 0.75f * q3) / (hh * hh);
-    M = 10.f * q2 * (1.f - 0.75f * q + 0.3f * q3) / 7.f;
-  }
-  const float ar = (M - 1.f) * kInvPIf / r;
-  const float gt = ar / r - sig;
-  const float inv = rcp32(rr);
-  a.pax -= m * ar * d0 / r;
-  a.pay -= m * ar * d1 / r;
-  a.pg0 -= gt * m * (d0 * d0 - d1 * d1) * inv;
-  a.pg1 -= gt * m * 2.f * d0 * d1 * inv;
-  a.pk += m * sig;
-}
-
 // The rest of a cell evaluation once the opening test is known.
 __device__ __forceinline__ uint64_t eval32_rest(const NodeF& nd, bool lv, bool nearc, bool open,
                                                 float ex, float ey, float r2f, double rx,
-                                                double ry, float ia, cptr<PartD> parts,
-                                                cptr<PartD> bigs, Acc32& a) {
+                                                double ry, cptr<PartD> parts, cptr<PartD> bigs,
+                                                Acc32& a) {
   const int count = nd.count_cmask & 0x0fffffff;
   if (lv && !open) {
-    // far cell: monopole + quadrupole, one compensated add per component
+    // far cell: monopole + quadrupole (stored as 4 q)
     const float inv = rcp32(r2f);
-    const float s = nd.mass * inv * kInvPIf;
-    const float c2 = (ex * ex - ey * ey) * inv;
-    const float s2 = 2.f * ex * ey * inv;
-    const float qx0 = nd.q0 * ex + nd.q2 * ey;
-    const float qx1 = nd.q1 * ey + nd.q2 * ex;
-    const float qq = ex * qx0 + ey * qx1;
-    const float ip = inv * inv * kInvPIf;
-    const float t4 = 4.f * qq * inv;
-    const float mono = nd.mass * ia - s;
-    a.ax.add(mono * ex + ip * (t4 * ex - qx0));
-    a.ay.add(mono * ey + ip * (t4 * ey - qx1));
-    a.g0.add(-s * c2);
-    a.g1.add(-s * s2);
-    a.pk -= ia * nd.mass;
+    const float s = nd.mass * inv;
+    const float u = inv * inv;
+    const float qx0 = fmaf(nd.q0, ex, nd.q2 * ey);
+    const float qx1 = fmaf(nd.q1, ey, nd.q2 * ex);
+    const float qq = fmaf(ex, qx0, ey * qx1);
+    const float kk = fmaf(qq * inv, u, -s);
+    const float u4 = 0.25f * u;
+    a.pax = fmaf(kk, ex, fmaf(-qx0, u4, a.pax));
+    a.pay = fmaf(kk, ey, fmaf(-qx1, u4, a.pay));
+    const float w = s * inv;
+    a.pg0 = fmaf(w, fmaf(ex, ex, -ey * ey), a.pg0);
+    a.pg1 = fmaf(w * ex, ey, a.pg1);
   }
   const uint64_t om = ballot(open);
   if (om == 0) return 0;
@@ -456,17 +455,21 @@ __device__ __forceinline__ uint64_t eval32_rest(const NodeF& nd, bool lv, bool n
       const PartD p = parts[nd.start + k];
       if (open) {
         const float d0 = (float)(p.x - rx), d1 = (float)(p.y - ry);
-        const float rr = fmaxf(d0 * d0 + d1 * d1, 1e-20f);
+        const float d1s = d1 * d1;
+        const float rr = fmaxf(fmaf(d0, d0, d1s), 1e-20f);
         const float hs = fabsf(p.h);
-        if (rr < 4.f * hs * hs && nearc && !signbit(p.h))
-          smooth32(d0, d1, rr, p.m, hs, ia, a);
-        else if (!(rr < 4.f * hs * hs) || !a.push((uint32_t)(nd.start + k)))
-          point32(d0, d1, rr, p.m, ia, a);
+        bool pt = true;
+        if (rr < 4.f * hs * hs) {
+          if (nearc && !signbit(p.h)) {
+            smooth32(d0, d1, d1s, rr, p.m, hs, a);
+            pt = false;
+          } else {
+            pt = !a.push((uint32_t)(nd.start + k));
+          }
+        }
+        if (pt) point32(d0, d1, d1s, rr, p.m, a);
       }
     }
-    // flushed only by lanes that took the leaf: a lane's partial sums must
-    // not depend on which cells its bundle mates open
-    if (open) a.flush();
     return 0;
   }
   if (count > 0 && ballot(nearc)) {
@@ -481,50 +484,44 @@ __device__ __forceinline__ uint64_t eval32_rest(const NodeF& nd, bool lv, bool n
           correction32(d0, d1, rr, p.m, p.h, a);
       }
     }
-    if (nearc) a.flush();
   }
   return om;
 }
 
-// FP32 cell evaluation: cell centres relative to the ray and the opening
-// test are FP64, so both walks open the same cells; multipoles and distant
-// particles are FP32 with compensated sums; near-field terms are queued for
-// FP64 (near_terms64).
-__device__ __forceinline__ uint64_t eval32(const NodeF& nd, bool lv, double rx, double ry,
-                                           float ia, cptr<PartD> parts, cptr<PartD> bigs,
-                                           Acc32& a) {
-  const double dx = nd.cx - rx, dy = nd.cy - ry;
-  const double r2 = dx * dx + dy * dy;
-  const bool nearc = lv && (r2 < nd.near2);
-  const bool open = lv && ((r2 < nd.r2crit) || nearc);
-  return eval32_rest(nd, lv, nearc, open, (float)dx, (float)dy, (float)r2, rx, ry, ia, parts,
-                     bigs, a);
-}
-
-// Walk of one tree, FP32 arithmetic with compensated accumulation.
-__device__ void walk32(const TreeDesc& t, double rx, double ry, bool active, Acc32& a,
-                       int32_t* s_node, uint64_t* s_mask, uint32_t* s_near, uint32_t* err) {
+// Walk of the trees of one plane, FP32 arithmetic with compensated
+// accumulation.  One loop serves every tree: when the stack is empty the next
+// tree starts with its root as the only child of a virtual entry (the cell
+// arrays carry three zero cells of padding for the loads of its siblings).
+__device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx, double ry,
+                       bool active, Acc32& a, int32_t* s_node, uint64_t* s_mask,
+                       uint64_t* s_near, uint32_t* err) {
   const int lane = lane_id();
   a.q = s_near + lane;
   a.nq = 0;
-  if (ballot(active) == 0) return;
-  const cptr<NodeF> nodes = as_const(t.nodesF);
-  const cptr<PartD> parts = as_const(t.partsD);
-  const cptr<PartD> bigs = as_const(t.bigD);
-  const float ia = (float)t.inv_area;
-  int sp = 0;
-  {
-    const NodeF root = nodes[0];
-    const uint64_t om = eval32(root, active, rx, ry, ia, parts, bigs, a);
-    if (om) push_entry(s_node, s_mask, sp, lane, root.child | (root.count_cmask & 0xf0000000),
-                       om, err);
-  }
-  while (sp > 0) {
-    --sp;
-    __builtin_amdgcn_wave_barrier();
-    const int entry = uni(s_node[sp]);
-    const uint64_t live = uni64(s_mask[sp]);
-    __builtin_amdgcn_wave_barrier();
+  const uint64_t amask = ballot(active);
+  if (amask == 0) return;
+  cptr<NodeF> nodes = nullptr;
+  cptr<PartD> parts = nullptr, bigs = nullptr;
+  int h = -1, sp = 0;
+  while (true) {
+    int entry;
+    uint64_t live;
+    if (sp == 0) {
+      if (++h >= ntrees) break;
+      const cptr<TreeDesc> td = as_const(trees) + h;
+      nodes = as_const(td->nodesF);
+      parts = as_const(td->partsD);
+      bigs = as_const(td->bigD);
+      a.tree = (uint32_t)h;
+      entry = 1 << 28;            // base 0, child mask {0}: the root
+      live = amask;
+    } else {
+      --sp;
+      __builtin_amdgcn_wave_barrier();
+      entry = uni(s_node[sp]);
+      live = uni64(s_mask[sp]);
+      __builtin_amdgcn_wave_barrier();
+    }
     const bool lv = (live >> lane) & 1ull;
     const int base = entry & 0x0fffffff;
     const int cm = (entry >> 28) & 0xf;
@@ -552,17 +549,39 @@ __device__ void walk32(const TreeDesc& t, double rx, double ry, bool active, Acc
       pe[k] = 0;
       if ((cm >> k) & 1) {
         const NodeF nd = nodes[base + k];
-        pm[k] = eval32_rest(nd, lv, nk[k], ok[k], exk[k], eyk[k], r2k[k], rx, ry, ia, parts,
-                            bigs, a);
+        pm[k] = eval32_rest(nd, lv, nk[k], ok[k], exk[k], eyk[k], r2k[k], rx, ry, parts, bigs,
+                            a);
         pe[k] = nd.child | (nd.count_cmask & 0xf0000000);
       }
     }
+    // one compensated add per entry, by the lanes the entry holds: a lane's
+    // sums do not depend on which cells its bundle mates open
+    if (lv) a.flush();
 #pragma unroll
     for (int k = 3; k >= 0; --k)
       if (pm[k]) push_entry(s_node, s_mask, sp, lane, pe[k], pm[k], err);
   }
-  a.flush();
-  near_terms64(parts, bigs, rx, ry, t.inv_area, a);
+  near_terms64(trees, rx, ry, a);
+}
+
+// The FP32 walk's sums in the reference's scale and signs, plus the FP64
+// near-field terms: out = alpha0, alpha1, kappa, gamma0, gamma1.
+__device__ inline void finish32(const Acc32& a, double* out) {
+  out[0] = kInvPI * a.ax.value() + a.fx;
+  out[1] = kInvPI * a.ay.value() + a.fy;
+  out[2] = kInvPI * a.k.value() + a.fk;
+  out[3] = -kInvPI * a.g0.value() + a.fg0;
+  out[4] = -2.0 * kInvPI * a.g1.value() + a.fg1;
+}
+
+// A tree's mass sheet for the FP32 walk, added to its outputs.
+__device__ inline void sheet32(const TreeDesc& t, double rx, double ry, double* out) {
+  if (t.inv_area != 0.0) {
+    const double sheet = t.inv_area * t.mass;
+    out[0] += sheet * (t.comx - rx);
+    out[1] += sheet * (t.comy - ry);
+    out[2] -= sheet;
+  }
 }
 
 // FP64 direct sum of the smoothed-particle kernel over every particle of a
@@ -641,7 +660,7 @@ __device__ inline double phys_coord(double th, double Dphys) {
 // wavefront.
 template <int MODE, bool COUNT>
 __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, WorkItem it,
-                                             int32_t* s_node, uint64_t* s_mask, uint32_t* s_near,
+                                             int32_t* s_node, uint64_t* s_mask, uint64_t* s_near,
                                              uint32_t* __restrict__ err) {
   const int lane = lane_id();
   const StepJob jb = as_const(jobs)[it.job];
@@ -660,18 +679,22 @@ __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, W
   double alpha0 = 0, alpha1 = 0, kappa = 0, gamma0 = 0, gamma1 = 0;
   Counts cnt;
   const int ntrees = uni(jb.ntrees);
-  for (int h = 0; h < ntrees; ++h) {
+  if (MODE == SLSIM_FP32C) {
+    // one set of FP32 sums over all halos of the plane, so that no FP64
+    // totals are live during the walks
+    Acc32 a;
+    walk32(jb.trees, ntrees, rx, ry, active, a, s_node, s_mask, s_near, err);
+    double f[5];
+    finish32(a, f);
+    for (int h = 0; h < ntrees; ++h) sheet32(as_const(jb.trees)[h], rx, ry, f);
+    alpha0 = f[0]; alpha1 = f[1]; kappa = f[2]; gamma0 = f[3]; gamma1 = f[4];
+  }
+  for (int h = 0; h < (MODE == SLSIM_FP32C ? 0 : ntrees); ++h) {
     const TreeDesc t = as_const(jb.trees)[h];
     // LensPlaneSingular::force sums the halos; force_halo and the plane each
     // flip alpha, so the plane's alpha is the tree's raw alpha
     // (MultiPlane/planes.cpp:151-152, include/particle_halo.h:432-433).
-    if (MODE == SLSIM_FP32C) {
-      Acc32 a;
-      walk32(t, rx, ry, active, a, s_node, s_mask, s_near, err);
-      alpha0 += a.ax.value() + a.fx; alpha1 += a.ay.value() + a.fy;
-      kappa += a.k.value() + a.fk;
-      gamma0 += a.g0.value() + a.fg0; gamma1 += a.g1.value() + a.fg1;
-    } else if (MODE == SLSIM_DIRECT64) {
+    if (MODE == SLSIM_DIRECT64) {
       Acc64 a;
       direct64(t, rx, ry, active, a);
       alpha0 += a.ax; alpha1 += a.ay; kappa += a.k; gamma0 += a.g0; gamma1 += a.g1;
@@ -701,7 +724,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
     uint32_t* __restrict__ err) {
   __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
   __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
-  __shared__ uint32_t s_near[kWavesPerBlock][kNearQueue * kWave];
+  __shared__ uint64_t s_near[kWavesPerBlock][kNearQueue * kWave];
   const int w = uni((int)(threadIdx.x >> 6));
   const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t wi = blk * kWavesPerBlock + (uint32_t)w;
@@ -727,7 +750,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step_queue(
     uint32_t* __restrict__ counter, uint32_t* __restrict__ err) {
   __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
   __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
-  __shared__ uint32_t s_near[kWavesPerBlock][kNearQueue * kWave];
+  __shared__ uint64_t s_near[kWavesPerBlock][kNearQueue * kWave];
   const int w = uni((int)(threadIdx.x >> 6));
   const int lane = lane_id();
   for (;;) {
@@ -952,7 +975,7 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_force(
     double* __restrict__ out, uint32_t* __restrict__ err) {
   __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
   __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
-  __shared__ uint32_t s_near[kWavesPerBlock][kNearQueue * kWave];
+  __shared__ uint64_t s_near[kWavesPerBlock][kNearQueue * kWave];
   const int w = uni((int)(threadIdx.x >> 6));
   const int64_t i = (blockIdx.x * (int64_t)kWavesPerBlock + w) * kWave + lane_id();
   const bool active = i < n;
@@ -961,9 +984,9 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_force(
   double r[5];
   if (F32) {
     Acc32 a;
-    walk32(t, rx, ry, active, a, s_node[w], s_mask[w], s_near[w], err);
-    r[0] = a.ax.value() + a.fx; r[1] = a.ay.value() + a.fy; r[2] = a.k.value() + a.fk;
-    r[3] = a.g0.value() + a.fg0; r[4] = a.g1.value() + a.fg1;
+    walk32(tree, 1, rx, ry, active, a, s_node[w], s_mask[w], s_near[w], err);
+    finish32(a, r);
+    sheet32(t, rx, ry, r);
   } else {
     Acc64 a;
     Counts c;

@@ -52,7 +52,7 @@ static_assert(sizeof(NodeD) == 96, "NodeD layout");
 struct alignas(16) NodeF {
   double cx, cy;
   double r2crit, near2;
-  float mass, q0, q1, q2;
+  float mass, q0, q1, q2;   // q0..q2: 4 times the quadrupole
   float box;
   int32_t child;
   int32_t start;
@@ -76,6 +76,7 @@ struct TreeDesc {
   const PartD* partsD;
   const PartD* bigD;
   double inv_area;
+  double mass, comx, comy;   // total mass and centre of mass (the mass sheet's terms)
   int32_t nnodes;
   int32_t nparts;
 };
@@ -172,6 +173,7 @@ struct slsim_plane_s {
   int32_t depth = 0;
   double p1[2] = {0, 0}, p2[2] = {0, 0};
   double theta = 0.1, inv_area = 0.0;
+  double mass = 0, com[2] = {0, 0};   // total mass and centre of mass (root cell)
   int bucket = 20;
   double build_ms = 0;
   size_t bytes = 0;

@@ -382,9 +382,10 @@ __global__ void k_pack_nodes(int64_t nnodes, const double* __restrict__ com,
   f.r2crit = a.r2crit;
   f.near2 = a.near2;
   f.mass = (float)a.mass;
-  f.q0 = (float)a.q0;
-  f.q1 = (float)a.q1;
-  f.q2 = (float)a.q2;
+  // 4 q: the FP32 walk's quadrupole term needs 4 (q . e) and (q . e) / 4
+  f.q0 = 4.f * (float)a.q0;
+  f.q1 = 4.f * (float)a.q1;
+  f.q2 = 4.f * (float)a.q2;
   f.box = (float)box;
   f.child = a.child;
   f.start = a.start;
@@ -706,7 +707,11 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
 
   // ---- pack walk records
   if ((rc = ctx->alloc(reinterpret_cast<void**>(&pl->nodesD), sizeof(NodeD) * nnodes))) return rc;
-  if ((rc = ctx->alloc(reinterpret_cast<void**>(&pl->nodesF), sizeof(NodeF) * nnodes))) return rc;
+  // three zero cells after the last: the FP32 walk loads the root's
+  // siblings' positions when it starts a tree
+  if ((rc = ctx->alloc(reinterpret_cast<void**>(&pl->nodesF), sizeof(NodeF) * (nnodes + 3))))
+    return rc;
+  SLSIM_HIP_CHECK(hipMemsetAsync(pl->nodesF + nnodes, 0, 3 * sizeof(NodeF), st));
   if ((rc = ctx->alloc(reinterpret_cast<void**>(&pl->partsD), sizeof(PartD) * n))) return rc;
   if ((rc = ctx->alloc(reinterpret_cast<void**>(&pl->npart), 4 * nnodes))) return rc;
   SLSIM_HIP_CHECK(hipMemcpyAsync(pl->npart, gcnt, 4 * nnodes, hipMemcpyDeviceToDevice, st));
@@ -716,12 +721,19 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
   SLSIM_HIP_CHECK(hipEventRecord(ev1, st));
   SLSIM_HIP_CHECK(hipStreamSynchronize(st));
   SLSIM_HIP_CHECK(hipGetLastError());
+  {
+    NodeD root;
+    SLSIM_HIP_CHECK(hipMemcpy(&root, pl->nodesD, sizeof(NodeD), hipMemcpyDeviceToHost));
+    pl->mass = root.mass;
+    pl->com[0] = root.cx;
+    pl->com[1] = root.cy;
+  }
   float ms_build = 0;
   SLSIM_HIP_CHECK(hipEventElapsedTime(&ms_build, ev0, ev1));
   pl->build_ms = ms_build;
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
-  pl->bytes = nnodes * (sizeof(NodeD) + sizeof(NodeF) + 16 + 16 + 4 + 4) +
+  pl->bytes = nnodes * (sizeof(NodeD) + sizeof(NodeF) + 16 + 16 + 4 + 4) + 3 * sizeof(NodeF) +
               n * sizeof(PartD) + hpairs * sizeof(PartD);
   return SLSIM_OK;
 }

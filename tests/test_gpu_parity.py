@@ -144,6 +144,27 @@ def test_plane_force_fp32c_close_to_fp64(ctx):
     assert rel_err(f32[:, 2:] - f64[:, 2:], f64[:, 2:]).max() <= KG_TOL
 
 
+def test_plane_force_mass_sheet(ctx):
+    """inv_area != 0 (the compensating negative mass sheet): FP64 against the
+    oracle, FP32C (which adds the sheet in closed form) against FP64."""
+    x, y, m, h = W.c1_particles()
+    span = max(x.max() - x.min(), y.max() - y.min())
+    ia = 1.0 / (0.25 * span * span)
+    dev = S.LensHaloParticles(ctx, x, y, m, h, 0.5, inv_area=ia)
+    rays = _c1_rays(128)
+    f64 = dev.force(rays, S.SLSIM_FP64)
+    o = O.Tree(x, y, m, h, inv_area=ia).force(rays)
+    assert rel_err(f64[:, :2] - o[:, :2], o[:, :2]).max() < 1e-11
+    assert rel_err(f64[:, 2:] - o[:, 2:], o[:, 2:]).max() < 1e-10
+    f32 = dev.force(rays, S.SLSIM_FP32C)
+    assert rel_err(f32[:, :2] - f64[:, :2], f64[:, :2]).max() <= FP32C_ALPHA_TOL
+    assert rel_err(f32[:, 2:3] - f64[:, 2:3], f64[:, 2:3]).max() <= KG_TOL
+    assert rel_err(f32[:, 3:] - f64[:, 3:], f64[:, 3:]).max() <= KG_TOL
+    # the sheet moves kappa by -inv_area * M everywhere
+    f0 = S.LensHaloParticles(ctx, x, y, m, h, 0.5).force(rays, S.SLSIM_FP64)
+    assert np.allclose(f64[:, 2] - f0[:, 2], -ia * m.astype(np.float64).sum(), rtol=1e-9)
+
+
 def test_direct_sum_matches_oracle(ctx):
     x, y, m, h = W.c1_particles()
     dev = S.LensHaloParticles(ctx, x, y, m, h, 0.5)
