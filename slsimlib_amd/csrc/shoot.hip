@@ -25,7 +25,7 @@ namespace {
 
 __device__ inline int lane_id() { return (int)__lane_id(); }
 
-__device__ inline uint64_t ballot(bool p) { return (uint64_t)__ballot((int)p); }
+__device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 __device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -342,53 +342,6 @@ __device__ inline void bspline_fused32(float r2, float inv, float h, float& Mr2,
   }
 }
 
-// Point mass (include/quadTree.h:1019-1046) at offset d, |d|^2 = rr, for a
-// particle at least 2h from the ray: s = m / r^2,
-//   AX -= s d0, AY -= s d1, G0 += s/r^2 (d0^2 - d1^2), G1 += s/r^2 d0 d1.
-__device__ inline void point32(float d0, float d1, float d1s, float rr, float m, Acc32& a) {
-  const float inv = rcp32(rr);
-  const float s = m * inv;
-  const float w = s * inv;
-  a.pax = fmaf(-s, d0, a.pax);
-  a.pay = fmaf(-s, d1, a.pay);
-  a.pg0 = fmaf(w, fmaf(d0, d0, -d1s), a.pg0);
-  a.pg1 = fmaf(w * d0, d1, a.pg1);
-}
-
-// Smoothed particle: the point mass with the reference's B-spline
-// correction folded in (algebraically the reference's two-step form),
-// s = m M(q) / r^2, sm = pi m sigma(q); of order m / h^2, nothing large cancels.
-__device__ inline void smooth32(float d0, float d1, float d1s, float rr, float m, float h,
-                                Acc32& a) {
-  const float inv = rcp32(rr);
-  float Mr2, sigpi;
-  bspline_fused32(rr, inv, h, Mr2, sigpi);
-  const float s = m * Mr2;
-  const float sm = m * sigpi;
-  const float w = (s - sm) * inv;
-  a.pax = fmaf(-s, d0, a.pax);
-  a.pay = fmaf(-s, d1, a.pay);
-  a.pg0 = fmaf(w, fmaf(d0, d0, -d1s), a.pg0);
-  a.pg1 = fmaf(w * d0, d1, a.pg1);
-  a.pk += sm;
-}
-
-// B-spline correction alone (the reference's second step), used only when
-// a lane's near-field queue is full: (M - 1) / r^2 = Mr2 - 1/r^2.
-__device__ inline void correction32(float d0, float d1, float rr, float m, float h, Acc32& a) {
-  const float inv = rcp32(rr);
-  float Mr2, sigpi;
-  bspline_fused32(rr, inv, h, Mr2, sigpi);
-  const float am = m * (Mr2 - inv);
-  const float gt = am - m * sigpi;
-  const float w = gt * inv;
-  a.pax = fmaf(-am, d0, a.pax);
-  a.pay = fmaf(-am, d1, a.pay);
-  a.pg0 = fmaf(w, fmaf(d0, d0, -d1 * d1), a.pg0);
-  a.pg1 = fmaf(w * d0, d1, a.pg1);
-  a.pk = fmaf(m, sigpi, a.pk);
-}
-
 // The queued near-field terms, after the walk (its registers are free then),
 // in FP64 with the reference's arithmetic: a bare point mass within 2h of the
 // ray (its cell is not near, or does not list it as big, so the reference
@@ -425,67 +378,88 @@ __device__ void near_terms64(const TreeDesc* __restrict__ trees, double rx, doub
   a.nq = 0;
 }
 
-// The rest of a cell evaluation once the opening test is known.
-__device__ __forceinline__ uint64_t eval32_rest(const NodeF& nd, bool lv, bool nearc, bool open,
-                                                float ex, float ey, float r2f, double rx,
-                                                double ry, cptr<PartD> parts, cptr<PartD> bigs,
-                                                Acc32& a) {
-  const int count = nd.count_cmask & 0x0fffffff;
-  if (lv && !open) {
-    // far cell: monopole + quadrupole (stored as 4 q)
-    const float inv = rcp32(r2f);
-    const float s = nd.mass * inv;
-    const float u = inv * inv;
-    const float qx0 = fmaf(nd.q0, ex, nd.q2 * ey);
-    const float qx1 = fmaf(nd.q1, ey, nd.q2 * ex);
-    const float qq = fmaf(ex, qx0, ey * qx1);
-    const float kk = fmaf(qq * inv, u, -s);
-    const float u4 = 0.25f * u;
-    a.pax = fmaf(kk, ex, fmaf(-qx0, u4, a.pax));
-    a.pay = fmaf(kk, ey, fmaf(-qx1, u4, a.pay));
-    const float w = s * inv;
-    a.pg0 = fmaf(w, fmaf(ex, ex, -ey * ey), a.pg0);
-    a.pg1 = fmaf(w * ex, ey, a.pg1);
-  }
-  const uint64_t om = ballot(open);
-  if (om == 0) return 0;
-  if (nd.child < 0) {
-    // leaf particles; particle - ray offsets formed in FP64
-    for (int k = 0; k < count; ++k) {
-      const PartD p = parts[nd.start + k];
-      if (open) {
-        const float d0 = (float)(p.x - rx), d1 = (float)(p.y - ry);
-        const float d1s = d1 * d1;
-        const float rr = fmaxf(fmaf(d0, d0, d1s), 1e-20f);
-        const float hs = fabsf(p.h);
-        bool pt = true;
-        if (rr < 4.f * hs * hs) {
-          if (nearc && !signbit(p.h)) {
-            smooth32(d0, d1, d1s, rr, p.m, hs, a);
-            pt = false;
-          } else {
-            pt = !a.push((uint32_t)(nd.start + k));
-          }
-        }
-        if (pt) point32(d0, d1, d1s, rr, p.m, a);
+// The FP32 sums are updated with the same instructions on every path: a
+// lane that takes no part in a term gets zero coefficients (selected, never
+// formed from an infinite 1/r^2), and the rare near-field branches only
+// change coefficients; divergent branches around the sums themselves would
+// cost register copies.
+
+// Far cell: monopole + quadrupole (stored as 4 q) for the lanes with far set.
+__device__ __forceinline__ void far32(const NodeF& nd, bool far, float ex, float ey, float r2f,
+                                      Acc32& a) {
+  const float inv = rcp32(r2f);
+  const float s = nd.mass * inv;
+  const float u = inv * inv;
+  const float qx0 = fmaf(nd.q0, ex, nd.q2 * ey);
+  const float qx1 = fmaf(nd.q1, ey, nd.q2 * ex);
+  const float qq = fmaf(ex, qx0, ey * qx1);
+  const float kk = far ? fmaf(qq * inv, u, -s) : 0.f;
+  const float u4 = far ? 0.25f * u : 0.f;
+  const float w = far ? s * inv : 0.f;
+  a.pax = fmaf(kk, ex, fmaf(-qx0, u4, a.pax));
+  a.pay = fmaf(kk, ey, fmaf(-qx1, u4, a.pay));
+  a.pg0 = fmaf(w, fmaf(ex, ex, -ey * ey), a.pg0);
+  a.pg1 = fmaf(w * ex, ey, a.pg1);
+}
+
+// Particles of an opened leaf; particle - ray offsets formed in FP64.
+__device__ __forceinline__ void leaf32(int start, int count, bool open, bool nearc, double rx,
+                                       double ry, cptr<PartD> parts, Acc32& a) {
+  for (int k = 0; k < count; ++k) {
+    const PartD p = parts[start + k];
+    const float d0 = (float)(p.x - rx), d1 = (float)(p.y - ry);
+    const float d1s = d1 * d1;
+    const float rr = fmaxf(fmaf(d0, d0, d1s), 1e-20f);
+    const float hs = fabsf(p.h);
+    const float inv = rcp32(rr);
+    float s = p.m * inv, w = s * inv, sm = 0.f;     // point mass
+    if (open && rr < 4.f * hs * hs) {
+      if (nearc && !signbit(p.h)) {
+        // smoothed: the B-spline correction folded into the point mass
+        float Mr2, sigpi;
+        bspline_fused32(rr, inv, hs, Mr2, sigpi);
+        s = p.m * Mr2;
+        sm = p.m * sigpi;
+        w = (s - sm) * inv;
+      } else if (a.push((uint32_t)(start + k))) {
+        s = 0.f;    // deferred to FP64
+        w = 0.f;
       }
     }
-    return 0;
+    s = open ? s : 0.f;
+    w = open ? w : 0.f;
+    a.pax = fmaf(-s, d0, a.pax);
+    a.pay = fmaf(-s, d1, a.pay);
+    a.pg0 = fmaf(w, fmaf(d0, d0, -d1s), a.pg0);
+    a.pg1 = fmaf(w * d0, d1, a.pg1);
+    a.pk += sm;
   }
-  if (count > 0 && ballot(nearc)) {
-    // the cell's own big particles: B-spline corrections (near field, FP64)
-    for (int k = 0; k < count; ++k) {
-      const PartD p = bigs[nd.start + k];
-      if (nearc) {
-        const float d0 = (float)(p.x - rx), d1 = (float)(p.y - ry);
-        const float rr = fmaxf(d0 * d0 + d1 * d1, 1e-20f);
-        if (rr < 4.0001f * p.h * p.h && !a.push((uint32_t)(nd.start + k) | kBigListBit) &&
-            rr < 4.f * p.h * p.h)
-          correction32(d0, d1, rr, p.m, p.h, a);
-      }
+}
+
+// An internal cell's own big particles: B-spline corrections (near field,
+// queued for FP64; FP32 here only when the lane's queue is full).
+__device__ __forceinline__ void bigs32(int start, int count, bool nearc, double rx, double ry,
+                                       cptr<PartD> bigs, Acc32& a) {
+  for (int k = 0; k < count; ++k) {
+    const PartD p = bigs[start + k];
+    const float d0 = (float)(p.x - rx), d1 = (float)(p.y - ry);
+    const float rr = fmaxf(fmaf(d0, d0, d1 * d1), 1e-20f);
+    float am = 0.f, w = 0.f, sk = 0.f;
+    if (nearc && rr < 4.0001f * p.h * p.h && !a.push((uint32_t)(start + k) | kBigListBit) &&
+        rr < 4.f * p.h * p.h) {
+      const float inv = rcp32(rr);
+      float Mr2, sigpi;
+      bspline_fused32(rr, inv, p.h, Mr2, sigpi);
+      am = p.m * (Mr2 - inv);            // m (M - 1) / r^2
+      sk = p.m * sigpi;
+      w = (am - sk) * inv;
     }
+    a.pax = fmaf(-am, d0, a.pax);
+    a.pay = fmaf(-am, d1, a.pay);
+    a.pg0 = fmaf(w, fmaf(d0, d0, -d1 * d1), a.pg0);
+    a.pg1 = fmaf(w * d0, d1, a.pg1);
+    a.pk += sk;
   }
-  return om;
 }
 
 // Walk of the trees of one plane, FP32 arithmetic with compensated
@@ -525,7 +499,8 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
     const bool lv = (live >> lane) & 1ull;
     const int base = entry & 0x0fffffff;
     const int cm = (entry >> 28) & 0xf;
-    // opening tests of the four children first: their loads are issued together
+    // the four children: opening tests (their loads issued together), then
+    // far terms; leaves and big-particle lists after, in one loop each
     float exk[4], eyk[4], r2k[4];
     bool nk[4], ok[4];
 #pragma unroll
@@ -543,15 +518,43 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
     }
     int32_t pe[4];
     uint64_t pm[4];
+    int leaves = 0, lists = 0;    // children (bits) with particles or a big list to take
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       pm[k] = 0;
       pe[k] = 0;
       if ((cm >> k) & 1) {
         const NodeF nd = nodes[base + k];
-        pm[k] = eval32_rest(nd, lv, nk[k], ok[k], exk[k], eyk[k], r2k[k], rx, ry, parts, bigs,
-                            a);
-        pe[k] = nd.child | (nd.count_cmask & 0xf0000000);
+        const bool far = lv && !ok[k];
+        if (ballot(far)) far32(nd, far, exk[k], eyk[k], r2k[k], a);
+        const uint64_t om = ballot(ok[k]);
+        if (om) {
+          if (nd.child < 0) {
+            leaves |= 1 << k;
+          } else {
+            pm[k] = om;
+            pe[k] = nd.child | (nd.count_cmask & 0xf0000000);
+            if ((nd.count_cmask & 0x0fffffff) > 0 && ballot(nk[k])) lists |= 1 << k;
+          }
+        }
+      }
+    }
+    const int opened = (ok[0] ? 1 : 0) | (ok[1] ? 2 : 0) | (ok[2] ? 4 : 0) | (ok[3] ? 8 : 0);
+    const int nearm = (nk[0] ? 1 : 0) | (nk[1] ? 2 : 0) | (nk[2] ? 4 : 0) | (nk[3] ? 8 : 0);
+#pragma nounroll
+    for (int k = 0; k < 4; ++k) {
+      if ((leaves >> k) & 1) {
+        const int st = nodes[base + k].start;
+        const int cnt = nodes[base + k].count_cmask & 0x0fffffff;
+        leaf32(st, cnt, (opened >> k) & 1, (nearm >> k) & 1, rx, ry, parts, a);
+      }
+    }
+#pragma nounroll
+    for (int k = 0; k < 4; ++k) {
+      if ((lists >> k) & 1) {
+        const int st = nodes[base + k].start;
+        const int cnt = nodes[base + k].count_cmask & 0x0fffffff;
+        bigs32(st, cnt, (nearm >> k) & 1, rx, ry, bigs, a);
       }
     }
     // one compensated add per entry, by the lanes the entry holds: a lane's
