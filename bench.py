@@ -201,6 +201,24 @@ def run_precision_form(S, W, ctx, n=1 << 22, nplanes=20, N=4096):
     x = S.GridMap(lens, N, (0.0, 0.0), rg, shoot=False).positions()
     out["fp32c_vs_fp64"] = compare(x, res["fp32c"][0], res["fp32c"][1], res["fp64"][0],
                                    res["fp64"][1])
+    # the rays where FP32C's deflection differs most, against the FP64 path's
+    # own response to a 1e-7-of-rms nudge of those rays (conditioning)
+    a32 = x - res["fp32c"][0]
+    a64 = x - res["fp64"][0]
+    n64 = np.linalg.norm(a64, axis=1)
+    ea = np.linalg.norm(a32 - a64, axis=1) / np.maximum(n64, 1e-3 * np.sqrt(np.mean(n64 ** 2)))
+    worst = np.argsort(ea)[-16:]
+    amp = 1e-7 * np.sqrt(np.mean(n64 ** 2))
+    resp = []
+    for k in range(4):
+        phi = 0.5 * np.pi * k + 0.3
+        xn = x[worst] + amp * np.array([np.cos(phi), np.sin(phi)])
+        yn, _ = lens.rayshooterInternal(xn, precision=S.SLSIM_FP64)
+        resp.append(np.linalg.norm((xn - yn) - a64[worst], axis=1) / np.maximum(
+            n64[worst], 1e-3 * np.sqrt(np.mean(n64 ** 2))))
+    out["alpha_rays_over_1e-5"] = int((ea > 1e-5).sum())
+    out["alpha_worst_rays"] = {"fp32c_err": ea[worst].tolist(),
+                               "fp64_nudge_response_max": np.max(resp, 0).tolist()}
     sub = np.arange(0, N * N, 256)
     t1 = time.perf_counter()
     yd, Ad = lens.rayshooterInternal(x[sub], precision=S.SLSIM_DIRECT64)
