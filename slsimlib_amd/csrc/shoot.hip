@@ -27,6 +27,13 @@ __device__ inline int lane_id() { return (int)__lane_id(); }
 
 __device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
+// Lane masks kept in scalar registers: a compare's mask without a vector
+// round trip, and a mask back to a per-lane condition at no cost.
+__device__ inline uint64_t lanes_lt(double a, double b) {
+  return __builtin_amdgcn_fcmp(a, b, 4 /* ordered less-than */);
+}
+__device__ inline bool lane_in(uint64_t mask) { return __builtin_amdgcn_inverse_ballot_w64(mask); }
+
 __device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __device__ inline uint64_t uni64(uint64_t v) {
@@ -384,18 +391,19 @@ __device__ void near_terms64(const TreeDesc* __restrict__ trees, double rx, doub
 // change coefficients; divergent branches around the sums themselves would
 // cost register copies.
 
-// Far cell: monopole + quadrupole (stored as 4 q) for the lanes with far set.
+// Far cell: monopole + quadrupole (stored as 4 q) for the lanes with far
+// set; the others get 1/r^2 = 0, which zeroes every term.
 __device__ __forceinline__ void far32(const NodeF& nd, bool far, float ex, float ey, float r2f,
                                       Acc32& a) {
-  const float inv = rcp32(r2f);
+  const float inv = far ? rcp32(r2f) : 0.f;
   const float s = nd.mass * inv;
   const float u = inv * inv;
   const float qx0 = fmaf(nd.q0, ex, nd.q2 * ey);
   const float qx1 = fmaf(nd.q1, ey, nd.q2 * ex);
   const float qq = fmaf(ex, qx0, ey * qx1);
-  const float kk = far ? fmaf(qq * inv, u, -s) : 0.f;
-  const float u4 = far ? 0.25f * u : 0.f;
-  const float w = far ? s * inv : 0.f;
+  const float kk = fmaf(qq * inv, u, -s);
+  const float u4 = 0.25f * u;
+  const float w = s * inv;
   a.pax = fmaf(kk, ex, fmaf(-qx0, u4, a.pax));
   a.pay = fmaf(kk, ey, fmaf(-qx1, u4, a.pay));
   a.pg0 = fmaf(w, fmaf(ex, ex, -ey * ey), a.pg0);
@@ -405,13 +413,14 @@ __device__ __forceinline__ void far32(const NodeF& nd, bool far, float ex, float
 // Particles of an opened leaf; particle - ray offsets formed in FP64.
 __device__ __forceinline__ void leaf32(int start, int count, bool open, bool nearc, double rx,
                                        double ry, cptr<PartD> parts, Acc32& a) {
+#pragma unroll 2
   for (int k = 0; k < count; ++k) {
     const PartD p = parts[start + k];
     const float d0 = (float)(p.x - rx), d1 = (float)(p.y - ry);
     const float d1s = d1 * d1;
     const float rr = fmaxf(fmaf(d0, d0, d1s), 1e-20f);
     const float hs = fabsf(p.h);
-    const float inv = rcp32(rr);
+    const float inv = open ? rcp32(rr) : 0.f;       // 0: no term for this lane
     float s = p.m * inv, w = s * inv, sm = 0.f;     // point mass
     if (open && rr < 4.f * hs * hs) {
       if (nearc && !signbit(p.h)) {
@@ -426,8 +435,6 @@ __device__ __forceinline__ void leaf32(int start, int count, bool open, bool nea
         w = 0.f;
       }
     }
-    s = open ? s : 0.f;
-    w = open ? w : 0.f;
     a.pax = fmaf(-s, d0, a.pax);
     a.pay = fmaf(-s, d1, a.pay);
     a.pg0 = fmaf(w, fmaf(d0, d0, -d1s), a.pg0);
@@ -496,57 +503,54 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
       live = uni64(s_mask[sp]);
       __builtin_amdgcn_wave_barrier();
     }
-    const bool lv = (live >> lane) & 1ull;
+    const bool lv = lane_in(live);
     const int base = entry & 0x0fffffff;
     const int cm = (entry >> 28) & 0xf;
-    // the four children: opening tests (their loads issued together), then
-    // far terms; leaves and big-particle lists after, in one loop each
+    // the four children: opening tests (their loads issued together, lane
+    // masks in scalar registers), then far terms; leaves and big-particle
+    // lists after, in one loop each
     float exk[4], eyk[4], r2k[4];
-    bool nk[4], ok[4];
+    uint64_t om[4], nm[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const double cx = nodes[base + k].cx, cy = nodes[base + k].cy;
       const double rc = nodes[base + k].r2crit, ne = nodes[base + k].near2;
       const double dx = cx - rx, dy = cy - ry;
       const double r2 = dx * dx + dy * dy;
-      const bool lk = lv && ((cm >> k) & 1);
-      nk[k] = lk && (r2 < ne);
-      ok[k] = lk && ((r2 < rc) || nk[k]);
+      const uint64_t lk = ((cm >> k) & 1) ? live : 0ull;
+      nm[k] = lanes_lt(r2, ne) & lk;
+      om[k] = (lanes_lt(r2, rc) | nm[k]) & lk;
       exk[k] = (float)dx;
       eyk[k] = (float)dy;
       r2k[k] = (float)r2;
     }
     int32_t pe[4];
-    uint64_t pm[4];
     int leaves = 0, lists = 0;    // children (bits) with particles or a big list to take
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      pm[k] = 0;
       pe[k] = 0;
       if ((cm >> k) & 1) {
         const NodeF nd = nodes[base + k];
-        const bool far = lv && !ok[k];
-        if (ballot(far)) far32(nd, far, exk[k], eyk[k], r2k[k], a);
-        const uint64_t om = ballot(ok[k]);
-        if (om) {
+        const uint64_t fm = live & ~om[k];
+        if (fm) far32(nd, lane_in(fm), exk[k], eyk[k], r2k[k], a);
+        if (om[k]) {
           if (nd.child < 0) {
             leaves |= 1 << k;
           } else {
-            pm[k] = om;
             pe[k] = nd.child | (nd.count_cmask & 0xf0000000);
-            if ((nd.count_cmask & 0x0fffffff) > 0 && ballot(nk[k])) lists |= 1 << k;
+            if ((nd.count_cmask & 0x0fffffff) > 0 && nm[k]) lists |= 1 << k;
           }
         }
       }
     }
-    const int opened = (ok[0] ? 1 : 0) | (ok[1] ? 2 : 0) | (ok[2] ? 4 : 0) | (ok[3] ? 8 : 0);
-    const int nearm = (nk[0] ? 1 : 0) | (nk[1] ? 2 : 0) | (nk[2] ? 4 : 0) | (nk[3] ? 8 : 0);
 #pragma nounroll
     for (int k = 0; k < 4; ++k) {
       if ((leaves >> k) & 1) {
         const int st = nodes[base + k].start;
         const int cnt = nodes[base + k].count_cmask & 0x0fffffff;
-        leaf32(st, cnt, (opened >> k) & 1, (nearm >> k) & 1, rx, ry, parts, a);
+        const uint64_t o = k == 0 ? om[0] : k == 1 ? om[1] : k == 2 ? om[2] : om[3];
+        const uint64_t n = k == 0 ? nm[0] : k == 1 ? nm[1] : k == 2 ? nm[2] : nm[3];
+        leaf32(st, cnt, lane_in(o), lane_in(n), rx, ry, parts, a);
       }
     }
 #pragma nounroll
@@ -554,7 +558,8 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
       if ((lists >> k) & 1) {
         const int st = nodes[base + k].start;
         const int cnt = nodes[base + k].count_cmask & 0x0fffffff;
-        bigs32(st, cnt, (nearm >> k) & 1, rx, ry, bigs, a);
+        const uint64_t n = k == 0 ? nm[0] : k == 1 ? nm[1] : k == 2 ? nm[2] : nm[3];
+        bigs32(st, cnt, lane_in(n), rx, ry, bigs, a);
       }
     }
     // one compensated add per entry, by the lanes the entry holds: a lane's
@@ -562,7 +567,7 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
     if (lv) a.flush();
 #pragma unroll
     for (int k = 3; k >= 0; --k)
-      if (pm[k]) push_entry(s_node, s_mask, sp, lane, pe[k], pm[k], err);
+      if (pe[k]) push_entry(s_node, s_mask, sp, lane, pe[k], om[k], err);
   }
   near_terms64(trees, rx, ry, a);
 }
