@@ -342,7 +342,7 @@ def main():
             traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
         out["roofline"] = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                           "traffic": traffic, "kernel": "k_plane_step<FP32C> (one lens plane)",
+                           "traffic": traffic, "kernel": "k_plane_step32 (FP32C walk + recursion, one lens plane)",
                            "flops_per_launch": per_launch,
                            "gather_bytes_per_launch": gather_bytes / tab["nplanes"],
                            "launch_ms": walk_launch_ms}

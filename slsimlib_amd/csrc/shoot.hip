@@ -277,10 +277,10 @@ __device__ void walk64(const TreeDesc& t, double rx, double ry, bool active, Acc
 // ---------------------------------------------------------------- FP32 walk
 // The FP32 walk sums pi times the reference's terms, with 1/pi, the signs
 // and the factor 2 of gamma1 taken out of the per-term arithmetic:
-//   AX, AY = pi sum alpha,  K = pi sum kappa (B-spline terms),
-//   G0 = -pi sum gamma0,    G1 = -(pi/2) sum gamma1,
-// restored once per walk in FP64 (finish32).  Cell quadrupoles are stored
-// as 4 q.  The mass sheet (inv_area: quadTree.h:861-888, 1035-1044) adds
+//   AX, AY = pi sum alpha,  G0 = -pi sum gamma0,  G1 = -(pi/2) sum gamma1,
+// restored once per walk in FP64 (finish32).  Kappa has no point-mass term
+// besides the sheet: its B-spline terms (pi sigma m, rare) go straight to an
+// FP64 sum.  Cell quadrupoles are stored as 4 q.  The mass sheet (inv_area: quadTree.h:861-888, 1035-1044) adds
 // inv_area m (c - x) to alpha and -inv_area m to kappa for every cell
 // monopole and point mass; every particle enters exactly once as one or the
 // other, so the sheet's total is inv_area M (c - x) and -inv_area M with M, c
@@ -300,23 +300,25 @@ struct CSum {   // Kahan-compensated FP32 sum
 // Near-field interactions deferred to FP64 (per lane, in LDS): the tree
 // index in the high word, the particle index in the low word with its top
 // bit set for an internal big-particle list entry.
-constexpr int kNearQueue = 8;
+constexpr int kNearQueue = 7;   // 7: the block's LDS stays under 20 KiB (8 blocks per CU)
 constexpr uint32_t kBigListBit = 0x80000000u;
 
 struct Acc32 {
-  CSum ax, ay, k, g0, g1;
+  CSum ax, ay, g0, g1;
   // plain partial sums of one popped stack entry (four cells), then one
   // compensated add per entry
-  float pax = 0.f, pay = 0.f, pk = 0.f, pg0 = 0.f, pg1 = 0.f;
+  float pax = 0.f, pay = 0.f, pg0 = 0.f, pg1 = 0.f;
   uint64_t* q = nullptr;   // this lane's queue: q[0], q[64], ... q[64 * (kNearQueue - 1)]
   int nq = 0;
   uint32_t tree = 0;       // index of the tree being walked (uniform)
-  // FP64 sums of the queued terms, in the reference's scale and signs
+  // FP64 sums of the queued terms and of the B-spline kappa terms (the only
+  // kappa terms besides the mass sheet), in the reference's scale and signs
   double fx = 0, fy = 0, fk = 0, fg0 = 0, fg1 = 0;
   __device__ inline void flush() {
-    ax.add(pax); ay.add(pay); k.add(pk); g0.add(pg0); g1.add(pg1);
-    pax = pay = pk = pg0 = pg1 = 0.f;
+    ax.add(pax); ay.add(pay); g0.add(pg0); g1.add(pg1);
+    pax = pay = pg0 = pg1 = 0.f;
   }
+  __device__ inline void kappa(float sigma_pi_m) { fk += kInvPI * (double)sigma_pi_m; }
   __device__ inline bool push(uint32_t idx) {
     if (nq >= kNearQueue) return false;
     q[64 * nq] = ((uint64_t)tree << 32) | idx;
@@ -421,15 +423,16 @@ __device__ __forceinline__ void leaf32(int start, int count, bool open, bool nea
     const float rr = fmaxf(fmaf(d0, d0, d1s), 1e-20f);
     const float hs = fabsf(p.h);
     const float inv = open ? rcp32(rr) : 0.f;       // 0: no term for this lane
-    float s = p.m * inv, w = s * inv, sm = 0.f;     // point mass
+    float s = p.m * inv, w = s * inv;              // point mass
     if (open && rr < 4.f * hs * hs) {
       if (nearc && !signbit(p.h)) {
         // smoothed: the B-spline correction folded into the point mass
         float Mr2, sigpi;
         bspline_fused32(rr, inv, hs, Mr2, sigpi);
         s = p.m * Mr2;
-        sm = p.m * sigpi;
+        const float sm = p.m * sigpi;
         w = (s - sm) * inv;
+        a.kappa(sm);
       } else if (a.push((uint32_t)(start + k))) {
         s = 0.f;    // deferred to FP64
         w = 0.f;
@@ -439,7 +442,6 @@ __device__ __forceinline__ void leaf32(int start, int count, bool open, bool nea
     a.pay = fmaf(-s, d1, a.pay);
     a.pg0 = fmaf(w, fmaf(d0, d0, -d1s), a.pg0);
     a.pg1 = fmaf(w * d0, d1, a.pg1);
-    a.pk += sm;
   }
 }
 
@@ -451,21 +453,21 @@ __device__ __forceinline__ void bigs32(int start, int count, bool nearc, double 
     const PartD p = bigs[start + k];
     const float d0 = (float)(p.x - rx), d1 = (float)(p.y - ry);
     const float rr = fmaxf(fmaf(d0, d0, d1 * d1), 1e-20f);
-    float am = 0.f, w = 0.f, sk = 0.f;
+    float am = 0.f, w = 0.f;
     if (nearc && rr < 4.0001f * p.h * p.h && !a.push((uint32_t)(start + k) | kBigListBit) &&
         rr < 4.f * p.h * p.h) {
       const float inv = rcp32(rr);
       float Mr2, sigpi;
       bspline_fused32(rr, inv, p.h, Mr2, sigpi);
       am = p.m * (Mr2 - inv);            // m (M - 1) / r^2
-      sk = p.m * sigpi;
+      const float sk = p.m * sigpi;
       w = (am - sk) * inv;
+      a.kappa(sk);
     }
     a.pax = fmaf(-am, d0, a.pax);
     a.pay = fmaf(-am, d1, a.pay);
     a.pg0 = fmaf(w, fmaf(d0, d0, -d1 * d1), a.pg0);
     a.pg1 = fmaf(w * d0, d1, a.pg1);
-    a.pk += sk;
   }
 }
 
@@ -481,28 +483,14 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
   a.nq = 0;
   const uint64_t amask = ballot(active);
   if (amask == 0) return;
-  cptr<NodeF> nodes = nullptr;
-  cptr<PartD> parts = nullptr, bigs = nullptr;
-  int h = -1, sp = 0;
+  int h = 0, sp = 0;
+  cptr<NodeF> nodes = as_const(as_const(trees)->nodesF);
+  cptr<PartD> parts = as_const(as_const(trees)->partsD);
+  cptr<PartD> bigs = as_const(as_const(trees)->bigD);
+  a.tree = 0;
+  int entry = 1 << 28;            // base 0, child mask {0}: the root
+  uint64_t live = amask;
   while (true) {
-    int entry;
-    uint64_t live;
-    if (sp == 0) {
-      if (++h >= ntrees) break;
-      const cptr<TreeDesc> td = as_const(trees) + h;
-      nodes = as_const(td->nodesF);
-      parts = as_const(td->partsD);
-      bigs = as_const(td->bigD);
-      a.tree = (uint32_t)h;
-      entry = 1 << 28;            // base 0, child mask {0}: the root
-      live = amask;
-    } else {
-      --sp;
-      __builtin_amdgcn_wave_barrier();
-      entry = uni(s_node[sp]);
-      live = uni64(s_mask[sp]);
-      __builtin_amdgcn_wave_barrier();
-    }
     const bool lv = lane_in(live);
     const int base = entry & 0x0fffffff;
     const int cm = (entry >> 28) & 0xf;
@@ -568,6 +556,22 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
 #pragma unroll
     for (int k = 3; k >= 0; --k)
       if (pe[k]) push_entry(s_node, s_mask, sp, lane, pe[k], om[k], err);
+    if (sp > 0) {
+      --sp;
+      __builtin_amdgcn_wave_barrier();
+      entry = uni(s_node[sp]);
+      live = uni64(s_mask[sp]);
+      __builtin_amdgcn_wave_barrier();
+    } else {
+      if (++h >= ntrees) break;
+      const cptr<TreeDesc> td = as_const(trees) + h;
+      nodes = as_const(td->nodesF);
+      parts = as_const(td->partsD);
+      bigs = as_const(td->bigD);
+      a.tree = (uint32_t)h;
+      entry = 1 << 28;
+      live = amask;
+    }
   }
   near_terms64(trees, rx, ry, a);
 }
@@ -577,7 +581,7 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
 __device__ inline void finish32(const Acc32& a, double* out) {
   out[0] = kInvPI * a.ax.value() + a.fx;
   out[1] = kInvPI * a.ay.value() + a.fy;
-  out[2] = kInvPI * a.k.value() + a.fk;
+  out[2] = a.fk;
   out[3] = -kInvPI * a.g0.value() + a.fg0;
   out[4] = -2.0 * kInvPI * a.g1.value() + a.fg1;
 }
@@ -727,9 +731,9 @@ __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, W
 // One plane step for every bundle of the listed jobs.  Grid: one wavefront
 // per work item, 4 per block (items == null: job 0, bundle = item index).
 template <int MODE, bool COUNT>
-__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
-    const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t nitems,
-    uint32_t* __restrict__ err) {
+__device__ __forceinline__ void plane_step_grid(const StepJob* __restrict__ jobs,
+                                                const WorkItem* __restrict__ items,
+                                                uint32_t nitems, uint32_t* __restrict__ err) {
   __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
   __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
   __shared__ uint64_t s_near[kWavesPerBlock][kNearQueue * kWave];
@@ -745,17 +749,13 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
   process_item<MODE, COUNT>(jobs, it, s_node[w], s_mask[w], s_near[w], err);
 }
 
-template __global__ void k_plane_step<SLSIM_FP64, false>(const StepJob*, const WorkItem*, uint32_t, uint32_t*);
-template __global__ void k_plane_step<SLSIM_FP32C, false>(const StepJob*, const WorkItem*, uint32_t, uint32_t*);
-template __global__ void k_plane_step<SLSIM_DIRECT64, false>(const StepJob*, const WorkItem*, uint32_t, uint32_t*);
-template __global__ void k_plane_step<SLSIM_FP64, true>(const StepJob*, const WorkItem*, uint32_t, uint32_t*);
-
 // Persistent variant: a fixed grid whose wavefronts take work items from a
 // queue counter until the list is exhausted (every wave reaches the exit).
 template <int MODE>
-__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step_queue(
-    const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t nitems,
-    uint32_t* __restrict__ counter, uint32_t* __restrict__ err) {
+__device__ __forceinline__ void plane_step_queue(const StepJob* __restrict__ jobs,
+                                                 const WorkItem* __restrict__ items,
+                                                 uint32_t nitems, uint32_t* __restrict__ counter,
+                                                 uint32_t* __restrict__ err) {
   __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
   __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
   __shared__ uint64_t s_near[kWavesPerBlock][kNearQueue * kWave];
@@ -771,6 +771,35 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step_queue(
     it.bundle = as_const(items)[idx].bundle;
     process_item<MODE, false>(jobs, it, s_node[w], s_mask[w], s_near[w], err);
   }
+}
+
+template <int MODE, bool COUNT>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
+    const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t nitems,
+    uint32_t* __restrict__ err) {
+  plane_step_grid<MODE, COUNT>(jobs, items, nitems, err);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step_queue(
+    const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t nitems,
+    uint32_t* __restrict__ counter, uint32_t* __restrict__ err) {
+  plane_step_queue<MODE>(jobs, items, nitems, counter, err);
+}
+
+// The FP32C walk at 8 waves per SIMD (64 VGPRs): measured faster than 7
+// waves without the few spills the cap costs.
+__global__ __launch_bounds__(kWave * kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(8)))
+void k_plane_step32(const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items,
+                    uint32_t nitems, uint32_t* __restrict__ err) {
+  plane_step_grid<SLSIM_FP32C, false>(jobs, items, nitems, err);
+}
+
+__global__ __launch_bounds__(kWave * kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(8)))
+void k_plane_step_queue32(const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items,
+                          uint32_t nitems, uint32_t* __restrict__ counter,
+                          uint32_t* __restrict__ err) {
+  plane_step_queue<SLSIM_FP32C>(jobs, items, nitems, counter, err);
 }
 
 __global__ void k_zero_u32(uint32_t* __restrict__ p, int n) {
@@ -1060,7 +1089,7 @@ void launch_plane_step(int precision, bool count, uint32_t nitems, hipStream_t s
   if (count)
     k_plane_step<SLSIM_FP64, true><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
   else if (precision == SLSIM_FP32C)
-    k_plane_step<SLSIM_FP32C, false><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+    k_plane_step32<<<nb, blk, 0, st>>>(jobs, items, nitems, err);
   else if (precision == SLSIM_DIRECT64)
     k_plane_step<SLSIM_DIRECT64, false><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
   else
@@ -1077,7 +1106,7 @@ void launch_plane_step_queue(int precision, unsigned nblocks, uint32_t nitems, h
   launch_zero_u32(st, counter, 1);
   const dim3 blk(kWave * kWavesPerBlock);
   if (precision == SLSIM_FP32C)
-    k_plane_step_queue<SLSIM_FP32C><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter, err);
+    k_plane_step_queue32<<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter, err);
   else if (precision == SLSIM_DIRECT64)
     k_plane_step_queue<SLSIM_DIRECT64><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter, err);
   else
