@@ -308,7 +308,7 @@ struct Acc32 {
   // plain partial sums of one popped stack entry (four cells), then one
   // compensated add per entry
   float pax = 0.f, pay = 0.f, pg0 = 0.f, pg1 = 0.f;
-  uint64_t* q = nullptr;   // this lane's queue: q[0], q[64], ... q[64 * (kNearQueue - 1)]
+  uint64_t* q = nullptr;   // the wave's queues: lane l's at q[l], q[64 + l], ...
   int nq = 0;
   uint32_t tree = 0;       // index of the tree being walked (uniform)
   // FP64 sums of the queued terms and of the B-spline kappa terms (the only
@@ -321,7 +321,7 @@ struct Acc32 {
   __device__ inline void kappa(float sigma_pi_m) { fk += kInvPI * (double)sigma_pi_m; }
   __device__ inline bool push(uint32_t idx) {
     if (nq >= kNearQueue) return false;
-    q[64 * nq] = ((uint64_t)tree << 32) | idx;
+    q[64 * nq + lane_id()] = ((uint64_t)tree << 32) | idx;
     ++nq;
     return true;
   }
@@ -351,6 +351,41 @@ __device__ inline void bspline_fused32(float r2, float inv, float h, float& Mr2,
   }
 }
 
+// The reference's B-spline correction (bspline64's formulas) with the
+// divisions taken as reciprocals: FP64-accurate, and light on registers
+// after the FP32 walk.
+__device__ inline void bspline64r(double d0, double d1, double rr, double Mass, double size,
+                                  Acc32& a) {
+  const double is = rcp64(size), is2 = is * is;
+  const double r = sqrt(rr);
+  const double ir = rcp64(r);
+  const double q = r * is;
+  double M, sigma;
+  if (q > 2) {
+    sigma = 0;
+    M = 1;
+  } else {
+    const double q2 = q * q, q3 = q2 * q;
+    sigma = (8 - 12 * q + 6 * q2 - q3) * 0.25;
+    if (q > 1) {
+      sigma *= (10.0 / 7.0) * kInvPI * is2;
+      M = (-1 + 20 * q2 * (1 - q + 0.375 * q2 - 0.05 * q3)) * (1.0 / 7.0);
+    } else {
+      sigma = (10.0 / 7.0) * kInvPI * is2 * (sigma - 1 + 3 * q - 3 * q2 + q3);
+      M = (10.0 / 7.0) * q2 * (1 - 0.75 * q + 0.3 * q3);
+    }
+  }
+  const double alpha_r = (M - 1) * kInvPI * ir;
+  const double gt = alpha_r * ir - sigma;
+  const double ma = Mass * alpha_r * ir;
+  a.fx -= ma * d0;
+  a.fy -= ma * d1;
+  const double mg = gt * Mass * ir * ir;
+  a.fg0 -= mg * (d0 * d0 - d1 * d1);
+  a.fg1 -= mg * 2 * d0 * d1;
+  a.fk += Mass * sigma;
+}
+
 // The queued near-field terms, after the walk (its registers are free then),
 // in FP64 with the reference's arithmetic: a bare point mass within 2h of the
 // ray (its cell is not near, or does not list it as big, so the reference
@@ -361,7 +396,7 @@ __device__ inline void bspline_fused32(float r2, float inv, float h, float& Mr2,
 __device__ void near_terms64(const TreeDesc* __restrict__ trees, double rx, double ry,
                              Acc32& a) {
   for (int i = 0; i < a.nq; ++i) {
-    const uint64_t e = a.q[64 * i];
+    const uint64_t e = a.q[64 * i + lane_id()];
     const uint32_t idx = (uint32_t)e;
     const TreeDesc* t = trees + (uint32_t)(e >> 32);
     const bool big = (idx & kBigListBit) != 0;
@@ -380,8 +415,7 @@ __device__ void near_terms64(const TreeDesc* __restrict__ trees, double rx, doub
       a.fg1 += d0 * d1 * tmp;
     } else {
       const double size = p.h;
-      if (rr < 4 * size * size)
-        bspline64(d0, d1, sqrt(rr), m, size, a.fx, a.fy, a.fk, a.fg0, a.fg1);
+      if (rr < 4 * size * size) bspline64r(d0, d1, rr, m, size, a);
     }
   }
   a.nq = 0;
@@ -479,7 +513,7 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
                        bool active, Acc32& a, int32_t* s_node, uint64_t* s_mask,
                        uint64_t* s_near, uint32_t* err) {
   const int lane = lane_id();
-  a.q = s_near + lane;
+  a.q = s_near;
   a.nq = 0;
   const uint64_t amask = ballot(active);
   if (amask == 0) return;
