@@ -366,6 +366,18 @@ def main():
             "fp64_tree": compare(x[sub], y64[sub], A64[sub], yd, Ad)}
         log("vs direct sum:", out["error_vs_fp64_direct_sum"])
 
+        # host-buffer path (slsim_shoot): rays in and y, A out over PCIe, rays
+        # Morton-ordered into bundles on the device; a ray's result does not
+        # depend on its bundle mates, so it equals the GridMap's bit for bit
+        lens.rayshooterInternal(x[:65536], precision=S.SLSIM_FP32C)
+        t0 = time.perf_counter()
+        yh, Ah = lens.rayshooterInternal(x, precision=S.SLSIM_FP32C)
+        th = time.perf_counter() - t0
+        out["host_buffers_fp32c"] = {
+            "rays_per_s": nrays / th, "seconds": th,
+            "bitwise_equal_to_gridmap": bool(np.array_equal(yh, y32) and np.array_equal(Ah, A32))}
+        log("host buffers:", out["host_buffers_fp32c"])
+
         # conditioning of kappa/gamma: the FP64 tree's own response when the
         # input rays move by 1e-7 of the rms deflection (the size of one FP32
         # rounding of a plane's deflection), to read fp32c_vs_fp64 against
@@ -402,8 +414,17 @@ def main():
                 break
         tcpu = float(np.median(times))
         cpu_rate = idx.size / tcpu
+        model = "unknown CPU"
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
         out["cpu_baseline"] = {"value": cpu_rate, "unit": "rays/s", "cores": nthreads,
-                               "kind": "port",
+                               "kind": "port", "cpu": model,
                                "sample": "%d rays (every %dth ray of the %dx%d grid), all 10 "
                                          "planes, median of %d runs; tree build %.1f s "
                                          "excluded" % (idx.size, stride, N, N, len(times),
