@@ -226,6 +226,29 @@ class LensHaloParticles {
     return p;
   }
 
+  // particle_halo.h:683-730: pp[i].Size = distance to the (Nsmooth+1)-th
+  // nearest particle in 3-D, itself included, computed on the device.
+  static void calculate_smoothing(int Nsmooth, PType* pp, size_t Npoints, bool verbose = false) {
+    (void)verbose;
+    using T = typename std::remove_reference<decltype(pp[0][0])>::type;
+    std::vector<T> x(Npoints), y(Npoints), z(Npoints);
+    for (size_t i = 0; i < Npoints; ++i) {
+      x[i] = pp[i][0];
+      y[i] = pp[i][1];
+      z[i] = pp[i][2];
+    }
+    std::vector<float> size(Npoints);
+    if constexpr (sizeof(T) == sizeof(float))
+      slsim_cpp::check(slsim_smoothing_sizes_f32(slsim_cpp::context(), (const float*)x.data(),
+                                                 (const float*)y.data(), (const float*)z.data(),
+                                                 Npoints, Nsmooth, size.data()));
+    else
+      slsim_cpp::check(slsim_smoothing_sizes_f64(slsim_cpp::context(), (const double*)x.data(),
+                                                 (const double*)y.data(), (const double*)z.data(),
+                                                 Npoints, Nsmooth, size.data()));
+    for (size_t i = 0; i < Npoints; ++i) pp[i].Size = size[i];
+  }
+
  private:
   // set_up, particle_halo.h:341-417
   void set_up(Point_2d theta_rotate, bool recenter, double theta_force) {

@@ -238,6 +238,21 @@ int slsim_server_destroy(slsim_server s);
 /* Monotonic clock (seconds) of the request timestamps. */
 double slsim_now(void);
 
+/* ------------------------------------------------------ smoothing sizes */
+/* Replaces LensHaloParticles<PType>::calculate_smoothing(int Nsmooth, PType* pp,
+ * size_t Npoints, bool verbose) (include/particle_halo.h:683-730), which sets
+ * pp[i].Size = TreeSimple::NNDistance(x_i, Nsmooth + 1) (include/simpleTree.h:
+ * 483-620): size_out[i] = distance from particle i to its (nsmooth+1)-th
+ * nearest particle in 3-D, itself included, with the differences taken in the
+ * coordinate type (double or float) and squared in double, as the reference
+ * does.  Positions are host arrays of n; nsmooth in [1, 119].  Errors:
+ * SLSIM_ERR_ARG when n <= nsmooth + 1 (the reference exits: "number of
+ * neighbors > total number of particles"). */
+int slsim_smoothing_sizes_f64(slsim_ctx ctx, const double* x, const double* y, const double* z,
+                              size_t n, int nsmooth, float* size_out);
+int slsim_smoothing_sizes_f32(slsim_ctx ctx, const float* x, const float* y, const float* z,
+                              size_t n, int nsmooth, float* size_out);
+
 /* ------------------------------------------------------- device buffers */
 int slsim_device_alloc(slsim_ctx ctx, size_t bytes, void** dev);
 int slsim_device_free(slsim_ctx ctx, void* dev);

@@ -41,6 +41,10 @@ _sigs = {
     "orc_tree_force": (None, [c_vp, P(c_d), c_sz, P(c_d), c_vp, c_int]),
     "orc_direct_plane": (None, [P(c_d), P(c_d), P(c_f), P(c_f), c_sz, c_d, P(c_d), c_sz, P(c_d),
                                 c_int]),
+    "orc_smoothing_f64": (c_int, [P(c_d), P(c_d), P(c_d), c_sz, c_int, c_int, P(c_f), c_int,
+                                  P(ctypes.c_int64), c_sz]),
+    "orc_smoothing_f32": (c_int, [P(c_f), P(c_f), P(c_f), c_sz, c_int, c_int, P(c_f), c_int,
+                                  P(ctypes.c_int64), c_sz]),
     "orc_bspline": (None, [P(c_d), c_d, c_d, c_d, P(c_d)]),
     "orc_lens_create": (c_vp, [c_d, c_d, c_d, c_d]),
     "orc_lens_free": (None, [c_vp]),
@@ -115,6 +119,27 @@ def direct_sum(x, y, m, h, rays, inv_area=0.0, nthreads=8):
     out = np.zeros((r.shape[0], 5))
     _lib.orc_direct_plane(_p(x), _p(y), _p(m, c_f), _p(h, c_f), m.size, inv_area, _p(r),
                           r.shape[0], _p(out), nthreads)
+    return out
+
+
+def smoothing_sizes(x, y, z, nsmooth, brute=False, which=None, nthreads=8):
+    """LensHaloParticles::calculate_smoothing: the distance from each particle
+    to its (nsmooth+1)-th nearest particle in 3-D (itself included), with the
+    coordinate type of x (float32 or float64).  which: optional particle
+    indices to compute (all by default).  brute: by selection over all
+    particles instead of the reference's tree search."""
+    t = np.float32 if np.asarray(x).dtype == np.float32 else np.float64
+    x, y, z = (np.ascontiguousarray(a, dtype=t) for a in (x, y, z))
+    ct = c_f if t == np.float32 else c_d
+    w = None if which is None else np.ascontiguousarray(which, dtype=np.int64)
+    nq = x.size if w is None else w.size
+    out = np.zeros(nq, np.float32)
+    fn = _lib.orc_smoothing_f32 if t == np.float32 else _lib.orc_smoothing_f64
+    rc = fn(_p(x, ct), _p(y, ct), _p(z, ct), x.size, int(nsmooth), int(bool(brute)),
+            _p(out, c_f), nthreads, None if w is None else _p(w, ctypes.c_int64),
+            0 if w is None else w.size)
+    if rc:
+        raise ValueError("number of neighbors > total number of particles")
     return out
 
 

@@ -21,6 +21,10 @@
 //   Lens::rayshooterInternal (threads)   FullRange/internal_rayshooter_multi.cpp:53-118
 //   Utilities::PositionFromIndex         TreeCode_link/utilities.cpp:149-158
 //   constants PI, Grav, arcsecTOradians  include/standard.h:38-51
+//   LensHaloParticles::calculate_smoothing include/particle_halo.h:683-730
+//   TreeSimple ctor / BuildTreeNB / _BuildTreeNB (median cut)
+//                                        include/simpleTree.h:264-278, 842-960
+//   TreeSimple::NNDistance / _findleaf   include/simpleTree.h:483-640
 //
 // Cosmological distances come from CosmoLib in the reference, which is not
 // part of the reference tree.  Here they are the flat-LCDM comoving distance
@@ -37,6 +41,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <queue>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -634,6 +639,162 @@ struct Lens {
   }
 };
 
+// ------------------------------------------------- smoothing sizes (k-NN)
+// LensHaloParticles::calculate_smoothing (include/particle_halo.h:683-730):
+// Size of particle i = TreeSimple::NNDistance(x_i, Nsmooth + 1)
+// (include/simpleTree.h:483-620) over a 3-D median-cut tree with buckets of
+// 2 Nsmooth (simpleTree.h:264-278, 842-960): the distance to the
+// (Nsmooth+1)-th nearest particle, the particle itself included.  Squared
+// distances are summed per dimension from pow(x_p[j] - x_i[j], 2) with the
+// difference in the coordinate type T (simpleTree.h:518-536).
+template <typename T>
+struct KdTree {
+  struct Node {
+    double p1[3], p2[3];
+    int level, first, n;      // particles idx[first, first + n)
+    int child1 = -1, child2 = -1, parent = -1;
+  };
+  const T* x[3];
+  std::vector<int> idx;
+  std::vector<Node> nodes;
+  int bucket;
+
+  KdTree(const T* xs, const T* ys, const T* zs, size_t n, int bucket_) : bucket(bucket_) {
+    x[0] = xs; x[1] = ys; x[2] = zs;
+    idx.resize(n);
+    for (size_t i = 0; i < n; ++i) idx[i] = (int)i;
+    Node root;
+    for (int j = 0; j < 3; ++j) {
+      root.p1[j] = root.p2[j] = x[j][0];
+      for (size_t i = 0; i < n; ++i) {
+        root.p1[j] = std::min(root.p1[j], (double)x[j][i]);
+        root.p2[j] = std::max(root.p2[j], (double)x[j][i]);
+      }
+    }
+    root.level = 0; root.first = 0; root.n = (int)n;
+    nodes.push_back(root);
+    build(0);
+  }
+  // _BuildTreeNB with median_cut: sort by the cut dimension, cut at n/2
+  // (simpleTree.h:882-960)
+  void build(int b) {
+    if (nodes[b].n <= bucket) return;
+    const int d = nodes[b].level % 3;
+    const int first = nodes[b].first, n = nodes[b].n;
+    std::stable_sort(idx.begin() + first, idx.begin() + first + n,
+                     [&](int a, int c) { return x[d][a] < x[d][c]; });
+    const int cut = n / 2;
+    const double xc = x[d][idx[first + cut]];
+    Node c1 = nodes[b], c2 = nodes[b];
+    c1.p2[d] = xc; c2.p1[d] = xc;
+    c1.level = c2.level = nodes[b].level + 1;
+    c1.first = first; c1.n = cut;
+    c2.first = first + cut; c2.n = n - cut;
+    c1.parent = c2.parent = b;
+    if (c1.n > 0) { nodes[b].child1 = (int)nodes.size(); nodes.push_back(c1); build(nodes[b].child1); }
+    if (c2.n > 0) { nodes[b].child2 = (int)nodes.size(); nodes.push_back(c2); build(nodes[b].child2); }
+  }
+  double r2of(int p, const T* q) const {
+    double r2 = 0;
+    for (int j = 0; j < 3; ++j) r2 += std::pow(x[j][p] - q[j], 2);
+    return r2;
+  }
+  // NNDistance (simpleTree.h:483-620)
+  double nn_distance(const T* q, int k) const {
+    int it = 0;                                   // _findleaf (simpleTree.h:625-640)
+    while (nodes[it].child1 >= 0 || nodes[it].child2 >= 0) {
+      const int d = nodes[it].level % 3;
+      const int c1 = nodes[it].child1, c2 = nodes[it].child2;
+      if (c1 >= 0 && nodes[c1].p2[d] > (double)q[d]) it = c1;
+      else if (c2 >= 0) it = c2;
+      else it = c1;
+    }
+    while (nodes[it].n < k) it = nodes[it].parent;
+    const int pass = it;
+    std::priority_queue<double> heap;
+    const Node& P = nodes[pass];
+    for (int i = 0; i < P.n; ++i) {
+      const double r2 = r2of(idx[P.first + i], q);
+      if (i < k) heap.push(r2);
+      else if (r2 < heap.top()) { heap.push(r2); heap.pop(); }
+    }
+    double bestr = std::sqrt(heap.top());
+    // walk from the top, skipping the pass subtree; a box is entered if it
+    // overlaps [q - bestr, q + bestr] along the dimension it was cut in
+    std::vector<int> stack;
+    stack.push_back(0);
+    while (!stack.empty()) {
+      const int b = stack.back();
+      stack.pop_back();
+      if (b == pass) continue;
+      const Node& B = nodes[b];
+      if (b != 0) {
+        const int d = (B.level - 1) % 3;
+        if (!(((double)q[d] + bestr) > B.p1[d] && ((double)q[d] - bestr) < B.p2[d])) continue;
+      }
+      if (B.child1 < 0 && B.child2 < 0) {
+        for (int i = 0; i < B.n; ++i) {
+          const double r2 = r2of(idx[B.first + i], q);
+          if (r2 < bestr * bestr) {
+            heap.push(r2);
+            heap.pop();
+            bestr = std::sqrt(heap.top());
+          }
+        }
+      } else {
+        if (B.child2 >= 0) stack.push_back(B.child2);
+        if (B.child1 >= 0) stack.push_back(B.child1);
+      }
+    }
+    return bestr;
+  }
+};
+
+// The definition itself: the k-th smallest squared distance over all
+// particles, by selection.
+template <typename T>
+double nn_distance_brute(const T* const* x, size_t n, const T* q, int k,
+                         std::vector<double>& buf) {
+  buf.resize(n);
+  for (size_t p = 0; p < n; ++p) {
+    double r2 = 0;
+    for (int j = 0; j < 3; ++j) r2 += std::pow(x[j][p] - q[j], 2);
+    buf[p] = r2;
+  }
+  std::nth_element(buf.begin(), buf.begin() + (k - 1), buf.end());
+  return std::sqrt(buf[k - 1]);
+}
+
+template <typename T>
+int smoothing(const T* xs, const T* ys, const T* zs, size_t n, int nsmooth, int brute,
+              float* out, int nthreads, const int64_t* which, size_t nwhich) {
+  const int k = nsmooth + 1;
+  if (n <= (size_t)k) return 1;           // "number of neighbors > total number of particles"
+  std::unique_ptr<KdTree<T>> tree;
+  if (!brute) tree.reset(new KdTree<T>(xs, ys, zs, n, 2 * nsmooth));
+  const T* x[3] = {xs, ys, zs};
+  const size_t nq = which ? nwhich : n;
+  std::vector<std::thread> th;
+  std::atomic<size_t> next(0);
+  for (int t = 0; t < std::max(1, nthreads); ++t) {
+    th.emplace_back([&]() {
+      std::vector<double> buf;
+      for (;;) {
+        const size_t i0 = next.fetch_add(256);
+        if (i0 >= nq) break;
+        for (size_t a = i0; a < std::min(nq, i0 + 256); ++a) {
+          const size_t i = which ? (size_t)which[a] : a;
+          const T q[3] = {xs[i], ys[i], zs[i]};
+          out[a] = (float)(brute ? nn_distance_brute(x, n, q, k, buf)
+                                 : tree->nn_distance(q, k));
+        }
+      }
+    });
+  }
+  for (auto& t : th) t.join();
+  return 0;
+}
+
 }  // namespace orc
 
 // ====================================================================== C ABI
@@ -836,6 +997,18 @@ void orc_gridmap_positions(size_t N, double c0, double c1, double range, double*
     x[2 * i] = center[0] + range * (1.0 * (i % N) / (N - 1) - 0.5);
     x[2 * i + 1] = center[1] + range * (1.0 * (i / N) / (N - 1) - 0.5);
   }
+}
+
+// Smoothing sizes: out[a] for particle which[a] (all particles when which is
+// null).  brute = 1 uses the selection definition instead of the reference's
+// tree search.  Returns 1 when n <= nsmooth + 1 (the reference exits).
+int orc_smoothing_f64(const double* x, const double* y, const double* z, size_t n, int nsmooth,
+                      int brute, float* out, int nthreads, const int64_t* which, size_t nwhich) {
+  return orc::smoothing<double>(x, y, z, n, nsmooth, brute, out, nthreads, which, nwhich);
+}
+int orc_smoothing_f32(const float* x, const float* y, const float* z, size_t n, int nsmooth,
+                      int brute, float* out, int nthreads, const int64_t* which, size_t nwhich) {
+  return orc::smoothing<float>(x, y, z, n, nsmooth, brute, out, nthreads, which, nwhich);
 }
 
 }  // extern "C"

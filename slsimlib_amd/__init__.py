@@ -109,6 +109,22 @@ class LensHaloParticles:
                   _ptr(h, ctypes.c_float), ctypes.c_size_t(n), float(theta), int(bucket),
                   float(inv_area), ctypes.byref(self.h)))
 
+    @staticmethod
+    def calculate_smoothing(ctx, Nsmooth, x, y, z):
+        """LensHaloParticles::calculate_smoothing (particle_halo.h:683-730) on
+        the device: for each particle the distance to its (Nsmooth+1)-th
+        nearest particle in 3-D (itself included), in the positions' type
+        (float64 or float32).  Returns the sizes (float32), which the
+        reference stores in ParticleType::Size."""
+        t = np.float32 if np.asarray(x).dtype == np.float32 else np.float64
+        xs, ys, zs = (np.ascontiguousarray(a, dtype=t) for a in (x, y, z))
+        ct = ctypes.c_float if t == np.float32 else ctypes.c_double
+        fn = lib.slsim_smoothing_sizes_f32 if t == np.float32 else lib.slsim_smoothing_sizes_f64
+        out = np.zeros(xs.size, np.float32)
+        _check(fn(ctx.h, _ptr(xs, ct), _ptr(ys, ct), _ptr(zs, ct), ctypes.c_size_t(xs.size),
+                  int(Nsmooth), _ptr(out, ctypes.c_float)))
+        return out
+
     def info(self):
         i = _abi.slsim_plane_info()
         _check(lib.slsim_plane_get_info(self.h, ctypes.byref(i)))

@@ -137,6 +137,10 @@ SIGNATURES = {
     "slsim_memcpy_d2h": (c_int, [slsim_ctx, c_vp, c_vp, c_size_t]),
     "slsim_memcpy_h2d": (c_int, [slsim_ctx, c_vp, c_vp, c_size_t]),
     "slsim_memcpy_d2d": (c_int, [slsim_ctx, c_vp, c_vp, c_size_t]),
+    "slsim_smoothing_sizes_f64": (c_int, [slsim_ctx, P(c_double), P(c_double), P(c_double),
+                                          c_size_t, c_int, P(c_float)]),
+    "slsim_smoothing_sizes_f32": (c_int, [slsim_ctx, P(c_float), P(c_float), P(c_float),
+                                          c_size_t, c_int, P(c_float)]),
     "slsim_coordist": (c_double, [c_double, c_double, c_double, c_double, c_double]),
     "slsim_inv_coordist": (c_double, [c_double, c_double, c_double, c_double]),
     "slsim_nfw_r200": (c_double, [c_double, c_double, c_double, c_double, c_double]),

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libslsim_amd.so")
 
-SOURCES = ["tree_build.hip", "shoot.hip", "api.hip", "batch.hip", "generator.cpp"]
+SOURCES = ["tree_build.hip", "shoot.hip", "api.hip", "batch.hip", "smoothing.hip", "generator.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

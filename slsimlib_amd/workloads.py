@@ -75,3 +75,27 @@ def build_field_lens(ctx, config=2, nplanes=10, n=1 << 20, z_lo=0.1, z_hi=2.0, z
 
 def grid_range(arcsec):
     return arcsec * arcsecTOradians
+
+
+def particles_3d(seed_, n, n_halos=20, conc=8.0, rmax=1.0, dtype=np.float64):
+    """3-D positions (Mpc) for the smoothing-size workload: n particles in
+    n_halos NFW spheres (scale radius rmax / conc, truncated at rmax, masses
+    log-uniform over two decades, centres uniform in a 10 Mpc box), split in
+    proportion to the halo masses.  Seeded numpy generator."""
+    rng = np.random.default_rng(seed_)
+    w = 10.0 ** rng.uniform(0, 2, n_halos)
+    counts = np.floor(n * w / w.sum()).astype(np.int64)
+    counts[: n - counts.sum()] += 1
+    # inverse CDF of the truncated NFW mass profile m(x) = ln(1+x) - x/(1+x)
+    xs = np.linspace(0.0, conc, 4097)
+    mx = np.log1p(xs) - xs / (1 + xs)
+    parts = []
+    for c in counts:
+        centre = rng.uniform(-5.0, 5.0, 3)
+        u = rng.uniform(0, mx[-1], c)
+        r = np.interp(u, mx, xs) * (rmax / conc)
+        v = rng.normal(size=(c, 3))
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        parts.append(centre + v * r[:, None])
+    p = np.concatenate(parts)[rng.permutation(n)]
+    return tuple(np.ascontiguousarray(p[:, j], dtype=dtype) for j in range(3))

@@ -359,3 +359,41 @@ def test_gridmap_positions():
 def test_same_place_raises():
     with pytest.raises(ValueError):
         O.Tree(np.zeros(5), np.arange(5.0), np.ones(5, np.float32), np.ones(5, np.float32))
+
+
+# ------------------------------------------------------------ smoothing sizes
+def _nn_numpy(x, y, z, nsmooth):
+    """The definition: sqrt of the (nsmooth+1)-th smallest squared distance,
+    differences in the coordinate type, squares and sums in float64."""
+    P = np.stack([x, y, z], 1)
+    d = (P[:, None, :] - P[None, :, :]).astype(np.float64)
+    r2 = (d[..., 0] ** 2 + d[..., 1] ** 2) + d[..., 2] ** 2
+    return np.sqrt(np.sort(r2, 1)[:, nsmooth]).astype(np.float32)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_smoothing_tree_search_equals_definition(dtype):
+    rng = np.random.default_rng(3)
+    # clustered: two dense clumps and a sparse background
+    p = np.concatenate([rng.normal(0, 0.01, (700, 3)), rng.normal(1, 0.1, (700, 3)),
+                        rng.uniform(-3, 3, (600, 3))]).astype(dtype)
+    x, y, z = p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy()
+    for k in (1, 8, 40):
+        tree = O.smoothing_sizes(x, y, z, k)
+        brute = O.smoothing_sizes(x, y, z, k, brute=True)
+        assert np.array_equal(tree, brute)
+        assert np.array_equal(brute, _nn_numpy(x, y, z, k))
+
+
+def test_smoothing_known_answers():
+    # evenly spaced points on a line: the (k+1)-th nearest of point 0, itself
+    # included, is point k
+    x = np.arange(50, dtype=np.float64)
+    zero = np.zeros(50)
+    s = O.smoothing_sizes(x, zero, zero, 7)
+    assert s[0] == 7 and s[49] == 7 and s[25] == 4
+    # coincident particles are at distance 0 from each other
+    s = O.smoothing_sizes(np.r_[np.zeros(10), np.arange(1.0, 11)], np.zeros(20), np.zeros(20), 5)
+    assert np.all(s[:10] == 0)
+    with pytest.raises(ValueError):
+        O.smoothing_sizes(x[:8], zero[:8], zero[:8], 7)
