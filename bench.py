@@ -175,6 +175,37 @@ def run_serving(S, W, ctx, n_models, n_requests, load, precision, seed=5999):
     return out
 
 
+def run_smoothing(S, W, ctx, n, nsmooth=64, cpu=True):
+    """Smoothing sizes (LensHaloParticles::calculate_smoothing) of n 3-D
+    particles in 20 NFW spheres, float64 positions: device time including the
+    host transfers, and the oracle's restatement of the reference's tree
+    search on 16 host threads over a 65,536-particle query subsample."""
+    x, y, z = W.particles_3d(9000, n)
+    S.LensHaloParticles.calculate_smoothing(ctx, nsmooth, x[:200000], y[:200000], z[:200000])
+    t0 = time.perf_counter()
+    g = S.LensHaloParticles.calculate_smoothing(ctx, nsmooth, x, y, z)
+    tg = time.perf_counter() - t0
+    out = {"particles": n, "nsmooth": nsmooth, "gpu_seconds": tg, "gpu_particles_per_s": n / tg}
+    if cpu:
+        import oracle as O
+        which = np.arange(0, n, max(1, n // 65536))
+        t0 = time.perf_counter()
+        O.smoothing_sizes(x, y, z, nsmooth, which=which[:1], nthreads=16)
+        tb = time.perf_counter() - t0                  # tree build (and one query)
+        t0 = time.perf_counter()
+        o = O.smoothing_sizes(x, y, z, nsmooth, which=which, nthreads=16)
+        tc = time.perf_counter() - t0
+        q_rate = which.size / max(tc - tb, 1e-9)
+        out.update({"cpu_build_seconds": tb, "cpu_seconds_subsample": tc,
+                    "cpu_queries": int(which.size), "cpu_queries_per_s": q_rate,
+                    "cpu_particles_per_s": n / (tb + n / q_rate), "cpu_threads": 16,
+                    "cpu_note": "tree build plus n queries at the subsample's query rate",
+                    "equal_to_oracle_on_subsample": bool(np.array_equal(g[which], o))})
+        out["gpu_over_cpu"] = out["gpu_particles_per_s"] / out["cpu_particles_per_s"]
+    log("smoothing:", out)
+    return out
+
+
 def run_precision_form(S, W, ctx, n=1 << 22, nplanes=20, N=4096):
     """Precision form: 20 planes x 4,194,304 float32-position particles,
     opening angle 0.2, 4096^2 rays; FP32C and FP64 trees against the FP64
@@ -240,6 +271,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip FP64, direct-sum, counts")
+    ap.add_argument("--smoothing", type=int, default=0, metavar="N",
+                    help="also time the smoothing sizes (Nsmooth 64) of N 3-D particles")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device-to-device copy of MIB MiB (HIP runtime), the "
                          "known-size kernel that scales HBM counters under rocprofv3 --pmc")
@@ -441,6 +474,9 @@ def main():
         out["copy_calibration_bytes"] = nb
         ctx.free(src)
         ctx.free(dst)
+
+    if args.smoothing:
+        out["smoothing"] = run_smoothing(S, W, ctx, args.smoothing, cpu=not args.no_cpu)
 
     if args.precision_form:
         out["precision_form"] = run_precision_form(S, W, ctx)
