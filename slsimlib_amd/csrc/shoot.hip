@@ -142,13 +142,16 @@ __device__ inline void push_entry(int32_t* s_node, uint64_t* s_mask, int& sp, in
 // Evaluates one cell for the lanes with lv set (include/quadTree.h:990-1128):
 // returns the mask of lanes that opened it when it is internal, 0 otherwise.
 template <bool COUNT>
-__device__ __forceinline__ uint64_t eval64(const NodeD& nd, bool lv, double rx, double ry,
+__device__ __forceinline__ uint64_t eval64(const NodeD& nd, uint64_t lvm, double rx, double ry,
                                            double ia, cptr<PartD> parts, cptr<PartD> bigs,
                                            Acc64& a, Counts& cnt) {
   const double dx = nd.cx - rx, dy = nd.cy - ry;
   const double r2 = dx * dx + dy * dy;
-  const bool nearc = lv && (r2 < nd.near2);
-  const bool open = lv && ((r2 < nd.r2crit) || nearc);
+  const uint64_t nm = lanes_lt(r2, nd.near2) & lvm;
+  const uint64_t omask = (lanes_lt(r2, nd.r2crit) | nm) & lvm;
+  const bool lv = lane_in(lvm);
+  const bool nearc = lane_in(nm);
+  const bool open = lane_in(omask);
   if (COUNT) cnt.wv++;
   if (COUNT && lv) cnt.v++;
   if (lv && !open) {
@@ -170,7 +173,7 @@ __device__ __forceinline__ uint64_t eval64(const NodeD& nd, bool lv, double rx, 
     a.ay += tmp * dy;
     a.k -= ia * nd.mass;
   }
-  const uint64_t om = ballot(open);
+  const uint64_t om = omask;
   if (om == 0) return 0;
   if (nd.child < 0) {
     // leaf: every particle as a point mass (include/quadTree.h:1014-1048)
@@ -193,7 +196,7 @@ __device__ __forceinline__ uint64_t eval64(const NodeD& nd, bool lv, double rx, 
     }
     if (COUNT && open) cnt.p += nd.count;
     if (COUNT) cnt.wp += nd.count;
-    if (ballot(nearc)) {
+    if (nm) {
       // the leaf's big particles: h < box (include/quadTree.h:379-396, 1051-1077)
       for (int k = 0; k < nd.count; ++k) {
         const PartD p = parts[nd.start + k];
@@ -212,7 +215,7 @@ __device__ __forceinline__ uint64_t eval64(const NodeD& nd, bool lv, double rx, 
     }
     return 0;
   }
-  if (nd.count > 0 && ballot(nearc)) {
+  if (nd.count > 0 && nm) {
     for (int k = 0; k < nd.count; ++k) {
       const PartD p = bigs[nd.start + k];
       if (nearc) {
@@ -244,7 +247,7 @@ __device__ void walk64(const TreeDesc& t, double rx, double ry, bool active, Acc
   int sp = 0;
   {
     const NodeD root = nodes[0];
-    const uint64_t om = eval64<COUNT>(root, active, rx, ry, ia, parts, bigs, a, cnt);
+    const uint64_t om = eval64<COUNT>(root, ballot(active), rx, ry, ia, parts, bigs, a, cnt);
     if (om) push_entry(s_node, s_mask, sp, lane, root.child | (root.cmask << 28), om, err);
   }
   while (sp > 0) {
@@ -253,7 +256,6 @@ __device__ void walk64(const TreeDesc& t, double rx, double ry, bool active, Acc
     const int entry = uni(s_node[sp]);
     const uint64_t live = uni64(s_mask[sp]);
     __builtin_amdgcn_wave_barrier();
-    const bool lv = (live >> lane) & 1ull;
     const int base = entry & 0x0fffffff;
     const int cm = (entry >> 28) & 0xf;
     int32_t pe[4];
@@ -264,7 +266,7 @@ __device__ void walk64(const TreeDesc& t, double rx, double ry, bool active, Acc
       pe[k] = 0;
       if ((cm >> k) & 1) {
         const NodeD nd = nodes[base + k];
-        pm[k] = eval64<COUNT>(nd, lv, rx, ry, ia, parts, bigs, a, cnt);
+        pm[k] = eval64<COUNT>(nd, live, rx, ry, ia, parts, bigs, a, cnt);
         pe[k] = nd.child | (nd.cmask << 28);
       }
     }
