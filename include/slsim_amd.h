@@ -35,7 +35,7 @@ enum {
 /* Arithmetic of the tree walk.  The plane-to-plane recursion is FP64 in both. */
 enum {
   SLSIM_FP64 = 0,             /* plain method: FP64 walk, reference arithmetic    */
-  SLSIM_FP32C = 1,            /* FP32 walk with compensated (TwoSum) accumulation */
+  SLSIM_FP32C = 1,            /* FP32 walk with Kahan-compensated accumulation    */
   SLSIM_DIRECT64 = 2          /* FP64 direct sum over every particle of a plane, no
                                  tree: the error-column oracle (O(N) per ray)      */
 };
@@ -233,6 +233,10 @@ int slsim_server_submit(slsim_server s, slsim_lens lens, int64_t N, double c0, d
  * host buffers (either may be NULL) and releases its device memory. */
 int slsim_server_wait(slsim_server s, uint64_t id, double* y_host, double* A_host,
                       slsim_request_info* info);
+/* Non-blocking: *done = 1 once request `id` has finished (collect it with
+ * slsim_server_wait, which then returns at once), 0 while it is queued or
+ * running. */
+int slsim_server_poll(slsim_server s, uint64_t id, int* done);
 int slsim_server_get_stats(slsim_server s, slsim_server_stats* st);
 int slsim_server_destroy(slsim_server s);
 /* Monotonic clock (seconds) of the request timestamps. */

@@ -424,6 +424,12 @@ class Server:
                                        ctypes.byref(rid)))
         return rid.value
 
+    def poll(self, rid):
+        """True once request rid has finished (non-blocking)."""
+        done = ctypes.c_int(0)
+        _check(lib.slsim_server_poll(self.h, ctypes.c_uint64(rid), ctypes.byref(done)))
+        return bool(done.value)
+
     def wait(self, rid, N=None, fetch=True):
         info = _abi.slsim_request_info()
         y = A = None

@@ -672,6 +672,16 @@ int slsim_server_wait(slsim_server s, uint64_t id, double* y_host, double* A_hos
   return rc;
 }
 
+int slsim_server_poll(slsim_server s, uint64_t id, int* done) {
+  if (!s || !done) return err_arg("null argument");
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->all.find(id);
+  if (it == s->all.end()) return err_arg("unknown request id");
+  const int st = it->second->state;
+  *done = (st == slsim_server_s::DONE || st == slsim_server_s::FAILED) ? 1 : 0;
+  return SLSIM_OK;
+}
+
 int slsim_server_get_stats(slsim_server s, slsim_server_stats* st) {
   if (!s || !st) return err_arg("null argument");
   std::lock_guard<std::mutex> g(s->mu);

@@ -1,6 +1,7 @@
 """Batched GridMaps and the serving loop: every request's result equals its
 single GridMap shoot bit for bit, whatever the schedule or the mix."""
 import os
+import time
 import subprocess
 import threading
 
@@ -110,6 +111,25 @@ def test_server_budget(ctx, lenses):
         _, _, info = srv.wait(rid, 512, fetch=False)
         assert info["status"] == 0
     assert srv.stats()["max_active"] == 1
+    srv.close()
+
+
+def test_server_poll(ctx, lenses):
+    """Submit / poll / collect: poll never blocks, turns true once the request
+    is done, and the collected result equals a lone GridMap's."""
+    srv = S.Server(ctx)
+    rg = W.grid_range(30.0)
+    rid = srv.submit(lenses["c1"], 256, (0.0, 0.0), rg, precision=S.SLSIM_FP64)
+    t0 = time.time()
+    while not srv.poll(rid):
+        assert time.time() - t0 < 60
+        time.sleep(0.002)
+    y, A, info = srv.wait(rid, 256)
+    assert info["status"] == 0
+    g = S.GridMap(lenses["c1"], 256, (0.0, 0.0), rg, S.SLSIM_FP64)
+    assert np.array_equal(y, g.y()) and np.array_equal(A, g.A())
+    with pytest.raises(S.SlsimError):
+        srv.poll(rid)                      # collected: no longer known
     srv.close()
 
 
