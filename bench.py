@@ -105,7 +105,7 @@ def run_batched(S, W, ctx, n_real, precision):
     return out
 
 
-def run_serving(S, W, ctx, n_models, n_requests, load, precision, seed=5999):
+def run_serving(S, W, ctx, n_models, n_requests, load, precision, seed=5999, rate=None):
     """Serving form: resident 10-plane x 2^20-particle lenses; Poisson arrivals
     of GridMap requests (lens, zs in [1,4], centre in the 5' field, side from
     {128,...,2048} with p = {.4,.3,.15,.1,.05}, 300"/2048 per pixel)."""
@@ -134,7 +134,10 @@ def run_serving(S, W, ctx, n_models, n_requests, load, precision, seed=5999):
         srv.wait(rid, fetch=False)
     cap_rays = sum(int(sides[i]) ** 2 for i in range(len(probe))) / (time.perf_counter() - tc)
     mean_rays = float(np.mean(sides.astype(np.float64) ** 2))
-    rate = load * cap_rays / mean_rays
+    if rate is None:
+        rate = load * cap_rays / mean_rays
+    else:
+        load = rate * mean_rays / cap_rays
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
     lat, infos = [], []
     lock = threading.Lock()
@@ -284,6 +287,8 @@ def main():
                     help="also run the serving form with this many resident lenses")
     ap.add_argument("--serving-requests", type=int, default=300)
     ap.add_argument("--serving-load", type=float, default=0.8)
+    ap.add_argument("--serving-rate", type=float, default=0.0,
+                    help="Poisson arrival rate (requests/s) instead of a fraction of capacity")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build's ray shooter runs on a single "
@@ -484,7 +489,8 @@ def main():
         out["batched"] = run_batched(S, W, ctx, args.batched, S.SLSIM_FP32C)
     if args.serving_models:
         out["serving"] = run_serving(S, W, ctx, args.serving_models, args.serving_requests,
-                                     args.serving_load, S.SLSIM_FP32C)
+                                     args.serving_load, S.SLSIM_FP32C,
+                                     rate=args.serving_rate or None)
 
     print(json.dumps(out), flush=True)
 

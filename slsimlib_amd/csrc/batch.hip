@@ -427,6 +427,7 @@ struct slsim_server_s {
     double* y = nullptr;
     double* A = nullptr;
     int32_t jmax = 0, step = 0;
+    int64_t nbundles = 0, next_bundle = 0;   // progress within plane `step`
     double aal = 0, bbl = 0;
     std::vector<StepJob> steps;
     State state = PENDING;
@@ -479,40 +480,101 @@ int slsim_server_s::admit(Req* r) {
   if ((rc = r->lens->upload_trees())) return rc;
   if (r->lens->plane_redshifts.size() > 1) uniform_source(r->lens, r->zs, &r->jmax, &r->aal, &r->bbl);
   make_steps(r->lens, r->st, r->nslots, nullptr, r->steps);
+  r->nbundles = r->nslots / kWave;
+  r->next_bundle = 0;
   launch_init_grid(stream, r->st, r->nslots, r->N, r->tiles_x, 3, r->c0, r->c1, r->range, r->jmax,
                    r->aal, r->bbl);
   SLSIM_HIP_CHECK(hipGetLastError());
   return SLSIM_OK;
 }
 
-// One plane step of every active request (one launch per precision).
+// One scheduler iteration: up to kIterBundles bundles of plane steps, so that
+// an iteration stays short however large the requests in flight are.  The
+// oldest request gets up to a quarter of the iteration (no request starves);
+// the rest goes to the requests with the least remaining work first.  A
+// request takes bundles of its current plane only; its rays are independent,
+// so how its bundles are split over iterations does not change a bit of its
+// result.  One launch per precision.
+constexpr int64_t kIterBundles = 32768;
+
 int slsim_server_s::iterate() {
-  size_t njobs = 0, nit = 0;
-  for (Req* r : active) {
-    ++njobs;
-    nit += r->nslots / kWave;
+  std::vector<Req*> order;
+  for (Req* r : active)
+    if (r->step < r->jmax) order.push_back(r);
+  if (order.empty()) return SLSIM_OK;
+  auto remaining = [](const Req* r) {
+    return (int64_t)(r->jmax - r->step) * r->nbundles - r->next_bundle;
+  };
+  Req* oldest = order.front();
+  for (Req* r : order)
+    if (r->info.t_submit < oldest->info.t_submit) oldest = r;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](const Req* a, const Req* b) { return remaining(a) < remaining(b); });
+  std::vector<std::pair<Req*, int64_t>> take;   // request, bundles this iteration
+  int64_t left = kIterBundles;
+  {
+    const int64_t t = std::min(oldest->nbundles - oldest->next_bundle, kIterBundles / 4);
+    take.push_back({oldest, t});
+    left -= t;
   }
-  int rc = stage(njobs, nit);
+  for (Req* r : order) {
+    if (left <= 0) break;
+    if (r == oldest) {
+      const int64_t more = std::min(r->nbundles - r->next_bundle - take[0].second, left);
+      take[0].second += more;
+      left -= more;
+      continue;
+    }
+    const int64_t t = std::min(r->nbundles - r->next_bundle, left);
+    take.push_back({r, t});
+    left -= t;
+  }
+  size_t nit = 0;
+  for (auto& t : take) nit += (size_t)t.second;
+  int rc = stage(take.size(), nit);
   if (rc) return rc;
+  // jobs and items of all precisions in one staging copy, grouped by precision
+  size_t nj = 0, ni = 0;
+  size_t job0[SLSIM_DIRECT64 + 2], item0[SLSIM_DIRECT64 + 2];
   for (int prec = SLSIM_FP64; prec <= SLSIM_DIRECT64; ++prec) {
-    size_t nj = 0, ni = 0;
-    for (Req* r : active) {
-      if (r->precision != prec || r->step >= r->jmax) continue;
+    job0[prec] = nj;
+    item0[prec] = ni;
+    for (auto& t : take) {
+      Req* r = t.first;
+      if (r->precision != prec || t.second == 0) continue;
       h_jobs[nj] = r->steps[r->step];
-      for (int64_t b = 0; b < r->nslots / kWave; ++b) h_items[ni++] = {(uint32_t)nj, (uint32_t)b};
+      const uint32_t local = (uint32_t)(nj - job0[prec]);
+      for (int64_t b = 0; b < t.second; ++b)
+        h_items[ni++] = {local, (uint32_t)(r->next_bundle + b)};
       ++nj;
     }
-    if (ni == 0) continue;
-    SLSIM_HIP_CHECK(hipMemcpyAsync(d_jobs.p, h_jobs, sizeof(StepJob) * nj, hipMemcpyHostToDevice,
-                                   stream));
-    SLSIM_HIP_CHECK(hipMemcpyAsync(d_items.p, h_items, sizeof(WorkItem) * ni,
-                                   hipMemcpyHostToDevice, stream));
-    launch_plane_step(prec, false, (uint32_t)ni, stream, static_cast<StepJob*>(d_jobs.p),
-                      static_cast<WorkItem*>(d_items.p), static_cast<uint32_t*>(d_err.p));
-    // the staging buffers are reused by the next group: wait for the copies
-    SLSIM_HIP_CHECK(hipStreamSynchronize(stream));
+  }
+  job0[SLSIM_DIRECT64 + 1] = nj;
+  item0[SLSIM_DIRECT64 + 1] = ni;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(d_jobs.p, h_jobs, sizeof(StepJob) * nj, hipMemcpyHostToDevice,
+                                 stream));
+  SLSIM_HIP_CHECK(hipMemcpyAsync(d_items.p, h_items, sizeof(WorkItem) * ni, hipMemcpyHostToDevice,
+                                 stream));
+  for (int prec = SLSIM_FP64; prec <= SLSIM_DIRECT64; ++prec) {
+    const size_t n_items = item0[prec + 1] - item0[prec];
+    if (n_items == 0) continue;
+    launch_plane_step(prec, false, (uint32_t)n_items, stream,
+                      static_cast<StepJob*>(d_jobs.p) + job0[prec],
+                      static_cast<WorkItem*>(d_items.p) + item0[prec],
+                      static_cast<uint32_t*>(d_err.p));
   }
   SLSIM_HIP_CHECK(hipGetLastError());
+  // the staging buffers are reused next iteration
+  SLSIM_HIP_CHECK(hipStreamSynchronize(stream));
+  for (auto& t : take) {
+    Req* r = t.first;
+    r->next_bundle += t.second;
+    if (r->next_bundle >= r->nbundles) {
+      r->next_bundle = 0;
+      ++r->step;
+      ++r->info.iterations;
+    }
+  }
   return SLSIM_OK;
 }
 
@@ -553,10 +615,9 @@ void slsim_server_s::loop() {
     stats.max_active = std::max<int64_t>(stats.max_active, (int64_t)active.size());
     lk.unlock();
     int rc = iterate();
-    // finalize the requests whose last plane was this step
+    // finalize the requests whose last plane is done
     std::vector<Req*> finished;
     for (Req* r : active) {
-      if (rc == SLSIM_OK && r->step < r->jmax) ++r->step;
       if (rc || r->step >= r->jmax) {
         if (rc == SLSIM_OK)
           launch_finalize_grid(stream, r->st, r->nslots, r->N, r->tiles_x, 3, r->y, r->A);
@@ -572,7 +633,6 @@ void slsim_server_s::loop() {
     ++stats.iterations;
     for (Req* r : finished) {
       r->info.t_done = t;
-      r->info.iterations = r->step;
       r->state = rc ? FAILED : DONE;
       r->status = rc;
       if (!rc) ++stats.completed;
