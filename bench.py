@@ -209,6 +209,57 @@ def run_smoothing(S, W, ctx, n, nsmooth=64, cpu=True):
     return out
 
 
+def run_formats(S, W, ctx, n, cpu=True, nsmooth=32):
+    """Particle files: n particles in 20 NFW spheres written as a csv5 table
+    (sizes given) and a csv4 table (sizes computed), read by
+    MakeParticleLenses; the glmb cache reopened; a 2048^2 float64 kappa map
+    written as FITS.  The oracle's single-thread restatement of readCSV
+    (getline, splitstring, stof, as the reference runs it) is timed on the
+    same csv5 file."""
+    import tempfile
+    x, y, z = W.particles_3d(9100, n, dtype=np.float32)
+    d = tempfile.mkdtemp(prefix="slsim_formats_")
+    m = np.full(n, np.float32(1e9))
+    h = np.full(n, np.float32(1e-3))
+    cols = np.stack([x, y, z, m, h], 1).astype(np.float32)
+    f5, f4 = os.path.join(d, "p5.csv"), os.path.join(d, "p4.csv")
+    np.savetxt(f5, cols, fmt="%.9g", delimiter=",", header="x,y,z,mass,size")
+    np.savetxt(f4, cols[:, :4], fmt="%.9g", delimiter=",", header="x,y,z,mass")
+    mb = os.path.getsize(f5) / 1e6
+    out = {"particles": n, "csv5_MB": mb, "nsmooth": nsmooth}
+    t0 = time.perf_counter()
+    mk = S.MakeParticleLenses(ctx, f5, S.SLSIM_FMT_CSV5, nsmooth)
+    t5 = time.perf_counter() - t0
+    out.update({"csv5_read_s": t5, "csv5_particles_per_s": n / t5, "csv5_MB_per_s": mb / t5})
+    t0 = time.perf_counter()
+    mk4 = S.MakeParticleLenses(ctx, f4, S.SLSIM_FMT_CSV4, nsmooth)
+    t4 = time.perf_counter() - t0
+    out.update({"csv4_with_device_sizes_s": t4, "csv4_particles_per_s": n / t4})
+    t0 = time.perf_counter()
+    S.MakeParticleLenses(ctx, f4, S.SLSIM_FMT_CSV4, nsmooth)       # reads the glmb cache
+    tg = time.perf_counter() - t0
+    out.update({"glmb_cache_read_s": tg, "glmb_particles_per_s": n / tg})
+    img = np.random.default_rng(3).normal(size=(2048, 2048))
+    t0 = time.perf_counter()
+    S.write_fits_pixelmap(os.path.join(d, "k.fits"), img, 1e-7, (0.0, 0.0))
+    tf = time.perf_counter() - t0
+    out.update({"fits_2048sq_f64_s": tf, "fits_MB_per_s": img.nbytes / 1e6 / tf})
+    if cpu:
+        from oracle import formats as F
+        t0 = time.perf_counter()
+        rc, q, _ = F.read_csv(f5, 5)
+        tc = time.perf_counter() - t0
+        out.update({"cpu_csv5_read_s": tc, "cpu_csv5_particles_per_s": n / tc, "cpu_threads": 1,
+                    "csv5_speedup": tc / t5,
+                    "csv5_equal_to_oracle": bool(rc == 0 and q.tobytes() == mk.data.tobytes())})
+    del mk, mk4
+    for fn in os.listdir(d):
+        os.remove(os.path.join(d, fn))
+    os.rmdir(d)
+    log("formats:", out)
+    return out
+
+
 def run_precision_form(S, W, ctx, n=1 << 22, nplanes=20, N=4096):
     """Precision form: 20 planes x 4,194,304 float32-position particles,
     opening angle 0.2, 4096^2 rays; FP32C and FP64 trees against the FP64
@@ -276,6 +327,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip FP64, direct-sum, counts")
     ap.add_argument("--smoothing", type=int, default=0, metavar="N",
                     help="also time the smoothing sizes (Nsmooth 64) of N 3-D particles")
+    ap.add_argument("--formats", type=int, default=0, metavar="N",
+                    help="also time the particle-file readers (csv5, csv4 with device "
+                         "sizes, glmb) on N particles and a 2048^2 FITS write")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device-to-device copy of MIB MiB (HIP runtime), the "
                          "known-size kernel that scales HBM counters under rocprofv3 --pmc")
@@ -482,6 +536,8 @@ def main():
 
     if args.smoothing:
         out["smoothing"] = run_smoothing(S, W, ctx, args.smoothing, cpu=not args.no_cpu)
+    if args.formats:
+        out["formats"] = run_formats(S, W, ctx, args.formats, cpu=not args.no_cpu)
 
     if args.precision_form:
         out["precision_form"] = run_precision_form(S, W, ctx)

@@ -9,6 +9,9 @@
 //   LensHaloParticles<PType>   include/particle_halo.h:58-113, 341-434
 //   Lens                       include/lens.h:75-349, MultiPlane/lens.cpp
 //   GridMap                    include/gridmap.h:31-180, TreeCode_link/gridmap.cpp
+//   PixelMap<T>::printFITS     ImageProcessing/pixelmap.cpp:1318-1388
+//   MakeParticleLenses, SimFileFormat  include/particle_halo.h:964-1080,
+//                              include/particle_types.h:32-43
 // compiles against this header and runs the rays on the GPU.  Link with
 // libslsim_amd.so.  Reference behaviours kept on purpose are marked
 // "(reference)".  Deliberate differences: a non-flat cosmology throws
@@ -171,6 +174,21 @@ struct ParticleType {
   float mass() { return Mass; }
 };
 
+static_assert(sizeof(ParticleType<float>) == sizeof(slsim_particle_f32),
+              "ParticleType<float> is the glmb record");
+
+// include/particle_types.h:32-43
+enum class SimFileFormat { glmb, csv3, csv4, csv5, csv6, gadget2, ascii };
+
+template <typename T = double>
+struct Point_3d {
+  Point_3d() { x[0] = x[1] = x[2] = 0.0; }
+  Point_3d(T xx, T yy, T zz) { x[0] = xx; x[1] = yy; x[2] = zz; }
+  T& operator[](int i) { return x[i]; }
+  T operator[](int i) const { return x[i]; }
+  T x[3];
+};
+
 enum class LensingVariable { DELAYT, ALPHA, ALPHA1, ALPHA2, KAPPA, GAMMA, GAMMA1, GAMMA2,
                              GAMMA3, INVMAG, PHI, SurfBrightness };
 
@@ -192,6 +210,8 @@ class COSMOLOGY {
 };
 
 // ------------------------------------------------------ particle lens halo
+class MakeParticleLenses;
+
 template <typename PType>
 class LensHaloParticles {
  public:
@@ -206,13 +226,61 @@ class LensHaloParticles {
     std::swap(pvector, trash_collector);
     set_up(theta_rotate, recenter, theta_force);
   }
+  // particle_halo.h:226-340: particles from an ASCII file ("# nparticles N"
+  // header; x y z [mass] rows, comoving Mpc), sizes from
+  // "<file>.<Nsmooth>sizes" or computed on the device and written there.
+  LensHaloParticles(const std::string& simulation_filename, SimFileFormat format, PosType redshift,
+                    int Nsmooth, const COSMOLOGY& cosmo, Point_2d theta_rotate, bool recenter,
+                    bool my_multimass, double my_inv_area, PosType MinPSize = 0,
+                    PosType rescale_mass = 1.0, bool verbose = false)
+      : zlens((float)redshift), inv_area(my_inv_area), min_size((float)MinPSize) {
+    (void)cosmo; (void)verbose;
+    if (format != SimFileFormat::ascii) throw std::invalid_argument("bad format");
+    size_t n = 0;
+    slsim_cpp::check(slsim_ascii_halo_count(simulation_filename.c_str(), my_multimass, &n));
+    using T = typename std::remove_reference<decltype(std::declval<PType&>()[0])>::type;
+    std::vector<T> xyz(3 * n);
+    std::vector<float> m(n), h(n);
+    slsim_ascii_halo_info info;
+    const double th[2] = {theta_rotate[0], theta_rotate[1]};
+    if constexpr (sizeof(T) == sizeof(float))
+      slsim_cpp::check(slsim_ascii_halo_load_f32(slsim_cpp::context(), simulation_filename.c_str(),
+                                                 redshift, Nsmooth, th, recenter, my_multimass,
+                                                 MinPSize, rescale_mass, n, (float*)xyz.data(),
+                                                 m.data(), h.data(), &info));
+    else
+      slsim_cpp::check(slsim_ascii_halo_load_f64(slsim_cpp::context(), simulation_filename.c_str(),
+                                                 redshift, Nsmooth, th, recenter, my_multimass,
+                                                 MinPSize, rescale_mass, n, (double*)xyz.data(),
+                                                 m.data(), h.data(), &info));
+    trash_collector.resize(n);
+    std::vector<T> x(n), y(n);
+    for (size_t i = 0; i < n; ++i) {
+      for (int k = 0; k < 3; ++k) trash_collector[i][k] = xyz[3 * i + k];
+      trash_collector[i].Mass = m[i];
+      trash_collector[i].Size = h[i];
+      trash_collector[i].type = 0;
+      x[i] = xyz[3 * i];
+      y[i] = xyz[3 * i + 1];
+    }
+    mass = info.mass;
+    for (int k = 0; k < 3; ++k) mcenter[k] = info.mcenter[k];
+    build<T>(x, y, m, h, theta_force_);
+  }
   LensHaloParticles(LensHaloParticles&& h) noexcept
       : zlens(h.zlens), inv_area(h.inv_area), min_size(h.min_size), mass(h.mass),
-        trash_collector(std::move(h.trash_collector)), plane(h.plane) {
+        theta_force_(h.theta_force_), trash_collector(std::move(h.trash_collector)),
+        plane(h.plane) {
     for (int i = 0; i < 3; ++i) mcenter[i] = h.mcenter[i];
     h.plane = nullptr;
   }
-  LensHaloParticles(const LensHaloParticles&) = delete;
+  // a copy holds its own device tree, built from the same particles
+  LensHaloParticles(const LensHaloParticles& h)
+      : zlens(h.zlens), inv_area(h.inv_area), min_size(h.min_size), mass(h.mass),
+        theta_force_(h.theta_force_), trash_collector(h.trash_collector) {
+    for (int i = 0; i < 3; ++i) mcenter[i] = h.mcenter[i];
+    rebuild();
+  }
   ~LensHaloParticles() {
     if (plane) slsim_plane_destroy(plane);
   }
@@ -249,9 +317,36 @@ class LensHaloParticles {
     for (size_t i = 0; i < Npoints; ++i) pp[i].Size = size[i];
   }
 
+ protected:
+  friend class MakeParticleLenses;
+  // particle_halo.h:194-207, the constructor MakeParticleLenses uses: the
+  // particles are copied (the reference shares them; set_up changes them only
+  // when recentring or rotating, which CreateHalos does not ask for).
+  LensHaloParticles(PType* pdata, size_t Nparticles, float redshift, const COSMOLOGY& cosmo,
+                    Point_2d theta_rotate, bool recenter, double my_inv_area, bool verbose = false)
+      : zlens(redshift), inv_area(my_inv_area), min_size(0) {
+    (void)cosmo; (void)verbose;
+    trash_collector.assign(pdata, pdata + Nparticles);
+    set_up(theta_rotate, recenter, 0.1);
+  }
+
  private:
+  void rebuild() {
+    const size_t n = trash_collector.size();
+    using T = typename std::remove_reference<decltype(std::declval<PType&>()[0])>::type;
+    std::vector<T> x(n), y(n);
+    std::vector<float> m(n), h(n);
+    for (size_t i = 0; i < n; ++i) {
+      x[i] = trash_collector[i][0];
+      y[i] = trash_collector[i][1];
+      m[i] = trash_collector[i].mass();
+      h[i] = trash_collector[i].size();
+    }
+    build<T>(x, y, m, h, theta_force_);
+  }
   // set_up, particle_halo.h:341-417
   void set_up(Point_2d theta_rotate, bool recenter, double theta_force) {
+    theta_force_ = theta_force;
     std::vector<PType>& pp = trash_collector;
     const size_t n = pp.size();
     mcenter[0] = mcenter[1] = mcenter[2] = 0;
@@ -309,6 +404,7 @@ class LensHaloParticles {
   double inv_area;
   float min_size;
   double mass = 0;
+  double theta_force_ = 0.1;
   double mcenter[3] = {0, 0, 0};
   std::vector<PType> trash_collector;
   slsim_plane plane = nullptr;
@@ -335,6 +431,13 @@ class Lens {
     T* halo = new T(std::move(halo_in));
     owned.emplace_back(halo, [](void* p) { delete static_cast<T*>(p); });
     slsim_cpp::check(slsim_lens_add_halo(lens, halo->release_plane(), halo->getZlens()));
+  }
+
+  // include/lens.h:149-170: the lens keeps a copy of the halo.
+  template <typename T>
+  void insertMainHalo(const T& halo_in, bool addplanes, bool verbose = false) {
+    T copy(halo_in);
+    moveinMainHalo(copy, addplanes, verbose);
   }
 
   int getNplanes() const {
@@ -454,6 +557,111 @@ class Lens {
 };
 typedef Lens* LensHndl;
 
+// ------------------------------------------------------- particle files
+// MakeParticleLenses (include/particle_halo.h:964-1080, MultiPlane/
+// particle_lens.cpp): reads csv3/csv4/csv5, Gadget-2 or glmb files (smoothing
+// sizes on the device, cached in "<filename>_S<Nsmooth>.glmb") and makes one
+// LensHaloParticles per particle type.  `data` is a std::vector-like view of
+// the particles, which the library holds.
+class MakeParticleLenses {
+ public:
+  struct Particles {
+    slsim_mpl h = nullptr;
+    ParticleType<float>* data() const {
+      slsim_particle_f32* p = nullptr;
+      size_t n = 0;
+      slsim_cpp::check(slsim_mpl_data(h, &p, &n));
+      return reinterpret_cast<ParticleType<float>*>(p);
+    }
+    size_t size() const {
+      slsim_particle_f32* p = nullptr;
+      size_t n = 0;
+      slsim_cpp::check(slsim_mpl_data(h, &p, &n));
+      return n;
+    }
+    ParticleType<float>& operator[](size_t i) const { return data()[i]; }
+    ParticleType<float>* begin() const { return data(); }
+    ParticleType<float>* end() const { return data() + size(); }
+  };
+
+  std::vector<LensHaloParticles<ParticleType<float>>*> halos;
+  Particles data;
+
+  MakeParticleLenses(const std::string& filename, SimFileFormat format, int Nsmooth, bool recenter,
+                     bool ignore_type_in_smoothing = false) {
+    slsim_cpp::check(slsim_mpl_create(slsim_cpp::context(), filename.c_str(), (int)format, Nsmooth,
+                                      recenter, ignore_type_in_smoothing, &data.h));
+  }
+  MakeParticleLenses(const std::string& filename, bool recenter) {
+    slsim_cpp::check(slsim_mpl_create_glmb(slsim_cpp::context(), filename.c_str(), recenter,
+                                           &data.h));
+  }
+  MakeParticleLenses(const MakeParticleLenses&) = delete;
+  ~MakeParticleLenses() {
+    for (auto p : halos) delete p;
+    if (data.h) slsim_mpl_destroy(data.h);
+  }
+
+  std::vector<size_t> getnp() const {
+    slsim_mpl_info i = info();
+    return std::vector<size_t>(i.nparticles, i.nparticles + 6);
+  }
+  std::vector<float> getmp() const {
+    slsim_mpl_info i = info();
+    return std::vector<float>(i.masses, i.masses + 6);
+  }
+  double sim_redshift() const { return info().z_original; }
+  double getZoriginal() const { return info().z_original; }
+  Point_3d<> getCenterOfMass() const {
+    slsim_mpl_info i = info();
+    return Point_3d<>(i.cm[0], i.cm[1], i.cm[2]);
+  }
+  void getBoundingBox(Point_3d<>& Xmin, Point_3d<>& Xmax) const {
+    slsim_mpl_info i = info();
+    Xmin = Point_3d<>(i.bbox_ll[0], i.bbox_ll[1], i.bbox_ll[2]);
+    Xmax = Point_3d<>(i.bbox_ur[0], i.bbox_ur[1], i.bbox_ur[2]);
+  }
+  // particle_lens.cpp:139-151: halos made before are deleted
+  void Recenter(Point_3d<> x) {
+    slsim_cpp::check(slsim_mpl_recenter(data.h, x.x));
+    for (auto p : halos) delete p;
+    halos.clear();
+  }
+  void radialCut(Point_3d<> center, double radius) {
+    slsim_cpp::check(slsim_mpl_radial_cut(data.h, center.x, radius));
+  }
+  void cylindricalCut(Point_2d center, double radius) {
+    slsim_cpp::check(slsim_mpl_cylindrical_cut(data.h, center.x, radius));
+  }
+  Point_3d<> densest_particle() {
+    Point_3d<> x;
+    slsim_cpp::check(slsim_mpl_densest_particle(data.h, x.x));
+    return x;
+  }
+  // particle_lens.cpp:153-209
+  void CreateHalos(const COSMOLOGY& cosmo, double redshift, double inv_area) {
+    slsim_cpp::check(slsim_mpl_physical_units(data.h, (float)cosmo.gethubble()));
+    for (auto p : halos) delete p;
+    halos.clear();
+    std::vector<size_t> np = getnp();
+    ParticleType<float>* pp = data.data();
+    size_t skip = 0;
+    for (int i = 0; i < 6; ++i) {
+      if (np[i] > 0)
+        halos.push_back(new LensHaloParticles<ParticleType<float>>(
+            pp + skip, np[i], (float)redshift, cosmo, Point_2d(), false, inv_area, false));
+      skip += np[i];
+    }
+  }
+
+ private:
+  slsim_mpl_info info() const {
+    slsim_mpl_info i;
+    slsim_cpp::check(slsim_mpl_get_info(data.h, &i));
+    return i;
+  }
+};
+
 // ---------------------------------------------------------------- pixel map
 template <typename T>
 class PixelMap {
@@ -469,10 +677,29 @@ class PixelMap {
   size_t getNy() const { return Ny_; }
   double getResolution() const { return res; }
   const std::vector<T>& data() const { return map; }
+  void setRAandDec(double RAin, double DECin) {
+    RA = RAin;
+    DEC = DECin;
+  }
+  // pixelmap.cpp:1318-1388: a FITS primary image with the reference's keys
+  // (written from the FITS standard; the reference uses cfitsio)
+  void printFITS(std::string filename, bool flipX = false, bool ctype = false,
+                 bool verbose = false) {
+    (void)verbose;
+    static_assert(std::is_same<T, double>::value || std::is_same<T, float>::value,
+                  "printFITS writes double or float maps");
+    if (filename.empty())
+      throw std::invalid_argument("Please enter a valid filename for the FITS file output");
+    slsim_cpp::check(slsim_fits_write_pixelmap(filename.c_str(), map.data(),
+                                               std::is_same<T, double>::value, Nx_, Ny_, res, c,
+                                               RA, DEC, units, flipX, ctype));
+  }
  private:
   size_t Nx_, Ny_;
   double res;
   double c[2];
+  double RA = 0, DEC = 0;
+  int units = 0;   // PixelMapUnits::ndef
   std::vector<T> map;
 };
 
@@ -564,6 +791,13 @@ struct GridMap {
       }
     }
     return map;
+  }
+
+  // gridmap.h:524-531
+  template <typename T>
+  void writeFits(LensingVariable lensvar, std::string filename) {
+    PixelMap<T> map = writePixelMap<T>(lensvar);
+    map.printFITS(filename);
   }
 
  private:

@@ -29,7 +29,9 @@ enum {
   SLSIM_ERR_SAME_PLACE = 4,   /* "particles in same place." (quadTree.h:327-329)  */
   SLSIM_ERR_TOO_DEEP = 5,     /* a cell below level 32 still holds > bucket       */
   SLSIM_ERR_NOT_FLAT = 6,     /* Om + OL != 1 (MultiPlane/lens.cpp:43-47)         */
-  SLSIM_ERR_STATE = 7         /* call out of order (e.g. serving not started)     */
+  SLSIM_ERR_STATE = 7,        /* call out of order (e.g. serving not started)     */
+  SLSIM_ERR_IO = 8            /* file missing, unreadable or malformed (reference:
+                                 std::runtime_error "file reading error" or exit) */
 };
 
 /* Arithmetic of the tree walk.  The plane-to-plane recursion is FP64 in both. */
@@ -287,6 +289,138 @@ int slsim_gen_nfw(uint64_t seed, size_t n, double mass, double conc, double z, d
 int slsim_gen_halo_field(uint64_t seed, size_t n, int n_halos, double Mmin, double Mmax,
                          double field_arcsec, double z, double Om, double OL, double h,
                          float softening, double* x, double* y, float* m, float* s);
+
+/* ------------------------------------------------------- particle files */
+/* ParticleType<float> (include/particle_types.h:46-58): the record of
+ * MakeParticleLenses::data and of glmb files, 24 bytes. */
+typedef struct {
+  float x[3];
+  float Mass;
+  float Size;
+  int32_t type;
+} slsim_particle_f32;
+
+/* SimFileFormat (include/particle_types.h:32-43), same order. */
+enum {
+  SLSIM_FMT_GLMB = 0,
+  SLSIM_FMT_CSV3 = 1,
+  SLSIM_FMT_CSV4 = 2,
+  SLSIM_FMT_CSV5 = 3,
+  SLSIM_FMT_CSV6 = 4,
+  SLSIM_FMT_GADGET2 = 5,
+  SLSIM_FMT_ASCII = 6
+};
+
+/* MakeParticleLenses (include/particle_halo.h:964-1080, MultiPlane/
+ * particle_lens.cpp): the particles of one simulation file, grouped by type. */
+typedef struct slsim_mpl_s* slsim_mpl;
+typedef struct {
+  size_t nparticles[6];         /* getnp()                                        */
+  float masses[6];              /* getmp(): per-type mass, 0 = individual masses  */
+  double z_original;            /* sim_redshift(); -1 for csv files               */
+  double cm[3];                 /* getCenterOfMass()                              */
+  double bbox_ll[3], bbox_ur[3];/* getBoundingBox()                               */
+  int32_t nsmooth;
+  int32_t pad;
+} slsim_mpl_info;
+
+/* MakeParticleLenses(filename, format, Nsmooth, recenter, ignore_type_in_
+ * smoothing) (particle_lens.cpp:22-107): a glmb file is read as is; otherwise
+ * the cache "<filename>_S<Nsmooth>.glmb" is read when it exists, else the
+ * csv3/csv4/csv5 table or Gadget-2 snapshot is read, smoothing sizes are
+ * computed on the device per particle type (csv3, csv4, gadget2) and the cache
+ * is written.  Then the centre of mass and bounding box; recenter subtracts the
+ * centre of mass.  csv6 and ascii fail with SLSIM_ERR_ARG as in the reference
+ * (its csv6 case falls through to the unsupported-format branch).  Errors:
+ * SLSIM_ERR_ARG (too few columns, a field stof rejects, bad format),
+ * SLSIM_ERR_IO (missing or malformed file). */
+int slsim_mpl_create(slsim_ctx ctx, const char* filename, int format, int nsmooth, int recenter,
+                     int ignore_type_in_smoothing, slsim_mpl* out);
+/* MakeParticleLenses(filename, recenter) for a glmb file (particle_lens.cpp:
+ * 109-139; its bounding box starts from the origin, as there). */
+int slsim_mpl_create_glmb(slsim_ctx ctx, const char* filename, int recenter, slsim_mpl* out);
+int slsim_mpl_destroy(slsim_mpl m);
+/* MakeParticleLenses::data: the library's particle array (valid until the next
+ * call that changes the particle count, or destroy); writable. */
+int slsim_mpl_data(slsim_mpl m, slsim_particle_f32** data, size_t* n);
+int slsim_mpl_get_info(slsim_mpl m, slsim_mpl_info* info);
+/* Recenter(Point_3d x) (particle_lens.cpp:139-151). */
+int slsim_mpl_recenter(slsim_mpl m, const double x[3]);
+/* radialCut / cylindricalCut (particle_lens.cpp:447-478, 497-528): drops particles
+ * farther than radius, then sorts by type when there is more than one. */
+int slsim_mpl_radial_cut(slsim_mpl m, const double center[3], double radius);
+int slsim_mpl_cylindrical_cut(slsim_mpl m, const double center[2], double radius);
+/* densest_particle() (particle_lens.cpp:480-495): max of Mass/Size^3. */
+int slsim_mpl_densest_particle(slsim_mpl m, double x[3]);
+/* The unit conversion of CreateHalos(cosmo, redshift, inv_area) (particle_lens.
+ * cpp:153-170): positions, sizes and masses times 1/h (h rounded to float, as
+ * there).  The caller then builds one plane per type with nparticles > 0 over
+ * data[skip, skip + nparticles[i]) (slsim_plane_create_f32, theta 0.1, bucket
+ * 20, inv_area), as CreateHalos does with LensHaloParticles. */
+int slsim_mpl_physical_units(slsim_mpl m, float hubble);
+
+/* MakeParticleLenses::writeSizesB / readSizesB (particle_halo.h:1037-1073):
+ * int32 Nsmooth, uint64 n, uint64 numbytype[6], double redshift, then n
+ * slsim_particle_f32 records, native byte order. */
+int slsim_glmb_write(const char* filename, const slsim_particle_f32* p, size_t n, int nsmooth,
+                     const size_t numbytype[6], double redshift);
+int slsim_glmb_read_header(const char* filename, int* nsmooth, size_t* n, size_t numbytype[6],
+                           double* redshift);
+int slsim_glmb_read(const char* filename, slsim_particle_f32* p, size_t n);
+
+/* GadgetFile<ParticleType<float>> (include/gadget.hh) as readGadget2 uses it:
+ * constructor + openFile (gadget.hh:83-268, read_gadget_head :296-330),
+ * readBlock("POS") and readBlock("MASS") (:567-610).  SnapFormat 2 (each block
+ * behind an 8-byte label record), either byte order, one file (a snapshot
+ * split over "<name>.0", "<name>.1", ... is refused).  Particles in file
+ * units, type from the block order, Mass from masstab or the MASS block.
+ * Errors: SLSIM_ERR_IO (no HEAD/POS/MASS block, sizes that disagree with
+ * the header; the reference exits). */
+int slsim_gadget2_header(const char* filename, int32_t npart[6], double masstab[6], double* time,
+                         double* redshift, double* omega, double* lambda, double* hubble);
+int slsim_gadget2_read(const char* filename, slsim_particle_f32* p, size_t n);
+
+/* LensHaloParticles<PType>(simulation_filename, SimFileFormat::ascii,
+ * redshift, Nsmooth, cosmo, theta_rotate, recenter, multimass, inv_area,
+ * MinPSize, rescale_mass) (particle_halo.h:226-340), all but the tree:
+ * readPositionFileASCII (:453-555: "# nparticles N" header, 4 columns
+ * x y z mass when multimass, else 3 columns), sizes from "<file>.<Nsmooth>
+ * sizes" (readSizesFile :557-632) or computed on the device and written there
+ * (writeSizes :734-762, six significant digits), MinPSize, rescale_mass (all
+ * particles but the first, as there), comoving -> physical (1/(1+z)), centre
+ * of mass, recentring, rotation.  The caller builds the plane from the outputs
+ * (slsim_plane_create_*, theta 0.1, bucket 20, inv_area).  Call _count first
+ * for n. */
+typedef struct {
+  double mass;                  /* LensHalo::getMass()                            */
+  double mcenter[3];            /* centre of mass (0 after recentring)            */
+  double densest_point[3];      /* position of the smallest particle              */
+  double rsize;                 /* 1e3, or the largest radius after recentring    */
+} slsim_ascii_halo_info;
+int slsim_ascii_halo_count(const char* filename, int multimass, size_t* n);
+int slsim_ascii_halo_load_f64(slsim_ctx ctx, const char* filename, double redshift, int nsmooth,
+                              const double theta_rotate[2], int recenter, int multimass,
+                              double min_size, double rescale_mass, size_t n, double* xyz,
+                              float* mass, float* size, slsim_ascii_halo_info* info);
+int slsim_ascii_halo_load_f32(slsim_ctx ctx, const char* filename, double redshift, int nsmooth,
+                              const double theta_rotate[2], int recenter, int multimass,
+                              double min_size, double rescale_mass, size_t n, float* xyz,
+                              float* mass, float* size, slsim_ascii_halo_info* info);
+
+/* ------------------------------------------------------------ pixel maps */
+/* PixelMap<T>::printFITS(filename, flipX, ctype) (ImageProcessing/pixelmap.cpp:
+ * 1318-1388), the writer behind GridMap::writeFits (include/gridmap.h:524-531):
+ * a FITS primary image (BITPIX -64 for double maps, -32 for float), nx the
+ * fast axis, then the keys WCSAXES, CRPIX1/2, CROTA2, CD1_1..CD2_2, Nx, Ny,
+ * "range x", RA_global, DEC_global, center_x/y, CRVAL1/2, UNITS (and CTYPE*,
+ * RADESYS, CUNIT* when ctype).  Written from the FITS standard (the reference
+ * uses cfitsio): keys longer than 8 characters or with spaces use the HIERARCH
+ * convention.  A leading '!' in filename is dropped; existing files are
+ * replaced.  units: PixelMapUnits (ndef, surfb, count_per_sec, ADU, mass,
+ * mass_density). */
+int slsim_fits_write_pixelmap(const char* filename, const void* map, int is_double, size_t nx,
+                              size_t ny, double resolution, const double center[2], double ra,
+                              double dec, int units, int flipx, int ctype);
 
 #ifdef __cplusplus
 }

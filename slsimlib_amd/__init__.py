@@ -12,6 +12,8 @@ import numpy as np
 
 from . import _abi
 from ._abi import SLSIM_DIRECT64, SLSIM_FP32C, SLSIM_FP64, SlsimError, lib
+from ._abi import (SLSIM_FMT_ASCII, SLSIM_FMT_CSV3, SLSIM_FMT_CSV4, SLSIM_FMT_CSV5,
+                   SLSIM_FMT_CSV6, SLSIM_FMT_GADGET2, SLSIM_FMT_GLMB)
 
 arcsecTOradians = 0.000004848136811   # include/standard.h:51
 Grav = 4.7788e-20                      # include/standard.h:43
@@ -20,9 +22,16 @@ __all__ = [
     "Context", "LensHaloParticles", "Lens", "GridMap", "Batch", "Server", "coorDist",
     "invCoorDist",
     "nfw_r200", "gen_nfw", "gen_halo_field", "SLSIM_FP64", "SLSIM_FP32C", "SLSIM_DIRECT64",
-    "SlsimError",
+    "SlsimError", "MakeParticleLenses", "PARTICLE_DTYPE", "glmb_write", "glmb_read",
+    "gadget2_read",
+    "write_fits_pixelmap", "SLSIM_FMT_GLMB", "SLSIM_FMT_CSV3", "SLSIM_FMT_CSV4",
+    "SLSIM_FMT_CSV5", "SLSIM_FMT_CSV6", "SLSIM_FMT_GADGET2", "SLSIM_FMT_ASCII",
     "arcsecTOradians", "Grav", "lib",
 ]
+
+# ParticleType<float> (include/particle_types.h:46-58), 24 bytes
+PARTICLE_DTYPE = np.dtype([("x", "<f4", (3,)), ("Mass", "<f4"), ("Size", "<f4"),
+                           ("type", "<i4")])
 
 
 def _ptr(a, ctype):
@@ -124,6 +133,48 @@ class LensHaloParticles:
         _check(fn(ctx.h, _ptr(xs, ct), _ptr(ys, ct), _ptr(zs, ct), ctypes.c_size_t(xs.size),
                   int(Nsmooth), _ptr(out, ctypes.c_float)))
         return out
+
+    @classmethod
+    def from_ascii(cls, ctx, filename, redshift, Nsmooth, theta_rotate=(0.0, 0.0),
+                   recenter=False, multimass=True, inv_area=0.0, MinPSize=0.0,
+                   rescale_mass=1.0, dtype=np.float64):
+        """LensHaloParticles<PType>(simulation_filename, SimFileFormat::ascii,
+        redshift, Nsmooth, cosmo, theta_rotate, recenter, multimass, inv_area,
+        MinPSize, rescale_mass) (particle_halo.h:226-340): positions from the
+        ASCII file, sizes from "<file>.<Nsmooth>sizes" or computed on the
+        device and written there, then the tree (theta 0.1, bucket 20).
+        dtype float64 -> ParticleType<double>, float32 -> ParticleType<float>.
+        The halo keeps .xyz, .mass, .size, .halo_info."""
+        xyz, m, h, info = cls.load_ascii(ctx, filename, redshift, Nsmooth, theta_rotate, recenter,
+                                         multimass, MinPSize, rescale_mass, dtype)
+        halo = cls(ctx, np.ascontiguousarray(xyz[:, 0]), np.ascontiguousarray(xyz[:, 1]), m, h,
+                   redshift, theta=0.1, bucket=20, inv_area=inv_area)
+        halo.xyz, halo.mass, halo.size, halo.halo_info = xyz, m, h, info
+        return halo
+
+    @staticmethod
+    def load_ascii(ctx, filename, redshift, Nsmooth, theta_rotate=(0.0, 0.0), recenter=False,
+                   multimass=True, MinPSize=0.0, rescale_mass=1.0, dtype=np.float64):
+        """The particle set of from_ascii without building a tree:
+        (xyz, mass, size, info)."""
+        fn = os.fsencode(filename)
+        n = ctypes.c_size_t()
+        _check(lib.slsim_ascii_halo_count(fn, int(bool(multimass)), ctypes.byref(n)))
+        n = n.value
+        t = np.float32 if np.dtype(dtype) == np.float32 else np.float64
+        xyz = np.zeros((n, 3), t)
+        m = np.zeros(n, np.float32)
+        h = np.zeros(n, np.float32)
+        th = np.array([theta_rotate[0], theta_rotate[1]], np.float64)
+        info = _abi.slsim_ascii_halo_info()
+        fn_load = lib.slsim_ascii_halo_load_f32 if t == np.float32 else lib.slsim_ascii_halo_load_f64
+        ct = ctypes.c_float if t == np.float32 else ctypes.c_double
+        _check(fn_load(ctx.h if ctx is not None else None, fn, float(redshift), int(Nsmooth),
+                       _ptr(th, ctypes.c_double), int(bool(recenter)), int(bool(multimass)),
+                       float(MinPSize), float(rescale_mass), ctypes.c_size_t(n), _ptr(xyz, ct),
+                       _ptr(m, ctypes.c_float), _ptr(h, ctypes.c_float), ctypes.byref(info)))
+        return xyz, m, h, {"mass": info.mass, "mcenter": tuple(info.mcenter),
+                           "densest_point": tuple(info.densest_point), "rsize": info.rsize}
 
     def info(self):
         i = _abi.slsim_plane_info()
@@ -329,6 +380,13 @@ class GridMap:
             raise ValueError("GridMap::writePixelMap() does not work for the input LensingVariable")
         return r.reshape(self.N, self.N)
 
+    def writeFits(self, var, filename, dtype=np.float64):
+        """GridMap::writeFits<T>(lensvar, filename) (gridmap.h:524-531): the
+        map of writePixelMap written by PixelMap::printFITS, resolution
+        range / (N - 1)."""
+        write_fits_pixelmap(filename, np.asarray(self.writePixelMap(var), dtype),
+                            self.range / (self.N - 1), self.center)
+
     def close(self):
         if self.y_dev:
             self.ctx.free(self.y_dev)
@@ -459,6 +517,171 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class MakeParticleLenses:
+    """MakeParticleLenses (particle_halo.h:964-1080, MultiPlane/
+    particle_lens.cpp): the particles of a simulation file grouped by type.
+
+    MakeParticleLenses(ctx, filename, format, Nsmooth, recenter,
+    ignore_type_in_smoothing) reads csv3/csv4/csv5, Gadget-2 or glmb files
+    (smoothing sizes on the device, cached in "<filename>_S<Nsmooth>.glmb");
+    MakeParticleLenses(ctx, filename, recenter=...) with format=None opens a
+    glmb file.  ctx may be None when no sizes have to be computed."""
+
+    def __init__(self, ctx, filename, format=None, Nsmooth=64, recenter=False,
+                 ignore_type_in_smoothing=False):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        ch = ctx.h if ctx is not None else None
+        fn = os.fsencode(filename)
+        if format is None:
+            _check(lib.slsim_mpl_create_glmb(ch, fn, int(bool(recenter)), ctypes.byref(self.h)))
+        else:
+            _check(lib.slsim_mpl_create(ch, fn, int(format), int(Nsmooth), int(bool(recenter)),
+                                        int(bool(ignore_type_in_smoothing)),
+                                        ctypes.byref(self.h)))
+        self.halos = []
+
+    @property
+    def data(self):
+        """The library's particle array as a structured numpy view
+        (PARTICLE_DTYPE), valid until the particle count changes."""
+        p = ctypes.POINTER(_abi.slsim_particle_f32)()
+        n = ctypes.c_size_t()
+        _check(lib.slsim_mpl_data(self.h, ctypes.byref(p), ctypes.byref(n)))
+        if n.value == 0:
+            return np.zeros(0, PARTICLE_DTYPE)
+        buf = (ctypes.c_char * (24 * n.value)).from_address(ctypes.addressof(p.contents))
+        return np.frombuffer(buf, PARTICLE_DTYPE)
+
+    def _info(self):
+        i = _abi.slsim_mpl_info()
+        _check(lib.slsim_mpl_get_info(self.h, ctypes.byref(i)))
+        return i
+
+    def getnp(self):
+        return [int(v) for v in self._info().nparticles]
+
+    def getmp(self):
+        return [float(v) for v in self._info().masses]
+
+    def sim_redshift(self):
+        return self._info().z_original
+
+    getZoriginal = sim_redshift
+
+    def getCenterOfMass(self):
+        return tuple(self._info().cm)
+
+    def getBoundingBox(self):
+        i = self._info()
+        return tuple(i.bbox_ll), tuple(i.bbox_ur)
+
+    def nsmooth(self):
+        return self._info().nsmooth
+
+    def Recenter(self, x):
+        c = np.asarray(x, np.float64)
+        _check(lib.slsim_mpl_recenter(self.h, _ptr(c, ctypes.c_double)))
+        self.halos = []
+
+    def radialCut(self, center, radius):
+        c = np.asarray(center, np.float64)
+        _check(lib.slsim_mpl_radial_cut(self.h, _ptr(c, ctypes.c_double), float(radius)))
+
+    def cylindricalCut(self, center, radius):
+        c = np.asarray(center, np.float64)
+        _check(lib.slsim_mpl_cylindrical_cut(self.h, _ptr(c, ctypes.c_double), float(radius)))
+
+    def densest_particle(self):
+        x = np.zeros(3, np.float64)
+        _check(lib.slsim_mpl_densest_particle(self.h, _ptr(x, ctypes.c_double)))
+        return tuple(x)
+
+    def CreateHalos(self, hubble, redshift, inv_area):
+        """CreateHalos(cosmo, redshift, inv_area) (particle_lens.cpp:153-209):
+        units times 1/h, then one LensHaloParticles (theta 0.1, bucket 20,
+        no recentring or rotation) per particle type with particles."""
+        _check(lib.slsim_mpl_physical_units(self.h, ctypes.c_float(hubble)))
+        d = self.data
+        self.halos = []
+        skip = 0
+        for n in self.getnp():
+            if n > 0:
+                part = d[skip:skip + n]
+                self.halos.append(LensHaloParticles(
+                    self.ctx, np.ascontiguousarray(part["x"][:, 0]),
+                    np.ascontiguousarray(part["x"][:, 1]), part["Mass"], part["Size"],
+                    redshift, theta=0.1, bucket=20, inv_area=inv_area))
+            skip += n
+        return self.halos
+
+    def close(self):
+        if self.h:
+            lib.slsim_mpl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def glmb_write(filename, particles, nsmooth, numbytype, redshift):
+    """MakeParticleLenses::writeSizesB (particle_halo.h:1037-1054)."""
+    p = np.ascontiguousarray(particles, PARTICLE_DTYPE)
+    nb = np.asarray(numbytype, np.uint64).astype(np.uintp)
+    _check(lib.slsim_glmb_write(os.fsencode(filename), p.ctypes.data_as(ctypes.c_void_p),
+                                ctypes.c_size_t(p.size), int(nsmooth),
+                                nb.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t)),
+                                float(redshift)))
+
+
+def glmb_read(filename):
+    """MakeParticleLenses::readSizesB (particle_halo.h:1056-1073):
+    (particles, nsmooth, numbytype, redshift)."""
+    ns = ctypes.c_int()
+    n = ctypes.c_size_t()
+    nb = (ctypes.c_size_t * 6)()
+    z = ctypes.c_double()
+    fn = os.fsencode(filename)
+    _check(lib.slsim_glmb_read_header(fn, ctypes.byref(ns), ctypes.byref(n), nb, ctypes.byref(z)))
+    p = np.zeros(n.value, PARTICLE_DTYPE)
+    _check(lib.slsim_glmb_read(fn, p.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(n.value)))
+    return p, ns.value, [int(v) for v in nb], z.value
+
+
+def gadget2_read(filename):
+    """GadgetFile<ParticleType<float>> header + readBlock("POS"/"MASS")
+    (include/gadget.hh): (header dict, particles in file units)."""
+    npart = (ctypes.c_int32 * 6)()
+    mt = (ctypes.c_double * 6)()
+    v = [ctypes.c_double() for _ in range(5)]
+    fn = os.fsencode(filename)
+    _check(lib.slsim_gadget2_header(fn, npart, mt, *[ctypes.byref(x) for x in v]))
+    n = sum(npart)
+    p = np.zeros(n, PARTICLE_DTYPE)
+    _check(lib.slsim_gadget2_read(fn, p.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(n)))
+    hd = {"npart": list(npart), "masstab": list(mt), "time": v[0].value, "redshift": v[1].value,
+          "omega": v[2].value, "lambda": v[3].value, "hubble": v[4].value}
+    return hd, p
+
+
+def write_fits_pixelmap(filename, image, resolution, center, ra=0.0, dec=0.0, units=0,
+                        flipX=False, ctype=False):
+    """PixelMap<T>::printFITS(filename, flipX, ctype) (ImageProcessing/
+    pixelmap.cpp:1318-1388) for an (Ny, Nx) float64 or float32 image."""
+    im = np.ascontiguousarray(image)
+    if im.dtype not in (np.float64, np.float32) or im.ndim != 2:
+        raise ValueError("image must be a 2-D float64 or float32 array")
+    c = np.asarray(center, np.float64)
+    _check(lib.slsim_fits_write_pixelmap(os.fsencode(filename), im.ctypes.data_as(ctypes.c_void_p),
+                                         int(im.dtype == np.float64), ctypes.c_size_t(im.shape[1]),
+                                         ctypes.c_size_t(im.shape[0]), float(resolution),
+                                         _ptr(c, ctypes.c_double), float(ra), float(dec),
+                                         int(units), int(bool(flipX)), int(bool(ctype))))
 
 
 def now():

@@ -13,6 +13,7 @@ SLSIM_ERR_SAME_PLACE = 4
 SLSIM_ERR_TOO_DEEP = 5
 SLSIM_ERR_NOT_FLAT = 6
 SLSIM_ERR_STATE = 7
+SLSIM_ERR_IO = 8
 
 SLSIM_FP64 = 0
 SLSIM_FP32C = 1
@@ -20,8 +21,12 @@ SLSIM_DIRECT64 = 2
 
 ERROR_NAMES = {
     1: "SLSIM_ERR_ARG", 2: "SLSIM_ERR_HIP", 3: "SLSIM_ERR_BUDGET", 4: "SLSIM_ERR_SAME_PLACE",
-    5: "SLSIM_ERR_TOO_DEEP", 6: "SLSIM_ERR_NOT_FLAT", 7: "SLSIM_ERR_STATE",
+    5: "SLSIM_ERR_TOO_DEEP", 6: "SLSIM_ERR_NOT_FLAT", 7: "SLSIM_ERR_STATE", 8: "SLSIM_ERR_IO",
 }
+
+# SimFileFormat (include/particle_types.h:32-43)
+SLSIM_FMT_GLMB, SLSIM_FMT_CSV3, SLSIM_FMT_CSV4, SLSIM_FMT_CSV5, SLSIM_FMT_CSV6, \
+    SLSIM_FMT_GADGET2, SLSIM_FMT_ASCII = range(7)
 
 
 class SlsimError(RuntimeError):
@@ -40,6 +45,23 @@ lib = ctypes.CDLL(LIB_PATH)
 slsim_ctx = ctypes.c_void_p
 slsim_plane = ctypes.c_void_p
 slsim_lens = ctypes.c_void_p
+
+
+class slsim_particle_f32(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float * 3), ("Mass", ctypes.c_float), ("Size", ctypes.c_float),
+                ("type", ctypes.c_int32)]
+
+
+class slsim_mpl_info(ctypes.Structure):
+    _fields_ = [("nparticles", ctypes.c_size_t * 6), ("masses", ctypes.c_float * 6),
+                ("z_original", ctypes.c_double), ("cm", ctypes.c_double * 3),
+                ("bbox_ll", ctypes.c_double * 3), ("bbox_ur", ctypes.c_double * 3),
+                ("nsmooth", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class slsim_ascii_halo_info(ctypes.Structure):
+    _fields_ = [("mass", ctypes.c_double), ("mcenter", ctypes.c_double * 3),
+                ("densest_point", ctypes.c_double * 3), ("rsize", ctypes.c_double)]
 
 
 class slsim_plane_info(ctypes.Structure):
@@ -83,6 +105,7 @@ slsim_server = ctypes.c_void_p
 P = ctypes.POINTER
 c_int, c_double, c_float, c_size_t = ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_size_t
 c_u64, c_i64, c_vp = ctypes.c_uint64, ctypes.c_int64, ctypes.c_void_p
+c_char_p = ctypes.c_char_p
 
 SIGNATURES = {
     "slsim_last_error": (ctypes.c_char_p, []),
@@ -142,6 +165,32 @@ SIGNATURES = {
                                           c_size_t, c_int, P(c_float)]),
     "slsim_smoothing_sizes_f32": (c_int, [slsim_ctx, P(c_float), P(c_float), P(c_float),
                                           c_size_t, c_int, P(c_float)]),
+    "slsim_mpl_create": (c_int, [slsim_ctx, c_char_p, c_int, c_int, c_int, c_int, P(c_vp)]),
+    "slsim_mpl_create_glmb": (c_int, [slsim_ctx, c_char_p, c_int, P(c_vp)]),
+    "slsim_mpl_destroy": (c_int, [c_vp]),
+    "slsim_mpl_data": (c_int, [c_vp, P(P(slsim_particle_f32)), P(c_size_t)]),
+    "slsim_mpl_get_info": (c_int, [c_vp, P(slsim_mpl_info)]),
+    "slsim_mpl_recenter": (c_int, [c_vp, P(c_double)]),
+    "slsim_mpl_radial_cut": (c_int, [c_vp, P(c_double), c_double]),
+    "slsim_mpl_cylindrical_cut": (c_int, [c_vp, P(c_double), c_double]),
+    "slsim_mpl_densest_particle": (c_int, [c_vp, P(c_double)]),
+    "slsim_mpl_physical_units": (c_int, [c_vp, c_float]),
+    "slsim_glmb_write": (c_int, [c_char_p, c_vp, c_size_t, c_int, P(c_size_t), c_double]),
+    "slsim_glmb_read_header": (c_int, [c_char_p, P(c_int), P(c_size_t), P(c_size_t),
+                                       P(c_double)]),
+    "slsim_glmb_read": (c_int, [c_char_p, c_vp, c_size_t]),
+    "slsim_gadget2_header": (c_int, [c_char_p, P(ctypes.c_int32), P(c_double), P(c_double),
+                                     P(c_double), P(c_double), P(c_double), P(c_double)]),
+    "slsim_gadget2_read": (c_int, [c_char_p, c_vp, c_size_t]),
+    "slsim_ascii_halo_count": (c_int, [c_char_p, c_int, P(c_size_t)]),
+    "slsim_ascii_halo_load_f64": (c_int, [slsim_ctx, c_char_p, c_double, c_int, P(c_double), c_int,
+                                          c_int, c_double, c_double, c_size_t, P(c_double),
+                                          P(c_float), P(c_float), P(slsim_ascii_halo_info)]),
+    "slsim_ascii_halo_load_f32": (c_int, [slsim_ctx, c_char_p, c_double, c_int, P(c_double), c_int,
+                                          c_int, c_double, c_double, c_size_t, P(c_float),
+                                          P(c_float), P(c_float), P(slsim_ascii_halo_info)]),
+    "slsim_fits_write_pixelmap": (c_int, [c_char_p, c_vp, c_int, c_size_t, c_size_t, c_double,
+                                          P(c_double), c_double, c_double, c_int, c_int, c_int]),
     "slsim_coordist": (c_double, [c_double, c_double, c_double, c_double, c_double]),
     "slsim_inv_coordist": (c_double, [c_double, c_double, c_double, c_double]),
     "slsim_nfw_r200": (c_double, [c_double, c_double, c_double, c_double, c_double]),

@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libslsim_amd.so")
 
-SOURCES = ["tree_build.hip", "shoot.hip", "api.hip", "batch.hip", "smoothing.hip", "generator.cpp"]
+SOURCES = ["tree_build.hip", "shoot.hip", "api.hip", "batch.hip", "smoothing.hip", "generator.cpp",
+           "formats.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -61,19 +62,23 @@ def build(verbose=False, jobs=4):
 
 
 def build_cpp_example(verbose=False):
-    """Compiles tests/cpp/dropin_gridmap.cpp, a reference-style C++ program
-    against include/glamer_amd.h linked with the library."""
+    """Compiles tests/cpp/dropin_gridmap.cpp and tests/cpp/dropin_formats.cpp,
+    reference-style C++ programs against include/glamer_amd.h linked with
+    the library."""
     root = os.path.dirname(HERE)
-    src = os.path.join(root, "tests", "cpp", "dropin_gridmap.cpp")
-    exe = os.path.join(root, "tests", "cpp", "dropin_gridmap")
-    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "include"), src,
-           "-L", HERE, "-lslsim_amd", "-Wl,-rpath,$ORIGIN/../../slsimlib_amd", "-o", exe]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
-    if out.returncode != 0:
-        raise RuntimeError("C++ example build failed:\n" + out.stdout.decode(errors="replace"))
-    return exe
+    exes = []
+    for name in ("dropin_gridmap", "dropin_formats"):
+        src = os.path.join(root, "tests", "cpp", name + ".cpp")
+        exe = os.path.join(root, "tests", "cpp", name)
+        cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "include"), src,
+               "-L", HERE, "-lslsim_amd", "-Wl,-rpath,$ORIGIN/../../slsimlib_amd", "-o", exe]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+        if out.returncode != 0:
+            raise RuntimeError("C++ example build failed:\n" + out.stdout.decode(errors="replace"))
+        exes.append(exe)
+    return exes
 
 
 def _wait(item):
