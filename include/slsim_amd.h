@@ -77,6 +77,23 @@ int slsim_plane_create_f32(slsim_ctx ctx, const float* x, const float* y, const 
                            slsim_plane* out);
 int slsim_plane_destroy(slsim_plane plane);
 
+/* LensHaloHalos<LensHaloNFW>(halos, redshift, cosmo, inv_area, theta_force)
+ * (include/particle_halo.h:1092-1227): a main halo made of n symmetric NFW
+ * halos, each LensHaloNFW(mass, Rsize, zlens, concentration, fratio = 1)
+ * (MultiPlane/lens_halos.cpp:389-440) at physical position (x, y) (what
+ * LensHalo::getX returns), on a TreeQuadHalos quadtree with bucket 5 and
+ * opening angle theta_force (include/quadTreeHalos.h).  The plane it returns
+ * goes into a lens with slsim_lens_add_halo like a particle plane, alone or
+ * merged with particle halos at the same redshift; slsim_plane_force gives
+ * TreeQuadHalos::force2D and slsim_plane_direct_sum the sum over every halo.
+ * Its walk is FP64 in every precision mode; n < 2^22.  Errors: SLSIM_ERR_ARG (Rsize or
+ * concentration <= 0, Rsize/rscale >= 99.98 where LensHaloNFW's tables end,
+ * negative mass), SLSIM_ERR_SAME_PLACE (all halos on one line), SLSIM_ERR_TOO_DEEP
+ * (more than 5 halos in one place). */
+int slsim_halos_create_nfw(slsim_ctx ctx, const double* x, const double* y, const float* mass,
+                           const float* rsize, const float* concentration, size_t n,
+                           float inv_area, float theta_force, slsim_plane* out);
+
 typedef struct {
   int64_t n_particles;
   int64_t n_nodes;

@@ -49,6 +49,14 @@ _sigs = {
     "orc_lens_create": (c_vp, [c_d, c_d, c_d, c_d]),
     "orc_lens_free": (None, [c_vp]),
     "orc_lens_add_halo": (None, [c_vp, c_vp, c_f]),
+    "orc_halotree_build": (c_vp, [P(c_d), P(c_d), P(c_f), P(c_f), P(c_f), c_sz, c_f, c_f,
+                                  P(c_int)]),
+    "orc_halotree_free": (None, [c_vp]),
+    "orc_halotree_force": (None, [c_vp, P(c_d), c_sz, c_int, P(c_d)]),
+    "orc_nfw_force": (None, [c_f, c_f, c_f, P(c_d), c_sz, P(c_d)]),
+    "orc_nfw_function": (c_d, [c_int, c_d]),
+    "orc_nfw_interpolate": (c_d, [c_int, c_d]),
+    "orc_lens_add_halotree": (None, [c_vp, c_vp, c_f]),
     "orc_lens_nplanes": (c_int, [c_vp]),
     "orc_lens_tables": (None, [c_vp, P(c_d), P(c_d), P(c_d), P(c_d)]),
     "orc_lens_shoot": (None, [c_vp, c_sz, P(c_d), P(c_d), P(c_d), P(c_d), c_int, c_vp]),
@@ -150,6 +158,55 @@ def bspline(xcm, r, mass, size):
     return out
 
 
+class HaloTree:
+    """LensHaloHalos<LensHaloNFW>'s TreeQuadHalos (include/quadTreeHalos.h):
+    NFW halos (float mass, Rsize, concentration) at physical positions."""
+
+    def __init__(self, x, y, m, rsize, conc, inv_area=0.0, theta=0.1):
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.m = np.ascontiguousarray(m, dtype=np.float32)
+        self.rsize = np.ascontiguousarray(rsize, dtype=np.float32)
+        self.conc = np.ascontiguousarray(conc, dtype=np.float32)
+        st = c_int()
+        self.t = _lib.orc_halotree_build(_p(self.x), _p(self.y), _p(self.m, c_f),
+                                         _p(self.rsize, c_f), _p(self.conc, c_f), self.x.size,
+                                         np.float32(inv_area), np.float32(theta),
+                                         ctypes.byref(st))
+        if st.value:
+            raise ValueError("halo tree build failed (halos in one place or out of the tables)")
+        self.owned = True
+
+    def force(self, rays, direct=False):
+        rays = np.ascontiguousarray(rays, dtype=np.float64).reshape(-1, 2)
+        out = np.zeros((rays.shape[0], 5))
+        _lib.orc_halotree_force(self.t, _p(rays), rays.shape[0], int(bool(direct)), _p(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "owned", False) and getattr(self, "t", None):
+            _lib.orc_halotree_free(self.t)
+
+
+def nfw_force(m, rsize, conc, xcm):
+    """One LensHaloNFW's force_halo at positions relative to it: (n, 5)."""
+    xcm = np.ascontiguousarray(xcm, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros((xcm.shape[0], 5))
+    _lib.orc_nfw_force(np.float32(m), np.float32(rsize), np.float32(conc), _p(xcm),
+                       xcm.shape[0], _p(out))
+    return out
+
+
+def nfw_function(which, x):
+    """LensHaloNFW's gfunction (0), ffunction (1), g2function (2)."""
+    return _lib.orc_nfw_function(int(which), float(x))
+
+
+def nfw_interpolate(which, x):
+    """InterpolateFromTable of the g (0), f (1), g2 (2) tables."""
+    return _lib.orc_nfw_interpolate(int(which), float(x))
+
+
 class Lens:
     """Lens with moveinMainHalo'd particle halos; shoot = rayshooterInternal."""
 
@@ -158,7 +215,10 @@ class Lens:
 
     def moveinMainHalo(self, tree, z):
         tree.owned = False          # the lens owns it now
-        _lib.orc_lens_add_halo(self.L, tree.t, np.float32(z))
+        if isinstance(tree, HaloTree):
+            _lib.orc_lens_add_halotree(self.L, tree.t, np.float32(z))
+        else:
+            _lib.orc_lens_add_halo(self.L, tree.t, np.float32(z))
 
     def tables(self):
         n = _lib.orc_lens_nplanes(self.L)

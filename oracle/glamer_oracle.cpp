@@ -25,6 +25,10 @@
 //   TreeSimple ctor / BuildTreeNB / _BuildTreeNB (median cut)
 //                                        include/simpleTree.h:264-278, 842-960
 //   TreeSimple::NNDistance / _findleaf   include/simpleTree.h:483-640
+//   LensHaloNFW tables, functions, force_halo  MultiPlane/lens_halos.cpp:352-581,
+//                                        1528-1672; AnalyticNSIE/nfw_lens.cpp:82-125
+//   TreeQuadHalos build / CalcMoments / walkTree_iter  include/quadTreeHalos.h:127-735
+//   LensHaloHalos::force_halo            include/particle_halo.h:1199-1209
 //
 // Cosmological distances come from CosmoLib in the reference, which is not
 // part of the reference tree.  Here they are the flat-LCDM comoving distance
@@ -422,14 +426,328 @@ struct TreeQuad {
   }
 };
 
-// LensHaloParticles with posHalo = (0,0): force_halo negates the tree's alpha
-// (include/particle_halo.h:426-434).
+// ------------------------------------------------------------ LensHaloNFW
+// Tables and profile functions (MultiPlane/lens_halos.cpp:352-353, 482-540,
+// 566-581; AnalyticNSIE/nfw_lens.cpp:82-125).
+struct NfwProfile {
+  static constexpr long NTABLE = 10000;
+  static constexpr double maxrm = 100.0;
+  std::vector<double> xtable, ftable, gtable, g2table;
+  static double gfunction(double x) {
+    double ans;
+    if (x < 1e-5) x = 1e-5;
+    ans = log(x / 2);
+    if (x == 1.0) { ans += 1.0; return ans; }
+    if (x > 1.0) { ans += 2 * atan(sqrt((x - 1) / (x + 1))) / sqrt(x * x - 1); return ans; }
+    ans += 2 * atanh(sqrt((1 - x) / (x + 1))) / sqrt(1 - x * x);
+    return ans;
+  }
+  static double ffunction(double x) {
+    if (x == 0) x = 1e-5;
+    if (x == 1.0) return 1.0 / 3.0;
+    if (x > 1.0) return (1 - 2 * atan(sqrt((x - 1) / (x + 1))) / sqrt(x * x - 1)) / (x * x - 1);
+    return (1 - 2 * atanh(sqrt((1 - x) / (x + 1))) / sqrt(1 - x * x)) / (x * x - 1);
+  }
+  static double g2function(double x) {
+    double ans, y;
+    if (x == 0) x = 1e-5;
+    if (x == 1) return 10 / 3. + 4 * log(0.5);
+    y = x * x - 1;
+    ans = 4 * log(x / 2) / x / x - 2 / y;
+    if (x < 1.0) { ans += 4 * (2 / x / x + 1 / y) * atanh(sqrt((1 - x) / (x + 1))) / sqrt(-y); return ans; }
+    ans += 4 * (2 / x / x + 1 / y) * atan(sqrt((x - 1) / (x + 1))) / sqrt(y);
+    return ans;
+  }
+  NfwProfile() {
+    const double dx = maxrm / (double)NTABLE;
+    xtable.resize(NTABLE); ftable.resize(NTABLE); gtable.resize(NTABLE); g2table.resize(NTABLE);
+    for (long i = 0; i < NTABLE; ++i) {
+      const double x = i * dx;
+      xtable[i] = x;
+      ftable[i] = ffunction(x);
+      gtable[i] = gfunction(x);
+      g2table[i] = g2function(x);
+    }
+  }
+  // which: 0 g, 1 f, 2 g2
+  double interpolate(int which, double y) const {
+    const int j = (int)(y / maxrm * NTABLE);
+    if (j == 0) return which == 0 ? gfunction(y) : which == 1 ? ffunction(y) : g2function(y);
+    if (j + 1 >= NTABLE) throw std::out_of_range("NFW table ends at maxrm");
+    const std::vector<double>& t = which == 0 ? gtable : which == 1 ? ftable : g2table;
+    return (t[j + 1] - t[j]) / (xtable[j + 1] - xtable[j]) * (y - xtable[j]) + t[j];
+  }
+  static const NfwProfile& get() {
+    static NfwProfile p;
+    return p;
+  }
+};
+
+// One LensHaloNFW(mass, Rsize, zlens, concentration, fratio = 1) at a
+// physical position (MultiPlane/lens_halos.cpp:389-440) and its force_halo
+// (LensHalo::force_halo -> force_point / force_halo_sym, :1528-1672).
+struct NfwHalo {
+  double pos[2];
+  float mass, Rsize, rscale, Rmax;
+  double xmax, gmax;
+  NfwHalo(double x, double y, float m, float rsize, float conc) {
+    pos[0] = x;
+    pos[1] = y;
+    mass = m;
+    Rsize = rsize;
+    rscale = Rsize / conc;
+    xmax = Rsize / rscale;
+    gmax = NfwProfile::get().interpolate(0, xmax);
+    Rmax = Rsize;
+  }
+  double alpha_h(double x) const { return -1.0 * NfwProfile::get().interpolate(0, x) / gmax; }
+  double kappa_h(double x) const { return 0.5 * x * x * NfwProfile::get().interpolate(1, x) / gmax; }
+  double gamma_h(double x) const { return -0.5 * x * x * NfwProfile::get().interpolate(2, x) / gmax; }
+  void force_halo(double* alpha, double* kappa, double* gamma, const double* xcm) const {
+    double rcm2 = xcm[0] * xcm[0] + xcm[1] * xcm[1];
+    // force_point without point subtraction
+    if (!(rcm2 < Rmax * Rmax)) {
+      const double prefac = mass / rcm2 / PI;
+      alpha[0] += -1.0 * prefac * xcm[0];
+      alpha[1] += -1.0 * prefac * xcm[1];
+      const double tmp = -2.0 * prefac / rcm2;
+      gamma[0] += 0.5 * (xcm[0] * xcm[0] - xcm[1] * xcm[1]) * tmp;
+      gamma[1] += xcm[0] * xcm[1] * tmp;
+      return;
+    }
+    // force_halo_sym
+    if (rcm2 < 1e-20) rcm2 = 1e-20;
+    if (rcm2 < Rmax * Rmax) {
+      const double prefac = mass / rcm2 / PI;
+      const double x = sqrt(rcm2) / rscale;
+      double tmp = (alpha_h(x) + 1.0 * false) * prefac;
+      alpha[0] += tmp * xcm[0];
+      alpha[1] += tmp * xcm[1];
+      *kappa += kappa_h(x) * prefac;
+      tmp = (gamma_h(x) + 2.0 * false) * prefac / rcm2;
+      gamma[0] += 0.5 * (xcm[0] * xcm[0] - xcm[1] * xcm[1]) * tmp;
+      gamma[1] += xcm[0] * xcm[1] * tmp;
+    } else {
+      const double prefac = mass / rcm2 / PI;
+      alpha[0] += -1.0 * prefac * xcm[0];
+      alpha[1] += -1.0 * prefac * xcm[1];
+      const double tmp = -2.0 * prefac / rcm2;
+      gamma[0] += 0.5 * (xcm[0] * xcm[0] - xcm[1] * xcm[1]) * tmp;
+      gamma[1] += xcm[0] * xcm[1] * tmp;
+    }
+  }
+};
+
+// ----------------------------------------------------------- TreeQuadHalos
+// include/quadTreeHalos.h:127-735 with LensHaloHalos' arguments (bucket 5,
+// no periodic buffer, no screening, no maximum range): pointer cells,
+// Utilities::quickPartition, the build and moments in the reference's order,
+// and walkTree_iter's depth-first walk.
+struct HaloTree {
+  struct Branch {
+    unsigned long* particles = nullptr;
+    unsigned long nparticles = 0;
+    std::vector<unsigned long> big;
+    double center[2] = {0, 0};
+    double mass = 0;
+    double quad[3] = {0, 0, 0};
+    double p1[2] = {0, 0}, p2[2] = {0, 0};
+    double r2crit = 0;
+    std::unique_ptr<Branch> child[4];
+    bool leaf() const { return !child[0]; }
+  };
+  std::vector<NfwHalo> halos;
+  std::vector<unsigned long> index;
+  std::vector<double> work;
+  double inv_area, theta;
+  int bucket = 5;
+  std::unique_ptr<Branch> top;
+
+  static void quickPartition(double pivot, unsigned long* pivotindex, unsigned long* particles,
+                             double* arr, unsigned long N) {
+    *pivotindex = 0;
+    for (unsigned long i = 0; i < N; ++i) {
+      if (arr[i] <= pivot) {
+        std::swap(arr[*pivotindex], arr[i]);
+        std::swap(particles[*pivotindex], particles[i]);
+        ++(*pivotindex);
+      }
+    }
+  }
+
+  HaloTree(std::vector<NfwHalo> h, float my_inv_area, float my_theta)
+      : halos(std::move(h)), inv_area(my_inv_area), theta(my_theta) {
+    const unsigned long N = halos.size();
+    index.resize(N);
+    for (unsigned long i = 0; i < N; ++i) index[i] = i;
+    if (N == 0) return;
+    double p1[2], p2[2];
+    for (int j = 0; j < 2; ++j) p1[j] = p2[j] = halos[0].pos[j];
+    for (unsigned long i = 0; i < N; ++i)
+      for (int j = 0; j < 2; ++j) {
+        if (halos[i].pos[j] < p1[j]) p1[j] = halos[i].pos[j];
+        if (halos[i].pos[j] > p2[j]) p2[j] = halos[i].pos[j];
+      }
+    if (N == 1) { p1[0] = -0.25; p1[1] = -0.25; p2[0] = 0.25; p2[1] = 0.25; }
+    const double lengths[2] = {p2[0] - p1[0], p2[1] - p1[1]};
+    if (lengths[0] == 0 || lengths[1] == 0) throw std::invalid_argument("particles in same place.");
+    const int j = lengths[0] > lengths[1] ? 1 : 0;
+    p2[j] = p1[j] + lengths[!j];
+    top.reset(new Branch);
+    top->particles = index.data();
+    top->nparticles = N;
+    for (int k = 0; k < 2; ++k) { top->p1[k] = p1[k]; top->p2[k] = p2[k]; }
+    work.resize(N);
+    build(top.get(), true, 0);
+    moments(top.get());
+  }
+
+  void build(Branch* b, bool at_top, int level) {
+    if (level > 200) throw std::runtime_error("halo tree too deep");
+    b->center[0] = (b->p1[0] + b->p2[0]) / 2;
+    b->center[1] = (b->p1[1] + b->p2[1]) / 2;
+    if (b->nparticles <= (unsigned long)bucket) {
+      b->big.assign(b->particles, b->particles + b->nparticles);
+      return;
+    }
+    double* x = work.data();
+    unsigned long* particles = b->particles;
+    const unsigned long n = b->nparticles;
+    for (unsigned long i = 0; i < n; ++i) x[i] = halos[particles[i]].Rmax;
+    const double half_box = (b->p2[0] - b->p1[0]) / 2;
+    unsigned long cut, cut2;
+    quickPartition(half_box, &cut, particles, x, n);
+    if (cut < n) {
+      if (at_top) cut2 = n - cut;
+      else quickPartition(2 * half_box, &cut2, &particles[cut], &x[cut], n - cut);
+      b->big.assign(particles + cut, particles + cut + cut2);
+    }
+    for (int k = 0; k < 4; ++k) b->child[k].reset(new Branch);
+    Branch* c0 = b->child[0].get(); Branch* c1 = b->child[1].get();
+    Branch* c2 = b->child[2].get(); Branch* c3 = b->child[3].get();
+    c0->p1[0] = b->center[0]; c0->p1[1] = b->center[1]; c0->p2[0] = b->p2[0]; c0->p2[1] = b->p2[1];
+    c1->p1[0] = b->p1[0]; c1->p1[1] = b->center[1]; c1->p2[0] = b->center[0]; c1->p2[1] = b->p2[1];
+    c2->p1[0] = b->center[0]; c2->p1[1] = b->p1[1]; c2->p2[0] = b->p2[0]; c2->p2[1] = b->center[1];
+    c3->p1[0] = b->p1[0]; c3->p1[1] = b->p1[1]; c3->p2[0] = b->center[0]; c3->p2[1] = b->center[1];
+    const double xcut = b->center[0], ycut = b->center[1];
+    unsigned long cutx, cuty;
+    for (unsigned long i = 0; i < n; ++i) x[i] = halos[particles[i]].pos[1];
+    quickPartition(ycut, &cuty, particles, x, n);
+    if (cuty > 0) {
+      for (unsigned long i = 0; i < cuty; ++i) x[i] = halos[particles[i]].pos[0];
+      quickPartition(xcut, &cutx, particles, x, cuty);
+      c3->nparticles = cutx; c3->particles = particles;
+      c2->nparticles = cuty - cutx; c2->particles = &particles[cutx];
+    }
+    if (cuty < n) {
+      for (unsigned long i = cuty; i < n; ++i) x[i - cuty] = halos[particles[i]].pos[0];
+      quickPartition(xcut, &cutx, &particles[cuty], x, n - cuty);
+      c1->nparticles = cutx; c1->particles = &particles[cuty];
+      c0->nparticles = n - cuty - cutx; c0->particles = &particles[cuty + cutx];
+    }
+    for (int k = 0; k < 4; ++k) build(b->child[k].get(), false, level + 1);
+  }
+
+  void moments(Branch* b) {
+    b->mass = 0;
+    for (unsigned long i = 0; i < b->nparticles; ++i) b->mass += halos[b->particles[i]].mass;
+    b->center[0] = b->center[1] = 0;
+    for (unsigned long i = 0; i < b->nparticles; ++i) {
+      const double tmp = halos[b->particles[i]].mass;
+      b->center[0] += tmp * halos[b->particles[i]].pos[0] / b->mass;
+      b->center[1] += tmp * halos[b->particles[i]].pos[1] / b->mass;
+    }
+    b->quad[0] = b->quad[1] = b->quad[2] = 0;
+    for (unsigned long i = 0; i < b->nparticles; ++i) {
+      const double xcm[2] = {halos[b->particles[i]].pos[0] - b->center[0],
+                             halos[b->particles[i]].pos[1] - b->center[1]};
+      const double xcut = pow(xcm[0], 2) + pow(xcm[1], 2);
+      const double tmp = halos[b->particles[i]].mass;
+      b->quad[0] += (xcut - 2 * xcm[0] * xcm[0]) * tmp;
+      b->quad[1] += (xcut - 2 * xcm[1] * xcm[1]) * tmp;
+      b->quad[2] += -2 * xcm[0] * xcm[1] * tmp;
+    }
+    double rcom = 0;
+    for (int i = 0; i < 2; ++i)
+      rcom += (pow(b->center[i] - b->p1[i], 2) > pow(b->center[i] - b->p2[i], 2))
+                  ? pow(b->center[i] - b->p1[i], 2) : pow(b->center[i] - b->p2[i], 2);
+    rcom = sqrt(rcom);
+    if (theta > 0.0) {
+      b->r2crit = 1.15470 * rcom / theta;
+      b->r2crit *= b->r2crit;
+    } else {
+      b->r2crit = 1.0e100;
+    }
+    if (!b->leaf())
+      for (int k = 0; k < 4; ++k) moments(b->child[k].get());
+  }
+
+  void walk(const Branch* b, const double* ray, double* alpha, double* kappa, double* gamma) const {
+    for (unsigned long t : b->big) {
+      const NfwHalo& h = halos[t];
+      const double xcm[2] = {h.pos[0] - ray[0], h.pos[1] - ray[1]};
+      h.force_halo(alpha, kappa, gamma, xcm);
+      const double mass = h.mass;
+      const double prefac = mass * inv_area;
+      alpha[0] += prefac * xcm[0];
+      alpha[1] += prefac * xcm[1];
+      *kappa -= mass * inv_area;
+    }
+    if (b->leaf()) return;
+    const double xcm[2] = {b->center[0] - ray[0], b->center[1] - ray[1]};
+    const double rcm2cell = xcm[0] * xcm[0] + xcm[1] * xcm[1];
+    if (rcm2cell > b->r2crit) {
+      const double screening = 1.0;
+      double tmp = b->mass * (inv_area - screening / rcm2cell / PI);
+      alpha[0] += tmp * xcm[0];
+      alpha[1] += tmp * xcm[1];
+      tmp = -2.0 * b->mass / PI / rcm2cell / rcm2cell * screening;
+      gamma[0] += 0.5 * (xcm[0] * xcm[0] - xcm[1] * xcm[1]) * tmp;
+      gamma[1] += xcm[0] * xcm[1] * tmp;
+      alpha[0] -= (b->quad[0] * xcm[0] + b->quad[2] * xcm[1]) / (rcm2cell * rcm2cell) / PI * screening;
+      alpha[1] -= (b->quad[1] * xcm[1] + b->quad[2] * xcm[0]) / (rcm2cell * rcm2cell) / PI * screening;
+      tmp = 4 * (b->quad[0] * xcm[0] * xcm[0] + b->quad[1] * xcm[1] * xcm[1] +
+                 2 * b->quad[2] * xcm[0] * xcm[1]) / (rcm2cell * rcm2cell * rcm2cell) / PI * screening;
+      alpha[0] += tmp * xcm[0];
+      alpha[1] += tmp * xcm[1];
+      *kappa -= b->mass * inv_area;
+      return;
+    }
+    for (int k = 0; k < 4; ++k) walk(b->child[k].get(), ray, alpha, kappa, gamma);
+  }
+
+  // force2D (include/quadTreeHalos.h:557-592)
+  void force2D(const double* ray, double* alpha, double* kappa, double* gamma) const {
+    alpha[0] = alpha[1] = gamma[0] = gamma[1] = gamma[2] = 0.0;
+    *kappa = 0.0;
+    if (halos.empty()) return;
+    walk(top.get(), ray, alpha, kappa, gamma);
+  }
+
+  // every halo with its background term, no tree
+  void direct(const double* ray, double* alpha, double* kappa, double* gamma) const {
+    alpha[0] = alpha[1] = gamma[0] = gamma[1] = gamma[2] = 0.0;
+    *kappa = 0.0;
+    for (const NfwHalo& h : halos) {
+      const double xcm[2] = {h.pos[0] - ray[0], h.pos[1] - ray[1]};
+      h.force_halo(alpha, kappa, gamma, xcm);
+      alpha[0] += h.mass * inv_area * xcm[0];
+      alpha[1] += h.mass * inv_area * xcm[1];
+      *kappa -= h.mass * inv_area;
+    }
+  }
+};
+
+// LensHaloParticles or LensHaloHalos with posHalo = (0,0): force_halo negates
+// the tree's alpha (include/particle_halo.h:426-434, 1199-1209).
 struct Halo {
   std::unique_ptr<TreeQuad> tree;
+  std::unique_ptr<HaloTree> htree;
   float z;
   void force_halo(const double* x, double* alpha, double* kappa, double* gamma,
                   Counters* cnt) const {
-    tree->force2D(x, alpha, kappa, gamma, cnt);
+    if (htree) htree->force2D(x, alpha, kappa, gamma);
+    else tree->force2D(x, alpha, kappa, gamma, cnt);
     alpha[0] *= -1;
     alpha[1] *= -1;
   }
@@ -956,6 +1274,57 @@ void orc_lens_add_halo(void* lp, void* tree, float z) {
   auto* L = static_cast<orc::Lens*>(lp);
   auto h = std::make_unique<orc::Halo>();
   h->tree.reset(static_cast<orc::TreeQuad*>(tree));
+  h->z = z;
+  L->moveinMainHalo(std::move(h));
+}
+
+// LensHaloHalos<LensHaloNFW>: a halo tree of n NFW halos (float mass, Rsize,
+// concentration; physical positions), inv_area and theta_force as floats.
+void* orc_halotree_build(const double* x, const double* y, const float* m, const float* rsize,
+                         const float* conc, size_t n, float inv_area, float theta, int* status) {
+  try {
+    std::vector<orc::NfwHalo> h;
+    for (size_t i = 0; i < n; ++i) h.emplace_back(x[i], y[i], m[i], rsize[i], conc[i]);
+    *status = 0;
+    return new orc::HaloTree(std::move(h), inv_area, theta);
+  } catch (const std::exception&) {
+    *status = 1;
+    return nullptr;
+  }
+}
+void orc_halotree_free(void* t) { delete static_cast<orc::HaloTree*>(t); }
+// force2D (direct = 0) or the sum over every halo (direct = 1), 5 per ray
+void orc_halotree_force(void* tp, const double* rays, size_t n, int direct, double* out) {
+  const auto* t = static_cast<orc::HaloTree*>(tp);
+  for (size_t i = 0; i < n; ++i) {
+    double a[2], k, g[3];
+    if (direct) t->direct(rays + 2 * i, a, &k, g);
+    else t->force2D(rays + 2 * i, a, &k, g);
+    out[5 * i] = a[0]; out[5 * i + 1] = a[1]; out[5 * i + 2] = k;
+    out[5 * i + 3] = g[0]; out[5 * i + 4] = g[1];
+  }
+}
+// one NFW halo's force_halo at positions relative to it (5 per point)
+void orc_nfw_force(float m, float rsize, float conc, const double* xcm, size_t n, double* out) {
+  orc::NfwHalo h(0, 0, m, rsize, conc);
+  for (size_t i = 0; i < n; ++i) {
+    double a[2] = {0, 0}, k = 0, g[3] = {0, 0, 0};
+    h.force_halo(a, &k, g, xcm + 2 * i);
+    out[5 * i] = a[0]; out[5 * i + 1] = a[1]; out[5 * i + 2] = k;
+    out[5 * i + 3] = g[0]; out[5 * i + 4] = g[1];
+  }
+}
+// the profile functions and their table interpolation (which: 0 g, 1 f, 2 g2)
+double orc_nfw_function(int which, double x) {
+  return which == 0 ? orc::NfwProfile::gfunction(x)
+         : which == 1 ? orc::NfwProfile::ffunction(x) : orc::NfwProfile::g2function(x);
+}
+double orc_nfw_interpolate(int which, double x) { return orc::NfwProfile::get().interpolate(which, x); }
+// Moves a halo tree into the lens as one LensHaloHalos at float redshift z.
+void orc_lens_add_halotree(void* lp, void* tree, float z) {
+  auto* L = static_cast<orc::Lens*>(lp);
+  auto h = std::make_unique<orc::Halo>();
+  h->htree.reset(static_cast<orc::HaloTree*>(tree));
   h->z = z;
   L->moveinMainHalo(std::move(h));
 }

@@ -19,7 +19,7 @@ arcsecTOradians = 0.000004848136811   # include/standard.h:51
 Grav = 4.7788e-20                      # include/standard.h:43
 
 __all__ = [
-    "Context", "LensHaloParticles", "Lens", "GridMap", "Batch", "Server", "coorDist",
+    "Context", "LensHaloParticles", "LensHaloHalos", "Lens", "GridMap", "Batch", "Server", "coorDist",
     "invCoorDist",
     "nfw_r200", "gen_nfw", "gen_halo_field", "SLSIM_FP64", "SLSIM_FP32C", "SLSIM_DIRECT64",
     "SlsimError", "MakeParticleLenses", "PARTICLE_DTYPE", "glmb_write", "glmb_read",
@@ -216,6 +216,37 @@ class LensHaloParticles:
             self.close()
         except Exception:
             pass
+
+
+class LensHaloHalos:
+    """LensHaloHalos<LensHaloNFW>(halos, redshift, cosmo, inv_area,
+    theta_force) (include/particle_halo.h:1092-1227): a main halo made of
+    symmetric NFW halos, LensHaloNFW(mass, Rsize, zlens, concentration)
+    each, at physical positions (x, y) (Mpc), on a TreeQuadHalos quadtree
+    (bucket 5).  Goes into a Lens with moveinMainHalo like a particle halo."""
+
+    def __init__(self, ctx, x, y, mass, rsize, concentration, z, inv_area=0.0, theta=0.1):
+        self.ctx = ctx
+        self.z = np.float32(z)
+        self.owned_by_lens = False
+        xs = np.ascontiguousarray(x, dtype=np.float64)
+        ys = np.ascontiguousarray(y, dtype=np.float64)
+        m = np.ascontiguousarray(mass, dtype=np.float32)
+        r = np.ascontiguousarray(rsize, dtype=np.float32)
+        c = np.ascontiguousarray(concentration, dtype=np.float32)
+        self.h = _abi.slsim_plane()
+        _check(lib.slsim_halos_create_nfw(ctx.h, _ptr(xs, ctypes.c_double), _ptr(ys, ctypes.c_double),
+                                          _ptr(m, ctypes.c_float), _ptr(r, ctypes.c_float),
+                                          _ptr(c, ctypes.c_float), ctypes.c_size_t(xs.size),
+                                          ctypes.c_float(inv_area), ctypes.c_float(theta),
+                                          ctypes.byref(self.h)))
+
+    # the plane handle's methods (a halo-of-halos plane is a slsim_plane)
+    info = LensHaloParticles.info
+    force = LensHaloParticles.force
+    direct_sum = LensHaloParticles.direct_sum
+    close = LensHaloParticles.close
+    __del__ = LensHaloParticles.__del__
 
 
 class Lens:

@@ -165,6 +165,9 @@ SIGNATURES = {
                                           c_size_t, c_int, P(c_float)]),
     "slsim_smoothing_sizes_f32": (c_int, [slsim_ctx, P(c_float), P(c_float), P(c_float),
                                           c_size_t, c_int, P(c_float)]),
+    "slsim_halos_create_nfw": (c_int, [slsim_ctx, P(c_double), P(c_double), P(c_float),
+                                       P(c_float), P(c_float), c_size_t, c_float, c_float,
+                                       P(slsim_plane)]),
     "slsim_mpl_create": (c_int, [slsim_ctx, c_char_p, c_int, c_int, c_int, c_int, P(c_vp)]),
     "slsim_mpl_create_glmb": (c_int, [slsim_ctx, c_char_p, c_int, P(c_vp)]),
     "slsim_mpl_destroy": (c_int, [c_vp]),

@@ -165,6 +165,12 @@ TreeDesc slsim_plane_s::desc() const {
   t.comy = com[1];
   t.nnodes = (int32_t)nnodes;
   t.nparts = (int32_t)n;
+  t.kind = kind;
+  t.nhalos = kind == kTreeHalos ? (int32_t)n : 0;
+  t.hnodes = hnodes;
+  t.hbig = hbig;
+  t.halos = halos;
+  t.nfw = nfw;
   return t;
 }
 
@@ -220,8 +226,8 @@ int slsim_ctx_synchronize(slsim_ctx ctx) {
 // ------------------------------------------------------------------ planes
 static int plane_free_device(slsim_plane p) {
   slsim_ctx c = p->ctx;
-  void* ptrs[] = {p->nodesD, p->nodesF, p->partsD, p->bigD,
-                  p->box_p1, p->box_p2, p->nbig,   p->npart};
+  void* ptrs[] = {p->nodesD, p->nodesF, p->partsD, p->bigD, p->box_p1, p->box_p2,
+                  p->nbig,   p->npart,  p->hnodes, p->hbig,  p->halos,  p->nfw};
   for (void* q : ptrs)
     if (q) c->release(q);
   return SLSIM_OK;
@@ -264,6 +270,25 @@ int slsim_plane_create_f32(slsim_ctx ctx, const float* x, const float* y, const 
   return plane_create<float>(ctx, x, y, m, h, n, theta, bucket, inv_area, out);
 }
 
+int slsim_halos_create_nfw(slsim_ctx ctx, const double* x, const double* y, const float* mass,
+                           const float* rsize, const float* concentration, size_t n,
+                           float inv_area, float theta_force, slsim_plane* out) {
+  if (!ctx || !out) return arg_error("null handle");
+  if (n == 0 || !x || !y || !mass || !rsize || !concentration) return arg_error("no halos");
+  if (n >= (size_t)1 << 22) return arg_error("more than 2^22 halos in one plane");
+  SLSIM_HIP_CHECK(hipSetDevice(ctx->device));
+  auto p = std::make_unique<slsim_plane_s>();
+  p->ctx = ctx;
+  const int rc = build_halo_plane(p.get(), x, y, mass, rsize, concentration, n, inv_area,
+                                  theta_force);
+  if (rc) {
+    plane_free_device(p.get());
+    return rc;
+  }
+  *out = p.release();
+  return SLSIM_OK;
+}
+
 int slsim_plane_destroy(slsim_plane plane) {
   if (!plane) return SLSIM_OK;
   (void)hipStreamSynchronize(plane->ctx->stream);
@@ -289,6 +314,7 @@ int slsim_plane_get_info(slsim_plane p, slsim_plane_info* info) {
 
 int slsim_plane_export_nodes(slsim_plane p, double* dbl, int64_t* ints) {
   if (!p || !dbl || !ints) return arg_error("null argument");
+  if (p->kind != kTreeParticles) return arg_error("export_nodes: not a particle plane");
   SLSIM_HIP_CHECK(hipSetDevice(p->ctx->device));
   return export_nodes(p, dbl, ints);
 }
@@ -310,7 +336,10 @@ int slsim_plane_force(slsim_plane p, size_t n, const double* rays, double* out, 
   SLSIM_HIP_CHECK(hipMemcpyAsync(dt, &t, sizeof(t), hipMemcpyHostToDevice, c->stream));
   SLSIM_HIP_CHECK(hipMemcpyAsync(dr, rays, 16 * n, hipMemcpyHostToDevice, c->stream));
   SLSIM_HIP_CHECK(hipMemsetAsync(err, 0, 4, c->stream));
-  launch_force(c->stream, precision, dt, dr, (int64_t)n, dout, err);
+  if (p->kind == kTreeHalos)
+    launch_halo_force(c->stream, dt, dr, (int64_t)n, 0, dout, err);
+  else
+    launch_force(c->stream, precision, dt, dr, (int64_t)n, dout, err);
   SLSIM_HIP_CHECK(hipGetLastError());
   uint32_t herr = 0;
   SLSIM_HIP_CHECK(hipMemcpyAsync(out, dout, 40 * n, hipMemcpyDeviceToHost, c->stream));
@@ -330,7 +359,15 @@ int slsim_plane_direct_sum(slsim_plane p, size_t n, const double* rays, double* 
   int rc;
   if ((rc = s.get(&dr, 2 * n)) || (rc = s.get(&dout, 5 * n))) return rc;
   SLSIM_HIP_CHECK(hipMemcpyAsync(dr, rays, 16 * n, hipMemcpyHostToDevice, c->stream));
-  launch_direct(c->stream, p->partsD, p->n, p->inv_area, dr, (int64_t)n, dout);
+  if (p->kind == kTreeHalos) {
+    TreeDesc* dt;
+    if ((rc = s.get(&dt, 1))) return rc;
+    TreeDesc t = p->desc();
+    SLSIM_HIP_CHECK(hipMemcpyAsync(dt, &t, sizeof(t), hipMemcpyHostToDevice, c->stream));
+    launch_halo_force(c->stream, dt, dr, (int64_t)n, 1, dout, nullptr);
+  } else {
+    launch_direct(c->stream, p->partsD, p->n, p->inv_area, dr, (int64_t)n, dout);
+  }
   SLSIM_HIP_CHECK(hipGetLastError());
   SLSIM_HIP_CHECK(hipMemcpyAsync(out, dout, 40 * n, hipMemcpyDeviceToHost, c->stream));
   SLSIM_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -379,10 +416,17 @@ int slsim_lens_s::upload_trees() {
   std::vector<TreeDesc> all;
   tree_off.clear();
   tree_cnt.clear();
+  tree_hcnt.clear();
   for (int mi : order) {
+    // particle trees first, then the halo-of-halos trees of the plane
     tree_off.push_back((int)all.size());
-    tree_cnt.push_back((int)main_planes[mi].size());
-    for (slsim_plane p : main_planes[mi]) all.push_back(p->desc());
+    int np = 0, nh = 0;
+    for (slsim_plane p : main_planes[mi])
+      if (p->kind == kTreeParticles) { all.push_back(p->desc()); ++np; }
+    for (slsim_plane p : main_planes[mi])
+      if (p->kind == kTreeHalos) { all.push_back(p->desc()); ++nh; }
+    tree_cnt.push_back(np);
+    tree_hcnt.push_back(nh);
   }
   if (trees_dev) ctx->release(trees_dev);
   trees_dev = nullptr;
@@ -431,6 +475,7 @@ void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint3
     s.nslots = (int32_t)nslots;
     s.j = j;
     s.ntrees = L->tree_cnt[j];
+    s.nhtrees = L->tree_hcnt[j];
     s.trees = L->trees_dev + L->tree_off[j];
     s.Dphys = L->Dl[j] / (1 + L->plane_redshifts[j]);
     s.Gfac = L->charge * L->Dl[j] / (1 + L->plane_redshifts[j]);
@@ -446,15 +491,17 @@ void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint3
 namespace {
 
 // Runs the plane steps [0, jmax) of one job on the context stream.
-int run_steps(slsim_ctx c, const StepJob* djobs, int jmax, uint32_t nbundles, int precision,
-              bool count, uint32_t* err, slsim_shoot_stats* stats) {
+int run_steps(slsim_ctx c, const StepJob* djobs, const std::vector<StepJob>& steps, int jmax,
+              uint32_t nbundles, int precision, bool count, uint32_t* err,
+              slsim_shoot_stats* stats) {
   std::vector<hipEvent_t> ev(2 * (size_t)jmax);
   const bool timed = stats != nullptr;
   if (timed)
     for (auto& e : ev) SLSIM_HIP_CHECK(hipEventCreate(&e));
   for (int j = 0; j < jmax; ++j) {
     if (timed) SLSIM_HIP_CHECK(hipEventRecord(ev[2 * j], c->stream));
-    launch_plane_step(precision, count, nbundles, c->stream, djobs + j, nullptr, err);
+    launch_plane_step(precision, count, nbundles, c->stream, djobs + j, nullptr, err,
+                      steps[j].nhtrees > 0);
     if (timed) SLSIM_HIP_CHECK(hipEventRecord(ev[2 * j + 1], c->stream));
   }
   SLSIM_HIP_CHECK(hipGetLastError());
@@ -524,7 +571,7 @@ int gridmap_run(slsim_lens L, size_t N, double c0, double c1, double range, doub
     if ((rc = s.get(&djobs, steps.size()))) return rc;
     SLSIM_HIP_CHECK(hipMemcpyAsync(djobs, steps.data(), sizeof(StepJob) * steps.size(),
                                    hipMemcpyHostToDevice, c->stream));
-    rc = run_steps(c, djobs, jmax, (uint32_t)nbundles, cnt_host ? SLSIM_FP64 : precision,
+    rc = run_steps(c, djobs, steps, jmax, (uint32_t)nbundles, cnt_host ? SLSIM_FP64 : precision,
                    cnt_host != nullptr, err, stats);
     if (rc) return rc;
   }
@@ -731,7 +778,8 @@ int slsim_shoot_device(slsim_lens L, size_t n, const double* x_dev, const double
     if ((rc = s.get(&djobs, steps.size()))) return rc;
     SLSIM_HIP_CHECK(hipMemcpyAsync(djobs, steps.data(), sizeof(StepJob) * steps.size(),
                                    hipMemcpyHostToDevice, c->stream));
-    rc = run_steps(c, djobs, steps_to_run, (uint32_t)nbundles, precision, false, err, stats);
+    rc = run_steps(c, djobs, steps, steps_to_run, (uint32_t)nbundles, precision, false, err,
+                   stats);
     if (rc) return rc;
   }
   launch_finalize_array(c->stream, st, (int64_t)n, perm, y_dev, A_dev);

@@ -89,6 +89,7 @@ struct slsim_batch_s {
   DevBlock d_greq, d_jobs, d_items, d_ctr, d_err;
   std::vector<size_t> job_off, item_off;
   std::vector<uint32_t> nitems;
+  std::vector<char> step_halos;   // per step: some job has halo-of-halos trees
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
 
@@ -155,6 +156,7 @@ int slsim_batch_s::prepare() {
   job_off.assign(max_steps + 1, 0);
   item_off.assign(max_steps + 1, 0);
   nitems.assign(max_steps, 0);
+  step_halos.assign(max_steps, 0);
   std::vector<int> ord(reqs.size());
   for (size_t r = 0; r < ord.size(); ++r) ord[r] = (int)r;
   if (sched.order == 2)
@@ -169,6 +171,7 @@ int slsim_batch_s::prepare() {
       if (j >= reqs[r]->jmax) continue;
       live.push_back({(uint32_t)(all_jobs.size() - job_off[j]), greq[r].nslots / kWave});
       all_jobs.push_back(steps[r][j]);
+      if (steps[r][j].nhtrees > 0) step_halos[j] = 1;
     }
     if (sched.order == 1) {
       int64_t maxb = 0;
@@ -218,10 +221,10 @@ int slsim_batch_s::enqueue() {
     if (sched.queue)
       launch_plane_step_queue(precision, (unsigned)(kNumCU * std::max(1, sched.blocks_per_cu)),
                               nitems[j], st, jobs + job_off[j], items + item_off[j],
-                              static_cast<uint32_t*>(d_ctr.p), err);
+                              static_cast<uint32_t*>(d_ctr.p), err, step_halos[j] != 0);
     else
       launch_plane_step(precision, false, nitems[j], st, jobs + job_off[j], items + item_off[j],
-                        err);
+                        err, step_halos[j] != 0);
   }
   launch_finalize_batch(st, static_cast<GridReq*>(d_greq.p), (int)greq.size(), total_slots);
   SLSIM_HIP_CHECK(hipGetLastError());
@@ -558,10 +561,12 @@ int slsim_server_s::iterate() {
   for (int prec = SLSIM_FP64; prec <= SLSIM_DIRECT64; ++prec) {
     const size_t n_items = item0[prec + 1] - item0[prec];
     if (n_items == 0) continue;
+    bool halos = false;
+    for (size_t q = job0[prec]; q < job0[prec + 1]; ++q) halos = halos || h_jobs[q].nhtrees > 0;
     launch_plane_step(prec, false, (uint32_t)n_items, stream,
                       static_cast<StepJob*>(d_jobs.p) + job0[prec],
                       static_cast<WorkItem*>(d_items.p) + item0[prec],
-                      static_cast<uint32_t*>(d_err.p));
+                      static_cast<uint32_t*>(d_err.p), halos);
   }
   SLSIM_HIP_CHECK(hipGetLastError());
   // the staging buffers are reused next iteration

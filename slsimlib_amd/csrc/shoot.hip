@@ -276,6 +276,202 @@ __device__ void walk64(const TreeDesc& t, double rx, double ry, bool active, Acc
   }
 }
 
+// ------------------------------------------------------ halo-of-halos walk
+// LensHaloNFW's profile functions (AnalyticNSIE/nfw_lens.cpp:82-125), which
+// InterpolateFromTable evaluates directly in the first table interval
+// (MultiPlane/lens_halos.cpp:566-581).
+__device__ inline double nfw_g(double x) {
+#pragma clang fp contract(off)
+  if (x < 1e-5) x = 1e-5;
+  const double ans = log(x / 2);
+  if (x == 1.0) return ans + 1.0;
+  if (x > 1.0) return ans + 2 * atan(sqrt((x - 1) / (x + 1))) / sqrt(x * x - 1);
+  return ans + 2 * atanh(sqrt((1 - x) / (x + 1))) / sqrt(1 - x * x);
+}
+__device__ inline double nfw_f(double x) {
+#pragma clang fp contract(off)
+  if (x == 0) x = 1e-5;
+  if (x == 1.0) return 1.0 / 3.0;
+  if (x > 1.0) return (1 - 2 * atan(sqrt((x - 1) / (x + 1))) / sqrt(x * x - 1)) / (x * x - 1);
+  return (1 - 2 * atanh(sqrt((1 - x) / (x + 1))) / sqrt(1 - x * x)) / (x * x - 1);
+}
+__device__ inline double nfw_g2(double x) {
+#pragma clang fp contract(off)
+  if (x == 0) x = 1e-5;
+  if (x == 1) return 10 / 3. + 4 * log(0.5);
+  const double y = x * x - 1;
+  const double ans = 4 * log(x / 2) / x / x - 2 / y;
+  if (x < 1.0) return ans + 4 * (2 / x / x + 1 / y) * atanh(sqrt((1 - x) / (x + 1))) / sqrt(-y);
+  return ans + 4 * (2 / x / x + 1 / y) * atan(sqrt((x - 1) / (x + 1))) / sqrt(y);
+}
+
+// InterpolateFromTable for the three tables at once: linear between the
+// entries j and j+1, j = (int)(y / maxrm * NTABLE) (kept in range; halos are
+// accepted only with Rsize / rscale below the table's end).
+__device__ inline void nfw_interp(cptr<NfwEntry> T, double y, double& g, double& f, double& g2) {
+#pragma clang fp contract(off)
+  int j = (int)(y / kNfwMaxrm * kNfwTable);
+  if (j == 0) {
+    g = nfw_g(y);
+    f = nfw_f(y);
+    g2 = nfw_g2(y);
+    return;
+  }
+  j = j < 0 ? 0 : (j > kNfwTable - 2 ? kNfwTable - 2 : j);
+  const NfwEntry a = T[j], b = T[j + 1];
+  const double w = b.x - a.x, d = y - a.x;
+  g = (b.g - a.g) / w * d + a.g;
+  f = (b.f - a.f) / w * d + a.f;
+  g2 = (b.g2 - a.g2) / w * d + a.g2;
+}
+
+// LensHalo::force_halo of a symmetric NFW without point subtraction
+// (MultiPlane/lens_halos.cpp:1528-1672): force_point beyond Rmax (a point
+// mass), force_halo_sym inside with alpha_h, kappa_h, gamma_h of
+// include/lens_halos.h:842-853.  x0, x1: position relative to the halo.
+__device__ inline void nfw_force(const HaloNFW& h, cptr<NfwEntry> T, double x0, double x1,
+                                 Acc64& a) {
+#pragma clang fp contract(off)
+  const double rcm2 = x0 * x0 + x1 * x1;
+  const double R2 = (double)(h.rmax * h.rmax);   // float Rmax squared in float
+  const double mass = h.mass;
+  if (rcm2 < R2) {
+    double r2 = rcm2;
+    if (r2 < 1e-20) r2 = 1e-20;
+    if (r2 < R2) {
+      const double prefac = mass / r2 / kPI;
+      const double x = sqrt(r2) / (double)h.rscale;
+      double g, f, g2;
+      nfw_interp(T, x, g, f, g2);
+      const double alpha_h = -1.0 * g / h.gmax;
+      double tmp = (alpha_h + 0.0) * prefac;
+      a.ax += tmp * x0;
+      a.ay += tmp * x1;
+      const double kappa_h = 0.5 * x * x * f / h.gmax;
+      a.k += kappa_h * prefac;
+      const double gamma_h = -0.5 * x * x * g2 / h.gmax;
+      tmp = (gamma_h + 0.0) * prefac / r2;
+      a.g0 += 0.5 * (x0 * x0 - x1 * x1) * tmp;
+      a.g1 += x0 * x1 * tmp;
+      return;
+    }
+    const double prefac = mass / r2 / kPI;
+    a.ax += -1.0 * prefac * x0;
+    a.ay += -1.0 * prefac * x1;
+    const double tmp = -2.0 * prefac / r2;
+    a.g0 += 0.5 * (x0 * x0 - x1 * x1) * tmp;
+    a.g1 += x0 * x1 * tmp;
+    return;
+  }
+  const double prefac = mass / rcm2 / kPI;
+  a.ax += -1.0 * prefac * x0;
+  a.ay += -1.0 * prefac * x1;
+  const double tmp = -2.0 * prefac / rcm2;
+  a.g0 += 0.5 * (x0 * x0 - x1 * x1) * tmp;
+  a.g1 += x0 * x1 * tmp;
+}
+
+// A halo of the tree with the uniform background the walk removes with it
+// (include/quadTreeHalos.h:668-693): the halo's force at halo - ray plus
+// inv_area m (halo - ray) and -inv_area m.
+__device__ inline void halo_term(const HaloNFW& h, cptr<NfwEntry> T, double rx, double ry,
+                                 double ia, Acc64& a) {
+#pragma clang fp contract(off)
+  const double x0 = h.x - rx, x1 = h.y - ry;
+  nfw_force(h, T, x0, x1, a);
+  const double m = h.mass;
+  const double prefac = m * ia;
+  a.ax += prefac * x0;
+  a.ay += prefac * x1;
+  a.k -= m * ia;
+}
+
+// One cell of TreeQuadHalos::walkTree_iter (include/quadTreeHalos.h:
+// 656-735) for the lanes in lvm: its big halos, then, for an internal cell,
+// the monopole + quadrupole where the cell is far.  Returns the lanes that
+// descend into its children.
+__device__ __forceinline__ uint64_t evalh64(const HNode& nd, uint64_t lvm, double rx, double ry,
+                                            double ia, const TreeDesc& t, Acc64& a) {
+#pragma clang fp contract(off)
+  const bool lv = lane_in(lvm);
+  const cptr<HaloNFW> halos = as_const(t.halos);
+  const cptr<int32_t> big = as_const(t.hbig);
+  const cptr<NfwEntry> T = as_const(t.nfw);
+  for (int k = 0; k < nd.nbig; ++k) {
+    const HaloNFW h = halos[big[nd.big_off + k]];
+    if (lv) halo_term(h, T, rx, ry, ia, a);
+  }
+  if (nd.child < 0) return 0;
+  const double dx = nd.cx - rx, dy = nd.cy - ry;
+  const double r2 = dx * dx + dy * dy;
+  const uint64_t farm = lanes_lt(nd.r2crit, r2) & lvm;
+  if (lane_in(farm)) {
+    double tmp = nd.mass * (ia - 1.0 / r2 / kPI);
+    a.ax += tmp * dx;
+    a.ay += tmp * dy;
+    tmp = -2.0 * nd.mass / kPI / r2 / r2;
+    a.g0 += 0.5 * (dx * dx - dy * dy) * tmp;
+    a.g1 += dx * dy * tmp;
+    a.ax -= (nd.q0 * dx + nd.q2 * dy) / (r2 * r2) / kPI;
+    a.ay -= (nd.q1 * dy + nd.q2 * dx) / (r2 * r2) / kPI;
+    tmp = 4 * (nd.q0 * dx * dx + nd.q1 * dy * dy + 2 * nd.q2 * dx * dy) / (r2 * r2 * r2) / kPI;
+    a.ax += tmp * dx;
+    a.ay += tmp * dy;
+    a.k -= nd.mass * ia;
+  }
+  return lvm & ~farm;
+}
+
+// Walk of one halo-of-halos tree for the whole wavefront (the traversal of
+// walk64: a popped entry evaluates four contiguous children).
+__device__ void walk_halos64(const TreeDesc& t, double rx, double ry, bool active, Acc64& a,
+                             int32_t* s_node, uint64_t* s_mask, uint32_t* err) {
+  const int lane = lane_id();
+  const uint64_t am = ballot(active);
+  if (am == 0) return;
+  const cptr<HNode> nodes = as_const(t.hnodes);
+  const double ia = t.inv_area;
+  int sp = 0;
+  {
+    const HNode root = nodes[0];
+    const uint64_t om = evalh64(root, am, rx, ry, ia, t, a);
+    if (om) push_entry(s_node, s_mask, sp, lane, root.child | (root.cmask << 28), om, err);
+  }
+  while (sp > 0) {
+    --sp;
+    __builtin_amdgcn_wave_barrier();
+    const int entry = uni(s_node[sp]);
+    const uint64_t live = uni64(s_mask[sp]);
+    __builtin_amdgcn_wave_barrier();
+    const int base = entry & 0x0fffffff;
+    const int cm = (entry >> 28) & 0xf;
+    int32_t pe[4];
+    uint64_t pm[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      pm[k] = 0;
+      pe[k] = 0;
+      if ((cm >> k) & 1) {
+        const HNode nd = nodes[base + k];
+        pm[k] = evalh64(nd, live, rx, ry, ia, t, a);
+        pe[k] = nd.child | (nd.cmask << 28);
+      }
+    }
+#pragma unroll
+    for (int k = 3; k >= 0; --k)
+      if (pm[k]) push_entry(s_node, s_mask, sp, lane, pe[k], pm[k], err);
+  }
+}
+
+// Every halo of the plane with its background term, no tree (the error
+// column of halo-of-halos planes).
+__device__ void halos_direct64(const TreeDesc& t, double rx, double ry, bool active, Acc64& a) {
+  if (!active) return;
+  const cptr<HaloNFW> halos = as_const(t.halos);
+  const cptr<NfwEntry> T = as_const(t.nfw);
+  for (int i = 0; i < t.nhalos; ++i) halo_term(halos[i], T, rx, ry, t.inv_area, a);
+}
+
 // ---------------------------------------------------------------- FP32 walk
 // The FP32 walk sums pi times the reference's terms, with 1/pi, the signs
 // and the factor 2 of gamma1 taken out of the per-term arithmetic:
@@ -706,7 +902,7 @@ __device__ inline double phys_coord(double th, double Dphys) {
 
 // One plane step of one 64-ray bundle (one work item); called by a whole
 // wavefront.
-template <int MODE, bool COUNT>
+template <int MODE, bool COUNT, bool HALOS>
 __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, WorkItem it,
                                              int32_t* s_node, uint64_t* s_mask, uint64_t* s_near,
                                              uint32_t* __restrict__ err) {
@@ -731,11 +927,23 @@ __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, W
     // one set of FP32 sums over all halos of the plane, so that no FP64
     // totals are live during the walks
     Acc32 a;
-    walk32(jb.trees, ntrees, rx, ry, active, a, s_node, s_mask, s_near, err);
+    // (a plane may hold halo-of-halos trees only; walk32 starts at trees[0])
+    if (!HALOS || ntrees > 0) walk32(jb.trees, ntrees, rx, ry, active, a, s_node, s_mask, s_near, err);
     double f[5];
     finish32(a, f);
     for (int h = 0; h < ntrees; ++h) sheet32(as_const(jb.trees)[h], rx, ry, f);
     alpha0 = f[0]; alpha1 = f[1]; kappa = f[2]; gamma0 = f[3]; gamma1 = f[4];
+  }
+  // halo-of-halos trees: FP64 in every mode (LensHaloHalos::force_halo and
+  // the plane each flip alpha, so the plane's alpha is the tree's raw alpha);
+  // compiled only into the kernels launched for planes that have them
+  const int nhtrees = HALOS ? uni(jb.nhtrees) : 0;
+  for (int h = 0; h < nhtrees; ++h) {
+    const TreeDesc t = as_const(jb.trees)[ntrees + h];
+    Acc64 a;
+    if (MODE == SLSIM_DIRECT64) halos_direct64(t, rx, ry, active, a);
+    else walk_halos64(t, rx, ry, active, a, s_node, s_mask, err);
+    alpha0 += a.ax; alpha1 += a.ay; kappa += a.k; gamma0 += a.g0; gamma1 += a.g1;
   }
   for (int h = 0; h < (MODE == SLSIM_FP32C ? 0 : ntrees); ++h) {
     const TreeDesc t = as_const(jb.trees)[h];
@@ -766,7 +974,7 @@ __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, W
 
 // One plane step for every bundle of the listed jobs.  Grid: one wavefront
 // per work item, 4 per block (items == null: job 0, bundle = item index).
-template <int MODE, bool COUNT>
+template <int MODE, bool COUNT, bool HALOS>
 __device__ __forceinline__ void plane_step_grid(const StepJob* __restrict__ jobs,
                                                 const WorkItem* __restrict__ items,
                                                 uint32_t nitems, uint32_t* __restrict__ err) {
@@ -782,12 +990,12 @@ __device__ __forceinline__ void plane_step_grid(const StepJob* __restrict__ jobs
     it.job = as_const(items)[wi].job;
     it.bundle = as_const(items)[wi].bundle;
   }
-  process_item<MODE, COUNT>(jobs, it, s_node[w], s_mask[w], s_near[w], err);
+  process_item<MODE, COUNT, HALOS>(jobs, it, s_node[w], s_mask[w], s_near[w], err);
 }
 
 // Persistent variant: a fixed grid whose wavefronts take work items from a
 // queue counter until the list is exhausted (every wave reaches the exit).
-template <int MODE>
+template <int MODE, bool HALOS>
 __device__ __forceinline__ void plane_step_queue(const StepJob* __restrict__ jobs,
                                                  const WorkItem* __restrict__ items,
                                                  uint32_t nitems, uint32_t* __restrict__ counter,
@@ -805,37 +1013,39 @@ __device__ __forceinline__ void plane_step_queue(const StepJob* __restrict__ job
     WorkItem it;
     it.job = as_const(items)[idx].job;
     it.bundle = as_const(items)[idx].bundle;
-    process_item<MODE, false>(jobs, it, s_node[w], s_mask[w], s_near[w], err);
+    process_item<MODE, false, HALOS>(jobs, it, s_node[w], s_mask[w], s_near[w], err);
   }
 }
 
-template <int MODE, bool COUNT>
+template <int MODE, bool COUNT, bool HALOS>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
     const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t nitems,
     uint32_t* __restrict__ err) {
-  plane_step_grid<MODE, COUNT>(jobs, items, nitems, err);
+  plane_step_grid<MODE, COUNT, HALOS>(jobs, items, nitems, err);
 }
 
-template <int MODE>
+template <int MODE, bool HALOS>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step_queue(
     const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t nitems,
     uint32_t* __restrict__ counter, uint32_t* __restrict__ err) {
-  plane_step_queue<MODE>(jobs, items, nitems, counter, err);
+  plane_step_queue<MODE, HALOS>(jobs, items, nitems, counter, err);
 }
 
 // The FP32C walk at 8 waves per SIMD (64 VGPRs): measured faster than 7
-// waves without the few spills the cap costs.
+// waves without the few spills the cap costs.  Planes with halo-of-halos
+// trees use k_plane_step<SLSIM_FP32C, false, true> (no cap: the FP64 halo
+// walk needs more registers).
 __global__ __launch_bounds__(kWave * kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(8)))
 void k_plane_step32(const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items,
                     uint32_t nitems, uint32_t* __restrict__ err) {
-  plane_step_grid<SLSIM_FP32C, false>(jobs, items, nitems, err);
+  plane_step_grid<SLSIM_FP32C, false, false>(jobs, items, nitems, err);
 }
 
 __global__ __launch_bounds__(kWave * kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(8)))
 void k_plane_step_queue32(const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items,
                           uint32_t nitems, uint32_t* __restrict__ counter,
                           uint32_t* __restrict__ err) {
-  plane_step_queue<SLSIM_FP32C>(jobs, items, nitems, counter, err);
+  plane_step_queue<SLSIM_FP32C, false>(jobs, items, nitems, counter, err);
 }
 
 __global__ void k_zero_u32(uint32_t* __restrict__ p, int n) {
@@ -1073,6 +1283,27 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_force(
 template __global__ void k_force<false>(const TreeDesc*, const double*, int64_t, double*, uint32_t*);
 template __global__ void k_force<true>(const TreeDesc*, const double*, int64_t, double*, uint32_t*);
 
+// Halo-of-halos tree walk alone at physical positions (TreeQuadHalos::force2D
+// outputs, 5 per ray); direct != 0 sums every halo instead.
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_halo_force(
+    const TreeDesc* __restrict__ tree, const double* __restrict__ rays, int64_t n, int direct,
+    double* __restrict__ out, uint32_t* __restrict__ err) {
+  __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
+  __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
+  const int w = uni((int)(threadIdx.x >> 6));
+  const int64_t i = (blockIdx.x * (int64_t)kWavesPerBlock + w) * kWave + lane_id();
+  const bool active = i < n;
+  const double rx = active ? rays[2 * i] : 0.0, ry = active ? rays[2 * i + 1] : 0.0;
+  const TreeDesc t = *as_const(tree);
+  Acc64 a;
+  if (direct) halos_direct64(t, rx, ry, active, a);
+  else walk_halos64(t, rx, ry, active, a, s_node[w], s_mask[w], err);
+  if (active) {
+    out[5 * i] = a.ax; out[5 * i + 1] = a.ay; out[5 * i + 2] = a.k;
+    out[5 * i + 3] = a.g0; out[5 * i + 4] = a.g1;
+  }
+}
+
 // FP64 direct sum of the smoothed-particle kernel over all particles
 // (no tree); particles staged through LDS in tiles of 256.
 __global__ __launch_bounds__(256) void k_direct(const PartD* __restrict__ parts, int64_t np,
@@ -1118,18 +1349,29 @@ inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b
 }
 
 void launch_plane_step(int precision, bool count, uint32_t nitems, hipStream_t st,
-                       const StepJob* jobs, const WorkItem* items, uint32_t* err) {
+                       const StepJob* jobs, const WorkItem* items, uint32_t* err, bool halos) {
   const unsigned nb = blocks_for(nitems, kWavesPerBlock);
   if (nb == 0) return;
   const dim3 blk(kWave * kWavesPerBlock);
+  if (halos) {
+    if (count)
+      k_plane_step<SLSIM_FP64, true, true><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+    else if (precision == SLSIM_FP32C)
+      k_plane_step<SLSIM_FP32C, false, true><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+    else if (precision == SLSIM_DIRECT64)
+      k_plane_step<SLSIM_DIRECT64, false, true><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+    else
+      k_plane_step<SLSIM_FP64, false, true><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+    return;
+  }
   if (count)
-    k_plane_step<SLSIM_FP64, true><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+    k_plane_step<SLSIM_FP64, true, false><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
   else if (precision == SLSIM_FP32C)
     k_plane_step32<<<nb, blk, 0, st>>>(jobs, items, nitems, err);
   else if (precision == SLSIM_DIRECT64)
-    k_plane_step<SLSIM_DIRECT64, false><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+    k_plane_step<SLSIM_DIRECT64, false, false><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
   else
-    k_plane_step<SLSIM_FP64, false><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+    k_plane_step<SLSIM_FP64, false, false><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
 }
 
 void launch_zero_u32(hipStream_t st, uint32_t* p, int n) {
@@ -1138,15 +1380,29 @@ void launch_zero_u32(hipStream_t st, uint32_t* p, int n) {
 
 void launch_plane_step_queue(int precision, unsigned nblocks, uint32_t nitems, hipStream_t st,
                              const StepJob* jobs, const WorkItem* items, uint32_t* counter,
-                             uint32_t* err) {
+                             uint32_t* err, bool halos) {
   launch_zero_u32(st, counter, 1);
   const dim3 blk(kWave * kWavesPerBlock);
+  if (halos) {
+    if (precision == SLSIM_FP32C)
+      k_plane_step_queue<SLSIM_FP32C, true><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter,
+                                                                     err);
+    else if (precision == SLSIM_DIRECT64)
+      k_plane_step_queue<SLSIM_DIRECT64, true><<<nblocks, blk, 0, st>>>(jobs, items, nitems,
+                                                                        counter, err);
+    else
+      k_plane_step_queue<SLSIM_FP64, true><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter,
+                                                                    err);
+    return;
+  }
   if (precision == SLSIM_FP32C)
     k_plane_step_queue32<<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter, err);
   else if (precision == SLSIM_DIRECT64)
-    k_plane_step_queue<SLSIM_DIRECT64><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter, err);
+    k_plane_step_queue<SLSIM_DIRECT64, false><<<nblocks, blk, 0, st>>>(jobs, items, nitems,
+                                                                       counter, err);
   else
-    k_plane_step_queue<SLSIM_FP64><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter, err);
+    k_plane_step_queue<SLSIM_FP64, false><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter,
+                                                                   err);
 }
 
 void launch_init_grid(hipStream_t st, const RayState& s, int64_t nslots, int64_t N,
@@ -1204,6 +1460,13 @@ void launch_force(hipStream_t st, int precision, const TreeDesc* tree, const dou
     k_force<true><<<nb, kWave * kWavesPerBlock, 0, st>>>(tree, rays, n, out, err);
   else
     k_force<false><<<nb, kWave * kWavesPerBlock, 0, st>>>(tree, rays, n, out, err);
+}
+
+void launch_halo_force(hipStream_t st, const TreeDesc* tree, const double* rays, int64_t n,
+                       int direct, double* out, uint32_t* err) {
+  const unsigned nb = blocks_for(n, kWave * kWavesPerBlock);
+  if (nb == 0) return;
+  k_halo_force<<<nb, kWave * kWavesPerBlock, 0, st>>>(tree, rays, n, direct, out, err);
 }
 
 void launch_direct(hipStream_t st, const PartD* parts, int64_t np, double ia, const double* rays,

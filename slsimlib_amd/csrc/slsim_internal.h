@@ -70,6 +70,42 @@ struct PartD {
 };
 static_assert(sizeof(PartD) == 24, "PartD layout");
 
+// Halo-of-halos planes (LensHaloHalos<LensHaloNFW>, include/particle_halo.h:
+// 1092-1227): cells of the TreeQuadHalos quadtree (include/quadTreeHalos.h),
+// children of a cell contiguous (child0..child3 in the reference's order).
+struct alignas(16) HNode {
+  double cx, cy;        // centre of mass
+  double mass;
+  double q0, q1, q2;    // quadrupole about the centre of mass
+  double r2crit;        // (1.15470 rcom / theta)^2
+  int32_t child;        // first child, -1 for a leaf
+  int32_t cmask;        // children with halos
+  int32_t big_off;      // the cell's big-halo list in the tree's index array
+  int32_t nbig;
+  double pad;
+};
+static_assert(sizeof(HNode) == 80, "HNode layout");
+
+// One symmetric LensHaloNFW (fratio 1) as the reference stores it: float
+// mass, Rmax and rscale, gmax = g(Rsize/rscale) from the tables.
+struct HaloNFW {
+  double x, y;          // physical position (LensHalo::getX)
+  double gmax;
+  float mass, rmax, rscale, pad;
+};
+static_assert(sizeof(HaloNFW) == 40, "HaloNFW layout");
+
+// LensHaloNFW's tables (MultiPlane/lens_halos.cpp:352-353, 482-540):
+// NTABLE entries over [0, maxrm), entry i = (x_i, g(x_i), f(x_i), g2(x_i)).
+constexpr int kNfwTable = 10000;
+constexpr double kNfwMaxrm = 100.0;
+struct NfwEntry {
+  double x, g, f, g2;
+};
+
+constexpr int32_t kTreeParticles = 0;
+constexpr int32_t kTreeHalos = 1;
+
 struct TreeDesc {
   const NodeD* nodesD;
   const NodeF* nodesF;
@@ -79,6 +115,13 @@ struct TreeDesc {
   double mass, comx, comy;   // total mass and centre of mass (the mass sheet's terms)
   int32_t nnodes;
   int32_t nparts;
+  // halo-of-halos trees (kind == kTreeHalos)
+  int32_t kind;
+  int32_t nhalos;
+  const HNode* hnodes;
+  const int32_t* hbig;       // big-halo lists (halo indices)
+  const HaloNFW* halos;
+  const NfwEntry* nfw;       // kNfwTable entries
 };
 
 // Per-slot ray state, structure of arrays (slot = bundle * 64 + lane).
@@ -109,8 +152,8 @@ struct StepJob {
   double* sbase;                // ray state, state_view(sbase, nslots)
   int32_t nslots;
   int32_t j;                    // plane index of this step
-  int32_t ntrees;
-  int32_t pad;
+  int32_t ntrees;               // particle trees, trees[0, ntrees)
+  int32_t nhtrees;              // halo-of-halos trees, trees[ntrees, ntrees + nhtrees)
   const TreeDesc* trees;        // the plane's halos (device array)
   double Dphys;                 // Dl[j] / (1 + z_j)
   double Gfac;                  // charge * Dl[j] / (1 + z_j)
@@ -182,6 +225,13 @@ struct slsim_plane_s {
   slsim::NodeF* nodesF = nullptr;
   slsim::PartD* partsD = nullptr;
   slsim::PartD* bigD = nullptr;
+  // halo-of-halos plane (kind == kTreeHalos)
+  int32_t kind = 0;
+  slsim::HNode* hnodes = nullptr;
+  int32_t* hbig = nullptr;
+  slsim::HaloNFW* halos = nullptr;
+  slsim::NfwEntry* nfw = nullptr;
+  std::vector<slsim::HaloNFW> halos_host;
   // build products kept for export and direct sums
   double* box_p1 = nullptr;           // 2 per node (p1x, p1y)
   double* box_p2 = nullptr;           // 2 per node
@@ -204,7 +254,7 @@ struct slsim_lens_s {
   std::vector<int> order;                              // plane j -> main plane index
   std::vector<double> plane_redshifts, Dl, dDl;
   slsim::TreeDesc* trees_dev = nullptr;                // per plane: halos' descs
-  std::vector<int> tree_off, tree_cnt;
+  std::vector<int> tree_off, tree_cnt, tree_hcnt;   // per plane: first desc, particle / halo trees
   bool trees_dirty = true;
   double coorDist(double z1, double z2) const;
   void combinePlanes();
@@ -215,12 +265,14 @@ struct slsim_lens_s {
 namespace slsim {
 struct FlatLCDM;
 // kernel launchers (shoot.hip)
+// halos: some job of the launch has halo-of-halos trees (selects the kernels
+// that walk them)
 void launch_plane_step(int precision, bool count, uint32_t nitems, hipStream_t st,
-                       const StepJob* jobs, const WorkItem* items, uint32_t* err);
+                       const StepJob* jobs, const WorkItem* items, uint32_t* err, bool halos);
 void launch_zero_u32(hipStream_t st, uint32_t* p, int n);
 void launch_plane_step_queue(int precision, unsigned nblocks, uint32_t nitems, hipStream_t st,
                              const StepJob* jobs, const WorkItem* items, uint32_t* counter,
-                             uint32_t* err);
+                             uint32_t* err, bool halos);
 void launch_init_grid(hipStream_t st, const RayState& s, int64_t nslots, int64_t N,
                       int64_t tiles_x, int tw, double c0, double c1, double range, int32_t jmax,
                       double aal, double bbl);
@@ -239,6 +291,11 @@ void launch_finalize_array(hipStream_t st, const RayState& s, int64_t n, const u
 void launch_ray_morton(hipStream_t st, const double* x, int64_t n, double x0, double y0,
                        double inv_w, uint64_t* code, uint32_t* idx);
 void launch_minmax_rays(hipStream_t st, const double* x, int64_t n, double* part, int nblk);
+int build_halo_plane(slsim_plane p, const double* x, const double* y, const float* mass,
+                     const float* rsize, const float* conc, size_t n, float inv_area,
+                     float theta_force);
+void launch_halo_force(hipStream_t st, const TreeDesc* tree, const double* rays, int64_t n,
+                       int direct, double* out, uint32_t* err);
 void launch_force(hipStream_t st, int precision, const TreeDesc* tree, const double* rays,
                   int64_t n, double* out, uint32_t* err);
 void launch_direct(hipStream_t st, const PartD* parts, int64_t np, double ia, const double* rays,

@@ -1,0 +1,154 @@
+"""Halo-of-halos planes (LensHaloHalos<LensHaloNFW>) on the device: the
+TreeQuadHalos walk and the sum over every halo against the oracle, a
+light-cone that mixes particle halos and NFW halos (one plane holding both)
+through GridMap against the oracle's Lens, and batches and the server with
+such a lens giving each request its lone GridMap's bits.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+import oracle as O
+import slsimlib_amd as S
+from slsimlib_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _halo_set(n, seed, spread=0.4, rmax=(0.01, 0.25)):
+    """n NFW halos: masses log-uniform 1e10-1e13, Rsize in rmax (a few
+    larger than their cells), concentrations 3-15, Gaussian positions."""
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=n) * spread
+    y = rng.normal(size=n) * spread
+    m = (10 ** rng.uniform(10, 13, n)).astype(np.float32)
+    rs = (10 ** rng.uniform(np.log10(rmax[0]), np.log10(rmax[1]), n)).astype(np.float32)
+    c = rng.uniform(3, 15, n).astype(np.float32)
+    return x, y, m, rs, c
+
+
+def _rays(x, y, rs, c, n, seed):
+    rng = np.random.default_rng(seed)
+    r = rng.normal(size=(n, 2)) * 0.5
+    k = min(64, x.size)
+    r[:k, 0] = x[:k] + (rs[:k] / c[:k]) * 0.004      # x < 0.01: the closed-form interval
+    r[:k, 1] = y[:k]
+    r[k, 0], r[k, 1] = x[k], y[k]                     # on a halo centre (r^2 clamp)
+    return r
+
+
+def _close(got, want, near=65):
+    for k in range(5):
+        scale = np.abs(want[near:, k]).max()
+        assert np.abs(got[near:, k] - want[near:, k]).max() <= 1e-11 * scale, k
+        assert np.all(np.abs(got[:near, k] - want[:near, k]) <=
+                      1e-9 * np.abs(want[:near, k]) + 1e-11 * scale), k
+
+
+@pytest.mark.parametrize("theta,ia", [(0.1, 0.0), (0.5, 1.5e-3), (1e-6, 0.0)])
+def test_halo_plane_force_matches_oracle(ctx, theta, ia):
+    x, y, m, rs, c = _halo_set(2000, 11)
+    print("case: halo plane theta=%g inv_area=%g" % (theta, ia), flush=True)
+    h = S.LensHaloHalos(ctx, x, y, m, rs, c, 0.4, inv_area=ia, theta=theta)
+    t = O.HaloTree(x, y, m, rs, c, inv_area=ia, theta=theta)
+    rays = _rays(x, y, rs, c, 4096, 12)
+    got = h.force(rays)
+    want = t.force(rays)
+    # rays within 0.01 rscale of a centre take the profile's closed forms,
+    # whose cancellation (the reference's own) magnifies the last-bit
+    # differences between the device's and the host's log/atanh: per ray
+    _close(got, want)
+    _close(h.direct_sum(rays[:512]), t.force(rays[:512], direct=True))
+    info = h.info()
+    assert info["n_particles"] == 2000 and info["n_nodes"] > 1
+
+
+def test_halo_plane_errors(ctx):
+    x, y, m, rs, c = _halo_set(10, 13)
+    c[3] = np.float32(150.0)                   # Rsize/rscale beyond the tables' end
+    with pytest.raises(S.SlsimError) as e:
+        S.LensHaloHalos(ctx, x, y, m, rs, c, 0.4)
+    assert e.value.code == 1
+    with pytest.raises(S.SlsimError) as e:
+        S.LensHaloHalos(ctx, np.zeros(8), np.zeros(8), m[:8], rs[:8], np.full(8, 5, np.float32),
+                        0.4)
+    assert e.value.code == 4                   # particles in same place
+
+
+def _mixed_lens(ctx, oracle=False):
+    n = 16384
+    px, py = S.gen_nfw(77, n, 1e13, 8.0, 0.5)
+    pm = np.full(n, np.float32(1e13 / n), np.float32)
+    ph = np.full(n, np.float32(2e-4), np.float32)
+    ax, ay, am, ars, ac = _halo_set(600, 14, spread=0.15, rmax=(0.005, 0.1))
+    bx, by, bm, brs, bc = _halo_set(300, 15, spread=0.2, rmax=(0.005, 0.05))
+    if oracle:
+        L = O.Lens(2.0)
+        L.moveinMainHalo(O.Tree(px, py, pm, ph), 0.5)
+        L.moveinMainHalo(O.HaloTree(ax, ay, am, ars, ac, inv_area=0.0), 0.3)
+        L.moveinMainHalo(O.HaloTree(bx, by, bm, brs, bc, inv_area=0.0), 0.5)
+        return L
+    lens = S.Lens(ctx, 2.0)
+    lens.moveinMainHalo(S.LensHaloParticles(ctx, px, py, pm, ph, 0.5))
+    lens.moveinMainHalo(S.LensHaloHalos(ctx, ax, ay, am, ars, ac, 0.3))
+    lens.moveinMainHalo(S.LensHaloHalos(ctx, bx, by, bm, brs, bc, 0.5))
+    return lens
+
+
+def test_mixed_lightcone_gridmap_matches_oracle(ctx):
+    lens = _mixed_lens(ctx)
+    L = _mixed_lens(ctx, oracle=True)
+    assert L.tables()["nplanes"] == 2
+    N, rg = 64, W.grid_range(60.0)
+    g = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    y, A = g.y(), g.A()
+    yr, Ar = L.shoot(O.gridmap_positions(N, (0.0, 0.0), rg))
+    assert np.abs(y - yr).max() <= 1e-12 * np.abs(yr).max()
+    assert np.abs(A - Ar).max() <= 1e-8
+    # FP32C: the particle walk in FP32 (compensated), the halo trees in FP64
+    g32 = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP32C)
+    x = g.positions()
+    a64, a32 = x - y, x - g32.y()
+    assert np.abs(a32 - a64).max() <= 1e-5 * np.abs(a64).max()
+
+
+def test_batch_and_server_with_halo_planes(ctx):
+    mixed = _mixed_lens(ctx)
+    plain, _ = W.build_c1_lens(ctx, n=16384)
+    rg = W.grid_range(60.0)
+    reqs = [(mixed, 48, (0.0, 0.0), rg, 0.0), (plain, 40, (1e-6, 0.0), W.grid_range(30.0), 0.0),
+            (mixed, 33, (2e-5, -1e-5), rg, 1.2)]
+    for prec in (S.SLSIM_FP64, S.SLSIM_FP32C):
+        ref = []
+        for lens, N, c, r, zs in reqs:
+            g = S.GridMap(lens, N, c, r, prec, zs=zs)
+            ref.append((g.y(), g.A()))
+            g.close()
+        b = S.Batch(ctx, reqs, prec)
+        for sched in ({}, {"tile_log2": 2, "queue": 1, "order": 1, "graph": 1}):
+            b.set_schedule(**sched)
+            b.run()
+            for i in range(len(reqs)):
+                y, A = b.output(i)
+                assert np.array_equal(y, ref[i][0]) and np.array_equal(A, ref[i][1]), (prec, i)
+    srv = S.Server(ctx, hbm_budget=4 << 30)
+    out = {}
+
+    def client(k):
+        r = reqs[k % len(reqs)]
+        rid = srv.submit(r[0], r[1], r[2], r[3], r[4], S.SLSIM_FP32C)
+        out[k] = srv.wait(rid, r[1])
+
+    th = [threading.Thread(target=client, args=(k,)) for k in range(6)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for k, (y, A, info) in out.items():
+        lens, N, c, r, zs = reqs[k % len(reqs)]
+        g = S.GridMap(lens, N, c, r, S.SLSIM_FP32C, zs=zs)
+        assert info["status"] == 0
+        assert np.array_equal(y, g.y()) and np.array_equal(A, g.A()), k
+        g.close()
+    srv.close()

@@ -1,0 +1,142 @@
+"""Pins the oracle's restatement of LensHaloNFW and TreeQuadHalos (the
+halo-of-halos planes of LensHaloHalos) against what the mathematics fixes:
+the NFW profile identities, Gauss's law and the tangential-shear identity for
+force_halo, the point mass beyond Rmax, and the tree against the sum over
+every halo as the opening angle goes to zero (including the reference's
+second count of a halo that is big in an internal cell).  CPU only.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+# ------------------------------------------------------------- NFW profile
+@pytest.mark.parametrize("x", [0.003, 0.05, 0.4, 0.93, 1.08, 2.5, 17.0, 80.0])
+def test_nfw_profile_identities(x):
+    g = O.nfw_function(0, x)
+    f = O.nfw_function(1, x)
+    g2 = O.nfw_function(2, x)
+    # projected mass: dM/dx = 2 pi x Sigma, i.e. g'(x) = x f(x)
+    h = 1e-3 * x
+    dg = (O.nfw_function(0, x + h) - O.nfw_function(0, x - h)) / (2 * h)
+    assert abs(dg - x * f) <= 1e-6 * abs(x * f)
+    # tangential shear: g2 = 4 g / x^2 - 2 f (mean convergence inside minus local)
+    assert abs(g2 - (4 * g / x ** 2 - 2 * f)) <= 1e-9 * max(abs(g2), 1e-12)
+
+
+def test_nfw_profile_continuous_at_one_and_tables():
+    # the closed forms cancel near x = 1 (the reference's own rounding), so
+    # continuity is checked 1e-4 away, where the slope moves them by < 1e-4
+    for w in range(3):
+        a, b, c = (O.nfw_function(w, v) for v in (1 - 1e-4, 1.0, 1 + 1e-4))
+        assert abs(a - b) < 2e-4 and abs(c - b) < 2e-4
+    # linear interpolation on the 0.01 grid: error <= dx^2/8 max|f''| on the interval
+    xs = np.linspace(0.012, 98.5, 400)
+    dx = 0.01
+    for w in range(3):
+        for x in xs:
+            exact = O.nfw_function(w, x)
+            j = int(x / dx)
+            hh = dx / 4
+            f2 = max(abs(O.nfw_function(w, t + hh) - 2 * O.nfw_function(w, t) +
+                         O.nfw_function(w, t - hh)) / hh ** 2
+                     for t in (j * dx + hh, (j + 0.5) * dx, (j + 1) * dx - hh))
+            bound = dx * dx / 8 * f2 * 1.5 + 1e-13 * max(1.0, abs(exact))
+            assert abs(O.nfw_interpolate(w, x) - exact) <= bound, (w, x)
+        # first interval: the function itself
+        assert O.nfw_interpolate(w, 0.004) == O.nfw_function(w, 0.004)
+
+
+# ------------------------------------------------------------ force_halo
+def _ring(m, rs, c, r):
+    return O.nfw_force(m, rs, c, np.array([[r, 0.0]]))[0]
+
+
+def test_nfw_force_gauss_law_and_shear_identity():
+    from scipy.integrate import quad
+    m, rsize, c = 1e12, 0.2, 8.0
+    for r in (0.004, 0.03, 0.11, 0.19):
+        out = _ring(m, rsize, c, r)
+        menc_alpha = -out[0] * np.pi * r            # alpha = -M(<r)/(pi r) xhat (tree sign)
+        kap = lambda rr: _ring(m, rsize, c, rr)[2]
+        menc_kappa, _ = quad(lambda rr: 2 * np.pi * rr * kap(rr), 1e-9, r, limit=400,
+                             epsrel=1e-8)
+        # tables interpolate g and f separately: agreement to their interpolation error
+        assert abs(menc_alpha - menc_kappa) <= 2e-3 * menc_alpha
+        # gamma_t = mean kappa inside - kappa, gamma_t = -gamma0 on the x axis
+        gt = -out[3]
+        expect = menc_alpha / (np.pi * r * r) - out[2]
+        assert abs(gt - expect) <= 1e-3 * abs(expect)
+        assert abs(out[1]) == 0.0 and abs(out[4]) == 0.0
+
+
+def test_nfw_force_point_mass_outside_and_continuity():
+    m, rsize, c = 3e11, 0.1, 5.0
+    rmax = float(np.float32(rsize))
+    for r in (rmax * 1.0001, 0.3, 2.0):
+        out = _ring(m, rsize, c, r)
+        pm = float(np.float32(m)) / (np.pi * r)          # LensHalo stores the mass as float
+        assert abs(-out[0] - pm) <= 1e-14 * pm
+        assert out[2] == 0.0
+        assert abs(out[3] - (-pm / r)) <= 1e-14 * pm / r     # gamma0 = -M/(pi r^2)
+    inside = _ring(m, rsize, c, rmax * (1 - 1e-9))
+    outside = _ring(m, rsize, c, rmax * (1 + 1e-9))
+    assert abs(inside[0] - outside[0]) <= 1e-6 * abs(outside[0])
+
+
+# ------------------------------------------------------------------- tree
+def _halos(n, seed, rmax_scale=1e-3):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=n) * 0.3
+    y = rng.normal(size=n) * 0.3
+    m = (10 ** rng.uniform(10, 13, n)).astype(np.float32)
+    rs = (rmax_scale * (0.5 + rng.random(n))).astype(np.float32)
+    c = (4 + 8 * rng.random(n)).astype(np.float32)
+    return x, y, m, rs, c
+
+
+def test_tree_small_halos_converges_to_direct_sum():
+    x, y, m, rs, c = _halos(3000, 1)
+    rng = np.random.default_rng(2)
+    rays = rng.normal(size=(300, 2)) * 0.3
+    rays[:20] = np.stack([x[:20] + 2e-4, y[:20]], 1)       # inside halos
+    t = O.HaloTree(x, y, m, rs, c, inv_area=0.0, theta=1e-6)
+    d = t.force(rays, direct=True)
+    f = t.force(rays)
+    scale = np.abs(d[:, :2]).max()
+    assert np.abs(f[:, :2] - d[:, :2]).max() <= 1e-11 * scale
+    assert np.abs(f[:, 2:] - d[:, 2:]).max() <= 1e-11 * np.abs(d[:, 2:]).max()
+    # opening angle 0.5: a multipole approximation, close but not equal
+    f5 = O.HaloTree(x, y, m, rs, c, inv_area=0.0, theta=0.5).force(rays)
+    err = np.abs(f5[:, :2] - d[:, :2]).max() / scale
+    assert 1e-9 < err < 3e-2
+
+
+def test_tree_counts_a_big_internal_halo_twice():
+    # one halo larger than half the root box is on the root's big list and
+    # again in its leaf (its cell keeps all halos): with every cell opened
+    # the tree gives the direct sum plus that halo once more
+    x, y, m, rs, c = _halos(400, 3)
+    rs[17] = 0.9 * (max(x.max() - x.min(), y.max() - y.min()))
+    t = O.HaloTree(x, y, m, rs, c, inv_area=0.0, theta=1e-6)
+    rng = np.random.default_rng(4)
+    rays = rng.normal(size=(50, 2)) * 0.3
+    d = t.force(rays, direct=True)
+    f = t.force(rays)
+    one = O.nfw_force(m[17], rs[17], c[17], np.stack([x[17] - rays[:, 0], y[17] - rays[:, 1]], 1))
+    assert np.abs(f - (d + one)).max() <= 1e-11 * np.abs(d).max()
+
+
+def test_tree_background_sheet():
+    # inv_area removes a uniform sheet of the halos' total mass: kappa
+    # lowered by inv_area * M for every ray whatever the cells
+    x, y, m, rs, c = _halos(500, 5)
+    ia = 2.5
+    rays = np.random.default_rng(6).normal(size=(40, 2)) * 0.3
+    for theta in (1e-6, 0.3):
+        a = O.HaloTree(x, y, m, rs, c, inv_area=0.0, theta=theta).force(rays)
+        b = O.HaloTree(x, y, m, rs, c, inv_area=ia, theta=theta).force(rays)
+        dk = b[:, 2] - a[:, 2]
+        assert np.allclose(dk, -float(np.float32(ia)) * float(np.sum(m.astype(np.float64))),
+                           rtol=1e-9, atol=0)
