@@ -260,6 +260,53 @@ def run_formats(S, W, ctx, n, cpu=True, nsmooth=32):
     return out
 
 
+def run_halos(S, W, ctx, n, cpu=True, N=2048):
+    """Halo-of-halos plane: one LensHaloHalos of n NFW halos (masses
+    log-uniform 1e10-1e14 Msun, Rsize = r200, c = 10 (M/1e12)^-0.1, centres
+    uniform in a 5' field) at z = 0.5, source at z = 2, N^2 FP64 GridMap.
+    The oracle's restatement of TreeQuadHalos on 16 host threads over a
+    65,536-ray subsample."""
+    rng = np.random.default_rng(4242)
+    z = 0.5
+    m = (10 ** rng.uniform(10, 14, n)).astype(np.float32)
+    rs = np.array([S.nfw_r200(float(v), z) for v in m], np.float32)
+    c = (10 * (m / 1e12) ** -0.1).astype(np.float32)
+    side = 300 * S.arcsecTOradians * S.coorDist(0.0, z) / (1 + z)
+    hx = (rng.random(n) - 0.5) * side
+    hy = (rng.random(n) - 0.5) * side
+    t0 = time.perf_counter()
+    halo = S.LensHaloHalos(ctx, hx, hy, m, rs, c, z)
+    build_s = time.perf_counter() - t0
+    lens = S.Lens(ctx, 2.0)
+    lens.moveinMainHalo(halo)
+    rg = W.grid_range(300.0)
+    g = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP64, shoot=False)
+    g.shoot()
+    t1 = time.perf_counter()
+    st = g.shoot()
+    ctx.synchronize()
+    dt = time.perf_counter() - t1
+    out = {"halos": n, "grid": N, "host_build_s": build_s, "rays_per_s": N * N / dt,
+           "walk_launch_ms": st["walk_ms"] / max(st["n_walk_launches"], 1),
+           "nodes": halo.info()["n_nodes"]}
+    if cpu:
+        import oracle as O
+        L = O.Lens(2.0)
+        L.moveinMainHalo(O.HaloTree(hx, hy, m, rs, c), z)
+        x = g.positions()[::max(1, (N * N) // 65536)]
+        t2 = time.perf_counter()
+        yr, Ar = L.shoot(x, nthreads=16)
+        tc = time.perf_counter() - t2
+        y = g.y()[::max(1, (N * N) // 65536)]
+        out.update({"cpu_rays_per_s": x.shape[0] / tc, "cpu_threads": 16,
+                    "cpu_sample_rays": int(x.shape[0]),
+                    "max_rel_y_vs_oracle": float(np.abs(y - yr).max() / np.abs(yr).max())})
+        out["gpu_over_cpu"] = out["rays_per_s"] / out["cpu_rays_per_s"]
+    g.close()
+    log("halos:", out)
+    return out
+
+
 def run_precision_form(S, W, ctx, n=1 << 22, nplanes=20, N=4096):
     """Precision form: 20 planes x 4,194,304 float32-position particles,
     opening angle 0.2, 4096^2 rays; FP32C and FP64 trees against the FP64
@@ -327,6 +374,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip FP64, direct-sum, counts")
     ap.add_argument("--smoothing", type=int, default=0, metavar="N",
                     help="also time the smoothing sizes (Nsmooth 64) of N 3-D particles")
+    ap.add_argument("--halos", type=int, default=0, metavar="N",
+                    help="also time a plane of N NFW halos (LensHaloHalos) on a 2048^2 GridMap")
     ap.add_argument("--formats", type=int, default=0, metavar="N",
                     help="also time the particle-file readers (csv5, csv4 with device "
                          "sizes, glmb) on N particles and a 2048^2 FITS write")
@@ -538,6 +587,8 @@ def main():
         out["smoothing"] = run_smoothing(S, W, ctx, args.smoothing, cpu=not args.no_cpu)
     if args.formats:
         out["formats"] = run_formats(S, W, ctx, args.formats, cpu=not args.no_cpu)
+    if args.halos:
+        out["halos"] = run_halos(S, W, ctx, args.halos, cpu=not args.no_cpu)
 
     if args.precision_form:
         out["precision_form"] = run_precision_form(S, W, ctx)

@@ -12,6 +12,8 @@
 //   PixelMap<T>::printFITS     ImageProcessing/pixelmap.cpp:1318-1388
 //   MakeParticleLenses, SimFileFormat  include/particle_halo.h:964-1080,
 //                              include/particle_types.h:32-43
+//   LensHaloNFW (symmetric), LensHaloHalos<LensHaloNFW>
+//                              include/lens_halos.h:745-865, particle_halo.h:1092-1227
 // compiles against this header and runs the rays on the GPU.  Link with
 // libslsim_amd.so.  Reference behaviours kept on purpose are marked
 // "(reference)".  Deliberate differences: a non-flat cosmology throws
@@ -407,6 +409,103 @@ class LensHaloParticles {
   double theta_force_ = 0.1;
   double mcenter[3] = {0, 0, 0};
   std::vector<PType> trash_collector;
+  slsim_plane plane = nullptr;
+};
+
+// ------------------------------------------------------- NFW halo planes
+enum class EllipMethod { Fourier, Pseudo, Schramm, Keeton };
+
+// LensHaloNFW (include/lens_halos.h:745-865, MultiPlane/lens_halos.cpp:
+// 389-440) as a member of a LensHaloHalos: symmetric profiles only (fratio
+// 1; others throw std::invalid_argument).  The position is set in angle
+// with setTheta and is theta * D_A(zlens) in physical Mpc, as the reference's.
+class LensHaloNFW {
+ public:
+  LensHaloNFW(float my_mass, float my_Rsize, PosType my_zlens, float my_concentration,
+              float my_fratio, float my_pa, const COSMOLOGY& cosmo,
+              EllipMethod my_ellip_method = EllipMethod::Pseudo)
+      : mass(my_mass), Rsize(my_Rsize), zlens(my_zlens), concentration(my_concentration) {
+    (void)my_pa; (void)my_ellip_method;
+    if (my_fratio != 1)
+      throw std::invalid_argument("LensHaloNFW: only fratio = 1 (symmetric) halos are built");
+    Dist = cosmo.angDist(my_zlens);
+  }
+  void setTheta(PosType PosX, PosType PosY) { posHalo[0] = PosX; posHalo[1] = PosY; }
+  void getX(PosType* MyPosHalo) const {
+    MyPosHalo[0] = posHalo[0] * Dist;
+    MyPosHalo[1] = posHalo[1] * Dist;
+  }
+  PosType operator[](int i) const { return posHalo[i] * Dist; }
+  float get_mass() const { return mass; }
+  float getRsize() const { return Rsize; }
+  float get_Rmax() const { return Rsize; }
+  PosType getZlens() const { return zlens; }
+  float getConcentration() const { return concentration; }
+
+ private:
+  float mass, Rsize;
+  PosType zlens;
+  float concentration;
+  PosType Dist = -1;
+  PosType posHalo[2] = {0, 0};
+};
+
+// LensHaloHalos<HType> (include/particle_halo.h:1092-1227): a main halo made
+// of halos on a TreeQuadHalos; the tree is built and walked by the library.
+template <typename HType>
+class LensHaloHalos {
+  static_assert(std::is_same<HType, LensHaloNFW>::value,
+                "LensHaloHalos is built for LensHaloNFW members");
+
+ public:
+  LensHaloHalos(std::vector<HType>& pvector, float redshift, const COSMOLOGY& cosmo,
+                float inv_area, float my_theta_force = 0.1, bool mu_periodic_buffer = false,
+                bool verbose = false)
+      : zlens(redshift) {
+    (void)cosmo; (void)verbose;
+    if (mu_periodic_buffer)
+      throw std::invalid_argument("LensHaloHalos: the periodic buffer is not built");
+    std::swap(pvector, trash_collector);
+    const size_t n = trash_collector.size();
+    std::vector<double> x(n), y(n);
+    std::vector<float> m(n), r(n), c(n);
+    mass = 0;
+    for (size_t i = 0; i < n; ++i) {
+      double p[2];
+      trash_collector[i].getX(p);
+      x[i] = p[0];
+      y[i] = p[1];
+      m[i] = trash_collector[i].get_mass();
+      r[i] = trash_collector[i].getRsize();
+      c[i] = trash_collector[i].getConcentration();
+      mass += m[i];
+    }
+    slsim_cpp::check(slsim_halos_create_nfw(slsim_cpp::context(), x.data(), y.data(), m.data(),
+                                            r.data(), c.data(), n, inv_area, my_theta_force,
+                                            &plane));
+  }
+  LensHaloHalos(LensHaloHalos&& h) noexcept
+      : zlens(h.zlens), mass(h.mass), trash_collector(std::move(h.trash_collector)),
+        plane(h.plane) {
+    h.plane = nullptr;
+  }
+  LensHaloHalos(const LensHaloHalos&) = delete;
+  ~LensHaloHalos() {
+    if (plane) slsim_plane_destroy(plane);
+  }
+  size_t getN() const { return trash_collector.size(); }
+  float getZlens() const { return zlens; }
+  double get_mass() const { return mass; }
+  slsim_plane release_plane() {
+    slsim_plane p = plane;
+    plane = nullptr;
+    return p;
+  }
+
+ private:
+  float zlens;
+  double mass = 0;
+  std::vector<HType> trash_collector;
   slsim_plane plane = nullptr;
 };
 

@@ -62,12 +62,12 @@ def build(verbose=False, jobs=4):
 
 
 def build_cpp_example(verbose=False):
-    """Compiles tests/cpp/dropin_gridmap.cpp and tests/cpp/dropin_formats.cpp,
+    """Compiles tests/cpp/dropin_gridmap.cpp, dropin_formats.cpp and dropin_halos.cpp,
     reference-style C++ programs against include/glamer_amd.h linked with
     the library."""
     root = os.path.dirname(HERE)
     exes = []
-    for name in ("dropin_gridmap", "dropin_formats"):
+    for name in ("dropin_gridmap", "dropin_formats", "dropin_halos"):
         src = os.path.join(root, "tests", "cpp", name + ".cpp")
         exe = os.path.join(root, "tests", "cpp", name)
         cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "include"), src,

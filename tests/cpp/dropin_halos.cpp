@@ -1,0 +1,56 @@
+// Reference-style user program against the drop-in header for a light-cone
+// that mixes a particle halo (z = 0.5) and a LensHaloHalos of 200 NFW halos
+// placed with setTheta (z = 0.3): shoots a 64 x 64 GridMap and writes x, y
+// and A of every ray as raw float64 to argv[1].
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "glamer_amd.h"
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: %s out.bin\n", argv[0]);
+    return 2;
+  }
+  COSMOLOGY cosmo(0.3, 0.7, 0.7);
+  const size_t n = 16384;
+  std::vector<double> x(n), y(n);
+  if (slsim_gen_nfw(2024, n, 5e12, 8.0, 0.5, 0.3, 0.7, 0.7, 0.0, 0.0, x.data(), y.data())) return 1;
+  std::vector<ParticleType<double>> particles(n);
+  for (size_t i = 0; i < n; ++i) {
+    particles[i].x[0] = x[i];
+    particles[i].x[1] = y[i];
+    particles[i].x[2] = 0.0;
+    particles[i].Mass = (float)(5e12 / n);
+    particles[i].Size = 2e-4f;
+    particles[i].type = 0;
+  }
+  std::vector<LensHaloNFW> nfw;
+  for (int i = 0; i < 200; ++i) {
+    const float mass = (float)std::pow(10.0, 10.5 + 2.0 * ((i * 37) % 100) / 100.0);
+    const float rsize = (float)(0.01 + 0.002 * (i % 17));
+    const float conc = (float)(4.0 + (i % 9));
+    nfw.emplace_back(mass, rsize, 0.3, conc, 1.0f, 0.0f, cosmo);
+    nfw.back().setTheta(((i % 20) - 9.5) * 3e-6, ((i / 20) - 4.5) * 5e-6);
+  }
+  long seed = -1;
+  Lens lens(&seed, 2.0, cosmo);
+  LensHaloParticles<ParticleType<double>> halo(particles, 0.5, cosmo, Point_2d(0, 0), false, 0);
+  lens.moveinMainHalo(halo, true);
+  LensHaloHalos<LensHaloNFW> field(nfw, 0.3f, cosmo, 0.0f);
+  lens.moveinMainHalo(field, true);
+  const double center[2] = {0.0, 0.0};
+  GridMap grid(&lens, 64, center, 60 * arcsecTOradians);
+  std::vector<double> out;
+  for (size_t i = 0; i < grid.getNumberOfPoints(); ++i) {
+    RAY r = grid.ray(i);
+    out.insert(out.end(), {r.x[0], r.x[1], r.y[0], r.y[1], r.A[0], r.A[1], r.A[2], r.A[3]});
+  }
+  FILE* f = std::fopen(argv[1], "wb");
+  if (!f) return 1;
+  std::fwrite(out.data(), sizeof(double), out.size(), f);
+  std::fclose(f);
+  std::printf("dropin_halos: %zu rays, %d planes\n", out.size() / 8, lens.getNplanes());
+  return 0;
+}

@@ -152,3 +152,41 @@ def test_batch_and_server_with_halo_planes(ctx):
         assert np.array_equal(y, g.y()) and np.array_equal(A, g.A()), k
         g.close()
     srv.close()
+
+
+def test_cpp_dropin_halos_program(ctx, tmp_path):
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests",
+                       "cpp", "dropin_halos")
+    if not os.path.exists(exe):
+        pytest.fail("tests/cpp/dropin_halos is not built (run __graft_entry__.build())")
+    out = tmp_path / "halos.bin"
+    r = subprocess.run([exe, str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    v = np.fromfile(out, dtype=np.float64).reshape(-1, 8)
+    # the same light-cone through the Python view: LensHaloNFW positions are
+    # theta * D_A(zlens)
+    n = 16384
+    px, py = S.gen_nfw(2024, n, 5e12, 8.0, 0.5)
+    pm = np.full(n, np.float32(5e12 / n), np.float32)
+    ph = np.full(n, np.float32(2e-4), np.float32)
+    i = np.arange(200)
+    m = np.array([np.float32(10.0 ** (10.5 + 2.0 * ((k * 37) % 100) / 100.0)) for k in i])
+    rs = np.array([np.float32(0.01 + 0.002 * (k % 17)) for k in i])
+    c = np.array([np.float32(4.0 + (k % 9)) for k in i])
+    dist = S.coorDist(0.0, 0.3) / (1 + 0.3)
+    hx = ((i % 20) - 9.5) * 3e-6 * dist
+    hy = ((i // 20) - 4.5) * 5e-6 * dist
+    lens = S.Lens(ctx, 2.0)
+    lens.moveinMainHalo(S.LensHaloParticles(ctx, px, py, pm, ph, 0.5))
+    lens.moveinMainHalo(S.LensHaloHalos(ctx, hx, hy, m, rs, c, 0.3))
+    g = S.GridMap(lens, 64, (0.0, 0.0), 60 * S.arcsecTOradians)
+    assert np.array_equal(v[:, 2:4], g.y()) and np.array_equal(v[:, 4:8], g.A())
+    L = O.Lens(2.0)
+    L.moveinMainHalo(O.Tree(px, py, pm, ph), 0.5)
+    L.moveinMainHalo(O.HaloTree(hx, hy, m, rs, c), 0.3)
+    yr, Ar = L.shoot(v[:, :2])
+    assert np.abs(v[:, 2:4] - yr).max() <= 1e-12 * np.abs(yr).max()
+    assert np.abs(v[:, 4:8] - Ar).max() <= 1e-8
+    g.close()
