@@ -57,7 +57,13 @@ const std::vector<NfwEntry>& nfw_tables() {
     const double dx = kNfwMaxrm / (double)kNfwTable;
     for (int i = 0; i < kNfwTable; ++i) {
       const double x = i * dx;
-      t[i] = {x, g_fn(x), f_fn(x), g2_fn(x)};
+      t[i] = {x, g_fn(x), f_fn(x), g2_fn(x), 0, 0, 0, 0};
+    }
+    for (int i = 0; i + 1 < kNfwTable; ++i) {
+      const double w = t[i + 1].x - t[i].x;
+      t[i].sg = (t[i + 1].g - t[i].g) / w;
+      t[i].sf = (t[i + 1].f - t[i].f) / w;
+      t[i].sg2 = (t[i + 1].g2 - t[i].g2) / w;
     }
   });
   return t;
@@ -199,6 +205,9 @@ int build_halo_plane(slsim_plane p, const double* x, const double* y, const floa
     h[i].rmax = rsize[i];
     h[i].rscale = rs;
     h[i].gmax = interp_g((double)xm);
+    h[i].inv_gmax = 1.0 / h[i].gmax;
+    h[i].inv_rscale = 1.0 / (double)rs;
+    h[i].mass_pi = (double)mass[i] / kPI;
     h[i].pad = 0;
     rmax[i] = rsize[i];
   }

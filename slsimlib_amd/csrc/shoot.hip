@@ -305,6 +305,15 @@ __device__ inline double nfw_g2(double x) {
   return ans + 4 * (2 / x / x + 1 / y) * atan(sqrt((x - 1) / (x + 1))) / sqrt(y);
 }
 
+// The closed forms below the first table interval (rays within 0.01 rscale
+// of a centre), kept out of line: their log/atan/atanh would otherwise set
+// the register count of every kernel that walks halo-of-halos trees.
+__device__ __attribute__((noinline)) void nfw_closed(double y, double* g, double* f, double* g2) {
+  *g = nfw_g(y);
+  *f = nfw_f(y);
+  *g2 = nfw_g2(y);
+}
+
 // InterpolateFromTable for the three tables at once: linear between the
 // entries j and j+1, j = (int)(y / maxrm * NTABLE) (kept in range; halos are
 // accepted only with Rsize / rscale below the table's end).
@@ -312,61 +321,57 @@ __device__ inline void nfw_interp(cptr<NfwEntry> T, double y, double& g, double&
 #pragma clang fp contract(off)
   int j = (int)(y / kNfwMaxrm * kNfwTable);
   if (j == 0) {
-    g = nfw_g(y);
-    f = nfw_f(y);
-    g2 = nfw_g2(y);
+    nfw_closed(y, &g, &f, &g2);
     return;
   }
   j = j < 0 ? 0 : (j > kNfwTable - 2 ? kNfwTable - 2 : j);
-  const NfwEntry a = T[j], b = T[j + 1];
-  const double w = b.x - a.x, d = y - a.x;
-  g = (b.g - a.g) / w * d + a.g;
-  f = (b.f - a.f) / w * d + a.f;
-  g2 = (b.g2 - a.g2) / w * d + a.g2;
+  const NfwEntry a = T[j];
+  const double d = y - a.x;
+  g = a.sg * d + a.g;
+  f = a.sf * d + a.f;
+  g2 = a.sg2 * d + a.g2;
 }
 
 // LensHalo::force_halo of a symmetric NFW without point subtraction
 // (MultiPlane/lens_halos.cpp:1528-1672): force_point beyond Rmax (a point
 // mass), force_halo_sym inside with alpha_h, kappa_h, gamma_h of
 // include/lens_halos.h:842-853.  x0, x1: position relative to the halo.
+// Divisions by r^2, pi, rscale and gmax are multiplications by reciprocals
+// (a last-bit difference from the reference's quotients).
 __device__ inline void nfw_force(const HaloNFW& h, cptr<NfwEntry> T, double x0, double x1,
                                  Acc64& a) {
 #pragma clang fp contract(off)
-  const double rcm2 = x0 * x0 + x1 * x1;
+  double r2 = x0 * x0 + x1 * x1;
   const double R2 = (double)(h.rmax * h.rmax);   // float Rmax squared in float
-  const double mass = h.mass;
-  if (rcm2 < R2) {
-    double r2 = rcm2;
+  if (r2 < R2) {
     if (r2 < 1e-20) r2 = 1e-20;
+    const double inv = rcp64(r2);
+    const double prefac = h.mass_pi * inv;
     if (r2 < R2) {
-      const double prefac = mass / r2 / kPI;
-      const double x = sqrt(r2) / (double)h.rscale;
+      const double x = sqrt(r2) * h.inv_rscale;
       double g, f, g2;
       nfw_interp(T, x, g, f, g2);
-      const double alpha_h = -1.0 * g / h.gmax;
-      double tmp = (alpha_h + 0.0) * prefac;
+      double tmp = -g * h.inv_gmax * prefac;
       a.ax += tmp * x0;
       a.ay += tmp * x1;
-      const double kappa_h = 0.5 * x * x * f / h.gmax;
-      a.k += kappa_h * prefac;
-      const double gamma_h = -0.5 * x * x * g2 / h.gmax;
-      tmp = (gamma_h + 0.0) * prefac / r2;
+      a.k += 0.5 * x * x * f * h.inv_gmax * prefac;
+      tmp = -0.5 * x * x * g2 * h.inv_gmax * prefac * inv;
       a.g0 += 0.5 * (x0 * x0 - x1 * x1) * tmp;
       a.g1 += x0 * x1 * tmp;
       return;
     }
-    const double prefac = mass / r2 / kPI;
     a.ax += -1.0 * prefac * x0;
     a.ay += -1.0 * prefac * x1;
-    const double tmp = -2.0 * prefac / r2;
+    const double tmp = -2.0 * prefac * inv;
     a.g0 += 0.5 * (x0 * x0 - x1 * x1) * tmp;
     a.g1 += x0 * x1 * tmp;
     return;
   }
-  const double prefac = mass / rcm2 / kPI;
+  const double inv = rcp64(r2);
+  const double prefac = h.mass_pi * inv;
   a.ax += -1.0 * prefac * x0;
   a.ay += -1.0 * prefac * x1;
-  const double tmp = -2.0 * prefac / rcm2;
+  const double tmp = -2.0 * prefac * inv;
   a.g0 += 0.5 * (x0 * x0 - x1 * x1) * tmp;
   a.g1 += x0 * x1 * tmp;
 }
@@ -406,15 +411,17 @@ __device__ __forceinline__ uint64_t evalh64(const HNode& nd, uint64_t lvm, doubl
   const double r2 = dx * dx + dy * dy;
   const uint64_t farm = lanes_lt(nd.r2crit, r2) & lvm;
   if (lane_in(farm)) {
-    double tmp = nd.mass * (ia - 1.0 / r2 / kPI);
+    const double inv = rcp64(r2);
+    const double i2p = inv * inv * kInvPI;
+    double tmp = nd.mass * (ia - inv * kInvPI);
     a.ax += tmp * dx;
     a.ay += tmp * dy;
-    tmp = -2.0 * nd.mass / kPI / r2 / r2;
+    tmp = -2.0 * nd.mass * i2p;
     a.g0 += 0.5 * (dx * dx - dy * dy) * tmp;
     a.g1 += dx * dy * tmp;
-    a.ax -= (nd.q0 * dx + nd.q2 * dy) / (r2 * r2) / kPI;
-    a.ay -= (nd.q1 * dy + nd.q2 * dx) / (r2 * r2) / kPI;
-    tmp = 4 * (nd.q0 * dx * dx + nd.q1 * dy * dy + 2 * nd.q2 * dx * dy) / (r2 * r2 * r2) / kPI;
+    a.ax -= (nd.q0 * dx + nd.q2 * dy) * i2p;
+    a.ay -= (nd.q1 * dy + nd.q2 * dx) * i2p;
+    tmp = 4 * (nd.q0 * dx * dx + nd.q1 * dy * dy + 2 * nd.q2 * dx * dy) * inv * i2p;
     a.ax += tmp * dx;
     a.ay += tmp * dy;
     a.k -= nd.mass * ia;
@@ -422,8 +429,12 @@ __device__ __forceinline__ uint64_t evalh64(const HNode& nd, uint64_t lvm, doubl
   return lvm & ~farm;
 }
 
-// Walk of one halo-of-halos tree for the whole wavefront (the traversal of
-// walk64: a popped entry evaluates four contiguous children).
+// Walk of one halo-of-halos tree for the whole wavefront.  A stack entry is
+// a block of four contiguous cells (first cell | cells-with-halos mask << 28)
+// with the lanes that reach it; the root is the block {cell 0}.  The cells
+// of a block are evaluated last to first and each one a lane descends into
+// is pushed at once, so the first cell's subtree is walked next (one copy of
+// the cell evaluation keeps the kernel's registers down).
 __device__ void walk_halos64(const TreeDesc& t, double rx, double ry, bool active, Acc64& a,
                              int32_t* s_node, uint64_t* s_mask, uint32_t* err) {
   const int lane = lane_id();
@@ -432,11 +443,7 @@ __device__ void walk_halos64(const TreeDesc& t, double rx, double ry, bool activ
   const cptr<HNode> nodes = as_const(t.hnodes);
   const double ia = t.inv_area;
   int sp = 0;
-  {
-    const HNode root = nodes[0];
-    const uint64_t om = evalh64(root, am, rx, ry, ia, t, a);
-    if (om) push_entry(s_node, s_mask, sp, lane, root.child | (root.cmask << 28), om, err);
-  }
+  push_entry(s_node, s_mask, sp, lane, 1 << 28, am, err);
   while (sp > 0) {
     --sp;
     __builtin_amdgcn_wave_barrier();
@@ -445,21 +452,13 @@ __device__ void walk_halos64(const TreeDesc& t, double rx, double ry, bool activ
     __builtin_amdgcn_wave_barrier();
     const int base = entry & 0x0fffffff;
     const int cm = (entry >> 28) & 0xf;
-    int32_t pe[4];
-    uint64_t pm[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      pm[k] = 0;
-      pe[k] = 0;
-      if ((cm >> k) & 1) {
-        const HNode nd = nodes[base + k];
-        pm[k] = evalh64(nd, live, rx, ry, ia, t, a);
-        pe[k] = nd.child | (nd.cmask << 28);
-      }
+#pragma unroll 1
+    for (int k = 3; k >= 0; --k) {
+      if (!((cm >> k) & 1)) continue;
+      const HNode nd = nodes[base + k];
+      const uint64_t om = evalh64(nd, live, rx, ry, ia, t, a);
+      if (om) push_entry(s_node, s_mask, sp, lane, nd.child | (nd.cmask << 28), om, err);
     }
-#pragma unroll
-    for (int k = 3; k >= 0; --k)
-      if (pm[k]) push_entry(s_node, s_mask, sp, lane, pe[k], pm[k], err);
   }
 }
 

@@ -87,20 +87,26 @@ struct alignas(16) HNode {
 static_assert(sizeof(HNode) == 80, "HNode layout");
 
 // One symmetric LensHaloNFW (fratio 1) as the reference stores it: float
-// mass, Rmax and rscale, gmax = g(Rsize/rscale) from the tables.
+// mass, Rmax and rscale, gmax = g(Rsize/rscale) from the tables; with the
+// reciprocals the walk multiplies by.
 struct HaloNFW {
   double x, y;          // physical position (LensHalo::getX)
   double gmax;
+  double inv_gmax;      // 1 / gmax
+  double inv_rscale;    // 1 / rscale
+  double mass_pi;       // mass / pi
   float mass, rmax, rscale, pad;
 };
-static_assert(sizeof(HaloNFW) == 40, "HaloNFW layout");
+static_assert(sizeof(HaloNFW) == 64, "HaloNFW layout");
 
 // LensHaloNFW's tables (MultiPlane/lens_halos.cpp:352-353, 482-540):
-// NTABLE entries over [0, maxrm), entry i = (x_i, g(x_i), f(x_i), g2(x_i)).
+// NTABLE entries over [0, maxrm), entry i = (x_i, g(x_i), f(x_i), g2(x_i))
+// and the slopes (t_{i+1} - t_i) / (x_{i+1} - x_i) of InterpolateFromTable.
 constexpr int kNfwTable = 10000;
 constexpr double kNfwMaxrm = 100.0;
 struct NfwEntry {
   double x, g, f, g2;
+  double sg, sf, sg2, pad;
 };
 
 constexpr int32_t kTreeParticles = 0;
