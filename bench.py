@@ -307,6 +307,38 @@ def run_halos(S, W, ctx, n, cpu=True, N=2048):
     return out
 
 
+def run_images(S, W, ctx, nsrc, cpu=True, N=2048):
+    """GridMap::find_images: nsrc source positions around the caustics of the
+    65,536-particle NFW halo lens on its N^2 GridMap (30" field).  The oracle's
+    serial restatement is timed on 8 sources; the reference runs sources on
+    threads, so its rate is quoted per host thread and for 16 threads."""
+    lens, _ = W.build_c1_lens(ctx)
+    g = S.GridMap(lens, N, (0.0, 0.0), W.grid_range(30.0), S.SLSIM_FP64)
+    x, y, A = g.positions(), g.y(), g.A()
+    rng = np.random.default_rng(77)
+    centre = np.argsort((x ** 2).sum(1))[:100000]
+    ys = y[rng.choice(centre, nsrc)] + rng.normal(size=(nsrc, 2)) * 1e-8
+    g.find_images(ys[:16])
+    t0 = time.perf_counter()
+    rays, mult, _ = g.find_images(ys, 64)
+    tg = time.perf_counter() - t0
+    out = {"sources": nsrc, "grid": N, "images": int(rays.shape[0]),
+           "max_multiplicity": int(mult.max()), "gpu_seconds": tg, "sources_per_s": nsrc / tg}
+    if cpu:
+        import oracle as O
+        t1 = time.perf_counter()
+        r0, m0, _ = O.find_images(N, x, y, A, ys[:8])
+        tc = (time.perf_counter() - t1) / 8
+        k = int(m0.sum())
+        out.update({"cpu_seconds_per_source": tc, "cpu_sources_per_s_16_threads": 16 / tc,
+                    "equal_to_oracle_first8": bool(np.array_equal(m0, mult[:8]) and
+                                                   np.array_equal(r0, rays[:k]))})
+        out["gpu_over_cpu16"] = out["sources_per_s"] / out["cpu_sources_per_s_16_threads"]
+    g.close()
+    log("images:", out)
+    return out
+
+
 def run_precision_form(S, W, ctx, n=1 << 22, nplanes=20, N=4096):
     """Precision form: 20 planes x 4,194,304 float32-position particles,
     opening angle 0.2, 4096^2 rays; FP32C and FP64 trees against the FP64
@@ -374,6 +406,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip FP64, direct-sum, counts")
     ap.add_argument("--smoothing", type=int, default=0, metavar="N",
                     help="also time the smoothing sizes (Nsmooth 64) of N 3-D particles")
+    ap.add_argument("--images", type=int, default=0, metavar="S",
+                    help="also time GridMap::find_images for S sources on a 2048^2 grid")
     ap.add_argument("--halos", type=int, default=0, metavar="N",
                     help="also time a plane of N NFW halos (LensHaloHalos) on a 2048^2 GridMap")
     ap.add_argument("--formats", type=int, default=0, metavar="N",
@@ -589,6 +623,8 @@ def main():
         out["formats"] = run_formats(S, W, ctx, args.formats, cpu=not args.no_cpu)
     if args.halos:
         out["halos"] = run_halos(S, W, ctx, args.halos, cpu=not args.no_cpu)
+    if args.images:
+        out["images"] = run_images(S, W, ctx, args.images, cpu=not args.no_cpu)
 
     if args.precision_form:
         out["precision_form"] = run_precision_form(S, W, ctx)

@@ -24,6 +24,7 @@
 
 #include <cmath>
 #include <cstddef>
+#include <list>
 #include <memory>
 #include <type_traits>
 #include <mutex>
@@ -834,22 +835,66 @@ struct GridMap {
     }
     std::lock_guard<std::mutex> hold(grid_mutex());
     slsim_ctx ctx = slsim_cpp::context();
-    void *yd = nullptr, *Ad = nullptr;
+    // the outputs stay in HBM for find_images
     slsim_cpp::check(slsim_device_alloc(ctx, 16 * n, &yd));
-    slsim_cpp::check(slsim_device_alloc(ctx, 32 * n, &Ad));
-    int rc = slsim_gridmap_shoot(lens->handle(), N1d, center[0], center[1], range, 0.0,
-                                 (double*)yd, (double*)Ad, slsim_cpp::precision(), nullptr);
+    int rc = slsim_device_alloc(ctx, 32 * n, &Ad);
+    if (rc == SLSIM_OK)
+      rc = slsim_gridmap_shoot(lens->handle(), N1d, center[0], center[1], range, 0.0,
+                               (double*)yd, (double*)Ad, slsim_cpp::precision(), nullptr);
     std::vector<double> y(2 * n), A(4 * n);
     if (rc == SLSIM_OK) rc = slsim_memcpy_d2h(ctx, y.data(), yd, 16 * n);
     if (rc == SLSIM_OK) rc = slsim_memcpy_d2h(ctx, A.data(), Ad, 32 * n);
-    slsim_device_free(ctx, yd);
-    slsim_device_free(ctx, Ad);
+    if (rc != SLSIM_OK) release_device();
     slsim_cpp::check(rc);
     for (size_t i = 0; i < n; ++i) {
       s_points[i].x[0] = y[2 * i];
       s_points[i].x[1] = y[2 * i + 1];
       for (int c = 0; c < 4; ++c) i_points[i].A.a[c] = A[4 * i + c];
       s_points[i].A = i_points[i].A;
+    }
+  }
+
+  GridMap(const GridMap&) = delete;
+  GridMap& operator=(const GridMap&) = delete;
+  ~GridMap() { release_device(); }
+
+  // include/gridmap.h:182-205
+  struct Triangle {
+    Triangle(size_t i, size_t j, size_t k) { index[0] = i; index[1] = j; index[2] = k; }
+    Triangle() { index[0] = index[1] = index[2] = 0; }
+    size_t operator[](int i) const { return index[i]; }
+    size_t index[3];
+  };
+
+  // gridmap.cpp:927-963, 1326-1348: the images of each source position, as
+  // rays (x the image, y the source, A the mean over the image's triangle)
+  std::list<RAY> find_images(std::vector<Point_2d>& ys, std::vector<int>& multiplicity) const {
+    std::vector<double> r;
+    std::vector<int64_t> t;
+    images(ys, multiplicity, r, t);
+    std::list<RAY> rays;
+    for (size_t i = 0; i < r.size() / 8; ++i) {
+      rays.emplace_back();
+      RAY& ray = rays.back();
+      ray.x = Point_2d(r[8 * i], r[8 * i + 1]);
+      ray.y = Point_2d(r[8 * i + 2], r[8 * i + 3]);
+      for (int k = 0; k < 4; ++k) ray.A[k] = r[8 * i + 4 + k];
+    }
+    return rays;
+  }
+  // gridmap.cpp:1034-1098
+  void find_images(Point_2d y, std::vector<Point_2d>& image_points,
+                   std::vector<Triangle>& triangles) const {
+    std::vector<Point_2d> ys(1, y);
+    std::vector<int> mult;
+    std::vector<double> r;
+    std::vector<int64_t> t;
+    images(ys, mult, r, t);
+    image_points.clear();
+    triangles.clear();
+    for (size_t i = 0; i < r.size() / 8; ++i) {
+      image_points.emplace_back(r[8 * i], r[8 * i + 1]);
+      triangles.emplace_back((size_t)t[3 * i], (size_t)t[3 * i + 1], (size_t)t[3 * i + 2]);
     }
   }
 
@@ -905,9 +950,39 @@ struct GridMap {
     static std::mutex m;
     return m;
   }
+  void release_device() {
+    slsim_ctx ctx = slsim_cpp::context();
+    if (yd) slsim_device_free(ctx, yd);
+    if (Ad) slsim_device_free(ctx, Ad);
+    yd = Ad = nullptr;
+  }
+  void images(std::vector<Point_2d>& ys, std::vector<int>& multiplicity, std::vector<double>& r,
+              std::vector<int64_t>& t) const {
+    const size_t ns = ys.size();
+    std::vector<double> yv(2 * ns);
+    for (size_t s = 0; s < ns; ++s) { yv[2 * s] = ys[s][0]; yv[2 * s + 1] = ys[s][1]; }
+    std::vector<int32_t> m(ns);
+    for (size_t cap : {size_t(64), size_t(4096)}) {
+      r.assign(8 * std::max<size_t>(ns * cap, 1), 0.0);
+      t.assign(3 * std::max<size_t>(ns * cap, 1), 0);
+      size_t n = 0;
+      const int rc = slsim_gridmap_find_images(slsim_cpp::context(), Ngrid_init, center[0],
+                                               center[1], x_range, (const double*)yd,
+                                               (const double*)Ad, ns, yv.data(), cap, m.data(),
+                                               r.data(), t.data(), &n);
+      if (rc == SLSIM_ERR_ARG && cap < 4096) continue;
+      slsim_cpp::check(rc);
+      r.resize(8 * n);
+      t.resize(3 * n);
+      break;
+    }
+    multiplicity.assign(m.begin(), m.end());
+  }
   int Ngrid_init, Ngrid_init2;
   double x_range;
   double center[2];
   std::vector<Point> i_points;
   std::vector<Point> s_points;
+  void* yd = nullptr;
+  void* Ad = nullptr;
 };

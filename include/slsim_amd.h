@@ -190,6 +190,24 @@ int slsim_plane_direct_sum(slsim_plane plane, size_t n, const double* rays, doub
 int slsim_plane_force(slsim_plane plane, size_t n, const double* rays, double* out,
                       int precision);
 
+/* GridMap::find_images(std::vector<Point_2d>& ys, std::vector<int>&
+ * multiplicity) and find_images(Point_2d y, image_points, triangles)
+ * (TreeCode_link/gridmap.cpp:927-1032, 1326-1348) on a grid shot by
+ * slsim_gridmap_shoot with the same N, centre and range (y_dev, A_dev its
+ * outputs): every grid cell is split into the triangles (k, k+N+1, k+1) and
+ * (k, k+N+1, k+N); a triangle whose source-plane image contains ys[s] (edges
+ * and a vertex counted by the reference's rules) gives an image at its
+ * centroid (edge midpoint, vertex) with A the mean of its corners' A.
+ * Outputs, source by source and in the reference's order within a source
+ * (by cell, first triangle first): rays[8 i] = (x0, x1, y0, y1, A0..A3),
+ * triangles[3 i] = the corners' grid indices, multiplicity[s], *nimages.
+ * rays and triangles hold nsrc * max_per_source entries (max_per_source in
+ * [1, 4096]); SLSIM_ERR_ARG if a source has more images.  N < 65536. */
+int slsim_gridmap_find_images(slsim_ctx ctx, size_t N, double c0, double c1, double range,
+                              const double* y_dev, const double* A_dev, size_t nsrc,
+                              const double* ys, size_t max_per_source, int32_t* multiplicity,
+                              double* rays, int64_t* triangles, size_t* nimages);
+
 /* --------------------------------------------------- batched GridMaps */
 /* Batching and serving have no counterpart in the reference, which shoots
  * one GridMap at a time under a process-wide mutex (gridmap.cpp:15,116-119);

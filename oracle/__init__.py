@@ -57,6 +57,8 @@ _sigs = {
     "orc_nfw_function": (c_d, [c_int, c_d]),
     "orc_nfw_interpolate": (c_d, [c_int, c_d]),
     "orc_lens_add_halotree": (None, [c_vp, c_vp, c_f]),
+    "orc_find_images": (c_sz, [c_sz, P(c_d), P(c_d), P(c_d), P(c_d), c_sz, P(ctypes.c_int32),
+                               P(c_d), P(ctypes.c_int64), c_sz]),
     "orc_lens_nplanes": (c_int, [c_vp]),
     "orc_lens_tables": (None, [c_vp, P(c_d), P(c_d), P(c_d), P(c_d)]),
     "orc_lens_shoot": (None, [c_vp, c_sz, P(c_d), P(c_d), P(c_d), P(c_d), c_int, c_vp]),
@@ -250,3 +252,24 @@ def gridmap_positions(N, center, range_):
     x = np.zeros((N * N, 2))
     _lib.orc_gridmap_positions(N, center[0], center[1], range_, _p(x))
     return x
+
+
+def find_images(N, x, y, A, ys, cap=None):
+    """GridMap::find_images over a grid (x, y: (N*N, 2), A: (N*N, 4)) for
+    source positions ys (n, 2): (rays (m, 8) = x, y, A; multiplicity;
+    triangles (m, 3)), sources in order."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    ys = np.ascontiguousarray(ys, dtype=np.float64).reshape(-1, 2)
+    ns = ys.shape[0]
+    cap = cap or 64 * max(ns, 1)
+    mult = np.zeros(ns, np.int32)
+    rays = np.zeros((cap, 8))
+    tris = np.zeros((cap, 3), np.int64)
+    n = _lib.orc_find_images(N, _p(x), _p(y), _p(A), _p(ys), ns,
+                             mult.ctypes.data_as(P(ctypes.c_int32)), _p(rays),
+                             tris.ctypes.data_as(P(ctypes.c_int64)), cap)
+    if n > cap:
+        raise ValueError("more images than cap")
+    return rays[:n], mult, tris[:n]

@@ -411,6 +411,27 @@ class GridMap:
             raise ValueError("GridMap::writePixelMap() does not work for the input LensingVariable")
         return r.reshape(self.N, self.N)
 
+    def find_images(self, ys, max_per_source=32):
+        """GridMap::find_images(ys, multiplicity) (gridmap.cpp:927-1032,
+        1326-1348) on the device over this grid's outputs: (rays (n, 8) =
+        image x, source y, A the mean over the image's triangle;
+        multiplicity per source; triangles (n, 3) grid indices), sources in
+        order, each source's images in the reference's order."""
+        ys = np.ascontiguousarray(ys, dtype=np.float64).reshape(-1, 2)
+        ns = ys.shape[0]
+        cap = int(max_per_source)
+        mult = np.zeros(ns, np.int32)
+        rays = np.zeros((max(ns * cap, 1), 8))
+        tris = np.zeros((max(ns * cap, 1), 3), np.int64)
+        n = ctypes.c_size_t()
+        _check(lib.slsim_gridmap_find_images(
+            self.ctx.h, ctypes.c_size_t(self.N), self.center[0], self.center[1], self.range,
+            self.y_dev, self.A_dev, ctypes.c_size_t(ns), _ptr(ys, ctypes.c_double),
+            ctypes.c_size_t(cap), mult.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            _ptr(rays, ctypes.c_double), tris.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            ctypes.byref(n)))
+        return rays[:n.value], mult, tris[:n.value]
+
     def writeFits(self, var, filename, dtype=np.float64):
         """GridMap::writeFits<T>(lensvar, filename) (gridmap.h:524-531): the
         map of writePixelMap written by PixelMap::printFITS, resolution

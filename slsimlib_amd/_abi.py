@@ -165,6 +165,10 @@ SIGNATURES = {
                                           c_size_t, c_int, P(c_float)]),
     "slsim_smoothing_sizes_f32": (c_int, [slsim_ctx, P(c_float), P(c_float), P(c_float),
                                           c_size_t, c_int, P(c_float)]),
+    "slsim_gridmap_find_images": (c_int, [slsim_ctx, c_size_t, c_double, c_double, c_double, c_vp,
+                                          c_vp, c_size_t, P(c_double), c_size_t,
+                                          P(ctypes.c_int32), P(c_double), P(ctypes.c_int64),
+                                          P(c_size_t)]),
     "slsim_halos_create_nfw": (c_int, [slsim_ctx, P(c_double), P(c_double), P(c_float),
                                        P(c_float), P(c_float), c_size_t, c_float, c_float,
                                        P(slsim_plane)]),

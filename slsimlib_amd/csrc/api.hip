@@ -34,24 +34,8 @@ int arg_error(const char* msg) {
   return SLSIM_ERR_ARG;
 }
 
-// Device scratch for one shoot call (ray state, jobs, permutations).
-struct Scratch {
-  slsim_ctx ctx;
-  std::vector<void*> ptrs;
-  explicit Scratch(slsim_ctx c) : ctx(c) {}
-  ~Scratch() {
-    for (void* p : ptrs) ctx->scratch_put(p);
-  }
-  template <typename T>
-  int get(T** p, size_t count) {
-    void* v = nullptr;
-    int rc = ctx->scratch_get(&v, std::max<size_t>(count, 1) * sizeof(T));
-    if (rc) return rc;
-    ptrs.push_back(v);
-    *p = static_cast<T*>(v);
-    return SLSIM_OK;
-  }
-};
+
+
 
 int alloc_state(Scratch& s, int64_t nslots, RayState* st) {
   char* d;
@@ -287,6 +271,23 @@ int slsim_halos_create_nfw(slsim_ctx ctx, const double* x, const double* y, cons
   }
   *out = p.release();
   return SLSIM_OK;
+}
+
+int slsim_gridmap_find_images(slsim_ctx ctx, size_t N, double c0, double c1, double range,
+                              const double* y_dev, const double* A_dev, size_t nsrc,
+                              const double* ys, size_t max_per_source, int32_t* multiplicity,
+                              double* rays, int64_t* triangles, size_t* nimages) {
+  if (!ctx || !nimages || (nsrc && (!ys || !multiplicity || !rays || !triangles)))
+    return arg_error("null argument");
+  if (N >= 2 && (!y_dev || !A_dev)) return arg_error("null grid outputs");
+  if (N >= 65536) return arg_error("find_images: N must be below 65536");
+  if (max_per_source == 0 || max_per_source > 4096)
+    return arg_error("find_images: max_per_source must be in [1, 4096]");
+  if (nsrc > ((size_t)1 << 31) / max_per_source)
+    return arg_error("find_images: nsrc * max_per_source must stay below 2^31");
+  SLSIM_HIP_CHECK(hipSetDevice(ctx->device));
+  return find_images(ctx, N, c0, c1, range, y_dev, A_dev, nsrc, ys, max_per_source, multiplicity,
+                     rays, triangles, nimages);
 }
 
 int slsim_plane_destroy(slsim_plane plane) {

@@ -1,0 +1,215 @@
+// images.hip -- GridMap::find_images on the device.
+//
+// GridMap::find_images (TreeCode_link/gridmap.cpp:927-1032, 1326-1348) finds
+// the images of source positions ys on a shot grid: every grid cell is split
+// into the triangles (k, k+N+1, k+1) and (k, k+N+1, k+N), and a triangle whose
+// source-plane image contains ys (sign tests of three cross products, edges
+// and a vertex included by the reference's rules) yields an image at its
+// centroid (edge midpoint, vertex) with A the mean of its three A matrices.
+// The device tests every (cell, source) pair: one thread per cell keeps its
+// four source-plane corners in registers and runs through the sources staged
+// in LDS; hits are appended per source, then ordered on the host by (cell,
+// triangle), which is the reference's order (rows, then cells, first
+// triangle first), and turned into rays.  The cross products are computed
+// without FMA contraction, as the reference's, so every sign decision and
+// every image is the reference's to the bit.
+
+#include <algorithm>
+#include <vector>
+
+#include "slsim_internal.h"
+
+namespace slsim {
+
+namespace {
+
+constexpr int kSrcTile = 256;
+
+// hit: cell k, code = triangle (bit 0) | case << 1 with case 0 inside,
+// 1..3 on the triangle's first/second/third edge, 4 on its first vertex
+struct ImageHit {
+  uint32_t cell;
+  uint32_t code;
+};
+
+__device__ inline int sgn(double v) { return (0.0 < v) - (v < 0.0); }
+
+// (y - a) ^ (b - a): Point_2d's cross product (include/point.h:129-131)
+__device__ inline int side(double y0, double y1, double a0, double a1, double b0, double b1) {
+#pragma clang fp contract(off)
+  const double u0 = y0 - a0, u1 = y1 - a1;
+  const double v0 = b0 - a0, v1 = b1 - a1;
+  return sgn(u0 * v1 - u1 * v0);
+}
+
+__global__ __launch_bounds__(256) void k_find_images(const double* __restrict__ y, int64_t N,
+                                                     const double* __restrict__ ys, int nsrc,
+                                                     uint32_t cap, uint32_t* __restrict__ count,
+                                                     ImageHit* __restrict__ hits) {
+  __shared__ double sy[2 * kSrcTile];
+  const int64_t ncell = (N - 1) * (N - 1);
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool valid = c < ncell;
+  int64_t k = 0;
+  double p0x = 0, p0y = 0, p1x = 0, p1y = 0, p2x = 0, p2y = 0, p3x = 0, p3y = 0;
+  if (valid) {
+    const int64_t j = c / (N - 1), i = c % (N - 1);
+    k = j * N + i;
+    const int64_t k1 = k + N + 1, k2 = k + 1, k3 = k + N;
+    p0x = y[2 * k];  p0y = y[2 * k + 1];
+    p1x = y[2 * k1]; p1y = y[2 * k1 + 1];
+    p2x = y[2 * k2]; p2y = y[2 * k2 + 1];
+    p3x = y[2 * k3]; p3y = y[2 * k3 + 1];
+  }
+  for (int t0 = 0; t0 < nsrc; t0 += kSrcTile) {
+    const int nt = min(kSrcTile, nsrc - t0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < 2 * nt; q += blockDim.x) sy[q] = ys[2 * t0 + q];
+    __syncthreads();
+    if (!valid) continue;
+    for (int s = 0; s < nt; ++s) {
+      const double q0 = sy[2 * s], q1 = sy[2 * s + 1];
+      // triangle (k, k1, k2)
+      const int sig1 = side(q0, q1, p0x, p0y, p1x, p1y);
+      int sig2 = side(q0, q1, p1x, p1y, p2x, p2y);
+      int sig3 = side(q0, q1, p2x, p2y, p0x, p0y);
+      int sum = sig1 + sig2 + sig3;
+      int code = -1;
+      if (sum == 3 || sum == -3) code = 0;
+      else if (sum == 2) code = sig1 == 0 ? 1 : (sig2 == 0 ? 2 : 3);
+      else if (sum != -2 && sig1 == 0 && sig3 == 0) code = 4;
+      if (code >= 0) {
+        const uint32_t at = atomicAdd(&count[t0 + s], 1u);
+        if (at < cap) hits[(int64_t)(t0 + s) * cap + at] = {(uint32_t)k, (uint32_t)(code << 1)};
+      }
+      // triangle (k, k1, k3); the edge k -> k1 is shared (same sig1)
+      sig2 = side(q0, q1, p1x, p1y, p3x, p3y);
+      sig3 = side(q0, q1, p3x, p3y, p0x, p0y);
+      sum = sig1 + sig2 + sig3;
+      code = -1;
+      if (sum == 3 || sum == -3) code = 0;
+      else if (sum == 2) code = sig1 == 0 ? 1 : (sig2 == 0 ? 2 : 3);
+      if (code >= 0) {
+        const uint32_t at = atomicAdd(&count[t0 + s], 1u);
+        if (at < cap) hits[(int64_t)(t0 + s) * cap + at] = {(uint32_t)k, (uint32_t)((code << 1) | 1)};
+      }
+    }
+  }
+}
+
+// The rays of the listed hits: x from the triangle's image-plane corners, y =
+// the source, A = ((A_a + A_b) + A_c) / 3.  list[h] = (source, hit slot).
+__global__ void k_image_rays(const ImageHit* __restrict__ hits, const uint32_t* __restrict__ list,
+                             int64_t nhit, uint32_t cap, const double* __restrict__ A, int64_t N,
+                             double c0, double c1, double range, const double* __restrict__ ys,
+                             double* __restrict__ rays, int64_t* __restrict__ tri) {
+#pragma clang fp contract(off)
+  const int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (h >= nhit) return;
+  const uint32_t src = list[2 * h], slot = list[2 * h + 1];
+  const ImageHit e = hits[(int64_t)src * cap + slot];
+  const int64_t k = e.cell;
+  const int t = e.code & 1, cs = (int)(e.code >> 1);
+  const int64_t v[3] = {k, k + N + 1, t == 0 ? k + 1 : k + N};
+  double x[3][2];
+  for (int q = 0; q < 3; ++q) grid_position(v[q], N, c0, c1, range, &x[q][0], &x[q][1]);
+  double px, py;
+  if (cs == 0) {
+    px = (x[0][0] + x[1][0] + x[2][0]) / 3;
+    py = (x[0][1] + x[1][1] + x[2][1]) / 3;
+  } else if (cs == 4) {
+    px = x[0][0];
+    py = x[0][1];
+  } else {
+    const int a = cs == 1 ? 0 : (cs == 2 ? 1 : 2), b = cs == 1 ? 1 : (cs == 2 ? 2 : 0);
+    px = (x[a][0] + x[b][0]) / 2;
+    py = (x[a][1] + x[b][1]) / 2;
+  }
+  double* r = rays + 8 * h;
+  r[0] = px;
+  r[1] = py;
+  r[2] = ys[2 * src];
+  r[3] = ys[2 * src + 1];
+  for (int m = 0; m < 4; ++m) {
+    double s = 0;
+    for (int q = 0; q < 3; ++q) s = s + A[4 * v[q] + m];
+    r[4 + m] = s / 3;
+  }
+  for (int q = 0; q < 3; ++q) tri[3 * h + q] = v[q];
+}
+
+}  // namespace
+
+int find_images(slsim_ctx c, size_t N, double c0, double c1, double range, const double* y_dev,
+                const double* A_dev, size_t nsrc, const double* ys, size_t cap,
+                int32_t* multiplicity, double* rays, int64_t* triangles, size_t* nimages) {
+  *nimages = 0;
+  for (size_t s = 0; s < nsrc; ++s) multiplicity[s] = 0;
+  if (N < 2 || nsrc == 0) return SLSIM_OK;
+  hipStream_t st = c->stream;
+  Scratch sc(c);
+  double* dys;
+  uint32_t* dcount;
+  ImageHit* dhits;
+  int rc;
+  if ((rc = sc.get(&dys, 2 * nsrc)) || (rc = sc.get(&dcount, nsrc)) ||
+      (rc = sc.get(&dhits, nsrc * cap)))
+    return rc;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dys, ys, 16 * nsrc, hipMemcpyHostToDevice, st));
+  launch_zero_u32(st, dcount, (int)nsrc);
+  const int64_t ncell = (int64_t)(N - 1) * (int64_t)(N - 1);
+  k_find_images<<<(unsigned)((ncell + 255) / 256), 256, 0, st>>>(y_dev, (int64_t)N, dys,
+                                                                  (int)nsrc, (uint32_t)cap,
+                                                                  dcount, dhits);
+  SLSIM_HIP_CHECK(hipGetLastError());
+  std::vector<uint32_t> cnt(nsrc);
+  SLSIM_HIP_CHECK(hipMemcpyAsync(cnt.data(), dcount, 4 * nsrc, hipMemcpyDeviceToHost, st));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(st));
+  for (size_t s = 0; s < nsrc; ++s)
+    if (cnt[s] > cap) {
+      set_error("find_images: source " + std::to_string(s) + " has " + std::to_string(cnt[s]) +
+                " images, more than max_per_source = " + std::to_string(cap));
+      return SLSIM_ERR_ARG;
+    }
+  std::vector<ImageHit> hits(nsrc * cap);
+  SLSIM_HIP_CHECK(hipMemcpyAsync(hits.data(), dhits, sizeof(ImageHit) * nsrc * cap,
+                                 hipMemcpyDeviceToHost, st));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(st));
+  // the reference's order: per source, by cell, first triangle first
+  std::vector<uint32_t> list;
+  for (size_t s = 0; s < nsrc; ++s) {
+    std::vector<uint32_t> ord(cnt[s]);
+    for (uint32_t q = 0; q < cnt[s]; ++q) ord[q] = q;
+    const ImageHit* hs = hits.data() + s * cap;
+    std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) {
+      const uint64_t ka = ((uint64_t)hs[a].cell << 1) | (hs[a].code & 1);
+      const uint64_t kb = ((uint64_t)hs[b].cell << 1) | (hs[b].code & 1);
+      return ka < kb;
+    });
+    for (uint32_t q : ord) {
+      list.push_back((uint32_t)s);
+      list.push_back(q);
+    }
+    multiplicity[s] = (int32_t)cnt[s];
+  }
+  const int64_t nhit = (int64_t)list.size() / 2;
+  *nimages = (size_t)nhit;
+  if (nhit == 0) return SLSIM_OK;
+  uint32_t* dlist;
+  double* drays;
+  int64_t* dtri;
+  if ((rc = sc.get(&dlist, list.size())) || (rc = sc.get(&drays, 8 * nhit)) ||
+      (rc = sc.get(&dtri, 3 * nhit)))
+    return rc;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dlist, list.data(), 4 * list.size(), hipMemcpyHostToDevice, st));
+  k_image_rays<<<(unsigned)((nhit + 255) / 256), 256, 0, st>>>(dhits, dlist, nhit, (uint32_t)cap,
+                                                               A_dev, (int64_t)N, c0, c1, range,
+                                                               dys, drays, dtri);
+  SLSIM_HIP_CHECK(hipGetLastError());
+  SLSIM_HIP_CHECK(hipMemcpyAsync(rays, drays, 64 * nhit, hipMemcpyDeviceToHost, st));
+  SLSIM_HIP_CHECK(hipMemcpyAsync(triangles, dtri, 24 * nhit, hipMemcpyDeviceToHost, st));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(st));
+  return SLSIM_OK;
+}
+
+}  // namespace slsim

@@ -1076,16 +1076,6 @@ __device__ inline void init_slot(const RayState& st, int64_t slot, bool valid, d
   st.bbl[slot] = bbl;
 }
 
-__device__ inline void grid_position(int64_t i, int64_t N, double c0, double c1, double range,
-                                     double* x0, double* x1) {
-#pragma clang fp contract(off)
-  *x0 = c0;
-  *x1 = c1;
-  if (N > 1) {
-    *x0 = c0 + range * (1.0 * (double)(i % N) / (double)(N - 1) - 0.5);
-    *x1 = c1 + range * (1.0 * (double)(i / N) / (double)(N - 1) - 0.5);
-  }
-}
 
 __device__ inline void finalize_slot(const RayState& st, int64_t slot, int64_t i,
                                      double* __restrict__ y, double* __restrict__ A) {

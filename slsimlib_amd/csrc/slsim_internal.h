@@ -198,6 +198,21 @@ void set_error(const std::string& msg);
     }                                                                           \
   } while (0)
 
+
+
+// GridMap ray i of an N x N grid (TreeCode_link/gridmap.cpp:293-321,
+// utilities.cpp:149-158), computed without contraction as the reference.
+__device__ inline void grid_position(int64_t i, int64_t N, double c0, double c1, double range,
+                                     double* x0, double* x1) {
+#pragma clang fp contract(off)
+  *x0 = c0;
+  *x1 = c1;
+  if (N > 1) {
+    *x0 = c0 + range * (1.0 * (double)(i % N) / (double)(N - 1) - 0.5);
+    *x1 = c1 + range * (1.0 * (double)(i / N) / (double)(N - 1) - 0.5);
+  }
+}
+
 }  // namespace slsim
 
 struct slsim_ctx_s {
@@ -269,6 +284,25 @@ struct slsim_lens_s {
 };
 
 namespace slsim {
+
+// Device scratch of one call, returned to the context's cache on scope exit.
+struct Scratch {
+  slsim_ctx ctx;
+  std::vector<void*> ptrs;
+  explicit Scratch(slsim_ctx c) : ctx(c) {}
+  ~Scratch() {
+    for (void* p : ptrs) ctx->scratch_put(p);
+  }
+  template <typename T>
+  int get(T** p, size_t count) {
+    void* v = nullptr;
+    int rc = ctx->scratch_get(&v, std::max<size_t>(count, 1) * sizeof(T));
+    if (rc) return rc;
+    ptrs.push_back(v);
+    *p = static_cast<T*>(v);
+    return SLSIM_OK;
+  }
+};
 struct FlatLCDM;
 // kernel launchers (shoot.hip)
 // halos: some job of the launch has halo-of-halos trees (selects the kernels
@@ -315,4 +349,9 @@ int export_nodes(slsim_plane pl, double* dbl, int64_t* ints);
 void uniform_source(const slsim_lens_s* L, double zs, int32_t* jmax, double* aal, double* bbl);
 void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint32_t* counters,
                 std::vector<StepJob>& steps);
+// GridMap::find_images over a shot grid (images.hip)
+int find_images(slsim_ctx c, size_t N, double c0, double c1, double range, const double* y_dev,
+                const double* A_dev, size_t nsrc, const double* ys, size_t cap,
+                int32_t* multiplicity, double* rays, int64_t* triangles, size_t* nimages);
+
 }  // namespace slsim

@@ -1,7 +1,8 @@
 // Reference-style user program against the drop-in header for a light-cone
 // that mixes a particle halo (z = 0.5) and a LensHaloHalos of 200 NFW halos
 // placed with setTheta (z = 0.3): shoots a 64 x 64 GridMap and writes x, y
-// and A of every ray as raw float64 to argv[1].
+// and A of every ray as raw float64 to argv[1]; with argv[2], the images
+// (GridMap::find_images) of four grid rays' source positions there.
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -51,6 +52,22 @@ int main(int argc, char** argv) {
   if (!f) return 1;
   std::fwrite(out.data(), sizeof(double), out.size(), f);
   std::fclose(f);
+  if (argc > 2) {
+    // GridMap::find_images for the source positions of a few grid rays
+    std::vector<Point_2d> ys;
+    for (size_t i : {size_t(1000), size_t(2080), size_t(2112), size_t(3000)})
+      ys.push_back(grid.source_point(i));
+    std::vector<int> mult;
+    std::list<RAY> imgs = grid.find_images(ys, mult);
+    std::vector<double> w;
+    for (int m : mult) w.push_back(m);
+    for (const RAY& r : imgs)
+      w.insert(w.end(), {r.x[0], r.x[1], r.y[0], r.y[1], r.A[0], r.A[1], r.A[2], r.A[3]});
+    FILE* g = std::fopen(argv[2], "wb");
+    if (!g) return 1;
+    std::fwrite(w.data(), sizeof(double), w.size(), g);
+    std::fclose(g);
+  }
   std::printf("dropin_halos: %zu rays, %d planes\n", out.size() / 8, lens.getNplanes());
   return 0;
 }

@@ -162,7 +162,8 @@ def test_cpp_dropin_halos_program(ctx, tmp_path):
     if not os.path.exists(exe):
         pytest.fail("tests/cpp/dropin_halos is not built (run __graft_entry__.build())")
     out = tmp_path / "halos.bin"
-    r = subprocess.run([exe, str(out)], capture_output=True, text=True, timeout=120)
+    img = tmp_path / "images.bin"
+    r = subprocess.run([exe, str(out), str(img)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     v = np.fromfile(out, dtype=np.float64).reshape(-1, 8)
     # the same light-cone through the Python view: LensHaloNFW positions are
@@ -183,6 +184,11 @@ def test_cpp_dropin_halos_program(ctx, tmp_path):
     lens.moveinMainHalo(S.LensHaloHalos(ctx, hx, hy, m, rs, c, 0.3))
     g = S.GridMap(lens, 64, (0.0, 0.0), 60 * S.arcsecTOradians)
     assert np.array_equal(v[:, 2:4], g.y()) and np.array_equal(v[:, 4:8], g.A())
+    # GridMap::find_images from C++ equals the Python view's
+    w = np.fromfile(img, dtype=np.float64)
+    ys = g.y()[[1000, 2080, 2112, 3000]]
+    rays, mult, _ = g.find_images(ys)
+    assert np.array_equal(w[:4], mult) and np.array_equal(w[4:].reshape(-1, 8), rays)
     L = O.Lens(2.0)
     L.moveinMainHalo(O.Tree(px, py, pm, ph), 0.5)
     L.moveinMainHalo(O.HaloTree(hx, hy, m, rs, c), 0.3)
