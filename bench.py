@@ -21,6 +21,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -109,7 +110,6 @@ def run_serving(S, W, ctx, n_models, n_requests, load, precision, seed=5999, rat
     """Serving form: resident 10-plane x 2^20-particle lenses; Poisson arrivals
     of GridMap requests (lens, zs in [1,4], centre in the 5' field, side from
     {128,...,2048} with p = {.4,.3,.15,.1,.05}, 300"/2048 per pixel)."""
-    import threading
     from concurrent.futures import ThreadPoolExecutor
     t0 = time.perf_counter()
     lenses = []
@@ -339,6 +339,53 @@ def run_images(S, W, ctx, nsrc, cpu=True, N=2048):
     return out
 
 
+def run_small_batches(S, W, ctx, nbatch, cpu=True, seed=606):
+    """Callers that shoot many small ray lists (adaptive Grid refinement,
+    critical-curve searches): nbatch lists of 64-4096 rays (log-uniform)
+    near the 65,536-particle NFW halo's centre, FP64.  One rayshooterInternal
+    call per list in sequence (the reference's pattern), against the same
+    lists submitted by 8 client threads to the server."""
+    lens, _ = W.build_c1_lens(ctx)
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(64), np.log(4096), nbatch)).astype(int)
+    lists = [(rng.random((int(k), 2)) - 0.5) * W.grid_range(30.0) for k in sizes]
+    total = int(sizes.sum())
+    lens.rayshooterInternal(lists[0])
+    t0 = time.perf_counter()
+    for x in lists:
+        lens.rayshooterInternal(x)
+    t_seq = time.perf_counter() - t0
+    srv = S.Server(ctx)
+    srv.wait_rays(srv.submit_rays(lens, lists[0], None, S.SLSIM_FP64), lists[0].shape[0])
+    lat = []
+    lock = threading.Lock()
+
+    def client(c):
+        for i in range(c, nbatch, 8):
+            t = time.perf_counter()
+            rid = srv.submit_rays(lens, lists[i], None, S.SLSIM_FP64)
+            srv.wait_rays(rid, lists[i].shape[0])
+            with lock:
+                lat.append(time.perf_counter() - t)
+
+    t1 = time.perf_counter()
+    th = [threading.Thread(target=client, args=(c,)) for c in range(8)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    t_srv = time.perf_counter() - t1
+    st = srv.stats()
+    srv.close()
+    out = {"batches": nbatch, "rays": total, "sequential_rays_per_s": total / t_seq,
+           "served_rays_per_s": total / t_srv, "served_over_sequential": t_seq / t_srv,
+           "served_p50_ms": 1e3 * float(np.percentile(lat, 50)),
+           "served_p99_ms": 1e3 * float(np.percentile(lat, 99)),
+           "max_active": st["max_active"], "iterations": st["iterations"]}
+    log("small batches:", out)
+    return out
+
+
 def run_precision_form(S, W, ctx, n=1 << 22, nplanes=20, N=4096):
     """Precision form: 20 planes x 4,194,304 float32-position particles,
     opening angle 0.2, 4096^2 rays; FP32C and FP64 trees against the FP64
@@ -406,6 +453,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip FP64, direct-sum, counts")
     ap.add_argument("--smoothing", type=int, default=0, metavar="N",
                     help="also time the smoothing sizes (Nsmooth 64) of N 3-D particles")
+    ap.add_argument("--small-batches", type=int, default=0, metavar="B",
+                    help="also time B small ray lists: sequential calls vs the server")
     ap.add_argument("--images", type=int, default=0, metavar="S",
                     help="also time GridMap::find_images for S sources on a 2048^2 grid")
     ap.add_argument("--halos", type=int, default=0, metavar="N",
@@ -625,6 +674,8 @@ def main():
         out["halos"] = run_halos(S, W, ctx, args.halos, cpu=not args.no_cpu)
     if args.images:
         out["images"] = run_images(S, W, ctx, args.images, cpu=not args.no_cpu)
+    if args.small_batches:
+        out["small_batches"] = run_small_batches(S, W, ctx, args.small_batches)
 
     if args.precision_form:
         out["precision_form"] = run_precision_form(S, W, ctx)

@@ -268,6 +268,16 @@ int slsim_server_submit(slsim_server s, slsim_lens lens, int64_t N, double c0, d
                         double range, double zs, int precision, uint64_t* id);
 /* Blocks until request `id` is done, copies y (2 N^2) and A (4 N^2) to the
  * host buffers (either may be NULL) and releases its device memory. */
+/* Lens::rayshooterInternal(Npoints, Point*) and (Npoints, LinkedPoint*,
+ * source_zs) (FullRange/internal_rayshooter_multi.cpp:53-251) as a served
+ * request: the callers that shoot many small batches (adaptive Grid
+ * refinement, critical curves: TreeCode_link/grid_maintenance.cpp:45-822,
+ * find_crit.cpp:1072,1118) join the continuous batching.  n rays at x (2 per
+ * ray, radians) with per-ray source redshifts zs (n values, or null for the
+ * lens's source), both copied at submit; slsim_server_wait then returns y (2 n)
+ * and A (4 n), bitwise those of slsim_shoot. */
+int slsim_server_submit_rays(slsim_server s, slsim_lens lens, size_t n, const double* x,
+                             const double* zs, int precision, uint64_t* id);
 int slsim_server_wait(slsim_server s, uint64_t id, double* y_host, double* A_host,
                       slsim_request_info* info);
 /* Non-blocking: *done = 1 once request `id` has finished (collect it with

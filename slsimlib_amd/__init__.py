@@ -534,6 +534,29 @@ class Server:
                                        ctypes.byref(rid)))
         return rid.value
 
+    def submit_rays(self, lens, x, zs=None, precision=SLSIM_FP32C):
+        """rayshooterInternal(Npoints, Point*) as a served request: rays at
+        x (n, 2), per-ray source redshifts zs (n,) or the lens's source.
+        Collect with wait_rays(rid, n)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 2)
+        zp = None
+        if zs is not None:
+            zs = np.ascontiguousarray(zs, dtype=np.float64)
+            zp = _ptr(zs, ctypes.c_double)
+        rid = ctypes.c_uint64()
+        _check(lib.slsim_server_submit_rays(self.h, lens.h, ctypes.c_size_t(x.shape[0]),
+                                            _ptr(x, ctypes.c_double), zp, int(precision),
+                                            ctypes.byref(rid)))
+        return rid.value
+
+    def wait_rays(self, rid, n):
+        info = _abi.slsim_request_info()
+        y = np.zeros((n, 2))
+        A = np.zeros((n, 4))
+        _check(lib.slsim_server_wait(self.h, ctypes.c_uint64(rid), _ptr(y, ctypes.c_double),
+                                     _ptr(A, ctypes.c_double), ctypes.byref(info)))
+        return y, A, {k: getattr(info, k) for k, _ in info._fields_}
+
     def poll(self, rid):
         """True once request rid has finished (non-blocking)."""
         done = ctypes.c_int(0)

@@ -151,6 +151,8 @@ SIGNATURES = {
     "slsim_server_submit": (c_int, [slsim_server, slsim_lens, c_i64, c_double, c_double,
                                     c_double, c_double, c_int, P(c_u64)]),
     "slsim_server_poll": (c_int, [slsim_server, c_u64, P(c_int)]),
+    "slsim_server_submit_rays": (c_int, [slsim_server, slsim_lens, c_size_t, P(c_double),
+                                         P(c_double), c_int, P(c_u64)]),
     "slsim_server_wait": (c_int, [slsim_server, c_u64, P(c_double), P(c_double),
                                   P(slsim_request_info)]),
     "slsim_server_get_stats": (c_int, [slsim_server, P(slsim_server_stats)]),

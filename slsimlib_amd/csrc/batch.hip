@@ -19,6 +19,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "cosmo.h"
 #include "slsim_internal.h"
 
 using namespace slsim;
@@ -423,6 +424,11 @@ struct slsim_server_s {
     int64_t N;
     double c0, c1, range, zs;
     int precision;
+    // ray-list request (rayshooterInternal): nrays > 0, positions and
+    // optional per-ray source redshifts copied at submit
+    int64_t nrays = 0;
+    std::vector<double> xh, zsh;
+    double *xd = nullptr, *zsd = nullptr, *zpl = nullptr, *dl = nullptr;
     int64_t tiles_x, nslots;
     size_t bytes;
     void* mem = nullptr;
@@ -478,6 +484,41 @@ int slsim_server_s::admit(Req* r) {
   const size_t sbytes = kStateBytesPerSlot * (size_t)r->nslots;
   char* base = static_cast<char*>(r->mem);
   carve_state(reinterpret_cast<double*>(base), r->nslots, &r->st);
+  if (r->nrays > 0) {
+    // ray list: x, zs, y, A and the lens tables after the state
+    slsim_lens_s* L = r->lens;
+    if ((rc = L->upload_trees())) return rc;
+    const int64_t n = r->nrays;
+    const int nplanes = (int)L->plane_redshifts.size() - 1;
+    double* p = reinterpret_cast<double*>(base + ((sbytes + 255) & ~(size_t)255));
+    r->xd = p;   p += 2 * n;
+    r->y = p;    p += 2 * n;
+    r->A = p;    p += 4 * n;
+    r->zpl = p;  p += L->plane_redshifts.size();
+    r->dl = p;   p += L->Dl.size();
+    r->zsd = r->zsh.empty() ? nullptr : p;
+    SLSIM_HIP_CHECK(hipMemcpyAsync(r->xd, r->xh.data(), 16 * n, hipMemcpyHostToDevice, stream));
+    if (r->zsd)
+      SLSIM_HIP_CHECK(hipMemcpyAsync(r->zsd, r->zsh.data(), 8 * n, hipMemcpyHostToDevice, stream));
+    SLSIM_HIP_CHECK(hipMemcpyAsync(r->zpl, L->plane_redshifts.data(),
+                                   8 * L->plane_redshifts.size(), hipMemcpyHostToDevice, stream));
+    SLSIM_HIP_CHECK(hipMemcpyAsync(r->dl, L->Dl.data(), 8 * L->Dl.size(), hipMemcpyHostToDevice,
+                                   stream));
+    r->jmax = 0;
+    if (nplanes > 0) {
+      if (r->zsd) r->jmax = nplanes;   // per-ray jmax in the state
+      else uniform_source(L, 0.0, &r->jmax, &r->aal, &r->bbl);
+    }
+    make_steps(L, r->st, r->nslots, nullptr, r->steps);
+    r->nbundles = r->nslots / kWave;
+    r->next_bundle = 0;
+    FlatLCDM cosmo{L->Om, L->OL, L->h};
+    launch_init_array(stream, r->st, r->nslots, n, nullptr, r->xd, nplanes > 0 ? r->zsd : nullptr,
+                      r->zsd ? 0 : r->jmax, r->aal, r->bbl, r->zpl, r->dl, nplanes, cosmo);
+    SLSIM_HIP_CHECK(hipGetLastError());
+    // xh and zsh live until the request is collected, after the stream has read them
+    return SLSIM_OK;
+  }
   r->y = reinterpret_cast<double*>(base + ((sbytes + 255) & ~(size_t)255));
   r->A = r->y + 2 * r->N * r->N;
   if ((rc = r->lens->upload_trees())) return rc;
@@ -624,7 +665,9 @@ void slsim_server_s::loop() {
     std::vector<Req*> finished;
     for (Req* r : active) {
       if (rc || r->step >= r->jmax) {
-        if (rc == SLSIM_OK)
+        if (rc == SLSIM_OK && r->nrays > 0)
+          launch_finalize_array(stream, r->st, r->nrays, nullptr, r->y, r->A);
+        else if (rc == SLSIM_OK)
           launch_finalize_grid(stream, r->st, r->nslots, r->N, r->tiles_x, 3, r->y, r->A);
         finished.push_back(r);
       }
@@ -703,6 +746,43 @@ int slsim_server_submit(slsim_server s, slsim_lens lens, int64_t N, double c0, d
   return SLSIM_OK;
 }
 
+int slsim_server_submit_rays(slsim_server s, slsim_lens lens, size_t n, const double* x,
+                             const double* zs, int precision, uint64_t* id) {
+  if (!s || !lens || !id || (n && !x)) return err_arg("null argument");
+  if (lens->ctx != s->ctx) return err_arg("lens of another context");
+  if (n == 0) return err_arg("no rays");
+  if (n >= ((size_t)1 << 31)) return err_arg("too many rays in one request");
+  if (precision < SLSIM_FP64 || precision > SLSIM_DIRECT64) return err_arg("bad precision");
+  auto r = std::make_unique<slsim_server_s::Req>();
+  r->lens = lens;
+  r->N = 0;
+  r->c0 = r->c1 = r->range = 0;
+  r->zs = 0;
+  r->precision = precision;
+  r->nrays = (int64_t)n;
+  r->xh.assign(x, x + 2 * n);
+  if (zs) r->zsh.assign(zs, zs + n);
+  r->tiles_x = 0;
+  r->nslots = (((int64_t)n + kWave - 1) / kWave) * kWave;
+  const size_t tables = lens->plane_redshifts.size() + lens->Dl.size();
+  r->bytes = kStateBytesPerSlot * (size_t)r->nslots + 256 + 8 * (8 * n + (zs ? n : 0) + tables);
+  r->info.t_submit = now_s();
+  r->info.rays = (int64_t)n;
+  std::lock_guard<std::mutex> g(s->mu);
+  ++s->stats.submitted;
+  if (r->bytes > s->budget) {
+    ++s->stats.rejected;
+    set_error("request needs more HBM than the server budget");
+    return SLSIM_ERR_BUDGET;
+  }
+  r->id = s->next_id++;
+  *id = r->id;
+  s->pending.push_back(r.get());
+  s->all[r->id] = std::move(r);
+  s->cv.notify_all();
+  return SLSIM_OK;
+}
+
 int slsim_server_wait(slsim_server s, uint64_t id, double* y_host, double* A_host,
                       slsim_request_info* info) {
   if (!s) return err_arg("null server");
@@ -716,7 +796,7 @@ int slsim_server_wait(slsim_server s, uint64_t id, double* y_host, double* A_hos
   lk.unlock();
   int rc = r->status;
   if (rc == SLSIM_OK) {
-    const size_t n = (size_t)r->N * r->N;
+    const size_t n = r->nrays > 0 ? (size_t)r->nrays : (size_t)r->N * r->N;
     if (y_host && hipMemcpy(y_host, r->y, 16 * n, hipMemcpyDeviceToHost) != hipSuccess) rc = SLSIM_ERR_HIP;
     if (A_host && hipMemcpy(A_host, r->A, 32 * n, hipMemcpyDeviceToHost) != hipSuccess) rc = SLSIM_ERR_HIP;
   }

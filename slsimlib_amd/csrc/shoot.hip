@@ -1161,7 +1161,7 @@ __global__ void k_init_array(RayState st, int64_t nslots, int64_t n, const uint3
   const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (slot >= nslots) return;
   const bool valid = slot < n;
-  const int64_t i = valid ? (int64_t)perm[slot] : 0;
+  const int64_t i = valid ? (perm ? (int64_t)perm[slot] : slot) : 0;   // no perm: ray s in slot s
   const double x0 = valid ? x[2 * i] : 0.0, x1 = valid ? x[2 * i + 1] : 0.0;
   st.th0[slot] = x0; st.th1[slot] = x1;
   st.sa0[slot] = x0; st.sa1[slot] = x1;
@@ -1193,7 +1193,7 @@ __global__ void k_finalize_array(RayState st, int64_t n, const uint32_t* __restr
                                  double* __restrict__ y, double* __restrict__ A) {
   const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (slot >= n) return;
-  const int64_t i = perm[slot];
+  const int64_t i = perm ? (int64_t)perm[slot] : slot;
   y[2 * i] = st.th0[slot];
   y[2 * i + 1] = st.th1[slot];
   A[4 * i] = st.A0[slot];

@@ -157,3 +157,46 @@ def test_cpp_dropin_program(ctx, tmp_path):
     yr, Ar = L.shoot(rays[:, :2], zs=zs)
     assert np.abs(rays[:, 2:4] - yr).max() <= 1e-12 * np.abs(yr).max()
     assert np.abs(rays[:, 4:8] - Ar).max() <= 1e-8
+
+
+def test_server_ray_requests_equal_rayshooter(ctx, lenses):
+    """rayshooterInternal-style ray lists (the many small batches of adaptive
+    refinement) served alongside GridMaps: each equals the lone call bitwise."""
+    rng = np.random.default_rng(17)
+    srv = S.Server(ctx, hbm_budget=8 << 30)
+    jobs = []
+    for k in range(24):
+        lens = [lenses["c1"], lenses["f1"], lenses["f2"]][k % 3]
+        n = int(rng.integers(1, 3000)) if k % 5 else 1
+        scale = 30.0 if k % 3 == 0 else 300.0
+        x = (rng.random((n, 2)) - 0.5) * W.grid_range(scale)
+        zs = rng.uniform(1.0, 4.0, n) if k % 4 == 1 else None
+        prec = S.SLSIM_FP32C if k % 2 else S.SLSIM_FP64
+        jobs.append((lens, x, zs, prec))
+    out = {}
+
+    def client(k):
+        lens, x, zs, prec = jobs[k]
+        rid = srv.submit_rays(lens, x, zs, prec)
+        out[k] = srv.wait_rays(rid, x.shape[0])
+        if k % 6 == 0:                       # GridMap requests in the same mix
+            r = _requests(lenses)[k % 6]
+            gid = srv.submit(r[0], r[1], r[2], r[3], r[4], S.SLSIM_FP32C)
+            out[("g", k)] = srv.wait(gid, r[1])
+
+    th = [threading.Thread(target=client, args=(k,)) for k in range(len(jobs))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for k, (lens, x, zs, prec) in enumerate(jobs):
+        y, A, info = out[k]
+        assert info["status"] == 0 and info["rays"] == x.shape[0]
+        y0, A0 = lens.rayshooterInternal(x, zs, prec)
+        assert np.array_equal(y, y0) and np.array_equal(A, A0), k
+    for k in range(0, len(jobs), 6):
+        r = _requests(lenses)[k % 6]
+        ys, As = _single(r, S.SLSIM_FP32C)
+        y, A, _ = out[("g", k)]
+        assert np.array_equal(y, ys) and np.array_equal(A, As)
+    srv.close()
