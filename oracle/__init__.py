@@ -63,6 +63,15 @@ _sigs = {
     "orc_lens_tables": (None, [c_vp, P(c_d), P(c_d), P(c_d), P(c_d)]),
     "orc_lens_shoot": (None, [c_vp, c_sz, P(c_d), P(c_d), P(c_d), P(c_d), c_int, c_vp]),
     "orc_gridmap_positions": (None, [c_sz, c_d, c_d, c_d, P(c_d)]),
+    "orc_find_crit": (c_sz, [c_int, c_sz, P(c_d), P(c_d), P(c_d), P(ctypes.c_int32), c_d, c_d,
+                             c_sz, c_sz, P(ctypes.c_int64), P(c_d), P(ctypes.c_int32),
+                             P(ctypes.c_int32), P(c_sz), P(c_sz), P(c_int)]),
+    "orc_find_boundaries": (c_sz, [P(ctypes.c_uint8), c_sz, c_sz, c_sz, c_sz, P(ctypes.c_int64),
+                                   P(c_d), P(ctypes.c_int32), P(c_sz)]),
+    "orc_find_islands": (c_sz, [P(ctypes.c_uint8), c_sz, c_sz, c_sz, c_sz, P(ctypes.c_int64),
+                                P(ctypes.c_int64), P(c_sz)]),
+    "orc_convex_hull": (c_sz, [P(c_d), c_sz, P(ctypes.c_int64)]),
+    "orc_in_curve": (None, [P(c_d), c_sz, P(c_d), c_sz, P(ctypes.c_int32)]),
 }
 for _n, (_r, _a) in _sigs.items():
     getattr(_lib, _n).restype = _r
@@ -273,3 +282,105 @@ def find_images(N, x, y, A, ys, cap=None):
     if n > cap:
         raise ValueError("more images than cap")
     return rays[:n], mult, tris[:n]
+
+
+# ------------------------------------------------------- critical curves
+CRIT_TANGENTIAL, CRIT_RADIAL, CRIT_PSEUDO = 1, 2, 3
+
+
+def _split(off, pts, nc):
+    return [pts[off[i]:off[i + 1]].copy() for i in range(nc)]
+
+
+def find_boundaries(bitmap):
+    """Utilities::find_boundaries(bitmap (ny, nx) bool): (curves in pixel
+    units, hits_edge)."""
+    bm = np.ascontiguousarray(bitmap, dtype=np.uint8)
+    ny, nx = bm.shape
+    cc, cp = 16, 4 * bm.size + 16
+    while True:
+        off = np.zeros(cc + 1, np.int64)
+        pts = np.zeros((cp, 2))
+        hits = np.zeros(cc, np.int32)
+        npt = c_sz()
+        nc = _lib.orc_find_boundaries(_p(bm, ctypes.c_uint8), bm.size, nx, cc, cp,
+                                      _p(off, ctypes.c_int64), _p(pts), _p(hits, ctypes.c_int32),
+                                      ctypes.byref(npt))
+        if nc <= cc:
+            return _split(off, pts, nc), hits[:nc].astype(bool)
+        cc = nc
+
+
+def find_islands(bitmap):
+    """Utilities::find_islands(bitmap (ny, nx) bool): pixel index lists."""
+    bm = np.ascontiguousarray(bitmap, dtype=np.uint8)
+    ny, nx = bm.shape
+    ci = 16
+    while True:
+        off = np.zeros(ci + 1, np.int64)
+        pix = np.zeros(bm.size + 1, np.int64)
+        npx = c_sz()
+        ni = _lib.orc_find_islands(_p(bm, ctypes.c_uint8), bm.size, nx, ci, bm.size + 1,
+                                   _p(off, ctypes.c_int64), _p(pix, ctypes.c_int64),
+                                   ctypes.byref(npx))
+        if ni <= ci:
+            return _split(off, pix, ni)
+        ci = ni
+
+
+def convex_hull(points):
+    """Utilities::convex_hull(P, hull): indices of the hull, counter-clockwise."""
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    h = np.zeros(2 * len(p) + 1, np.int64)
+    n = _lib.orc_convex_hull(_p(p), len(p), _p(h, ctypes.c_int64))
+    return h[:n]
+
+
+def in_curve(poly, points):
+    """Utilities::inCurve(x, H) for each of the points."""
+    h = np.ascontiguousarray(poly, dtype=np.float64).reshape(-1, 2)
+    q = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros(len(q), np.int32)
+    _lib.orc_in_curve(_p(h), len(h), _p(q), len(q), _p(out, ctypes.c_int32))
+    return out.astype(bool)
+
+
+def _crit(mode, N, x, y, A, mult, range_, invmag):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y if y is not None else x, dtype=np.float64)
+    A = np.ascontiguousarray(A if A is not None else np.zeros((N * N, 4)), dtype=np.float64)
+    m = np.ascontiguousarray(mult if mult is not None else np.zeros(N * N), dtype=np.int32)
+    cc, cp = 16, 4 * N * N + 16
+    while True:
+        off = np.zeros(cc + 1, np.int64)
+        pts = np.zeros((cp, 2))
+        types = np.zeros(cc, np.int32)
+        hits = np.zeros(cc, np.int32)
+        nh, npt, rep = c_sz(), c_sz(), c_int()
+        nc = _lib.orc_find_crit(mode, N, _p(x), _p(y), _p(A), _p(m, ctypes.c_int32), range_,
+                                invmag, cc, cp, _p(off, ctypes.c_int64), _p(pts),
+                                _p(types, ctypes.c_int32), _p(hits, ctypes.c_int32),
+                                ctypes.byref(nh), ctypes.byref(npt), ctypes.byref(rep))
+        if nc <= cc and npt.value <= cp:
+            return (_split(off, pts, nc), hits[:nh.value].astype(bool), types[:nc],
+                    bool(rep.value))
+        cc, cp = max(cc, nc), max(cp, npt.value)
+
+
+def find_crit(N, x, y, A, range_):
+    """GridMap::find_crit over a grid (x, y (N*N, 2), A (N*N, 4)): (curves,
+    hits_boundary, crit types, whether hits_boundary was replaced)."""
+    return _crit(0, N, x, y, A, None, range_, 0.0)
+
+
+def find_magnification_contour(N, x, A, range_, invmag):
+    """GridMap::find_magnification_contour: (curves, hits_boundary, -, replaced)."""
+    return _crit(1, N, x, None, A, None, range_, invmag)
+
+
+def find_boundaries_of_caustics(N, x, y, A):
+    """GridMap::find_boundaries_of_caustics: the boundaries (pixel units) of
+    the grid positions with more than one image, and hits_edge."""
+    _, mult, _ = find_images(N, x, y, A, x, cap=64 * N * N)
+    c, h, _, _ = _crit(2, N, x, y, A, mult, 0.0, 0.0)
+    return c, h, mult

@@ -1,0 +1,222 @@
+"""Pins the oracle's restatement of GridMap's critical-curve routines
+(find_crit, find_magnification_contour, find_boundaries_of_caustics and the
+Utilities contour walk, island fill, convex hull and inCurve under them)
+against what fixes them independently: scipy's connected-component labelling
+and convex hull, matplotlib's point-in-polygon test, the marching-squares
+invariants, and the closed-form critical curves of the singular and cored
+isothermal lenses.  CPU only.
+"""
+import numpy as np
+import pytest
+from scipy import ndimage
+from scipy.spatial import ConvexHull
+
+import oracle as O
+
+
+def _blobs(n, seed, k=6, holes=True):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:n, 0:n]
+    bm = np.zeros((n, n), bool)
+    for _ in range(k):
+        cx, cy = rng.uniform(3, n - 4, 2)
+        a, b = rng.uniform(1.5, n / 6, 2)
+        bm |= ((xx - cx) / a) ** 2 + ((yy - cy) / b) ** 2 < 1
+    if holes:
+        bm &= ~(((xx - n / 2) / 2.2) ** 2 + ((yy - n / 2) / 2.2) ** 2 < 1)
+    return bm
+
+
+def _padded(bm):
+    p = bm.copy()
+    p[0, :] = p[-1, :] = p[:, 0] = p[:, -1] = False
+    return p
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_boundaries_marching_squares_invariants(seed):
+    bm = _blobs(64, seed)
+    curves, hits = O.find_boundaries(bm)
+    p = _padded(bm)
+    # contours = 8-connected components of the inside + 4-connected holes
+    _, nin = ndimage.label(p, structure=np.ones((3, 3)))
+    out_lab, nout = ndimage.label(~p)
+    border = set(np.unique(np.concatenate([out_lab[0], out_lab[-1], out_lab[:, 0],
+                                           out_lab[:, -1]])))
+    assert len(curves) == nin + (nout - len(border - {0}))
+    # every point is the midpoint of one inside/outside pixel pair, and every
+    # such pair appears exactly once
+    seen = set()
+    for c in curves:
+        for x, y in c:
+            if x % 1 == 0.5:
+                a, b = (int(y), int(x - 0.5)), (int(y), int(x + 0.5))
+            else:
+                a, b = (int(y - 0.5), int(x)), (int(y + 0.5), int(x))
+            assert p[a] != p[b]
+            assert (a, b) not in seen
+            seen.add((a, b))
+        # closed: consecutive points are in one cell
+        d = np.abs(np.diff(np.vstack([c, c[:1]]), axis=0))
+        assert np.all(d.max(1) <= 1.0) and np.all(d.sum(1) > 0)
+    pairs = int(np.sum(p[:, 1:] != p[:, :-1]) + np.sum(p[1:, :] != p[:-1, :]))
+    assert len(seen) == pairs
+    assert hits.dtype == bool and len(hits) == len(curves)
+
+
+def test_boundaries_hits_edge():
+    bm = np.zeros((20, 20), bool)
+    bm[5:9, 5:9] = True
+    assert list(O.find_boundaries(bm)[1]) == [False]
+    bm[5:9, 1:5] = True                     # reaches column 1, next to the padded edge
+    assert list(O.find_boundaries(bm)[1]) == [True]
+    assert O.find_boundaries(np.ones((10, 10), bool))[1].tolist() == [True]
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_islands_are_connected_components(seed):
+    bm = _blobs(64, 100 + seed, holes=False)
+    p = _padded(bm)
+    lab, nlab = ndimage.label(p, structure=np.ones((3, 3)))
+    _, nout = ndimage.label(~p)
+    if nout != 1:
+        pytest.skip("blobs enclose a hole")
+    isl = O.find_islands(bm)
+    curves, _ = O.find_boundaries(bm)
+    assert len(isl) == nlab == len(curves)
+    got = sorted(tuple(sorted(v)) for v in isl)
+    want = sorted(tuple(np.flatnonzero(lab.ravel() == i + 1)) for i in range(nlab))
+    assert got == want
+    for v in isl:
+        assert len(set(v)) == len(v)
+
+
+def test_convex_hull_matches_scipy():
+    rng = np.random.default_rng(3)
+    for n in (4, 10, 100, 2000):
+        pts = rng.normal(size=(n, 2))
+        h = O.convex_hull(pts)
+        assert set(h) == set(ConvexHull(pts).vertices)
+        # counter-clockwise from the lexicographically smallest point
+        q = pts[h]
+        assert h[0] == np.lexsort((pts[:, 1], pts[:, 0]))[0]
+        e1, e2 = np.roll(q, -1, 0) - q, np.roll(q, -2, 0) - np.roll(q, -1, 0)
+        cr = e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]
+        assert np.all(cr > 0)
+    # points on the edges of a square are not vertices; three points or fewer
+    # come back as they are
+    g = np.array([[0, 0], [1, 0], [2, 0], [2, 1], [2, 2], [1, 2], [0, 2], [0, 1], [1, 1.0]])
+    assert sorted(O.convex_hull(g)) == [0, 2, 4, 6]
+    assert list(O.convex_hull(g[:3])) == [0, 1, 2]
+
+
+def test_in_curve_matches_matplotlib():
+    from matplotlib.path import Path
+    rng = np.random.default_rng(5)
+    t = np.sort(rng.uniform(0, 2 * np.pi, 40))
+    r = 1 + 0.4 * np.sin(5 * t)
+    poly = np.stack([r * np.cos(t), r * np.sin(t)], 1)
+    q = rng.uniform(-1.6, 1.6, size=(3000, 2))
+    assert np.array_equal(O.in_curve(poly, q), Path(poly).contains_points(q))
+    assert not O.in_curve(poly[:2], q[:5]).any()
+
+
+# --------------------------------------------------------- lens fields
+def _grid(N, rng_):
+    x = O.gridmap_positions(N, (0.0, 0.0), rng_)
+    return x
+
+
+def _isothermal(x, te, tc, centre=(0.0, 0.0)):
+    """alpha and A = I - d alpha / d x of psi = te sqrt(|x - c|^2 + tc^2)."""
+    d = x - np.asarray(centre)
+    s = np.sqrt((d ** 2).sum(1) + tc * tc)
+    alpha = te * d / s[:, None]
+    A = np.zeros((len(x), 4))
+    A[:, 0] = 1 - te * (1 / s - d[:, 0] ** 2 / s ** 3)
+    A[:, 3] = 1 - te * (1 / s - d[:, 1] ** 2 / s ** 3)
+    A[:, 1] = A[:, 2] = te * d[:, 0] * d[:, 1] / s ** 3
+    return alpha, A
+
+
+def _radius(c, centre=(0.0, 0.0)):
+    return np.hypot(c[:, 0] - centre[0], c[:, 1] - centre[1])
+
+
+def test_find_crit_singular_isothermal():
+    N, rg, te = 96, 2.0, 0.5
+    x = _grid(N, rg)
+    a, A = _isothermal(x, te, 0.0)
+    curves, hits, types, rep = O.find_crit(N, x, x - a, A, rg)
+    res = rg / (N - 1)
+    assert rep and list(types) == [O.CRIT_TANGENTIAL, O.CRIT_PSEUDO]
+    assert np.all(np.abs(_radius(curves[0]) - te) <= res)
+    # the pseudo curve: image positions of the hull of the island's sources,
+    # which are the pixels nearest the centre (|y| = te - |x|)
+    assert np.all(_radius(curves[1]) < 0.25 * te)
+    assert list(hits) == [False, False]
+
+
+def test_find_crit_cored_isothermal_and_order():
+    N, rg = 160, 5.0
+    x = _grid(N, rg)
+    res = rg / (N - 1)
+    # two cored lenses with radial curves, one singular lens (pseudo curve)
+    lenses = [((-1.3, 1.2), 0.55, 0.12), ((1.2, -1.2), 0.5, 0.1), ((1.3, 1.4), 0.35, 0.0)]
+    alpha = np.zeros_like(x)
+    A = np.zeros((len(x), 4))
+    A[:, 0] = A[:, 3] = 1.0
+    for c, te, tc in lenses:
+        a1, A1 = _isothermal(x, te, tc, c)
+        alpha += a1
+        A += A1
+        A[:, 0] -= 1
+        A[:, 3] -= 1
+    curves, hits, types, rep = O.find_crit(N, x, x - alpha, A, rg)
+    T, R, P = O.CRIT_TANGENTIAL, O.CRIT_RADIAL, O.CRIT_PSEUDO
+    # the scan meets the lens at y = -1.2 first, then the two at y ~ 1.3
+    assert list(types) == [T, R, T, R, T, P]
+    order = [lenses[1], lenses[0], lenses[2]]
+    for q, (c, te, tc) in enumerate(order):
+        rt = np.sqrt(te * te - tc * tc)
+        # the other lenses' convergence and shear move the curve by a few pixels
+        assert np.median(np.abs(_radius(curves[2 * q], c) - rt)) <= 4 * res
+        if tc > 0:
+            rr = np.sqrt((te * tc * tc) ** (2 / 3) - tc * tc)
+            assert np.max(np.abs(_radius(curves[2 * q + 1], c) - rr)) <= 2 * res
+    assert len(hits) == len(curves)
+
+
+def test_find_crit_without_critical_pixels_keeps_hits():
+    N = 16
+    x = _grid(N, 1.0)
+    A = np.tile([1.0, 0.0, 0.0, 1.0], (N * N, 1))
+    curves, hits, types, rep = O.find_crit(N, x, x, A, 1.0)
+    assert curves == [] and not rep
+
+
+def test_magnification_contour_circle():
+    N, rg, te = 80, 2.0, 0.4
+    x = _grid(N, rg)
+    a, A = _isothermal(x, te, 0.0)
+    res = rg / (N - 1)
+    for thr in (-0.5, 0.0, 0.3):
+        curves, hits, _, rep = O.find_magnification_contour(N, x, A, rg, thr)
+        # det A = 1 - te / |x| < thr inside |x| = te / (1 - thr)
+        assert rep and len(curves) == 1
+        assert np.all(np.abs(_radius(curves[0]) - te / (1 - thr)) <= res)
+
+
+def test_boundaries_of_caustics_isothermal():
+    N, rg, te = 48, 2.0, 0.5
+    x = _grid(N, rg)
+    a, A = _isothermal(x, te, 0.0)
+    curves, hits, mult = O.find_boundaries_of_caustics(N, x, x - a, A)
+    # an SIS has two images inside |y| = te, one outside (pixel units here)
+    res = rg / (N - 1)
+    assert len(curves) == 1
+    r = np.hypot(curves[0][:, 0] * res - rg / 2, curves[0][:, 1] * res - rg / 2)
+    assert np.all(np.abs(r - te) <= 2 * res)
+    # (sources whose image falls off the grid have none)
+    r = np.hypot(x[:, 0], x[:, 1])
+    assert np.all(mult[r > te + 2 * res] <= 1) and np.all(mult[r < te - 3 * res] >= 2)
