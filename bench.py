@@ -339,6 +339,71 @@ def run_images(S, W, ctx, nsrc, cpu=True, N=2048):
     return out
 
 
+def run_crit(S, W, ctx, N, cpu=True, Nm=768, Nc=128):
+    """GridMap::find_crit and find_magnification_contour on the 65,536-particle
+    NFW halo lens's N^2 GridMap (30" field), and find_boundaries_of_caustics
+    on an Nc^2 grid of the same lens (6" field; the reference tests every grid
+    position against every triangle, N^4 work).  The oracle's serial
+    restatements (the reference's routines are serial) are timed on the same
+    grid outputs, find_crit on an Nm^2 grid (its radial-curve placement is
+    quadratic in the number of curves: 46,300 curves at 768^2)."""
+    lens, _ = W.build_c1_lens(ctx)
+
+    def crit_on(n):
+        g = S.GridMap(lens, n, (0.0, 0.0), W.grid_range(30.0), S.SLSIM_FP64)
+        g.find_crit()
+        t0 = time.perf_counter()
+        res = g.find_crit()
+        return g, res, time.perf_counter() - t0
+
+    g, (curves, hits, types, _), tc = crit_on(N)
+    t0 = time.perf_counter()
+    cont, _, _, _ = g.find_magnification_contour(0.0)
+    tm = time.perf_counter() - t0
+    out = {"grid": N, "curves": len(curves), "points": int(sum(len(c) for c in curves)),
+           "tangential": int(np.sum(types == S.SLSIM_CRIT_TANGENTIAL)),
+           "radial": int(np.sum(types == S.SLSIM_CRIT_RADIAL)),
+           "pseudo": int(np.sum(types == S.SLSIM_CRIT_PSEUDO)),
+           "find_crit_seconds": tc, "magnification_contour_seconds": tm,
+           "contours": len(cont)}
+    gm, (cm, hm, tym, _), tcm = crit_on(Nm)
+    out.update({"grid_m": Nm, "curves_m": len(cm), "find_crit_seconds_m": tcm})
+    gc = S.GridMap(lens, Nc, (0.0, 0.0), W.grid_range(6.0), S.SLSIM_FP64)
+    gc.find_boundaries_of_caustics()
+    t0 = time.perf_counter()
+    bc, _, mult = gc.find_boundaries_of_caustics()
+    tb = time.perf_counter() - t0
+    out.update({"caustics_grid": Nc, "caustic_boundaries": len(bc),
+                "boundaries_of_caustics_seconds": tb})
+    if cpu:
+        import oracle as O
+        x, A = g.positions(), g.A()
+        t1 = time.perf_counter()
+        O.find_magnification_contour(N, x, A, g.range, 0.0)
+        out["cpu_magnification_contour_seconds"] = time.perf_counter() - t1
+        x, y, A = gm.positions(), gm.y(), gm.A()
+        t1 = time.perf_counter()
+        c0, h0, t0_, _ = O.find_crit(Nm, x, y, A, gm.range)
+        out["cpu_find_crit_seconds_m"] = time.perf_counter() - t1
+        out["find_crit_equal_to_oracle_m"] = bool(
+            len(c0) == len(cm) and np.array_equal(t0_, tym) and np.array_equal(h0, hm)
+            and all(np.array_equal(a, b) for a, b in zip(c0, cm)))
+        xc, yc, Ac = gc.positions(), gc.y(), gc.A()
+        t1 = time.perf_counter()
+        b0, _, m0 = O.find_boundaries_of_caustics(Nc, xc, yc, Ac)
+        out["cpu_boundaries_of_caustics_seconds"] = time.perf_counter() - t1
+        out["boundaries_of_caustics_equal_to_oracle"] = bool(
+            np.array_equal(m0, mult) and len(b0) == len(bc) and
+            all(np.array_equal(a, b) for a, b in zip(b0, bc)))
+        out["find_crit_speedup_m"] = out["cpu_find_crit_seconds_m"] / tcm
+        out["magnification_contour_speedup"] = out["cpu_magnification_contour_seconds"] / tm
+        out["boundaries_of_caustics_speedup"] = out["cpu_boundaries_of_caustics_seconds"] / tb
+    for h in (gc, gm, g):
+        h.close()
+    log("crit:", out)
+    return out
+
+
 def run_small_batches(S, W, ctx, nbatch, cpu=True, seed=606):
     """Callers that shoot many small ray lists (adaptive Grid refinement,
     critical-curve searches): nbatch lists of 64-4096 rays (log-uniform)
@@ -457,6 +522,8 @@ def main():
                     help="also time B small ray lists: sequential calls vs the server")
     ap.add_argument("--images", type=int, default=0, metavar="S",
                     help="also time GridMap::find_images for S sources on a 2048^2 grid")
+    ap.add_argument("--crit", type=int, default=0, metavar="N",
+                    help="also time GridMap::find_crit on an N^2 grid (and the other contours)")
     ap.add_argument("--halos", type=int, default=0, metavar="N",
                     help="also time a plane of N NFW halos (LensHaloHalos) on a 2048^2 GridMap")
     ap.add_argument("--formats", type=int, default=0, metavar="N",
@@ -676,6 +743,8 @@ def main():
         out["images"] = run_images(S, W, ctx, args.images, cpu=not args.no_cpu)
     if args.small_batches:
         out["small_batches"] = run_small_batches(S, W, ctx, args.small_batches)
+    if args.crit:
+        out["crit"] = run_crit(S, W, ctx, args.crit, cpu=not args.no_cpu)
 
     if args.precision_form:
         out["precision_form"] = run_precision_form(S, W, ctx)

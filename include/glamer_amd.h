@@ -87,6 +87,109 @@ struct Point_2d {
   double x[2];
 };
 
+// include/point.h:35
+enum class CritType { ND = 0, radial = 2, tangential = 1, pseudo = 3 };
+
+namespace slsim_cpp {
+// Reads and releases a curve list of the library.
+template <typename P>
+void take_curves(slsim_curves h, std::vector<std::vector<P>>& curves, size_t offset,
+                 std::vector<int32_t>& types, std::vector<int32_t>& hits, int& replaced) {
+  slsim_curves_info info;
+  int rc = slsim_curves_get_info(h, &info);
+  std::vector<int64_t> off(info.n_curves + 1);
+  std::vector<double> pts(2 * info.n_points + 2);
+  types.assign(info.n_curves + 1, 0);
+  hits.assign(info.n_curves + 1, 0);
+  if (rc == SLSIM_OK) rc = slsim_curves_read(h, off.data(), pts.data(), types.data(), hits.data());
+  slsim_curves_destroy(h);
+  check(rc);
+  types.resize(info.n_curves);
+  hits.resize(info.n_curves);
+  replaced = info.hits_replaced;
+  curves.resize(offset + info.n_curves);
+  for (size_t c = 0; c < info.n_curves; ++c) {
+    std::vector<P>& v = curves[offset + c];
+    v.resize((size_t)(off[c + 1] - off[c]));
+    for (size_t q = 0; q < v.size(); ++q) {
+      v[q][0] = pts[2 * (off[c] + q)];
+      v[q][1] = pts[2 * (off[c] + q) + 1];
+    }
+  }
+}
+}  // namespace slsim_cpp
+
+namespace Utilities {
+// include/concave_hull.h:200-494: the boundaries of the true pixels in pixel
+// units, pixel (0,0) at (0,0); the edge pixels of bitmap are set false first,
+// as the reference does to the caller's bitmap.
+template <typename P>
+void find_boundaries(std::vector<bool>& bitmap, long nx, std::vector<std::vector<P>>& points,
+                     std::vector<bool>& hits_edge, bool add_to_vector = false,
+                     bool outer_only = false) {
+  const size_t n = bitmap.size();
+  const long ny = nx > 0 ? (long)(n / nx) : 0;
+  if (nx <= 0 || n != (size_t)(nx * ny)) throw std::invalid_argument("invalid size");
+  std::vector<uint8_t> bm(n);
+  for (size_t i = 0; i < n; ++i) bm[i] = bitmap[i];
+  slsim_curves h = nullptr;
+  slsim_cpp::check(slsim_find_boundaries(bm.data(), (size_t)nx, (size_t)ny, &h));
+  for (long i = 0; i < nx; ++i) bitmap[i] = bitmap[i + nx * (ny - 1)] = false;
+  for (long j = 0; j < ny; ++j) bitmap[j * nx] = bitmap[nx - 1 + j * nx] = false;
+  const size_t offset = add_to_vector ? points.size() : 0;
+  if (!add_to_vector) hits_edge.resize(0);
+  std::vector<int32_t> types, hits;
+  int replaced;
+  std::vector<std::vector<P>> found;
+  slsim_cpp::take_curves(h, found, 0, types, hits, replaced);
+  if (outer_only && found.size() > 1) {
+    found.resize(1);
+    hits.resize(1);
+  }
+  points.resize(offset + found.size());
+  for (size_t c = 0; c < found.size(); ++c) points[offset + c] = found[c];
+  for (int32_t b : hits) hits_edge.push_back(b != 0);
+}
+
+// include/concave_hull.h:560-609: indices of the convex hull, counter-clockwise
+template <typename T>
+void convex_hull(const std::vector<T>& P, std::vector<size_t>& hull) {
+  std::vector<double> p(2 * P.size());
+  for (size_t i = 0; i < P.size(); ++i) {
+    p[2 * i] = P[i][0];
+    p[2 * i + 1] = P[i][1];
+  }
+  std::vector<int64_t> h(2 * P.size() + 1);
+  size_t nh = 0;
+  slsim_cpp::check(slsim_convex_hull(p.data(), P.size(), h.data(), &nh));
+  hull.assign(h.begin(), h.begin() + nh);
+}
+// include/concave_hull.h:513-556: the hull's points
+template <typename T>
+std::vector<T> convex_hull(const std::vector<T>& PP) {
+  if (PP.size() <= 3) return PP;
+  std::vector<size_t> h;
+  convex_hull(PP, h);
+  std::vector<T> out;
+  for (size_t i : h) out.push_back(PP[i]);
+  return out;
+}
+
+// include/concave_hull.h:1022-1063
+template <typename Ptype>
+bool inCurve(const Ptype& x, const std::vector<Ptype>& H) {
+  std::vector<double> h(2 * H.size());
+  for (size_t i = 0; i < H.size(); ++i) {
+    h[2 * i] = H[i][0];
+    h[2 * i + 1] = H[i][1];
+  }
+  const double q[2] = {x[0], x[1]};
+  int32_t in = 0;
+  slsim_cpp::check(slsim_in_curve(h.data(), H.size(), q, 1, &in));
+  return in != 0;
+}
+}  // namespace Utilities
+
 template <typename T = double>
 struct Matrix2x2 {
   T a[4];
@@ -896,6 +999,49 @@ struct GridMap {
       image_points.emplace_back(r[8 * i], r[8 * i + 1]);
       triangles.emplace_back((size_t)t[3 * i], (size_t)t[3 * i + 1], (size_t)t[3 * i + 2]);
     }
+  }
+
+  // gridmap.cpp:525-647: tangential curves (A's smaller eigenvalue < 0),
+  // radial curves (both < 0) after the tangential curve holding them, and a
+  // pseudo curve after a tangential curve with none; the classification on
+  // the device, the boundary walk on the host
+  void find_crit(std::vector<std::vector<Point_2d>>& points, std::vector<bool>& hits_boundary,
+                 std::vector<CritType>& crit_type) {
+    slsim_curves h = nullptr;
+    slsim_cpp::check(slsim_gridmap_find_crit(slsim_cpp::context(), Ngrid_init, center[0],
+                                             center[1], x_range, (const double*)yd,
+                                             (const double*)Ad, &h));
+    std::vector<int32_t> t, hb;
+    int replaced;
+    slsim_cpp::take_curves(h, points, 0, t, hb, replaced);
+    crit_type.resize(t.size());
+    for (size_t i = 0; i < t.size(); ++i) crit_type[i] = (CritType)t[i];
+    if (replaced) hits_boundary.assign(hb.begin(), hb.end());
+  }
+  // gridmap.cpp:493-522
+  void find_magnification_contour(std::vector<std::vector<Point_2d>>& curves,
+                                  std::vector<bool>& hits_boundary, double invmag) {
+    slsim_curves h = nullptr;
+    slsim_cpp::check(slsim_gridmap_find_magnification_contour(
+        slsim_cpp::context(), Ngrid_init, center[0], center[1], x_range, (const double*)Ad, invmag,
+        &h));
+    std::vector<int32_t> t, hb;
+    int replaced;
+    slsim_cpp::take_curves(h, curves, 0, t, hb, replaced);
+    if (replaced) hits_boundary.assign(hb.begin(), hb.end());
+  }
+  // gridmap.h:278-296: boundaries (pixel units) of the grid positions with
+  // more than one image as sources
+  void find_boundaries_of_caustics(std::vector<std::vector<Point_2d>>& boundaries,
+                                   std::vector<bool>& hits_edge) {
+    slsim_curves h = nullptr;
+    slsim_cpp::check(slsim_gridmap_find_boundaries_of_caustics(
+        slsim_cpp::context(), Ngrid_init, center[0], center[1], x_range, (const double*)yd,
+        nullptr, &h));
+    std::vector<int32_t> t, hb;
+    int replaced;
+    slsim_cpp::take_curves(h, boundaries, 0, t, hb, replaced);
+    hits_edge.assign(hb.begin(), hb.end());
   }
 
   size_t getNumberOfPoints() const { return (size_t)Ngrid_init * Ngrid_init2; }

@@ -208,6 +208,66 @@ int slsim_gridmap_find_images(slsim_ctx ctx, size_t N, double c0, double c1, dou
                               const double* ys, size_t max_per_source, int32_t* multiplicity,
                               double* rays, int64_t* triangles, size_t* nimages);
 
+/* ---------------------------------------- critical curves and contours */
+/* Curve types, CritType's values (include/point.h:35). */
+enum {
+  SLSIM_CRIT_ND = 0,
+  SLSIM_CRIT_TANGENTIAL = 1,
+  SLSIM_CRIT_RADIAL = 2,
+  SLSIM_CRIT_PSEUDO = 3
+};
+/* Curves found on a grid (host memory), read with slsim_curves_read. */
+typedef struct slsim_curves_s* slsim_curves;
+typedef struct {
+  size_t n_curves;
+  size_t n_points;      /* over all curves */
+  int32_t hits_replaced;/* 0: the reference leaves the caller's hits_boundary as it was */
+} slsim_curves_info;
+
+/* GridMap::find_crit(points, hits_boundary, crit_type)
+ * (TreeCode_link/gridmap.cpp:525-647) over a grid shot by slsim_gridmap_shoot
+ * with the same N, centre and range (y_dev, A_dev its outputs): the
+ * boundaries (Utilities::find_boundaries, include/concave_hull.h:200-494) of
+ * the pixels with A's smaller eigenvalue negative (tangential) and with both
+ * negative (radial), each radial curve placed after the tangential curve
+ * holding its first point, and for a tangential curve with no radial curve a
+ * pseudo curve (the image positions of the convex hull of its island's
+ * source positions).  Positions in the grid's units.  *out is released with
+ * slsim_curves_destroy. */
+int slsim_gridmap_find_crit(slsim_ctx ctx, size_t N, double c0, double c1, double range,
+                            const double* y_dev, const double* A_dev, slsim_curves* out);
+/* GridMap::find_magnification_contour(curves, hits_boundary, invmag)
+ * (gridmap.cpp:493-522): the boundaries of the pixels with det A < invmag. */
+int slsim_gridmap_find_magnification_contour(slsim_ctx ctx, size_t N, double c0, double c1,
+                                             double range, const double* A_dev, double invmag,
+                                             slsim_curves* out);
+/* GridMap::find_boundaries_of_caustics(boundaries, hits_edge)
+ * (include/gridmap.h:278-296): the boundaries, in pixel units, of the grid
+ * positions that have more than one image when taken as sources
+ * (find_images' triangle rules).  multiplicity (N*N host ints) may be NULL. */
+int slsim_gridmap_find_boundaries_of_caustics(slsim_ctx ctx, size_t N, double c0, double c1,
+                                              double range, const double* y_dev,
+                                              int32_t* multiplicity, slsim_curves* out);
+/* Utilities::find_boundaries(bitmap, nx, points, hits_edge)
+ * (include/concave_hull.h:200-494) on a host bitmap of nx x ny bytes (nonzero
+ * inside), on the host: the boundaries in pixel units, edge pixels counted
+ * as outside, with hits_edge per curve. */
+int slsim_find_boundaries(const uint8_t* bitmap, size_t nx, size_t ny, slsim_curves* out);
+/* Utilities::convex_hull(P, hull) (include/concave_hull.h:560-609), on the
+ * host: indices into points (2 n doubles), counter-clockwise from the
+ * smallest x; hull holds 2 n entries. */
+int slsim_convex_hull(const double* points, size_t n, int64_t* hull, size_t* nhull);
+/* Utilities::inCurve(x, H) (include/concave_hull.h:1022-1063) for each of the
+ * m points against the polygon poly (n points), on the host. */
+int slsim_in_curve(const double* poly, size_t n, const double* points, size_t m,
+                   int32_t* inside);
+int slsim_curves_get_info(slsim_curves curves, slsim_curves_info* info);
+/* offsets: n_curves + 1 (points of curve c are offsets[c] .. offsets[c+1]),
+ * points: 2 n_points, types and hits_boundary: n_curves each. */
+int slsim_curves_read(slsim_curves curves, int64_t* offsets, double* points, int32_t* types,
+                      int32_t* hits_boundary);
+int slsim_curves_destroy(slsim_curves curves);
+
 /* --------------------------------------------------- batched GridMaps */
 /* Batching and serving have no counterpart in the reference, which shoots
  * one GridMap at a time under a process-wide mutex (gridmap.cpp:15,116-119);

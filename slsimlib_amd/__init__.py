@@ -12,6 +12,7 @@ import numpy as np
 
 from . import _abi
 from ._abi import SLSIM_DIRECT64, SLSIM_FP32C, SLSIM_FP64, SlsimError, lib
+from ._abi import SLSIM_CRIT_ND, SLSIM_CRIT_PSEUDO, SLSIM_CRIT_RADIAL, SLSIM_CRIT_TANGENTIAL
 from ._abi import (SLSIM_FMT_ASCII, SLSIM_FMT_CSV3, SLSIM_FMT_CSV4, SLSIM_FMT_CSV5,
                    SLSIM_FMT_CSV6, SLSIM_FMT_GADGET2, SLSIM_FMT_GLMB)
 
@@ -313,6 +314,57 @@ class Lens:
             pass
 
 
+def _read_curves(h):
+    try:
+        info = _abi.slsim_curves_info()
+        _check(lib.slsim_curves_get_info(h, ctypes.byref(info)))
+        nc, npt = info.n_curves, info.n_points
+        off = np.zeros(nc + 1, np.int64)
+        pts = np.zeros((max(npt, 1), 2))
+        types = np.zeros(max(nc, 1), np.int32)
+        hits = np.zeros(max(nc, 1), np.int32)
+        _check(lib.slsim_curves_read(h, _ptr(off, ctypes.c_int64), _ptr(pts, ctypes.c_double),
+                                     _ptr(types, ctypes.c_int32), _ptr(hits, ctypes.c_int32)))
+        curves = [pts[off[i]:off[i + 1]].copy() for i in range(nc)]
+        return curves, hits[:nc].astype(bool), types[:nc], bool(info.hits_replaced)
+    finally:
+        lib.slsim_curves_destroy(h)
+
+
+def find_boundaries(bitmap):
+    """Utilities::find_boundaries(bitmap, nx, points, hits_edge)
+    (include/concave_hull.h:200-494) of a (ny, nx) bitmap: (curves in pixel
+    units, hits_edge)."""
+    bm = np.ascontiguousarray(bitmap, dtype=np.uint8)
+    ny, nx = bm.shape
+    h = _abi.slsim_curves()
+    _check(lib.slsim_find_boundaries(_ptr(bm, ctypes.c_uint8), nx, ny, ctypes.byref(h)))
+    c, hits, _, _ = _read_curves(h)
+    return c, hits
+
+
+def convex_hull(points):
+    """Utilities::convex_hull(P, hull) (include/concave_hull.h:560-609):
+    hull indices into points, counter-clockwise."""
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    hull = np.zeros(2 * len(p) + 1, np.int64)
+    n = ctypes.c_size_t()
+    _check(lib.slsim_convex_hull(_ptr(p, ctypes.c_double), len(p), _ptr(hull, ctypes.c_int64),
+                                 ctypes.byref(n)))
+    return hull[:n.value]
+
+
+def in_curve(poly, points):
+    """Utilities::inCurve(x, H) (include/concave_hull.h:1022-1063) for each
+    point."""
+    h = np.ascontiguousarray(poly, dtype=np.float64).reshape(-1, 2)
+    q = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros(max(len(q), 1), np.int32)
+    _check(lib.slsim_in_curve(_ptr(h, ctypes.c_double), len(h), _ptr(q, ctypes.c_double), len(q),
+                              _ptr(out, ctypes.c_int32)))
+    return out[:len(q)].astype(bool)
+
+
 class GridMap:
     """GridMap(lens, N, center, range) (TreeCode_link/gridmap.cpp:93-120):
     shoots the N x N grid on the device; outputs stay in HBM until read."""
@@ -431,6 +483,39 @@ class GridMap:
             _ptr(rays, ctypes.c_double), tris.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
             ctypes.byref(n)))
         return rays[:n.value], mult, tris[:n.value]
+
+    def find_crit(self):
+        """GridMap::find_crit(points, hits_boundary, crit_type)
+        (gridmap.cpp:525-647) on the device over this grid's A: (curves, a
+        list of (n, 2) arrays in the grid's units; hits_boundary (bool);
+        crit types SLSIM_CRIT_*; whether hits_boundary was replaced, False
+        when no pixel is critical)."""
+        h = _abi.slsim_curves()
+        _check(lib.slsim_gridmap_find_crit(self.ctx.h, ctypes.c_size_t(self.N), self.center[0],
+                                           self.center[1], self.range, self.y_dev, self.A_dev,
+                                           ctypes.byref(h)))
+        return _read_curves(h)
+
+    def find_magnification_contour(self, invmag):
+        """GridMap::find_magnification_contour(curves, hits_boundary, invmag)
+        (gridmap.cpp:493-522): (curves, hits_boundary, -, replaced)."""
+        h = _abi.slsim_curves()
+        _check(lib.slsim_gridmap_find_magnification_contour(
+            self.ctx.h, ctypes.c_size_t(self.N), self.center[0], self.center[1], self.range,
+            self.A_dev, float(invmag), ctypes.byref(h)))
+        return _read_curves(h)
+
+    def find_boundaries_of_caustics(self):
+        """GridMap::find_boundaries_of_caustics(boundaries, hits_edge)
+        (gridmap.h:278-296): (boundaries in pixel units, hits_edge, the
+        multiplicity of every grid position taken as a source)."""
+        h = _abi.slsim_curves()
+        mult = np.zeros(self.N * self.N, np.int32)
+        _check(lib.slsim_gridmap_find_boundaries_of_caustics(
+            self.ctx.h, ctypes.c_size_t(self.N), self.center[0], self.center[1], self.range,
+            self.y_dev, _ptr(mult, ctypes.c_int32), ctypes.byref(h)))
+        c, hits, _, _ = _read_curves(h)
+        return c, hits, mult
 
     def writeFits(self, var, filename, dtype=np.float64):
         """GridMap::writeFits<T>(lensvar, filename) (gridmap.h:524-531): the

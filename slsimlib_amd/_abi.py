@@ -101,6 +101,15 @@ class slsim_server_stats(ctypes.Structure):
 
 slsim_batch = ctypes.c_void_p
 slsim_server = ctypes.c_void_p
+slsim_curves = ctypes.c_void_p
+
+# CritType (include/point.h:35)
+SLSIM_CRIT_ND, SLSIM_CRIT_TANGENTIAL, SLSIM_CRIT_RADIAL, SLSIM_CRIT_PSEUDO = range(4)
+
+
+class slsim_curves_info(ctypes.Structure):
+    _fields_ = [("n_curves", ctypes.c_size_t), ("n_points", ctypes.c_size_t),
+                ("hits_replaced", ctypes.c_int32)]
 
 P = ctypes.POINTER
 c_int, c_double, c_float, c_size_t = ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_size_t
@@ -171,6 +180,21 @@ SIGNATURES = {
                                           c_vp, c_size_t, P(c_double), c_size_t,
                                           P(ctypes.c_int32), P(c_double), P(ctypes.c_int64),
                                           P(c_size_t)]),
+    "slsim_gridmap_find_crit": (c_int, [slsim_ctx, c_size_t, c_double, c_double, c_double, c_vp,
+                                        c_vp, P(slsim_curves)]),
+    "slsim_gridmap_find_magnification_contour": (c_int, [slsim_ctx, c_size_t, c_double, c_double,
+                                                         c_double, c_vp, c_double,
+                                                         P(slsim_curves)]),
+    "slsim_gridmap_find_boundaries_of_caustics": (c_int, [slsim_ctx, c_size_t, c_double, c_double,
+                                                          c_double, c_vp, P(ctypes.c_int32),
+                                                          P(slsim_curves)]),
+    "slsim_find_boundaries": (c_int, [P(ctypes.c_uint8), c_size_t, c_size_t, P(slsim_curves)]),
+    "slsim_convex_hull": (c_int, [P(c_double), c_size_t, P(c_i64), P(c_size_t)]),
+    "slsim_in_curve": (c_int, [P(c_double), c_size_t, P(c_double), c_size_t, P(ctypes.c_int32)]),
+    "slsim_curves_get_info": (c_int, [slsim_curves, P(slsim_curves_info)]),
+    "slsim_curves_read": (c_int, [slsim_curves, P(c_i64), P(c_double), P(ctypes.c_int32),
+                                  P(ctypes.c_int32)]),
+    "slsim_curves_destroy": (c_int, [slsim_curves]),
     "slsim_halos_create_nfw": (c_int, [slsim_ctx, P(c_double), P(c_double), P(c_float),
                                        P(c_float), P(c_float), c_size_t, c_float, c_float,
                                        P(slsim_plane)]),

@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libslsim_amd.so")
 
 SOURCES = ["tree_build.hip", "shoot.hip", "api.hip", "batch.hip", "smoothing.hip", "generator.cpp",
-           "formats.cpp", "halos.hip", "images.hip"]
+           "formats.cpp", "halos.hip", "images.hip", "crit.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -288,6 +289,91 @@ int slsim_gridmap_find_images(slsim_ctx ctx, size_t N, double c0, double c1, dou
   SLSIM_HIP_CHECK(hipSetDevice(ctx->device));
   return find_images(ctx, N, c0, c1, range, y_dev, A_dev, nsrc, ys, max_per_source, multiplicity,
                      rays, triangles, nimages);
+}
+
+static int curves_call(slsim_ctx ctx, size_t N, slsim_curves* out,
+                       const std::function<int(slsim_curves_s*)>& body) {
+  if (!ctx || !out) return arg_error("null argument");
+  *out = nullptr;
+  if (N >= 65536) return arg_error("critical curves: N must be below 65536");
+  SLSIM_HIP_CHECK(hipSetDevice(ctx->device));
+  std::unique_ptr<slsim_curves_s> r(new slsim_curves_s);
+  const int rc = body(r.get());
+  if (rc == SLSIM_OK) *out = r.release();
+  return rc;
+}
+
+int slsim_gridmap_find_crit(slsim_ctx ctx, size_t N, double c0, double c1, double range,
+                            const double* y_dev, const double* A_dev, slsim_curves* out) {
+  if (N >= 1 && (!y_dev || !A_dev)) return arg_error("null grid outputs");
+  return curves_call(ctx, N, out, [&](slsim_curves_s* r) {
+    return find_crit(ctx, N, c0, c1, range, y_dev, A_dev, r);
+  });
+}
+
+int slsim_gridmap_find_magnification_contour(slsim_ctx ctx, size_t N, double c0, double c1,
+                                             double range, const double* A_dev, double invmag,
+                                             slsim_curves* out) {
+  if (N >= 1 && !A_dev) return arg_error("null grid outputs");
+  return curves_call(ctx, N, out, [&](slsim_curves_s* r) {
+    return find_magnification_contour(ctx, N, c0, c1, range, A_dev, invmag, r);
+  });
+}
+
+int slsim_gridmap_find_boundaries_of_caustics(slsim_ctx ctx, size_t N, double c0, double c1,
+                                              double range, const double* y_dev,
+                                              int32_t* multiplicity, slsim_curves* out) {
+  if (N >= 1 && !y_dev) return arg_error("null grid outputs");
+  return curves_call(ctx, N, out, [&](slsim_curves_s* r) {
+    return find_boundaries_of_caustics(ctx, N, c0, c1, range, y_dev, multiplicity, r);
+  });
+}
+
+int slsim_find_boundaries(const uint8_t* bitmap, size_t nx, size_t ny, slsim_curves* out) {
+  if (!out || (nx * ny && !bitmap)) return arg_error("null argument");
+  *out = nullptr;
+  std::unique_ptr<slsim_curves_s> r(new slsim_curves_s);
+  const int rc = find_boundaries_host(bitmap, nx, ny, r.get());
+  if (rc == SLSIM_OK) *out = r.release();
+  return rc;
+}
+
+int slsim_convex_hull(const double* points, size_t n, int64_t* hull, size_t* nhull) {
+  if (!nhull || (n && (!points || !hull))) return arg_error("null argument");
+  const std::vector<int64_t> h = convex_hull_host(points, n);
+  std::copy(h.begin(), h.end(), hull);
+  *nhull = h.size();
+  return SLSIM_OK;
+}
+
+int slsim_in_curve(const double* poly, size_t n, const double* points, size_t m, int32_t* inside) {
+  if ((n && !poly) || (m && (!points || !inside))) return arg_error("null argument");
+  for (size_t i = 0; i < m; ++i) inside[i] = in_curve_host(poly, n, points[2 * i], points[2 * i + 1]);
+  return SLSIM_OK;
+}
+
+int slsim_curves_get_info(slsim_curves c, slsim_curves_info* info) {
+  if (!c || !info) return arg_error("null argument");
+  info->n_curves = c->types.size();
+  info->n_points = c->pts.size() / 2;
+  info->hits_replaced = c->hits_replaced;
+  return SLSIM_OK;
+}
+
+int slsim_curves_read(slsim_curves c, int64_t* offsets, double* points, int32_t* types,
+                      int32_t* hits) {
+  if (!c || !offsets || (!points && !c->pts.empty()) || (!c->types.empty() && (!types || !hits)))
+    return arg_error("null argument");
+  std::copy(c->off.begin(), c->off.end(), offsets);
+  std::copy(c->pts.begin(), c->pts.end(), points);
+  std::copy(c->types.begin(), c->types.end(), types);
+  std::copy(c->hits.begin(), c->hits.end(), hits);
+  return SLSIM_OK;
+}
+
+int slsim_curves_destroy(slsim_curves c) {
+  delete c;
+  return SLSIM_OK;
 }
 
 int slsim_plane_destroy(slsim_plane plane) {

@@ -42,6 +42,67 @@ __device__ inline int side(double y0, double y1, double a0, double a1, double b0
   return sgn(u0 * v1 - u1 * v0);
 }
 
+// Triangle (a, b, c) against source q, as one triangle of find_images:
+// -1 no image, else 0 inside, 1..3 on the first/second/third edge, 4 on the
+// first vertex (the first triangle of a cell only)
+__device__ inline int tri_code(double q0, double q1, double a0, double a1, double b0, double b1,
+                               double c0, double c1, bool first) {
+  const int s1 = side(q0, q1, a0, a1, b0, b1);
+  const int s2 = side(q0, q1, b0, b1, c0, c1);
+  const int s3 = side(q0, q1, c0, c1, a0, a1);
+  const int sum = s1 + s2 + s3;
+  if (sum == 3 || sum == -3) return 0;
+  if (sum == 2) return s1 == 0 ? 1 : (s2 == 0 ? 2 : 3);
+  if (first && sum != -2 && s1 == 0 && s3 == 0) return 4;
+  return -1;
+}
+
+// Images of every grid position as a source (find_boundaries_of_caustics'
+// find_images over i_points, include/gridmap.h:284-288), counted per source.
+// Each triangle tests only the grid positions in its source-plane bounding
+// box widened by one column and row each side; positions outside it are not
+// inside, on an edge or on a vertex of the triangle.  A triangle with a
+// non-finite corner, whose sign tests give zeros, tests every position.
+__global__ __launch_bounds__(256) void k_count_grid_images(const double* __restrict__ y,
+                                                           int64_t N, double g0, double g1,
+                                                           double range,
+                                                           int32_t* __restrict__ mult) {
+#pragma clang fp contract(off)
+  const int64_t ncell = (N - 1) * (N - 1);
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int64_t k = (c / (N - 1)) * N + c % (N - 1);
+  const double p0x = y[2 * k], p0y = y[2 * k + 1];
+  const double p1x = y[2 * (k + N + 1)], p1y = y[2 * (k + N + 1) + 1];
+  for (int t = 0; t < 2; ++t) {
+    const int64_t k2 = t == 0 ? k + 1 : k + N;
+    const double p2x = y[2 * k2], p2y = y[2 * k2 + 1];
+    int64_t ilo = 0, ihi = N - 1, jlo = 0, jhi = N - 1;
+    const bool finite = isfinite(p0x) && isfinite(p0y) && isfinite(p1x) && isfinite(p1y) &&
+                        isfinite(p2x) && isfinite(p2y);
+    if (finite) {
+      const double s = (double)(N - 1) / range;
+      const double fx0 = ((fmin(fmin(p0x, p1x), p2x) - g0) * s) + 0.5 * (double)(N - 1);
+      const double fx1 = ((fmax(fmax(p0x, p1x), p2x) - g0) * s) + 0.5 * (double)(N - 1);
+      const double fy0 = ((fmin(fmin(p0y, p1y), p2y) - g1) * s) + 0.5 * (double)(N - 1);
+      const double fy1 = ((fmax(fmax(p0y, p1y), p2y) - g1) * s) + 0.5 * (double)(N - 1);
+      const double top = (double)(N - 1);
+      if (fx1 < -1.0 || fy1 < -1.0 || fx0 > top + 1.0 || fy0 > top + 1.0) continue;
+      ilo = (int64_t)fmax(floor(fx0) - 1.0, 0.0);
+      ihi = (int64_t)fmin(ceil(fx1) + 1.0, top);
+      jlo = (int64_t)fmax(floor(fy0) - 1.0, 0.0);
+      jhi = (int64_t)fmin(ceil(fy1) + 1.0, top);
+    }
+    for (int64_t j = jlo; j <= jhi; ++j)
+      for (int64_t i = ilo; i <= ihi; ++i) {
+        const int64_t s = j * N + i;
+        double q0, q1;
+        grid_position(s, N, g0, g1, range, &q0, &q1);
+        if (tri_code(q0, q1, p0x, p0y, p1x, p1y, p2x, p2y, t == 0) >= 0) atomicAdd(&mult[s], 1);
+      }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_find_images(const double* __restrict__ y, int64_t N,
                                                      const double* __restrict__ ys, int nsrc,
                                                      uint32_t cap, uint32_t* __restrict__ count,
@@ -139,6 +200,19 @@ __global__ void k_image_rays(const ImageHit* __restrict__ hits, const uint32_t* 
 }
 
 }  // namespace
+
+int count_grid_images(slsim_ctx c, size_t N, double c0, double c1, double range,
+                      const double* y_dev, int32_t* mult_dev) {
+  hipStream_t st = c->stream;
+  launch_zero_u32(st, (uint32_t*)mult_dev, (int)(N * N));
+  const int64_t ncell = N >= 2 ? (int64_t)(N - 1) * (int64_t)(N - 1) : 0;
+  if (ncell > 0) {
+    k_count_grid_images<<<(unsigned)((ncell + 255) / 256), 256, 0, st>>>(y_dev, (int64_t)N, c0,
+                                                                        c1, range, mult_dev);
+    SLSIM_HIP_CHECK(hipGetLastError());
+  }
+  return SLSIM_OK;
+}
 
 int find_images(slsim_ctx c, size_t N, double c0, double c1, double range, const double* y_dev,
                 const double* A_dev, size_t nsrc, const double* ys, size_t cap,

@@ -202,8 +202,8 @@ void set_error(const std::string& msg);
 
 // GridMap ray i of an N x N grid (TreeCode_link/gridmap.cpp:293-321,
 // utilities.cpp:149-158), computed without contraction as the reference.
-__device__ inline void grid_position(int64_t i, int64_t N, double c0, double c1, double range,
-                                     double* x0, double* x1) {
+__host__ __device__ inline void grid_position(int64_t i, int64_t N, double c0, double c1,
+                                              double range, double* x0, double* x1) {
 #pragma clang fp contract(off)
   *x0 = c0;
   *x1 = c1;
@@ -283,6 +283,15 @@ struct slsim_lens_s {
   int upload_trees();
 };
 
+// curves found on a grid: points of curve c are pts[2 off[c] .. 2 off[c + 1])
+struct slsim_curves_s {
+  std::vector<int64_t> off{0};
+  std::vector<double> pts;
+  std::vector<int32_t> types;
+  std::vector<int32_t> hits;
+  int hits_replaced = 0;
+};
+
 namespace slsim {
 
 // Device scratch of one call, returned to the context's cache on scope exit.
@@ -353,5 +362,19 @@ void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint3
 int find_images(slsim_ctx c, size_t N, double c0, double c1, double range, const double* y_dev,
                 const double* A_dev, size_t nsrc, const double* ys, size_t cap,
                 int32_t* multiplicity, double* rays, int64_t* triangles, size_t* nimages);
+// images of every grid position as a source, per position (images.hip)
+int count_grid_images(slsim_ctx c, size_t N, double c0, double c1, double range,
+                      const double* y_dev, int32_t* mult_dev);
+// critical curves and contours of a shot grid (crit.hip)
+int find_crit(slsim_ctx c, size_t N, double c0, double c1, double range, const double* y_dev,
+              const double* A_dev, slsim_curves_s* out);
+int find_magnification_contour(slsim_ctx c, size_t N, double c0, double c1, double range,
+                               const double* A_dev, double invmag, slsim_curves_s* out);
+int find_boundaries_of_caustics(slsim_ctx c, size_t N, double c0, double c1, double range,
+                                const double* y_dev, int32_t* mult_host, slsim_curves_s* out);
+// Utilities::find_boundaries, convex_hull and inCurve on host data (crit.hip)
+int find_boundaries_host(const uint8_t* bitmap, size_t nx, size_t ny, slsim_curves_s* out);
+std::vector<int64_t> convex_hull_host(const double* p, size_t n);
+bool in_curve_host(const double* poly, size_t n, double x0, double x1);
 
 }  // namespace slsim

@@ -2,7 +2,10 @@
 // that mixes a particle halo (z = 0.5) and a LensHaloHalos of 200 NFW halos
 // placed with setTheta (z = 0.3): shoots a 64 x 64 GridMap and writes x, y
 // and A of every ray as raw float64 to argv[1]; with argv[2], the images
-// (GridMap::find_images) of four grid rays' source positions there.
+// (GridMap::find_images) of four grid rays' source positions there; with
+// argv[3], the critical curves (GridMap::find_crit) and the contours of
+// GridMap::find_magnification_contour at invmag = 0.5, each curve as
+// (type, hits_boundary, n, n points).
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -64,6 +67,26 @@ int main(int argc, char** argv) {
     for (const RAY& r : imgs)
       w.insert(w.end(), {r.x[0], r.x[1], r.y[0], r.y[1], r.A[0], r.A[1], r.A[2], r.A[3]});
     FILE* g = std::fopen(argv[2], "wb");
+    if (!g) return 1;
+    std::fwrite(w.data(), sizeof(double), w.size(), g);
+    std::fclose(g);
+  }
+  if (argc > 3) {
+    std::vector<std::vector<Point_2d>> curves, contours;
+    std::vector<bool> hits, chits;
+    std::vector<CritType> types;
+    grid.find_crit(curves, hits, types);
+    grid.find_magnification_contour(contours, chits, 0.5);
+    std::vector<double> w = {(double)curves.size(), (double)contours.size()};
+    for (size_t c = 0; c < curves.size(); ++c) {
+      w.insert(w.end(), {(double)(int)types[c], (double)hits[c], (double)curves[c].size()});
+      for (const Point_2d& p : curves[c]) w.insert(w.end(), {p[0], p[1]});
+    }
+    for (size_t c = 0; c < contours.size(); ++c) {
+      w.insert(w.end(), {0.0, (double)chits[c], (double)contours[c].size()});
+      for (const Point_2d& p : contours[c]) w.insert(w.end(), {p[0], p[1]});
+    }
+    FILE* g = std::fopen(argv[3], "wb");
     if (!g) return 1;
     std::fwrite(w.data(), sizeof(double), w.size(), g);
     std::fclose(g);

@@ -220,3 +220,35 @@ def test_boundaries_of_caustics_isothermal():
     # (sources whose image falls off the grid have none)
     r = np.hypot(x[:, 0], x[:, 1])
     assert np.all(mult[r > te + 2 * res] <= 1) and np.all(mult[r < te - 3 * res] >= 2)
+
+
+# ------------------------------------------- the product's host utilities
+# Utilities::find_boundaries, convex_hull and inCurve run on the host in the
+# product too (the contour walk of find_crit); they equal the oracle's.
+def test_product_find_boundaries_equals_oracle():
+    import slsimlib_amd as S
+    rng = np.random.default_rng(1)
+    cases = [_blobs(64, s) for s in range(8)]
+    cases += [rng.random((37, 53)) < rng.uniform(0.2, 0.8) for _ in range(12)]   # saddles
+    cases += [np.ones((9, 9), bool), np.zeros((5, 7), bool), np.ones((1, 1), bool)]
+    for bm in cases:
+        a, ha = S.find_boundaries(bm)
+        b, hb = O.find_boundaries(bm)
+        assert len(a) == len(b) and np.array_equal(ha, hb)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
+def test_product_hull_and_in_curve_equal_oracle():
+    import slsimlib_amd as S
+    rng = np.random.default_rng(2)
+    for n in (1, 3, 4, 50, 5000):
+        p = rng.normal(size=(n, 2))
+        assert np.array_equal(S.convex_hull(p), O.convex_hull(p))
+    # ties and collinear points: a coarse lattice with repeats
+    p = np.floor(rng.uniform(0, 6, size=(400, 2)))
+    assert np.array_equal(S.convex_hull(p), O.convex_hull(p))
+    t = np.linspace(0, 2 * np.pi, 30, endpoint=False)
+    poly = np.stack([np.cos(t), np.sin(3 * t) * 0.5 + np.sin(t)], 1)
+    q = np.concatenate([rng.uniform(-1.5, 1.5, size=(2000, 2)), poly, poly[::2] * [1.0, 0.0]])
+    assert np.array_equal(S.in_curve(poly, q), O.in_curve(poly, q))

@@ -1,0 +1,161 @@
+"""GridMap::find_crit, find_magnification_contour and
+find_boundaries_of_caustics on the device against the oracle's restatement
+on the same grid outputs: every curve point, curve type, hits_boundary flag
+and multiplicity equal.  Cases: the 65,536-particle NFW halo's grid, three
+isothermal lenses (tangential, radial and pseudo curves, radial curves moved
+after their tangential ones), a field of matrices whose smaller eigenvalue
+is zero to the last bit (the eigenvalue rounding), and a source grid with a
+non-finite position."""
+import numpy as np
+import pytest
+
+import oracle as O
+import slsimlib_amd as S
+from slsimlib_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def shot(ctx):
+    lens, _ = W.build_c1_lens(ctx, n=65536)
+    N, rg = 256, W.grid_range(30.0)
+    g = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    yield g, g.positions(), g.y(), g.A()
+    g.close()
+
+
+class _Field:
+    """Grid outputs uploaded from the host, viewed as a GridMap."""
+
+    def __init__(self, ctx, N, centre, range_, y, A):
+        self.ctx, self.N, self.center, self.range = ctx, N, tuple(centre), range_
+        self.y_dev = ctx.alloc(16 * N * N)
+        self.A_dev = ctx.alloc(32 * N * N)
+        ctx.h2d(self.y_dev, np.ascontiguousarray(y, dtype=np.float64))
+        ctx.h2d(self.A_dev, np.ascontiguousarray(A, dtype=np.float64))
+
+    find_crit = S.GridMap.find_crit
+    find_magnification_contour = S.GridMap.find_magnification_contour
+    find_boundaries_of_caustics = S.GridMap.find_boundaries_of_caustics
+
+    def close(self):
+        self.ctx.free(self.y_dev)
+        self.ctx.free(self.A_dev)
+
+
+def _same(got, want):
+    c1, h1, t1, r1 = got
+    c2, h2, t2, r2 = want
+    assert r1 == r2
+    assert len(c1) == len(c2)
+    assert np.array_equal(h1, h2)
+    assert np.array_equal(t1, t2)
+    for a, b in zip(c1, c2):
+        assert np.array_equal(a, b)
+
+
+def _isothermal(x, lenses):
+    alpha = np.zeros_like(x)
+    A = np.zeros((len(x), 4))
+    A[:, 0] = A[:, 3] = 1.0
+    for c, te, tc in lenses:
+        d = x - np.asarray(c)
+        s = np.sqrt((d ** 2).sum(1) + tc * tc)
+        alpha += te * d / s[:, None]
+        A[:, 0] -= te * (1 / s - d[:, 0] ** 2 / s ** 3)
+        A[:, 3] -= te * (1 / s - d[:, 1] ** 2 / s ** 3)
+        A[:, 1] += te * d[:, 0] * d[:, 1] / s ** 3
+        A[:, 2] += te * d[:, 0] * d[:, 1] / s ** 3
+    return x - alpha, A
+
+
+def test_find_crit_particle_halo(shot):
+    g, x, y, A = shot
+    print("case: find_crit NFW particle halo N=256", flush=True)
+    got = g.find_crit()
+    _same(got, O.find_crit(g.N, x, y, A, g.range))
+    assert S.SLSIM_CRIT_TANGENTIAL in got[2]
+
+
+def test_find_crit_isothermal_lenses(ctx):
+    N, rg, centre = 160, 5.0, (0.25, -0.5)
+    x = O.gridmap_positions(N, centre, rg)
+    lenses = [((-1.05, 0.7), 0.55, 0.12), ((1.45, -1.7), 0.5, 0.1), ((1.55, 0.9), 0.35, 0.0)]
+    y, A = _isothermal(x, lenses)
+    f = _Field(ctx, N, centre, rg, y, A)
+    print("case: find_crit three isothermal lenses N=160", flush=True)
+    got = f.find_crit()
+    _same(got, O.find_crit(N, x, y, A, rg))
+    T, R, P = S.SLSIM_CRIT_TANGENTIAL, S.SLSIM_CRIT_RADIAL, S.SLSIM_CRIT_PSEUDO
+    assert list(got[2]) == [T, R, T, R, T, P]
+    # no critical pixel: hits_boundary is left to the caller
+    one = np.tile([1.0, 0.0, 0.0, 1.0], (N * N, 1))
+    f2 = _Field(ctx, N, centre, rg, x, one)
+    c, h, t, rep = f2.find_crit()
+    assert c == [] and not rep
+    f.close()
+    f2.close()
+
+
+def test_find_crit_eigenvalue_rounding(ctx):
+    # lambda_1 = m - sqrt(|m^2 - det|) within an ulp of zero on every pixel:
+    # the classification follows the rounding of the square root
+    N = 256
+    rng = np.random.default_rng(9)
+    n = N * N
+    m = rng.uniform(0.2, 2.0, n)
+    d = rng.uniform(-1.0, 1.0, n) * m
+    off = np.sqrt(m * m - d * d) * (1 + rng.integers(-3, 4, n) * 2.0 ** -52)
+    A = np.stack([m + d, off, off, m - d], 1)
+    flip = rng.random(n) < 0.3                       # some pixels radial too
+    A[flip] *= -1
+    x = O.gridmap_positions(N, (0.0, 0.0), 1.0)
+    y = rng.normal(size=(n, 2))
+    f = _Field(ctx, N, (0.0, 0.0), 1.0, y, A)
+    print("case: find_crit eigenvalue rounding N=256", flush=True)
+    got = f.find_crit()
+    want = O.find_crit(N, x, y, A, 1.0)
+    _same(got, want)
+    assert len(got[0]) > 100
+    f.close()
+
+
+@pytest.mark.parametrize("invmag", [-0.3, 0.0, 0.5])
+def test_magnification_contour(shot, invmag):
+    g, x, y, A = shot
+    print("case: find_magnification_contour invmag=%g" % invmag, flush=True)
+    _same(g.find_magnification_contour(invmag),
+          O.find_magnification_contour(g.N, x, A, g.range, invmag))
+
+
+def test_boundaries_of_caustics_particle_halo(ctx):
+    lens, _ = W.build_c1_lens(ctx, n=65536)
+    N, rg = 96, W.grid_range(6.0)            # a 6" field: positions with up to ~13 images
+    g = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    x, y, A = g.positions(), g.y(), g.A()
+    print("case: find_boundaries_of_caustics NFW particle halo N=96 6 arcsec", flush=True)
+    c, h, mult = g.find_boundaries_of_caustics()
+    c0, h0, m0 = O.find_boundaries_of_caustics(N, x, y, A)
+    assert np.array_equal(mult, m0)
+    assert mult.max() >= 3
+    assert np.array_equal(h, h0) and len(c) == len(c0)
+    for a, b in zip(c, c0):
+        assert np.array_equal(a, b)
+    g.close()
+
+
+def test_boundaries_of_caustics_singular_and_nonfinite(ctx):
+    N, rg = 64, 2.0
+    x = O.gridmap_positions(N, (0.0, 0.0), rg)
+    y, A = _isothermal(x, [((0.013, -0.021), 0.5, 0.0)])
+    y[1000] = np.nan                                 # a corner the sign tests see as zeros
+    f = _Field(ctx, N, (0.0, 0.0), rg, y, A)
+    print("case: find_boundaries_of_caustics SIS with a NaN source position N=64", flush=True)
+    c, h, mult = f.find_boundaries_of_caustics()
+    c0, h0, m0 = O.find_boundaries_of_caustics(N, x, y, A)
+    assert np.array_equal(mult, m0)
+    assert np.array_equal(h, h0) and len(c) == len(c0)
+    for a, b in zip(c, c0):
+        assert np.array_equal(a, b)
+    f.close()

@@ -163,7 +163,9 @@ def test_cpp_dropin_halos_program(ctx, tmp_path):
         pytest.fail("tests/cpp/dropin_halos is not built (run __graft_entry__.build())")
     out = tmp_path / "halos.bin"
     img = tmp_path / "images.bin"
-    r = subprocess.run([exe, str(out), str(img)], capture_output=True, text=True, timeout=120)
+    crit = tmp_path / "crit.bin"
+    r = subprocess.run([exe, str(out), str(img), str(crit)], capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     v = np.fromfile(out, dtype=np.float64).reshape(-1, 8)
     # the same light-cone through the Python view: LensHaloNFW positions are
@@ -189,6 +191,19 @@ def test_cpp_dropin_halos_program(ctx, tmp_path):
     ys = g.y()[[1000, 2080, 2112, 3000]]
     rays, mult, _ = g.find_images(ys)
     assert np.array_equal(w[:4], mult) and np.array_equal(w[4:].reshape(-1, 8), rays)
+    # GridMap::find_crit and find_magnification_contour from C++ equal the
+    # Python view's
+    q = np.fromfile(crit, dtype=np.float64)
+    curves, hits, types, _ = g.find_crit()
+    contours, chits, _, _ = g.find_magnification_contour(0.5)
+    assert (q[0], q[1]) == (len(curves), len(contours)) and len(curves) > 0
+    at = 2
+    for cv, hb, ty in list(zip(curves, hits, types)) + list(zip(contours, chits,
+                                                                [0] * len(chits))):
+        assert (q[at], q[at + 1], q[at + 2]) == (ty, hb, len(cv))
+        assert np.array_equal(q[at + 3:at + 3 + 2 * len(cv)].reshape(-1, 2), cv)
+        at += 3 + 2 * len(cv)
+    assert at == len(q)
     L = O.Lens(2.0)
     L.moveinMainHalo(O.Tree(px, py, pm, ph), 0.5)
     L.moveinMainHalo(O.HaloTree(hx, hy, m, rs, c), 0.3)
