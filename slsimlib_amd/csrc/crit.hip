@@ -34,41 +34,62 @@ enum { kByEigen = 0, kByInvmag = 1, kByMult = 2 };
 // bit 1: lambda_0 < 0 and lambda_1 < 0 (radial).  Matrix2x2::eigenv
 // (include/point.h:404-411), det (point.h:315-317), no contraction.  counts
 // take the bits before the outermost pixels are cleared (the reference counts
-// before find_boundaries pads the bitmap).
-__global__ __launch_bounds__(256) void k_crit_pixels(const double* __restrict__ A,
+// before find_boundaries pads the bitmap).  A grid-stride loop over a fixed
+// number of blocks, counts summed per wavefront by ballots and per block in
+// LDS: one atomic per block (one per wavefront of a full-grid launch puts
+// 1e5 atomics on one address, which costs more than reading A).
+constexpr int kPixBlocks = 2048;
+__global__ __launch_bounds__(256) void k_crit_pixels(const double2* __restrict__ A,
                                                      const int32_t* __restrict__ mult, int64_t N,
                                                      int mode, double invmag,
                                                      uint8_t* __restrict__ px,
                                                      unsigned long long* __restrict__ counts) {
 #pragma clang fp contract(off)
+  __shared__ unsigned long long part[2][4];
   const int64_t n = N * N;
-  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  uint32_t bits = 0;
-  if (k < n) {
-    if (mode == kByMult) {
-      bits = mult[k] > 1;
-    } else {
-      const double a0 = A[4 * k], a1 = A[4 * k + 1], a2 = A[4 * k + 2], a3 = A[4 * k + 3];
-      const double det = a0 * a3 - a1 * a2;
-      if (mode == kByInvmag) {
-        bits = det < invmag;
+  unsigned long long c0 = 0, c1 = 0;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = base + threadIdx.x;
+    uint32_t bits = 0;
+    if (k < n) {
+      if (mode == kByMult) {
+        bits = mult[k] > 1;
       } else {
-        const double m = (a0 + a3) / 2;
-        const double s = sqrt(fabs(m * m - det));
-        const double l0 = m + s, l1 = m - s;
-        bits = (l1 < 0 ? 1u : 0u) | (l0 < 0 && l1 < 0 ? 2u : 0u);
+        const double2 lo = A[2 * k], hi = A[2 * k + 1];
+        const double det = lo.x * hi.y - lo.y * hi.x;
+        if (mode == kByInvmag) {
+          bits = det < invmag;
+        } else {
+          const double m = (lo.x + hi.y) / 2;
+          const double s = sqrt(fabs(m * m - det));
+          const double l0 = m + s, l1 = m - s;
+          bits = (l1 < 0 ? 1u : 0u) | (l0 < 0 && l1 < 0 ? 2u : 0u);
+        }
       }
     }
+    c0 += __popcll(__ballot(bits & 1));
+    c1 += __popcll(__ballot((bits >> 1) & 1));
+    if (k < n) {
+      const int64_t i = k % N, j = k / N;
+      if (i == 0 || i == N - 1 || j == 0 || j == N - 1) bits = 0;
+      px[k] = (uint8_t)bits;
+    }
   }
-  const unsigned long long w0 = __ballot(bits & 1), w1 = __ballot((bits >> 1) & 1);
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    if (w0) atomicAdd(&counts[0], (unsigned long long)__popcll(w0));
-    if (w1) atomicAdd(&counts[1], (unsigned long long)__popcll(w1));
+    part[0][w] = c0;
+    part[1][w] = c1;
   }
-  if (k < n) {
-    const int64_t i = k % N, j = k / N;
-    if (i == 0 || i == N - 1 || j == 0 || j == N - 1) bits = 0;
-    px[k] = (uint8_t)bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long s0 = 0, s1 = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+      s0 += part[0][q];
+      s1 += part[1][q];
+    }
+    if (s0) atomicAdd(&counts[0], s0);
+    if (s1) atomicAdd(&counts[1], s1);
   }
 }
 
@@ -318,7 +339,8 @@ int classify(slsim_ctx c, Scratch& sc, int64_t N, int mode, const double* A_dev,
     return rc;
   launch_zero_u32(st, (uint32_t*)dcnt, 4);
   const unsigned blocks = (unsigned)((n + 255) / 256);
-  k_crit_pixels<<<blocks, 256, 0, st>>>(A_dev, mult_dev, N, mode, invmag, dpx, dcnt);
+  k_crit_pixels<<<(unsigned)std::min<int64_t>(blocks, kPixBlocks), 256, 0, st>>>(
+      (const double2*)A_dev, mult_dev, N, mode, invmag, dpx, dcnt);
   SLSIM_HIP_CHECK(hipGetLastError());
   k_crit_edges<<<blocks, 256, 0, st>>>(dpx, N, nbits, dedge, out.nwords);
   SLSIM_HIP_CHECK(hipGetLastError());

@@ -15,6 +15,7 @@
 // every image is the reference's to the bit.
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "slsim_internal.h"
@@ -22,8 +23,6 @@
 namespace slsim {
 
 namespace {
-
-constexpr int kSrcTile = 256;
 
 // hit: cell k, code = triangle (bit 0) | case << 1 with case 0 inside,
 // 1..3 on the triangle's first/second/third edge, 4 on its first vertex
@@ -103,57 +102,78 @@ __global__ __launch_bounds__(256) void k_count_grid_images(const double* __restr
   }
 }
 
+// Sources binned on a G x G grid over their extent (CSR by bin, row-major):
+// bin (bx, by) holds sorted positions sy[start[by G + bx] .. start[by G + bx + 1])
+// of sources src[...]; sources with a non-finite coordinate are listed apart.
+struct SourceBins {
+  const double2* sy;
+  const int32_t* src;
+  const int32_t* start;
+  const int32_t* odd;      // non-finite sources
+  int32_t nodd;
+  int32_t G;
+  double x0, y0, inv_w;    // inverse bin width (same both axes)
+};
+
+__device__ inline void add_hit(uint32_t* count, ImageHit* hits, uint32_t cap, int32_t s,
+                               uint32_t cell, uint32_t code) {
+  const uint32_t at = atomicAdd(&count[s], 1u);
+  if (at < cap) hits[(int64_t)s * cap + at] = {cell, code};
+}
+
+// One thread per cell; each of its two triangles tests the sources in the
+// bins its source-plane bounding box covers (widened by a bin each side); a
+// source outside that box fails the sign tests unless the triangle is
+// degenerate to 1e-16 of its size.  A triangle with a non-finite corner,
+// whose sign tests give zeros, tests every source, and every triangle tests
+// the non-finite sources, as the reference's all-pairs search does.
 __global__ __launch_bounds__(256) void k_find_images(const double* __restrict__ y, int64_t N,
                                                      const double* __restrict__ ys, int nsrc,
-                                                     uint32_t cap, uint32_t* __restrict__ count,
+                                                     SourceBins b, uint32_t cap,
+                                                     uint32_t* __restrict__ count,
                                                      ImageHit* __restrict__ hits) {
-  __shared__ double sy[2 * kSrcTile];
+#pragma clang fp contract(off)
   const int64_t ncell = (N - 1) * (N - 1);
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const bool valid = c < ncell;
-  int64_t k = 0;
-  double p0x = 0, p0y = 0, p1x = 0, p1y = 0, p2x = 0, p2y = 0, p3x = 0, p3y = 0;
-  if (valid) {
-    const int64_t j = c / (N - 1), i = c % (N - 1);
-    k = j * N + i;
-    const int64_t k1 = k + N + 1, k2 = k + 1, k3 = k + N;
-    p0x = y[2 * k];  p0y = y[2 * k + 1];
-    p1x = y[2 * k1]; p1y = y[2 * k1 + 1];
-    p2x = y[2 * k2]; p2y = y[2 * k2 + 1];
-    p3x = y[2 * k3]; p3y = y[2 * k3 + 1];
-  }
-  for (int t0 = 0; t0 < nsrc; t0 += kSrcTile) {
-    const int nt = min(kSrcTile, nsrc - t0);
-    __syncthreads();
-    for (int q = threadIdx.x; q < 2 * nt; q += blockDim.x) sy[q] = ys[2 * t0 + q];
-    __syncthreads();
-    if (!valid) continue;
-    for (int s = 0; s < nt; ++s) {
-      const double q0 = sy[2 * s], q1 = sy[2 * s + 1];
-      // triangle (k, k1, k2)
-      const int sig1 = side(q0, q1, p0x, p0y, p1x, p1y);
-      int sig2 = side(q0, q1, p1x, p1y, p2x, p2y);
-      int sig3 = side(q0, q1, p2x, p2y, p0x, p0y);
-      int sum = sig1 + sig2 + sig3;
-      int code = -1;
-      if (sum == 3 || sum == -3) code = 0;
-      else if (sum == 2) code = sig1 == 0 ? 1 : (sig2 == 0 ? 2 : 3);
-      else if (sum != -2 && sig1 == 0 && sig3 == 0) code = 4;
-      if (code >= 0) {
-        const uint32_t at = atomicAdd(&count[t0 + s], 1u);
-        if (at < cap) hits[(int64_t)(t0 + s) * cap + at] = {(uint32_t)k, (uint32_t)(code << 1)};
+  if (c >= ncell) return;
+  const int64_t k = (c / (N - 1)) * N + c % (N - 1);
+  const double p0x = y[2 * k], p0y = y[2 * k + 1];
+  const double p1x = y[2 * (k + N + 1)], p1y = y[2 * (k + N + 1) + 1];
+  for (int t = 0; t < 2; ++t) {
+    const int64_t k2 = t == 0 ? k + 1 : k + N;
+    const double p2x = y[2 * k2], p2y = y[2 * k2 + 1];
+    const uint32_t code0 = (uint32_t)t;
+    const bool first = t == 0;
+    const bool finite = isfinite(p0x) && isfinite(p0y) && isfinite(p1x) && isfinite(p1y) &&
+                        isfinite(p2x) && isfinite(p2y);
+    if (!finite) {
+      for (int s = 0; s < nsrc; ++s) {
+        const int cd = tri_code(ys[2 * s], ys[2 * s + 1], p0x, p0y, p1x, p1y, p2x, p2y, first);
+        if (cd >= 0) add_hit(count, hits, cap, s, (uint32_t)k, ((uint32_t)cd << 1) | code0);
       }
-      // triangle (k, k1, k3); the edge k -> k1 is shared (same sig1)
-      sig2 = side(q0, q1, p1x, p1y, p3x, p3y);
-      sig3 = side(q0, q1, p3x, p3y, p0x, p0y);
-      sum = sig1 + sig2 + sig3;
-      code = -1;
-      if (sum == 3 || sum == -3) code = 0;
-      else if (sum == 2) code = sig1 == 0 ? 1 : (sig2 == 0 ? 2 : 3);
-      if (code >= 0) {
-        const uint32_t at = atomicAdd(&count[t0 + s], 1u);
-        if (at < cap) hits[(int64_t)(t0 + s) * cap + at] = {(uint32_t)k, (uint32_t)((code << 1) | 1)};
+      continue;
+    }
+    const double top = (double)(b.G - 1);
+    const double fx0 = (fmin(fmin(p0x, p1x), p2x) - b.x0) * b.inv_w;
+    const double fx1 = (fmax(fmax(p0x, p1x), p2x) - b.x0) * b.inv_w;
+    const double fy0 = (fmin(fmin(p0y, p1y), p2y) - b.y0) * b.inv_w;
+    const double fy1 = (fmax(fmax(p0y, p1y), p2y) - b.y0) * b.inv_w;
+    if (b.G > 0 && !(fx1 < -1.0 || fy1 < -1.0 || fx0 > top + 1.0 || fy0 > top + 1.0)) {
+      const int bx0 = (int)fmax(floor(fx0) - 1.0, 0.0), bx1 = (int)fmin(floor(fx1) + 1.0, top);
+      const int by0 = (int)fmax(floor(fy0) - 1.0, 0.0), by1 = (int)fmin(floor(fy1) + 1.0, top);
+      for (int by = by0; by <= by1; ++by) {
+        const int32_t e = b.start[by * b.G + bx1 + 1];
+        for (int32_t q = b.start[by * b.G + bx0]; q < e; ++q) {
+          const double2 sq = b.sy[q];
+          const int cd = tri_code(sq.x, sq.y, p0x, p0y, p1x, p1y, p2x, p2y, first);
+          if (cd >= 0) add_hit(count, hits, cap, b.src[q], (uint32_t)k, ((uint32_t)cd << 1) | code0);
+        }
       }
+    }
+    for (int32_t q = 0; q < b.nodd; ++q) {
+      const int32_t s = b.odd[q];
+      const int cd = tri_code(ys[2 * s], ys[2 * s + 1], p0x, p0y, p1x, p1y, p2x, p2y, first);
+      if (cd >= 0) add_hit(count, hits, cap, s, (uint32_t)k, ((uint32_t)cd << 1) | code0);
     }
   }
 }
@@ -231,9 +251,76 @@ int find_images(slsim_ctx c, size_t N, double c0, double c1, double range, const
     return rc;
   SLSIM_HIP_CHECK(hipMemcpyAsync(dys, ys, 16 * nsrc, hipMemcpyHostToDevice, st));
   launch_zero_u32(st, dcount, (int)nsrc);
+  // bin the finite sources: about one a bin, at most 1024 x 1024 bins
+  SourceBins bins{};
+  std::vector<int32_t> odd, order;
+  double bx0 = HUGE_VAL, bx1 = -HUGE_VAL, by0 = HUGE_VAL, by1 = -HUGE_VAL;
+  for (size_t q = 0; q < nsrc; ++q) {
+    const double a = ys[2 * q], b = ys[2 * q + 1];
+    if (!std::isfinite(a) || !std::isfinite(b)) {
+      odd.push_back((int32_t)q);
+      continue;
+    }
+    order.push_back((int32_t)q);
+    bx0 = std::min(bx0, a); bx1 = std::max(bx1, a);
+    by0 = std::min(by0, b); by1 = std::max(by1, b);
+  }
+  const int64_t nfin = (int64_t)order.size();
+  int G = 0;
+  double inv_w = 0.0;
+  if (nfin > 0) {
+    G = (int)std::min<double>(1024.0, std::max(1.0, std::ceil(std::sqrt((double)nfin))));
+    const double ext = std::max(bx1 - bx0, by1 - by0);
+    inv_w = ext > 0 && std::isfinite(ext) ? (double)G / ext : 0.0;
+    if (!(inv_w > 0 && std::isfinite(inv_w))) {
+      G = 1;
+      inv_w = 0.0;
+    }
+  }
+  const int64_t nbins = (int64_t)G * G;
+  std::vector<int32_t> bstart((size_t)nbins + 1, 0), bin((size_t)nfin);
+  for (int64_t q = 0; q < nfin; ++q) {
+    const int32_t s = order[q];
+    const double fx = (ys[2 * s] - bx0) * inv_w, fy = (ys[2 * s + 1] - by0) * inv_w;
+    const int ix = (int)std::min<double>(G - 1, std::max(0.0, std::floor(fx)));
+    const int iy = (int)std::min<double>(G - 1, std::max(0.0, std::floor(fy)));
+    bin[q] = iy * G + ix;
+    ++bstart[bin[q] + 1];
+  }
+  for (int64_t q = 0; q < nbins; ++q) bstart[q + 1] += bstart[q];
+  std::vector<int32_t> fill(bstart.begin(), bstart.end() - 1), bsrc((size_t)nfin);
+  std::vector<double> bsy(2 * (size_t)nfin);
+  for (int64_t q = 0; q < nfin; ++q) {
+    const int32_t at = fill[bin[q]]++;
+    bsrc[at] = order[q];
+    bsy[2 * at] = ys[2 * order[q]];
+    bsy[2 * at + 1] = ys[2 * order[q] + 1];
+  }
+  int32_t *dsrc, *dstart, *dodd;
+  double* dbsy;
+  if ((rc = sc.get(&dsrc, (size_t)nfin)) || (rc = sc.get(&dstart, (size_t)nbins + 1)) ||
+      (rc = sc.get(&dodd, odd.size())) || (rc = sc.get(&dbsy, 2 * (size_t)nfin)))
+    return rc;
+  if (nfin > 0) {
+    SLSIM_HIP_CHECK(hipMemcpyAsync(dsrc, bsrc.data(), 4 * nfin, hipMemcpyHostToDevice, st));
+    SLSIM_HIP_CHECK(hipMemcpyAsync(dbsy, bsy.data(), 16 * nfin, hipMemcpyHostToDevice, st));
+  }
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dstart, bstart.data(), 4 * (nbins + 1), hipMemcpyHostToDevice,
+                                 st));
+  if (!odd.empty())
+    SLSIM_HIP_CHECK(hipMemcpyAsync(dodd, odd.data(), 4 * odd.size(), hipMemcpyHostToDevice, st));
+  bins.sy = (const double2*)dbsy;
+  bins.src = dsrc;
+  bins.start = dstart;
+  bins.odd = dodd;
+  bins.nodd = (int32_t)odd.size();
+  bins.G = G;
+  bins.x0 = bx0;
+  bins.y0 = by0;
+  bins.inv_w = inv_w;
   const int64_t ncell = (int64_t)(N - 1) * (int64_t)(N - 1);
   k_find_images<<<(unsigned)((ncell + 255) / 256), 256, 0, st>>>(y_dev, (int64_t)N, dys,
-                                                                  (int)nsrc, (uint32_t)cap,
+                                                                  (int)nsrc, bins, (uint32_t)cap,
                                                                   dcount, dhits);
   SLSIM_HIP_CHECK(hipGetLastError());
   std::vector<uint32_t> cnt(nsrc);

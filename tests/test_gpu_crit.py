@@ -159,3 +159,25 @@ def test_boundaries_of_caustics_singular_and_nonfinite(ctx):
     for a, b in zip(c, c0):
         assert np.array_equal(a, b)
     f.close()
+
+
+def test_find_images_nonfinite_corner_and_spread_sources(ctx):
+    # GridMap::find_images with a NaN grid position (its triangles test every
+    # source) and sources spread over and beyond the grid, one exactly on a
+    # grid vertex: equal to the oracle's all-pairs search
+    N, rg = 96, 2.0
+    x = O.gridmap_positions(N, (0.0, 0.0), rg)
+    y, A = _isothermal(x, [((0.01, -0.02), 0.4, 0.05)])
+    y[2 * N + 7] = (np.nan, 0.0)
+    f = _Field(ctx, N, (0.0, 0.0), rg, y, A)
+    f.find_images = S.GridMap.find_images.__get__(f)
+    rng = np.random.default_rng(21)
+    ys = np.concatenate([rng.uniform(-1.5, 1.5, size=(300, 2)), y[[500, 4000]],
+                         rng.normal(size=(50, 2)) * 0.05])
+    print("case: find_images NaN grid corner, 352 sources", flush=True)
+    rays, mult, tris = f.find_images(ys, 64)
+    r0, m0, t0 = O.find_images(N, x, y, A, ys)
+    assert np.array_equal(mult, m0) and np.array_equal(tris, t0)
+    assert np.array_equal(rays, r0, equal_nan=True)
+    assert mult.max() >= 3
+    f.close()
