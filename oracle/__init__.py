@@ -51,6 +51,9 @@ _sigs = {
     "orc_lens_add_halo": (None, [c_vp, c_vp, c_f]),
     "orc_halotree_build": (c_vp, [P(c_d), P(c_d), P(c_f), P(c_f), P(c_f), c_sz, c_f, c_f,
                                   P(c_int)]),
+    "orc_halotree_build_field": (c_vp, [P(c_d), P(c_d), P(c_f), P(c_f), P(c_f), c_sz, c_d, c_d,
+                                         c_d, c_int, P(c_int)]),
+    "orc_lens_add_field_plane": (None, [c_vp, c_vp, c_d]),
     "orc_halotree_free": (None, [c_vp]),
     "orc_halotree_force": (None, [c_vp, P(c_d), c_sz, c_int, P(c_d)]),
     "orc_nfw_force": (None, [c_f, c_f, c_f, P(c_d), c_sz, P(c_d)]),
@@ -199,6 +202,32 @@ class HaloTree:
             _lib.orc_halotree_free(self.t)
 
 
+class FieldTree(HaloTree):
+    """LensPlaneTree's TreeQuadHalos (MultiPlane/planes.cpp:13-19): NFW halos
+    with sigma_background; with inv_screening_scale != 0 bucket 5, theta 0.1,
+    the periodic buffer and screening, else theta 0.5 (theta and periodic may
+    be given to test the parts apart)."""
+
+    def __init__(self, x, y, m, rsize, conc, sigma_background=0.0, inv_screening_scale=0.0,
+                 theta=None, periodic=None):
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.m = np.ascontiguousarray(m, dtype=np.float32)
+        self.rsize = np.ascontiguousarray(rsize, dtype=np.float32)
+        self.conc = np.ascontiguousarray(conc, dtype=np.float32)
+        scr = inv_screening_scale != 0
+        theta = (0.1 if scr else 0.5) if theta is None else theta
+        periodic = scr if periodic is None else periodic
+        st = c_int()
+        self.t = _lib.orc_halotree_build_field(
+            _p(self.x), _p(self.y), _p(self.m, c_f), _p(self.rsize, c_f), _p(self.conc, c_f),
+            self.x.size, float(sigma_background), float(theta), float(inv_screening_scale),
+            int(bool(periodic)), ctypes.byref(st))
+        if st.value:
+            raise ValueError("halo tree build failed (halos in one place or out of the tables)")
+        self.owned = True
+
+
 def nfw_force(m, rsize, conc, xcm):
     """One LensHaloNFW's force_halo at positions relative to it: (n, 5)."""
     xcm = np.ascontiguousarray(xcm, dtype=np.float64).reshape(-1, 2)
@@ -223,6 +252,11 @@ class Lens:
 
     def __init__(self, zsource, Om=0.3, OL=0.7, h=0.7):
         self.L = _lib.orc_lens_create(zsource, Om, OL, h)
+
+    def addFieldPlane(self, tree, z):
+        """A LensPlaneTree at redshift z (double) made of a FieldTree."""
+        tree.owned = False
+        _lib.orc_lens_add_field_plane(self.L, tree.t, float(z))
 
     def moveinMainHalo(self, tree, z):
         tree.owned = False          # the lens owns it now

@@ -539,10 +539,14 @@ struct NfwHalo {
 };
 
 // ----------------------------------------------------------- TreeQuadHalos
-// include/quadTreeHalos.h:127-735 with LensHaloHalos' arguments (bucket 5,
-// no periodic buffer, no screening, no maximum range): pointer cells,
-// Utilities::quickPartition, the build and moments in the reference's order,
-// and walkTree_iter's depth-first walk.
+// include/quadTreeHalos.h:127-735 with bucket 5 and no maximum range:
+// pointer cells, Utilities::quickPartition, the build and moments in the
+// reference's order, walkTree_iter's depth-first walk with the screening
+// factor exp(-r^2 s^2) on the cells' far terms (quadTreeHalos.h:679-733), and
+// force2D's periodic buffer (nine walks at the ray shifted by the halos'
+// extents, :566-580).  LensHaloHalos uses no screening and no buffer;
+// LensPlaneTree (MultiPlane/planes.cpp:13-19) uses both, or neither with
+// theta 0.5.
 struct HaloTree {
   struct Branch {
     unsigned long* particles = nullptr;
@@ -560,6 +564,9 @@ struct HaloTree {
   std::vector<unsigned long> index;
   std::vector<double> work;
   double inv_area, theta;
+  double s2 = 0;               // inv_screening_scale2
+  bool periodic = false;
+  double xl = 0, yl = 0;       // original_xl, original_yl
   int bucket = 5;
   std::unique_ptr<Branch> top;
 
@@ -575,8 +582,10 @@ struct HaloTree {
     }
   }
 
-  HaloTree(std::vector<NfwHalo> h, float my_inv_area, float my_theta)
-      : halos(std::move(h)), inv_area(my_inv_area), theta(my_theta) {
+  HaloTree(std::vector<NfwHalo> h, double my_inv_area, double my_theta, double inv_screening = 0,
+           bool my_periodic = false)
+      : halos(std::move(h)), inv_area(my_inv_area), theta(my_theta),
+        s2(inv_screening * inv_screening), periodic(my_periodic) {
     const unsigned long N = halos.size();
     index.resize(N);
     for (unsigned long i = 0; i < N; ++i) index[i] = i;
@@ -590,6 +599,8 @@ struct HaloTree {
       }
     if (N == 1) { p1[0] = -0.25; p1[1] = -0.25; p2[0] = 0.25; p2[1] = 0.25; }
     const double lengths[2] = {p2[0] - p1[0], p2[1] - p1[1]};
+    xl = lengths[0];
+    yl = lengths[1];
     if (lengths[0] == 0 || lengths[1] == 0) throw std::invalid_argument("particles in same place.");
     const int j = lengths[0] > lengths[1] ? 1 : 0;
     p2[j] = p1[j] + lengths[!j];
@@ -686,6 +697,8 @@ struct HaloTree {
     for (unsigned long t : b->big) {
       const NfwHalo& h = halos[t];
       const double xcm[2] = {h.pos[0] - ray[0], h.pos[1] - ray[1]};
+      // LensHalo::force_halo(..., subtract_point = false, screening) leaves
+      // the screening unused for a symmetric halo (lens_halos.cpp:1555-1600)
       h.force_halo(alpha, kappa, gamma, xcm);
       const double mass = h.mass;
       const double prefac = mass * inv_area;
@@ -697,7 +710,7 @@ struct HaloTree {
     const double xcm[2] = {b->center[0] - ray[0], b->center[1] - ray[1]};
     const double rcm2cell = xcm[0] * xcm[0] + xcm[1] * xcm[1];
     if (rcm2cell > b->r2crit) {
-      const double screening = 1.0;
+      const double screening = exp(-rcm2cell * s2);
       double tmp = b->mass * (inv_area - screening / rcm2cell / PI);
       alpha[0] += tmp * xcm[0];
       alpha[1] += tmp * xcm[1];
@@ -721,7 +734,15 @@ struct HaloTree {
     alpha[0] = alpha[1] = gamma[0] = gamma[1] = gamma[2] = 0.0;
     *kappa = 0.0;
     if (halos.empty()) return;
-    walk(top.get(), ray, alpha, kappa, gamma);
+    if (periodic) {
+      for (int i = -1; i < 2; ++i)
+        for (int j = -1; j < 2; ++j) {
+          const double r[2] = {ray[0] + i * xl, ray[1] + j * yl};
+          walk(top.get(), r, alpha, kappa, gamma);
+        }
+    } else {
+      walk(top.get(), ray, alpha, kappa, gamma);
+    }
   }
 
   // every halo with its background term, no tree
@@ -782,6 +803,9 @@ struct Lens {
   std::vector<std::unique_ptr<Halo>> halos;
   std::vector<std::unique_ptr<Plane>> main_planes;
   std::vector<double> main_Dl, main_z;
+  // field planes (LensPlaneTree), redshifts as doubles (lens.cpp:910-918)
+  std::vector<std::unique_ptr<Plane>> field_planes;
+  std::vector<double> field_Dl, field_z;
   // combined
   std::vector<Plane*> planes;
   std::vector<double> plane_redshifts, Dl, dDl;
@@ -816,11 +840,13 @@ struct Lens {
     }
   }
 
-  // combinePlanes, MultiPlane/lens.cpp:2397-2515 (no field planes)
+  // combinePlanes, MultiPlane/lens.cpp:2397-2515: field then main distances,
+  // sorted together (ties keep field planes first)
   void combinePlanes() {
     planes.clear();
     plane_redshifts.clear();
-    Dl = main_Dl;
+    Dl = field_Dl;
+    Dl.insert(Dl.end(), main_Dl.begin(), main_Dl.end());
     dDl.clear();
     std::vector<size_t> idx(Dl.size());
     for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
@@ -828,9 +854,15 @@ struct Lens {
     std::sort(Dl.begin(), Dl.end());
     for (size_t i = 1; i < Dl.size(); ++i)
       if ((Dl[i] - Dl[i - 1]) < MIN_PLANE_DIST) Dl[i] = Dl[i - 1] + MIN_PLANE_DIST;
+    const size_t nf = field_Dl.size();
     for (size_t i : idx) {
-      plane_redshifts.push_back(main_z[i]);
-      planes.push_back(main_planes[i].get());
+      if (i < nf) {
+        plane_redshifts.push_back(field_z[i]);
+        planes.push_back(field_planes[i].get());
+      } else {
+        plane_redshifts.push_back(main_z[i - nf]);
+        planes.push_back(main_planes[i - nf].get());
+      }
     }
     if (!plane_redshifts.empty() && zsource <= plane_redshifts.back())
       zsource = plane_redshifts.back() + 0.1;
@@ -838,6 +870,19 @@ struct Lens {
     Dl.push_back(cosmo.coorDist(0, zsource));
     dDl.push_back(Dl[0]);
     for (size_t i = 1; i < Dl.size(); ++i) dDl.push_back(Dl[i] - Dl[i - 1]);
+  }
+
+  // a LensPlaneTree field plane at redshift z: its force is the tree's
+  // force2D (planes.cpp:65-68), i.e. a plane of one halo-of-halos at (0,0)
+  void addFieldPlane(std::unique_ptr<Halo> h, double z) {
+    Halo* raw = h.get();
+    halos.push_back(std::move(h));
+    auto p = std::make_unique<Plane>();
+    p->halos.push_back(raw);
+    field_planes.push_back(std::move(p));
+    field_z.push_back(z);
+    field_Dl.push_back(cosmo.coorDist(0, z));
+    combinePlanes();
   }
 
   // moveinMainHalo(addplanes = true), include/lens.h:172-185
@@ -1292,6 +1337,21 @@ void* orc_halotree_build(const double* x, const double* y, const float* m, const
     return nullptr;
   }
 }
+// LensPlaneTree's tree: inv_area (sigma_background), theta, screening and
+// the periodic buffer as doubles
+void* orc_halotree_build_field(const double* x, const double* y, const float* m,
+                               const float* rsize, const float* conc, size_t n, double inv_area,
+                               double theta, double inv_screening, int periodic, int* status) {
+  try {
+    std::vector<orc::NfwHalo> h;
+    for (size_t i = 0; i < n; ++i) h.emplace_back(x[i], y[i], m[i], rsize[i], conc[i]);
+    *status = 0;
+    return new orc::HaloTree(std::move(h), inv_area, theta, inv_screening, periodic != 0);
+  } catch (const std::exception&) {
+    *status = 1;
+    return nullptr;
+  }
+}
 void orc_halotree_free(void* t) { delete static_cast<orc::HaloTree*>(t); }
 // force2D (direct = 0) or the sum over every halo (direct = 1), 5 per ray
 void orc_halotree_force(void* tp, const double* rays, size_t n, int direct, double* out) {
@@ -1327,6 +1387,15 @@ void orc_lens_add_halotree(void* lp, void* tree, float z) {
   h->htree.reset(static_cast<orc::HaloTree*>(tree));
   h->z = z;
   L->moveinMainHalo(std::move(h));
+}
+
+// a LensPlaneTree field plane (the lens takes the tree) at redshift z
+void orc_lens_add_field_plane(void* lp, void* tree, double z) {
+  auto* L = static_cast<orc::Lens*>(lp);
+  auto h = std::make_unique<orc::Halo>();
+  h->htree.reset(static_cast<orc::HaloTree*>(tree));
+  h->z = (float)z;
+  L->addFieldPlane(std::move(h), z);
 }
 
 int orc_lens_nplanes(void* lp) { return (int)static_cast<orc::Lens*>(lp)->planes.size(); }

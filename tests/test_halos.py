@@ -140,3 +140,59 @@ def test_tree_background_sheet():
         dk = b[:, 2] - a[:, 2]
         assert np.allclose(dk, -float(np.float32(ia)) * float(np.sum(m.astype(np.float64))),
                            rtol=1e-9, atol=0)
+
+
+# ------------------------------------------------ LensPlaneTree field trees
+def test_field_tree_without_screening_is_the_halos_tree():
+    x, y, m, rs, c = _halos(800, 7)
+    rays = np.random.default_rng(8).normal(size=(200, 2)) * 0.3
+    a = O.HaloTree(x, y, m, rs, c, inv_area=0.25, theta=0.5).force(rays)
+    b = O.FieldTree(x, y, m, rs, c, sigma_background=0.25).force(rays)   # theta 0.5, no buffer
+    assert np.array_equal(a, b)
+
+
+def test_field_tree_periodic_buffer_is_nine_shifted_walks():
+    x, y, m, rs, c = _halos(600, 9)
+    rays = np.random.default_rng(10).normal(size=(60, 2)) * 0.3
+    xl, yl = x.max() - x.min(), y.max() - y.min()
+    one = O.FieldTree(x, y, m, rs, c, 0.1, theta=0.3, periodic=False)
+    both = O.FieldTree(x, y, m, rs, c, 0.1, theta=0.3, periodic=True).force(rays)
+    want = sum(one.force(rays + [i * xl, j * yl]) for i in (-1, 0, 1) for j in (-1, 0, 1))
+    assert np.allclose(both, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+
+
+def test_field_tree_screening_scales_far_cells():
+    # two compact clusters 5 apart: a ray at the first sees the second as one
+    # far cell, whose monopole and quadrupole are scaled by exp(-r^2 s^2);
+    # the near halos' own forces and the background are not screened
+    rng = np.random.default_rng(11)
+    n = 60
+    x = np.concatenate([rng.normal(size=n) * 0.01, 5.0 + rng.normal(size=n) * 1e-3])
+    y = np.concatenate([rng.normal(size=n) * 0.01, rng.normal(size=n) * 1e-3])
+    m = (10 ** rng.uniform(10, 12, 2 * n)).astype(np.float32)
+    rs = np.full(2 * n, 1e-4, np.float32)
+    c = np.full(2 * n, 8.0, np.float32)
+    ray = np.array([[0.003, -0.002]])
+    s = 0.1
+    plain = O.FieldTree(x, y, m, rs, c, 0.7, theta=0.5, periodic=False).force(ray)[0]
+    scr = O.FieldTree(x, y, m, rs, c, 0.7, inv_screening_scale=s, theta=0.5,
+                      periodic=False).force(ray)[0]
+    far = O.FieldTree(x[n:], y[n:], m[n:], rs[n:], c[n:], 0.0, theta=1e-6).force(ray, direct=True)[0]
+    com = np.array([np.sum(m[n:] * x[n:]), np.sum(m[n:] * y[n:])]) / np.sum(m[n:].astype(float))
+    f = np.exp(-np.sum((com - ray[0]) ** 2) * s * s)
+    want = plain + (f - 1) * far
+    assert np.allclose(scr[:2], want[:2], rtol=1e-6, atol=0)
+    assert np.allclose(scr[3:], want[3:], rtol=1e-5, atol=0)
+    assert scr[2] == plain[2]                       # kappa: the background only
+
+
+def test_field_plane_in_lens_tables():
+    x, y, m, rs, c = _halos(100, 12)
+    L = O.Lens(2.0)
+    L.moveinMainHalo(O.HaloTree(x, y, m, rs, c), 0.5)
+    L.addFieldPlane(O.FieldTree(x, y, m, rs, c, 0.0, 0.2), 0.31234567)
+    L.addFieldPlane(O.FieldTree(x, y, m, rs, c, 0.0, 0.2), 1.2)
+    t = L.tables()
+    assert t["nplanes"] == 3
+    # field redshifts stay doubles; the main plane's is a float
+    assert list(t["z"][:3]) == [0.31234567, float(np.float32(0.5)), 1.2]
