@@ -93,6 +93,19 @@ int slsim_plane_destroy(slsim_plane plane);
 int slsim_halos_create_nfw(slsim_ctx ctx, const double* x, const double* y, const float* mass,
                            const float* rsize, const float* concentration, size_t n,
                            float inv_area, float theta_force, slsim_plane* out);
+/* LensPlaneTree(z, halos, Nhalos, sigma_background, inv_screening_scale)
+ * (MultiPlane/planes.cpp:13-19, force :65-68): a field plane of n symmetric
+ * NFW halos (as slsim_halos_create_nfw) on a TreeQuadHalos with
+ * inv_area = sigma_background; with inv_screening_scale != 0 the tree has
+ * bucket 5, theta 0.1, the periodic buffer (nine walks at the ray shifted by
+ * the halos' extents) and its far cells screened by
+ * exp(-r^2 inv_screening_scale^2) (include/quadTreeHalos.h:557-735), else
+ * theta 0.5 and neither.  It goes into a lens with slsim_lens_add_field_plane;
+ * slsim_plane_force gives its force. */
+int slsim_plane_tree_create_nfw(slsim_ctx ctx, const double* x, const double* y,
+                                const float* mass, const float* rsize,
+                                const float* concentration, size_t n, double sigma_background,
+                                double inv_screening_scale, slsim_plane* out);
 
 typedef struct {
   int64_t n_particles;
@@ -125,6 +138,12 @@ int slsim_lens_destroy(slsim_lens lens);
 /* Lens::moveinMainHalo(halo, addplanes=true) (include/lens.h:172-185): the
  * lens takes ownership of `plane`, placed at float redshift z. */
 int slsim_lens_add_halo(slsim_lens lens, slsim_plane plane, float z);
+/* A field plane (MultiPlane/lens.cpp:349-455 builds them in createFieldPlanes)
+ * at redshift z (double, as Lens::setFieldDist leaves it): the lens takes
+ * ownership of `plane`, a slsim_plane_tree_create_nfw plane, and combinePlanes
+ * (lens.cpp:2397-2515) sorts it with the main planes by distance (ahead of a
+ * main plane at the same distance). */
+int slsim_lens_add_field_plane(slsim_lens lens, slsim_plane plane, double z);
 /* Number of lens planes, and the plane tables plane_redshifts, Dl, dDl (each
  * nplanes + 1 long, the source pseudo-plane last) and charge = 4 pi G. */
 int slsim_lens_tables(slsim_lens lens, int* nplanes, double* z, double* Dl, double* dDl,

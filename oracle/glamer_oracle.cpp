@@ -745,17 +745,24 @@ struct HaloTree {
     }
   }
 
-  // every halo with its background term, no tree
+  // every halo with its background term, no tree (nine shifted copies of the
+  // ray with the periodic buffer)
   void direct(const double* ray, double* alpha, double* kappa, double* gamma) const {
     alpha[0] = alpha[1] = gamma[0] = gamma[1] = gamma[2] = 0.0;
     *kappa = 0.0;
-    for (const NfwHalo& h : halos) {
-      const double xcm[2] = {h.pos[0] - ray[0], h.pos[1] - ray[1]};
-      h.force_halo(alpha, kappa, gamma, xcm);
-      alpha[0] += h.mass * inv_area * xcm[0];
-      alpha[1] += h.mass * inv_area * xcm[1];
-      *kappa -= h.mass * inv_area;
-    }
+    for (int i = -1; i < 2; ++i)
+      for (int j = -1; j < 2; ++j) {
+        if (!periodic && (i != 0 || j != 0)) continue;
+        const double r[2] = {periodic ? ray[0] + i * xl : ray[0],
+                             periodic ? ray[1] + j * yl : ray[1]};
+        for (const NfwHalo& h : halos) {
+          const double xcm[2] = {h.pos[0] - r[0], h.pos[1] - r[1]};
+          h.force_halo(alpha, kappa, gamma, xcm);
+          alpha[0] += h.mass * inv_area * xcm[0];
+          alpha[1] += h.mass * inv_area * xcm[1];
+          *kappa -= h.mass * inv_area;
+        }
+      }
   }
 };
 

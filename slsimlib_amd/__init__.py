@@ -250,6 +250,38 @@ class LensHaloHalos:
     __del__ = LensHaloParticles.__del__
 
 
+class LensPlaneTree:
+    """LensPlaneTree(z, halos, Nhalos, sigma_background, inv_screening_scale)
+    (MultiPlane/planes.cpp:13-19): a field plane of symmetric NFW halos at
+    physical positions (x, y) on a TreeQuadHalos with sigma_background as its
+    background; inv_screening_scale != 0 screens its far cells and adds the
+    periodic buffer (theta 0.1), else theta 0.5.  z is kept as a double.  Goes
+    into a Lens with addFieldPlane; force() is LensPlaneTree::force's tree
+    part (TreeQuadHalos::force2D)."""
+
+    def __init__(self, ctx, z, x, y, mass, rsize, concentration, sigma_background=0.0,
+                 inv_screening_scale=0.0):
+        self.ctx = ctx
+        self.z = float(z)
+        self.owned_by_lens = False
+        xs = np.ascontiguousarray(x, dtype=np.float64)
+        ys = np.ascontiguousarray(y, dtype=np.float64)
+        m = np.ascontiguousarray(mass, dtype=np.float32)
+        r = np.ascontiguousarray(rsize, dtype=np.float32)
+        c = np.ascontiguousarray(concentration, dtype=np.float32)
+        self.h = _abi.slsim_plane()
+        _check(lib.slsim_plane_tree_create_nfw(
+            ctx.h, _ptr(xs, ctypes.c_double), _ptr(ys, ctypes.c_double), _ptr(m, ctypes.c_float),
+            _ptr(r, ctypes.c_float), _ptr(c, ctypes.c_float), ctypes.c_size_t(xs.size),
+            float(sigma_background), float(inv_screening_scale), ctypes.byref(self.h)))
+
+    info = LensHaloParticles.info
+    force = LensHaloParticles.force
+    direct_sum = LensHaloParticles.direct_sum
+    close = LensHaloParticles.close
+    __del__ = LensHaloParticles.__del__
+
+
 class Lens:
     """Lens(seed, zs, cosmo) with flat LCDM (MultiPlane/lens.cpp:59-77)."""
 
@@ -260,6 +292,13 @@ class Lens:
         self.halos = []
         _check(lib.slsim_lens_create(ctx.h, float(zsource), float(Om), float(OL), float(h),
                                      ctypes.byref(self.h)))
+
+    def addFieldPlane(self, plane):
+        """A LensPlaneTree as a field plane at its (double) redshift; the
+        lens sorts it with the main planes (Lens::combinePlanes)."""
+        _check(lib.slsim_lens_add_field_plane(self.h, plane.h, float(plane.z)))
+        plane.owned_by_lens = True
+        self.halos.append(plane)
 
     def moveinMainHalo(self, halo):
         _check(lib.slsim_lens_add_halo(self.h, halo.h, ctypes.c_float(float(halo.z))))

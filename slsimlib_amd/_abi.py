@@ -195,6 +195,10 @@ SIGNATURES = {
     "slsim_curves_read": (c_int, [slsim_curves, P(c_i64), P(c_double), P(ctypes.c_int32),
                                   P(ctypes.c_int32)]),
     "slsim_curves_destroy": (c_int, [slsim_curves]),
+    "slsim_plane_tree_create_nfw": (c_int, [slsim_ctx, P(c_double), P(c_double), P(c_float),
+                                            P(c_float), P(c_float), c_size_t, c_double, c_double,
+                                            P(slsim_plane)]),
+    "slsim_lens_add_field_plane": (c_int, [slsim_lens, slsim_plane, c_double]),
     "slsim_halos_create_nfw": (c_int, [slsim_ctx, P(c_double), P(c_double), P(c_float),
                                        P(c_float), P(c_float), c_size_t, c_float, c_float,
                                        P(slsim_plane)]),

@@ -156,6 +156,11 @@ TreeDesc slsim_plane_s::desc() const {
   t.hbig = hbig;
   t.halos = halos;
   t.nfw = nfw;
+  t.hs2 = hs2;
+  t.hxl = hxl;
+  t.hyl = hyl;
+  t.hperiodic = hperiodic;
+  t.pad2 = 0;
   return t;
 }
 
@@ -253,6 +258,31 @@ int slsim_plane_create_f32(slsim_ctx ctx, const float* x, const float* y, const 
                            const float* h, size_t n, double theta, int bucket, double inv_area,
                            slsim_plane* out) {
   return plane_create<float>(ctx, x, y, m, h, n, theta, bucket, inv_area, out);
+}
+
+int slsim_plane_tree_create_nfw(slsim_ctx ctx, const double* x, const double* y,
+                                const float* mass, const float* rsize,
+                                const float* concentration, size_t n, double sigma_background,
+                                double inv_screening_scale, slsim_plane* out) {
+  if (!ctx || !out) return arg_error("null handle");
+  if (n == 0 || !x || !y || !mass || !rsize || !concentration) return arg_error("no halos");
+  if (n >= (size_t)1 << 22) return arg_error("more than 2^22 halos in one plane");
+  SLSIM_HIP_CHECK(hipSetDevice(ctx->device));
+  auto p = std::make_unique<slsim_plane_s>();
+  p->ctx = ctx;
+  // planes.cpp:13-19: screened trees take theta 0.1 and the periodic buffer,
+  // unscreened ones TreeQuadHalos' defaults (theta 0.5, no buffer)
+  const bool scr = inv_screening_scale != 0;
+  const int rc = build_halo_plane(p.get(), x, y, mass, rsize, concentration, n, sigma_background,
+                                  scr ? 0.1 : 0.5);
+  if (rc) {
+    plane_free_device(p.get());
+    return rc;
+  }
+  p->hs2 = inv_screening_scale * inv_screening_scale;
+  p->hperiodic = scr;
+  *out = p.release();
+  return SLSIM_OK;
 }
 
 int slsim_halos_create_nfw(slsim_ctx ctx, const double* x, const double* y, const float* mass,
@@ -469,17 +499,31 @@ double slsim_lens_s::coorDist(double z1, double z2) const {
   return c.coorDist(z1, z2);
 }
 
+// MultiPlane/lens.cpp:2397-2515: the field planes' distances then the main
+// planes', sorted together (a field plane ahead of a main plane at the same
+// distance)
 void slsim_lens_s::combinePlanes() {
   plane_redshifts.clear();
-  Dl = main_Dl;
+  combined.clear();
+  Dl = field_Dl;
+  Dl.insert(Dl.end(), main_Dl.begin(), main_Dl.end());
   dDl.clear();
-  order.resize(Dl.size());
+  std::vector<int> order(Dl.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return Dl[a] < Dl[b]; });
   std::sort(Dl.begin(), Dl.end());
   for (size_t i = 1; i < Dl.size(); ++i)
     if ((Dl[i] - Dl[i - 1]) < kMinPlaneDist) Dl[i] = Dl[i - 1] + kMinPlaneDist;
-  for (int i : order) plane_redshifts.push_back(main_z[i]);
+  const int nf = (int)field_Dl.size();
+  for (int i : order) {
+    if (i < nf) {
+      plane_redshifts.push_back(field_z[i]);
+      combined.push_back({field_planes[i]});
+    } else {
+      plane_redshifts.push_back(main_z[i - nf]);
+      combined.push_back(main_planes[i - nf]);
+    }
+  }
   if (!plane_redshifts.empty() && zsource <= plane_redshifts.back())
     zsource = plane_redshifts.back() + 0.1;
   plane_redshifts.push_back(zsource);
@@ -504,13 +548,13 @@ int slsim_lens_s::upload_trees() {
   tree_off.clear();
   tree_cnt.clear();
   tree_hcnt.clear();
-  for (int mi : order) {
+  for (const std::vector<slsim_plane>& trees : combined) {
     // particle trees first, then the halo-of-halos trees of the plane
     tree_off.push_back((int)all.size());
     int np = 0, nh = 0;
-    for (slsim_plane p : main_planes[mi])
+    for (slsim_plane p : trees)
       if (p->kind == kTreeParticles) { all.push_back(p->desc()); ++np; }
-    for (slsim_plane p : main_planes[mi])
+    for (slsim_plane p : trees)
       if (p->kind == kTreeHalos) { all.push_back(p->desc()); ++nh; }
     tree_cnt.push_back(np);
     tree_hcnt.push_back(nh);
@@ -747,6 +791,20 @@ int slsim_lens_add_halo(slsim_lens L, slsim_plane plane, float z) {
     L->main_z.insert(L->main_z.begin() + i, halo_z);
     mD.insert(mD.begin() + i, halo_Dl);
   }
+  L->combinePlanes();
+  return SLSIM_OK;
+}
+
+int slsim_lens_add_field_plane(slsim_lens L, slsim_plane plane, double z) {
+  if (!L || !plane) return arg_error("null handle");
+  if (plane->ctx != L->ctx) return arg_error("plane and lens belong to different contexts");
+  if (plane->kind != kTreeHalos) return arg_error("a field plane is a tree of halos");
+  for (slsim_plane p : L->owned)
+    if (p == plane) return arg_error("halo already in the lens");
+  L->owned.push_back(plane);
+  L->field_planes.push_back(plane);
+  L->field_z.push_back(z);
+  L->field_Dl.push_back(L->coorDist(0, z));
   L->combinePlanes();
   return SLSIM_OK;
 }

@@ -304,7 +304,7 @@ int slsim_batch_create(slsim_ctx ctx, int n, const slsim_lens* lenses, const int
     q->c1 = center[2 * i + 1];
     q->range = range[i];
     q->zs = zs ? zs[i] : 0.0;
-    for (auto& pl : lenses[i]->main_planes) {
+    for (auto& pl : lenses[i]->combined) {
       int64_t s = 0;
       for (slsim_plane p : pl) s += p->n;
       q->nplane_max = std::max(q->nplane_max, s);

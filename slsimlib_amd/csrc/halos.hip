@@ -182,8 +182,8 @@ void build_cell(HBuild& B, size_t ci, bool top, int level) {
 
 // Builds the plane's halo tree and copies it to HBM.
 int build_halo_plane(slsim_plane p, const double* x, const double* y, const float* mass,
-                     const float* rsize, const float* conc, size_t n, float inv_area,
-                     float theta_force) {
+                     const float* rsize, const float* conc, size_t n, double inv_area,
+                     double theta_force) {
   // LensHaloNFW(mass, Rsize, zlens, concentration, fratio 1, ...)
   // (MultiPlane/lens_halos.cpp:389-440): rscale = Rsize / c and xmax =
   // Rsize / rscale in float, gmax = g(xmax) from the table, Rmax = Rsize
@@ -233,6 +233,8 @@ int build_halo_plane(slsim_plane p, const double* x, const double* y, const floa
     p2[0] = p2[1] = 0.25;
   }
   const double lengths[2] = {p2[0] - p1[0], p2[1] - p1[1]};
+  p->hxl = lengths[0];   // original_xl, original_yl: the periodic buffer's shifts
+  p->hyl = lengths[1];
   if (lengths[0] == 0 || lengths[1] == 0) {
     set_error("particles in same place.");
     return SLSIM_ERR_SAME_PLACE;

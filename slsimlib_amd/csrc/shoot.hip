@@ -412,8 +412,11 @@ __device__ __forceinline__ uint64_t evalh64(const HNode& nd, uint64_t lvm, doubl
   const uint64_t farm = lanes_lt(nd.r2crit, r2) & lvm;
   if (lane_in(farm)) {
     const double inv = rcp64(r2);
-    const double i2p = inv * inv * kInvPI;
-    double tmp = nd.mass * (ia - inv * kInvPI);
+    // LensPlaneTree's screening exp(-r^2 s^2) of the far terms (:679-733);
+    // exactly 1 for other trees
+    const double scr = t.hs2 != 0.0 ? exp(-r2 * t.hs2) : 1.0;
+    const double i2p = inv * inv * kInvPI * scr;
+    double tmp = nd.mass * (ia - inv * kInvPI * scr);
     a.ax += tmp * dx;
     a.ay += tmp * dy;
     tmp = -2.0 * nd.mass * i2p;
@@ -469,6 +472,28 @@ __device__ void halos_direct64(const TreeDesc& t, double rx, double ry, bool act
   const cptr<HaloNFW> halos = as_const(t.halos);
   const cptr<NfwEntry> T = as_const(t.nfw);
   for (int i = 0; i < t.nhalos; ++i) halo_term(halos[i], T, rx, ry, t.inv_area, a);
+}
+
+// TreeQuadHalos::force2D (include/quadTreeHalos.h:557-592): the walk (or the
+// sum over every halo), with the periodic buffer nine of them at the ray
+// shifted by the halos' extents, x shift outer.
+template <bool DIRECT>
+__device__ void halo_tree_force64(const TreeDesc& t, double rx, double ry, bool active, Acc64& a,
+                                  int32_t* s_node, uint64_t* s_mask, uint32_t* err) {
+#pragma clang fp contract(off)
+  if (!t.hperiodic) {
+    if (DIRECT) halos_direct64(t, rx, ry, active, a);
+    else walk_halos64(t, rx, ry, active, a, s_node, s_mask, err);
+    return;
+  }
+#pragma unroll 1
+  for (int i = -1; i < 2; ++i)
+#pragma unroll 1
+    for (int j = -1; j < 2; ++j) {
+      const double sx = rx + i * t.hxl, sy = ry + j * t.hyl;
+      if (DIRECT) halos_direct64(t, sx, sy, active, a);
+      else walk_halos64(t, sx, sy, active, a, s_node, s_mask, err);
+    }
 }
 
 // ---------------------------------------------------------------- FP32 walk
@@ -940,8 +965,7 @@ __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, W
   for (int h = 0; h < nhtrees; ++h) {
     const TreeDesc t = as_const(jb.trees)[ntrees + h];
     Acc64 a;
-    if (MODE == SLSIM_DIRECT64) halos_direct64(t, rx, ry, active, a);
-    else walk_halos64(t, rx, ry, active, a, s_node, s_mask, err);
+    halo_tree_force64<MODE == SLSIM_DIRECT64>(t, rx, ry, active, a, s_node, s_mask, err);
     alpha0 += a.ax; alpha1 += a.ay; kappa += a.k; gamma0 += a.g0; gamma1 += a.g1;
   }
   for (int h = 0; h < (MODE == SLSIM_FP32C ? 0 : ntrees); ++h) {
@@ -1285,8 +1309,8 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_halo_force(
   const double rx = active ? rays[2 * i] : 0.0, ry = active ? rays[2 * i + 1] : 0.0;
   const TreeDesc t = *as_const(tree);
   Acc64 a;
-  if (direct) halos_direct64(t, rx, ry, active, a);
-  else walk_halos64(t, rx, ry, active, a, s_node[w], s_mask[w], err);
+  if (direct) halo_tree_force64<true>(t, rx, ry, active, a, s_node[w], s_mask[w], err);
+  else halo_tree_force64<false>(t, rx, ry, active, a, s_node[w], s_mask[w], err);
   if (active) {
     out[5 * i] = a.ax; out[5 * i + 1] = a.ay; out[5 * i + 2] = a.k;
     out[5 * i + 3] = a.g0; out[5 * i + 4] = a.g1;

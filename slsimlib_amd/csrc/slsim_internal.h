@@ -128,6 +128,11 @@ struct TreeDesc {
   const int32_t* hbig;       // big-halo lists (halo indices)
   const HaloNFW* halos;
   const NfwEntry* nfw;       // kNfwTable entries
+  // LensPlaneTree's walk options: screening exp(-r^2 hs2) on far cells, and
+  // the periodic buffer (nine walks at the ray shifted by the halos' extents)
+  double hs2, hxl, hyl;
+  int32_t hperiodic;
+  int32_t pad2;
 };
 
 // Per-slot ray state, structure of arrays (slot = bundle * 64 + lane).
@@ -253,6 +258,8 @@ struct slsim_plane_s {
   slsim::HaloNFW* halos = nullptr;
   slsim::NfwEntry* nfw = nullptr;
   std::vector<slsim::HaloNFW> halos_host;
+  double hs2 = 0, hxl = 0, hyl = 0;   // screening and the buffer's shifts (field planes)
+  int hperiodic = 0;
   // build products kept for export and direct sums
   double* box_p1 = nullptr;           // 2 per node (p1x, p1y)
   double* box_p2 = nullptr;           // 2 per node
@@ -271,8 +278,11 @@ struct slsim_lens_s {
   std::vector<std::vector<slsim_plane>> main_planes;   // halos per main plane
   std::vector<double> main_Dl, main_z;
   std::vector<slsim_plane> owned;
-  // combined (combinePlanes)
-  std::vector<int> order;                              // plane j -> main plane index
+  // field planes (LensPlaneTree): one halo-of-halos tree each, double redshifts
+  std::vector<slsim_plane> field_planes;
+  std::vector<double> field_z, field_Dl;
+  // combined (combinePlanes): the trees of plane j
+  std::vector<std::vector<slsim_plane>> combined;
   std::vector<double> plane_redshifts, Dl, dDl;
   slsim::TreeDesc* trees_dev = nullptr;                // per plane: halos' descs
   std::vector<int> tree_off, tree_cnt, tree_hcnt;   // per plane: first desc, particle / halo trees
@@ -341,8 +351,8 @@ void launch_ray_morton(hipStream_t st, const double* x, int64_t n, double x0, do
                        double inv_w, uint64_t* code, uint32_t* idx);
 void launch_minmax_rays(hipStream_t st, const double* x, int64_t n, double* part, int nblk);
 int build_halo_plane(slsim_plane p, const double* x, const double* y, const float* mass,
-                     const float* rsize, const float* conc, size_t n, float inv_area,
-                     float theta_force);
+                     const float* rsize, const float* conc, size_t n, double inv_area,
+                     double theta_force);
 void launch_halo_force(hipStream_t st, const TreeDesc* tree, const double* rays, int64_t n,
                        int direct, double* out, uint32_t* err);
 void launch_force(hipStream_t st, int precision, const TreeDesc* tree, const double* rays,

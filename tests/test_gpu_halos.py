@@ -211,3 +211,67 @@ def test_cpp_dropin_halos_program(ctx, tmp_path):
     assert np.abs(v[:, 2:4] - yr).max() <= 1e-12 * np.abs(yr).max()
     assert np.abs(v[:, 4:8] - Ar).max() <= 1e-8
     g.close()
+
+
+# --------------------------------------------------- LensPlaneTree field planes
+@pytest.mark.parametrize("sigma,scr", [(0.0, 0.0), (2.0e3, 0.0), (1.0e3, 2.0)])
+def test_field_plane_force_matches_oracle(ctx, sigma, scr):
+    x, y, m, rs, c = _halo_set(1500, 31)
+    print("case: field plane sigma=%g screening=%g" % (sigma, scr), flush=True)
+    p = S.LensPlaneTree(ctx, 0.7, x, y, m, rs, c, sigma, scr)
+    t = O.FieldTree(x, y, m, rs, c, sigma, scr)
+    rays = _rays(x, y, rs, c, 2048, 32)
+    _close(p.force(rays), t.force(rays))
+    _close(p.direct_sum(rays[:256]), t.force(rays[:256], direct=True))
+
+
+def _field_lens(ctx, oracle=False):
+    n = 16384
+    px, py = S.gen_nfw(91, n, 1e13, 8.0, 0.5)
+    pm = np.full(n, np.float32(1e13 / n), np.float32)
+    ph = np.full(n, np.float32(2e-4), np.float32)
+    ax, ay, am, ars, ac = _halo_set(500, 33, spread=0.15, rmax=(0.005, 0.08))
+    bx, by, bm, brs, bc = _halo_set(400, 34, spread=0.2, rmax=(0.005, 0.05))
+    za, zb = 0.2713, 1.1372
+    if oracle:
+        L = O.Lens(2.5)
+        L.moveinMainHalo(O.Tree(px, py, pm, ph), 0.5)
+        L.addFieldPlane(O.FieldTree(ax, ay, am, ars, ac, 50.0, 0.0), za)
+        L.addFieldPlane(O.FieldTree(bx, by, bm, brs, bc, 80.0, 3.0), zb)
+        return L
+    lens = S.Lens(ctx, 2.5)
+    lens.moveinMainHalo(S.LensHaloParticles(ctx, px, py, pm, ph, 0.5))
+    lens.addFieldPlane(S.LensPlaneTree(ctx, za, ax, ay, am, ars, ac, 50.0, 0.0))
+    lens.addFieldPlane(S.LensPlaneTree(ctx, zb, bx, by, bm, brs, bc, 80.0, 3.0))
+    return lens
+
+
+def test_field_planes_lightcone_matches_oracle(ctx):
+    lens = _field_lens(ctx)
+    L = _field_lens(ctx, oracle=True)
+    t = L.tables()
+    assert t["nplanes"] == 3
+    print("case: light-cone with two field planes, GridMap 64", flush=True)
+    N, rg = 64, W.grid_range(60.0)
+    g = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    y, A = g.y(), g.A()
+    yr, Ar = L.shoot(O.gridmap_positions(N, (0.0, 0.0), rg))
+    assert np.abs(y - yr).max() <= 1e-12 * np.abs(yr).max()
+    assert np.abs(A - Ar).max() <= 1e-8
+    g32 = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP32C)
+    x = g.positions()
+    assert np.abs((x - g32.y()) - (x - y)).max() <= 1e-5 * np.abs(x - y).max()
+    # batches and the server give each request its lone GridMap's bits
+    reqs = [(lens, 40, (0.0, 0.0), rg, 0.0), (lens, 33, (2e-5, 1e-5), rg, 1.4)]
+    ref = []
+    for lz, n_, c_, r_, zs in reqs:
+        gg = S.GridMap(lz, n_, c_, r_, S.SLSIM_FP32C, zs=zs)
+        ref.append((gg.y(), gg.A()))
+        gg.close()
+    b = S.Batch(ctx, reqs, S.SLSIM_FP32C)
+    b.run()
+    for i in range(len(reqs)):
+        yb, Ab = b.output(i)
+        assert np.array_equal(yb, ref[i][0]) and np.array_equal(Ab, ref[i][1]), i
+    g.close()
+    g32.close()
