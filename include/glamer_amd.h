@@ -37,6 +37,10 @@
 
 typedef double PosType;
 typedef double KappaType;
+#ifndef IndexType_declare
+#define IndexType_declare
+typedef size_t IndexType;   // include/standard.h:110-113 (reference)
+#endif
 
 #ifndef arcsecTOradians
 #define arcsecTOradians 0.000004848136811   // include/standard.h:51 (reference)
@@ -613,6 +617,63 @@ class LensHaloHalos {
   slsim_plane plane = nullptr;
 };
 
+// MultiPlane/planes.cpp:13-68, include/planes.h:35-63: a field plane of
+// halos on a TreeQuadHalos (sigma_background as its uniform background; a
+// nonzero inv_screening_scale screens its far cells and adds the periodic
+// buffer), built and walked by the library.  force() is TreeQuadHalos::force2D
+// at a physical position; phi and gamma[2] are 0.
+class LensPlaneTree {
+ public:
+  LensPlaneTree(float z_, LensHaloNFW** my_halos, IndexType Nhalos, PosType my_sigma_background,
+                PosType my_inv_screening_scale = 0)
+      : z(z_), halos(my_halos, my_halos + Nhalos) {
+    std::vector<double> x(Nhalos), y(Nhalos);
+    std::vector<float> m(Nhalos), r(Nhalos), c(Nhalos);
+    for (IndexType i = 0; i < Nhalos; ++i) {
+      double p[2];
+      my_halos[i]->getX(p);
+      x[i] = p[0];
+      y[i] = p[1];
+      m[i] = my_halos[i]->get_mass();
+      r[i] = my_halos[i]->getRsize();
+      c[i] = my_halos[i]->getConcentration();
+    }
+    slsim_cpp::check(slsim_plane_tree_create_nfw(slsim_cpp::context(), x.data(), y.data(),
+                                                 m.data(), r.data(), c.data(), Nhalos,
+                                                 my_sigma_background, my_inv_screening_scale,
+                                                 &plane));
+  }
+  LensPlaneTree(LensPlaneTree&& p) noexcept : z(p.z), halos(std::move(p.halos)), plane(p.plane) {
+    p.plane = nullptr;
+  }
+  LensPlaneTree(const LensPlaneTree&) = delete;
+  ~LensPlaneTree() {
+    if (plane) slsim_plane_destroy(plane);
+  }
+  void force(PosType* alpha, KappaType* kappa, KappaType* gamma, KappaType* phi, PosType* xx) {
+    double out[5];
+    slsim_cpp::check(slsim_plane_force(plane, 1, xx, out, SLSIM_FP64));
+    alpha[0] = out[0];
+    alpha[1] = out[1];
+    *kappa = out[2];
+    gamma[0] = out[3];
+    gamma[1] = out[4];
+    gamma[2] = 0;
+    *phi = 0;
+  }
+  std::vector<LensHaloNFW*> getHalos() { return halos; }
+  slsim_plane release_plane() {
+    slsim_plane p = plane;
+    plane = nullptr;
+    return p;
+  }
+  float z;
+
+ private:
+  std::vector<LensHaloNFW*> halos;
+  slsim_plane plane = nullptr;
+};
+
 // --------------------------------------------------------------------- lens
 class Lens {
  public:
@@ -634,6 +695,14 @@ class Lens {
     T* halo = new T(std::move(halo_in));
     owned.emplace_back(halo, [](void* p) { delete static_cast<T*>(p); });
     slsim_cpp::check(slsim_lens_add_halo(lens, halo->release_plane(), halo->getZlens()));
+  }
+
+  // A field plane at redshift z (the reference builds its field planes in
+  // Lens::createFieldPlanes, MultiPlane/lens.cpp:349-455, with double
+  // redshifts, and combinePlanes sorts them with the main planes); the lens
+  // takes the plane.
+  void insertFieldPlane(LensPlaneTree& plane, PosType z) {
+    slsim_cpp::check(slsim_lens_add_field_plane(lens, plane.release_plane(), z));
   }
 
   // include/lens.h:149-170: the lens keeps a copy of the halo.

@@ -391,6 +391,10 @@ __device__ inline void halo_term(const HaloNFW& h, cptr<NfwEntry> T, double rx, 
   a.k -= m * ia;
 }
 
+// exp(-r2 s2), out of line: only screened field planes call it, and inlined
+// it would set the halo kernels' register count
+__device__ __noinline__ double screening(double r2, double s2) { return exp(-r2 * s2); }
+
 // One cell of TreeQuadHalos::walkTree_iter (include/quadTreeHalos.h:
 // 656-735) for the lanes in lvm: its big halos, then, for an internal cell,
 // the monopole + quadrupole where the cell is far.  Returns the lanes that
@@ -414,7 +418,7 @@ __device__ __forceinline__ uint64_t evalh64(const HNode& nd, uint64_t lvm, doubl
     const double inv = rcp64(r2);
     // LensPlaneTree's screening exp(-r^2 s^2) of the far terms (:679-733);
     // exactly 1 for other trees
-    const double scr = t.hs2 != 0.0 ? exp(-r2 * t.hs2) : 1.0;
+    const double scr = t.hs2 != 0.0 ? screening(r2, t.hs2) : 1.0;
     const double i2p = inv * inv * kInvPI * scr;
     double tmp = nd.mass * (ia - inv * kInvPI * scr);
     a.ax += tmp * dx;
@@ -481,19 +485,17 @@ template <bool DIRECT>
 __device__ void halo_tree_force64(const TreeDesc& t, double rx, double ry, bool active, Acc64& a,
                                   int32_t* s_node, uint64_t* s_mask, uint32_t* err) {
 #pragma clang fp contract(off)
-  if (!t.hperiodic) {
-    if (DIRECT) halos_direct64(t, rx, ry, active, a);
-    else walk_halos64(t, rx, ry, active, a, s_node, s_mask, err);
-    return;
-  }
+  const int nshift = t.hperiodic ? 9 : 1;
 #pragma unroll 1
-  for (int i = -1; i < 2; ++i)
-#pragma unroll 1
-    for (int j = -1; j < 2; ++j) {
-      const double sx = rx + i * t.hxl, sy = ry + j * t.hyl;
-      if (DIRECT) halos_direct64(t, sx, sy, active, a);
-      else walk_halos64(t, sx, sy, active, a, s_node, s_mask, err);
+  for (int q = 0; q < nshift; ++q) {
+    double sx = rx, sy = ry;
+    if (nshift > 1) {
+      sx = rx + (q / 3 - 1) * t.hxl;
+      sy = ry + (q % 3 - 1) * t.hyl;
     }
+    if (DIRECT) halos_direct64(t, sx, sy, active, a);
+    else walk_halos64(t, sx, sy, active, a, s_node, s_mask, err);
+  }
 }
 
 // ---------------------------------------------------------------- FP32 walk

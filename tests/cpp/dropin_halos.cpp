@@ -5,7 +5,9 @@
 // (GridMap::find_images) of four grid rays' source positions there; with
 // argv[3], the critical curves (GridMap::find_crit) and the contours of
 // GridMap::find_magnification_contour at invmag = 0.5, each curve as
-// (type, hits_boundary, n, n points).
+// (type, hits_boundary, n, n points); with argv[4], a lens of one screened
+// LensPlaneTree field plane of the same halos at z = 0.9: its force at one
+// position (5 values) and a 32 x 32 GridMap's x, y, A.
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -42,6 +44,7 @@ int main(int argc, char** argv) {
   Lens lens(&seed, 2.0, cosmo);
   LensHaloParticles<ParticleType<double>> halo(particles, 0.5, cosmo, Point_2d(0, 0), false, 0);
   lens.moveinMainHalo(halo, true);
+  std::vector<LensHaloNFW> nfw2 = nfw;   // for the field plane below
   LensHaloHalos<LensHaloNFW> field(nfw, 0.3f, cosmo, 0.0f);
   lens.moveinMainHalo(field, true);
   const double center[2] = {0.0, 0.0};
@@ -90,6 +93,25 @@ int main(int argc, char** argv) {
     if (!g) return 1;
     std::fwrite(w.data(), sizeof(double), w.size(), g);
     std::fclose(g);
+  }
+  if (argc > 4) {
+    std::vector<LensHaloNFW*> ptrs;
+    for (LensHaloNFW& h : nfw2) ptrs.push_back(&h);
+    LensPlaneTree fp(0.9f, ptrs.data(), ptrs.size(), 100.0, 2.0);
+    double xx[2] = {1e-3, -2e-3}, a[2], k, g[3], ph;
+    fp.force(a, &k, g, &ph, xx);
+    std::vector<double> w = {a[0], a[1], k, g[0], g[1]};
+    Lens lens2(&seed, 2.0, cosmo);
+    lens2.insertFieldPlane(fp, 0.9);
+    GridMap grid2(&lens2, 32, center, 60 * arcsecTOradians);
+    for (size_t i = 0; i < grid2.getNumberOfPoints(); ++i) {
+      RAY r = grid2.ray(i);
+      w.insert(w.end(), {r.x[0], r.x[1], r.y[0], r.y[1], r.A[0], r.A[1], r.A[2], r.A[3]});
+    }
+    FILE* g4 = std::fopen(argv[4], "wb");
+    if (!g4) return 1;
+    std::fwrite(w.data(), sizeof(double), w.size(), g4);
+    std::fclose(g4);
   }
   std::printf("dropin_halos: %zu rays, %d planes\n", out.size() / 8, lens.getNplanes());
   return 0;

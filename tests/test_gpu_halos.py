@@ -164,8 +164,9 @@ def test_cpp_dropin_halos_program(ctx, tmp_path):
     out = tmp_path / "halos.bin"
     img = tmp_path / "images.bin"
     crit = tmp_path / "crit.bin"
-    r = subprocess.run([exe, str(out), str(img), str(crit)], capture_output=True, text=True,
-                       timeout=120)
+    fpl = tmp_path / "field.bin"
+    r = subprocess.run([exe, str(out), str(img), str(crit), str(fpl)], capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     v = np.fromfile(out, dtype=np.float64).reshape(-1, 8)
     # the same light-cone through the Python view: LensHaloNFW positions are
@@ -204,6 +205,16 @@ def test_cpp_dropin_halos_program(ctx, tmp_path):
         assert np.array_equal(q[at + 3:at + 3 + 2 * len(cv)].reshape(-1, 2), cv)
         at += 3 + 2 * len(cv)
     assert at == len(q)
+    # LensPlaneTree from C++: its force and a lens of it equal the Python view's
+    f = np.fromfile(fpl, dtype=np.float64)
+    fp = S.LensPlaneTree(ctx, 0.9, hx, hy, m, rs, c, 100.0, 2.0)
+    assert np.array_equal(f[:5], fp.force(np.array([[1e-3, -2e-3]]))[0])
+    lens2 = S.Lens(ctx, 2.0)
+    lens2.addFieldPlane(fp)
+    g2 = S.GridMap(lens2, 32, (0.0, 0.0), 60 * S.arcsecTOradians)
+    v2 = f[5:].reshape(-1, 8)
+    assert np.array_equal(v2[:, 2:4], g2.y()) and np.array_equal(v2[:, 4:8], g2.A())
+    g2.close()
     L = O.Lens(2.0)
     L.moveinMainHalo(O.Tree(px, py, pm, ph), 0.5)
     L.moveinMainHalo(O.HaloTree(hx, hy, m, rs, c), 0.3)
