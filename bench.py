@@ -260,12 +260,15 @@ def run_formats(S, W, ctx, n, cpu=True, nsmooth=32):
     return out
 
 
-def run_halos(S, W, ctx, n, cpu=True, N=2048):
+def run_halos(S, W, ctx, n, cpu=True, N=2048, field=False):
     """Halo-of-halos plane: one LensHaloHalos of n NFW halos (masses
     log-uniform 1e10-1e14 Msun, Rsize = r200, c = 10 (M/1e12)^-0.1, centres
     uniform in a 5' field) at z = 0.5, source at z = 2, N^2 FP64 GridMap.
-    The oracle's restatement of TreeQuadHalos on 16 host threads over a
-    65,536-ray subsample."""
+    With field=True the same halos as a screened LensPlaneTree field plane
+    (screening length a quarter of the field, periodic buffer: nine walks per
+    ray, sigma_background the halos' mean surface density).  The oracle's
+    restatement of TreeQuadHalos on 16 host threads over a 65,536-ray
+    subsample."""
     rng = np.random.default_rng(4242)
     z = 0.5
     m = (10 ** rng.uniform(10, 14, n)).astype(np.float32)
@@ -274,11 +277,19 @@ def run_halos(S, W, ctx, n, cpu=True, N=2048):
     side = 300 * S.arcsecTOradians * S.coorDist(0.0, z) / (1 + z)
     hx = (rng.random(n) - 0.5) * side
     hy = (rng.random(n) - 0.5) * side
+    sigma = float(np.sum(m.astype(np.float64))) / side ** 2
+    scr = 4.0 / side
     t0 = time.perf_counter()
-    halo = S.LensHaloHalos(ctx, hx, hy, m, rs, c, z)
+    if field:
+        halo = S.LensPlaneTree(ctx, z, hx, hy, m, rs, c, sigma, scr)
+    else:
+        halo = S.LensHaloHalos(ctx, hx, hy, m, rs, c, z)
     build_s = time.perf_counter() - t0
     lens = S.Lens(ctx, 2.0)
-    lens.moveinMainHalo(halo)
+    if field:
+        lens.addFieldPlane(halo)
+    else:
+        lens.moveinMainHalo(halo)
     rg = W.grid_range(300.0)
     g = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP64, shoot=False)
     g.shoot()
@@ -286,13 +297,17 @@ def run_halos(S, W, ctx, n, cpu=True, N=2048):
     st = g.shoot()
     ctx.synchronize()
     dt = time.perf_counter() - t1
-    out = {"halos": n, "grid": N, "host_build_s": build_s, "rays_per_s": N * N / dt,
+    out = {"halos": n, "field_plane": bool(field), "grid": N, "host_build_s": build_s,
+           "rays_per_s": N * N / dt,
            "walk_launch_ms": st["walk_ms"] / max(st["n_walk_launches"], 1),
            "nodes": halo.info()["n_nodes"]}
     if cpu:
         import oracle as O
         L = O.Lens(2.0)
-        L.moveinMainHalo(O.HaloTree(hx, hy, m, rs, c), z)
+        if field:
+            L.addFieldPlane(O.FieldTree(hx, hy, m, rs, c, sigma, scr), z)
+        else:
+            L.moveinMainHalo(O.HaloTree(hx, hy, m, rs, c), z)
         x = g.positions()[::max(1, (N * N) // 65536)]
         t2 = time.perf_counter()
         yr, Ar = L.shoot(x, nthreads=16)
@@ -524,6 +539,8 @@ def main():
                     help="also time GridMap::find_images for S sources on a 2048^2 grid")
     ap.add_argument("--crit", type=int, default=0, metavar="N",
                     help="also time GridMap::find_crit on an N^2 grid (and the other contours)")
+    ap.add_argument("--field-plane", type=int, default=0, metavar="N",
+                    help="also time a screened LensPlaneTree field plane of N NFW halos")
     ap.add_argument("--halos", type=int, default=0, metavar="N",
                     help="also time a plane of N NFW halos (LensHaloHalos) on a 2048^2 GridMap")
     ap.add_argument("--formats", type=int, default=0, metavar="N",
@@ -739,6 +756,9 @@ def main():
         out["formats"] = run_formats(S, W, ctx, args.formats, cpu=not args.no_cpu)
     if args.halos:
         out["halos"] = run_halos(S, W, ctx, args.halos, cpu=not args.no_cpu)
+    if args.field_plane:
+        out["field_plane"] = run_halos(S, W, ctx, args.field_plane, cpu=not args.no_cpu,
+                                       field=True)
     if args.images:
         out["images"] = run_images(S, W, ctx, args.images, cpu=not args.no_cpu)
     if args.small_batches:

@@ -20,6 +20,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <iterator>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "slsim_internal.h"
@@ -379,20 +382,35 @@ int find_crit(slsim_ctx c, size_t N_, double c0, double c1, double range, const 
   std::vector<Curve> cs;
   std::vector<std::vector<int64_t>> seeds;
   bool replaced = false;
+  // the two bitmaps are walked at once, the radial one on a second thread
+  Walked wr;
+  int rc_rad = SLSIM_OK;
+  std::string err_rad;
+  std::thread radial;
+  if (cl.count[1] > 0)
+    radial = std::thread([&] {
+      rc_rad = walk_all(cl.px.data(), N, N, 1, cl.edge.data() + cl.nwords, cl.nwords, false, wr);
+      if (rc_rad) err_rad = slsim_last_error();
+    });
   if (cl.count[0] > 0) {
     Walked w;
-    if ((rc = walk_all(cl.px.data(), N, N, 0, cl.edge.data(), cl.nwords, true, w))) return rc;
-    cs = curves_of(w, SLSIM_CRIT_TANGENTIAL);
-    seeds = std::move(w.inside);
-    replaced = true;
+    rc = walk_all(cl.px.data(), N, N, 0, cl.edge.data(), cl.nwords, true, w);
+    if (rc == SLSIM_OK) {
+      cs = curves_of(w, SLSIM_CRIT_TANGENTIAL);
+      seeds = std::move(w.inside);
+      replaced = true;
+    }
+  }
+  if (radial.joinable()) radial.join();
+  if (rc) return rc;
+  if (rc_rad) {
+    set_error(err_rad);
+    return rc_rad;
   }
   const size_t ntan = cs.size();
   if (cl.count[1] > 0) {
-    Walked w;
-    if ((rc = walk_all(cl.px.data(), N, N, 1, cl.edge.data() + cl.nwords, cl.nwords, false, w)))
-      return rc;
-    std::vector<Curve> rad = curves_of(w, SLSIM_CRIT_RADIAL);
-    cs.insert(cs.end(), rad.begin(), rad.end());
+    std::vector<Curve> rad = curves_of(wr, SLSIM_CRIT_RADIAL);
+    cs.insert(cs.end(), std::make_move_iterator(rad.begin()), std::make_move_iterator(rad.end()));
   }
   // Each radial curve, in order, moves to just after the first tangential
   // curve (in list order) that holds its first point (gridmap.cpp:578-597).
