@@ -54,6 +54,9 @@ _sigs = {
     "orc_halotree_build_field": (c_vp, [P(c_d), P(c_d), P(c_f), P(c_f), P(c_f), c_sz, c_d, c_d,
                                          c_d, c_int, P(c_int)]),
     "orc_lens_add_field_plane": (None, [c_vp, c_vp, c_d]),
+    "orc_halotree_build_mixed": (c_vp, [P(c_d), P(c_d), P(c_f), P(c_f), P(c_int), P(c_d), c_sz,
+                                        c_d, c_d, c_d, c_int, P(c_int)]),
+    "orc_powerlaw_force": (None, [c_f, c_f, c_d, P(c_d), c_sz, P(c_d)]),
     "orc_halotree_free": (None, [c_vp]),
     "orc_halotree_force": (None, [c_vp, P(c_d), c_sz, c_int, P(c_d)]),
     "orc_nfw_force": (None, [c_f, c_f, c_f, P(c_d), c_sz, P(c_d)]),
@@ -226,6 +229,38 @@ class FieldTree(HaloTree):
         if st.value:
             raise ValueError("halo tree build failed (halos in one place or out of the tables)")
         self.owned = True
+
+
+class MixedTree(HaloTree):
+    """A TreeQuadHalos of NFW (type 0, param = concentration) and power-law
+    (type 1, param = beta) halos; theta, screening and buffer as given."""
+
+    def __init__(self, x, y, m, rsize, types, params, inv_area=0.0, theta=0.1,
+                 inv_screening_scale=0.0, periodic=False):
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.m = np.ascontiguousarray(m, dtype=np.float32)
+        self.rsize = np.ascontiguousarray(rsize, dtype=np.float32)
+        self.types = np.ascontiguousarray(types, dtype=np.int32)
+        self.params = np.ascontiguousarray(params, dtype=np.float64)
+        st = c_int()
+        self.t = _lib.orc_halotree_build_mixed(
+            _p(self.x), _p(self.y), _p(self.m, c_f), _p(self.rsize, c_f), _p(self.types, c_int),
+            _p(self.params), self.x.size, float(inv_area), float(theta),
+            float(inv_screening_scale), int(bool(periodic)), ctypes.byref(st))
+        if st.value:
+            raise ValueError("halo tree build failed")
+        self.owned = True
+
+
+def powerlaw_force(m, rsize, beta, xcm):
+    """LensHaloPowerLaw::force_halo (no point subtraction) at positions
+    relative to the halo: (n, 5) = alpha0, alpha1, kappa, gamma0, gamma1."""
+    xcm = np.ascontiguousarray(xcm, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros((xcm.shape[0], 5))
+    _lib.orc_powerlaw_force(np.float32(m), np.float32(rsize), float(beta), _p(xcm),
+                            xcm.shape[0], _p(out))
+    return out
 
 
 def nfw_force(m, rsize, conc, xcm):

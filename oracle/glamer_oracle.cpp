@@ -486,10 +486,16 @@ struct NfwProfile {
 // One LensHaloNFW(mass, Rsize, zlens, concentration, fratio = 1) at a
 // physical position (MultiPlane/lens_halos.cpp:389-440) and its force_halo
 // (LensHalo::force_halo -> force_point / force_halo_sym, :1528-1672).
+// A symmetric halo of a halo-of-halos or field tree: LensHaloNFW
+// (lens_halos.cpp:389-440; alpha_h, kappa_h, gamma_h from its tables,
+// include/lens_halos.h) or LensHaloPowerLaw (lens_halos.cpp:805-866,
+// lens_halos.h:979-997: rscale 1, xmax = Rmax = Rsize, kappa ~ r^-beta).
 struct NfwHalo {
   double pos[2];
   float mass, Rsize, rscale, Rmax;
   double xmax, gmax;
+  bool power_law = false;
+  double beta = 0;
   NfwHalo(double x, double y, float m, float rsize, float conc) {
     pos[0] = x;
     pos[1] = y;
@@ -500,9 +506,40 @@ struct NfwHalo {
     gmax = NfwProfile::get().interpolate(0, xmax);
     Rmax = Rsize;
   }
-  double alpha_h(double x) const { return -1.0 * NfwProfile::get().interpolate(0, x) / gmax; }
-  double kappa_h(double x) const { return 0.5 * x * x * NfwProfile::get().interpolate(1, x) / gmax; }
-  double gamma_h(double x) const { return -0.5 * x * x * NfwProfile::get().interpolate(2, x) / gmax; }
+  static NfwHalo power(double x, double y, float m, float rsize, double my_beta) {
+    if (my_beta >= 2.0) throw std::invalid_argument("bad gamma");
+    if (rsize <= 0.0f || std::isnan(rsize)) throw std::runtime_error("Bad halo");
+    NfwHalo h(x, y, m, rsize, 1.0f);
+    h.power_law = true;
+    h.beta = my_beta;
+    h.rscale = 1;
+    h.xmax = h.Rsize / h.rscale;
+    h.gmax = 0;
+    h.Rmax = h.Rsize;
+    return h;
+  }
+  double alpha_h(double x) const {
+    if (power_law) {
+      if (x < xmax * 1.0e-6) x = 1e-6 * xmax;
+      return -1.0 * pow(x / xmax, -beta + 2);
+    }
+    return -1.0 * NfwProfile::get().interpolate(0, x) / gmax;
+  }
+  double kappa_h(double x) const {
+    if (power_law) {
+      if (x == 0) x = 1e-6 * xmax;
+      const double tmp = x / xmax;
+      return 0.5 * (-beta + 2) * pow(tmp, 2 - beta);
+    }
+    return 0.5 * x * x * NfwProfile::get().interpolate(1, x) / gmax;
+  }
+  double gamma_h(double x) const {
+    if (power_law) {
+      if (x == 0) x = 1e-6 * xmax;
+      return -beta * pow(x / xmax, -beta + 2);
+    }
+    return -0.5 * x * x * NfwProfile::get().interpolate(2, x) / gmax;
+  }
   void force_halo(double* alpha, double* kappa, double* gamma, const double* xcm) const {
     double rcm2 = xcm[0] * xcm[0] + xcm[1] * xcm[1];
     // force_point without point subtraction
@@ -1344,6 +1381,37 @@ void* orc_halotree_build(const double* x, const double* y, const float* m, const
     return nullptr;
   }
 }
+// A tree of mixed halos: type[i] 0 = NFW (param = concentration), 1 = power
+// law (param = beta); inv_area, theta, screening and the buffer as given.
+void* orc_halotree_build_mixed(const double* x, const double* y, const float* m,
+                               const float* rsize, const int* type, const double* param, size_t n,
+                               double inv_area, double theta, double inv_screening, int periodic,
+                               int* status) {
+  try {
+    std::vector<orc::NfwHalo> h;
+    for (size_t i = 0; i < n; ++i) {
+      if (type[i] == 1) h.push_back(orc::NfwHalo::power(x[i], y[i], m[i], rsize[i], param[i]));
+      else h.emplace_back(x[i], y[i], m[i], rsize[i], (float)param[i]);
+    }
+    *status = 0;
+    return new orc::HaloTree(std::move(h), inv_area, theta, inv_screening, periodic != 0);
+  } catch (const std::exception&) {
+    *status = 1;
+    return nullptr;
+  }
+}
+// one power-law halo's force_halo at positions relative to it (5 per point)
+void orc_powerlaw_force(float m, float rsize, double beta, const double* xcm, size_t n,
+                        double* out) {
+  const orc::NfwHalo h = orc::NfwHalo::power(0, 0, m, rsize, beta);
+  for (size_t i = 0; i < n; ++i) {
+    double a[2] = {0, 0}, k = 0, g[3] = {0, 0, 0};
+    h.force_halo(a, &k, g, xcm + 2 * i);
+    out[5 * i] = a[0]; out[5 * i + 1] = a[1]; out[5 * i + 2] = k;
+    out[5 * i + 3] = g[0]; out[5 * i + 4] = g[1];
+  }
+}
+
 // LensPlaneTree's tree: inv_area (sigma_background), theta, screening and
 // the periodic buffer as doubles
 void* orc_halotree_build_field(const double* x, const double* y, const float* m,

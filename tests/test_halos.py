@@ -196,3 +196,47 @@ def test_field_plane_in_lens_tables():
     assert t["nplanes"] == 3
     # field redshifts stay doubles; the main plane's is a float
     assert list(t["z"][:3]) == [0.31234567, float(np.float32(0.5)), 1.2]
+
+
+# ----------------------------------------------------- LensHaloPowerLaw
+@pytest.mark.parametrize("beta", [0.3, 1.0, 1.7, -0.5])
+def test_powerlaw_gauss_law_and_shear(beta):
+    from scipy.integrate import quad
+    m, R = 2e12, 0.15
+    Rf = float(np.float32(R))
+    mf = float(np.float32(m))
+    for r in (1e-3, 0.02, 0.09, 0.149):
+        out = O.powerlaw_force(m, R, beta, np.array([[r, 0.0]]))[0]
+        # M(<r) = m (r/R)^(2-beta) inside, alpha = -M(<r)/(pi r) x-hat
+        menc = mf * (r / Rf) ** (2 - beta)
+        assert abs(-out[0] * np.pi * r - menc) <= 1e-12 * menc
+        kap = lambda rr: O.powerlaw_force(m, R, beta, np.array([[rr, 0.0]]))[0][2]
+        mk, _ = quad(lambda rr: 2 * np.pi * rr * kap(rr), 0, r, limit=200, epsrel=1e-10)
+        assert abs(mk - menc) <= 1e-6 * menc
+        gt = -out[3]
+        assert abs(gt - (menc / (np.pi * r * r) - out[2])) <= 1e-10 * abs(gt)
+    # beyond Rmax: the point mass
+    out = O.powerlaw_force(m, R, beta, np.array([[0.4, 0.0]]))[0]
+    assert abs(-out[0] - mf / (np.pi * 0.4)) <= 1e-14 * mf / 0.4 and out[2] == 0.0
+
+
+def test_powerlaw_rejects_beta_two():
+    with pytest.raises(ValueError):
+        O.MixedTree([0.0, 1.0], [0.0, 1.0], [1e12, 1e12], [0.1, 0.1], [1, 1], [2.0, 1.0])
+
+
+def test_mixed_tree_converges_to_direct_sum():
+    x, y, m, rs, c = _halos(1200, 21)
+    rng = np.random.default_rng(22)
+    types = (rng.random(x.size) < 0.4).astype(np.int32)
+    params = np.where(types == 1, rng.uniform(0.2, 1.8, x.size), c.astype(np.float64))
+    rays = rng.normal(size=(200, 2)) * 0.3
+    rays[:10] = np.stack([x[:10] + 2e-4, y[:10]], 1)
+    t = O.MixedTree(x, y, m, rs, types, params, inv_area=0.3, theta=1e-6)
+    d = t.force(rays, direct=True)
+    f = t.force(rays)
+    assert np.abs(f - d).max() <= 1e-11 * np.abs(d).max()
+    # NFW-only mixed trees are the NFW tree, bit for bit
+    t0 = O.MixedTree(x, y, m, rs, np.zeros(x.size, np.int32), c.astype(np.float64),
+                     inv_area=float(np.float32(0.3)), theta=float(np.float32(0.2)))
+    assert np.array_equal(t0.force(rays), O.HaloTree(x, y, m, rs, c, 0.3, 0.2).force(rays))
