@@ -107,6 +107,25 @@ int slsim_plane_tree_create_nfw(slsim_ctx ctx, const double* x, const double* y,
                                 const float* concentration, size_t n, double sigma_background,
                                 double inv_screening_scale, slsim_plane* out);
 
+/* Halo kinds of slsim_halos_create. */
+enum {
+  SLSIM_HALO_NFW = 0,         /* LensHaloNFW, param = concentration (used as float) */
+  SLSIM_HALO_POWERLAW = 1     /* LensHaloPowerLaw, param = beta (kappa ~ r^-beta, < 2) */
+};
+/* A TreeQuadHalos of n symmetric halos of mixed kinds (fratio = 1; at
+ * physical positions, float mass and Rsize = Rmax): LensHaloNFW
+ * (MultiPlane/lens_halos.cpp:389-440) or LensHaloPowerLaw (lens_halos.cpp:
+ * 805-866, profile include/lens_halos.h:979-997).  For a LensHaloHalos pass
+ * its float inv_area and theta_force, no screening and periodic = 0 (add it
+ * with slsim_lens_add_halo); for a LensPlaneTree pass sigma_background and
+ * either (theta 0.1, inv_screening_scale, periodic 1) or (0.5, 0, 0) (add it
+ * with slsim_lens_add_field_plane).  Errors as slsim_halos_create_nfw, and
+ * SLSIM_ERR_ARG for beta >= 2 (the reference's "bad gamma") or a bad kind. */
+int slsim_halos_create(slsim_ctx ctx, const double* x, const double* y, const float* mass,
+                       const float* rsize, const int32_t* kind, const double* param, size_t n,
+                       double inv_area, double theta_force, double inv_screening_scale,
+                       int periodic, slsim_plane* out);
+
 typedef struct {
   int64_t n_particles;
   int64_t n_nodes;

@@ -13,6 +13,7 @@ import numpy as np
 from . import _abi
 from ._abi import SLSIM_DIRECT64, SLSIM_FP32C, SLSIM_FP64, SlsimError, lib
 from ._abi import SLSIM_CRIT_ND, SLSIM_CRIT_PSEUDO, SLSIM_CRIT_RADIAL, SLSIM_CRIT_TANGENTIAL
+from ._abi import SLSIM_HALO_NFW, SLSIM_HALO_POWERLAW
 from ._abi import (SLSIM_FMT_ASCII, SLSIM_FMT_CSV3, SLSIM_FMT_CSV4, SLSIM_FMT_CSV5,
                    SLSIM_FMT_CSV6, SLSIM_FMT_GADGET2, SLSIM_FMT_GLMB)
 
@@ -219,6 +220,22 @@ class LensHaloParticles:
             pass
 
 
+def _halos_plane(ctx, x, y, mass, rsize, kinds, params, inv_area, theta, scr, periodic):
+    xs = np.ascontiguousarray(x, dtype=np.float64)
+    ys = np.ascontiguousarray(y, dtype=np.float64)
+    m = np.ascontiguousarray(mass, dtype=np.float32)
+    r = np.ascontiguousarray(rsize, dtype=np.float32)
+    k = np.ascontiguousarray(kinds, dtype=np.int32)
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    h = _abi.slsim_plane()
+    _check(lib.slsim_halos_create(ctx.h, _ptr(xs, ctypes.c_double), _ptr(ys, ctypes.c_double),
+                                  _ptr(m, ctypes.c_float), _ptr(r, ctypes.c_float),
+                                  _ptr(k, ctypes.c_int32), _ptr(p, ctypes.c_double),
+                                  ctypes.c_size_t(xs.size), float(inv_area), float(theta),
+                                  float(scr), int(periodic), ctypes.byref(h)))
+    return h
+
+
 class LensHaloHalos:
     """LensHaloHalos<LensHaloNFW>(halos, redshift, cosmo, inv_area,
     theta_force) (include/particle_halo.h:1092-1227): a main halo made of
@@ -241,6 +258,18 @@ class LensHaloHalos:
                                           _ptr(c, ctypes.c_float), ctypes.c_size_t(xs.size),
                                           ctypes.c_float(inv_area), ctypes.c_float(theta),
                                           ctypes.byref(self.h)))
+
+    @classmethod
+    def mixed(cls, ctx, x, y, mass, rsize, kinds, params, z, inv_area=0.0, theta=0.1):
+        """LensHaloHalos of symmetric halos of mixed kinds: SLSIM_HALO_NFW
+        (param = concentration) or SLSIM_HALO_POWERLAW (param = beta)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.z = np.float32(z)
+        self.owned_by_lens = False
+        self.h = _halos_plane(ctx, x, y, mass, rsize, kinds, params,
+                              float(np.float32(inv_area)), float(np.float32(theta)), 0.0, 0)
+        return self
 
     # the plane handle's methods (a halo-of-halos plane is a slsim_plane)
     info = LensHaloParticles.info
@@ -274,6 +303,20 @@ class LensPlaneTree:
             ctx.h, _ptr(xs, ctypes.c_double), _ptr(ys, ctypes.c_double), _ptr(m, ctypes.c_float),
             _ptr(r, ctypes.c_float), _ptr(c, ctypes.c_float), ctypes.c_size_t(xs.size),
             float(sigma_background), float(inv_screening_scale), ctypes.byref(self.h)))
+
+    @classmethod
+    def mixed(cls, ctx, z, x, y, mass, rsize, kinds, params, sigma_background=0.0,
+              inv_screening_scale=0.0):
+        """A LensPlaneTree of symmetric halos of mixed kinds (SLSIM_HALO_NFW:
+        param = concentration, SLSIM_HALO_POWERLAW: param = beta)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.z = float(z)
+        self.owned_by_lens = False
+        scr = float(inv_screening_scale) != 0.0
+        self.h = _halos_plane(ctx, x, y, mass, rsize, kinds, params, sigma_background,
+                              0.1 if scr else 0.5, inv_screening_scale, int(scr))
+        return self
 
     info = LensHaloParticles.info
     force = LensHaloParticles.force

@@ -103,6 +103,9 @@ slsim_batch = ctypes.c_void_p
 slsim_server = ctypes.c_void_p
 slsim_curves = ctypes.c_void_p
 
+# halo kinds of slsim_halos_create
+SLSIM_HALO_NFW, SLSIM_HALO_POWERLAW = 0, 1
+
 # CritType (include/point.h:35)
 SLSIM_CRIT_ND, SLSIM_CRIT_TANGENTIAL, SLSIM_CRIT_RADIAL, SLSIM_CRIT_PSEUDO = range(4)
 
@@ -199,6 +202,9 @@ SIGNATURES = {
                                             P(c_float), P(c_float), c_size_t, c_double, c_double,
                                             P(slsim_plane)]),
     "slsim_lens_add_field_plane": (c_int, [slsim_lens, slsim_plane, c_double]),
+    "slsim_halos_create": (c_int, [slsim_ctx, P(c_double), P(c_double), P(c_float), P(c_float),
+                                   P(ctypes.c_int32), P(c_double), c_size_t, c_double, c_double,
+                                   c_double, c_int, P(slsim_plane)]),
     "slsim_halos_create_nfw": (c_int, [slsim_ctx, P(c_double), P(c_double), P(c_float),
                                        P(c_float), P(c_float), c_size_t, c_float, c_float,
                                        P(slsim_plane)]),

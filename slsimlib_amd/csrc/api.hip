@@ -273,14 +273,37 @@ int slsim_plane_tree_create_nfw(slsim_ctx ctx, const double* x, const double* y,
   // planes.cpp:13-19: screened trees take theta 0.1 and the periodic buffer,
   // unscreened ones TreeQuadHalos' defaults (theta 0.5, no buffer)
   const bool scr = inv_screening_scale != 0;
-  const int rc = build_halo_plane(p.get(), x, y, mass, rsize, concentration, n, sigma_background,
-                                  scr ? 0.1 : 0.5);
+  const std::vector<double> param(concentration, concentration + n);
+  const int rc = build_halo_plane(p.get(), x, y, mass, rsize, nullptr, param.data(), n,
+                                  sigma_background, scr ? 0.1 : 0.5);
   if (rc) {
     plane_free_device(p.get());
     return rc;
   }
   p->hs2 = inv_screening_scale * inv_screening_scale;
   p->hperiodic = scr;
+  *out = p.release();
+  return SLSIM_OK;
+}
+
+int slsim_halos_create(slsim_ctx ctx, const double* x, const double* y, const float* mass,
+                       const float* rsize, const int32_t* kind, const double* param, size_t n,
+                       double inv_area, double theta_force, double inv_screening_scale,
+                       int periodic, slsim_plane* out) {
+  if (!ctx || !out) return arg_error("null handle");
+  if (n == 0 || !x || !y || !mass || !rsize || !kind || !param) return arg_error("no halos");
+  if (n >= (size_t)1 << 22) return arg_error("more than 2^22 halos in one plane");
+  SLSIM_HIP_CHECK(hipSetDevice(ctx->device));
+  auto p = std::make_unique<slsim_plane_s>();
+  p->ctx = ctx;
+  const int rc = build_halo_plane(p.get(), x, y, mass, rsize, kind, param, n, inv_area,
+                                  theta_force);
+  if (rc) {
+    plane_free_device(p.get());
+    return rc;
+  }
+  p->hs2 = inv_screening_scale * inv_screening_scale;
+  p->hperiodic = periodic != 0;
   *out = p.release();
   return SLSIM_OK;
 }
@@ -294,7 +317,8 @@ int slsim_halos_create_nfw(slsim_ctx ctx, const double* x, const double* y, cons
   SLSIM_HIP_CHECK(hipSetDevice(ctx->device));
   auto p = std::make_unique<slsim_plane_s>();
   p->ctx = ctx;
-  const int rc = build_halo_plane(p.get(), x, y, mass, rsize, concentration, n, inv_area,
+  const std::vector<double> param(concentration, concentration + n);
+  const int rc = build_halo_plane(p.get(), x, y, mass, rsize, nullptr, param.data(), n, inv_area,
                                   theta_force);
   if (rc) {
     plane_free_device(p.get());
@@ -548,14 +572,21 @@ int slsim_lens_s::upload_trees() {
   tree_off.clear();
   tree_cnt.clear();
   tree_hcnt.clear();
+  tree_hkind.clear();
   for (const std::vector<slsim_plane>& trees : combined) {
     // particle trees first, then the halo-of-halos trees of the plane
     tree_off.push_back((int)all.size());
     int np = 0, nh = 0;
     for (slsim_plane p : trees)
       if (p->kind == kTreeParticles) { all.push_back(p->desc()); ++np; }
+    int hk = 0;
     for (slsim_plane p : trees)
-      if (p->kind == kTreeHalos) { all.push_back(p->desc()); ++nh; }
+      if (p->kind == kTreeHalos) {
+        all.push_back(p->desc());
+        ++nh;
+        hk = std::max(hk, p->hpowerlaw ? 2 : 1);
+      }
+    tree_hkind.push_back(hk);
     tree_cnt.push_back(np);
     tree_hcnt.push_back(nh);
   }
@@ -607,6 +638,8 @@ void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint3
     s.j = j;
     s.ntrees = L->tree_cnt[j];
     s.nhtrees = L->tree_hcnt[j];
+    s.hkind = L->tree_hkind[j];
+    s.hpad = 0;
     s.trees = L->trees_dev + L->tree_off[j];
     s.Dphys = L->Dl[j] / (1 + L->plane_redshifts[j]);
     s.Gfac = L->charge * L->Dl[j] / (1 + L->plane_redshifts[j]);
@@ -632,7 +665,7 @@ int run_steps(slsim_ctx c, const StepJob* djobs, const std::vector<StepJob>& ste
   for (int j = 0; j < jmax; ++j) {
     if (timed) SLSIM_HIP_CHECK(hipEventRecord(ev[2 * j], c->stream));
     launch_plane_step(precision, count, nbundles, c->stream, djobs + j, nullptr, err,
-                      steps[j].nhtrees > 0);
+                      steps[j].nhtrees > 0 ? steps[j].hkind : 0);
     if (timed) SLSIM_HIP_CHECK(hipEventRecord(ev[2 * j + 1], c->stream));
   }
   SLSIM_HIP_CHECK(hipGetLastError());

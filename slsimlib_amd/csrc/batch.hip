@@ -90,7 +90,7 @@ struct slsim_batch_s {
   DevBlock d_greq, d_jobs, d_items, d_ctr, d_err;
   std::vector<size_t> job_off, item_off;
   std::vector<uint32_t> nitems;
-  std::vector<char> step_halos;   // per step: some job has halo-of-halos trees
+  std::vector<char> step_halos;   // per step: the largest StepJob::hkind of its jobs
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
 
@@ -172,7 +172,8 @@ int slsim_batch_s::prepare() {
       if (j >= reqs[r]->jmax) continue;
       live.push_back({(uint32_t)(all_jobs.size() - job_off[j]), greq[r].nslots / kWave});
       all_jobs.push_back(steps[r][j]);
-      if (steps[r][j].nhtrees > 0) step_halos[j] = 1;
+      if (steps[r][j].nhtrees > 0)
+        step_halos[j] = (char)std::max<int>(step_halos[j], steps[r][j].hkind);
     }
     if (sched.order == 1) {
       int64_t maxb = 0;
@@ -222,10 +223,10 @@ int slsim_batch_s::enqueue() {
     if (sched.queue)
       launch_plane_step_queue(precision, (unsigned)(kNumCU * std::max(1, sched.blocks_per_cu)),
                               nitems[j], st, jobs + job_off[j], items + item_off[j],
-                              static_cast<uint32_t*>(d_ctr.p), err, step_halos[j] != 0);
+                              static_cast<uint32_t*>(d_ctr.p), err, step_halos[j]);
     else
       launch_plane_step(precision, false, nitems[j], st, jobs + job_off[j], items + item_off[j],
-                        err, step_halos[j] != 0);
+                        err, step_halos[j]);
   }
   launch_finalize_batch(st, static_cast<GridReq*>(d_greq.p), (int)greq.size(), total_slots);
   SLSIM_HIP_CHECK(hipGetLastError());
@@ -602,8 +603,9 @@ int slsim_server_s::iterate() {
   for (int prec = SLSIM_FP64; prec <= SLSIM_DIRECT64; ++prec) {
     const size_t n_items = item0[prec + 1] - item0[prec];
     if (n_items == 0) continue;
-    bool halos = false;
-    for (size_t q = job0[prec]; q < job0[prec + 1]; ++q) halos = halos || h_jobs[q].nhtrees > 0;
+    int halos = 0;
+    for (size_t q = job0[prec]; q < job0[prec + 1]; ++q)
+      if (h_jobs[q].nhtrees > 0) halos = std::max(halos, (int)h_jobs[q].hkind);
     launch_plane_step(prec, false, (uint32_t)n_items, stream,
                       static_cast<StepJob*>(d_jobs.p) + job0[prec],
                       static_cast<WorkItem*>(d_items.p) + item0[prec],

@@ -182,34 +182,57 @@ void build_cell(HBuild& B, size_t ci, bool top, int level) {
 
 // Builds the plane's halo tree and copies it to HBM.
 int build_halo_plane(slsim_plane p, const double* x, const double* y, const float* mass,
-                     const float* rsize, const float* conc, size_t n, double inv_area,
-                     double theta_force) {
+                     const float* rsize, const int32_t* kind, const double* param, size_t n,
+                     double inv_area, double theta_force) {
   // LensHaloNFW(mass, Rsize, zlens, concentration, fratio 1, ...)
   // (MultiPlane/lens_halos.cpp:389-440): rscale = Rsize / c and xmax =
-  // Rsize / rscale in float, gmax = g(xmax) from the table, Rmax = Rsize
+  // Rsize / rscale in float, gmax = g(xmax) from the table, Rmax = Rsize.
+  // LensHaloPowerLaw(mass, Rsize, zlens, beta, fratio 1, ...)
+  // (lens_halos.cpp:805-866): beta < 2, rscale 1, xmax = Rmax = Rsize.
   std::vector<HaloNFW> h(n);
   std::vector<float> rmax(n);
   for (size_t i = 0; i < n; ++i) {
-    const float rs = rsize[i] / conc[i];
-    const float xm = rsize[i] / rs;
-    if (!(rsize[i] > 0) || !(conc[i] > 0) || !(rs > 0) || !(xm < 99.98f) || !(mass[i] >= 0) ||
-        !std::isfinite(x[i]) || !std::isfinite(y[i])) {
-      set_error("halo " + std::to_string(i) +
-                ": need Rsize > 0, concentration > 0 with Rsize/rscale below 99.98 (the NFW "
-                "tables end at x = 100), mass >= 0 and a finite position");
+    const int k = kind ? kind[i] : 0;
+    if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !(mass[i] >= 0) || (k != 0 && k != 1)) {
+      set_error("halo " + std::to_string(i) + ": need a finite position, mass >= 0 and a kind "
+                "of 0 (NFW) or 1 (power law)");
       return SLSIM_ERR_ARG;
     }
     h[i].x = x[i];
     h[i].y = y[i];
     h[i].mass = mass[i];
     h[i].rmax = rsize[i];
+    h[i].mass_pi = (double)mass[i] / kPI;
+    rmax[i] = rsize[i];
+    if (k == 1) {
+      const double beta = param[i];
+      if (!(beta < 2.0) || !(rsize[i] > 0)) {
+        set_error("halo " + std::to_string(i) + ": a power law needs beta < 2 (\"bad gamma\") "
+                  "and Rsize > 0");
+        return SLSIM_ERR_ARG;
+      }
+      h[i].rscale = 1.0f;
+      h[i].gmax = beta;
+      h[i].inv_gmax = 0;
+      h[i].inv_rscale = 1.0;
+      h[i].pad = 1.0f;
+      p->hpowerlaw = 1;
+      continue;
+    }
+    const float conc = (float)param[i];
+    const float rs = rsize[i] / conc;
+    const float xm = rsize[i] / rs;
+    if (!(rsize[i] > 0) || !(conc > 0) || !(rs > 0) || !(xm < 99.98f)) {
+      set_error("halo " + std::to_string(i) +
+                ": need Rsize > 0, concentration > 0 with Rsize/rscale below 99.98 (the NFW "
+                "tables end at x = 100), mass >= 0 and a finite position");
+      return SLSIM_ERR_ARG;
+    }
     h[i].rscale = rs;
     h[i].gmax = interp_g((double)xm);
     h[i].inv_gmax = 1.0 / h[i].gmax;
     h[i].inv_rscale = 1.0 / (double)rs;
-    h[i].mass_pi = (double)mass[i] / kPI;
     h[i].pad = 0;
-    rmax[i] = rsize[i];
   }
   HBuild B;
   B.x = x;

@@ -338,6 +338,33 @@ __device__ inline void nfw_interp(cptr<NfwEntry> T, double y, double& g, double&
 // include/lens_halos.h:842-853.  x0, x1: position relative to the halo.
 // Divisions by r^2, pi, rscale and gmax are multiplications by reciprocals
 // (a last-bit difference from the reference's quotients).
+// LensHaloPowerLaw's force_halo_sym inside Rmax (rscale 1, xmax = Rsize,
+// include/lens_halos.h:979-997): alpha_h clamps x below 1e-6 xmax, kappa_h
+// and gamma_h only x = 0 (unreachable after the r^2 clamp).  Out of line, the
+// five terms returned by value: NFW-only trees keep their registers.
+struct Terms5 {
+  double ax, ay, k, g0, g1;
+};
+__device__ __noinline__ Terms5 powerlaw_inside(double xmax, double beta, double r2, double inv,
+                                               double prefac, double x0, double x1) {
+#pragma clang fp contract(off)
+  const double x = sqrt(r2);
+  const double xa = x < xmax * 1.0e-6 ? 1e-6 * xmax : x;
+  const double e = -beta + 2;
+  const double pa = pow(xa / xmax, e);
+  const double pk = xa == x ? pa : pow(x / xmax, e);
+  Terms5 t;
+  double tmp = -1.0 * pa * prefac;
+  t.ax = tmp * x0;
+  t.ay = tmp * x1;
+  t.k = 0.5 * e * pk * prefac;
+  tmp = -beta * pk * prefac * inv;
+  t.g0 = 0.5 * (x0 * x0 - x1 * x1) * tmp;
+  t.g1 = x0 * x1 * tmp;
+  return t;
+}
+
+template <bool PL>
 __device__ inline void nfw_force(const HaloNFW& h, cptr<NfwEntry> T, double x0, double x1,
                                  Acc64& a) {
 #pragma clang fp contract(off)
@@ -348,6 +375,15 @@ __device__ inline void nfw_force(const HaloNFW& h, cptr<NfwEntry> T, double x0, 
     const double inv = rcp64(r2);
     const double prefac = h.mass_pi * inv;
     if (r2 < R2) {
+      if (PL && h.pad != 0.0f) {   // a power-law halo: gmax holds beta
+        const Terms5 t = powerlaw_inside((double)h.rmax, h.gmax, r2, inv, prefac, x0, x1);
+        a.ax += t.ax;
+        a.ay += t.ay;
+        a.k += t.k;
+        a.g0 += t.g0;
+        a.g1 += t.g1;
+        return;
+      }
       const double x = sqrt(r2) * h.inv_rscale;
       double g, f, g2;
       nfw_interp(T, x, g, f, g2);
@@ -379,11 +415,12 @@ __device__ inline void nfw_force(const HaloNFW& h, cptr<NfwEntry> T, double x0, 
 // A halo of the tree with the uniform background the walk removes with it
 // (include/quadTreeHalos.h:668-693): the halo's force at halo - ray plus
 // inv_area m (halo - ray) and -inv_area m.
+template <bool PL>
 __device__ inline void halo_term(const HaloNFW& h, cptr<NfwEntry> T, double rx, double ry,
                                  double ia, Acc64& a) {
 #pragma clang fp contract(off)
   const double x0 = h.x - rx, x1 = h.y - ry;
-  nfw_force(h, T, x0, x1, a);
+  nfw_force<PL>(h, T, x0, x1, a);
   const double m = h.mass;
   const double prefac = m * ia;
   a.ax += prefac * x0;
@@ -399,6 +436,7 @@ __device__ __noinline__ double screening(double r2, double s2) { return exp(-r2 
 // 656-735) for the lanes in lvm: its big halos, then, for an internal cell,
 // the monopole + quadrupole where the cell is far.  Returns the lanes that
 // descend into its children.
+template <bool PL>
 __device__ __forceinline__ uint64_t evalh64(const HNode& nd, uint64_t lvm, double rx, double ry,
                                             double ia, const TreeDesc& t, Acc64& a) {
 #pragma clang fp contract(off)
@@ -408,7 +446,7 @@ __device__ __forceinline__ uint64_t evalh64(const HNode& nd, uint64_t lvm, doubl
   const cptr<NfwEntry> T = as_const(t.nfw);
   for (int k = 0; k < nd.nbig; ++k) {
     const HaloNFW h = halos[big[nd.big_off + k]];
-    if (lv) halo_term(h, T, rx, ry, ia, a);
+    if (lv) halo_term<PL>(h, T, rx, ry, ia, a);
   }
   if (nd.child < 0) return 0;
   const double dx = nd.cx - rx, dy = nd.cy - ry;
@@ -442,6 +480,7 @@ __device__ __forceinline__ uint64_t evalh64(const HNode& nd, uint64_t lvm, doubl
 // of a block are evaluated last to first and each one a lane descends into
 // is pushed at once, so the first cell's subtree is walked next (one copy of
 // the cell evaluation keeps the kernel's registers down).
+template <bool PL>
 __device__ void walk_halos64(const TreeDesc& t, double rx, double ry, bool active, Acc64& a,
                              int32_t* s_node, uint64_t* s_mask, uint32_t* err) {
   const int lane = lane_id();
@@ -463,7 +502,7 @@ __device__ void walk_halos64(const TreeDesc& t, double rx, double ry, bool activ
     for (int k = 3; k >= 0; --k) {
       if (!((cm >> k) & 1)) continue;
       const HNode nd = nodes[base + k];
-      const uint64_t om = evalh64(nd, live, rx, ry, ia, t, a);
+      const uint64_t om = evalh64<PL>(nd, live, rx, ry, ia, t, a);
       if (om) push_entry(s_node, s_mask, sp, lane, nd.child | (nd.cmask << 28), om, err);
     }
   }
@@ -471,17 +510,18 @@ __device__ void walk_halos64(const TreeDesc& t, double rx, double ry, bool activ
 
 // Every halo of the plane with its background term, no tree (the error
 // column of halo-of-halos planes).
+template <bool PL>
 __device__ void halos_direct64(const TreeDesc& t, double rx, double ry, bool active, Acc64& a) {
   if (!active) return;
   const cptr<HaloNFW> halos = as_const(t.halos);
   const cptr<NfwEntry> T = as_const(t.nfw);
-  for (int i = 0; i < t.nhalos; ++i) halo_term(halos[i], T, rx, ry, t.inv_area, a);
+  for (int i = 0; i < t.nhalos; ++i) halo_term<PL>(halos[i], T, rx, ry, t.inv_area, a);
 }
 
 // TreeQuadHalos::force2D (include/quadTreeHalos.h:557-592): the walk (or the
 // sum over every halo), with the periodic buffer nine of them at the ray
 // shifted by the halos' extents, x shift outer.
-template <bool DIRECT>
+template <bool DIRECT, bool PL>
 __device__ void halo_tree_force64(const TreeDesc& t, double rx, double ry, bool active, Acc64& a,
                                   int32_t* s_node, uint64_t* s_mask, uint32_t* err) {
 #pragma clang fp contract(off)
@@ -493,8 +533,8 @@ __device__ void halo_tree_force64(const TreeDesc& t, double rx, double ry, bool 
       sx = rx + (q / 3 - 1) * t.hxl;
       sy = ry + (q % 3 - 1) * t.hyl;
     }
-    if (DIRECT) halos_direct64(t, sx, sy, active, a);
-    else walk_halos64(t, sx, sy, active, a, s_node, s_mask, err);
+    if (DIRECT) halos_direct64<PL>(t, sx, sy, active, a);
+    else walk_halos64<PL>(t, sx, sy, active, a, s_node, s_mask, err);
   }
 }
 
@@ -928,7 +968,8 @@ __device__ inline double phys_coord(double th, double Dphys) {
 
 // One plane step of one 64-ray bundle (one work item); called by a whole
 // wavefront.
-template <int MODE, bool COUNT, bool HALOS>
+// HK: 0 no halo-of-halos trees, 1 NFW members only, 2 power-law members too
+template <int MODE, bool COUNT, int HK>
 __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, WorkItem it,
                                              int32_t* s_node, uint64_t* s_mask, uint64_t* s_near,
                                              uint32_t* __restrict__ err) {
@@ -954,7 +995,7 @@ __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, W
     // totals are live during the walks
     Acc32 a;
     // (a plane may hold halo-of-halos trees only; walk32 starts at trees[0])
-    if (!HALOS || ntrees > 0) walk32(jb.trees, ntrees, rx, ry, active, a, s_node, s_mask, s_near, err);
+    if (HK == 0 || ntrees > 0) walk32(jb.trees, ntrees, rx, ry, active, a, s_node, s_mask, s_near, err);
     double f[5];
     finish32(a, f);
     for (int h = 0; h < ntrees; ++h) sheet32(as_const(jb.trees)[h], rx, ry, f);
@@ -963,11 +1004,12 @@ __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, W
   // halo-of-halos trees: FP64 in every mode (LensHaloHalos::force_halo and
   // the plane each flip alpha, so the plane's alpha is the tree's raw alpha);
   // compiled only into the kernels launched for planes that have them
-  const int nhtrees = HALOS ? uni(jb.nhtrees) : 0;
+  const int nhtrees = HK > 0 ? uni(jb.nhtrees) : 0;
   for (int h = 0; h < nhtrees; ++h) {
     const TreeDesc t = as_const(jb.trees)[ntrees + h];
     Acc64 a;
-    halo_tree_force64<MODE == SLSIM_DIRECT64>(t, rx, ry, active, a, s_node, s_mask, err);
+    halo_tree_force64<MODE == SLSIM_DIRECT64, HK == 2>(t, rx, ry, active, a, s_node, s_mask,
+                                                      err);
     alpha0 += a.ax; alpha1 += a.ay; kappa += a.k; gamma0 += a.g0; gamma1 += a.g1;
   }
   for (int h = 0; h < (MODE == SLSIM_FP32C ? 0 : ntrees); ++h) {
@@ -999,7 +1041,7 @@ __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, W
 
 // One plane step for every bundle of the listed jobs.  Grid: one wavefront
 // per work item, 4 per block (items == null: job 0, bundle = item index).
-template <int MODE, bool COUNT, bool HALOS>
+template <int MODE, bool COUNT, int HK>
 __device__ __forceinline__ void plane_step_grid(const StepJob* __restrict__ jobs,
                                                 const WorkItem* __restrict__ items,
                                                 uint32_t nitems, uint32_t* __restrict__ err) {
@@ -1015,12 +1057,12 @@ __device__ __forceinline__ void plane_step_grid(const StepJob* __restrict__ jobs
     it.job = as_const(items)[wi].job;
     it.bundle = as_const(items)[wi].bundle;
   }
-  process_item<MODE, COUNT, HALOS>(jobs, it, s_node[w], s_mask[w], s_near[w], err);
+  process_item<MODE, COUNT, HK>(jobs, it, s_node[w], s_mask[w], s_near[w], err);
 }
 
 // Persistent variant: a fixed grid whose wavefronts take work items from a
 // queue counter until the list is exhausted (every wave reaches the exit).
-template <int MODE, bool HALOS>
+template <int MODE, int HK>
 __device__ __forceinline__ void plane_step_queue(const StepJob* __restrict__ jobs,
                                                  const WorkItem* __restrict__ items,
                                                  uint32_t nitems, uint32_t* __restrict__ counter,
@@ -1038,27 +1080,27 @@ __device__ __forceinline__ void plane_step_queue(const StepJob* __restrict__ job
     WorkItem it;
     it.job = as_const(items)[idx].job;
     it.bundle = as_const(items)[idx].bundle;
-    process_item<MODE, false, HALOS>(jobs, it, s_node[w], s_mask[w], s_near[w], err);
+    process_item<MODE, false, HK>(jobs, it, s_node[w], s_mask[w], s_near[w], err);
   }
 }
 
-template <int MODE, bool COUNT, bool HALOS>
+template <int MODE, bool COUNT, int HK>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
     const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t nitems,
     uint32_t* __restrict__ err) {
-  plane_step_grid<MODE, COUNT, HALOS>(jobs, items, nitems, err);
+  plane_step_grid<MODE, COUNT, HK>(jobs, items, nitems, err);
 }
 
-template <int MODE, bool HALOS>
+template <int MODE, int HK>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step_queue(
     const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t nitems,
     uint32_t* __restrict__ counter, uint32_t* __restrict__ err) {
-  plane_step_queue<MODE, HALOS>(jobs, items, nitems, counter, err);
+  plane_step_queue<MODE, HK>(jobs, items, nitems, counter, err);
 }
 
 // The FP32C walk at 8 waves per SIMD (64 VGPRs): measured faster than 7
 // waves without the few spills the cap costs.  Planes with halo-of-halos
-// trees use k_plane_step<SLSIM_FP32C, false, true> (no cap: the FP64 halo
+// trees use k_plane_step<SLSIM_FP32C, false, 1 or 2> (no cap: the FP64 halo
 // walk needs more registers).
 __global__ __launch_bounds__(kWave * kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(8)))
 void k_plane_step32(const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items,
@@ -1311,8 +1353,8 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_halo_force(
   const double rx = active ? rays[2 * i] : 0.0, ry = active ? rays[2 * i + 1] : 0.0;
   const TreeDesc t = *as_const(tree);
   Acc64 a;
-  if (direct) halo_tree_force64<true>(t, rx, ry, active, a, s_node[w], s_mask[w], err);
-  else halo_tree_force64<false>(t, rx, ry, active, a, s_node[w], s_mask[w], err);
+  if (direct) halo_tree_force64<true, true>(t, rx, ry, active, a, s_node[w], s_mask[w], err);
+  else halo_tree_force64<false, true>(t, rx, ry, active, a, s_node[w], s_mask[w], err);
   if (active) {
     out[5 * i] = a.ax; out[5 * i + 1] = a.ay; out[5 * i + 2] = a.k;
     out[5 * i + 3] = a.g0; out[5 * i + 4] = a.g1;
@@ -1363,61 +1405,60 @@ namespace {
 inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 }
 
+template <int HK>
+void launch_step_hk(int precision, bool count, unsigned nb, hipStream_t st, const StepJob* jobs,
+                    const WorkItem* items, uint32_t nitems, uint32_t* err) {
+  const dim3 blk(kWave * kWavesPerBlock);
+  if (count)
+    k_plane_step<SLSIM_FP64, true, HK><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+  else if (precision == SLSIM_FP32C && HK == 0)
+    k_plane_step32<<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+  else if (precision == SLSIM_FP32C)
+    k_plane_step<SLSIM_FP32C, false, HK><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+  else if (precision == SLSIM_DIRECT64)
+    k_plane_step<SLSIM_DIRECT64, false, HK><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+  else
+    k_plane_step<SLSIM_FP64, false, HK><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+}
+
 void launch_plane_step(int precision, bool count, uint32_t nitems, hipStream_t st,
-                       const StepJob* jobs, const WorkItem* items, uint32_t* err, bool halos) {
+                       const StepJob* jobs, const WorkItem* items, uint32_t* err, int halos) {
   const unsigned nb = blocks_for(nitems, kWavesPerBlock);
   if (nb == 0) return;
-  const dim3 blk(kWave * kWavesPerBlock);
-  if (halos) {
-    if (count)
-      k_plane_step<SLSIM_FP64, true, true><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
-    else if (precision == SLSIM_FP32C)
-      k_plane_step<SLSIM_FP32C, false, true><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
-    else if (precision == SLSIM_DIRECT64)
-      k_plane_step<SLSIM_DIRECT64, false, true><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
-    else
-      k_plane_step<SLSIM_FP64, false, true><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
-    return;
-  }
-  if (count)
-    k_plane_step<SLSIM_FP64, true, false><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
-  else if (precision == SLSIM_FP32C)
-    k_plane_step32<<<nb, blk, 0, st>>>(jobs, items, nitems, err);
-  else if (precision == SLSIM_DIRECT64)
-    k_plane_step<SLSIM_DIRECT64, false, false><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
-  else
-    k_plane_step<SLSIM_FP64, false, false><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+  if (halos == 2) launch_step_hk<2>(precision, count, nb, st, jobs, items, nitems, err);
+  else if (halos == 1) launch_step_hk<1>(precision, count, nb, st, jobs, items, nitems, err);
+  else launch_step_hk<0>(precision, count, nb, st, jobs, items, nitems, err);
 }
 
 void launch_zero_u32(hipStream_t st, uint32_t* p, int n) {
   k_zero_u32<<<(n + 255) / 256, 256, 0, st>>>(p, n);
 }
 
+template <int HK>
+void launch_queue_hk(int precision, unsigned nblocks, uint32_t nitems, hipStream_t st,
+                     const StepJob* jobs, const WorkItem* items, uint32_t* counter,
+                     uint32_t* err) {
+  const dim3 blk(kWave * kWavesPerBlock);
+  if (precision == SLSIM_FP32C && HK == 0)
+    k_plane_step_queue32<<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter, err);
+  else if (precision == SLSIM_FP32C)
+    k_plane_step_queue<SLSIM_FP32C, HK><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter,
+                                                                 err);
+  else if (precision == SLSIM_DIRECT64)
+    k_plane_step_queue<SLSIM_DIRECT64, HK><<<nblocks, blk, 0, st>>>(jobs, items, nitems,
+                                                                    counter, err);
+  else
+    k_plane_step_queue<SLSIM_FP64, HK><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter,
+                                                                err);
+}
+
 void launch_plane_step_queue(int precision, unsigned nblocks, uint32_t nitems, hipStream_t st,
                              const StepJob* jobs, const WorkItem* items, uint32_t* counter,
-                             uint32_t* err, bool halos) {
+                             uint32_t* err, int halos) {
   launch_zero_u32(st, counter, 1);
-  const dim3 blk(kWave * kWavesPerBlock);
-  if (halos) {
-    if (precision == SLSIM_FP32C)
-      k_plane_step_queue<SLSIM_FP32C, true><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter,
-                                                                     err);
-    else if (precision == SLSIM_DIRECT64)
-      k_plane_step_queue<SLSIM_DIRECT64, true><<<nblocks, blk, 0, st>>>(jobs, items, nitems,
-                                                                        counter, err);
-    else
-      k_plane_step_queue<SLSIM_FP64, true><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter,
-                                                                    err);
-    return;
-  }
-  if (precision == SLSIM_FP32C)
-    k_plane_step_queue32<<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter, err);
-  else if (precision == SLSIM_DIRECT64)
-    k_plane_step_queue<SLSIM_DIRECT64, false><<<nblocks, blk, 0, st>>>(jobs, items, nitems,
-                                                                       counter, err);
-  else
-    k_plane_step_queue<SLSIM_FP64, false><<<nblocks, blk, 0, st>>>(jobs, items, nitems, counter,
-                                                                   err);
+  if (halos == 2) launch_queue_hk<2>(precision, nblocks, nitems, st, jobs, items, counter, err);
+  else if (halos == 1) launch_queue_hk<1>(precision, nblocks, nitems, st, jobs, items, counter, err);
+  else launch_queue_hk<0>(precision, nblocks, nitems, st, jobs, items, counter, err);
 }
 
 void launch_init_grid(hipStream_t st, const RayState& s, int64_t nslots, int64_t N,

@@ -89,13 +89,16 @@ static_assert(sizeof(HNode) == 80, "HNode layout");
 // One symmetric LensHaloNFW (fratio 1) as the reference stores it: float
 // mass, Rmax and rscale, gmax = g(Rsize/rscale) from the tables; with the
 // reciprocals the walk multiplies by.
+// A symmetric halo of a halo tree.  NFW: gmax = g(xmax) of the tables.  A
+// power law (kind 1, LensHaloPowerLaw): gmax holds beta, rscale = 1, xmax =
+// rmax = Rsize.
 struct HaloNFW {
   double x, y;          // physical position (LensHalo::getX)
   double gmax;
   double inv_gmax;      // 1 / gmax
   double inv_rscale;    // 1 / rscale
   double mass_pi;       // mass / pi
-  float mass, rmax, rscale, pad;
+  float mass, rmax, rscale, pad;   // pad: the kind, 0 NFW, 1 power law
 };
 static_assert(sizeof(HaloNFW) == 64, "HaloNFW layout");
 
@@ -165,6 +168,8 @@ struct StepJob {
   int32_t j;                    // plane index of this step
   int32_t ntrees;               // particle trees, trees[0, ntrees)
   int32_t nhtrees;              // halo-of-halos trees, trees[ntrees, ntrees + nhtrees)
+  int32_t hkind;                // 0 no halo trees, 1 NFW members only, 2 power laws too
+  int32_t hpad;
   const TreeDesc* trees;        // the plane's halos (device array)
   double Dphys;                 // Dl[j] / (1 + z_j)
   double Gfac;                  // charge * Dl[j] / (1 + z_j)
@@ -260,6 +265,7 @@ struct slsim_plane_s {
   std::vector<slsim::HaloNFW> halos_host;
   double hs2 = 0, hxl = 0, hyl = 0;   // screening and the buffer's shifts (field planes)
   int hperiodic = 0;
+  int hpowerlaw = 0;                  // some member is a power law
   // build products kept for export and direct sums
   double* box_p1 = nullptr;           // 2 per node (p1x, p1y)
   double* box_p2 = nullptr;           // 2 per node
@@ -286,6 +292,7 @@ struct slsim_lens_s {
   std::vector<double> plane_redshifts, Dl, dDl;
   slsim::TreeDesc* trees_dev = nullptr;                // per plane: halos' descs
   std::vector<int> tree_off, tree_cnt, tree_hcnt;   // per plane: first desc, particle / halo trees
+  std::vector<int> tree_hkind;                     // per plane: StepJob::hkind
   bool trees_dirty = true;
   double coorDist(double z1, double z2) const;
   void combinePlanes();
@@ -327,11 +334,11 @@ struct FlatLCDM;
 // halos: some job of the launch has halo-of-halos trees (selects the kernels
 // that walk them)
 void launch_plane_step(int precision, bool count, uint32_t nitems, hipStream_t st,
-                       const StepJob* jobs, const WorkItem* items, uint32_t* err, bool halos);
+                       const StepJob* jobs, const WorkItem* items, uint32_t* err, int halos);
 void launch_zero_u32(hipStream_t st, uint32_t* p, int n);
 void launch_plane_step_queue(int precision, unsigned nblocks, uint32_t nitems, hipStream_t st,
                              const StepJob* jobs, const WorkItem* items, uint32_t* counter,
-                             uint32_t* err, bool halos);
+                             uint32_t* err, int halos);
 void launch_init_grid(hipStream_t st, const RayState& s, int64_t nslots, int64_t N,
                       int64_t tiles_x, int tw, double c0, double c1, double range, int32_t jmax,
                       double aal, double bbl);
@@ -350,9 +357,11 @@ void launch_finalize_array(hipStream_t st, const RayState& s, int64_t n, const u
 void launch_ray_morton(hipStream_t st, const double* x, int64_t n, double x0, double y0,
                        double inv_w, uint64_t* code, uint32_t* idx);
 void launch_minmax_rays(hipStream_t st, const double* x, int64_t n, double* part, int nblk);
+// kind[i]: 0 NFW (param = concentration, a float), 1 power law (param = beta);
+// kind == nullptr: all NFW
 int build_halo_plane(slsim_plane p, const double* x, const double* y, const float* mass,
-                     const float* rsize, const float* conc, size_t n, double inv_area,
-                     double theta_force);
+                     const float* rsize, const int32_t* kind, const double* param, size_t n,
+                     double inv_area, double theta_force);
 void launch_halo_force(hipStream_t st, const TreeDesc* tree, const double* rays, int64_t n,
                        int direct, double* out, uint32_t* err);
 void launch_force(hipStream_t st, int precision, const TreeDesc* tree, const double* rays,

@@ -286,3 +286,50 @@ def test_field_planes_lightcone_matches_oracle(ctx):
         assert np.array_equal(yb, ref[i][0]) and np.array_equal(Ab, ref[i][1]), i
     g.close()
     g32.close()
+
+
+# ------------------------------------------- power-law members (LensHaloPowerLaw)
+def _mixed_set(n, seed, frac=0.4):
+    x, y, m, rs, c = _halo_set(n, seed)
+    rng = np.random.default_rng(seed + 1000)
+    kinds = (rng.random(n) < frac).astype(np.int32)
+    params = np.where(kinds == 1, rng.uniform(-0.5, 1.8, n), c.astype(np.float64))
+    return x, y, m, rs, c, kinds, params
+
+
+@pytest.mark.parametrize("ia", [0.0, 1.5e-3])
+def test_mixed_halo_plane_force_matches_oracle(ctx, ia):
+    x, y, m, rs, c, kinds, params = _mixed_set(1800, 41)
+    print("case: mixed NFW / power-law halo plane inv_area=%g" % ia, flush=True)
+    h = S.LensHaloHalos.mixed(ctx, x, y, m, rs, kinds, params, 0.4, inv_area=ia, theta=0.1)
+    t = O.MixedTree(x, y, m, rs, kinds, params, inv_area=float(np.float32(ia)),
+                    theta=float(np.float32(0.1)))
+    rays = _rays(x, y, rs, c, 4096, 42)
+    _close(h.force(rays), t.force(rays))
+    _close(h.direct_sum(rays[:512]), t.force(rays[:512], direct=True))
+    with pytest.raises(S.SlsimError) as e:
+        bad = params.copy()
+        bad[np.flatnonzero(kinds == 1)[0]] = 2.0        # "bad gamma"
+        S.LensHaloHalos.mixed(ctx, x, y, m, rs, kinds, bad, 0.4)
+    assert e.value.code == 1
+
+
+def test_mixed_field_plane_lightcone_matches_oracle(ctx):
+    x, y, m, rs, c, kinds, params = _mixed_set(700, 43)
+    n = 16384
+    px, py = S.gen_nfw(92, n, 1e13, 8.0, 0.5)
+    pm = np.full(n, np.float32(1e13 / n), np.float32)
+    ph = np.full(n, np.float32(2e-4), np.float32)
+    lens = S.Lens(ctx, 2.0)
+    lens.moveinMainHalo(S.LensHaloParticles(ctx, px, py, pm, ph, 0.5))
+    lens.addFieldPlane(S.LensPlaneTree.mixed(ctx, 0.8123, x, y, m, rs, kinds, params, 60.0, 2.5))
+    L = O.Lens(2.0)
+    L.moveinMainHalo(O.Tree(px, py, pm, ph), 0.5)
+    L.addFieldPlane(O.MixedTree(x, y, m, rs, kinds, params, 60.0, 0.1, 2.5, True), 0.8123)
+    print("case: light-cone with a mixed screened field plane, GridMap 64", flush=True)
+    N, rg = 64, W.grid_range(60.0)
+    g = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    yr, Ar = L.shoot(O.gridmap_positions(N, (0.0, 0.0), rg))
+    assert np.abs(g.y() - yr).max() <= 1e-12 * np.abs(yr).max()
+    assert np.abs(g.A() - Ar).max() <= 1e-8
+    g.close()
