@@ -527,17 +527,11 @@ enum class EllipMethod { Fourier, Pseudo, Schramm, Keeton };
 // 389-440) as a member of a LensHaloHalos: symmetric profiles only (fratio
 // 1; others throw std::invalid_argument).  The position is set in angle
 // with setTheta and is theta * D_A(zlens) in physical Mpc, as the reference's.
-class LensHaloNFW {
+// include/lens_halos.h: what a halo tree reads of a LensHalo (position,
+// mass, Rsize) and the kind and parameter of its profile.
+class LensHalo {
  public:
-  LensHaloNFW(float my_mass, float my_Rsize, PosType my_zlens, float my_concentration,
-              float my_fratio, float my_pa, const COSMOLOGY& cosmo,
-              EllipMethod my_ellip_method = EllipMethod::Pseudo)
-      : mass(my_mass), Rsize(my_Rsize), zlens(my_zlens), concentration(my_concentration) {
-    (void)my_pa; (void)my_ellip_method;
-    if (my_fratio != 1)
-      throw std::invalid_argument("LensHaloNFW: only fratio = 1 (symmetric) halos are built");
-    Dist = cosmo.angDist(my_zlens);
-  }
+  virtual ~LensHalo() = default;
   void setTheta(PosType PosX, PosType PosY) { posHalo[0] = PosX; posHalo[1] = PosY; }
   void getX(PosType* MyPosHalo) const {
     MyPosHalo[0] = posHalo[0] * Dist;
@@ -548,22 +542,68 @@ class LensHaloNFW {
   float getRsize() const { return Rsize; }
   float get_Rmax() const { return Rsize; }
   PosType getZlens() const { return zlens; }
-  float getConcentration() const { return concentration; }
+  // SLSIM_HALO_* and its parameter (concentration or beta)
+  virtual int32_t profile_kind() const = 0;
+  virtual double profile_param() const = 0;
 
- private:
+ protected:
+  LensHalo(float my_mass, float my_Rsize, PosType my_zlens, const COSMOLOGY& cosmo)
+      : mass(my_mass), Rsize(my_Rsize), zlens(my_zlens) {
+    Dist = cosmo.angDist(my_zlens);
+  }
   float mass, Rsize;
   PosType zlens;
-  float concentration;
   PosType Dist = -1;
   PosType posHalo[2] = {0, 0};
+};
+
+// MultiPlane/lens_halos.cpp:389-440 (symmetric: fratio = 1)
+class LensHaloNFW : public LensHalo {
+ public:
+  LensHaloNFW(float my_mass, float my_Rsize, PosType my_zlens, float my_concentration,
+              float my_fratio, float my_pa, const COSMOLOGY& cosmo,
+              EllipMethod my_ellip_method = EllipMethod::Pseudo)
+      : LensHalo(my_mass, my_Rsize, my_zlens, cosmo), concentration(my_concentration) {
+    (void)my_pa; (void)my_ellip_method;
+    if (my_fratio != 1)
+      throw std::invalid_argument("LensHaloNFW: only fratio = 1 (symmetric) halos are built");
+  }
+  float getConcentration() const { return concentration; }
+  int32_t profile_kind() const override { return SLSIM_HALO_NFW; }
+  double profile_param() const override { return concentration; }
+
+ private:
+  float concentration;
+};
+
+// MultiPlane/lens_halos.cpp:805-866 (symmetric: fratio = 1): kappa ~ r^-beta
+// inside Rsize, beta < 2
+class LensHaloPowerLaw : public LensHalo {
+ public:
+  LensHaloPowerLaw(float my_mass, float my_Rsize, PosType my_zlens, PosType my_beta,
+                   float my_fratio, float my_pa, const COSMOLOGY& cosmo,
+                   EllipMethod my_ellip_method = EllipMethod::Pseudo)
+      : LensHalo(my_mass, my_Rsize, my_zlens, cosmo), beta(my_beta) {
+    (void)my_pa; (void)my_ellip_method;
+    if (my_beta >= 2.0) throw std::invalid_argument("bad gamma");
+    if (my_Rsize <= 0.0 || std::isnan(my_Rsize)) throw std::runtime_error("Bad halo");
+    if (my_fratio != 1)
+      throw std::invalid_argument("LensHaloPowerLaw: only fratio = 1 (symmetric) halos are built");
+  }
+  int32_t profile_kind() const override { return SLSIM_HALO_POWERLAW; }
+  double profile_param() const override { return beta; }
+
+ private:
+  PosType beta;
 };
 
 // LensHaloHalos<HType> (include/particle_halo.h:1092-1227): a main halo made
 // of halos on a TreeQuadHalos; the tree is built and walked by the library.
 template <typename HType>
 class LensHaloHalos {
-  static_assert(std::is_same<HType, LensHaloNFW>::value,
-                "LensHaloHalos is built for LensHaloNFW members");
+  static_assert(std::is_same<HType, LensHaloNFW>::value ||
+                    std::is_same<HType, LensHaloPowerLaw>::value,
+                "LensHaloHalos is built for LensHaloNFW and LensHaloPowerLaw members");
 
  public:
   LensHaloHalos(std::vector<HType>& pvector, float redshift, const COSMOLOGY& cosmo,
@@ -575,8 +615,9 @@ class LensHaloHalos {
       throw std::invalid_argument("LensHaloHalos: the periodic buffer is not built");
     std::swap(pvector, trash_collector);
     const size_t n = trash_collector.size();
-    std::vector<double> x(n), y(n);
-    std::vector<float> m(n), r(n), c(n);
+    std::vector<double> x(n), y(n), prm(n);
+    std::vector<float> m(n), r(n);
+    std::vector<int32_t> kind(n);
     mass = 0;
     for (size_t i = 0; i < n; ++i) {
       double p[2];
@@ -585,12 +626,13 @@ class LensHaloHalos {
       y[i] = p[1];
       m[i] = trash_collector[i].get_mass();
       r[i] = trash_collector[i].getRsize();
-      c[i] = trash_collector[i].getConcentration();
+      kind[i] = trash_collector[i].profile_kind();
+      prm[i] = trash_collector[i].profile_param();
       mass += m[i];
     }
-    slsim_cpp::check(slsim_halos_create_nfw(slsim_cpp::context(), x.data(), y.data(), m.data(),
-                                            r.data(), c.data(), n, inv_area, my_theta_force,
-                                            &plane));
+    slsim_cpp::check(slsim_halos_create(slsim_cpp::context(), x.data(), y.data(), m.data(),
+                                        r.data(), kind.data(), prm.data(), n, inv_area,
+                                        my_theta_force, 0.0, 0, &plane));
   }
   LensHaloHalos(LensHaloHalos&& h) noexcept
       : zlens(h.zlens), mass(h.mass), trash_collector(std::move(h.trash_collector)),
@@ -624,11 +666,12 @@ class LensHaloHalos {
 // at a physical position; phi and gamma[2] are 0.
 class LensPlaneTree {
  public:
-  LensPlaneTree(float z_, LensHaloNFW** my_halos, IndexType Nhalos, PosType my_sigma_background,
+  LensPlaneTree(float z_, LensHalo** my_halos, IndexType Nhalos, PosType my_sigma_background,
                 PosType my_inv_screening_scale = 0)
       : z(z_), halos(my_halos, my_halos + Nhalos) {
-    std::vector<double> x(Nhalos), y(Nhalos);
-    std::vector<float> m(Nhalos), r(Nhalos), c(Nhalos);
+    std::vector<double> x(Nhalos), y(Nhalos), prm(Nhalos);
+    std::vector<float> m(Nhalos), r(Nhalos);
+    std::vector<int32_t> kind(Nhalos);
     for (IndexType i = 0; i < Nhalos; ++i) {
       double p[2];
       my_halos[i]->getX(p);
@@ -636,12 +679,15 @@ class LensPlaneTree {
       y[i] = p[1];
       m[i] = my_halos[i]->get_mass();
       r[i] = my_halos[i]->getRsize();
-      c[i] = my_halos[i]->getConcentration();
+      kind[i] = my_halos[i]->profile_kind();
+      prm[i] = my_halos[i]->profile_param();
     }
-    slsim_cpp::check(slsim_plane_tree_create_nfw(slsim_cpp::context(), x.data(), y.data(),
-                                                 m.data(), r.data(), c.data(), Nhalos,
-                                                 my_sigma_background, my_inv_screening_scale,
-                                                 &plane));
+    // planes.cpp:13-19: screened trees take theta 0.1 and the periodic buffer
+    const bool scr = my_inv_screening_scale != 0;
+    slsim_cpp::check(slsim_halos_create(slsim_cpp::context(), x.data(), y.data(), m.data(),
+                                        r.data(), kind.data(), prm.data(), Nhalos,
+                                        my_sigma_background, scr ? 0.1 : 0.5,
+                                        my_inv_screening_scale, scr, &plane));
   }
   LensPlaneTree(LensPlaneTree&& p) noexcept : z(p.z), halos(std::move(p.halos)), plane(p.plane) {
     p.plane = nullptr;
@@ -661,7 +707,7 @@ class LensPlaneTree {
     gamma[2] = 0;
     *phi = 0;
   }
-  std::vector<LensHaloNFW*> getHalos() { return halos; }
+  std::vector<LensHalo*> getHalos() { return halos; }
   slsim_plane release_plane() {
     slsim_plane p = plane;
     plane = nullptr;
@@ -670,7 +716,7 @@ class LensPlaneTree {
   float z;
 
  private:
-  std::vector<LensHaloNFW*> halos;
+  std::vector<LensHalo*> halos;
   slsim_plane plane = nullptr;
 };
 

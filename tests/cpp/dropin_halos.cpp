@@ -6,8 +6,9 @@
 // argv[3], the critical curves (GridMap::find_crit) and the contours of
 // GridMap::find_magnification_contour at invmag = 0.5, each curve as
 // (type, hits_boundary, n, n points); with argv[4], a lens of one screened
-// LensPlaneTree field plane of the same halos at z = 0.9: its force at one
-// position (5 values) and a 32 x 32 GridMap's x, y, A.
+// LensPlaneTree field plane of the same halos and 30 LensHaloPowerLaw halos
+// at z = 0.9: its force at one position (5 values) and a 32 x 32 GridMap's
+// x, y, A.
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -95,8 +96,16 @@ int main(int argc, char** argv) {
     std::fclose(g);
   }
   if (argc > 4) {
-    std::vector<LensHaloNFW*> ptrs;
+    // the NFW halos and 30 power-law halos (LensHaloPowerLaw) on one field plane
+    std::vector<LensHaloPowerLaw> pls;
+    for (int i = 0; i < 30; ++i) {
+      pls.emplace_back((float)std::pow(10.0, 10.0 + 0.05 * i), (float)(0.01 + 0.001 * i), 0.3,
+                       -0.4 + 0.07 * i, 1.0f, 0.0f, cosmo);
+      pls.back().setTheta(((i % 6) - 2.5) * 7e-6, ((i / 6) - 2.0) * 6e-6);
+    }
+    std::vector<LensHalo*> ptrs;
     for (LensHaloNFW& h : nfw2) ptrs.push_back(&h);
+    for (LensHaloPowerLaw& h : pls) ptrs.push_back(&h);
     LensPlaneTree fp(0.9f, ptrs.data(), ptrs.size(), 100.0, 2.0);
     double xx[2] = {1e-3, -2e-3}, a[2], k, g[3], ph;
     fp.force(a, &k, g, &ph, xx);

@@ -207,7 +207,17 @@ def test_cpp_dropin_halos_program(ctx, tmp_path):
     assert at == len(q)
     # LensPlaneTree from C++: its force and a lens of it equal the Python view's
     f = np.fromfile(fpl, dtype=np.float64)
-    fp = S.LensPlaneTree(ctx, 0.9, hx, hy, m, rs, c, 100.0, 2.0)
+    j = np.arange(30)
+    pm_ = np.array([np.float32(10.0 ** (10.0 + 0.05 * k)) for k in j])
+    pr_ = np.array([np.float32(0.01 + 0.001 * k) for k in j])
+    pb_ = np.array([-0.4 + 0.07 * k for k in j])
+    px_ = ((j % 6) - 2.5) * 7e-6 * dist
+    py_ = ((j // 6) - 2.0) * 6e-6 * dist
+    fp = S.LensPlaneTree.mixed(
+        ctx, 0.9, np.concatenate([hx, px_]), np.concatenate([hy, py_]),
+        np.concatenate([m, pm_]), np.concatenate([rs, pr_]),
+        np.concatenate([np.zeros(hx.size, np.int32), np.ones(30, np.int32)]),
+        np.concatenate([c.astype(np.float64), pb_]), 100.0, 2.0)
     assert np.array_equal(f[:5], fp.force(np.array([[1e-3, -2e-3]]))[0])
     lens2 = S.Lens(ctx, 2.0)
     lens2.addFieldPlane(fp)
