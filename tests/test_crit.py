@@ -252,3 +252,17 @@ def test_product_hull_and_in_curve_equal_oracle():
     poly = np.stack([np.cos(t), np.sin(3 * t) * 0.5 + np.sin(t)], 1)
     q = np.concatenate([rng.uniform(-1.5, 1.5, size=(2000, 2)), poly, poly[::2] * [1.0, 0.0]])
     assert np.array_equal(S.in_curve(poly, q), O.in_curve(poly, q))
+
+
+def test_product_find_boundaries_small_and_thin():
+    import slsimlib_amd as S
+    rng = np.random.default_rng(4)
+    for shape in ((1, 1), (1, 7), (7, 1), (2, 2), (3, 3), (3, 9), (4, 2)):
+        for _ in range(5):
+            bm = rng.random(shape) < 0.6
+            a, ha = S.find_boundaries(bm)
+            b, hb = O.find_boundaries(bm)
+            assert len(a) == len(b) and np.array_equal(ha, hb)
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y)
+    assert S.find_boundaries(np.zeros((0, 5), bool))[0] == []

@@ -181,3 +181,24 @@ def test_find_images_nonfinite_corner_and_spread_sources(ctx):
     assert np.array_equal(rays, r0, equal_nan=True)
     assert mult.max() >= 3
     f.close()
+
+
+@pytest.mark.parametrize("N", [1, 2, 3])
+def test_tiny_grids(ctx, N):
+    # grids too small to hold a boundary: no curves; find_crit still reports
+    # whether it replaced hits_boundary (any critical pixel, edges included)
+    x = O.gridmap_positions(N, (0.0, 0.0), 1.0)
+    y, A = _isothermal(x, [((0.01, 0.02), 2.0, 0.0)])
+    f = _Field(ctx, N, (0.0, 0.0), 1.0, y, A)
+    f.find_images = S.GridMap.find_images.__get__(f)
+    print("case: tiny grid N=%d" % N, flush=True)
+    _same(f.find_crit(), O.find_crit(N, x, y, A, 1.0))
+    _same(f.find_magnification_contour(0.0), O.find_magnification_contour(N, x, A, 1.0, 0.0))
+    c, h, mult = f.find_boundaries_of_caustics()
+    c0, h0, m0 = O.find_boundaries_of_caustics(N, x, y, A)
+    assert np.array_equal(mult, m0) and len(c) == len(c0) == 0
+    ys = np.array([[0.0, 0.0], [0.3, -0.2]])
+    rays, mult, tris = f.find_images(ys, 16)
+    r0, m1, t0 = O.find_images(N, x, y, A, ys)
+    assert np.array_equal(mult, m1) and np.array_equal(rays, r0) and np.array_equal(tris, t0)
+    f.close()
