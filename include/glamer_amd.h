@@ -1072,6 +1072,57 @@ struct GridMap {
     }
   }
 
+  // gridmap.cpp:26-91: Nx x Ny points, Ny = (int)(Nx rangeY / rangeX), square
+  // cells of rangeX / (Nx - 1), point ii + jj Nx at (xo + ii res, yo + jj res)
+  // from the lower-left corner (xo, yo); shot as a ray list.  find_images,
+  // find_crit and the contours are built for N x N grids only.
+  GridMap(LensHndl lens, unsigned long Nx, const PosType center_[2], PosType rangeX,
+          PosType rangeY)
+      : Ngrid_init(Nx), x_range(rangeX), rect(true) {
+    if (Nx <= 0) throw std::runtime_error("cannot make GridMap with no points");
+    if (rangeX <= 0 || rangeY <= 0) throw std::runtime_error("cannot make GridMap with no range");
+    center[0] = center_[0];
+    center[1] = center_[1];
+    axisratio = rangeY / rangeX;
+    Ngrid_init2 = (int)(Ngrid_init * axisratio);
+    const size_t n = (size_t)Ngrid_init * Ngrid_init2;
+    i_points.resize(n);
+    s_points.resize(n);
+    if (n == 0) return;
+    i_points[0].head = n;
+    s_points[0].head = n;
+    const double xo = center[0] - 0.5 * rangeX, yo = center[1] - 0.5 * rangeY;
+    const double res = rangeX / (Ngrid_init - 1);
+    std::vector<double> x(2 * n), y(2 * n), A(4 * n);
+    for (long ii = 0; ii < Ngrid_init; ++ii) {
+      const double xx = xo + ii * res;
+      for (long jj = 0; jj < Ngrid_init2; ++jj) {
+        const size_t i = (size_t)(ii + jj * Ngrid_init);
+        i_points[i].id = i;
+        i_points[i].x[0] = xx;
+        i_points[i].x[1] = yo + jj * res;
+        i_points[i].gridsize = res;
+        x[2 * i] = i_points[i].x[0];
+        x[2 * i + 1] = i_points[i].x[1];
+      }
+    }
+    {
+      std::lock_guard<std::mutex> hold(grid_mutex());
+      slsim_cpp::check(slsim_shoot(lens->handle(), n, x.data(), nullptr, y.data(), A.data(),
+                                   slsim_cpp::precision(), nullptr));
+    }
+    for (size_t i = 0; i < n; ++i) {
+      s_points[i].id = i;
+      s_points[i].gridsize = res;
+      i_points[i].image = &s_points[i];
+      s_points[i].image = &i_points[i];
+      s_points[i].x[0] = y[2 * i];
+      s_points[i].x[1] = y[2 * i + 1];
+      for (int c = 0; c < 4; ++c) i_points[i].A.a[c] = A[4 * i + c];
+      s_points[i].A = i_points[i].A;
+    }
+  }
+
   GridMap(const GridMap&) = delete;
   GridMap& operator=(const GridMap&) = delete;
   ~GridMap() { release_device(); }
@@ -1087,6 +1138,7 @@ struct GridMap {
   // gridmap.cpp:927-963, 1326-1348: the images of each source position, as
   // rays (x the image, y the source, A the mean over the image's triangle)
   std::list<RAY> find_images(std::vector<Point_2d>& ys, std::vector<int>& multiplicity) const {
+    square_only();
     std::vector<double> r;
     std::vector<int64_t> t;
     images(ys, multiplicity, r, t);
@@ -1103,6 +1155,7 @@ struct GridMap {
   // gridmap.cpp:1034-1098
   void find_images(Point_2d y, std::vector<Point_2d>& image_points,
                    std::vector<Triangle>& triangles) const {
+    square_only();
     std::vector<Point_2d> ys(1, y);
     std::vector<int> mult;
     std::vector<double> r;
@@ -1122,6 +1175,7 @@ struct GridMap {
   // the device, the boundary walk on the host
   void find_crit(std::vector<std::vector<Point_2d>>& points, std::vector<bool>& hits_boundary,
                  std::vector<CritType>& crit_type) {
+    square_only();
     slsim_curves h = nullptr;
     slsim_cpp::check(slsim_gridmap_find_crit(slsim_cpp::context(), Ngrid_init, center[0],
                                              center[1], x_range, (const double*)yd,
@@ -1136,6 +1190,7 @@ struct GridMap {
   // gridmap.cpp:493-522
   void find_magnification_contour(std::vector<std::vector<Point_2d>>& curves,
                                   std::vector<bool>& hits_boundary, double invmag) {
+    square_only();
     slsim_curves h = nullptr;
     slsim_cpp::check(slsim_gridmap_find_magnification_contour(
         slsim_cpp::context(), Ngrid_init, center[0], center[1], x_range, (const double*)Ad, invmag,
@@ -1149,6 +1204,7 @@ struct GridMap {
   // more than one image as sources
   void find_boundaries_of_caustics(std::vector<std::vector<Point_2d>>& boundaries,
                                    std::vector<bool>& hits_edge) {
+    square_only();
     slsim_curves h = nullptr;
     slsim_cpp::check(slsim_gridmap_find_boundaries_of_caustics(
         slsim_cpp::context(), Ngrid_init, center[0], center[1], x_range, (const double*)yd,
@@ -1162,6 +1218,7 @@ struct GridMap {
   size_t getNumberOfPoints() const { return (size_t)Ngrid_init * Ngrid_init2; }
   int getInitNgrid() const { return Ngrid_init; }
   double getXRange() const { return x_range; }
+  double getYRange() const { return x_range * axisratio; }
   double getResolution() const { return x_range / (Ngrid_init - 1); }
   Point* operator[](size_t i) { return i_points.data() + i; }
   Point_2d image_point(size_t i) { return Point_2d(i_points[i].x[0], i_points[i].x[1]); }
@@ -1239,8 +1296,15 @@ struct GridMap {
     }
     multiplicity.assign(m.begin(), m.end());
   }
+  void square_only() const {
+    if (rect)
+      throw std::invalid_argument("GridMap: image and critical-curve finding are built for "
+                                  "N x N grids");
+  }
   int Ngrid_init, Ngrid_init2;
   double x_range;
+  double axisratio = 1.0;
+  bool rect = false;
   double center[2];
   std::vector<Point> i_points;
   std::vector<Point> s_points;

@@ -1,7 +1,8 @@
 // Reference-style user program against the drop-in header: builds a
 // projected-NFW particle halo, moves it into a Lens, shoots a GridMap and a
 // list of RAYs with per-ray source redshifts, and writes the results as raw
-// float64 to the file named by argv[1] (read back by the parity test).
+// float64 to the file named by argv[1] (read back by the parity test); with
+// argv[2], a rectangular GridMap's rays there.
 #include <cstdio>
 #include <vector>
 
@@ -50,6 +51,30 @@ int main(int argc, char** argv) {
   for (auto& r : rays)
     out.insert(out.end(), {r.x[0], r.x[1], r.y[0], r.y[1], r.A[0], r.A[1], r.A[2], r.A[3]});
   out.push_back(kap[0]);
+  if (argc > 2) {
+    // the rectangular constructor (gridmap.cpp:26-91): 40 x 24 points
+    const double c2[2] = {1e-6, -2e-6};
+    GridMap rect(&lens, 40, c2, 30 * arcsecTOradians, 18.5 * arcsecTOradians);
+    std::vector<double> w = {(double)rect.getInitNgrid(), (double)rect.getNumberOfPoints()};
+    for (size_t i = 0; i < rect.getNumberOfPoints(); ++i) {
+      RAY r = rect.ray(i);
+      w.insert(w.end(), {r.x[0], r.x[1], r.y[0], r.y[1], r.A[0], r.A[1], r.A[2], r.A[3]});
+    }
+    bool threw = false;
+    try {
+      std::vector<std::vector<Point_2d>> cv;
+      std::vector<bool> hb;
+      std::vector<CritType> ct;
+      rect.find_crit(cv, hb, ct);
+    } catch (const std::invalid_argument&) {
+      threw = true;
+    }
+    w.push_back(threw);
+    FILE* g = std::fopen(argv[2], "wb");
+    if (!g) return 1;
+    std::fwrite(w.data(), sizeof(double), w.size(), g);
+    std::fclose(g);
+  }
   FILE* f = std::fopen(argv[1], "wb");
   if (!f) return 1;
   std::fwrite(out.data(), sizeof(double), out.size(), f);

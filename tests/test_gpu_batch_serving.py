@@ -138,7 +138,8 @@ def test_cpp_dropin_program(ctx, tmp_path):
     if not os.path.exists(exe):
         pytest.fail("tests/cpp/dropin_gridmap is not built (run __graft_entry__.build())")
     out = tmp_path / "dropin.bin"
-    r = subprocess.run([exe, str(out)], capture_output=True, text=True, timeout=120)
+    rect = tmp_path / "rect.bin"
+    r = subprocess.run([exe, str(out), str(rect)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     v = np.fromfile(out, dtype=np.float64)
     grid = v[:64 * 64 * 8].reshape(-1, 8)
@@ -157,6 +158,21 @@ def test_cpp_dropin_program(ctx, tmp_path):
     yr, Ar = L.shoot(rays[:, :2], zs=zs)
     assert np.abs(rays[:, 2:4] - yr).max() <= 1e-12 * np.abs(yr).max()
     assert np.abs(rays[:, 4:8] - Ar).max() <= 1e-8
+    # the rectangular GridMap: the reference's positions (lower-left corner,
+    # square cells of rangeX / (Nx - 1), Ny = int(Nx rangeY / rangeX))
+    q = np.fromfile(rect, dtype=np.float64)
+    nx, npts = int(q[0]), int(q[1])
+    rx_, ry_ = 30 * S.arcsecTOradians, 18.5 * S.arcsecTOradians
+    ny = int(nx * (ry_ / rx_))
+    assert (nx, npts) == (40, 40 * ny) and q[-1] == 1.0     # find_crit refuses it
+    v2 = q[2:-1].reshape(-1, 8)
+    res = rx_ / (nx - 1)
+    ii, jj = np.arange(npts) % nx, np.arange(npts) // nx
+    assert np.array_equal(v2[:, 0], (1e-6 - 0.5 * rx_) + ii * res)
+    assert np.array_equal(v2[:, 1], (-2e-6 - 0.5 * ry_) + jj * res)
+    yr, Ar = L.shoot(v2[:, :2])
+    assert np.abs(v2[:, 2:4] - yr).max() <= 1e-12 * np.abs(yr).max()
+    assert np.abs(v2[:, 4:8] - Ar).max() <= 1e-8
 
 
 def test_server_ray_requests_equal_rayshooter(ctx, lenses):
