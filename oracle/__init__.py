@@ -69,6 +69,11 @@ _sigs = {
     "orc_lens_tables": (None, [c_vp, P(c_d), P(c_d), P(c_d), P(c_d)]),
     "orc_lens_shoot": (None, [c_vp, c_sz, P(c_d), P(c_d), P(c_d), P(c_d), c_int, c_vp]),
     "orc_gridmap_positions": (None, [c_sz, c_d, c_d, c_d, P(c_d)]),
+    "orc_find_images2": (c_sz, [c_sz, c_sz, P(c_d), P(c_d), P(c_d), P(c_d), c_sz,
+                                P(ctypes.c_int32), P(c_d), P(ctypes.c_int64), c_sz]),
+    "orc_find_crit2": (c_sz, [c_int, c_sz, c_sz, P(c_d), P(c_d), P(c_d), P(ctypes.c_int32), c_d,
+                              c_d, c_sz, c_sz, P(ctypes.c_int64), P(c_d), P(ctypes.c_int32),
+                              P(ctypes.c_int32), P(c_sz), P(c_sz), P(c_int)]),
     "orc_find_crit": (c_sz, [c_int, c_sz, P(c_d), P(c_d), P(c_d), P(ctypes.c_int32), c_d, c_d,
                              c_sz, c_sz, P(ctypes.c_int64), P(c_d), P(ctypes.c_int32),
                              P(ctypes.c_int32), P(c_sz), P(c_sz), P(c_int)]),
@@ -332,10 +337,25 @@ def gridmap_positions(N, center, range_):
     return x
 
 
-def find_images(N, x, y, A, ys, cap=None):
-    """GridMap::find_images over a grid (x, y: (N*N, 2), A: (N*N, 4)) for
-    source positions ys (n, 2): (rays (m, 8) = x, y, A; multiplicity;
-    triangles (m, 3)), sources in order."""
+def gridmap_rect_positions(nx, center, range_x, range_y):
+    """GridMap(lens, Nx, center, rangeX, rangeY)'s points
+    (TreeCode_link/gridmap.cpp:26-91): ny = int(nx rangeY / rangeX) rows,
+    point ii + jj nx at (x0 + ii res, y0 + jj res), res = rangeX / (nx - 1),
+    (x0, y0) the lower-left corner.  Returns (positions (nx*ny, 2), ny)."""
+    ny = int(nx * (range_y / range_x))
+    xo = center[0] - 0.5 * range_x
+    yo = center[1] - 0.5 * range_y
+    res = range_x / (nx - 1)
+    k = np.arange(nx * ny)
+    return np.stack([xo + (k % nx) * res, yo + (k // nx) * res], 1), ny
+
+
+def find_images(N, x, y, A, ys, cap=None, ny=None):
+    """GridMap::find_images over a grid (x, y: (N*ny, 2), A: (N*ny, 4); ny
+    rows of N points, ny = N by default) for source positions ys (n, 2):
+    (rays (m, 8) = x, y, A; multiplicity; triangles (m, 3)), sources in
+    order."""
+    ny = N if ny is None else ny
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(y, dtype=np.float64)
     A = np.ascontiguousarray(A, dtype=np.float64)
@@ -345,7 +365,7 @@ def find_images(N, x, y, A, ys, cap=None):
     mult = np.zeros(ns, np.int32)
     rays = np.zeros((cap, 8))
     tris = np.zeros((cap, 3), np.int64)
-    n = _lib.orc_find_images(N, _p(x), _p(y), _p(A), _p(ys), ns,
+    n = _lib.orc_find_images2(N, ny, _p(x), _p(y), _p(A), _p(ys), ns,
                              mult.ctypes.data_as(P(ctypes.c_int32)), _p(rays),
                              tris.ctypes.data_as(P(ctypes.c_int64)), cap)
     if n > cap:
@@ -414,19 +434,20 @@ def in_curve(poly, points):
     return out.astype(bool)
 
 
-def _crit(mode, N, x, y, A, mult, range_, invmag):
+def _crit(mode, N, x, y, A, mult, range_, invmag, ny=None):
+    ny = N if ny is None else ny
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(y if y is not None else x, dtype=np.float64)
-    A = np.ascontiguousarray(A if A is not None else np.zeros((N * N, 4)), dtype=np.float64)
-    m = np.ascontiguousarray(mult if mult is not None else np.zeros(N * N), dtype=np.int32)
-    cc, cp = 16, 4 * N * N + 16
+    A = np.ascontiguousarray(A if A is not None else np.zeros((N * ny, 4)), dtype=np.float64)
+    m = np.ascontiguousarray(mult if mult is not None else np.zeros(N * ny), dtype=np.int32)
+    cc, cp = 16, 4 * N * ny + 16
     while True:
         off = np.zeros(cc + 1, np.int64)
         pts = np.zeros((cp, 2))
         types = np.zeros(cc, np.int32)
         hits = np.zeros(cc, np.int32)
         nh, npt, rep = c_sz(), c_sz(), c_int()
-        nc = _lib.orc_find_crit(mode, N, _p(x), _p(y), _p(A), _p(m, ctypes.c_int32), range_,
+        nc = _lib.orc_find_crit2(mode, N, ny, _p(x), _p(y), _p(A), _p(m, ctypes.c_int32), range_,
                                 invmag, cc, cp, _p(off, ctypes.c_int64), _p(pts),
                                 _p(types, ctypes.c_int32), _p(hits, ctypes.c_int32),
                                 ctypes.byref(nh), ctypes.byref(npt), ctypes.byref(rep))
@@ -436,20 +457,22 @@ def _crit(mode, N, x, y, A, mult, range_, invmag):
         cc, cp = max(cc, nc), max(cp, npt.value)
 
 
-def find_crit(N, x, y, A, range_):
-    """GridMap::find_crit over a grid (x, y (N*N, 2), A (N*N, 4)): (curves,
-    hits_boundary, crit types, whether hits_boundary was replaced)."""
-    return _crit(0, N, x, y, A, None, range_, 0.0)
+def find_crit(N, x, y, A, range_, ny=None):
+    """GridMap::find_crit over a grid (x, y (N*ny, 2), A (N*ny, 4); range_
+    the x range): (curves, hits_boundary, crit types, whether hits_boundary
+    was replaced)."""
+    return _crit(0, N, x, y, A, None, range_, 0.0, ny)
 
 
-def find_magnification_contour(N, x, A, range_, invmag):
+def find_magnification_contour(N, x, A, range_, invmag, ny=None):
     """GridMap::find_magnification_contour: (curves, hits_boundary, -, replaced)."""
-    return _crit(1, N, x, None, A, None, range_, invmag)
+    return _crit(1, N, x, None, A, None, range_, invmag, ny)
 
 
-def find_boundaries_of_caustics(N, x, y, A):
+def find_boundaries_of_caustics(N, x, y, A, ny=None):
     """GridMap::find_boundaries_of_caustics: the boundaries (pixel units) of
     the grid positions with more than one image, and hits_edge."""
-    _, mult, _ = find_images(N, x, y, A, x, cap=64 * N * N)
-    c, h, _, _ = _crit(2, N, x, y, A, mult, 0.0, 0.0)
+    ny = N if ny is None else ny
+    _, mult, _ = find_images(N, x, y, A, x, cap=64 * N * ny, ny=ny)
+    c, h, _, _ = _crit(2, N, x, y, A, mult, 0.0, 0.0, ny)
     return c, h, mult

@@ -258,15 +258,16 @@ struct Out {
   bool hits_replaced = false;
 };
 
-void rescale(Out& o, size_t N, double range, const double* x) {
-  const double res = range / (double)(N - 1);
+void rescale(Out& o, size_t nx, double range, const double* x) {
+  const double res = range / (double)(nx - 1);
   for (auto& c : o.curves)
     for (P2& p : c) p = {{p[0] * res + x[0], p[1] * res + x[1]}};
 }
 
-// GridMap::find_crit
-void find_crit(size_t N, const double* x, const double* y, const double* A, double range, Out& o) {
-  const size_t n = N * N;
+// GridMap::find_crit on an N x ny grid (ny rows of N points)
+void find_crit(size_t N, size_t ny, const double* x, const double* y, const double* A,
+               double range, Out& o) {
+  const size_t n = N * ny;
   std::vector<bool> bm(n);
   size_t count = 0;
   double lam[2];
@@ -371,14 +372,15 @@ extern "C" {
 // A (4 N^2).  Returns the number of curves (nothing written when a cap is
 // short); hits holds *nhits flags (find_crit keeps the caller's flags when it
 // finds no tangential pixel: *replaced = 0).
-size_t orc_find_crit(int mode, size_t N, const double* x, const double* y, const double* A,
-                     const int32_t* mult, double range, double invmag, size_t cap_curves,
-                     size_t cap_points, int64_t* off, double* pts, int32_t* types, int32_t* hits,
-                     size_t* nhits, size_t* npoints, int* replaced) {
+size_t orc_find_crit2(int mode, size_t N, size_t ny, const double* x, const double* y,
+                      const double* A, const int32_t* mult, double range, double invmag,
+                      size_t cap_curves, size_t cap_points, int64_t* off, double* pts,
+                      int32_t* types, int32_t* hits, size_t* nhits, size_t* npoints,
+                      int* replaced) {
   Out o;
-  const size_t n = N * N;
+  const size_t n = N * ny;
   if (mode == 0) {
-    find_crit(N, x, y, A, range, o);
+    find_crit(N, ny, x, y, A, range, o);
   } else if (mode == 1) {
     std::vector<bool> bm(n);
     size_t count = 0;
@@ -400,6 +402,14 @@ size_t orc_find_crit(int mode, size_t N, const double* x, const double* y, const
   }
   *nhits = o.hits.size();
   return emit(o, cap_curves, cap_points, off, pts, types, hits, npoints, replaced);
+}
+
+size_t orc_find_crit(int mode, size_t N, const double* x, const double* y, const double* A,
+                     const int32_t* mult, double range, double invmag, size_t cap_curves,
+                     size_t cap_points, int64_t* off, double* pts, int32_t* types, int32_t* hits,
+                     size_t* nhits, size_t* npoints, int* replaced) {
+  return orc_find_crit2(mode, N, N, x, y, A, mult, range, invmag, cap_curves, cap_points, off,
+                        pts, types, hits, nhits, npoints, replaced);
 }
 
 // Utilities::find_boundaries on a bitmap (pixel units), for tests of the walk

@@ -29,12 +29,14 @@ struct Hit {
 };
 
 // find_images(Point_2d y, image_points, triangles) for one source
-void one_source(size_t N, const double* xg, const double* yg, P2 y, std::vector<Hit>& out) {
+// (rows j < Ngrid_init2 - 1, cells i < Ngrid_init - 1, row stride Ngrid_init)
+void one_source(size_t nx, size_t ny, const double* xg, const double* yg, P2 y,
+                std::vector<Hit>& out) {
   auto X = [&](size_t k) { return P2{xg[2 * k], xg[2 * k + 1]}; };
   auto S = [&](size_t k) { return P2{yg[2 * k], yg[2 * k + 1]}; };
-  for (size_t j = 0; j + 1 < N; ++j) {
-    size_t k = j * N, k1 = k + N + 1, k2 = k + 1, k3 = k + N;
-    for (size_t i = 0; i + 1 < N; ++i) {
+  for (size_t j = 0; j + 1 < ny; ++j) {
+    size_t k = j * nx, k1 = k + nx + 1, k2 = k + 1, k3 = k + nx;
+    for (size_t i = 0; i + 1 < nx; ++i) {
       const int sig1 = sign(cross(sub(y, S(k)), sub(S(k1), S(k))));
       int sig2 = sign(cross(sub(y, S(k1)), sub(S(k2), S(k1))));
       int sig3 = sign(cross(sub(y, S(k2)), sub(S(k), S(k2))));
@@ -79,13 +81,14 @@ extern "C" {
 // xg, yg: 2 per grid point; A: 4 per grid point; ys: 2 per source.  Writes
 // up to cap rays (8 doubles) and triangles (3 indices); returns the number of
 // images (all of them, also beyond cap); multiplicity per source.
-size_t orc_find_images(size_t N, const double* xg, const double* yg, const double* A,
-                       const double* ys, size_t nsrc, int32_t* multiplicity, double* rays,
-                       int64_t* tris, size_t cap) {
+// An nx x ny grid (ny rows of nx points); orc_find_images is the N x N case.
+size_t orc_find_images2(size_t nx, size_t ny, const double* xg, const double* yg,
+                        const double* A, const double* ys, size_t nsrc, int32_t* multiplicity,
+                        double* rays, int64_t* tris, size_t cap) {
   size_t n = 0;
   for (size_t s = 0; s < nsrc; ++s) {
     std::vector<Hit> h;
-    if (N >= 2) one_source(N, xg, yg, P2{ys[2 * s], ys[2 * s + 1]}, h);
+    if (nx >= 2 && ny >= 2) one_source(nx, ny, xg, yg, P2{ys[2 * s], ys[2 * s + 1]}, h);
     multiplicity[s] = (int32_t)h.size();
     for (const Hit& e : h) {
       if (n < cap) {
@@ -102,6 +105,12 @@ size_t orc_find_images(size_t N, const double* xg, const double* yg, const doubl
     }
   }
   return n;
+}
+
+size_t orc_find_images(size_t N, const double* xg, const double* yg, const double* A,
+                       const double* ys, size_t nsrc, int32_t* multiplicity, double* rays,
+                       int64_t* tris, size_t cap) {
+  return orc_find_images2(N, N, xg, yg, A, ys, nsrc, multiplicity, rays, tris, cap);
 }
 
 }  // extern "C"

@@ -266,3 +266,24 @@ def test_product_find_boundaries_small_and_thin():
             for x, y in zip(a, b):
                 assert np.array_equal(x, y)
     assert S.find_boundaries(np.zeros((0, 5), bool))[0] == []
+
+
+def test_rectangular_grid_positions_and_crit():
+    # GridMap(lens, Nx, center, rangeX, rangeY): a lattice of spacing
+    # rangeX / (Nx - 1) from the lower-left corner, int(Nx rangeY / rangeX) rows
+    nx, c, rx, ry = 96, (0.1, -0.05), 2.0, 1.2
+    x, ny = O.gridmap_rect_positions(nx, c, rx, ry)
+    res = rx / (nx - 1)
+    assert ny == 57 and x.shape == (nx * ny, 2)
+    assert np.allclose(np.diff(x.reshape(ny, nx, 2)[:, :, 0], axis=1), res, rtol=1e-12)
+    assert np.allclose(np.diff(x.reshape(ny, nx, 2)[:, :, 1], axis=0), res, rtol=1e-12)
+    assert tuple(x[0]) == (c[0] - 0.5 * rx, c[1] - 0.5 * ry)
+    # an SIS in the middle: its tangential curve at the Einstein radius
+    te = 0.35
+    a, A = _isothermal(x, te, 0.0, centre=(0.11, -0.04))
+    curves, hits, types, rep = O.find_crit(nx, x, x - a, A, rx, ny=ny)
+    assert rep and list(types) == [O.CRIT_TANGENTIAL, O.CRIT_PSEUDO]
+    assert np.all(np.abs(_radius(curves[0], (0.11, -0.04)) - te) <= res)
+    cont, _, _, _ = O.find_magnification_contour(nx, x, A, rx, 0.2, ny=ny)
+    assert len(cont) == 1
+    assert np.all(np.abs(_radius(cont[0], (0.11, -0.04)) - te / 0.8) <= res)
