@@ -1074,8 +1074,8 @@ struct GridMap {
 
   // gridmap.cpp:26-91: Nx x Ny points, Ny = (int)(Nx rangeY / rangeX), square
   // cells of rangeX / (Nx - 1), point ii + jj Nx at (xo + ii res, yo + jj res)
-  // from the lower-left corner (xo, yo); shot as a ray list.  find_images,
-  // find_crit and the contours are built for N x N grids only.
+  // from the lower-left corner (xo, yo); shot as a ray list, outputs kept in
+  // HBM for find_images and the critical curves.
   GridMap(LensHndl lens, unsigned long Nx, const PosType center_[2], PosType rangeX,
           PosType rangeY)
       : Ngrid_init(Nx), x_range(rangeX), rect(true) {
@@ -1083,6 +1083,7 @@ struct GridMap {
     if (rangeX <= 0 || rangeY <= 0) throw std::runtime_error("cannot make GridMap with no range");
     center[0] = center_[0];
     center[1] = center_[1];
+    y_range = rangeY;
     axisratio = rangeY / rangeX;
     Ngrid_init2 = (int)(Ngrid_init * axisratio);
     const size_t n = (size_t)Ngrid_init * Ngrid_init2;
@@ -1108,8 +1109,15 @@ struct GridMap {
     }
     {
       std::lock_guard<std::mutex> hold(grid_mutex());
-      slsim_cpp::check(slsim_shoot(lens->handle(), n, x.data(), nullptr, y.data(), A.data(),
-                                   slsim_cpp::precision(), nullptr));
+      slsim_ctx ctx = slsim_cpp::context();
+      int rc = slsim_shoot(lens->handle(), n, x.data(), nullptr, y.data(), A.data(),
+                           slsim_cpp::precision(), nullptr);
+      if (rc == SLSIM_OK) rc = slsim_device_alloc(ctx, 16 * n, &yd);
+      if (rc == SLSIM_OK) rc = slsim_device_alloc(ctx, 32 * n, &Ad);
+      if (rc == SLSIM_OK) rc = slsim_memcpy_h2d(ctx, yd, y.data(), 16 * n);
+      if (rc == SLSIM_OK) rc = slsim_memcpy_h2d(ctx, Ad, A.data(), 32 * n);
+      if (rc != SLSIM_OK) release_device();
+      slsim_cpp::check(rc);
     }
     for (size_t i = 0; i < n; ++i) {
       s_points[i].id = i;
@@ -1138,7 +1146,6 @@ struct GridMap {
   // gridmap.cpp:927-963, 1326-1348: the images of each source position, as
   // rays (x the image, y the source, A the mean over the image's triangle)
   std::list<RAY> find_images(std::vector<Point_2d>& ys, std::vector<int>& multiplicity) const {
-    square_only();
     std::vector<double> r;
     std::vector<int64_t> t;
     images(ys, multiplicity, r, t);
@@ -1155,7 +1162,6 @@ struct GridMap {
   // gridmap.cpp:1034-1098
   void find_images(Point_2d y, std::vector<Point_2d>& image_points,
                    std::vector<Triangle>& triangles) const {
-    square_only();
     std::vector<Point_2d> ys(1, y);
     std::vector<int> mult;
     std::vector<double> r;
@@ -1175,11 +1181,10 @@ struct GridMap {
   // the device, the boundary walk on the host
   void find_crit(std::vector<std::vector<Point_2d>>& points, std::vector<bool>& hits_boundary,
                  std::vector<CritType>& crit_type) {
-    square_only();
     slsim_curves h = nullptr;
-    slsim_cpp::check(slsim_gridmap_find_crit(slsim_cpp::context(), Ngrid_init, center[0],
-                                             center[1], x_range, (const double*)yd,
-                                             (const double*)Ad, &h));
+    const slsim_grid gd = grid_desc();
+    slsim_cpp::check(slsim_grid_find_crit(slsim_cpp::context(), &gd, (const double*)yd,
+                                          (const double*)Ad, &h));
     std::vector<int32_t> t, hb;
     int replaced;
     slsim_cpp::take_curves(h, points, 0, t, hb, replaced);
@@ -1190,11 +1195,10 @@ struct GridMap {
   // gridmap.cpp:493-522
   void find_magnification_contour(std::vector<std::vector<Point_2d>>& curves,
                                   std::vector<bool>& hits_boundary, double invmag) {
-    square_only();
     slsim_curves h = nullptr;
-    slsim_cpp::check(slsim_gridmap_find_magnification_contour(
-        slsim_cpp::context(), Ngrid_init, center[0], center[1], x_range, (const double*)Ad, invmag,
-        &h));
+    const slsim_grid gd = grid_desc();
+    slsim_cpp::check(slsim_grid_find_magnification_contour(slsim_cpp::context(), &gd,
+                                                           (const double*)Ad, invmag, &h));
     std::vector<int32_t> t, hb;
     int replaced;
     slsim_cpp::take_curves(h, curves, 0, t, hb, replaced);
@@ -1204,11 +1208,10 @@ struct GridMap {
   // more than one image as sources
   void find_boundaries_of_caustics(std::vector<std::vector<Point_2d>>& boundaries,
                                    std::vector<bool>& hits_edge) {
-    square_only();
     slsim_curves h = nullptr;
-    slsim_cpp::check(slsim_gridmap_find_boundaries_of_caustics(
-        slsim_cpp::context(), Ngrid_init, center[0], center[1], x_range, (const double*)yd,
-        nullptr, &h));
+    const slsim_grid gd = grid_desc();
+    slsim_cpp::check(slsim_grid_find_boundaries_of_caustics(slsim_cpp::context(), &gd,
+                                                            (const double*)yd, nullptr, &h));
     std::vector<int32_t> t, hb;
     int replaced;
     slsim_cpp::take_curves(h, boundaries, 0, t, hb, replaced);
@@ -1284,10 +1287,10 @@ struct GridMap {
       r.assign(8 * std::max<size_t>(ns * cap, 1), 0.0);
       t.assign(3 * std::max<size_t>(ns * cap, 1), 0);
       size_t n = 0;
-      const int rc = slsim_gridmap_find_images(slsim_cpp::context(), Ngrid_init, center[0],
-                                               center[1], x_range, (const double*)yd,
-                                               (const double*)Ad, ns, yv.data(), cap, m.data(),
-                                               r.data(), t.data(), &n);
+      const slsim_grid gd = grid_desc();
+      const int rc = slsim_grid_find_images(slsim_cpp::context(), &gd, (const double*)yd,
+                                            (const double*)Ad, ns, yv.data(), cap, m.data(),
+                                            r.data(), t.data(), &n);
       if (rc == SLSIM_ERR_ARG && cap < 4096) continue;
       slsim_cpp::check(rc);
       r.resize(8 * n);
@@ -1296,13 +1299,22 @@ struct GridMap {
     }
     multiplicity.assign(m.begin(), m.end());
   }
-  void square_only() const {
-    if (rect)
-      throw std::invalid_argument("GridMap: image and critical-curve finding are built for "
-                                  "N x N grids");
+  // the grid for the library's slsim_grid_* entry points
+  slsim_grid grid_desc() const {
+    slsim_grid g;
+    g.nx = (size_t)Ngrid_init;
+    g.ny = (size_t)Ngrid_init2;
+    g.center[0] = center[0];
+    g.center[1] = center[1];
+    g.range_x = x_range;
+    g.range_y = rect ? y_range : x_range;
+    g.layout = rect ? SLSIM_GRID_RECT : SLSIM_GRID_SQUARE;
+    g.pad = 0;
+    return g;
   }
   int Ngrid_init, Ngrid_init2;
   double x_range;
+  double y_range = 0;
   double axisratio = 1.0;
   bool rect = false;
   double center[2];

@@ -246,6 +246,28 @@ int slsim_gridmap_find_images(slsim_ctx ctx, size_t N, double c0, double c1, dou
                               const double* ys, size_t max_per_source, int32_t* multiplicity,
                               double* rays, int64_t* triangles, size_t* nimages);
 
+/* A GridMap's grid for the slsim_grid_* entry points: SLSIM_GRID_SQUARE is
+ * GridMap(lens, N, center, range) (nx = ny = N, TreeCode_link/gridmap.cpp:
+ * 93-120, points as slsim_gridmap_shoot generates them); SLSIM_GRID_RECT is
+ * GridMap(lens, Nx, center, rangeX, rangeY) (gridmap.cpp:26-91: ny = (int)(nx
+ * rangeY / rangeX) rows, point ii + jj nx at the lower-left corner plus
+ * (ii, jj) rangeX / (nx - 1)).  The grid's y and A (device) are in point
+ * order.  nx, ny < 65536. */
+enum { SLSIM_GRID_SQUARE = 0, SLSIM_GRID_RECT = 1 };
+typedef struct {
+  size_t nx, ny;
+  double center[2];
+  double range_x, range_y;   /* full widths (range_y unused for a square grid) */
+  int32_t layout;
+  int32_t pad;
+} slsim_grid;
+/* slsim_gridmap_find_images on any GridMap grid (GridMap::find_images loops
+ * rows j < Ngrid_init2 - 1 and cells i < Ngrid_init - 1, gridmap.cpp:965-1032). */
+int slsim_grid_find_images(slsim_ctx ctx, const slsim_grid* grid, const double* y_dev,
+                           const double* A_dev, size_t nsrc, const double* ys,
+                           size_t max_per_source, int32_t* multiplicity, double* rays,
+                           int64_t* triangles, size_t* nimages);
+
 /* ---------------------------------------- critical curves and contours */
 /* Curve types, CritType's values (include/point.h:35). */
 enum {
@@ -299,6 +321,15 @@ int slsim_convex_hull(const double* points, size_t n, int64_t* hull, size_t* nhu
  * m points against the polygon poly (n points), on the host. */
 int slsim_in_curve(const double* poly, size_t n, const double* points, size_t m,
                    int32_t* inside);
+/* The three above on any GridMap grid (the bitmaps are nx wide). */
+int slsim_grid_find_crit(slsim_ctx ctx, const slsim_grid* grid, const double* y_dev,
+                         const double* A_dev, slsim_curves* out);
+int slsim_grid_find_magnification_contour(slsim_ctx ctx, const slsim_grid* grid,
+                                          const double* A_dev, double invmag,
+                                          slsim_curves* out);
+int slsim_grid_find_boundaries_of_caustics(slsim_ctx ctx, const slsim_grid* grid,
+                                           const double* y_dev, int32_t* multiplicity,
+                                           slsim_curves* out);
 int slsim_curves_get_info(slsim_curves curves, slsim_curves_info* info);
 /* offsets: n_curves + 1 (points of curve c are offsets[c] .. offsets[c+1]),
  * points: 2 n_points, types and hits_boundary: n_curves each. */

@@ -459,12 +459,55 @@ class GridMap:
         self.range = float(range_)
         self.precision = precision
         self.zs = float(zs)
+        self.ny = self.N
+        self.range_y = self.range
+        self.layout = _abi.SLSIM_GRID_SQUARE
         n = self.N * self.N
         self.y_dev = self.ctx.alloc(16 * n)
         self.A_dev = self.ctx.alloc(32 * n)
         self.stats = {}
         if shoot:
             self.shoot()
+
+    @classmethod
+    def rect(cls, lens, Nx, center, range_x, range_y, precision=SLSIM_FP64):
+        """GridMap(lens, Nx, center, rangeX, rangeY) (gridmap.cpp:26-91):
+        ny = int(Nx rangeY / rangeX) rows of Nx points, square cells of
+        rangeX / (Nx - 1) from the lower-left corner; the points are shot as
+        a ray list and the outputs stay in HBM."""
+        self = cls.__new__(cls)
+        self.lens, self.ctx = lens, lens.ctx
+        self.N = int(Nx)
+        self.center = (float(center[0]), float(center[1]))
+        self.range, self.range_y = float(range_x), float(range_y)
+        self.precision, self.zs = precision, 0.0
+        self.ny = int(self.N * (self.range_y / self.range))
+        self.layout = _abi.SLSIM_GRID_RECT
+        n = self.N * self.ny
+        self.y_dev = self.ctx.alloc(16 * max(n, 1))
+        self.A_dev = self.ctx.alloc(32 * max(n, 1))
+        self.stats = {}
+        x = np.ascontiguousarray(self.positions())
+        xd = self.ctx.alloc(16 * max(n, 1))
+        try:
+            self.ctx.h2d(xd, x)
+            st = _abi.slsim_shoot_stats()
+            _check(lib.slsim_shoot_device(lens.h, ctypes.c_size_t(n),
+                                          ctypes.cast(xd, ctypes.POINTER(ctypes.c_double)), None,
+                                          ctypes.cast(self.y_dev, ctypes.POINTER(ctypes.c_double)),
+                                          ctypes.cast(self.A_dev, ctypes.POINTER(ctypes.c_double)),
+                                          int(precision), ctypes.byref(st)))
+        finally:
+            self.ctx.free(xd)
+        return self
+
+    def _grid(self):
+        g = _abi.slsim_grid()
+        g.nx, g.ny = self.N, self.ny
+        g.center[0], g.center[1] = self.center
+        g.range_x, g.range_y = self.range, self.range_y
+        g.layout = self.layout
+        return g
 
     def shoot(self):
         st = _abi.slsim_shoot_stats()
@@ -477,8 +520,14 @@ class GridMap:
         return self.stats
 
     def positions(self):
-        """Image positions of the grid (x fastest), PositionFromIndex."""
+        """Image positions of the grid (x fastest): PositionFromIndex, or the
+        rectangular constructor's lattice."""
         N = self.N
+        if self.layout == _abi.SLSIM_GRID_RECT:
+            k = np.arange(N * self.ny)
+            res = self.range / (N - 1)
+            return np.stack([(self.center[0] - 0.5 * self.range) + (k % N) * res,
+                             (self.center[1] - 0.5 * self.range_y) + (k // N) * res], 1)
         i = np.arange(N * N)
         if N == 1:
             return np.array([[self.center[0], self.center[1]]])
@@ -487,12 +536,12 @@ class GridMap:
         return np.stack([x0, x1], axis=1)
 
     def y(self):
-        out = np.zeros((self.N * self.N, 2))
+        out = np.zeros((self.N * self.ny, 2))
         self.ctx.d2h(out, self.y_dev)
         return out
 
     def A(self):
-        out = np.zeros((self.N * self.N, 4))
+        out = np.zeros((self.N * self.ny, 4))
         self.ctx.d2h(out, self.A_dev)
         return out
 
@@ -540,10 +589,10 @@ class GridMap:
         elif v == "INVMAG":
             r = A[:, 0] * A[:, 3] - A[:, 1] * A[:, 2]
         elif v == "DELAYT":
-            r = np.zeros(self.N * self.N)
+            r = np.zeros(self.N * self.ny)
         else:
             raise ValueError("GridMap::writePixelMap() does not work for the input LensingVariable")
-        return r.reshape(self.N, self.N)
+        return r.reshape(self.ny, self.N)
 
     def find_images(self, ys, max_per_source=32):
         """GridMap::find_images(ys, multiplicity) (gridmap.cpp:927-1032,
@@ -558,8 +607,9 @@ class GridMap:
         rays = np.zeros((max(ns * cap, 1), 8))
         tris = np.zeros((max(ns * cap, 1), 3), np.int64)
         n = ctypes.c_size_t()
-        _check(lib.slsim_gridmap_find_images(
-            self.ctx.h, ctypes.c_size_t(self.N), self.center[0], self.center[1], self.range,
+        g = self._grid()
+        _check(lib.slsim_grid_find_images(
+            self.ctx.h, ctypes.byref(g),
             self.y_dev, self.A_dev, ctypes.c_size_t(ns), _ptr(ys, ctypes.c_double),
             ctypes.c_size_t(cap), mult.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
             _ptr(rays, ctypes.c_double), tris.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
@@ -573,18 +623,18 @@ class GridMap:
         crit types SLSIM_CRIT_*; whether hits_boundary was replaced, False
         when no pixel is critical)."""
         h = _abi.slsim_curves()
-        _check(lib.slsim_gridmap_find_crit(self.ctx.h, ctypes.c_size_t(self.N), self.center[0],
-                                           self.center[1], self.range, self.y_dev, self.A_dev,
-                                           ctypes.byref(h)))
+        g = self._grid()
+        _check(lib.slsim_grid_find_crit(self.ctx.h, ctypes.byref(g), self.y_dev, self.A_dev,
+                                        ctypes.byref(h)))
         return _read_curves(h)
 
     def find_magnification_contour(self, invmag):
         """GridMap::find_magnification_contour(curves, hits_boundary, invmag)
         (gridmap.cpp:493-522): (curves, hits_boundary, -, replaced)."""
         h = _abi.slsim_curves()
-        _check(lib.slsim_gridmap_find_magnification_contour(
-            self.ctx.h, ctypes.c_size_t(self.N), self.center[0], self.center[1], self.range,
-            self.A_dev, float(invmag), ctypes.byref(h)))
+        g = self._grid()
+        _check(lib.slsim_grid_find_magnification_contour(self.ctx.h, ctypes.byref(g), self.A_dev,
+                                                         float(invmag), ctypes.byref(h)))
         return _read_curves(h)
 
     def find_boundaries_of_caustics(self):
@@ -592,10 +642,11 @@ class GridMap:
         (gridmap.h:278-296): (boundaries in pixel units, hits_edge, the
         multiplicity of every grid position taken as a source)."""
         h = _abi.slsim_curves()
-        mult = np.zeros(self.N * self.N, np.int32)
-        _check(lib.slsim_gridmap_find_boundaries_of_caustics(
-            self.ctx.h, ctypes.c_size_t(self.N), self.center[0], self.center[1], self.range,
-            self.y_dev, _ptr(mult, ctypes.c_int32), ctypes.byref(h)))
+        mult = np.zeros(self.N * self.ny, np.int32)
+        g = self._grid()
+        _check(lib.slsim_grid_find_boundaries_of_caustics(self.ctx.h, ctypes.byref(g), self.y_dev,
+                                                          _ptr(mult, ctypes.c_int32),
+                                                          ctypes.byref(h)))
         c, hits, _, _ = _read_curves(h)
         return c, hits, mult
 

@@ -110,6 +110,16 @@ SLSIM_HALO_NFW, SLSIM_HALO_POWERLAW = 0, 1
 SLSIM_CRIT_ND, SLSIM_CRIT_TANGENTIAL, SLSIM_CRIT_RADIAL, SLSIM_CRIT_PSEUDO = range(4)
 
 
+SLSIM_GRID_SQUARE, SLSIM_GRID_RECT = 0, 1
+
+
+class slsim_grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_size_t), ("ny", ctypes.c_size_t),
+                ("center", ctypes.c_double * 2), ("range_x", ctypes.c_double),
+                ("range_y", ctypes.c_double), ("layout", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
 class slsim_curves_info(ctypes.Structure):
     _fields_ = [("n_curves", ctypes.c_size_t), ("n_points", ctypes.c_size_t),
                 ("hits_replaced", ctypes.c_int32)]
@@ -191,6 +201,14 @@ SIGNATURES = {
     "slsim_gridmap_find_boundaries_of_caustics": (c_int, [slsim_ctx, c_size_t, c_double, c_double,
                                                           c_double, c_vp, P(ctypes.c_int32),
                                                           P(slsim_curves)]),
+    "slsim_grid_find_images": (c_int, [slsim_ctx, P(slsim_grid), c_vp, c_vp, c_size_t,
+                                       P(c_double), c_size_t, P(ctypes.c_int32), P(c_double),
+                                       P(ctypes.c_int64), P(c_size_t)]),
+    "slsim_grid_find_crit": (c_int, [slsim_ctx, P(slsim_grid), c_vp, c_vp, P(slsim_curves)]),
+    "slsim_grid_find_magnification_contour": (c_int, [slsim_ctx, P(slsim_grid), c_vp, c_double,
+                                                      P(slsim_curves)]),
+    "slsim_grid_find_boundaries_of_caustics": (c_int, [slsim_ctx, P(slsim_grid), c_vp,
+                                                       P(ctypes.c_int32), P(slsim_curves)]),
     "slsim_find_boundaries": (c_int, [P(ctypes.c_uint8), c_size_t, c_size_t, P(slsim_curves)]),
     "slsim_convex_hull": (c_int, [P(c_double), c_size_t, P(c_i64), P(c_size_t)]),
     "slsim_in_curve": (c_int, [P(c_double), c_size_t, P(c_double), c_size_t, P(ctypes.c_int32)]),

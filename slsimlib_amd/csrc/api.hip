@@ -328,59 +328,119 @@ int slsim_halos_create_nfw(slsim_ctx ctx, const double* x, const double* y, cons
   return SLSIM_OK;
 }
 
-int slsim_gridmap_find_images(slsim_ctx ctx, size_t N, double c0, double c1, double range,
-                              const double* y_dev, const double* A_dev, size_t nsrc,
-                              const double* ys, size_t max_per_source, int32_t* multiplicity,
-                              double* rays, int64_t* triangles, size_t* nimages) {
+// the grid of a slsim_grid, checked
+static int grid_geom(const slsim_grid* gd, GridGeom* g) {
+  if (!gd) return arg_error("null grid");
+  if (gd->layout != SLSIM_GRID_SQUARE && gd->layout != SLSIM_GRID_RECT)
+    return arg_error("grid layout must be SLSIM_GRID_SQUARE or SLSIM_GRID_RECT");
+  if (gd->layout == SLSIM_GRID_SQUARE && gd->nx != gd->ny)
+    return arg_error("a square grid has nx == ny");
+  if (gd->nx >= 65536 || gd->ny >= 65536) return arg_error("grid sides must be below 65536");
+  if (!(gd->range_x > 0) || (gd->layout == SLSIM_GRID_RECT && !(gd->range_y > 0)))
+    return arg_error("grid ranges must be positive");
+  *g = GridGeom{(int64_t)gd->nx, (int64_t)gd->ny, gd->center[0], gd->center[1], gd->range_x,
+                gd->layout == SLSIM_GRID_RECT ? gd->range_y : gd->range_x,
+                gd->layout == SLSIM_GRID_RECT, 0};
+  return SLSIM_OK;
+}
+
+static slsim_grid square_grid(size_t N, double c0, double c1, double range) {
+  slsim_grid g;
+  g.nx = g.ny = N;
+  g.center[0] = c0;
+  g.center[1] = c1;
+  g.range_x = g.range_y = range;
+  g.layout = SLSIM_GRID_SQUARE;
+  g.pad = 0;
+  return g;
+}
+
+int slsim_grid_find_images(slsim_ctx ctx, const slsim_grid* grid, const double* y_dev,
+                           const double* A_dev, size_t nsrc, const double* ys,
+                           size_t max_per_source, int32_t* multiplicity, double* rays,
+                           int64_t* triangles, size_t* nimages) {
   if (!ctx || !nimages || (nsrc && (!ys || !multiplicity || !rays || !triangles)))
     return arg_error("null argument");
-  if (N >= 2 && (!y_dev || !A_dev)) return arg_error("null grid outputs");
-  if (N >= 65536) return arg_error("find_images: N must be below 65536");
+  GridGeom g;
+  int rc;
+  if ((rc = grid_geom(grid, &g))) return rc;
+  if (g.nx * g.ny >= 1 && (!y_dev || !A_dev)) return arg_error("null grid outputs");
   if (max_per_source == 0 || max_per_source > 4096)
     return arg_error("find_images: max_per_source must be in [1, 4096]");
   if (nsrc > ((size_t)1 << 31) / max_per_source)
     return arg_error("find_images: nsrc * max_per_source must stay below 2^31");
   SLSIM_HIP_CHECK(hipSetDevice(ctx->device));
-  return find_images(ctx, N, c0, c1, range, y_dev, A_dev, nsrc, ys, max_per_source, multiplicity,
-                     rays, triangles, nimages);
+  return find_images(ctx, g, y_dev, A_dev, nsrc, ys, max_per_source, multiplicity, rays,
+                     triangles, nimages);
 }
 
-static int curves_call(slsim_ctx ctx, size_t N, slsim_curves* out,
-                       const std::function<int(slsim_curves_s*)>& body) {
+int slsim_gridmap_find_images(slsim_ctx ctx, size_t N, double c0, double c1, double range,
+                              const double* y_dev, const double* A_dev, size_t nsrc,
+                              const double* ys, size_t max_per_source, int32_t* multiplicity,
+                              double* rays, int64_t* triangles, size_t* nimages) {
+  const slsim_grid g = square_grid(N, c0, c1, range);
+  return slsim_grid_find_images(ctx, &g, y_dev, A_dev, nsrc, ys, max_per_source, multiplicity,
+                                rays, triangles, nimages);
+}
+
+static int curves_call(slsim_ctx ctx, const slsim_grid* grid, slsim_curves* out,
+                       const std::function<int(const GridGeom&, slsim_curves_s*)>& body) {
   if (!ctx || !out) return arg_error("null argument");
   *out = nullptr;
-  if (N >= 65536) return arg_error("critical curves: N must be below 65536");
+  GridGeom g;
+  int rc;
+  if ((rc = grid_geom(grid, &g))) return rc;
   SLSIM_HIP_CHECK(hipSetDevice(ctx->device));
   std::unique_ptr<slsim_curves_s> r(new slsim_curves_s);
-  const int rc = body(r.get());
+  rc = body(g, r.get());
   if (rc == SLSIM_OK) *out = r.release();
   return rc;
 }
 
+int slsim_grid_find_crit(slsim_ctx ctx, const slsim_grid* grid, const double* y_dev,
+                         const double* A_dev, slsim_curves* out) {
+  if (!y_dev || !A_dev) return arg_error("null grid outputs");
+  return curves_call(ctx, grid, out, [&](const GridGeom& g, slsim_curves_s* r) {
+    return find_crit(ctx, g, y_dev, A_dev, r);
+  });
+}
+
+int slsim_grid_find_magnification_contour(slsim_ctx ctx, const slsim_grid* grid,
+                                          const double* A_dev, double invmag,
+                                          slsim_curves* out) {
+  if (!A_dev) return arg_error("null grid outputs");
+  return curves_call(ctx, grid, out, [&](const GridGeom& g, slsim_curves_s* r) {
+    return find_magnification_contour(ctx, g, A_dev, invmag, r);
+  });
+}
+
+int slsim_grid_find_boundaries_of_caustics(slsim_ctx ctx, const slsim_grid* grid,
+                                           const double* y_dev, int32_t* multiplicity,
+                                           slsim_curves* out) {
+  if (!y_dev) return arg_error("null grid outputs");
+  return curves_call(ctx, grid, out, [&](const GridGeom& g, slsim_curves_s* r) {
+    return find_boundaries_of_caustics(ctx, g, y_dev, multiplicity, r);
+  });
+}
+
 int slsim_gridmap_find_crit(slsim_ctx ctx, size_t N, double c0, double c1, double range,
                             const double* y_dev, const double* A_dev, slsim_curves* out) {
-  if (N >= 1 && (!y_dev || !A_dev)) return arg_error("null grid outputs");
-  return curves_call(ctx, N, out, [&](slsim_curves_s* r) {
-    return find_crit(ctx, N, c0, c1, range, y_dev, A_dev, r);
-  });
+  const slsim_grid g = square_grid(N, c0, c1, range);
+  return slsim_grid_find_crit(ctx, &g, y_dev, A_dev, out);
 }
 
 int slsim_gridmap_find_magnification_contour(slsim_ctx ctx, size_t N, double c0, double c1,
                                              double range, const double* A_dev, double invmag,
                                              slsim_curves* out) {
-  if (N >= 1 && !A_dev) return arg_error("null grid outputs");
-  return curves_call(ctx, N, out, [&](slsim_curves_s* r) {
-    return find_magnification_contour(ctx, N, c0, c1, range, A_dev, invmag, r);
-  });
+  const slsim_grid g = square_grid(N, c0, c1, range);
+  return slsim_grid_find_magnification_contour(ctx, &g, A_dev, invmag, out);
 }
 
 int slsim_gridmap_find_boundaries_of_caustics(slsim_ctx ctx, size_t N, double c0, double c1,
                                               double range, const double* y_dev,
                                               int32_t* multiplicity, slsim_curves* out) {
-  if (N >= 1 && !y_dev) return arg_error("null grid outputs");
-  return curves_call(ctx, N, out, [&](slsim_curves_s* r) {
-    return find_boundaries_of_caustics(ctx, N, c0, c1, range, y_dev, multiplicity, r);
-  });
+  const slsim_grid g = square_grid(N, c0, c1, range);
+  return slsim_grid_find_boundaries_of_caustics(ctx, &g, y_dev, multiplicity, out);
 }
 
 int slsim_find_boundaries(const uint8_t* bitmap, size_t nx, size_t ny, slsim_curves* out) {

@@ -43,13 +43,14 @@ enum { kByEigen = 0, kByInvmag = 1, kByMult = 2 };
 // 1e5 atomics on one address, which costs more than reading A).
 constexpr int kPixBlocks = 2048;
 __global__ __launch_bounds__(256) void k_crit_pixels(const double2* __restrict__ A,
-                                                     const int32_t* __restrict__ mult, int64_t N,
+                                                     const int32_t* __restrict__ mult, int64_t nx,
+                                                     int64_t ny,
                                                      int mode, double invmag,
                                                      uint8_t* __restrict__ px,
                                                      unsigned long long* __restrict__ counts) {
 #pragma clang fp contract(off)
   __shared__ unsigned long long part[2][4];
-  const int64_t n = N * N;
+  const int64_t n = nx * ny;
   unsigned long long c0 = 0, c1 = 0;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
        base += (int64_t)gridDim.x * blockDim.x) {
@@ -74,8 +75,8 @@ __global__ __launch_bounds__(256) void k_crit_pixels(const double2* __restrict__
     c0 += __popcll(__ballot(bits & 1));
     c1 += __popcll(__ballot((bits >> 1) & 1));
     if (k < n) {
-      const int64_t i = k % N, j = k / N;
-      if (i == 0 || i == N - 1 || j == 0 || j == N - 1) bits = 0;
+      const int64_t i = k % nx, j = k / nx;
+      if (i == 0 || i == nx - 1 || j == 0 || j == ny - 1) bits = 0;
       px[k] = (uint8_t)bits;
     }
   }
@@ -96,12 +97,13 @@ __global__ __launch_bounds__(256) void k_crit_pixels(const double2* __restrict__
   }
 }
 
-// Cells k < N (N - 1) with k % N != N - 1 whose four pixels (k, k+1, k+N,
-// k+N+1) differ in bit b: bit k % 64 of word k / 64 of edge[b].
+// Cells k < n - N with k % N != N - 1 (N points a row, n in all) whose four
+// pixels (k, k+1, k+N, k+N+1) differ in bit b: bit k % 64 of word k / 64 of
+// edge[b].
 __global__ __launch_bounds__(256) void k_crit_edges(const uint8_t* __restrict__ px, int64_t N,
+                                                    int64_t n,
                                                     int nbits, unsigned long long* __restrict__ edge,
                                                     int64_t nwords) {
-  const int64_t n = N * N;
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   uint32_t q = 0;
   if (k < n - N && k % N != N - 1) {
@@ -329,10 +331,10 @@ struct Classified {
   unsigned long long count[2] = {0, 0};
 };
 
-int classify(slsim_ctx c, Scratch& sc, int64_t N, int mode, const double* A_dev,
+int classify(slsim_ctx c, Scratch& sc, int64_t N, int64_t ny, int mode, const double* A_dev,
              const int32_t* mult_dev, double invmag, int nbits, Classified& out) {
   hipStream_t st = c->stream;
-  const int64_t n = N * N;
+  const int64_t n = N * ny;
   out.nwords = (n + 63) / 64;
   uint8_t* dpx;
   unsigned long long *dedge, *dcnt;
@@ -343,9 +345,9 @@ int classify(slsim_ctx c, Scratch& sc, int64_t N, int mode, const double* A_dev,
   launch_zero_u32(st, (uint32_t*)dcnt, 4);
   const unsigned blocks = (unsigned)((n + 255) / 256);
   k_crit_pixels<<<(unsigned)std::min<int64_t>(blocks, kPixBlocks), 256, 0, st>>>(
-      (const double2*)A_dev, mult_dev, N, mode, invmag, dpx, dcnt);
+      (const double2*)A_dev, mult_dev, N, ny, mode, invmag, dpx, dcnt);
   SLSIM_HIP_CHECK(hipGetLastError());
-  k_crit_edges<<<blocks, 256, 0, st>>>(dpx, N, nbits, dedge, out.nwords);
+  k_crit_edges<<<blocks, 256, 0, st>>>(dpx, N, n, nbits, dedge, out.nwords);
   SLSIM_HIP_CHECK(hipGetLastError());
   out.px.resize((size_t)n);
   out.edge.resize((size_t)(nbits * out.nwords));
@@ -357,12 +359,13 @@ int classify(slsim_ctx c, Scratch& sc, int64_t N, int mode, const double* A_dev,
   return SLSIM_OK;
 }
 
-// pixel units -> p * resolution + x of pixel 0 (gridmap.cpp:516-520, 591-595)
-void rescale(std::vector<Curve>& cs, int64_t N, double c0, double c1, double range) {
+// pixel units -> p * resolution + x of pixel 0 (gridmap.cpp:516-520, 591-595;
+// GridMap::getResolution = x range / (Ngrid_init - 1))
+void rescale(std::vector<Curve>& cs, const GridGeom& g) {
 #pragma clang fp contract(off)
-  const double res = range / (double)(N - 1);
+  const double res = g.rx / (double)(g.nx - 1);
   double x0, x1;
-  grid_position(0, N, c0, c1, range, &x0, &x1);
+  geom_position(g, 0, &x0, &x1);
   for (Curve& c : cs)
     for (size_t q = 0; q < c.pts.size(); q += 2) {
       c.pts[q] = c.pts[q] * res + x0;
@@ -372,13 +375,13 @@ void rescale(std::vector<Curve>& cs, int64_t N, double c0, double c1, double ran
 
 }  // namespace
 
-int find_crit(slsim_ctx c, size_t N_, double c0, double c1, double range, const double* y_dev,
-              const double* A_dev, slsim_curves_s* out) {
-  const int64_t N = (int64_t)N_;
+int find_crit(slsim_ctx c, const GridGeom& g, const double* y_dev, const double* A_dev,
+              slsim_curves_s* out) {
+  const int64_t N = g.nx, ny = g.ny;
   Scratch sc(c);
   Classified cl;
   int rc;
-  if ((rc = classify(c, sc, N, kByEigen, A_dev, nullptr, 0.0, 2, cl))) return rc;
+  if ((rc = classify(c, sc, N, ny, kByEigen, A_dev, nullptr, 0.0, 2, cl))) return rc;
   std::vector<Curve> cs;
   std::vector<std::vector<int64_t>> seeds;
   bool replaced = false;
@@ -389,12 +392,12 @@ int find_crit(slsim_ctx c, size_t N_, double c0, double c1, double range, const 
   std::thread radial;
   if (cl.count[1] > 0)
     radial = std::thread([&] {
-      rc_rad = walk_all(cl.px.data(), N, N, 1, cl.edge.data() + cl.nwords, cl.nwords, false, wr);
+      rc_rad = walk_all(cl.px.data(), N, ny, 1, cl.edge.data() + cl.nwords, cl.nwords, false, wr);
       if (rc_rad) err_rad = slsim_last_error();
     });
   if (cl.count[0] > 0) {
     Walked w;
-    rc = walk_all(cl.px.data(), N, N, 0, cl.edge.data(), cl.nwords, true, w);
+    rc = walk_all(cl.px.data(), N, ny, 0, cl.edge.data(), cl.nwords, true, w);
     if (rc == SLSIM_OK) {
       cs = curves_of(w, SLSIM_CRIT_TANGENTIAL);
       seeds = std::move(w.inside);
@@ -438,7 +441,8 @@ int find_crit(slsim_ctx c, size_t N_, double c0, double c1, double range, const 
       box[i] = b;
     }
     constexpr double kBucket = 16.0;
-    const int64_t nb = std::max<int64_t>(1, (int64_t)std::ceil((double)N / kBucket) + 1);
+    const int64_t nb =
+        std::max<int64_t>(1, (int64_t)std::ceil((double)std::max(N, ny) / kBucket) + 1);
     auto bucket = [&](double v) {
       return std::min<int64_t>(nb - 1, std::max<int64_t>(0, (int64_t)(v / kBucket)));
     };
@@ -472,7 +476,7 @@ int find_crit(slsim_ctx c, size_t N_, double c0, double c1, double range, const 
     for (size_t j : loose) ordered.push_back(std::move(cs[j]));
     cs = std::move(ordered);
   }
-  rescale(cs, N, c0, c1, range);
+  rescale(cs, g);
   // a tangential curve with no radial curve after it gets a pseudo curve:
   // the image positions of the convex hull of its island's source positions
   // (gridmap.cpp:605-645).  Whether a curve needs one depends only on the
@@ -517,7 +521,7 @@ int find_crit(slsim_ctx c, size_t N_, double c0, double c1, double range, const 
       pc.hit = false;
       pc.pts.resize(2 * h.size());
       for (size_t v = 0; v < h.size(); ++v)
-        grid_position(isl[h[v]], N, c0, c1, range, &pc.pts[2 * v], &pc.pts[2 * v + 1]);
+        geom_position(g, isl[h[v]], &pc.pts[2 * v], &pc.pts[2 * v + 1]);
       full.push_back(std::move(pc));
       ++q;
     }
@@ -527,41 +531,41 @@ int find_crit(slsim_ctx c, size_t N_, double c0, double c1, double range, const 
   return SLSIM_OK;
 }
 
-int find_magnification_contour(slsim_ctx c, size_t N_, double c0, double c1, double range,
-                               const double* A_dev, double invmag, slsim_curves_s* out) {
-  const int64_t N = (int64_t)N_;
+int find_magnification_contour(slsim_ctx c, const GridGeom& g, const double* A_dev,
+                               double invmag, slsim_curves_s* out) {
+  const int64_t N = g.nx, ny = g.ny;
   Scratch sc(c);
   Classified cl;
   int rc;
-  if ((rc = classify(c, sc, N, kByInvmag, A_dev, nullptr, invmag, 1, cl))) return rc;
+  if ((rc = classify(c, sc, N, ny, kByInvmag, A_dev, nullptr, invmag, 1, cl))) return rc;
   std::vector<Curve> cs;
   bool replaced = false;
   if (cl.count[0] > 0) {
     Walked w;
-    if ((rc = walk_all(cl.px.data(), N, N, 0, cl.edge.data(), cl.nwords, false, w))) return rc;
+    if ((rc = walk_all(cl.px.data(), N, ny, 0, cl.edge.data(), cl.nwords, false, w))) return rc;
     cs = curves_of(w, 0);
     replaced = true;
   }
-  rescale(cs, N, c0, c1, range);
+  rescale(cs, g);
   to_result(cs, replaced, out);
   return SLSIM_OK;
 }
 
-int find_boundaries_of_caustics(slsim_ctx c, size_t N_, double c0, double c1, double range,
-                                const double* y_dev, int32_t* mult_host, slsim_curves_s* out) {
-  const int64_t N = (int64_t)N_;
+int find_boundaries_of_caustics(slsim_ctx c, const GridGeom& g, const double* y_dev,
+                                int32_t* mult_host, slsim_curves_s* out) {
+  const int64_t N = g.nx, ny = g.ny;
   Scratch sc(c);
   int32_t* dmult;
   int rc;
-  if ((rc = sc.get(&dmult, (size_t)(N * N)))) return rc;
-  if ((rc = count_grid_images(c, N_, c0, c1, range, y_dev, dmult))) return rc;
+  if ((rc = sc.get(&dmult, (size_t)(N * ny)))) return rc;
+  if ((rc = count_grid_images(c, g, y_dev, dmult))) return rc;
   if (mult_host)
-    SLSIM_HIP_CHECK(hipMemcpyAsync(mult_host, dmult, 4 * (size_t)(N * N), hipMemcpyDeviceToHost,
+    SLSIM_HIP_CHECK(hipMemcpyAsync(mult_host, dmult, 4 * (size_t)(N * ny), hipMemcpyDeviceToHost,
                                    c->stream));
   Classified cl;
-  if ((rc = classify(c, sc, N, kByMult, nullptr, dmult, 0.0, 1, cl))) return rc;
+  if ((rc = classify(c, sc, N, ny, kByMult, nullptr, dmult, 0.0, 1, cl))) return rc;
   Walked w;
-  if ((rc = walk_all(cl.px.data(), N, N, 0, cl.edge.data(), cl.nwords, false, w))) return rc;
+  if ((rc = walk_all(cl.px.data(), N, ny, 0, cl.edge.data(), cl.nwords, false, w))) return rc;
   to_result(curves_of(w, 0), true, out);
   return SLSIM_OK;
 }

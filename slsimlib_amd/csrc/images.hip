@@ -62,41 +62,51 @@ __device__ inline int tri_code(double q0, double q1, double a0, double a1, doubl
 // box widened by one column and row each side; positions outside it are not
 // inside, on an edge or on a vertex of the triangle.  A triangle with a
 // non-finite corner, whose sign tests give zeros, tests every position.
+// fractional grid column / row of a position (inverse of geom_position, to
+// within rounding)
+__device__ inline double geom_col(const GridGeom& g, double x) {
+  if (!g.rect) return (x - g.c0) * ((double)(g.nx - 1) / g.rx) + 0.5 * (double)(g.nx - 1);
+  return (x - (g.c0 - 0.5 * g.rx)) * ((double)(g.nx - 1) / g.rx);
+}
+__device__ inline double geom_row(const GridGeom& g, double y) {
+  if (!g.rect) return (y - g.c1) * ((double)(g.nx - 1) / g.rx) + 0.5 * (double)(g.nx - 1);
+  return (y - (g.c1 - 0.5 * g.ry)) * ((double)(g.nx - 1) / g.rx);
+}
+
 __global__ __launch_bounds__(256) void k_count_grid_images(const double* __restrict__ y,
-                                                           int64_t N, double g0, double g1,
-                                                           double range,
+                                                           GridGeom g,
                                                            int32_t* __restrict__ mult) {
 #pragma clang fp contract(off)
-  const int64_t ncell = (N - 1) * (N - 1);
+  const int64_t nx = g.nx, ny = g.ny;
+  const int64_t ncell = (nx - 1) * (ny - 1);
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (c >= ncell) return;
-  const int64_t k = (c / (N - 1)) * N + c % (N - 1);
+  const int64_t k = (c / (nx - 1)) * nx + c % (nx - 1);
   const double p0x = y[2 * k], p0y = y[2 * k + 1];
-  const double p1x = y[2 * (k + N + 1)], p1y = y[2 * (k + N + 1) + 1];
+  const double p1x = y[2 * (k + nx + 1)], p1y = y[2 * (k + nx + 1) + 1];
   for (int t = 0; t < 2; ++t) {
-    const int64_t k2 = t == 0 ? k + 1 : k + N;
+    const int64_t k2 = t == 0 ? k + 1 : k + nx;
     const double p2x = y[2 * k2], p2y = y[2 * k2 + 1];
-    int64_t ilo = 0, ihi = N - 1, jlo = 0, jhi = N - 1;
+    int64_t ilo = 0, ihi = nx - 1, jlo = 0, jhi = ny - 1;
     const bool finite = isfinite(p0x) && isfinite(p0y) && isfinite(p1x) && isfinite(p1y) &&
                         isfinite(p2x) && isfinite(p2y);
     if (finite) {
-      const double s = (double)(N - 1) / range;
-      const double fx0 = ((fmin(fmin(p0x, p1x), p2x) - g0) * s) + 0.5 * (double)(N - 1);
-      const double fx1 = ((fmax(fmax(p0x, p1x), p2x) - g0) * s) + 0.5 * (double)(N - 1);
-      const double fy0 = ((fmin(fmin(p0y, p1y), p2y) - g1) * s) + 0.5 * (double)(N - 1);
-      const double fy1 = ((fmax(fmax(p0y, p1y), p2y) - g1) * s) + 0.5 * (double)(N - 1);
-      const double top = (double)(N - 1);
-      if (fx1 < -1.0 || fy1 < -1.0 || fx0 > top + 1.0 || fy0 > top + 1.0) continue;
+      const double fx0 = geom_col(g, fmin(fmin(p0x, p1x), p2x));
+      const double fx1 = geom_col(g, fmax(fmax(p0x, p1x), p2x));
+      const double fy0 = geom_row(g, fmin(fmin(p0y, p1y), p2y));
+      const double fy1 = geom_row(g, fmax(fmax(p0y, p1y), p2y));
+      const double topx = (double)(nx - 1), topy = (double)(ny - 1);
+      if (fx1 < -1.0 || fy1 < -1.0 || fx0 > topx + 1.0 || fy0 > topy + 1.0) continue;
       ilo = (int64_t)fmax(floor(fx0) - 1.0, 0.0);
-      ihi = (int64_t)fmin(ceil(fx1) + 1.0, top);
+      ihi = (int64_t)fmin(ceil(fx1) + 1.0, topx);
       jlo = (int64_t)fmax(floor(fy0) - 1.0, 0.0);
-      jhi = (int64_t)fmin(ceil(fy1) + 1.0, top);
+      jhi = (int64_t)fmin(ceil(fy1) + 1.0, topy);
     }
     for (int64_t j = jlo; j <= jhi; ++j)
       for (int64_t i = ilo; i <= ihi; ++i) {
-        const int64_t s = j * N + i;
+        const int64_t s = j * nx + i;
         double q0, q1;
-        grid_position(s, N, g0, g1, range, &q0, &q1);
+        geom_position(g, s, &q0, &q1);
         if (tri_code(q0, q1, p0x, p0y, p1x, p1y, p2x, p2y, t == 0) >= 0) atomicAdd(&mult[s], 1);
       }
   }
@@ -127,20 +137,20 @@ __device__ inline void add_hit(uint32_t* count, ImageHit* hits, uint32_t cap, in
 // degenerate to 1e-16 of its size.  A triangle with a non-finite corner,
 // whose sign tests give zeros, tests every source, and every triangle tests
 // the non-finite sources, as the reference's all-pairs search does.
-__global__ __launch_bounds__(256) void k_find_images(const double* __restrict__ y, int64_t N,
-                                                     const double* __restrict__ ys, int nsrc,
-                                                     SourceBins b, uint32_t cap,
+__global__ __launch_bounds__(256) void k_find_images(const double* __restrict__ y, int64_t nx,
+                                                     int64_t ny, const double* __restrict__ ys,
+                                                     int nsrc, SourceBins b, uint32_t cap,
                                                      uint32_t* __restrict__ count,
                                                      ImageHit* __restrict__ hits) {
 #pragma clang fp contract(off)
-  const int64_t ncell = (N - 1) * (N - 1);
+  const int64_t ncell = (nx - 1) * (ny - 1);
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (c >= ncell) return;
-  const int64_t k = (c / (N - 1)) * N + c % (N - 1);
+  const int64_t k = (c / (nx - 1)) * nx + c % (nx - 1);
   const double p0x = y[2 * k], p0y = y[2 * k + 1];
-  const double p1x = y[2 * (k + N + 1)], p1y = y[2 * (k + N + 1) + 1];
+  const double p1x = y[2 * (k + nx + 1)], p1y = y[2 * (k + nx + 1) + 1];
   for (int t = 0; t < 2; ++t) {
-    const int64_t k2 = t == 0 ? k + 1 : k + N;
+    const int64_t k2 = t == 0 ? k + 1 : k + nx;
     const double p2x = y[2 * k2], p2y = y[2 * k2 + 1];
     const uint32_t code0 = (uint32_t)t;
     const bool first = t == 0;
@@ -181,9 +191,9 @@ __global__ __launch_bounds__(256) void k_find_images(const double* __restrict__ 
 // The rays of the listed hits: x from the triangle's image-plane corners, y =
 // the source, A = ((A_a + A_b) + A_c) / 3.  list[h] = (source, hit slot).
 __global__ void k_image_rays(const ImageHit* __restrict__ hits, const uint32_t* __restrict__ list,
-                             int64_t nhit, uint32_t cap, const double* __restrict__ A, int64_t N,
-                             double c0, double c1, double range, const double* __restrict__ ys,
-                             double* __restrict__ rays, int64_t* __restrict__ tri) {
+                             int64_t nhit, uint32_t cap, const double* __restrict__ A, GridGeom g,
+                             const double* __restrict__ ys, double* __restrict__ rays,
+                             int64_t* __restrict__ tri) {
 #pragma clang fp contract(off)
   const int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (h >= nhit) return;
@@ -191,9 +201,9 @@ __global__ void k_image_rays(const ImageHit* __restrict__ hits, const uint32_t* 
   const ImageHit e = hits[(int64_t)src * cap + slot];
   const int64_t k = e.cell;
   const int t = e.code & 1, cs = (int)(e.code >> 1);
-  const int64_t v[3] = {k, k + N + 1, t == 0 ? k + 1 : k + N};
+  const int64_t v[3] = {k, k + g.nx + 1, t == 0 ? k + 1 : k + g.nx};
   double x[3][2];
-  for (int q = 0; q < 3; ++q) grid_position(v[q], N, c0, c1, range, &x[q][0], &x[q][1]);
+  for (int q = 0; q < 3; ++q) geom_position(g, v[q], &x[q][0], &x[q][1]);
   double px, py;
   if (cs == 0) {
     px = (x[0][0] + x[1][0] + x[2][0]) / 3;
@@ -221,25 +231,23 @@ __global__ void k_image_rays(const ImageHit* __restrict__ hits, const uint32_t* 
 
 }  // namespace
 
-int count_grid_images(slsim_ctx c, size_t N, double c0, double c1, double range,
-                      const double* y_dev, int32_t* mult_dev) {
+int count_grid_images(slsim_ctx c, const GridGeom& g, const double* y_dev, int32_t* mult_dev) {
   hipStream_t st = c->stream;
-  launch_zero_u32(st, (uint32_t*)mult_dev, (int)(N * N));
-  const int64_t ncell = N >= 2 ? (int64_t)(N - 1) * (int64_t)(N - 1) : 0;
+  SLSIM_HIP_CHECK(hipMemsetAsync(mult_dev, 0, 4 * (size_t)(g.nx * g.ny), st));
+  const int64_t ncell = g.nx >= 2 && g.ny >= 2 ? (g.nx - 1) * (g.ny - 1) : 0;
   if (ncell > 0) {
-    k_count_grid_images<<<(unsigned)((ncell + 255) / 256), 256, 0, st>>>(y_dev, (int64_t)N, c0,
-                                                                        c1, range, mult_dev);
+    k_count_grid_images<<<(unsigned)((ncell + 255) / 256), 256, 0, st>>>(y_dev, g, mult_dev);
     SLSIM_HIP_CHECK(hipGetLastError());
   }
   return SLSIM_OK;
 }
 
-int find_images(slsim_ctx c, size_t N, double c0, double c1, double range, const double* y_dev,
-                const double* A_dev, size_t nsrc, const double* ys, size_t cap,
-                int32_t* multiplicity, double* rays, int64_t* triangles, size_t* nimages) {
+int find_images(slsim_ctx c, const GridGeom& g, const double* y_dev, const double* A_dev,
+                size_t nsrc, const double* ys, size_t cap, int32_t* multiplicity, double* rays,
+                int64_t* triangles, size_t* nimages) {
   *nimages = 0;
   for (size_t s = 0; s < nsrc; ++s) multiplicity[s] = 0;
-  if (N < 2 || nsrc == 0) return SLSIM_OK;
+  if (g.nx < 2 || g.ny < 2 || nsrc == 0) return SLSIM_OK;
   hipStream_t st = c->stream;
   Scratch sc(c);
   double* dys;
@@ -318,8 +326,8 @@ int find_images(slsim_ctx c, size_t N, double c0, double c1, double range, const
   bins.x0 = bx0;
   bins.y0 = by0;
   bins.inv_w = inv_w;
-  const int64_t ncell = (int64_t)(N - 1) * (int64_t)(N - 1);
-  k_find_images<<<(unsigned)((ncell + 255) / 256), 256, 0, st>>>(y_dev, (int64_t)N, dys,
+  const int64_t ncell = (g.nx - 1) * (g.ny - 1);
+  k_find_images<<<(unsigned)((ncell + 255) / 256), 256, 0, st>>>(y_dev, g.nx, g.ny, dys,
                                                                   (int)nsrc, bins, (uint32_t)cap,
                                                                   dcount, dhits);
   SLSIM_HIP_CHECK(hipGetLastError());
@@ -364,8 +372,7 @@ int find_images(slsim_ctx c, size_t N, double c0, double c1, double range, const
     return rc;
   SLSIM_HIP_CHECK(hipMemcpyAsync(dlist, list.data(), 4 * list.size(), hipMemcpyHostToDevice, st));
   k_image_rays<<<(unsigned)((nhit + 255) / 256), 256, 0, st>>>(dhits, dlist, nhit, (uint32_t)cap,
-                                                               A_dev, (int64_t)N, c0, c1, range,
-                                                               dys, drays, dtri);
+                                                               A_dev, g, dys, drays, dtri);
   SLSIM_HIP_CHECK(hipGetLastError());
   SLSIM_HIP_CHECK(hipMemcpyAsync(rays, drays, 64 * nhit, hipMemcpyDeviceToHost, st));
   SLSIM_HIP_CHECK(hipMemcpyAsync(triangles, dtri, 24 * nhit, hipMemcpyDeviceToHost, st));

@@ -223,6 +223,32 @@ __host__ __device__ inline void grid_position(int64_t i, int64_t N, double c0, d
   }
 }
 
+// A GridMap's points: the N x N grid of GridMap(lens, N, center, range)
+// (grid_position) or the Nx x Ny grid of GridMap(lens, Nx, center, rangeX,
+// rangeY) (TreeCode_link/gridmap.cpp:26-91: point ii + jj Nx at the lower-left
+// corner plus (ii, jj) rangeX / (Nx - 1)).
+struct GridGeom {
+  int64_t nx, ny;
+  double c0, c1, rx, ry;
+  int32_t rect;
+  int32_t pad;
+};
+inline GridGeom square_geom(int64_t N, double c0, double c1, double range) {
+  return GridGeom{N, N, c0, c1, range, range, 0, 0};
+}
+__host__ __device__ inline void geom_position(const GridGeom& g, int64_t k, double* x0,
+                                              double* x1) {
+#pragma clang fp contract(off)
+  if (!g.rect) {
+    grid_position(k, g.nx, g.c0, g.c1, g.rx, x0, x1);
+    return;
+  }
+  const double xo = g.c0 - 0.5 * g.rx, yo = g.c1 - 0.5 * g.ry;
+  const double res = g.rx / (double)(g.nx - 1);
+  *x0 = xo + (double)(k % g.nx) * res;
+  *x1 = yo + (double)(k / g.nx) * res;
+}
+
 }  // namespace slsim
 
 struct slsim_ctx_s {
@@ -378,19 +404,18 @@ void uniform_source(const slsim_lens_s* L, double zs, int32_t* jmax, double* aal
 void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint32_t* counters,
                 std::vector<StepJob>& steps);
 // GridMap::find_images over a shot grid (images.hip)
-int find_images(slsim_ctx c, size_t N, double c0, double c1, double range, const double* y_dev,
-                const double* A_dev, size_t nsrc, const double* ys, size_t cap,
-                int32_t* multiplicity, double* rays, int64_t* triangles, size_t* nimages);
+int find_images(slsim_ctx c, const GridGeom& g, const double* y_dev, const double* A_dev,
+                size_t nsrc, const double* ys, size_t cap, int32_t* multiplicity, double* rays,
+                int64_t* triangles, size_t* nimages);
 // images of every grid position as a source, per position (images.hip)
-int count_grid_images(slsim_ctx c, size_t N, double c0, double c1, double range,
-                      const double* y_dev, int32_t* mult_dev);
+int count_grid_images(slsim_ctx c, const GridGeom& g, const double* y_dev, int32_t* mult_dev);
 // critical curves and contours of a shot grid (crit.hip)
-int find_crit(slsim_ctx c, size_t N, double c0, double c1, double range, const double* y_dev,
-              const double* A_dev, slsim_curves_s* out);
-int find_magnification_contour(slsim_ctx c, size_t N, double c0, double c1, double range,
-                               const double* A_dev, double invmag, slsim_curves_s* out);
-int find_boundaries_of_caustics(slsim_ctx c, size_t N, double c0, double c1, double range,
-                                const double* y_dev, int32_t* mult_host, slsim_curves_s* out);
+int find_crit(slsim_ctx c, const GridGeom& g, const double* y_dev, const double* A_dev,
+              slsim_curves_s* out);
+int find_magnification_contour(slsim_ctx c, const GridGeom& g, const double* A_dev,
+                               double invmag, slsim_curves_s* out);
+int find_boundaries_of_caustics(slsim_ctx c, const GridGeom& g, const double* y_dev,
+                                int32_t* mult_host, slsim_curves_s* out);
 // Utilities::find_boundaries, convex_hull and inCurve on host data (crit.hip)
 int find_boundaries_host(const uint8_t* bitmap, size_t nx, size_t ny, slsim_curves_s* out);
 std::vector<int64_t> convex_hull_host(const double* p, size_t n);

@@ -60,16 +60,15 @@ int main(int argc, char** argv) {
       RAY r = rect.ray(i);
       w.insert(w.end(), {r.x[0], r.x[1], r.y[0], r.y[1], r.A[0], r.A[1], r.A[2], r.A[3]});
     }
-    bool threw = false;
-    try {
-      std::vector<std::vector<Point_2d>> cv;
-      std::vector<bool> hb;
-      std::vector<CritType> ct;
-      rect.find_crit(cv, hb, ct);
-    } catch (const std::invalid_argument&) {
-      threw = true;
+    // its magnification contours at invmag = 0.5 (count, then each as n, points)
+    std::vector<std::vector<Point_2d>> cv;
+    std::vector<bool> hb;
+    rect.find_magnification_contour(cv, hb, 0.5);
+    w.push_back((double)cv.size());
+    for (auto& c : cv) {
+      w.push_back((double)c.size());
+      for (const Point_2d& p : c) w.insert(w.end(), {p[0], p[1]});
     }
-    w.push_back(threw);
     FILE* g = std::fopen(argv[2], "wb");
     if (!g) return 1;
     std::fwrite(w.data(), sizeof(double), w.size(), g);

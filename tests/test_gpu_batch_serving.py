@@ -164,8 +164,8 @@ def test_cpp_dropin_program(ctx, tmp_path):
     nx, npts = int(q[0]), int(q[1])
     rx_, ry_ = 30 * S.arcsecTOradians, 18.5 * S.arcsecTOradians
     ny = int(nx * (ry_ / rx_))
-    assert (nx, npts) == (40, 40 * ny) and q[-1] == 1.0     # find_crit refuses it
-    v2 = q[2:-1].reshape(-1, 8)
+    assert (nx, npts) == (40, 40 * ny)
+    v2 = q[2:2 + 8 * npts].reshape(-1, 8)
     res = rx_ / (nx - 1)
     ii, jj = np.arange(npts) % nx, np.arange(npts) // nx
     assert np.array_equal(v2[:, 0], (1e-6 - 0.5 * rx_) + ii * res)
@@ -173,6 +173,16 @@ def test_cpp_dropin_program(ctx, tmp_path):
     yr, Ar = L.shoot(v2[:, :2])
     assert np.abs(v2[:, 2:4] - yr).max() <= 1e-12 * np.abs(yr).max()
     assert np.abs(v2[:, 4:8] - Ar).max() <= 1e-8
+    # its magnification contours from C++ equal the oracle's on the C++ grid
+    c0, _, _, _ = O.find_magnification_contour(nx, v2[:, :2], v2[:, 4:8], rx_, 0.5, ny=ny)
+    at = 2 + 8 * npts
+    assert q[at] == len(c0)
+    at += 1
+    for cv in c0:
+        assert q[at] == len(cv)
+        assert np.array_equal(q[at + 1:at + 1 + 2 * len(cv)].reshape(-1, 2), cv)
+        at += 1 + 2 * len(cv)
+    assert at == len(q)
 
 
 def test_server_ray_requests_equal_rayshooter(ctx, lenses):

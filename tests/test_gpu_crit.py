@@ -28,13 +28,17 @@ def shot(ctx):
 class _Field:
     """Grid outputs uploaded from the host, viewed as a GridMap."""
 
-    def __init__(self, ctx, N, centre, range_, y, A):
+    def __init__(self, ctx, N, centre, range_, y, A, ny=None, range_y=None):
         self.ctx, self.N, self.center, self.range = ctx, N, tuple(centre), range_
-        self.y_dev = ctx.alloc(16 * N * N)
-        self.A_dev = ctx.alloc(32 * N * N)
+        self.ny = N if ny is None else ny
+        self.range_y = range_ if range_y is None else range_y
+        self.layout = S._abi.SLSIM_GRID_SQUARE if ny is None else S._abi.SLSIM_GRID_RECT
+        self.y_dev = ctx.alloc(16 * N * self.ny)
+        self.A_dev = ctx.alloc(32 * N * self.ny)
         ctx.h2d(self.y_dev, np.ascontiguousarray(y, dtype=np.float64))
         ctx.h2d(self.A_dev, np.ascontiguousarray(A, dtype=np.float64))
 
+    _grid = S.GridMap._grid
     find_crit = S.GridMap.find_crit
     find_magnification_contour = S.GridMap.find_magnification_contour
     find_boundaries_of_caustics = S.GridMap.find_boundaries_of_caustics
@@ -202,3 +206,48 @@ def test_tiny_grids(ctx, N):
     r0, m1, t0 = O.find_images(N, x, y, A, ys)
     assert np.array_equal(mult, m1) and np.array_equal(rays, r0) and np.array_equal(tris, t0)
     f.close()
+
+
+def test_rectangular_grids(ctx):
+    # GridMap(lens, Nx, center, rangeX, rangeY): images, critical curves,
+    # contours and caustic boundaries on an Nx x Ny grid equal the oracle's
+    nx, c, rx, ry = 120, (0.05, -0.1), 4.0, 2.6
+    x, ny = O.gridmap_rect_positions(nx, c, rx, ry)
+    lenses = [((-0.9, 0.2), 0.5, 0.1), ((0.9, -0.4), 0.35, 0.0)]
+    y, A = _isothermal(x, lenses)
+    f = _Field(ctx, nx, c, rx, y, A, ny=ny, range_y=ry)
+    f.find_images = S.GridMap.find_images.__get__(f)
+    print("case: rectangular grid %d x %d" % (nx, ny), flush=True)
+    got = f.find_crit()
+    _same(got, O.find_crit(nx, x, y, A, rx, ny=ny))
+    T, R, P = S.SLSIM_CRIT_TANGENTIAL, S.SLSIM_CRIT_RADIAL, S.SLSIM_CRIT_PSEUDO
+    assert sorted(got[2]) == sorted([T, R, T, P])
+    _same(f.find_magnification_contour(0.3), O.find_magnification_contour(nx, x, A, rx, 0.3,
+                                                                          ny=ny))
+    cb, hb, mb = f.find_boundaries_of_caustics()
+    c0, h0, m0 = O.find_boundaries_of_caustics(nx, x, y, A, ny=ny)
+    assert np.array_equal(mb, m0) and np.array_equal(hb, h0) and len(cb) == len(c0)
+    for a, b in zip(cb, c0):
+        assert np.array_equal(a, b)
+    rng = np.random.default_rng(8)
+    ys = y[rng.integers(0, nx * ny, 300)] + rng.normal(size=(300, 2)) * 1e-3
+    rays, mult, tris = f.find_images(ys, 32)
+    r0, m1, t0 = O.find_images(nx, x, y, A, ys, ny=ny)
+    assert np.array_equal(mult, m1) and np.array_equal(tris, t0) and np.array_equal(rays, r0)
+    assert mult.max() >= 3
+    f.close()
+
+
+def test_rectangular_gridmap_shoot(ctx, shot):
+    lens = shot[0].lens
+    rg = W.grid_range(30.0)
+    g = S.GridMap.rect(lens, 64, (1e-6, 0.0), rg, 0.55 * rg)
+    x, ny = O.gridmap_rect_positions(64, (1e-6, 0.0), rg, 0.55 * rg)
+    assert g.ny == ny and np.array_equal(g.positions(), x)
+    # the same rays as a ray list (rayshooterInternal): the grid's outputs bit for bit
+    yl, Al = lens.rayshooterInternal(x)
+    assert np.array_equal(g.y(), yl) and np.array_equal(g.A(), Al)
+    assert g.writePixelMap("KAPPA").shape == (ny, 64)
+    got = g.find_crit()
+    _same(got, O.find_crit(64, x, g.y(), g.A(), rg, ny=ny))
+    g.close()
