@@ -1218,6 +1218,36 @@ struct GridMap {
     hits_edge.assign(hb.begin(), hb.end());
   }
 
+  // gridmap.cpp:323-331: the area of the pixels with det A < 0
+  PosType EinsteinArea() const {
+    size_t count = 0;
+    const size_t N = (size_t)Ngrid_init * Ngrid_init2;
+    for (size_t i = 0; i < N; ++i)
+      if (i_points[i].invmag() < 0) ++count;
+    return count * x_range * x_range / Ngrid_init / Ngrid_init;
+  }
+  // gridmap.cpp:148-163: every ray unlensed (A = I, y = x), also in HBM
+  void deLens() {
+    const size_t n = (size_t)Ngrid_init * Ngrid_init2;
+    std::vector<double> y(2 * n), A(4 * n);
+    for (size_t i = 0; i < n; ++i) {
+      i_points[i].A.setToI();
+      s_points[i].A.setToI();
+      i_points[i].dt = 0;
+      s_points[i].dt = 0;
+      s_points[i].x[0] = i_points[i].x[0];
+      s_points[i].x[1] = i_points[i].x[1];
+      y[2 * i] = i_points[i].x[0];
+      y[2 * i + 1] = i_points[i].x[1];
+      A[4 * i] = A[4 * i + 3] = 1.0;
+    }
+    if (yd && n) {
+      slsim_ctx ctx = slsim_cpp::context();
+      slsim_cpp::check(slsim_memcpy_h2d(ctx, yd, y.data(), 16 * n));
+      slsim_cpp::check(slsim_memcpy_h2d(ctx, Ad, A.data(), 32 * n));
+    }
+  }
+
   size_t getNumberOfPoints() const { return (size_t)Ngrid_init * Ngrid_init2; }
   int getInitNgrid() const { return Ngrid_init; }
   double getXRange() const { return x_range; }

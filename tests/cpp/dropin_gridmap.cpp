@@ -60,6 +60,7 @@ int main(int argc, char** argv) {
       RAY r = rect.ray(i);
       w.insert(w.end(), {r.x[0], r.x[1], r.y[0], r.y[1], r.A[0], r.A[1], r.A[2], r.A[3]});
     }
+    w.push_back(rect.EinsteinArea());
     // its magnification contours at invmag = 0.5 (count, then each as n, points)
     std::vector<std::vector<Point_2d>> cv;
     std::vector<bool> hb;
@@ -69,6 +70,14 @@ int main(int argc, char** argv) {
       w.push_back((double)c.size());
       for (const Point_2d& p : c) w.insert(w.end(), {p[0], p[1]});
     }
+    // deLens: no pixel left with det A < 0, y = x, and no contour below 0.5
+    rect.deLens();
+    std::vector<std::vector<Point_2d>> cv2;
+    std::vector<bool> hb2;
+    rect.find_magnification_contour(cv2, hb2, 0.5);
+    w.push_back(rect.EinsteinArea());
+    w.push_back((double)cv2.size());
+    w.push_back(rect.ray(100).y[0] == rect.ray(100).x[0] ? 1.0 : 0.0);
     FILE* g = std::fopen(argv[2], "wb");
     if (!g) return 1;
     std::fwrite(w.data(), sizeof(double), w.size(), g);

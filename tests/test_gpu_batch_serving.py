@@ -173,16 +173,21 @@ def test_cpp_dropin_program(ctx, tmp_path):
     yr, Ar = L.shoot(v2[:, :2])
     assert np.abs(v2[:, 2:4] - yr).max() <= 1e-12 * np.abs(yr).max()
     assert np.abs(v2[:, 4:8] - Ar).max() <= 1e-8
+    # GridMap::EinsteinArea: the pixels with det A < 0 times the pixel area
+    A2 = v2[:, 4:8]
+    neg = int(np.sum(A2[:, 0] * A2[:, 3] - A2[:, 1] * A2[:, 2] < 0))
+    assert q[2 + 8 * npts] == neg * rx_ * rx_ / nx / nx
     # its magnification contours from C++ equal the oracle's on the C++ grid
     c0, _, _, _ = O.find_magnification_contour(nx, v2[:, :2], v2[:, 4:8], rx_, 0.5, ny=ny)
-    at = 2 + 8 * npts
+    at = 3 + 8 * npts
     assert q[at] == len(c0)
     at += 1
     for cv in c0:
         assert q[at] == len(cv)
         assert np.array_equal(q[at + 1:at + 1 + 2 * len(cv)].reshape(-1, 2), cv)
         at += 1 + 2 * len(cv)
-    assert at == len(q)
+    # after deLens: no Einstein area, no contour, y = x
+    assert list(q[at:]) == [0.0, 0.0, 1.0]
 
 
 def test_server_ray_requests_equal_rayshooter(ctx, lenses):
