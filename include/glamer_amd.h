@@ -769,19 +769,15 @@ class Lens {
     slsim_cpp::check(slsim_lens_tables(lens, &n, z.data(), nullptr, nullptr, nullptr));
     return toggle ? zs_implant : z.back();
   }
-  // MultiPlane/lens.cpp:2609-2657 (nearest = false)
+  // MultiPlane/lens.cpp:2609-2657
   short ResetSourcePlane(PosType z, bool nearest = false, bool verbose = false) {
-    (void)nearest; (void)verbose;
-    if (z <= 0.0) return 0;
-    slsim_cpp::check(slsim_lens_reset_source(lens, z));
+    (void)verbose;
+    int32_t j = 0;
+    double zi = 0;
+    slsim_cpp::check(slsim_lens_reset_source_plane(lens, z, nearest, &j, &zi));
     toggle = true;
-    zs_implant = z;
-    int n = getNplanes();
-    std::vector<double> zp(n + 1);
-    slsim_cpp::check(slsim_lens_tables(lens, &n, zp.data(), nullptr, nullptr, nullptr));
-    short j = 0;
-    while (j < n && zp[j] < z) ++j;
-    return j;
+    zs_implant = zi;
+    return (short)j;
   }
   void RevertSourcePlane() {
     slsim_cpp::check(slsim_lens_reset_source(lens, -1));

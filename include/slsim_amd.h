@@ -170,6 +170,15 @@ int slsim_lens_tables(slsim_lens lens, int* nplanes, double* z, double* Dl, doub
 /* Lens::ResetSourcePlane(z, nearest=false) (MultiPlane/lens.cpp:2609-2657);
  * z <= 0 reverts to the lens source. */
 int slsim_lens_reset_source(slsim_lens lens, double z);
+/* Lens::ResetSourcePlane(z, nearest) (MultiPlane/lens.cpp:2609-2657) in full:
+ * *index = the plane index the reference returns (Numerical Recipes' locate
+ * on the plane distances, one back when the source distance equals the
+ * previous plane's or, with nearest, when z is below the redshift halfway in
+ * distance to the previous plane), *zs_implant = the implanted source
+ * redshift (with nearest, that plane's redshift when there is one).  As the
+ * reference, z <= 0 changes nothing but turns the implanted source on. */
+int slsim_lens_reset_source_plane(slsim_lens lens, double z, int nearest, int32_t* index,
+                                  double* zs_implant);
 
 /* ------------------------------------------------------------- ray shooting */
 typedef struct {

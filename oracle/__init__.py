@@ -476,3 +476,56 @@ def find_boundaries_of_caustics(N, x, y, A, ny=None):
     _, mult, _ = find_images(N, x, y, A, x, cap=64 * N * ny, ny=ny)
     c, h, _, _ = _crit(2, N, x, y, A, mult, 0.0, 0.0, ny)
     return c, h, mult
+
+
+# ------------------------------------------------ Lens::ResetSourcePlane
+def _nr_locate(xx, x):
+    """Numerical Recipes' locate (Numerical Recipes in C, 2nd ed., sec. 3.4)
+    on the 0-based table xx, returning the book's 1-based j: xx[j-1] <= x <
+    xx[j], 1 at x == xx[0], n - 1 at x == xx[n-1]."""
+    n = len(xx)
+    jl, ju = 0, n + 1
+    ascnd = xx[n - 1] >= xx[0]
+    while ju - jl > 1:
+        jm = (ju + jl) >> 1
+        if (x >= xx[jm - 1]) == ascnd:
+            jl = jm
+        else:
+            ju = jm
+    if x == xx[0]:
+        return 1
+    if x == xx[n - 1]:
+        return n - 1
+    return jl
+
+
+def inv_coordist(D, Om=0.3, OL=0.7, h=0.7):
+    """The redshift at comoving distance D (bisection on coorDist)."""
+    lo, hi = 0.0, 1.0
+    while coorDist(0.0, hi, Om, OL, h) < D and hi < 1e4:
+        hi *= 2
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if mid in (lo, hi):
+            break
+        if coorDist(0.0, mid, Om, OL, h) < D:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def reset_source_plane(tables, z, nearest, Om=0.3, OL=0.7, h=0.7):
+    """Lens::ResetSourcePlane(z, nearest) (MultiPlane/lens.cpp:2609-2657) on
+    a lens's plane tables (Lens.tables()): (plane index, implanted source
+    redshift)."""
+    zp, Dl, dDl, n = tables["z"], tables["Dl"], tables["dDl"], tables["nplanes"]
+    Ds = coorDist(0.0, z, Om, OL, h)
+    j = _nr_locate(Dl[:n], Ds) if len(Dl) > 1 and n > 0 else 0
+    if j > 0:
+        if Dl[j - 1] == Ds:
+            j -= 1
+        elif nearest:
+            if z < inv_coordist(Dl[j] - 0.5 * dDl[j], Om, OL, h):
+                j -= 1
+    return j, (float(zp[j]) if nearest and j < n else z)

@@ -348,8 +348,19 @@ class Lens:
         halo.owned_by_lens = True
         self.halos.append(halo)
 
-    def ResetSourcePlane(self, z):
-        _check(lib.slsim_lens_reset_source(self.h, float(z)))
+    def ResetSourcePlane(self, z, nearest=False):
+        """Lens::ResetSourcePlane(z, nearest) (MultiPlane/lens.cpp:2609-2657):
+        returns (the reference's plane index, the implanted source redshift).
+        z <= 0 turns the implanted source on and changes nothing else, as the
+        reference does; RevertSourcePlane goes back to the lens source."""
+        j = ctypes.c_int32()
+        zi = ctypes.c_double()
+        _check(lib.slsim_lens_reset_source_plane(self.h, float(z), int(bool(nearest)),
+                                                 ctypes.byref(j), ctypes.byref(zi)))
+        return j.value, zi.value
+
+    def RevertSourcePlane(self):
+        _check(lib.slsim_lens_reset_source(self.h, -1.0))
 
     def tables(self):
         n = ctypes.c_int()

@@ -926,6 +926,47 @@ int slsim_lens_reset_source(slsim_lens L, double z) {
   return SLSIM_OK;
 }
 
+// Numerical Recipes' locate (Press et al., Numerical Recipes in C, 2nd ed.,
+// sec. 3.4; GLAMER's locateD is its double version): for the 1-based table
+// xx[1..n], the jl with xx[jl] <= x < xx[jl+1] by bisection, 1 when x ==
+// xx[1] and n - 1 when x == xx[n].
+static unsigned long nr_locate(const double* xx, unsigned long n, double x) {
+  unsigned long jl = 0, ju = n + 1;
+  const bool ascnd = xx[n] >= xx[1];
+  while (ju - jl > 1) {
+    const unsigned long jm = (ju + jl) >> 1;
+    if ((x >= xx[jm]) == ascnd) jl = jm;
+    else ju = jm;
+  }
+  if (x == xx[1]) return 1;
+  if (x == xx[n]) return n - 1;
+  return jl;
+}
+
+int slsim_lens_reset_source_plane(slsim_lens L, double z, int nearest, int32_t* index,
+                                  double* zs_implant) {
+  if (!L) return arg_error("null lens");
+  L->toggle_source_plane = true;
+  unsigned long j = 0;
+  if (z > 0) {
+    const double Ds = L->coorDist(0, z);
+    const unsigned long n = L->plane_redshifts.size() - 1;   // lensing planes
+    if (L->Dl.size() > 1) j = nr_locate(L->Dl.data() - 1, n, Ds);
+    if (j > 0) {
+      if (L->Dl[j - 1] == Ds) {
+        --j;
+      } else if (nearest) {
+        const double z1 = slsim_inv_coordist(L->Om, L->OL, L->h, L->Dl[j] - 0.5 * L->dDl[j]);
+        if (z < z1) --j;
+      }
+    }
+    L->zs_implant = (nearest && j < n) ? L->plane_redshifts[j] : z;
+  }
+  if (index) *index = (int32_t)j;
+  if (zs_implant) *zs_implant = L->zs_implant;
+  return SLSIM_OK;
+}
+
 int slsim_gridmap_shoot(slsim_lens L, size_t N, double c0, double c1, double range, double zs,
                         double* y_dev, double* A_dev, int precision, slsim_shoot_stats* stats) {
   if (!y_dev || !A_dev) return arg_error("null output buffer");

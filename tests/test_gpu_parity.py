@@ -329,11 +329,24 @@ def test_reset_source_plane(field):
     lens = field["lens"]
     N, rg = 64, W.grid_range(300.0)
     pos = O.gridmap_positions(N, (0.0, 0.0), rg)
-    lens.ResetSourcePlane(1.2)
+    j, zi = lens.ResetSourcePlane(1.2)
     g = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP64)
-    lens.ResetSourcePlane(-1)
+    lens.RevertSourcePlane()
+    assert zi == 1.2
     y_ref, A_ref = field["oracle"].shoot(pos, zs=np.full(pos.shape[0], 1.2))
     assert_fields_close(pos, g.y(), g.A(), y_ref, A_ref, 1e-9, 1e-8)
+    # the plane index and, with nearest, the implanted redshift as the
+    # oracle's restatement of ResetSourcePlane gives them
+    t = field["oracle"].tables()
+    zp = t["z"]
+    cases = [0.05, zp[0], 0.5 * (zp[0] + zp[1]), zp[3], zp[3] + 1e-9, 1.2,
+             zp[-2], 0.5 * (zp[-2] + zp[-1]), zp[-1] + 0.5]
+    for z in cases:
+        for nearest in (False, True):
+            got = lens.ResetSourcePlane(float(z), nearest)
+            want = O.reset_source_plane(t, float(z), nearest)
+            assert got[0] == want[0] and abs(got[1] - want[1]) <= 1e-15 * want[1], (z, nearest)
+    lens.RevertSourcePlane()
 
 
 # ------------------------------------------------------------ small cases

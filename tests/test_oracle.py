@@ -397,3 +397,24 @@ def test_smoothing_known_answers():
     assert np.all(s[:10] == 0)
     with pytest.raises(ValueError):
         O.smoothing_sizes(x[:8], zero[:8], zero[:8], 7)
+
+
+def test_reset_source_plane_choice():
+    # nearest: the plane nearest in comoving distance; not nearest: the first
+    # plane at or beyond the source, one back when the source sits on a plane
+    tab = {"nplanes": 4, "z": np.array([0.2, 0.5, 0.9, 1.4, 3.0])}
+    tab["Dl"] = np.array([O.coorDist(0, z) for z in tab["z"]])
+    tab["dDl"] = np.diff(np.concatenate([[0.0], tab["Dl"]]))
+    for z in np.linspace(0.05, 2.5, 200):
+        Ds = O.coorDist(0, z)
+        j, zi = O.reset_source_plane(tab, z, False)
+        assert j == int(np.sum(tab["Dl"][:4] < Ds)) and zi == z
+        j, zi = O.reset_source_plane(tab, z, True)
+        if Ds <= tab["Dl"][0] or Ds > tab["Dl"][3] + 0.5 * (tab["Dl"][4] - tab["Dl"][3]):
+            continue
+        near = int(np.argmin(np.abs(tab["Dl"][:4] - Ds)))
+        assert j == near and zi == tab["z"][near], z
+    # on a plane: that plane, with or without nearest
+    for k in range(4):
+        assert O.reset_source_plane(tab, tab["z"][k], False)[0] == k
+        assert O.reset_source_plane(tab, tab["z"][k], True) == (k, tab["z"][k])
