@@ -740,8 +740,27 @@ class Lens {
     (void)addplanes; (void)verbose;
     T* halo = new T(std::move(halo_in));
     owned.emplace_back(halo, [](void* p) { delete static_cast<T*>(p); });
+    main_zlens.push_back(halo->getZlens());
     slsim_cpp::check(slsim_lens_add_halo(lens, halo->release_plane(), halo->getZlens()));
   }
+
+  // MultiPlane/lens.cpp:1025-1036
+  void clearMainHalos(bool verbose = false) {
+    (void)verbose;
+    slsim_cpp::check(slsim_lens_clear_main_halos(lens));
+    owned.clear();
+    main_zlens.clear();
+  }
+  std::size_t getNMainHalos() const { return main_zlens.size(); }
+  // include/lens.h:102-117: the first main halo's redshift and angular
+  // size distance
+  PosType getZlens() const {
+    if (main_zlens.empty()) throw std::runtime_error("error, no main lens present");
+    return main_zlens[0];
+  }
+  PosType getAngDistLens() const { return cosmo_.angDist(getZlens()); }
+  PosType getfov() const { return fieldofview; }
+  void setfov(PosType fov) { fieldofview = fov; }
 
   // A field plane at redshift z (the reference builds its field planes in
   // Lens::createFieldPlanes, MultiPlane/lens.cpp:349-455, with double
@@ -867,6 +886,8 @@ class Lens {
   slsim_lens lens = nullptr;
   bool toggle = false;
   double zs_implant = 0;
+  PosType fieldofview = 0;
+  std::vector<PosType> main_zlens;   // redshifts of the main halos, in order
   std::vector<std::unique_ptr<void, void (*)(void*)>> owned;
 };
 typedef Lens* LensHndl;

@@ -163,6 +163,10 @@ int slsim_lens_add_halo(slsim_lens lens, slsim_plane plane, float z);
  * (lens.cpp:2397-2515) sorts it with the main planes by distance (ahead of a
  * main plane at the same distance). */
 int slsim_lens_add_field_plane(slsim_lens lens, slsim_plane plane, double z);
+/* Lens::clearMainHalos (MultiPlane/lens.cpp:1025-1036): every main halo and
+ * its plane is released and the planes are combined again (field planes
+ * stay). */
+int slsim_lens_clear_main_halos(slsim_lens lens);
 /* Number of lens planes, and the plane tables plane_redshifts, Dl, dDl (each
  * nplanes + 1 long, the source pseudo-plane last) and charge = 4 pi G. */
 int slsim_lens_tables(slsim_lens lens, int* nplanes, double* z, double* Dl, double* dDl,

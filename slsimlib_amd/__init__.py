@@ -336,6 +336,17 @@ class Lens:
         _check(lib.slsim_lens_create(ctx.h, float(zsource), float(Om), float(OL), float(h),
                                      ctypes.byref(self.h)))
 
+    def clearMainHalos(self):
+        """Lens::clearMainHalos (MultiPlane/lens.cpp:1025-1036)."""
+        _check(lib.slsim_lens_clear_main_halos(self.h))
+        keep = []
+        for hh in self.halos:
+            if isinstance(hh, LensPlaneTree):
+                keep.append(hh)
+            else:
+                hh.h = None           # released by the lens
+        self.halos = keep
+
     def addFieldPlane(self, plane):
         """A LensPlaneTree as a field plane at its (double) redshift; the
         lens sorts it with the main planes (Lens::combinePlanes)."""

@@ -888,6 +888,25 @@ int slsim_lens_add_halo(slsim_lens L, slsim_plane plane, float z) {
   return SLSIM_OK;
 }
 
+int slsim_lens_clear_main_halos(slsim_lens L) {
+  if (!L) return arg_error("null lens");
+  (void)hipStreamSynchronize(L->ctx->stream);
+  std::vector<slsim_plane> keep;
+  for (slsim_plane p : L->owned) {
+    bool main = false;
+    for (const auto& pl : L->main_planes)
+      for (slsim_plane q : pl) main = main || q == p;
+    if (main) slsim_plane_destroy(p);
+    else keep.push_back(p);
+  }
+  L->owned.swap(keep);
+  L->main_planes.clear();
+  L->main_z.clear();
+  L->main_Dl.clear();
+  L->combinePlanes();
+  return SLSIM_OK;
+}
+
 int slsim_lens_add_field_plane(slsim_lens L, slsim_plane plane, double z) {
   if (!L || !plane) return arg_error("null handle");
   if (plane->ctx != L->ctx) return arg_error("plane and lens belong to different contexts");

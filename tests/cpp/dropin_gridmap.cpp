@@ -83,6 +83,12 @@ int main(int argc, char** argv) {
     std::fwrite(w.data(), sizeof(double), w.size(), g);
     std::fclose(g);
   }
+  // Lens accessors, then clearMainHalos: a lens with no plane shoots y = x
+  if (lens.getNMainHalos() != 1 || lens.getZlens() != 0.5f ||
+      lens.getAngDistLens() != cosmo.angDist((double)0.5f))
+    return 3;
+  lens.clearMainHalos();
+  if (lens.getNMainHalos() != 0 || lens.getNplanes() != 0) return 4;
   FILE* f = std::fopen(argv[1], "wb");
   if (!f) return 1;
   std::fwrite(out.data(), sizeof(double), out.size(), f);

@@ -343,3 +343,20 @@ def test_mixed_field_plane_lightcone_matches_oracle(ctx):
     assert np.abs(g.y() - yr).max() <= 1e-12 * np.abs(yr).max()
     assert np.abs(g.A() - Ar).max() <= 1e-8
     g.close()
+
+
+def test_clear_main_halos_keeps_field_planes(ctx):
+    lens = _field_lens(ctx)
+    lens.clearMainHalos()
+    ax, ay, am, ars, ac = _halo_set(500, 33, spread=0.15, rmax=(0.005, 0.08))
+    bx, by, bm, brs, bc = _halo_set(400, 34, spread=0.2, rmax=(0.005, 0.05))
+    L = O.Lens(2.5)
+    L.addFieldPlane(O.FieldTree(ax, ay, am, ars, ac, 50.0, 0.0), 0.2713)
+    L.addFieldPlane(O.FieldTree(bx, by, bm, brs, bc, 80.0, 3.0), 1.1372)
+    print("case: clearMainHalos with two field planes", flush=True)
+    N, rg = 32, W.grid_range(60.0)
+    g = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    yr, Ar = L.shoot(O.gridmap_positions(N, (0.0, 0.0), rg))
+    assert np.abs(g.y() - yr).max() <= 1e-12 * np.abs(yr).max()
+    assert np.abs(g.A() - Ar).max() <= 1e-8
+    g.close()
