@@ -735,13 +735,37 @@ class Lens {
   Lens(const Lens&) = delete;
 
   // include/lens.h:172-185: the lens takes possession of the halo.
+  // addplanes = false: the nearest main plane (MultiPlane/lens.cpp:822-840)
   template <typename T>
   void moveinMainHalo(T& halo_in, bool addplanes, bool verbose = false) {
-    (void)addplanes; (void)verbose;
+    (void)verbose;
     T* halo = new T(std::move(halo_in));
     owned.emplace_back(halo, [](void* p) { delete static_cast<T*>(p); });
     main_zlens.push_back(halo->getZlens());
-    slsim_cpp::check(slsim_lens_add_halo(lens, halo->release_plane(), halo->getZlens()));
+    if (addplanes)
+      slsim_cpp::check(slsim_lens_add_halo(lens, halo->release_plane(), halo->getZlens()));
+    else
+      slsim_cpp::check(
+          slsim_lens_add_halo_nearest(lens, halo->release_plane(), halo->getZlens()));
+  }
+
+  // include/lens.h:220-236: copies of the halos added to the existing ones
+  template <typename T>
+  void insertMainHalos(std::vector<T>& my_halos, bool addplanes, bool verbose = false) {
+    for (T& h : my_halos) insertMainHalo(h, addplanes, verbose);
+  }
+  // include/lens.h:194-207, 244-263: the halos replace the main halos, one
+  // plane per redshift
+  template <typename T>
+  void replaceMainHalo(const T& halo_in, bool addplanes, bool verbose = false) {
+    (void)addplanes;
+    clearMainHalos(verbose);
+    insertMainHalo(halo_in, true, verbose);
+  }
+  template <typename T>
+  void replaceMainHalos(std::vector<T>& my_halos, bool verbose) {
+    clearMainHalos(verbose);
+    insertMainHalos(my_halos, true, verbose);
   }
 
   // MultiPlane/lens.cpp:1025-1036

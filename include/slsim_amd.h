@@ -157,6 +157,12 @@ int slsim_lens_destroy(slsim_lens lens);
 /* Lens::moveinMainHalo(halo, addplanes=true) (include/lens.h:172-185): the
  * lens takes ownership of `plane`, placed at float redshift z. */
 int slsim_lens_add_halo(slsim_lens lens, slsim_plane plane, float z);
+/* Lens::moveinMainHalo(halo, addplanes=false) (include/lens.h:172-185 ->
+ * addMainHaloToNearestPlane, MultiPlane/lens.cpp:822-840): the halo joins the
+ * main plane closest in distance, whatever its own redshift.  On a lens with
+ * no main plane the reference makes a plane of the halo and then adds the
+ * halo to it again, so it deflects twice: kept. */
+int slsim_lens_add_halo_nearest(slsim_lens lens, slsim_plane plane, float z);
 /* A field plane (MultiPlane/lens.cpp:349-455 builds them in createFieldPlanes)
  * at redshift z (double, as Lens::setFieldDist leaves it): the lens takes
  * ownership of `plane`, a slsim_plane_tree_create_nfw plane, and combinePlanes

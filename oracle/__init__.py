@@ -63,6 +63,7 @@ _sigs = {
     "orc_nfw_function": (c_d, [c_int, c_d]),
     "orc_nfw_interpolate": (c_d, [c_int, c_d]),
     "orc_lens_add_halotree": (None, [c_vp, c_vp, c_f]),
+    "orc_lens_add_nearest": (None, [c_vp, c_vp, c_f, c_int]),
     "orc_find_images": (c_sz, [c_sz, P(c_d), P(c_d), P(c_d), P(c_d), c_sz, P(ctypes.c_int32),
                                P(c_d), P(ctypes.c_int64), c_sz]),
     "orc_lens_nplanes": (c_int, [c_vp]),
@@ -298,9 +299,12 @@ class Lens:
         tree.owned = False
         _lib.orc_lens_add_field_plane(self.L, tree.t, float(z))
 
-    def moveinMainHalo(self, tree, z):
+    def moveinMainHalo(self, tree, z, addplanes=True):
         tree.owned = False          # the lens owns it now
-        if isinstance(tree, HaloTree):
+        if not addplanes:
+            _lib.orc_lens_add_nearest(self.L, tree.t, np.float32(z),
+                                      int(isinstance(tree, HaloTree)))
+        elif isinstance(tree, HaloTree):
             _lib.orc_lens_add_halotree(self.L, tree.t, np.float32(z))
         else:
             _lib.orc_lens_add_halo(self.L, tree.t, np.float32(z))

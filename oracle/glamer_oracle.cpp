@@ -937,6 +937,28 @@ struct Lens {
     combinePlanes();
   }
 
+  // addMainHaloToNearestPlane, MultiPlane/lens.cpp:822-840: the halo goes to
+  // the main plane closest in distance; with no main plane it makes one and
+  // is then added to it a second time.  Utilities::closest is not in the
+  // reference tree: the first plane at the least |Dl - halo_Dl| is taken.
+  void moveinMainHaloNearest(std::unique_ptr<Halo> h) {
+    Halo* raw = h.get();
+    halos.push_back(std::move(h));
+    const double halo_Dl = cosmo.coorDist(0, raw->z);
+    if (main_Dl.empty()) {
+      auto p = std::make_unique<Plane>();
+      p->halos.push_back(raw);
+      main_planes.push_back(std::move(p));
+      main_z.push_back(raw->z);
+      main_Dl.push_back(halo_Dl);
+    }
+    size_t i = 0;
+    for (size_t k = 1; k < main_Dl.size(); ++k)
+      if (std::fabs(main_Dl[k] - halo_Dl) < std::fabs(main_Dl[i] - halo_Dl)) i = k;
+    main_planes[i]->halos.push_back(raw);
+    combinePlanes();
+  }
+
   // FindSourcePlane, MultiPlane/lens.cpp:2659-2681
   void FindSourcePlane(double zs, long& jmax, double& Dls, double& Ds) const {
     Ds = cosmo.coorDist(0, zs);
@@ -1462,6 +1484,16 @@ void orc_lens_add_halotree(void* lp, void* tree, float z) {
   h->htree.reset(static_cast<orc::HaloTree*>(tree));
   h->z = z;
   L->moveinMainHalo(std::move(h));
+}
+
+// moveinMainHalo(halo, addplanes=false): particle tree (kind 0) or halo tree
+void orc_lens_add_nearest(void* lp, void* tree, float z, int halotree) {
+  auto* L = static_cast<orc::Lens*>(lp);
+  auto h = std::make_unique<orc::Halo>();
+  if (halotree) h->htree.reset(static_cast<orc::HaloTree*>(tree));
+  else h->tree.reset(static_cast<orc::TreeQuad*>(tree));
+  h->z = z;
+  L->moveinMainHaloNearest(std::move(h));
 }
 
 // a LensPlaneTree field plane (the lens takes the tree) at redshift z

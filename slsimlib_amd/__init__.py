@@ -354,8 +354,11 @@ class Lens:
         plane.owned_by_lens = True
         self.halos.append(plane)
 
-    def moveinMainHalo(self, halo):
-        _check(lib.slsim_lens_add_halo(self.h, halo.h, ctypes.c_float(float(halo.z))))
+    def moveinMainHalo(self, halo, addplanes=True):
+        """Lens::moveinMainHalo(halo, addplanes) (include/lens.h:172-185):
+        addplanes=False puts the halo on the nearest main plane."""
+        f = lib.slsim_lens_add_halo if addplanes else lib.slsim_lens_add_halo_nearest
+        _check(f(self.h, halo.h, ctypes.c_float(float(halo.z))))
         halo.owned_by_lens = True
         self.halos.append(halo)
 

@@ -221,6 +221,7 @@ SIGNATURES = {
                                             P(slsim_plane)]),
     "slsim_lens_add_field_plane": (c_int, [slsim_lens, slsim_plane, c_double]),
     "slsim_lens_clear_main_halos": (c_int, [slsim_lens]),
+    "slsim_lens_add_halo_nearest": (c_int, [slsim_lens, slsim_plane, c_float]),
     "slsim_lens_reset_source_plane": (c_int, [slsim_lens, c_double, c_int, P(ctypes.c_int32),
                                               P(c_double)]),
     "slsim_halos_create": (c_int, [slsim_ctx, P(c_double), P(c_double), P(c_float), P(c_float),

@@ -921,6 +921,31 @@ int slsim_lens_add_field_plane(slsim_lens L, slsim_plane plane, double z) {
   return SLSIM_OK;
 }
 
+int slsim_lens_add_halo_nearest(slsim_lens L, slsim_plane plane, float z) {
+  if (!L || !plane) return arg_error("null handle");
+  if (plane->ctx != L->ctx) return arg_error("plane and lens belong to different contexts");
+  for (slsim_plane p : L->owned)
+    if (p == plane) return arg_error("halo already in the lens");
+  L->owned.push_back(plane);
+  const double halo_z = z;
+  const double halo_Dl = L->coorDist(0, halo_z);
+  auto& mD = L->main_Dl;
+  if (mD.empty()) {
+    // a plane of the halo; the reference then adds the halo to the closest
+    // plane too, so it is on this plane twice (MultiPlane/lens.cpp:828-837)
+    L->main_planes.push_back({plane});
+    L->main_z.push_back(halo_z);
+    mD.push_back(halo_Dl);
+  }
+  // the closest main plane in distance (the first of equals)
+  size_t i = 0;
+  for (size_t k = 1; k < mD.size(); ++k)
+    if (std::fabs(mD[k] - halo_Dl) < std::fabs(mD[i] - halo_Dl)) i = k;
+  L->main_planes[i].push_back(plane);
+  L->combinePlanes();
+  return SLSIM_OK;
+}
+
 int slsim_lens_tables(slsim_lens L, int* nplanes, double* z, double* Dl, double* dDl,
                       double* charge) {
   if (!L) return arg_error("null lens");

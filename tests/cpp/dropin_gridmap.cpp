@@ -29,6 +29,7 @@ int main(int argc, char** argv) {
   }
   long seed = -1;
   Lens lens(&seed, 2.0, cosmo);
+  std::vector<ParticleType<double>> particles2 = particles;  // the halo takes its vector
   LensHaloParticles<ParticleType<double>> halo(particles, 0.5, cosmo, Point_2d(0, 0), false, 0);
   lens.moveinMainHalo(halo, true);
 
@@ -89,6 +90,16 @@ int main(int argc, char** argv) {
     return 3;
   lens.clearMainHalos();
   if (lens.getNMainHalos() != 0 || lens.getNplanes() != 0) return 4;
+  // addplanes = false on a lens with no plane: one plane; replaceMainHalos
+  // leaves one halo on one plane; clearMainHalos empties the lens again
+  LensHaloParticles<ParticleType<double>> halo2(particles2, 0.5, cosmo, Point_2d(0, 0), false, 0);
+  lens.insertMainHalo(halo2, false);
+  if (lens.getNMainHalos() != 1 || lens.getNplanes() != 1) return 5;
+  std::vector<LensHaloParticles<ParticleType<double>>> more;
+  more.push_back(halo2);
+  lens.replaceMainHalos(more, false);
+  if (lens.getNMainHalos() != 1 || lens.getNplanes() != 1) return 6;
+  lens.clearMainHalos();
   FILE* f = std::fopen(argv[1], "wb");
   if (!f) return 1;
   std::fwrite(out.data(), sizeof(double), out.size(), f);

@@ -360,3 +360,41 @@ def test_clear_main_halos_keeps_field_planes(ctx):
     assert np.abs(g.y() - yr).max() <= 1e-12 * np.abs(yr).max()
     assert np.abs(g.A() - Ar).max() <= 1e-8
     g.close()
+
+
+@pytest.mark.gpu
+def test_nearest_plane_insertion_matches_oracle(ctx):
+    """moveinMainHalo(halo, addplanes=false) (include/lens.h:172-185): the
+    halo joins the nearest main plane; on an empty lens the reference adds it
+    to its own new plane twice."""
+    n = 4096
+    px, py = S.gen_nfw(78, n, 1e13, 8.0, 0.5)
+    pm = np.full(n, np.float32(1e13 / n), np.float32)
+    ph = np.full(n, np.float32(2e-4), np.float32)
+    hx, hy, m, rs, c = _halo_set(200, 16, spread=0.15, rmax=(0.005, 0.05))
+    qx, qy = S.gen_nfw(79, n, 5e12, 6.0, 0.3)
+    N, rg = 48, W.grid_range(60.0)
+    pos = O.gridmap_positions(N, (0.0, 0.0), rg)
+
+    lens = S.Lens(ctx, 2.0)
+    lens.moveinMainHalo(S.LensHaloParticles(ctx, px, py, pm, ph, 0.5))
+    lens.moveinMainHalo(S.LensHaloParticles(ctx, qx, qy, pm, ph, 0.3))
+    lens.moveinMainHalo(S.LensHaloHalos(ctx, hx, hy, m, rs, c, 0.35), addplanes=False)
+    L = O.Lens(2.0)
+    L.moveinMainHalo(O.Tree(px, py, pm, ph), 0.5)
+    L.moveinMainHalo(O.Tree(qx, qy, pm, ph), 0.3)
+    L.moveinMainHalo(O.HaloTree(hx, hy, m, rs, c, inv_area=0.0), 0.35, addplanes=False)
+    assert lens.tables()["nplanes"] == L.tables()["nplanes"] == 2
+    g = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    yr, Ar = L.shoot(pos)
+    assert np.abs(g.y() - yr).max() <= 1e-12 * np.abs(yr).max()
+    assert np.abs(g.A() - Ar).max() <= 1e-8
+
+    one = S.Lens(ctx, 2.0)
+    one.moveinMainHalo(S.LensHaloParticles(ctx, px, py, pm, ph, 0.5), addplanes=False)
+    E = O.Lens(2.0)
+    E.moveinMainHalo(O.Tree(px, py, pm, ph), 0.5, addplanes=False)
+    g1 = S.GridMap(one, N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    yr, Ar = E.shoot(pos)
+    assert np.abs(g1.y() - yr).max() <= 1e-12 * np.abs(yr).max()
+    assert np.abs(g1.A() - Ar).max() <= 1e-8

@@ -240,3 +240,51 @@ def test_mixed_tree_converges_to_direct_sum():
     t0 = O.MixedTree(x, y, m, rs, np.zeros(x.size, np.int32), c.astype(np.float64),
                      inv_area=float(np.float32(0.3)), theta=float(np.float32(0.2)))
     assert np.array_equal(t0.force(rays), O.HaloTree(x, y, m, rs, c, 0.3, 0.2).force(rays))
+
+
+# ---------------------------------------- moveinMainHalo(addplanes=false)
+def _tree(seed):
+    rng = np.random.default_rng(seed)
+    n = 600
+    x = rng.normal(0, 0.05, n)
+    y = rng.normal(0, 0.05, n)
+    m = np.full(n, np.float32(1e10), np.float32)
+    h = np.full(n, np.float32(1e-3), np.float32)
+    return x, y, m, h
+
+
+def test_nearest_plane_insertion_reduces_to_addplanes():
+    """addMainHaloToNearestPlane (MultiPlane/lens.cpp:822-840): a halo at
+    z = 0.42 next to main planes at 0.3 and 0.5 deflects as if it were at the
+    plane nearer in comoving distance; on an empty lens it deflects twice."""
+    rays = np.random.default_rng(5).uniform(-1e-4, 1e-4, (300, 2))
+    a, b, c = _tree(1), _tree(2), _tree(3)
+    d03, d05, d = O.coorDist(0, 0.3), O.coorDist(0, 0.5), O.coorDist(0, np.float32(0.42))
+    znear = 0.5 if abs(d05 - d) < abs(d03 - d) else 0.3
+
+    L = O.Lens(2.0)
+    L.moveinMainHalo(O.Tree(*a), 0.3)
+    L.moveinMainHalo(O.Tree(*b), 0.5)
+    L.moveinMainHalo(O.Tree(*c), 0.42, addplanes=False)
+    R = O.Lens(2.0)
+    R.moveinMainHalo(O.Tree(*a), 0.3)
+    R.moveinMainHalo(O.Tree(*b), 0.5)
+    R.moveinMainHalo(O.Tree(*c), znear)
+    assert L.tables()["nplanes"] == 2
+    for got, want in zip(L.shoot(rays), R.shoot(rays)):
+        assert np.array_equal(got, want)
+    # the other plane is not the nearest: moving the halo there changes y
+    F = O.Lens(2.0)
+    F.moveinMainHalo(O.Tree(*a), 0.3)
+    F.moveinMainHalo(O.Tree(*b), 0.5)
+    F.moveinMainHalo(O.Tree(*c), 0.8 - znear)
+    assert not np.allclose(L.shoot(rays)[0], F.shoot(rays)[0], rtol=0, atol=1e-12)
+
+    E = O.Lens(2.0)
+    E.moveinMainHalo(O.Tree(*a), 0.5, addplanes=False)
+    T = O.Lens(2.0)
+    T.moveinMainHalo(O.Tree(*a), 0.5)
+    T.moveinMainHalo(O.Tree(*a), 0.5)
+    assert E.tables()["nplanes"] == T.tables()["nplanes"] == 1
+    for got, want in zip(E.shoot(rays), T.shoot(rays)):
+        assert np.array_equal(got, want)
