@@ -331,8 +331,8 @@ class LensHaloParticles {
                     Point_2d theta_rotate, bool recenter, double my_inv_area,
                     float MinPSize = 0, double max_range = -1, bool verbose = false,
                     double theta_force = 0.1)
-      : zlens(redshift), inv_area(my_inv_area), min_size(MinPSize) {
-    (void)cosmo; (void)max_range; (void)verbose;
+      : zlens(redshift), inv_area(my_inv_area), min_size(MinPSize), max_range_(max_range) {
+    (void)cosmo; (void)verbose;
     std::swap(pvector, trash_collector);
     set_up(theta_rotate, recenter, theta_force);
   }
@@ -379,7 +379,8 @@ class LensHaloParticles {
   }
   LensHaloParticles(LensHaloParticles&& h) noexcept
       : zlens(h.zlens), inv_area(h.inv_area), min_size(h.min_size), mass(h.mass),
-        theta_force_(h.theta_force_), trash_collector(std::move(h.trash_collector)),
+        theta_force_(h.theta_force_), max_range_(h.max_range_),
+        trash_collector(std::move(h.trash_collector)),
         plane(h.plane) {
     for (int i = 0; i < 3; ++i) mcenter[i] = h.mcenter[i];
     h.plane = nullptr;
@@ -387,7 +388,8 @@ class LensHaloParticles {
   // a copy holds its own device tree, built from the same particles
   LensHaloParticles(const LensHaloParticles& h)
       : zlens(h.zlens), inv_area(h.inv_area), min_size(h.min_size), mass(h.mass),
-        theta_force_(h.theta_force_), trash_collector(h.trash_collector) {
+        theta_force_(h.theta_force_), max_range_(h.max_range_),
+        trash_collector(h.trash_collector) {
     for (int i = 0; i < 3; ++i) mcenter[i] = h.mcenter[i];
     rebuild();
   }
@@ -489,13 +491,13 @@ class LensHaloParticles {
   void build(const std::vector<T>& x, const std::vector<T>& y, const std::vector<float>& m,
              const std::vector<float>& h, double theta_force) {
     if constexpr (sizeof(T) == sizeof(float))
-      slsim_cpp::check(slsim_plane_create_f32(slsim_cpp::context(), (const float*)x.data(),
-                                              (const float*)y.data(), m.data(), h.data(),
-                                              x.size(), theta_force, 20, inv_area, &plane));
+      slsim_cpp::check(slsim_plane_create_range_f32(
+          slsim_cpp::context(), (const float*)x.data(), (const float*)y.data(), m.data(),
+          h.data(), x.size(), theta_force, 20, inv_area, max_range_, &plane));
     else
-      slsim_cpp::check(slsim_plane_create_f64(slsim_cpp::context(), (const double*)x.data(),
-                                              (const double*)y.data(), m.data(), h.data(),
-                                              x.size(), theta_force, 20, inv_area, &plane));
+      slsim_cpp::check(slsim_plane_create_range_f64(
+          slsim_cpp::context(), (const double*)x.data(), (const double*)y.data(), m.data(),
+          h.data(), x.size(), theta_force, 20, inv_area, max_range_, &plane));
   }
   // rotate_particles, particle_halo.h:634-680
   void rotate_particles(PosType tx, PosType ty) {
@@ -515,6 +517,7 @@ class LensHaloParticles {
   float min_size;
   double mass = 0;
   double theta_force_ = 0.1;
+  double max_range_ = -1;   // the tree's maximum range (particle_halo.h:416)
   double mcenter[3] = {0, 0, 0};
   std::vector<PType> trash_collector;
   slsim_plane plane = nullptr;

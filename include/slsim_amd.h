@@ -75,6 +75,17 @@ int slsim_plane_create_f64(slsim_ctx ctx, const double* x, const double* y, cons
 int slsim_plane_create_f32(slsim_ctx ctx, const float* x, const float* y, const float* m,
                            const float* h, size_t n, double theta, int bucket, double inv_area,
                            slsim_plane* out);
+/* The same with TreeQuadParticles' maximum_range, as the file-reading
+ * LensHaloParticles constructor passes its max_range (include/particle_halo.h:
+ * 416; include/quadTree.h:266, 370-380): with max_range > 0 a cell at centre
+ * c stays a leaf when theta > 1.732 box / max(|c| - max_range, 1.732 box);
+ * max_range <= 0 is the plain build. */
+int slsim_plane_create_range_f64(slsim_ctx ctx, const double* x, const double* y, const float* m,
+                                 const float* h, size_t n, double theta, int bucket,
+                                 double inv_area, double max_range, slsim_plane* out);
+int slsim_plane_create_range_f32(slsim_ctx ctx, const float* x, const float* y, const float* m,
+                                 const float* h, size_t n, double theta, int bucket,
+                                 double inv_area, double max_range, slsim_plane* out);
 int slsim_plane_destroy(slsim_plane plane);
 
 /* LensHaloHalos<LensHaloNFW>(halos, redshift, cosmo, inv_area, theta_force)

@@ -35,6 +35,8 @@ class Counters(ctypes.Structure):
 _sigs = {
     "orc_coordist": (c_d, [c_d, c_d, c_d, c_d, c_d]),
     "orc_tree_build": (c_vp, [P(c_d), P(c_d), P(c_f), P(c_f), c_sz, c_d, c_int, c_d, P(c_int)]),
+    "orc_tree_build_range": (c_vp, [P(c_d), P(c_d), P(c_f), P(c_f), c_sz, c_d, c_int, c_d, c_d,
+                                    P(c_int)]),
     "orc_tree_free": (None, [c_vp]),
     "orc_tree_nnodes": (c_sz, [c_vp]),
     "orc_tree_export": (None, [c_vp, P(c_d), P(ctypes.c_int64)]),
@@ -105,15 +107,16 @@ def _counts(arr):
 class Tree:
     """TreeQuadParticles over (x, y, m, h); positions promote to float64."""
 
-    def __init__(self, x, y, m, h, theta=0.1, bucket=20, inv_area=0.0):
+    def __init__(self, x, y, m, h, theta=0.1, bucket=20, inv_area=0.0, max_range=-1.0):
         self.x = np.ascontiguousarray(x, dtype=np.float64)
         self.y = np.ascontiguousarray(y, dtype=np.float64)
         self.m = np.ascontiguousarray(m, dtype=np.float32)
         self.hh = np.ascontiguousarray(h, dtype=np.float32)
         self.inv_area = inv_area
         err = c_int(0)
-        self.t = _lib.orc_tree_build(_p(self.x), _p(self.y), _p(self.m, c_f), _p(self.hh, c_f),
-                                     self.m.size, inv_area, bucket, theta, ctypes.byref(err))
+        self.t = _lib.orc_tree_build_range(_p(self.x), _p(self.y), _p(self.m, c_f),
+                                           _p(self.hh, c_f), self.m.size, inv_area, bucket,
+                                           theta, max_range, ctypes.byref(err))
         if not self.t:
             raise ValueError("particles in same place.")
         self.owned = True

@@ -149,6 +149,7 @@ struct TreeQuad {
   double inv_area = 0;
   int Nbucket = 20;
   double force_theta = 0.1;
+  double max_range = -1;   // LensHaloParticles' maximum range (particle_halo.h:416)
   std::vector<size_t> index;
   std::vector<Node> nodes;   // nodes[0] = root
   std::vector<double> workspace;
@@ -197,7 +198,7 @@ struct TreeQuad {
     workspace.shrink_to_fit();
   }
 
-  // _BuildQTreeNB, include/quadTree.h:358-565 (max_range = -1, MultiRadius = MultiMass = true)
+  // _BuildQTreeNB, include/quadTree.h:358-565 (MultiRadius = MultiMass = true)
   void build_rec(int ni) {
     Node* cb = &nodes[ni];
     cb->center[0] = (cb->p1[0] + cb->p2[0]) / 2;
@@ -205,6 +206,13 @@ struct TreeQuad {
     cb->quad[0] = cb->quad[1] = cb->quad[2] = 0;
     cb->mass = 0.0;
     double theta_range = 2 * force_theta;
+    if (max_range > 0) {
+      // quadTree.h:370-376: cells far beyond max_range stay unsplit
+      const double boxsize = 1.732 * (cb->p2[0] - cb->p1[0]);
+      const double d =
+          std::sqrt(cb->center[0] * cb->center[0] + cb->center[1] * cb->center[1]) - max_range;
+      theta_range = boxsize / (d > boxsize ? d : boxsize);
+    }
     const double box = cb->p2[0] - cb->p1[0];
     size_t* particles = cb->particles;
     const size_t n = cb->nparticles;
@@ -1240,9 +1248,20 @@ double orc_coordist(double Om, double OL, double h, double z1, double z2) {
 
 // Builds a TreeQuadParticles over n particles. Positions are given as double
 // (float positions promote exactly). Returns null and sets *err on failure.
+void* orc_tree_build_range(const double* x, const double* y, const float* m, const float* h,
+                           size_t n, double inv_area, int bucket, double theta, double max_range,
+                           int* err);
 void* orc_tree_build(const double* x, const double* y, const float* m, const float* h,
                      size_t n, double inv_area, int bucket, double theta, int* err) {
+  return orc_tree_build_range(x, y, m, h, n, inv_area, bucket, theta, -1.0, err);
+}
+
+// the same with TreeQuadParticles' maximum_range (quadTree.h:266, 370-376)
+void* orc_tree_build_range(const double* x, const double* y, const float* m, const float* h,
+                           size_t n, double inv_area, int bucket, double theta, double max_range,
+                           int* err) {
   auto t = new orc::TreeQuad();
+  t->max_range = max_range;
   t->px.assign(x, x + n);
   t->py.assign(y, y + n);
   t->pm.assign(m, m + n);

@@ -96,9 +96,10 @@ class LensHaloParticles:
 
     x, y: positions in physical Mpc (float64 -> ParticleType<double>,
     float32 -> ParticleType<float>); m: masses (Msun, stored float32);
-    h: sizes (Mpc, float32)."""
+    h: sizes (Mpc, float32); max_range > 0: the tree's maximum range
+    (particle_halo.h:58-75, 416), beyond which cells stay unsplit."""
 
-    def __init__(self, ctx, x, y, m, h, z, theta=0.1, bucket=20, inv_area=0.0):
+    def __init__(self, ctx, x, y, m, h, z, theta=0.1, bucket=20, inv_area=0.0, max_range=-1.0):
         self.ctx = ctx
         self.z = np.float32(z)
         self.owned_by_lens = False
@@ -109,16 +110,16 @@ class LensHaloParticles:
         if np.asarray(x).dtype == np.float32:
             xs = np.ascontiguousarray(x, dtype=np.float32)
             ys = np.ascontiguousarray(y, dtype=np.float32)
-            fn = lib.slsim_plane_create_f32
+            fn = lib.slsim_plane_create_range_f32
             ct = ctypes.c_float
         else:
             xs = np.ascontiguousarray(x, dtype=np.float64)
             ys = np.ascontiguousarray(y, dtype=np.float64)
-            fn = lib.slsim_plane_create_f64
+            fn = lib.slsim_plane_create_range_f64
             ct = ctypes.c_double
         _check(fn(ctx.h, _ptr(xs, ct), _ptr(ys, ct), _ptr(m, ctypes.c_float),
                   _ptr(h, ctypes.c_float), ctypes.c_size_t(n), float(theta), int(bucket),
-                  float(inv_area), ctypes.byref(self.h)))
+                  float(inv_area), float(max_range), ctypes.byref(self.h)))
 
     @staticmethod
     def calculate_smoothing(ctx, Nsmooth, x, y, z):

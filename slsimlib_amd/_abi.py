@@ -139,6 +139,12 @@ SIGNATURES = {
                                        c_size_t, c_double, c_int, c_double, P(slsim_plane)]),
     "slsim_plane_create_f32": (c_int, [slsim_ctx, P(c_float), P(c_float), P(c_float), P(c_float),
                                        c_size_t, c_double, c_int, c_double, P(slsim_plane)]),
+    "slsim_plane_create_range_f64": (c_int, [slsim_ctx, P(c_double), P(c_double), P(c_float),
+                                             P(c_float), c_size_t, c_double, c_int, c_double,
+                                             c_double, P(slsim_plane)]),
+    "slsim_plane_create_range_f32": (c_int, [slsim_ctx, P(c_float), P(c_float), P(c_float),
+                                             P(c_float), c_size_t, c_double, c_int, c_double,
+                                             c_double, P(slsim_plane)]),
     "slsim_plane_destroy": (c_int, [slsim_plane]),
     "slsim_plane_get_info": (c_int, [slsim_plane, P(slsim_plane_info)]),
     "slsim_plane_export_nodes": (c_int, [slsim_plane, P(c_double), P(c_i64)]),

@@ -225,7 +225,8 @@ static int plane_free_device(slsim_plane p) {
 
 template <typename T>
 static int plane_create(slsim_ctx ctx, const T* x, const T* y, const float* m, const float* h,
-                        size_t n, double theta, int bucket, double inv_area, slsim_plane* out) {
+                        size_t n, double theta, int bucket, double inv_area, slsim_plane* out,
+                        double max_range = -1) {
   if (!ctx || !out) return arg_error("null handle");
   if (n == 0 || !x || !y || !m || !h) return arg_error("empty particle set");
   if (n >= (size_t)1 << 31) return arg_error("more than 2^31 particles in one plane");
@@ -237,6 +238,7 @@ static int plane_create(slsim_ctx ctx, const T* x, const T* y, const float* m, c
   p->theta = theta;
   p->bucket = bucket;
   p->inv_area = inv_area;
+  p->max_range = max_range;
   int rc = build_plane<T>(ctx, x, y, m, h, n, p.get());
   if (rc) {
     plane_free_device(p.get());
@@ -258,6 +260,18 @@ int slsim_plane_create_f32(slsim_ctx ctx, const float* x, const float* y, const 
                            const float* h, size_t n, double theta, int bucket, double inv_area,
                            slsim_plane* out) {
   return plane_create<float>(ctx, x, y, m, h, n, theta, bucket, inv_area, out);
+}
+
+int slsim_plane_create_range_f64(slsim_ctx ctx, const double* x, const double* y, const float* m,
+                                 const float* h, size_t n, double theta, int bucket,
+                                 double inv_area, double max_range, slsim_plane* out) {
+  return plane_create<double>(ctx, x, y, m, h, n, theta, bucket, inv_area, out, max_range);
+}
+
+int slsim_plane_create_range_f32(slsim_ctx ctx, const float* x, const float* y, const float* m,
+                                 const float* h, size_t n, double theta, int bucket,
+                                 double inv_area, double max_range, slsim_plane* out) {
+  return plane_create<float>(ctx, x, y, m, h, n, theta, bucket, inv_area, out, max_range);
 }
 
 int slsim_plane_tree_create_nfw(slsim_ctx ctx, const double* x, const double* y,

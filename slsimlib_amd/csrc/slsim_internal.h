@@ -275,6 +275,7 @@ struct slsim_plane_s {
   double theta = 0.1, inv_area = 0.0;
   double mass = 0, com[2] = {0, 0};   // total mass and centre of mass (root cell)
   int bucket = 20;
+  double max_range = -1;   // > 0: cells far beyond it are not split
   double build_ms = 0;
   size_t bytes = 0;
   std::vector<int64_t> level_offset;   // BFS level starts (host copy)

@@ -96,11 +96,24 @@ __global__ void k_gather(const uint32_t* __restrict__ perm, const T* __restrict_
   hs[i] = h[p];
 }
 
+// A cell splits when it holds more than `bucket` particles and, with a
+// maximum range, unless it lies so far beyond it that theta exceeds
+// 1.732 box / max(|centre| - max_range, 1.732 box) (include/quadTree.h:370-380;
+// products and sums rounded one by one, as on the host).
 __global__ void k_split_flags(int nlev, const int32_t* __restrict__ cnt, int bucket, int level,
-                              int32_t* __restrict__ flag, int32_t* __restrict__ too_deep) {
+                              const double* __restrict__ p1, const double* __restrict__ p2,
+                              double theta, double max_range, int32_t* __restrict__ flag,
+                              int32_t* __restrict__ too_deep) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nlev) return;
   bool split = cnt[i] > bucket;
+  if (split && max_range > 0) {
+    const double cx = (p1[2 * i] + p2[2 * i]) / 2, cy = (p1[2 * i + 1] + p2[2 * i + 1]) / 2;
+    const double boxsize = __dmul_rn(1.732, p2[2 * i] - p1[2 * i]);
+    const double d = sqrt(__dadd_rn(__dmul_rn(cx, cx), __dmul_rn(cy, cy))) - max_range;
+    const double theta_range = boxsize / (d > boxsize ? d : boxsize);
+    split = !(theta > theta_range);
+  }
   if (split && level >= kMaxLevels) {
     *too_deep = 1;
     split = false;
@@ -556,7 +569,8 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
     Level& L = levels.back();
     int32_t *flag, *offs;
     if ((rc = tmp.get(&flag, L.nlev)) || (rc = tmp.get(&offs, L.nlev))) return rc;
-    k_split_flags<<<nblocks(L.nlev), kBlock, 0, st>>>(L.nlev, L.cnt, pl->bucket, level, flag,
+    k_split_flags<<<nblocks(L.nlev), kBlock, 0, st>>>(L.nlev, L.cnt, pl->bucket, level, L.p1,
+                                                      L.p2, pl->theta, pl->max_range, flag,
                                                       too_deep);
     size_t scan_bytes = 0;
     SLSIM_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, flag, offs, L.nlev, st));

@@ -408,3 +408,35 @@ def test_multi_halo_plane_matches_oracle(ctx):
     assert np.array_equal(g.counts(), cnt)
     g32 = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP32C)
     assert_fields_close(pos, g32.y(), g32.A(), g.y(), g.A(), FP32C_ALPHA_TOL, KG_TOL)
+
+
+def test_tree_max_range_matches_oracle(ctx):
+    """LensHaloParticles' max_range (include/particle_halo.h:58-75, 416;
+    include/quadTree.h:370-380): the device tree's cells equal the oracle's
+    bit for bit (coarse leaves far beyond the range included), for float64
+    and float32 positions, and a lens of it shoots as the oracle's does."""
+    rng = np.random.default_rng(21)
+    r = 0.05 * rng.random(4000) ** 2
+    ph = 2 * np.pi * rng.random(4000)
+    x = np.concatenate([r * np.cos(ph), 1.0 + 0.01 * rng.random(3000)])
+    y = np.concatenate([r * np.sin(ph), 0.01 * rng.random(3000)])
+    m = np.full(x.size, 1e8, np.float32)
+    h = np.full(x.size, 1e-5, np.float32)
+    for R in (0.02, 0.3):
+        dev = S.LensHaloParticles(ctx, x, y, m, h, 0.5, max_range=R)
+        orc = O.Tree(x, y, m, h, max_range=R)
+        _compare_trees(dev, orc)
+        q = orc.export_nodes()[1]
+        assert np.any((q[:, 3] == 1) & (q[:, 1] > 20))       # coarse leaves exist
+        xf, yf = x.astype(np.float32), y.astype(np.float32)
+        _compare_trees(S.LensHaloParticles(ctx, xf, yf, m, h, 0.5, max_range=R),
+                       O.Tree(xf, yf, m, h, max_range=R))
+    lens = S.Lens(ctx, 2.0)
+    lens.moveinMainHalo(S.LensHaloParticles(ctx, x, y, m, h, 0.5, max_range=0.02))
+    L = O.Lens(2.0)
+    L.moveinMainHalo(O.Tree(x, y, m, h, max_range=0.02), 0.5)
+    N, rg = 48, W.grid_range(60.0)
+    g = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    yr, Ar = L.shoot(O.gridmap_positions(N, (0.0, 0.0), rg))
+    assert np.abs(g.y() - yr).max() <= 1e-12 * np.abs(yr).max()
+    assert np.abs(g.A() - Ar).max() <= 1e-8

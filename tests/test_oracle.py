@@ -418,3 +418,56 @@ def test_reset_source_plane_choice():
     for k in range(4):
         assert O.reset_source_plane(tab, tab["z"][k], False)[0] == k
         assert O.reset_source_plane(tab, tab["z"][k], True) == (k, tab["z"][k])
+
+
+def _two_clusters(seed):
+    """A cluster at the origin and a dense one near (1, 0): the far one lies
+    beyond a small maximum range by many times its own cell's size."""
+    rng = np.random.default_rng(seed)
+    x0, y0 = _nfw_like(4000, seed, scale=0.05)
+    x1 = 1.0 + 0.01 * rng.random(3000)
+    y1 = 0.01 * rng.random(3000)
+    x = np.concatenate([x0, x1])
+    y = np.concatenate([y0, y1])
+    m = np.full(x.size, 1e8, np.float32)
+    h = np.full(x.size, 1e-5, np.float32)
+    return x, y, m, h
+
+
+def test_tree_max_range():
+    """TreeQuadParticles' maximum_range (include/quadTree.h:370-380): a cell
+    with more than bucket particles stays a leaf only when it lies beyond
+    max_range by at least 1.732 box / theta; far beyond every cell the tree is
+    the plain one; the walk over the coarse leaves still gives the direct sum
+    to the opening angle's accuracy."""
+    x, y, m, h = _two_clusters(21)
+    T0 = O.Tree(x, y, m, h)
+    Tfar = O.Tree(x, y, m, h, max_range=1e3)
+    d0, q0 = T0.export_nodes()
+    d1, q1 = Tfar.export_nodes()
+    assert np.array_equal(d0, d1) and np.array_equal(q0, q1)
+
+    R = 0.02
+    T = O.Tree(x, y, m, h, max_range=R)
+    d, q = T.export_nodes()
+    assert T.nnodes() < T0.nnodes()
+    leaves = q[:, 3] == 1
+    coarse = leaves & (q[:, 1] > 20)
+    assert coarse.any() and q[coarse, 1].sum() >= 2500
+    rc = np.hypot((d[:, 0] + d[:, 2]) / 2, (d[:, 1] + d[:, 3]) / 2)
+    far = (rc - R) >= (1.732 / 0.1) * (d[:, 2] - d[:, 0]) * (1 - 1e-9)
+    assert np.all(far[coarse])                     # coarse leaves only far out
+    assert not np.any(leaves & (q[:, 1] > 20) & ~far)   # nearer ones all split
+    kids = _children(q)
+    for i, ks in enumerate(kids):
+        if ks:
+            assert q[i, 1] > 20 and not far[i]
+    assert q[leaves, 1].sum() == x.size
+
+    rng = np.random.default_rng(3)
+    rays = np.concatenate([(rng.random((150, 2)) - 0.5) * 0.1,
+                           [1.0, 0.0] + rng.random((150, 2)) * 0.012])
+    dd = O.direct_sum(x, y, m, h, rays)
+    scale = np.abs(dd).max(0)
+    for t in (T0, T):
+        assert np.all(np.abs(t.force(rays) - dd).max(0) <= 2e-3 * scale)
