@@ -348,6 +348,22 @@ class Lens:
                 hh.h = None           # released by the lens
         self.halos = keep
 
+    def moveinMainHalos(self, halos, addplanes=True):
+        """Lens::insertMainHalos (include/lens.h:219-237) for halos the lens
+        takes (their device planes move in)."""
+        for hh in halos:
+            self.moveinMainHalo(hh, addplanes)
+
+    def replaceMainHalos(self, halos):
+        """Lens::replaceMainHalos (include/lens.h:244-263): the halos replace
+        the main halos, one plane per redshift."""
+        self.clearMainHalos()
+        self.moveinMainHalos(halos, True)
+
+    def getNMainHalos(self):
+        """Lens::getNMainHalos (include/lens.h:303, 887-891)."""
+        return sum(1 for hh in self.halos if not isinstance(hh, LensPlaneTree))
+
     def addFieldPlane(self, plane):
         """A LensPlaneTree as a field plane at its (double) redshift; the
         lens sorts it with the main planes (Lens::combinePlanes)."""

@@ -398,3 +398,26 @@ def test_nearest_plane_insertion_matches_oracle(ctx):
     yr, Ar = E.shoot(pos)
     assert np.abs(g1.y() - yr).max() <= 1e-12 * np.abs(yr).max()
     assert np.abs(g1.A() - Ar).max() <= 1e-8
+
+
+@pytest.mark.gpu
+def test_replace_main_halos_equals_a_fresh_lens(ctx):
+    """Lens::replaceMainHalos (include/lens.h:244-263): after it the lens
+    shoots as a new lens holding only the new halos."""
+    n = 4096
+    px, py = S.gen_nfw(80, n, 1e13, 8.0, 0.5)
+    qx, qy = S.gen_nfw(81, n, 4e12, 6.0, 0.4)
+    pm = np.full(n, np.float32(1e13 / n), np.float32)
+    ph = np.full(n, np.float32(2e-4), np.float32)
+    lens = S.Lens(ctx, 2.0)
+    lens.moveinMainHalos([S.LensHaloParticles(ctx, px, py, pm, ph, 0.5)])
+    lens.replaceMainHalos([S.LensHaloParticles(ctx, qx, qy, pm, ph, 0.4),
+                           S.LensHaloParticles(ctx, px, py, pm, ph, 0.7)])
+    assert lens.getNMainHalos() == 2 and lens.tables()["nplanes"] == 2
+    fresh = S.Lens(ctx, 2.0)
+    fresh.moveinMainHalo(S.LensHaloParticles(ctx, qx, qy, pm, ph, 0.4))
+    fresh.moveinMainHalo(S.LensHaloParticles(ctx, px, py, pm, ph, 0.7))
+    N, rg = 32, W.grid_range(40.0)
+    a = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    b = S.GridMap(fresh, N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    assert np.array_equal(a.y(), b.y()) and np.array_equal(a.A(), b.A())
