@@ -17,11 +17,13 @@
 // compiles against this header and runs the rays on the GPU.  Link with
 // libslsim_amd.so.  Reference behaviours kept on purpose are marked
 // "(reference)".  Deliberate differences: a non-flat cosmology throws
-// std::invalid_argument instead of exit(1); moveinMainHalo always adds the
-// halo as addMainHaloToPlane does; LensHaloParticles takes an optional
-// trailing opening angle (the reference fixes 0.1).
+// std::invalid_argument instead of exit(1); LensHaloParticles takes an
+// optional trailing opening angle (the reference fixes 0.1); a copied
+// GridMap links its own points (the reference's copy links into the
+// original's).
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <list>
@@ -334,6 +336,21 @@ class LensHaloParticles {
       : zlens(redshift), inv_area(my_inv_area), min_size(MinPSize), max_range_(max_range) {
     (void)cosmo; (void)verbose;
     std::swap(pvector, trash_collector);
+    pp = trash_collector.data();
+    Npoints = trash_collector.size();
+    set_up(theta_rotate, recenter, theta_force);
+  }
+  // particle_halo.h:78-93: does not take possession of the particles.  As in
+  // the reference, recentring and rotation change the caller's array; the
+  // device tree is built from it here, so the caller may free it afterwards
+  // unless it reads particles back through operator[].
+  LensHaloParticles(PType* begin, size_t n, float redshift, const COSMOLOGY& cosmo,
+                    Point_2d theta_rotate, bool recenter, double my_inv_area,
+                    float MinPSize = 0, double max_range = -1, bool verbose = false,
+                    double theta_force = 0.1)
+      : zlens(redshift), inv_area(my_inv_area), min_size(MinPSize), max_range_(max_range),
+        pp(begin), Npoints(n) {
+    (void)cosmo; (void)verbose;
     set_up(theta_rotate, recenter, theta_force);
   }
   // particle_halo.h:226-340: particles from an ASCII file ("# nparticles N"
@@ -373,6 +390,8 @@ class LensHaloParticles {
       x[i] = xyz[3 * i];
       y[i] = xyz[3 * i + 1];
     }
+    pp = trash_collector.data();
+    Npoints = n;
     mass = info.mass;
     for (int k = 0; k < 3; ++k) mcenter[k] = info.mcenter[k];
     build<T>(x, y, m, h, theta_force_);
@@ -381,25 +400,48 @@ class LensHaloParticles {
       : zlens(h.zlens), inv_area(h.inv_area), min_size(h.min_size), mass(h.mass),
         theta_force_(h.theta_force_), max_range_(h.max_range_),
         trash_collector(std::move(h.trash_collector)),
-        plane(h.plane) {
+        pp(h.owns() ? trash_collector.data() : h.pp), Npoints(h.Npoints), plane(h.plane) {
     for (int i = 0; i < 3; ++i) mcenter[i] = h.mcenter[i];
+    for (int i = 0; i < 3; ++i) densest_point[i] = h.densest_point[i];
+    h.pp = nullptr;
+    h.Npoints = 0;
     h.plane = nullptr;
   }
-  // a copy holds its own device tree, built from the same particles
+  // a copy holds its own device tree, built from the same particles (its own
+  // copy of them when the original owns them, else the same borrowed array)
   LensHaloParticles(const LensHaloParticles& h)
       : zlens(h.zlens), inv_area(h.inv_area), min_size(h.min_size), mass(h.mass),
         theta_force_(h.theta_force_), max_range_(h.max_range_),
-        trash_collector(h.trash_collector) {
+        trash_collector(h.trash_collector),
+        pp(h.owns() ? trash_collector.data() : h.pp), Npoints(h.Npoints) {
     for (int i = 0; i < 3; ++i) mcenter[i] = h.mcenter[i];
+    for (int i = 0; i < 3; ++i) densest_point[i] = h.densest_point[i];
     rebuild();
   }
   ~LensHaloParticles() {
     if (plane) slsim_plane_destroy(plane);
   }
-  size_t getN() const { return trash_collector.size(); }
+  size_t getN() const { return Npoints; }
   float getZlens() const { return zlens; }
   double get_mass() const { return mass; }
-  PType& operator[](size_t i) { return trash_collector[i]; }
+  PType& operator[](size_t i) { return pp[i]; }
+  // particle_halo.h:107-110: the centre of mass and the smallest particle's
+  // position in input coordinates (after recentring and rotation)
+  Point_3d<> CenterOfMass() { return Point_3d<>(mcenter[0], mcenter[1], mcenter[2]); }
+  Point_3d<> DensestPoint() {
+    return Point_3d<>(densest_point[0], densest_point[1], densest_point[2]);
+  }
+  // particle_halo.h:436-443: rotates the particles (the caller's array when
+  // borrowed) and rebuilds the tree with the reference's defaults there
+  // (opening angle 0.1, no maximum range)
+  void rotate(Point_2d theta) {
+    rotate_particles(theta[0], theta[1]);
+    if (plane) slsim_plane_destroy(plane);
+    plane = nullptr;
+    theta_force_ = 0.1;
+    max_range_ = -1;
+    rebuild();
+  }
   slsim_plane release_plane() {
     slsim_plane p = plane;
     plane = nullptr;
@@ -432,35 +474,38 @@ class LensHaloParticles {
  protected:
   friend class MakeParticleLenses;
   // particle_halo.h:194-207, the constructor MakeParticleLenses uses: the
-  // particles are copied (the reference shares them; set_up changes them only
+  // particles are borrowed, as in the reference (set_up changes them only
   // when recentring or rotating, which CreateHalos does not ask for).
   LensHaloParticles(PType* pdata, size_t Nparticles, float redshift, const COSMOLOGY& cosmo,
                     Point_2d theta_rotate, bool recenter, double my_inv_area, bool verbose = false)
-      : zlens(redshift), inv_area(my_inv_area), min_size(0) {
+      : zlens(redshift), inv_area(my_inv_area), min_size(0), pp(pdata), Npoints(Nparticles) {
     (void)cosmo; (void)verbose;
-    trash_collector.assign(pdata, pdata + Nparticles);
     set_up(theta_rotate, recenter, 0.1);
   }
 
  private:
+  bool owns() const { return !trash_collector.empty(); }
   void rebuild() {
-    const size_t n = trash_collector.size();
+    const size_t n = Npoints;
     using T = typename std::remove_reference<decltype(std::declval<PType&>()[0])>::type;
     std::vector<T> x(n), y(n);
     std::vector<float> m(n), h(n);
     for (size_t i = 0; i < n; ++i) {
-      x[i] = trash_collector[i][0];
-      y[i] = trash_collector[i][1];
-      m[i] = trash_collector[i].mass();
-      h[i] = trash_collector[i].size();
+      x[i] = pp[i][0];
+      y[i] = pp[i][1];
+      m[i] = pp[i].mass();
+      h[i] = pp[i].size();
     }
     build<T>(x, y, m, h, theta_force_);
   }
   // set_up, particle_halo.h:341-417
   void set_up(Point_2d theta_rotate, bool recenter, double theta_force) {
     theta_force_ = theta_force;
-    std::vector<PType>& pp = trash_collector;
-    const size_t n = pp.size();
+    const size_t n = Npoints;
+    size_t smallest = 0;
+    for (size_t i = 0; i < n; ++i)
+      if (pp[i].Size < pp[smallest].Size) smallest = i;
+    for (int j = 0; j < 3; ++j) densest_point[j] = n ? pp[smallest][j] : 0.0;
     mcenter[0] = mcenter[1] = mcenter[2] = 0;
     mass = 0;
     for (size_t i = 0; i < n; ++i) {
@@ -473,19 +518,12 @@ class LensHaloParticles {
       // particle moves (particle_halo.h:395-411)
       for (size_t i = 0; i < n; ++i) {
         for (int j = 0; j < 3; ++j) pp[i][j] -= mcenter[j];
+        for (int j = 0; j < 3; ++j) densest_point[j] -= mcenter[j];
         mcenter[0] = mcenter[1] = mcenter[2] = 0;
       }
     }
     rotate_particles(theta_rotate[0], theta_rotate[1]);
-    std::vector<float> m(n), h(n);
-    for (size_t i = 0; i < n; ++i) {
-      m[i] = pp[i].mass();
-      h[i] = pp[i].size();
-    }
-    using T = typename std::remove_reference<decltype(pp[0][0])>::type;
-    std::vector<T> x(n), y(n);
-    for (size_t i = 0; i < n; ++i) { x[i] = pp[i][0]; y[i] = pp[i][1]; }
-    build<T>(x, y, m, h, theta_force);
+    rebuild();
   }
   template <typename T>
   void build(const std::vector<T>& x, const std::vector<T>& y, const std::vector<float>& m,
@@ -504,11 +542,18 @@ class LensHaloParticles {
     if (tx == 0.0 && ty == 0.0) return;
     const double cx = std::cos(tx), sx = std::sin(tx), cy = std::cos(ty), sy = std::sin(ty);
     const double c[3][3] = {{cy, 0, -sy}, {-sy * sx, cx, -cy * sx}, {cx, sx, cy * cx}};
-    for (auto& p : trash_collector) {
+    for (size_t i = 0; i < Npoints; ++i) {
+      PType& p = pp[i];
       double t[3] = {0, 0, 0};
       for (int j = 0; j < 3; ++j)
         for (int r = 0; r < 3; ++r) t[r] += c[r][j] * p[j];
       for (int j = 0; j < 3; ++j) p[j] = t[j];
+    }
+    for (double* v : {mcenter, densest_point}) {
+      double t[3] = {0, 0, 0};
+      for (int j = 0; j < 3; ++j)
+        for (int r = 0; r < 3; ++r) t[r] += c[r][j] * v[j];
+      for (int j = 0; j < 3; ++j) v[j] = t[j];
     }
   }
 
@@ -519,7 +564,10 @@ class LensHaloParticles {
   double theta_force_ = 0.1;
   double max_range_ = -1;   // the tree's maximum range (particle_halo.h:416)
   double mcenter[3] = {0, 0, 0};
-  std::vector<PType> trash_collector;
+  double densest_point[3] = {0, 0, 0};
+  std::vector<PType> trash_collector;   // the particles when the halo owns them
+  PType* pp = nullptr;                  // the particles (owned or borrowed)
+  size_t Npoints = 0;
   slsim_plane plane = nullptr;
 };
 
@@ -1028,12 +1076,27 @@ class MakeParticleLenses {
 template <typename T>
 class PixelMap {
  public:
+  PixelMap() : Nx_(0), Ny_(0), res(0) { c[0] = c[1] = 0; }
+  // pixelmap.cpp:111-130: square pixels of `resolution`, the map's lower-left
+  // boundary at center - (N resolution) / 2
   PixelMap(const double center[2], size_t Nx, size_t Ny, double resolution)
       : Nx_(Nx), Ny_(Ny), res(resolution), map(Nx * Ny) {
     c[0] = center[0];
     c[1] = center[1];
   }
   T& operator[](size_t i) { return map[i]; }
+  const T& operator[](size_t i) const { return map[i]; }
+  // pixelmap.cpp: every pixel set to zero
+  void Clean() { std::fill(map.begin(), map.end(), T(0)); }
+  // pixelmap.cpp:2227-2242, 2266-2269: the pixel holding x, -1 outside
+  long find_index(const double x[]) const {
+    const long ix = (long)((x[0] - (c[0] - (Nx_ * res) / 2.)) / res);
+    const long iy = (long)((x[1] - (c[1] - (Ny_ * res) / 2.)) / res);
+    if (ix < 0 || ix >= (long)Nx_) return -1;
+    if (iy < 0 || iy >= (long)Ny_) return -1;
+    return ix + (long)Nx_ * iy;
+  }
+  Point_2d getCenter() const { return Point_2d(c[0], c[1]); }
   size_t size() const { return map.size(); }
   size_t getNx() const { return Nx_; }
   size_t getNy() const { return Ny_; }
@@ -1175,9 +1238,83 @@ struct GridMap {
     }
   }
 
-  GridMap(const GridMap&) = delete;
+  // gridmap.cpp:122-144: a grid without lensing (y = x, A = I); its outputs
+  // are put in HBM too, so that the device consumers accept it
+  GridMap(unsigned long N1d, const double center_[2], double range)
+      : Ngrid_init(N1d), Ngrid_init2(N1d), x_range(range) {
+    if (N1d < 1) throw std::runtime_error("GridMap size is < 1!");
+    if (range <= 0) throw std::runtime_error("GridMap range is <= 0!");
+    center[0] = center_[0];
+    center[1] = center_[1];
+    const size_t n = (size_t)N1d * N1d;
+    i_points.resize(n);
+    s_points.resize(n);
+    i_points[0].head = n;
+    s_points[0].head = n;
+    for (size_t i = 0; i < n; ++i) {
+      for (Point* p : {&i_points[i], &s_points[i]}) {
+        p->id = i;
+        p->gridsize = range / (N1d - 1);
+        position_from_index(i, p->x);
+      }
+      i_points[i].image = &s_points[i];
+      s_points[i].image = &i_points[i];
+    }
+    std::lock_guard<std::mutex> hold(grid_mutex());
+    upload();
+  }
+
+  // gridmap.h:139-180: moving keeps the points (and their links) and the
+  // device outputs; a copy relinks its own points and puts its own outputs
+  // in HBM (the reference's copy keeps links into the original's points)
+  GridMap(GridMap&& grid) noexcept
+      : Ngrid_init(grid.Ngrid_init), Ngrid_init2(grid.Ngrid_init2), x_range(grid.x_range),
+        y_range(grid.y_range), axisratio(grid.axisratio), rect(grid.rect),
+        i_points(std::move(grid.i_points)), s_points(std::move(grid.s_points)), yd(grid.yd),
+        Ad(grid.Ad) {
+    center[0] = grid.center[0];
+    center[1] = grid.center[1];
+    grid.yd = grid.Ad = nullptr;
+  }
+  GridMap(GridMap& grid)
+      : Ngrid_init(grid.Ngrid_init), Ngrid_init2(grid.Ngrid_init2), x_range(grid.x_range),
+        y_range(grid.y_range), axisratio(grid.axisratio), rect(grid.rect),
+        i_points(grid.i_points), s_points(grid.s_points) {
+    center[0] = grid.center[0];
+    center[1] = grid.center[1];
+    for (size_t i = 0; i < i_points.size(); ++i) {
+      i_points[i].image = &s_points[i];
+      s_points[i].image = &i_points[i];
+    }
+    std::lock_guard<std::mutex> hold(grid_mutex());
+    upload();
+  }
+  GridMap& operator=(GridMap&& grid) noexcept {
+    if (this == &grid) return *this;
+    release_device();
+    Ngrid_init = grid.Ngrid_init;
+    Ngrid_init2 = grid.Ngrid_init2;
+    x_range = grid.x_range;
+    y_range = grid.y_range;
+    axisratio = grid.axisratio;
+    rect = grid.rect;
+    center[0] = grid.center[0];
+    center[1] = grid.center[1];
+    std::swap(i_points, grid.i_points);
+    std::swap(s_points, grid.s_points);
+    std::swap(yd, grid.yd);
+    std::swap(Ad, grid.Ad);
+    return *this;
+  }
   GridMap& operator=(const GridMap&) = delete;
   ~GridMap() { release_device(); }
+
+  // gridmap.cpp:148-160: the rays shot again through `lens` (for example
+  // after its source plane changed), as the reference does it: a new grid
+  // made by the rectangular constructor with this grid's size and ranges
+  GridMap ReInitialize(LensHndl lens) {
+    return GridMap(lens, Ngrid_init, center, x_range, getYRange());
+  }
 
   // include/gridmap.h:182-205
   struct Triangle {
@@ -1273,7 +1410,6 @@ struct GridMap {
   // gridmap.cpp:148-163: every ray unlensed (A = I, y = x), also in HBM
   void deLens() {
     const size_t n = (size_t)Ngrid_init * Ngrid_init2;
-    std::vector<double> y(2 * n), A(4 * n);
     for (size_t i = 0; i < n; ++i) {
       i_points[i].A.setToI();
       s_points[i].A.setToI();
@@ -1281,15 +1417,8 @@ struct GridMap {
       s_points[i].dt = 0;
       s_points[i].x[0] = i_points[i].x[0];
       s_points[i].x[1] = i_points[i].x[1];
-      y[2 * i] = i_points[i].x[0];
-      y[2 * i + 1] = i_points[i].x[1];
-      A[4 * i] = A[4 * i + 3] = 1.0;
     }
-    if (yd && n) {
-      slsim_ctx ctx = slsim_cpp::context();
-      slsim_cpp::check(slsim_memcpy_h2d(ctx, yd, y.data(), 16 * n));
-      slsim_cpp::check(slsim_memcpy_h2d(ctx, Ad, A.data(), 32 * n));
-    }
+    upload();
   }
 
   size_t getNumberOfPoints() const { return (size_t)Ngrid_init * Ngrid_init2; }
@@ -1301,40 +1430,77 @@ struct GridMap {
   Point_2d image_point(size_t i) { return Point_2d(i_points[i].x[0], i_points[i].x[1]); }
   Point_2d source_point(size_t i) { return Point_2d(s_points[i].x[0], s_points[i].x[1]); }
   RAY ray(size_t i) { return RAY(i_points[i]); }
+  Point_2d getCenter() const { return Point_2d(center[0], center[1]); }
 
   // gridmap.h:533-603
   template <typename T>
   PixelMap<T> writePixelMap(LensingVariable lensvar) {
     PixelMap<T> map(center, Ngrid_init, Ngrid_init2, x_range / (Ngrid_init - 1));
     const size_t N = map.size();
-    for (size_t i = 0; i < N; ++i) {
-      const Point& p = i_points[i];
-      const double a0 = p.x[0] - p.image->x[0], a1 = p.x[1] - p.image->x[1];
-      switch (lensvar) {
-        case LensingVariable::ALPHA: map[i] = std::sqrt(a0 * a0 + a1 * a1); break;
-        case LensingVariable::ALPHA1: map[i] = a0; break;
-        case LensingVariable::ALPHA2: map[i] = a1; break;
-        case LensingVariable::KAPPA: map[i] = p.kappa(); break;
-        case LensingVariable::GAMMA:
-          map[i] = std::sqrt(p.gamma1() * p.gamma1() + p.gamma2() * p.gamma2());
-          break;
-        case LensingVariable::GAMMA1: map[i] = p.gamma1(); break;
-        case LensingVariable::GAMMA2: map[i] = p.gamma2(); break;
-        case LensingVariable::GAMMA3: map[i] = p.gamma3(); break;
-        case LensingVariable::INVMAG: map[i] = p.invmag(); break;
-        case LensingVariable::DELAYT: map[i] = p.dt; break;
-        case LensingVariable::SurfBrightness: map[i] = p.surface_brightness; break;
-        default:
-          throw std::runtime_error(
-              "GridMap::writePixelMapUniform() does not work for the input LensingVariable");
-      }
-    }
+    for (size_t i = 0; i < N; ++i)
+      map[i] = point_value(i_points[i], lensvar, "GridMap::writePixelMapUniform()");
     return map;
   }
 
   // gridmap.h:524-531
   template <typename T>
   void writeFits(LensingVariable lensvar, std::string filename) {
+    PixelMap<T> map = writePixelMap<T>(lensvar);
+    map.printFITS(filename);
+  }
+
+  // gridmap.h:98-106, 506-524: an Nx x Ny map of pixels x_range / (Nx - 1)
+  // wide around `center`, each grid point's value put in the pixel holding
+  // the point (points outside the map dropped; where two points share a
+  // pixel, the later in index order is kept -- the reference's threads keep
+  // either)
+  template <typename T>
+  PixelMap<T> writePixelMapUniform(const PosType center_[], size_t Nx, size_t Ny,
+                                   LensingVariable lensvar) {
+    if (getNumberOfPoints() == 0) return PixelMap<T>();
+    PixelMap<T> map(center_, Nx, Ny, x_range / (Nx - 1));
+    map.Clean();
+    writePixelMapUniform(map, lensvar);
+    return map;
+  }
+  // gridmap.h:605-628
+  template <typename T>
+  void writePixelMapUniform(PixelMap<T>& map, LensingVariable lensvar) {
+    if (getNumberOfPoints() == 0) return;
+    map.Clean();
+    const size_t n = getNumberOfPoints();
+    for (size_t i = 0; i < n; ++i) {
+      const double v = point_value(i_points[i], lensvar, "PixelMap<T>::AddGrid()");
+      const long index = map.find_index(i_points[i].x);
+      if (index != -1) map[index] = v;
+    }
+  }
+  // gridmap.h:691-740: the uniform map written to filename + a tag naming
+  // the quantity (".kappa.fits", ...)
+  template <typename T>
+  void writeFitsUniform(const PosType center_[], size_t Nx, size_t Ny, LensingVariable lensvar,
+                        std::string filename) {
+    std::string tag;
+    switch (lensvar) {
+      case LensingVariable::DELAYT: tag = ".dt.fits"; break;
+      case LensingVariable::ALPHA1: tag = ".alpha1.fits"; break;
+      case LensingVariable::ALPHA2: tag = ".alpha2.fits"; break;
+      case LensingVariable::ALPHA: tag = ".alpha.fits"; break;
+      case LensingVariable::KAPPA: tag = ".kappa.fits"; break;
+      case LensingVariable::GAMMA1: tag = ".gamma1.fits"; break;
+      case LensingVariable::GAMMA2: tag = ".gamma2.fits"; break;
+      case LensingVariable::GAMMA3: tag = ".gamma3.fits"; break;
+      case LensingVariable::GAMMA: tag = ".gamma.fits"; break;
+      case LensingVariable::INVMAG: tag = ".invmag.fits"; break;
+      case LensingVariable::SurfBrightness: tag = ".surfbright.fits"; break;
+      default: break;
+    }
+    PixelMap<T> map = writePixelMapUniform<T>(center_, Nx, Ny, lensvar);
+    map.printFITS(filename + tag);
+  }
+  // gridmap.h:108-116: the grid as it is (writePixelMap), as FITS
+  template <typename T>
+  void writeFitsUniform(LensingVariable lensvar, std::string filename) {
     PixelMap<T> map = writePixelMap<T>(lensvar);
     map.printFITS(filename);
   }
@@ -1350,6 +1516,56 @@ struct GridMap {
     if (yd) slsim_device_free(ctx, yd);
     if (Ad) slsim_device_free(ctx, Ad);
     yd = Ad = nullptr;
+  }
+  // the points' y and A into HBM (allocated on first use)
+  void upload() {
+    const size_t n = (size_t)Ngrid_init * Ngrid_init2;
+    if (n == 0) return;
+    std::vector<double> y(2 * n), A(4 * n);
+    for (size_t i = 0; i < n; ++i) {
+      y[2 * i] = s_points[i].x[0];
+      y[2 * i + 1] = s_points[i].x[1];
+      for (int c = 0; c < 4; ++c) A[4 * i + c] = i_points[i].A.a[c];
+    }
+    slsim_ctx ctx = slsim_cpp::context();
+    int rc = SLSIM_OK;
+    if (!yd) rc = slsim_device_alloc(ctx, 16 * n, &yd);
+    if (rc == SLSIM_OK && !Ad) rc = slsim_device_alloc(ctx, 32 * n, &Ad);
+    if (rc == SLSIM_OK) rc = slsim_memcpy_h2d(ctx, yd, y.data(), 16 * n);
+    if (rc == SLSIM_OK) rc = slsim_memcpy_h2d(ctx, Ad, A.data(), 32 * n);
+    if (rc != SLSIM_OK) release_device();
+    slsim_cpp::check(rc);
+  }
+  // Utilities::PositionFromIndex (utilities.cpp:149-158) on this grid
+  void position_from_index(size_t i, double* x) const {
+    if (Ngrid_init == 1) {
+      x[0] = center[0];
+      x[1] = center[1];
+      return;
+    }
+    x[0] = center[0] + x_range * (1.0 * (i % Ngrid_init) / (Ngrid_init - 1) - 0.5);
+    x[1] = center[1] + x_range * (1.0 * (i / Ngrid_init) / (Ngrid_init - 1) - 0.5);
+  }
+  // one point's lensing quantity (gridmap.h:542-600, 640-681)
+  static double point_value(const Point& p, LensingVariable lensvar, const char* who) {
+    const double a0 = p.x[0] - p.image->x[0], a1 = p.x[1] - p.image->x[1];
+    switch (lensvar) {
+      case LensingVariable::ALPHA: return std::sqrt(a0 * a0 + a1 * a1);
+      case LensingVariable::ALPHA1: return a0;
+      case LensingVariable::ALPHA2: return a1;
+      case LensingVariable::KAPPA: return p.kappa();
+      case LensingVariable::GAMMA:
+        return std::sqrt(p.gamma1() * p.gamma1() + p.gamma2() * p.gamma2());
+      case LensingVariable::GAMMA1: return p.gamma1();
+      case LensingVariable::GAMMA2: return p.gamma2();
+      case LensingVariable::GAMMA3: return p.gamma3();
+      case LensingVariable::INVMAG: return p.invmag();
+      case LensingVariable::DELAYT: return p.dt;
+      case LensingVariable::SurfBrightness: return p.surface_brightness;
+      default:
+        throw std::runtime_error(std::string(who) +
+                                 " does not work for the input LensingVariable");
+    }
   }
   void images(std::vector<Point_2d>& ys, std::vector<int>& multiplicity, std::vector<double>& r,
               std::vector<int64_t>& t) const {

@@ -348,3 +348,76 @@ def fits_value(text):
         return int(v)
     except ValueError:
         return float(v)
+
+
+def _rotation(tx, ty):
+    """The matrix of LensHaloParticles::rotate_particles
+    (include/particle_halo.h:634-648): rows (cy, 0, -sy), (-sy sx, cx, -cy sx),
+    (cx, sx, cy cx)."""
+    cx, sx, cy, sy = np.cos(tx), np.sin(tx), np.cos(ty), np.sin(ty)
+    return np.array([[cy, 0, -sy], [-sy * sx, cx, -cy * sx], [cx, sx, cy * cx]])
+
+
+def _rotate(c, v):
+    """t[r] = sum over j of c[r][j] v[j], j ascending (particle_halo.h:653-661)."""
+    v = np.asarray(v, np.float64)
+    t = np.zeros(v.shape)
+    for j in range(3):
+        for r in range(3):
+            t[..., r] += c[r, j] * v[..., j]
+    return t
+
+
+def particle_setup(xyz, mass, size, recenter=False, theta_rotate=(0.0, 0.0)):
+    """LensHaloParticles::set_up (include/particle_halo.h:341-417) on the
+    particles of the vector and pointer constructors (:58-93): the densest
+    point is the first particle of smallest Size; the centre of mass sums
+    x * mass() (position type times float32, in double) and the mass in
+    double, particle order; recentring subtracts the centre inside the
+    particle loop after which the centre is zeroed, so only the first particle
+    moves (reference); then the particles, the centre and the densest point
+    are rotated (:634-680).  Returns (xyz, mcenter, densest_point, mass)."""
+    T = xyz.dtype.type
+    xyz = xyz.copy()
+    smallest = 0
+    for i in range(1, len(size)):
+        if size[i] < size[smallest]:
+            smallest = i
+    dp = xyz[smallest].astype(np.float64)
+    prod = xyz.astype(np.float64) * mass.astype(np.float64)[:, None]
+    if T is np.float32:
+        prod = (xyz * mass[:, None]).astype(np.float64)
+    mc = np.array([np.cumsum(prod[:, k])[-1] for k in range(3)])
+    mtot = np.cumsum(mass.astype(np.float64))[-1]
+    mc = mc / mtot
+    if recenter and len(xyz):
+        xyz[0] = (xyz[0].astype(np.float64) - mc).astype(T)
+        dp = dp - mc
+        mc = np.zeros(3)
+    tx, ty = theta_rotate
+    if not (tx == 0.0 and ty == 0.0):
+        c = _rotation(tx, ty)
+        xyz = _rotate(c, xyz).astype(T)
+        mc = _rotate(c, mc)
+        dp = _rotate(c, dp)
+    return xyz, mc, dp, mtot
+
+
+def pixelmap_uniform(x, values, center, nx, ny, resolution):
+    """GridMap::writePixelMapUniform (include/gridmap.h:506-524, 605-681) into
+    a PixelMap(center, nx, ny, resolution) (ImageProcessing/pixelmap.cpp:
+    111-130): the map is cleaned, then each grid point's value goes to the
+    pixel PixelMap::find_index gives for its image position
+    (pixelmap.cpp:2227-2242: (long)((x - p1) / resolution) per axis,
+    p1 = center - (N resolution) / 2, -1 outside).  Where points share a
+    pixel the reference's threads keep any one; here the last in index order.
+    Returns the map (ny, nx)."""
+    out = np.zeros(nx * ny)
+    p1x = center[0] - (nx * resolution) / 2.
+    p1y = center[1] - (ny * resolution) / 2.
+    for i in range(x.shape[0]):
+        ix = int((x[i, 0] - p1x) / resolution)
+        iy = int((x[i, 1] - p1y) / resolution)
+        if 0 <= ix < nx and 0 <= iy < ny:
+            out[ix + nx * iy] = values[i]
+    return out.reshape(ny, nx)

@@ -67,7 +67,7 @@ def build_cpp_example(verbose=False):
     the library."""
     root = os.path.dirname(HERE)
     exes = []
-    for name in ("dropin_gridmap", "dropin_formats", "dropin_halos"):
+    for name in ("dropin_gridmap", "dropin_formats", "dropin_halos", "dropin_boundary"):
         src = os.path.join(root, "tests", "cpp", name + ".cpp")
         exe = os.path.join(root, "tests", "cpp", name)
         cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "include"), src,

@@ -367,3 +367,56 @@ def test_fits_pixelmap(tmp_path, dtype, flip, ctype):
         assert v["CTYPE1"] == "RA---TAN" and v["RADESYS"] == "ICRS"
     with pytest.raises(S.SlsimError):
         S.write_fits_pixelmap(fn, img, res, center, units=9)
+
+
+def test_particle_setup_oracle():
+    """LensHaloParticles::set_up's restatement: the centre of mass and
+    densest point of a hand case, the recentring quirk (only the first
+    particle moves), and the rotation: its x and y rows are those of the
+    rotation R_x(tx) R_y(-ty) (scipy), its z row keeps the reference's cos(tx)
+    where a rotation has cos(tx) sin(ty)."""
+    from scipy.spatial.transform import Rotation
+    xyz = np.array([[1.0, 2.0, 3.0], [-1.0, 0.5, 0.0], [4.0, -2.0, 1.0]])
+    m = np.array([1.0, 2.0, 1.0], np.float32)
+    h = np.array([0.3, 0.1, 0.1], np.float32)
+    q, mc, dp, mt = F.particle_setup(xyz, m, h)
+    assert mt == 4.0
+    assert np.allclose(mc, [(1 - 2 + 4) / 4, (2 + 1 - 2) / 4, (3 + 0 + 1) / 4], rtol=0, atol=1e-15)
+    assert np.array_equal(dp, xyz[1]) and np.array_equal(q, xyz)
+    q, mc2, dp2, _ = F.particle_setup(xyz, m, h, recenter=True)
+    assert np.allclose(q[0], xyz[0] - mc) and np.array_equal(q[1:], xyz[1:])
+    assert np.array_equal(mc2, np.zeros(3)) and np.allclose(dp2, xyz[1] - mc)
+    tx, ty = 0.3, -0.2
+    rng = np.random.default_rng(5)
+    p = rng.normal(size=(50, 3))
+    q, _, _, _ = F.particle_setup(p, np.ones(50, np.float32), np.ones(50, np.float32),
+                                  theta_rotate=(tx, ty))
+    R = (Rotation.from_euler("x", tx) * Rotation.from_euler("y", -ty)).as_matrix()
+    assert np.allclose(q[:, :2], p @ R[:2].T, rtol=0, atol=1e-14)
+    e = np.eye(3)
+    qz, _, _, _ = F.particle_setup(e, np.ones(3, np.float32), np.ones(3, np.float32),
+                                   theta_rotate=(tx, ty))
+    assert np.allclose(qz[:, 2], [np.cos(tx), np.sin(tx), np.cos(ty) * np.cos(tx)])
+    assert not np.isclose(qz[0, 2], R[2, 0])
+
+
+def test_pixelmap_uniform_oracle():
+    """writePixelMapUniform's restatement: on the grid's own geometry (N x N
+    pixels of range / (N - 1) around the grid centre) pixel i holds point i;
+    on a map of twice the pixel size each pixel keeps the last of its four
+    points; points outside the map are dropped."""
+    N, rg, c = 8, 3.0, (0.25, -0.5)
+    i = np.arange(N * N)
+    x = np.stack([c[0] + rg * ((i % N) / (N - 1) - 0.5), c[1] + rg * ((i // N) / (N - 1) - 0.5)], 1)
+    vals = np.arange(N * N, dtype=np.float64) + 1
+    assert np.array_equal(F.pixelmap_uniform(x, vals, c, N, N, rg / (N - 1)).reshape(-1), vals)
+    res = 2 * rg / (N - 1)
+    got = F.pixelmap_uniform(x, vals, c, N // 2, N // 2, res)
+    for py in range(N // 2):
+        for px in range(N // 2):
+            pts = [(2 * py + b) * N + 2 * px + a for b in (0, 1) for a in (0, 1)]
+            assert got[py, px] == vals[max(pts)]
+    small = F.pixelmap_uniform(x, vals, c, 2, 2, rg / (N - 1))
+    inside = [k for k in range(N * N) if abs(x[k, 0] - c[0]) < rg / (N - 1)
+              and abs(x[k, 1] - c[1]) < rg / (N - 1)]
+    assert len(inside) == 4 and sorted(small.reshape(-1)) == sorted(vals[inside])
