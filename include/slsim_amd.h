@@ -37,7 +37,13 @@ enum {
 /* Arithmetic of the tree walk.  The plane-to-plane recursion is FP64 in both. */
 enum {
   SLSIM_FP64 = 0,             /* plain method: FP64 walk, reference arithmetic    */
-  SLSIM_FP32C = 1,            /* FP32 walk with Kahan-compensated accumulation    */
+  SLSIM_FP32C = 1,            /* FP32 walk with Kahan-compensated accumulation,
+                                 guarded: rays whose result could differ from the
+                                 FP64 walk's by a flipped opening test (an FP64
+                                 test within the FP32 ray's error bound of its
+                                 threshold, on a cell whose flip matters) and
+                                 rays with |alpha| below 1% of the call's rms
+                                 are shot again in FP64                           */
   SLSIM_DIRECT64 = 2          /* FP64 direct sum over every particle of a plane, no
                                  tree: the error-column oracle (O(N) per ray)      */
 };
@@ -153,7 +159,11 @@ int slsim_plane_get_info(slsim_plane plane, slsim_plane_info* info);
 /*
  * Exports the node set in depth-first pre-order (child0 first, the order of
  * walkTree_recur, include/quadTree.h:1079-1086).  Per node:
- *   dbl[13 i + 0..12] = p1x, p1y, p2x, p2y, cx, cy, mass, q0, q1, q2, r2crit, 0, 0
+ *   dbl[13 i + 0..12] = p1x, p1y, p2x, p2y, cx, cy, mass, q0, q1, q2, r2crit, flip, 0
+ * flip: the cell's flip error (float): the largest change of the raw tree
+ * deflection when the cell's opening test is decided the other way, at
+ * the distance where it flips (its multipole against the exact particle
+ * sum); the precision form's guard uses it.
  *   ints[6 i + 0..5]  = level, nparticles, nbig, is_leaf, 0, 0
  * nbig counts the reference's big_particles list of the node.
  */
@@ -207,7 +217,8 @@ typedef struct {
   double walk_ms;           /* sum of walk-kernel times (HIP events)         */
   double walk_ms_max_plane; /* slowest single walk launch                    */
   int32_t n_walk_launches;
-  int32_t pad;
+  int32_t n_guarded;        /* SLSIM_FP32C: rays the precision guard shot
+                               again in FP64 (see SLSIM_FP32C)              */
 } slsim_shoot_stats;
 
 /*

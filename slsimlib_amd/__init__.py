@@ -424,6 +424,22 @@ class Lens:
             stats.update({k: getattr(st, k) for k, _ in st._fields_ if k != "pad"})
         return y, A
 
+    def shoot_into(self, x, y, A, zs=None, precision=SLSIM_FP64):
+        """rayshooterInternal into caller-owned host arrays: x (n, 2) in,
+        y (n, 2) and A (n, 4) out, all C-contiguous float64."""
+        n = x.shape[0]
+        for a, k in ((x, 2), (y, 2), (A, 4)):
+            if a.dtype != np.float64 or not a.flags.c_contiguous or a.shape != (n, k):
+                raise ValueError("x, y, A must be C-contiguous float64 (n,2), (n,2), (n,4)")
+        zp = None
+        if zs is not None:
+            if zs.dtype != np.float64 or not zs.flags.c_contiguous or zs.size != n:
+                raise ValueError("zs must be C-contiguous float64 of n values")
+            zp = _ptr(zs, ctypes.c_double)
+        _check(lib.slsim_shoot(self.h, ctypes.c_size_t(n), _ptr(x, ctypes.c_double), zp,
+                               _ptr(y, ctypes.c_double), _ptr(A, ctypes.c_double),
+                               int(precision), None))
+
     def close(self):
         if self.h:
             lib.slsim_lens_destroy(self.h)

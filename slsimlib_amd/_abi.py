@@ -75,7 +75,7 @@ class slsim_plane_info(ctypes.Structure):
 class slsim_shoot_stats(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("walk_ms", ctypes.c_double),
                 ("walk_ms_max_plane", ctypes.c_double), ("n_walk_launches", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("n_guarded", ctypes.c_int32)]
 
 
 class slsim_schedule(ctypes.Structure):

@@ -148,6 +148,9 @@ TreeDesc slsim_plane_s::desc() const {
   t.mass = mass;
   t.comx = com[0];
   t.comy = com[1];
+  t.dipx = dip[0];
+  t.dipy = dip[1];
+  t.sheetd = sheetd;
   t.nnodes = (int32_t)nnodes;
   t.nparts = (int32_t)n;
   t.kind = kind;
@@ -217,7 +220,8 @@ int slsim_ctx_synchronize(slsim_ctx ctx) {
 static int plane_free_device(slsim_plane p) {
   slsim_ctx c = p->ctx;
   void* ptrs[] = {p->nodesD, p->nodesF, p->partsD, p->bigD, p->box_p1, p->box_p2,
-                  p->nbig,   p->npart,  p->hnodes, p->hbig,  p->halos,  p->nfw};
+                  p->nbig,   p->npart,  p->hnodes, p->hbig,  p->halos,  p->nfw,
+                  p->sheetd};
   for (void* q : ptrs)
     if (q) c->release(q);
   return SLSIM_OK;
@@ -699,11 +703,10 @@ void uniform_source(const slsim_lens_s* L, double zs, int32_t* jmax, double* aal
   }
 }
 
-// Per-plane step descriptors for one ray job.
-void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint32_t* counters,
-                std::vector<StepJob>& steps) {
+// Per-plane step descriptors for one ray job (one per lens plane).
+void make_steps_into(const slsim_lens_s* L, const RayState& st, int64_t nslots,
+                     uint32_t* counters, StepJob* steps) {
   const int nplanes = (int)L->plane_redshifts.size() - 1;
-  steps.resize(nplanes);
   for (int j = 0; j < nplanes; ++j) {
     StepJob& s = steps[j];
     std::memset(&s, 0, sizeof(s));
@@ -721,7 +724,19 @@ void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint3
     s.bb = L->Dl[j] / L->Dl[j + 1];
     s.charge = L->charge;
     s.counters = counters;
+    double ascale = 0;
+    for (slsim_plane p : L->combined[j])
+      if (p->kind == kTreeParticles && p->p2[0] > p->p1[0])
+        ascale += std::fabs(p->mass) / (kPI * 0.5 * (p->p2[0] - p->p1[0]));
+    s.ascale = ascale;
+    s.eps = kGuardEps;
   }
+}
+
+void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint32_t* counters,
+                std::vector<StepJob>& steps) {
+  steps.resize(L->plane_redshifts.size() - 1);
+  make_steps_into(L, st, nslots, counters, steps.data());
 }
 
 }  // namespace slsim
@@ -729,11 +744,11 @@ void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint3
 namespace {
 
 // Runs the plane steps [0, jmax) of one job on the context stream.
-int run_steps(slsim_ctx c, const StepJob* djobs, const std::vector<StepJob>& steps, int jmax,
+int run_steps(slsim_ctx c, const StepJob* djobs, const StepJob* steps, int jmax,
               uint32_t nbundles, int precision, bool count, uint32_t* err,
               slsim_shoot_stats* stats) {
-  std::vector<hipEvent_t> ev(2 * (size_t)jmax);
   const bool timed = stats != nullptr;
+  std::vector<hipEvent_t> ev(timed ? 2 * (size_t)jmax : 0);
   if (timed)
     for (auto& e : ev) SLSIM_HIP_CHECK(hipEventCreate(&e));
   for (int j = 0; j < jmax; ++j) {
@@ -768,6 +783,160 @@ int check_err(slsim_ctx c, uint32_t* err) {
     set_error("traversal stack overflow");
     return SLSIM_ERR_TOO_DEEP;
   }
+  return SLSIM_OK;
+}
+
+template <typename ScratchT>
+int guard_refine(slsim_lens L, const RayState& st, int64_t nslots, const GuardMap& m,
+                 int64_t nrays, const double* zs_dev, double* y_dev, double* A_dev, ScratchT& s,
+                 int32_t* nguard);
+
+// rayshooterInternal on device buffers (FullRange/internal_rayshooter_multi.cpp:
+// 53-251): rays Morton-ordered into 64-ray bundles, one launch per plane, the
+// results scattered back to input order.  Device scratch comes from `s`
+// (the context's cache, or the fixed array of the no-allocation entry
+// point); the lens's trees must be resident.  Host buffers are fixed-size
+// arrays, so a call allocates nothing beyond `s`.
+template <typename ScratchT>
+int shoot_array(slsim_lens L, size_t n, const double* x_dev, const double* zs_dev,
+                double* y_dev, double* A_dev, int precision, slsim_shoot_stats* stats,
+                ScratchT& s, int lanes = kWave) {
+  const double t0 = now_ms();
+  if (precision < SLSIM_FP64 || precision > SLSIM_DIRECT64) return arg_error("bad precision");
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (n == 0) return SLSIM_OK;
+  if (!x_dev || !y_dev || !A_dev) return arg_error("null buffer");
+  if (n >= (size_t)1 << 32) return arg_error("too many rays in one call");
+  if (L->trees_dirty) return arg_error("lens trees are not on the device");
+  slsim_ctx c = L->ctx;
+  const int nplanes = (int)L->plane_redshifts.size() - 1;
+  if (nplanes > kMaxPlanesFixed && !ScratchT::kMayAllocate)
+    return arg_error("too many lens planes for the no-allocation entry point");
+  if (lanes < 1 || lanes > kWave) return arg_error("bad rays per bundle");
+  const int64_t nbundles = ((int64_t)n + lanes - 1) / lanes;
+  const int64_t nslots = nbundles * kWave;
+  int rc;
+  RayState st;
+  {
+    char* d;
+    if ((rc = s.get(&d, kStateBytesPerSlot * (size_t)nslots))) return rc;
+    st = state_view(reinterpret_cast<double*>(d), nslots);
+  }
+  uint32_t* err;
+  if ((rc = s.get(&err, 1))) return rc;
+  SLSIM_HIP_CHECK(hipMemsetAsync(err, 0, 4, c->stream));
+  // Morton order of the rays: each wavefront gets a compact bundle
+  const int nblk = (int)std::min<int64_t>(kMinmaxBlocks, (n + 255) / 256);
+  double* part;
+  if ((rc = s.get(&part, 4 * (size_t)nblk))) return rc;
+  launch_minmax_rays(c->stream, x_dev, (int64_t)n, part, nblk);
+  double hp[4 * kMinmaxBlocks];
+  SLSIM_HIP_CHECK(hipMemcpyAsync(hp, part, 32 * (size_t)nblk, hipMemcpyDeviceToHost, c->stream));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(c->stream));
+  double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+  for (int b = 0; b < nblk; ++b) {
+    lo[0] = std::min(lo[0], hp[4 * b]); lo[1] = std::min(lo[1], hp[4 * b + 1]);
+    hi[0] = std::max(hi[0], hp[4 * b + 2]); hi[1] = std::max(hi[1], hp[4 * b + 3]);
+  }
+  const double w = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), 1e-300);
+  uint64_t *code, *code_s;
+  uint32_t *idx, *perm;
+  if ((rc = s.get(&code, n)) || (rc = s.get(&code_s, n)) || (rc = s.get(&idx, n)) ||
+      (rc = s.get(&perm, n)))
+    return rc;
+  launch_ray_morton(c->stream, x_dev, (int64_t)n, lo[0], lo[1], 1.0 / w, code, idx);
+  size_t sb = 0;
+  SLSIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sb, code, code_s, idx, perm, (int)n,
+                                                     0, 30, c->stream));
+  char* stmp;
+  if ((rc = s.get(&stmp, sb))) return rc;
+  SLSIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(stmp, sb, code, code_s, idx, perm, (int)n, 0,
+                                                     30, c->stream));
+  int32_t jmax = 0;
+  double aal = 0, bbl = 0;
+  if (nplanes > 0 && !zs_dev) uniform_source(L, 0.0, &jmax, &aal, &bbl);
+  double *zpl, *dl;
+  if ((rc = s.get(&zpl, L->plane_redshifts.size())) || (rc = s.get(&dl, L->Dl.size()))) return rc;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(zpl, L->plane_redshifts.data(), 8 * L->plane_redshifts.size(),
+                                 hipMemcpyHostToDevice, c->stream));
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dl, L->Dl.data(), 8 * L->Dl.size(), hipMemcpyHostToDevice, c->stream));
+  FlatLCDM cosmo{L->Om, L->OL, L->h};
+  launch_init_array(c->stream, st, nslots, (int64_t)n, perm, x_dev, nplanes > 0 ? zs_dev : nullptr,
+                    jmax, aal, bbl, zpl, dl, nplanes, cosmo, lanes);
+  const int steps_to_run = zs_dev ? nplanes : jmax;
+  if (steps_to_run > 0) {
+    StepJob steps_fixed[kMaxPlanesFixed];
+    std::vector<StepJob> steps_more;
+    StepJob* steps = steps_fixed;
+    if (nplanes > kMaxPlanesFixed) {
+      steps_more.resize(nplanes);
+      steps = steps_more.data();
+    }
+    make_steps_into(L, st, nslots, nullptr, steps);
+    StepJob* djobs;
+    if ((rc = s.get(&djobs, (size_t)nplanes))) return rc;
+    SLSIM_HIP_CHECK(hipMemcpyAsync(djobs, steps, sizeof(StepJob) * (size_t)nplanes,
+                                   hipMemcpyHostToDevice, c->stream));
+    rc = run_steps(c, djobs, steps, steps_to_run, (uint32_t)nbundles, precision, false, err,
+                   stats);
+    if (rc) return rc;
+  }
+  launch_finalize_array(c->stream, st, nslots, (int64_t)n, perm, y_dev, A_dev, lanes);
+  SLSIM_HIP_CHECK(hipGetLastError());
+  if ((rc = check_err(c, err))) return rc;
+  if (precision == SLSIM_FP32C && steps_to_run > 0 && lanes == kWave) {
+    GuardMap m{};
+    m.perm = perm;
+    m.x = x_dev;
+    m.zs = zs_dev;
+    m.n = (int64_t)n;
+    int32_t ng = 0;
+    if ((rc = guard_refine(L, st, nslots, m, (int64_t)n, zs_dev, y_dev, A_dev, s, &ng))) return rc;
+    if (stats) stats->n_guarded = ng;
+  }
+  if (stats) stats->total_ms = now_ms() - t0;
+  return SLSIM_OK;
+}
+
+// The precision form's guard (SLSIM_FP32C): after the FP32 walk's shoot,
+// the rays the walk flagged and the rays with |alpha| below kGuardSmallAlpha
+// of the call's rms are shot again in FP64 (a ray's result does not depend
+// on its bundle mates, so these equal the FP64 path's) and their outputs
+// replaced.
+template <typename ScratchT>
+int guard_refine(slsim_lens L, const RayState& st, int64_t nslots, const GuardMap& m,
+                 int64_t nrays, const double* zs_dev, double* y_dev, double* A_dev, ScratchT& s,
+                 int32_t* nguard) {
+  slsim_ctx c = L->ctx;
+  int rc;
+  double *acc, *part;
+  uint32_t* count;
+  if ((rc = s.get(&acc, 1)) || (rc = s.get(&count, 1)) || (rc = s.get(&part, kGuardBlocks)))
+    return rc;
+  SLSIM_HIP_CHECK(hipMemsetAsync(acc, 0, 8, c->stream));
+  SLSIM_HIP_CHECK(hipMemsetAsync(count, 0, 4, c->stream));
+  launch_guard_sumsq(c->stream, st, nslots, m, part, acc);
+  const uint32_t cap = (uint32_t)nrays;
+  uint32_t* idx;
+  double *xl, *zl = nullptr;
+  if ((rc = s.get(&idx, cap)) || (rc = s.get(&xl, 2 * (size_t)cap))) return rc;
+  if (zs_dev && (rc = s.get(&zl, cap))) return rc;
+  launch_guard_flag(c->stream, st, nslots, m, nrays, acc, count, cap, idx, xl, zl);
+  uint32_t h = 0;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(&h, count, 4, hipMemcpyDeviceToHost, c->stream));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (nguard) *nguard = (int32_t)h;
+  if (h == 0) return SLSIM_OK;
+  double *yt, *At;
+  if ((rc = s.get(&yt, 2 * (size_t)h)) || (rc = s.get(&At, 4 * (size_t)h))) return rc;
+  // few scattered rays: fewer per bundle, so that each wavefront walks what
+  // its few rays need (a full bundle of scattered rays walks their union)
+  int lanes = 1;
+  while (lanes < kWave && (int64_t)h > (int64_t)lanes * kGuardWaves) lanes *= 2;
+  if ((rc = shoot_array(L, h, xl, zl, yt, At, SLSIM_FP64, nullptr, s, lanes))) return rc;
+  launch_guard_scatter(c->stream, h, idx, yt, At, y_dev, A_dev);
+  SLSIM_HIP_CHECK(hipGetLastError());
+  SLSIM_HIP_CHECK(hipStreamSynchronize(c->stream));
   return SLSIM_OK;
 }
 
@@ -809,12 +978,28 @@ int gridmap_run(slsim_lens L, size_t N, double c0, double c1, double range, doub
     if ((rc = s.get(&djobs, steps.size()))) return rc;
     SLSIM_HIP_CHECK(hipMemcpyAsync(djobs, steps.data(), sizeof(StepJob) * steps.size(),
                                    hipMemcpyHostToDevice, c->stream));
-    rc = run_steps(c, djobs, steps, jmax, (uint32_t)nbundles, cnt_host ? SLSIM_FP64 : precision,
+    rc = run_steps(c, djobs, steps.data(), jmax, (uint32_t)nbundles,
+                   cnt_host ? SLSIM_FP64 : precision,
                    cnt_host != nullptr, err, stats);
     if (rc) return rc;
   }
   if (y_dev && A_dev)
     launch_finalize_grid(c->stream, st, nslots, (int64_t)N, tiles_x, 3, y_dev, A_dev);
+  if (y_dev && A_dev && precision == SLSIM_FP32C && !cnt_host && jmax > 0) {
+    if ((rc = check_err(c, err))) return rc;
+    GuardMap m{};
+    m.N = (int64_t)N;
+    m.tiles_x = tiles_x;
+    m.tw = 3;
+    m.c0 = c0;
+    m.c1 = c1;
+    m.range = range;
+    int32_t ng = 0;
+    if ((rc = guard_refine(L, st, nslots, m, (int64_t)N * (int64_t)N, nullptr, y_dev, A_dev, s,
+                           &ng)))
+      return rc;
+    if (stats) stats->n_guarded = ng;
+  }
   if (cnt_host) {
     uint64_t* dcnt;
     if ((rc = s.get(&dcnt, 5 * N * N))) return rc;
@@ -843,6 +1028,15 @@ int gridmap_run(slsim_lens L, size_t N, double c0, double c1, double range, doub
 }
 
 }  // namespace
+
+namespace slsim {
+int guard_refine_ctx(slsim_lens L, const RayState& st, int64_t nslots, const GuardMap& m,
+                     int64_t nrays, const double* zs_dev, double* y_dev, double* A_dev,
+                     int32_t* nguard) {
+  Scratch s(L->ctx);
+  return guard_refine(L, st, nslots, m, nrays, zs_dev, y_dev, A_dev, s, nguard);
+}
+}  // namespace slsim
 
 extern "C" {
 
@@ -1049,81 +1243,12 @@ int slsim_gridmap_bundle_stats(slsim_lens L, size_t N, double c0, double c1, dou
 
 int slsim_shoot_device(slsim_lens L, size_t n, const double* x_dev, const double* zs_dev,
                        double* y_dev, double* A_dev, int precision, slsim_shoot_stats* stats) {
-  const double t0 = now_ms();
   if (!L) return arg_error("null lens");
-  if (precision < SLSIM_FP64 || precision > SLSIM_DIRECT64) return arg_error("bad precision");
-  if (stats) std::memset(stats, 0, sizeof(*stats));
-  if (n == 0) return SLSIM_OK;
-  if (!x_dev || !y_dev || !A_dev) return arg_error("null buffer");
-  if (n >= (size_t)1 << 32) return arg_error("too many rays in one call");
-  slsim_ctx c = L->ctx;
-  SLSIM_HIP_CHECK(hipSetDevice(c->device));
+  SLSIM_HIP_CHECK(hipSetDevice(L->ctx->device));
   int rc = L->upload_trees();
   if (rc) return rc;
-  const int nplanes = (int)L->plane_redshifts.size() - 1;
-  const int64_t nbundles = ((int64_t)n + kWave - 1) / kWave;
-  const int64_t nslots = nbundles * kWave;
-  Scratch s(c);
-  RayState st;
-  if ((rc = alloc_state(s, nslots, &st))) return rc;
-  uint32_t* err;
-  if ((rc = s.get(&err, 1))) return rc;
-  SLSIM_HIP_CHECK(hipMemsetAsync(err, 0, 4, c->stream));
-  // Morton order of the rays: each wavefront gets a compact bundle
-  const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
-  double* part;
-  if ((rc = s.get(&part, 4 * (size_t)nblk))) return rc;
-  launch_minmax_rays(c->stream, x_dev, (int64_t)n, part, nblk);
-  std::vector<double> hp(4 * (size_t)nblk);
-  SLSIM_HIP_CHECK(hipMemcpyAsync(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost, c->stream));
-  SLSIM_HIP_CHECK(hipStreamSynchronize(c->stream));
-  double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
-  for (int b = 0; b < nblk; ++b) {
-    lo[0] = std::min(lo[0], hp[4 * b]); lo[1] = std::min(lo[1], hp[4 * b + 1]);
-    hi[0] = std::max(hi[0], hp[4 * b + 2]); hi[1] = std::max(hi[1], hp[4 * b + 3]);
-  }
-  const double w = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), 1e-300);
-  uint64_t *code, *code_s;
-  uint32_t *idx, *perm;
-  if ((rc = s.get(&code, n)) || (rc = s.get(&code_s, n)) || (rc = s.get(&idx, n)) ||
-      (rc = s.get(&perm, n)))
-    return rc;
-  launch_ray_morton(c->stream, x_dev, (int64_t)n, lo[0], lo[1], 1.0 / w, code, idx);
-  size_t sb = 0;
-  SLSIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sb, code, code_s, idx, perm, (int)n,
-                                                     0, 30, c->stream));
-  char* stmp;
-  if ((rc = s.get(&stmp, sb))) return rc;
-  SLSIM_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(stmp, sb, code, code_s, idx, perm, (int)n, 0,
-                                                     30, c->stream));
-  int32_t jmax = 0;
-  double aal = 0, bbl = 0;
-  if (nplanes > 0 && !zs_dev) uniform_source(L, 0.0, &jmax, &aal, &bbl);
-  double *zpl, *dl;
-  if ((rc = s.get(&zpl, L->plane_redshifts.size())) || (rc = s.get(&dl, L->Dl.size()))) return rc;
-  SLSIM_HIP_CHECK(hipMemcpyAsync(zpl, L->plane_redshifts.data(), 8 * L->plane_redshifts.size(),
-                                 hipMemcpyHostToDevice, c->stream));
-  SLSIM_HIP_CHECK(hipMemcpyAsync(dl, L->Dl.data(), 8 * L->Dl.size(), hipMemcpyHostToDevice, c->stream));
-  FlatLCDM cosmo{L->Om, L->OL, L->h};
-  launch_init_array(c->stream, st, nslots, (int64_t)n, perm, x_dev, nplanes > 0 ? zs_dev : nullptr,
-                    jmax, aal, bbl, zpl, dl, nplanes, cosmo);
-  const int steps_to_run = zs_dev ? nplanes : jmax;
-  if (steps_to_run > 0) {
-    std::vector<StepJob> steps;
-    make_steps(L, st, nslots, nullptr, steps);
-    StepJob* djobs;
-    if ((rc = s.get(&djobs, steps.size()))) return rc;
-    SLSIM_HIP_CHECK(hipMemcpyAsync(djobs, steps.data(), sizeof(StepJob) * steps.size(),
-                                   hipMemcpyHostToDevice, c->stream));
-    rc = run_steps(c, djobs, steps, steps_to_run, (uint32_t)nbundles, precision, false, err,
-                   stats);
-    if (rc) return rc;
-  }
-  launch_finalize_array(c->stream, st, (int64_t)n, perm, y_dev, A_dev);
-  SLSIM_HIP_CHECK(hipGetLastError());
-  if ((rc = check_err(c, err))) return rc;
-  if (stats) stats->total_ms = now_ms() - t0;
-  return SLSIM_OK;
+  Scratch s(L->ctx);
+  return shoot_array(L, n, x_dev, zs_dev, y_dev, A_dev, precision, stats, s);
 }
 
 int slsim_shoot(slsim_lens L, size_t n, const double* x, const double* zs, double* y, double* A,

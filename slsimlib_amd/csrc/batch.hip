@@ -275,11 +275,32 @@ int slsim_batch_s::run(slsim_shoot_stats* stats) {
   SLSIM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  // the precision form's guard, request by request (as a lone GridMap's)
+  int32_t guarded = 0;
+  if (precision == SLSIM_FP32C) {
+    for (size_t r = 0; r < reqs.size(); ++r) {
+      const GridReq& g = greq[r];
+      if (g.jmax <= 0) continue;
+      GuardMap m{};
+      m.N = g.N;
+      m.tiles_x = g.tiles_x;
+      m.tw = g.tw;
+      m.c0 = g.c0;
+      m.c1 = g.c1;
+      m.range = g.range;
+      int32_t ng = 0;
+      if ((rc = guard_refine_ctx(reqs[r]->lens, g.st, g.nslots, m, g.N * g.N, nullptr, g.y, g.A,
+                                 &ng)))
+        return rc;
+      guarded += ng;
+    }
+  }
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->total_ms = 1e3 * (now_s() - t0);
     stats->walk_ms = ms;
     stats->n_walk_launches = max_steps;
+    stats->n_guarded = guarded;
   }
   return SLSIM_OK;
 }
@@ -668,7 +689,7 @@ void slsim_server_s::loop() {
     for (Req* r : active) {
       if (rc || r->step >= r->jmax) {
         if (rc == SLSIM_OK && r->nrays > 0)
-          launch_finalize_array(stream, r->st, r->nrays, nullptr, r->y, r->A);
+          launch_finalize_array(stream, r->st, r->nslots, r->nrays, nullptr, r->y, r->A);
         else if (rc == SLSIM_OK)
           launch_finalize_grid(stream, r->st, r->nslots, r->N, r->tiles_x, 3, r->y, r->A);
         finished.push_back(r);
@@ -678,6 +699,26 @@ void slsim_server_s::loop() {
     uint32_t herr = 0;
     if (rc == SLSIM_OK && hipMemcpy(&herr, d_err.p, 4, hipMemcpyDeviceToHost) == hipSuccess && herr)
       rc = SLSIM_ERR_TOO_DEEP;
+    // the precision form's guard of each finished request, as a lone shoot's
+    for (Req* r : finished) {
+      if (rc || r->precision != SLSIM_FP32C || r->jmax <= 0) continue;
+      GuardMap m{};
+      if (r->nrays > 0) {
+        m.x = r->xd;
+        m.zs = r->zsd;
+        m.n = r->nrays;
+      } else {
+        m.N = r->N;
+        m.tiles_x = r->tiles_x;
+        m.tw = 3;
+        m.c0 = r->c0;
+        m.c1 = r->c1;
+        m.range = r->range;
+      }
+      int32_t ng = 0;
+      rc = guard_refine_ctx(r->lens, r->st, r->nslots, m, r->nrays > 0 ? r->nrays : r->N * r->N,
+                            r->nrays > 0 ? r->zsd : nullptr, r->y, r->A, &ng);
+    }
     const double t = now_s();
     lk.lock();
     ++stats.iterations;

@@ -685,6 +685,26 @@ __device__ void near_terms64(const TreeDesc* __restrict__ trees, double rx, doub
   a.nq = 0;
 }
 
+// The guard of the precision form: the FP32 ray can be up to dx (physical,
+// this ray's bound) from where the FP64 ray is, so an opening test whose r^2
+// lies within 2 sqrt(T) dx of its threshold T (dx << sqrt(T)) may go the
+// other way in FP64 and change the deflection by the cell's flip error.
+// With D = (2 dx)^2 such tests are (r^2 - T)^2 < T D: the lanes that pass add
+// the cell's flip error to their risk.  A near test (r^2 < 5.83 box^2,
+// gating big particles' B-spline corrections) that is marginal where a big
+// particle lies within 2h of the ray adds an infinite risk (near_marginal).
+__device__ __forceinline__ void guard_open(cptr<NodeF> nd, double r2, double T, uint64_t lk,
+                                           double D, float& risk) {
+  const double d = r2 - T;
+  const uint64_t mk = lanes_lt(d * d, T * D) & lk;
+  if (mk) risk += lane_in(mk) ? fabsf(nd->flip) : 0.f;
+}
+
+// |r^2 - near2| within the band, from the FP32 r^2 (its rounding allowed for)
+__device__ inline bool near_marginal(float r2f, float nef, double D) {
+  return fabsf(r2f - nef) < sqrtf(nef * (float)D) + nef * (1.0f / 4194304.0f);
+}
+
 // The FP32 sums are updated with the same instructions on every path: a
 // lane that takes no part in a term gets zero coefficients (selected, never
 // formed from an infinite 1/r^2), and the rare near-field branches only
@@ -711,8 +731,10 @@ __device__ __forceinline__ void far32(const NodeF& nd, bool far, float ex, float
 }
 
 // Particles of an opened leaf; particle - ray offsets formed in FP64.
+template <bool GUARD>
 __device__ __forceinline__ void leaf32(int start, int count, bool open, bool nearc, double rx,
-                                       double ry, cptr<PartD> parts, Acc32& a) {
+                                       double ry, cptr<PartD> parts, Acc32& a, float r2f,
+                                       float nef, double D, float& risk) {
 #pragma unroll 2
   for (int k = 0; k < count; ++k) {
     const PartD p = parts[start + k];
@@ -723,6 +745,8 @@ __device__ __forceinline__ void leaf32(int start, int count, bool open, bool nea
     const float inv = open ? rcp32(rr) : 0.f;       // 0: no term for this lane
     float s = p.m * inv, w = s * inv;              // point mass
     if (open && rr < 4.f * hs * hs) {
+      // a big particle within 2h: its correction hangs on the near test
+      if (GUARD && !signbit(p.h) && near_marginal(r2f, nef, D)) risk = __builtin_inff();
       if (nearc && !signbit(p.h)) {
         // smoothed: the B-spline correction folded into the point mass
         float Mr2, sigpi;
@@ -773,9 +797,10 @@ __device__ __forceinline__ void bigs32(int start, int count, bool nearc, double 
 // accumulation.  One loop serves every tree: when the stack is empty the next
 // tree starts with its root as the only child of a virtual entry (the cell
 // arrays carry three zero cells of padding for the loads of its siblings).
+template <bool GUARD>
 __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx, double ry,
                        bool active, Acc32& a, int32_t* s_node, uint64_t* s_mask,
-                       uint64_t* s_near, uint32_t* err) {
+                       uint64_t* s_near, uint32_t* err, double D, float& risk) {
   const int lane = lane_id();
   a.q = s_near;
   a.nq = 0;
@@ -785,6 +810,7 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
   cptr<NodeF> nodes = as_const(as_const(trees)->nodesF);
   cptr<PartD> parts = as_const(as_const(trees)->partsD);
   cptr<PartD> bigs = as_const(as_const(trees)->bigD);
+  cptr<float2> sheetd = as_const(as_const(trees)->sheetd);
   a.tree = 0;
   int entry = 1 << 28;            // base 0, child mask {0}: the root
   uint64_t live = amask;
@@ -800,15 +826,16 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       const double cx = nodes[base + k].cx, cy = nodes[base + k].cy;
-      const double rc = nodes[base + k].r2crit, ne = nodes[base + k].near2;
+      const double T = nodes[base + k].T, ne = nodes[base + k].near2;
       const double dx = cx - rx, dy = cy - ry;
       const double r2 = dx * dx + dy * dy;
       const uint64_t lk = ((cm >> k) & 1) ? live : 0ull;
       nm[k] = lanes_lt(r2, ne) & lk;
-      om[k] = (lanes_lt(r2, rc) | nm[k]) & lk;
+      om[k] = lanes_lt(r2, T) & lk;   // r^2 < r2crit or r^2 < near2
       exk[k] = (float)dx;
       eyk[k] = (float)dy;
       r2k[k] = (float)r2;
+      if (GUARD && lk) guard_open(nodes + base + k, r2, T, lk, D, risk);
     }
     int32_t pe[4];
     int leaves = 0, lists = 0;    // children (bits) with particles or a big list to take
@@ -818,13 +845,28 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
       if ((cm >> k) & 1) {
         const NodeF nd = nodes[base + k];
         const uint64_t fm = live & ~om[k];
-        if (fm) far32(nd, lane_in(fm), exk[k], eyk[k], r2k[k], a);
+        if (fm) {
+          far32(nd, lane_in(fm), exk[k], eyk[k], r2k[k], a);
+          if (sheetd) {
+            // masses of both signs: the far cell's share of the mass sheet
+            const float2 dl = sheetd[base + k];
+            const float sel = lane_in(fm) ? 1.f : 0.f;
+            a.pax = fmaf(sel, dl.x, a.pax);
+            a.pay = fmaf(sel, dl.y, a.pay);
+          }
+        }
         if (om[k]) {
           if (nd.child < 0) {
             leaves |= 1 << k;
           } else {
             pe[k] = nd.child | (nd.count_cmask & 0xf0000000);
             if ((nd.count_cmask & 0x0fffffff) > 0 && nm[k]) lists |= 1 << k;
+            if (GUARD && (nd.count_cmask & 0x0fffffff) > 0) {
+              // a big list: its corrections hang on the near test
+              const uint64_t lk = ((cm >> k) & 1) ? live : 0ull;
+              const bool mg = lane_in(lk) && near_marginal(r2k[k], (float)nd.near2, D);
+              if (mg) risk = __builtin_inff();
+            }
           }
         }
       }
@@ -836,7 +878,8 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
         const int cnt = nodes[base + k].count_cmask & 0x0fffffff;
         const uint64_t o = k == 0 ? om[0] : k == 1 ? om[1] : k == 2 ? om[2] : om[3];
         const uint64_t n = k == 0 ? nm[0] : k == 1 ? nm[1] : k == 2 ? nm[2] : nm[3];
-        leaf32(st, cnt, lane_in(o), lane_in(n), rx, ry, parts, a);
+        leaf32<GUARD>(st, cnt, lane_in(o), lane_in(n), rx, ry, parts, a, r2k[k],
+                      (float)nodes[base + k].near2, D, risk);
       }
     }
 #pragma nounroll
@@ -866,6 +909,7 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
       nodes = as_const(td->nodesF);
       parts = as_const(td->partsD);
       bigs = as_const(td->bigD);
+      sheetd = as_const(td->sheetd);
       a.tree = (uint32_t)h;
       entry = 1 << 28;
       live = amask;
@@ -884,12 +928,20 @@ __device__ inline void finish32(const Acc32& a, double* out) {
   out[4] = -2.0 * kInvPI * a.g1.value() + a.fg1;
 }
 
-// A tree's mass sheet for the FP32 walk, added to its outputs.
+// A tree's mass sheet for the FP32 walk, added to its outputs: inv_area
+// (S - M x) with S the sum of m x over the particles, which is M c when the
+// masses share a sign (the walk adds the far cells' partition terms of a
+// tree with masses of both signs).
 __device__ inline void sheet32(const TreeDesc& t, double rx, double ry, double* out) {
   if (t.inv_area != 0.0) {
     const double sheet = t.inv_area * t.mass;
-    out[0] += sheet * (t.comx - rx);
-    out[1] += sheet * (t.comy - ry);
+    if (t.sheetd) {
+      out[0] += t.inv_area * t.dipx - sheet * rx;
+      out[1] += t.inv_area * t.dipy - sheet * ry;
+    } else {
+      out[0] += sheet * (t.comx - rx);
+      out[1] += sheet * (t.comy - ry);
+    }
     out[2] -= sheet;
   }
 }
@@ -919,6 +971,33 @@ __device__ void direct64(const TreeDesc& t, double rx, double ry, bool active, A
     const double size = fabs((double)p.h);
     if (rr < 4 * size * size) bspline64(d0, d1, sqrt(rr), m, size, a.ax, a.ay, a.k, a.g0, a.g1);
   }
+}
+
+// The precision form's bounds, carried with the recursion: the FP32 walk's
+// deflection may be off by eps max(|alpha|, ascale) plus the flip errors of
+// its marginal tests (risk, walk units); a position error eth moves the
+// plane's deflection by at most |G| eth (G the step's Jacobian term); both
+// enter SumPrevAlphas and the next position as the recursion combines them.
+// The risk is summed over the planes in radians (|charge| times the walk's).
+__device__ inline void guard_step(const StepJob& jb, int slot, double alpha0, double alpha1,
+                                  double kappa, double gamma0, double gamma1, float risk) {
+  const RayState st = state_view(jb.sbase, jb.nslots);
+  const double amag = sqrt(alpha0 * alpha0 + alpha1 * alpha1);
+  const double ea = jb.eps * fmax(amag, jb.ascale) + (double)risk;
+  st.risk[slot] += (float)(fabs(jb.charge) * (double)risk);
+  const double gn = fabs(jb.Gfac) * (fabs(kappa) + sqrt(gamma0 * gamma0 + gamma1 * gamma1));
+  const double eth = st.eth[slot];
+  const double eS = (double)st.eS[slot] + gn * eth + fabs(jb.charge) * ea;
+  double aa, bb;
+  if (jb.j < st.jmax[slot] - 1) {
+    aa = jb.aa;
+    bb = jb.bb;
+  } else {
+    aa = st.aal[slot];
+    bb = st.bbl[slot];
+  }
+  st.eS[slot] = (float)eS;
+  st.eth[slot] = (float)(fabs(bb) * eth + fabs(aa) * eS);
 }
 
 // FP64 plane recursion step (include/lens.h:1003-1066), without FMA
@@ -988,14 +1067,21 @@ __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, W
   }
   const double rx = phys_coord(th0, jb.Dphys), ry = phys_coord(th1, jb.Dphys);
   double alpha0 = 0, alpha1 = 0, kappa = 0, gamma0 = 0, gamma1 = 0;
+  float plane_risk = 0.f;
   Counts cnt;
   const int ntrees = uni(jb.ntrees);
   if (MODE == SLSIM_FP32C) {
     // one set of FP32 sums over all halos of the plane, so that no FP64
     // totals are live during the walks
     Acc32 a;
+    // (2 dx)^2, dx this ray's position bound, physical units
+    const double dx2 = active ? 2.0 * (double)st.eth[slot] * jb.Dphys : 0.0;
+    const double D = dx2 * dx2;
+    float risk = 0.f;
     // (a plane may hold halo-of-halos trees only; walk32 starts at trees[0])
-    if (HK == 0 || ntrees > 0) walk32(jb.trees, ntrees, rx, ry, active, a, s_node, s_mask, s_near, err);
+    if (HK == 0 || ntrees > 0)
+      walk32<true>(jb.trees, ntrees, rx, ry, active, a, s_node, s_mask, s_near, err, D, risk);
+    plane_risk = risk;
     double f[5];
     finish32(a, f);
     for (int h = 0; h < ntrees; ++h) sheet32(as_const(jb.trees)[h], rx, ry, f);
@@ -1036,6 +1122,7 @@ __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, W
     uint32_t* c = jb.counters + kCountStride * (size_t)slot;
     c[0] += cnt.v; c[1] += cnt.c; c[2] += cnt.p; c[3] += cnt.b; c[4] += cnt.s;
   }
+  if (MODE == SLSIM_FP32C) guard_step(jb, slot, alpha0, alpha1, kappa, gamma0, gamma1, plane_risk);
   recursion_step(jb, slot, alpha0, alpha1, kappa, gamma0, gamma1);
 }
 
@@ -1142,6 +1229,9 @@ __device__ inline void init_slot(const RayState& st, int64_t slot, bool valid, d
   st.jmax[slot] = valid ? jmax : 0;
   st.aal[slot] = aal;
   st.bbl[slot] = bbl;
+  st.eth[slot] = 0.f;
+  st.eS[slot] = 0.f;
+  st.risk[slot] = 0.f;
 }
 
 
@@ -1221,15 +1311,23 @@ __global__ void k_finalize_batch(const GridReq* __restrict__ reqs, int nreq, int
 // Array rays: slot s < n holds ray perm[s] (Morton order).  Per-ray source
 // redshifts resolve jmax, Dls and Ds as Lens::FindSourcePlane does
 // (MultiPlane/lens.cpp:2659-2681).
+// `lanes` rays per 64-slot bundle (sparse bundles for a few scattered rays):
+// sorted ray k in slot (k / lanes) 64 + k % lanes.
+__device__ inline int64_t array_ray_index(int64_t slot, int lanes) {
+  const int64_t l = slot & 63;
+  return l < lanes ? (slot >> 6) * lanes + l : -1;
+}
+
 __global__ void k_init_array(RayState st, int64_t nslots, int64_t n, const uint32_t* __restrict__ perm,
                              const double* __restrict__ x, const double* __restrict__ zs,
                              int32_t jmax_u, double aal_u, double bbl_u,
                              const double* __restrict__ zpl, const double* __restrict__ Dl,
-                             int32_t nplanes, FlatLCDM cosmo) {
+                             int32_t nplanes, FlatLCDM cosmo, int lanes) {
   const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (slot >= nslots) return;
-  const bool valid = slot < n;
-  const int64_t i = valid ? (perm ? (int64_t)perm[slot] : slot) : 0;   // no perm: ray s in slot s
+  const int64_t k = array_ray_index(slot, lanes);
+  const bool valid = k >= 0 && k < n;
+  const int64_t i = valid ? (perm ? (int64_t)perm[k] : k) : 0;   // no perm: ray k in place
   const double x0 = valid ? x[2 * i] : 0.0, x1 = valid ? x[2 * i + 1] : 0.0;
   st.th0[slot] = x0; st.th1[slot] = x1;
   st.sa0[slot] = x0; st.sa1[slot] = x1;
@@ -1255,19 +1353,108 @@ __global__ void k_init_array(RayState st, int64_t nslots, int64_t n, const uint3
   st.jmax[slot] = valid ? jm : 0;
   st.aal[slot] = aal;
   st.bbl[slot] = bbl;
+  st.eth[slot] = 0.f;
+  st.eS[slot] = 0.f;
+  st.risk[slot] = 0.f;
 }
 
-__global__ void k_finalize_array(RayState st, int64_t n, const uint32_t* __restrict__ perm,
-                                 double* __restrict__ y, double* __restrict__ A) {
+__global__ void k_finalize_array(RayState st, int64_t nslots, int64_t n,
+                                 const uint32_t* __restrict__ perm, double* __restrict__ y,
+                                 double* __restrict__ A, int lanes) {
   const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (slot >= n) return;
-  const int64_t i = perm ? (int64_t)perm[slot] : slot;
+  if (slot >= nslots) return;
+  const int64_t k = array_ray_index(slot, lanes);
+  if (k < 0 || k >= n) return;
+  const int64_t i = perm ? (int64_t)perm[k] : k;
   y[2 * i] = st.th0[slot];
   y[2 * i + 1] = st.th1[slot];
   A[4 * i] = st.A0[slot];
   A[4 * i + 1] = st.A1[slot];
   A[4 * i + 2] = st.A2[slot];
   A[4 * i + 3] = st.A3[slot];
+}
+
+// ----------------------------------------------- precision-form guard
+// A call's slots: ray i and its image position.  Grid: the tiles of a
+// GridMap; array: slot s < n holds ray perm[s] of x.
+__device__ inline bool slot_ray(const GuardMap& m, int64_t slot, int64_t* i, double* x0,
+                                double* x1) {
+  if (m.x == nullptr) {
+    if (!grid_slot(slot, m.N, m.tiles_x, m.tw, i)) return false;
+    grid_position(*i, m.N, m.c0, m.c1, m.range, x0, x1);
+    return true;
+  }
+  if (slot >= m.n) return false;
+  *i = m.perm ? (int64_t)m.perm[slot] : slot;
+  *x0 = m.x[2 * *i];
+  *x1 = m.x[2 * *i + 1];
+  return true;
+}
+
+// sum of |alpha|^2 = |x - y|^2 over the call's rays (y on the source plane
+// is the state's theta after the last step): per-block partial sums in a
+// fixed order, then k_guard_total adds them in block order (repeatable)
+__global__ void k_guard_sumsq(RayState st, int64_t nslots, GuardMap m, double* __restrict__ part) {
+  __shared__ double sh[256];
+  double v = 0;
+  for (int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; slot < nslots;
+       slot += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i;
+    double x0, x1;
+    if (!slot_ray(m, slot, &i, &x0, &x1)) continue;
+    const double a0 = x0 - st.th0[slot], a1 = x1 - st.th1[slot];
+    v += a0 * a0 + a1 * a1;
+  }
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_guard_total(const double* __restrict__ part, int nblk, double* __restrict__ acc) {
+  if (threadIdx.x != 0) return;
+  double t = 0;
+  for (int b = 0; b < nblk; ++b) t += part[b];
+  *acc = t;
+}
+
+// rays to shoot again in FP64: flagged by the walk's guard, or with |alpha|
+// below kGuardSmallAlpha of the rms (acc: the sum of |alpha|^2 over n rays)
+__global__ void k_guard_flag(RayState st, int64_t nslots, GuardMap m, int64_t nrays,
+                             const double* __restrict__ acc, uint32_t* __restrict__ count,
+                             uint32_t cap, uint32_t* __restrict__ idx, double* __restrict__ xl,
+                             double* __restrict__ zl) {
+  const int64_t slot = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (slot >= nslots) return;
+  int64_t i;
+  double x0, x1;
+  if (!slot_ray(m, slot, &i, &x0, &x1)) return;
+  const double a0 = x0 - st.th0[slot], a1 = x1 - st.th1[slot];
+  const double ms = *acc / (double)nrays;   // mean |alpha|^2
+  const double a2 = a0 * a0 + a1 * a1;
+  const double floor2 = 1e-6 * ms;
+  const double lim = kGuardTol / kGuardSafety * sqrt(fmax(a2, floor2));
+  if ((double)st.risk[slot] <= lim && a2 >= kGuardSmallAlpha * kGuardSmallAlpha * ms) return;
+  const uint32_t k = atomicAdd(count, 1u);
+  if (k >= cap) return;
+  idx[k] = (uint32_t)i;
+  xl[2 * k] = x0;
+  xl[2 * k + 1] = x1;
+  if (zl) zl[k] = m.zs[i];
+}
+
+__global__ void k_guard_scatter(uint32_t n, const uint32_t* __restrict__ idx,
+                                const double* __restrict__ yt, const double* __restrict__ At,
+                                double* __restrict__ y, double* __restrict__ A) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t i = idx[k];
+  y[2 * i] = yt[2 * k];
+  y[2 * i + 1] = yt[2 * k + 1];
+  for (int c = 0; c < 4; ++c) A[4 * i + c] = At[4 * k + c];
 }
 
 // 30-bit Morton code of each ray over the rays' bounding box.
@@ -1324,7 +1511,8 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_force(
   double r[5];
   if (F32) {
     Acc32 a;
-    walk32(tree, 1, rx, ry, active, a, s_node[w], s_mask[w], s_near[w], err);
+    float unused = 0.f;   // no recursion here: no guard
+    walk32<false>(tree, 1, rx, ry, active, a, s_node[w], s_mask[w], s_near[w], err, 0.0, unused);
     finish32(a, r);
     sheet32(t, rx, ry, r);
   } else {
@@ -1489,14 +1677,34 @@ void launch_finalize_batch(hipStream_t st, const GridReq* reqs, int nreq, int64_
 void launch_init_array(hipStream_t st, const RayState& s, int64_t nslots, int64_t n,
                        const uint32_t* perm, const double* x, const double* zs, int32_t jmax,
                        double aal, double bbl, const double* zpl, const double* Dl,
-                       int32_t nplanes, const FlatLCDM& cosmo) {
+                       int32_t nplanes, const FlatLCDM& cosmo, int lanes) {
   k_init_array<<<blocks_for(nslots, 256), 256, 0, st>>>(s, nslots, n, perm, x, zs, jmax, aal, bbl,
-                                                        zpl, Dl, nplanes, cosmo);
+                                                        zpl, Dl, nplanes, cosmo, lanes);
 }
 
-void launch_finalize_array(hipStream_t st, const RayState& s, int64_t n, const uint32_t* perm,
-                           double* y, double* A) {
-  k_finalize_array<<<blocks_for(n, 256), 256, 0, st>>>(s, n, perm, y, A);
+void launch_finalize_array(hipStream_t st, const RayState& s, int64_t nslots, int64_t n,
+                           const uint32_t* perm, double* y, double* A, int lanes) {
+  k_finalize_array<<<blocks_for(nslots, 256), 256, 0, st>>>(s, nslots, n, perm, y, A, lanes);
+}
+
+void launch_guard_sumsq(hipStream_t st, const RayState& s, int64_t nslots, const GuardMap& m,
+                        double* part, double* acc) {
+  const unsigned nb = (unsigned)std::min<int64_t>(kGuardBlocks, (nslots + 255) / 256);
+  if (nb == 0) return;
+  k_guard_sumsq<<<nb, 256, 0, st>>>(s, nslots, m, part);
+  k_guard_total<<<1, 64, 0, st>>>(part, (int)nb, acc);
+}
+
+void launch_guard_flag(hipStream_t st, const RayState& s, int64_t nslots, const GuardMap& m,
+                       int64_t nrays, const double* acc, uint32_t* count, uint32_t cap,
+                       uint32_t* idx, double* xl, double* zl) {
+  k_guard_flag<<<blocks_for(nslots, 256), 256, 0, st>>>(s, nslots, m, nrays, acc, count, cap, idx,
+                                                        xl, zl);
+}
+
+void launch_guard_scatter(hipStream_t st, uint32_t n, const uint32_t* idx, const double* yt,
+                          const double* At, double* y, double* A) {
+  k_guard_scatter<<<blocks_for(n, 256), 256, 0, st>>>(n, idx, yt, At, y, A);
 }
 
 void launch_ray_morton(hipStream_t st, const double* x, int64_t n, double x0, double y0,

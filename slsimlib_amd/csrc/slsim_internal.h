@@ -48,12 +48,14 @@ struct alignas(16) NodeD {
 static_assert(sizeof(NodeD) == 96, "NodeD layout");
 
 // Cell record of the FP32 walk (64 B).  The opening test stays FP64 so that
-// both walks open exactly the same cells for a ray.
+// both walks open exactly the same cells for a ray: r^2 < T with T =
+// max(r2crit, near2) (r^2 < r2crit or r^2 < near2), near2 gating the big
+// particles.
 struct alignas(16) NodeF {
   double cx, cy;
-  double r2crit, near2;
+  double T, near2;
   float mass, q0, q1, q2;   // q0..q2: 4 times the quadrupole
-  float box;
+  float flip;               // the cell's flip error (walk units), tree_build.hip
   int32_t child;
   int32_t start;
   int32_t count_cmask;  // count | cmask << 28
@@ -122,6 +124,9 @@ struct TreeDesc {
   const PartD* bigD;
   double inv_area;
   double mass, comx, comy;   // total mass and centre of mass (the mass sheet's terms)
+  double dipx, dipy;         // sum of m x over the particles
+  const float2* sheetd;      // masses of both signs (else null): per cell
+                             // pi inv_area (M_c c - D_c), see tree_build.hip
   int32_t nnodes;
   int32_t nparts;
   // halo-of-halos trees (kind == kTreeHalos)
@@ -146,8 +151,13 @@ struct RayState {
   double* G0; double* G1; double* G2; double* G3;   // SumPrevAG
   double* aal; double* bbl;     // (aa, bb) of the last plane before the source
   int32_t* jmax;                // planes in front of the source; 0 = inactive slot
+  // precision form only: bounds on how far the FP32-walk ray can be from the
+  // FP64 ray (eth: theta on the current plane, eS: SumPrevAlphas, radians),
+  // and the risk: the summed flip errors (radians at the source) of the
+  // opening tests that fell within that distance of their thresholds
+  float* eth; float* eS; float* risk;
 };
-constexpr size_t kStateBytesPerSlot = 14 * sizeof(double) + sizeof(int32_t);
+constexpr size_t kStateBytesPerSlot = 14 * sizeof(double) + sizeof(int32_t) + 3 * sizeof(float);
 
 // The 15 arrays of a ray state laid out back to back from `d` (n slots each).
 __host__ __device__ inline RayState state_view(double* d, int64_t n) {
@@ -158,6 +168,9 @@ __host__ __device__ inline RayState state_view(double* d, int64_t n) {
   s.G0 = d + 8 * n;  s.G1 = d + 9 * n;  s.G2 = d + 10 * n; s.G3 = d + 11 * n;
   s.aal = d + 12 * n; s.bbl = d + 13 * n;
   s.jmax = reinterpret_cast<int32_t*>(d + 14 * n);
+  s.eth = reinterpret_cast<float*>(s.jmax + n);
+  s.eS = s.eth + n;
+  s.risk = s.eS + n;
   return s;
 }
 
@@ -176,6 +189,40 @@ struct StepJob {
   double aa, bb;                // dDl[j+1]/Dl[j+1], Dl[j]/Dl[j+1]
   double charge;
   uint32_t* counters;           // kCountStride per slot or null
+  // precision form's guard: the plane's deflection scale (walk units,
+  // sum over its particle trees of M / (pi R), R half the root box) and the
+  // FP32 walk's error per unit of deflection
+  double ascale;
+  double eps;
+};
+
+// The precision form's guard.  The FP32 walk's deflection is taken to be
+// off by at most kGuardEps times the larger of the ray's deflection and the
+// plane's deflection scale (measured: below 1e-7 of the rms per plane).  A
+// ray is shot again in FP64 when its risk, times kGuardSafety, exceeds
+// kGuardTol of max(|alpha|, 1e-3 rms) (the error measure of the precision
+// form's bound), or when |alpha| is below kGuardSmallAlpha of the call's
+// rms: the FP32 walk's absolute error is a fraction of the field's
+// deflections, so its relative error grows where alpha vanishes.
+constexpr double kGuardEps = 16.0 / 16777216.0;
+constexpr double kGuardTol = 1e-5;
+constexpr double kGuardSafety = 2.0;
+constexpr double kGuardSmallAlpha = 0.01;
+// the guard's FP64 shoot puts fewer than 64 rays in a bundle while that
+// still leaves at most this many bundles (about the waves the GPU holds)
+constexpr int64_t kGuardWaves = 4096;
+
+// A call's slots for the guard kernels: x == null: the tiles of an N x N
+// GridMap; else slot s < n holds ray perm[s] (perm == null: ray s) of x,
+// with per-ray source redshifts zs (or null).
+struct GuardMap {
+  int64_t N, tiles_x;
+  int32_t tw, pad;
+  double c0, c1, range;
+  const uint32_t* perm;
+  const double* x;
+  const double* zs;
+  int64_t n;
 };
 
 struct WorkItem {
@@ -274,6 +321,9 @@ struct slsim_plane_s {
   double p1[2] = {0, 0}, p2[2] = {0, 0};
   double theta = 0.1, inv_area = 0.0;
   double mass = 0, com[2] = {0, 0};   // total mass and centre of mass (root cell)
+  double dip[2] = {0, 0};             // sum of m x over the particles (signed masses)
+  bool mixed_sign = false;            // masses of both signs
+  float2* sheetd = nullptr;           // per cell: the sheet's partition term (mixed signs)
   int bucket = 20;
   double max_range = -1;   // > 0: cells far beyond it are not split
   double build_ms = 0;
@@ -338,8 +388,15 @@ struct slsim_curves_s {
 
 namespace slsim {
 
+// Host-side fixed sizes of a shoot call: partial blocks of the ray bounding
+// box, and plane steps held in a stack array (more planes use the heap,
+// except in the no-allocation entry point).
+constexpr int kMinmaxBlocks = 1024;
+constexpr int kMaxPlanesFixed = 256;
+
 // Device scratch of one call, returned to the context's cache on scope exit.
 struct Scratch {
+  static constexpr bool kMayAllocate = true;
   slsim_ctx ctx;
   std::vector<void*> ptrs;
   explicit Scratch(slsim_ctx c) : ctx(c) {}
@@ -375,14 +432,23 @@ void launch_counts_grid(hipStream_t st, const uint32_t* c, int64_t nslots, int64
                         int64_t tiles_x, int tw, uint64_t* out);
 void launch_init_batch(hipStream_t st, const GridReq* reqs, int nreq, int64_t total);
 void launch_finalize_batch(hipStream_t st, const GridReq* reqs, int nreq, int64_t total);
+// array rays: `lanes` rays per 64-slot bundle (64: dense)
 void launch_init_array(hipStream_t st, const RayState& s, int64_t nslots, int64_t n,
                        const uint32_t* perm, const double* x, const double* zs, int32_t jmax,
                        double aal, double bbl, const double* zpl, const double* Dl,
-                       int32_t nplanes, const FlatLCDM& cosmo);
-void launch_finalize_array(hipStream_t st, const RayState& s, int64_t n, const uint32_t* perm,
-                           double* y, double* A);
+                       int32_t nplanes, const FlatLCDM& cosmo, int lanes = 64);
+void launch_finalize_array(hipStream_t st, const RayState& s, int64_t nslots, int64_t n,
+                           const uint32_t* perm, double* y, double* A, int lanes = 64);
 void launch_ray_morton(hipStream_t st, const double* x, int64_t n, double x0, double y0,
                        double inv_w, uint64_t* code, uint32_t* idx);
+constexpr int kGuardBlocks = 1024;   // partial sums of the guard's reduction
+void launch_guard_sumsq(hipStream_t st, const RayState& s, int64_t nslots, const GuardMap& m,
+                        double* part, double* acc);
+void launch_guard_flag(hipStream_t st, const RayState& s, int64_t nslots, const GuardMap& m,
+                       int64_t nrays, const double* acc, uint32_t* count, uint32_t cap,
+                       uint32_t* idx, double* xl, double* zl);
+void launch_guard_scatter(hipStream_t st, uint32_t n, const uint32_t* idx, const double* yt,
+                          const double* At, double* y, double* A);
 void launch_minmax_rays(hipStream_t st, const double* x, int64_t n, double* part, int nblk);
 // kind[i]: 0 NFW (param = concentration, a float), 1 power law (param = beta);
 // kind == nullptr: all NFW
@@ -400,6 +466,11 @@ template <typename T>
 int build_plane(slsim_ctx ctx, const T* x, const T* y, const float* m, const float* h, size_t n,
                 slsim_plane pl);
 int export_nodes(slsim_plane pl, double* dbl, int64_t* ints);
+// the precision form's guard over one finished shoot of L (api.hip): flagged
+// and small-|alpha| rays shot again in FP64 on the context stream
+int guard_refine_ctx(slsim_lens L, const RayState& st, int64_t nslots, const GuardMap& m,
+                     int64_t nrays, const double* zs_dev, double* y_dev, double* A_dev,
+                     int32_t* nguard);
 // lens helpers (api.hip)
 void uniform_source(const slsim_lens_s* L, double zs, int32_t* jmax, double* aal, double* bbl);
 void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint32_t* counters,

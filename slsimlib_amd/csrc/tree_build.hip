@@ -319,6 +319,176 @@ __global__ __launch_bounds__(1024) void k_moments_big(
   }
 }
 
+// Flip error of a cell: how much a ray's deflection changes when the cell's
+// opening test is decided the other way, at the distance where that test
+// flips (r^2 = max(r2crit, 5.83 box^2), the opening test of
+// include/quadTree.h:1010): the cell's monopole + quadrupole term (the
+// reference's far-cell formula, :1089-1123, with its factor 4) against the
+// exact sum over its particles, point masses and, where the point is within
+// 2h of a particle, B-spline corrected (quadTree.h:147-190; as a bound, for
+// every particle), at eight points around the centre of mass.  The largest
+// difference (walk units, alpha before the halo's sign flip) is kept.
+constexpr int kFlipDirs = 8;
+
+__device__ inline void flip_point(int k, double cx, double cy, double R, double* px, double* py) {
+  const double phi = (k + 0.5) * (kPI / 4);
+  *px = cx + R * cos(phi);
+  *py = cy + R * sin(phi);
+}
+
+// partial sums over particles [s + k0, s + c) with stride `step` of the
+// deflection at (qx, qy): point masses (pm) and B-spline corrected (bs)
+__device__ inline void flip_sums(int32_t s, int32_t c, int32_t k0, int32_t step,
+                                 const double* __restrict__ xs, const double* __restrict__ ys,
+                                 const float* __restrict__ ms, const float* __restrict__ hs,
+                                 double qx, double qy, double* pm, double* bs) {
+  pm[0] = pm[1] = bs[0] = bs[1] = 0;
+  for (int32_t k = k0; k < c; k += step) {
+    const double x = xs[s + k], y = ys[s + k], m = ms[s + k], h = hs[s + k];
+    const double d0 = x - qx, d1 = y - qy;
+    double rr = d0 * d0 + d1 * d1;
+    if (rr < 1e-20) rr = 1e-20;
+    const double pref = m / (kPI * rr);
+    pm[0] -= pref * d0;
+    pm[1] -= pref * d1;
+    double b0 = -pref * d0, b1 = -pref * d1;
+    if (rr < 4 * h * h) {
+      const double r = sqrt(rr), q = r / h, q2 = q * q, q3 = q2 * q;
+      const double M = q > 1 ? (-1 + 20 * q2 * (1 - q + 3 * q2 / 8 - q3 / 20)) / 7
+                             : 10 * q2 * (1 - 3 * q / 4 + 3 * q3 / 10) / 7;
+      const double ar = (M - 1) / (kPI * r);
+      b0 -= m * ar * d0 / r;
+      b1 -= m * ar * d1 / r;
+    }
+    bs[0] += b0;
+    bs[1] += b1;
+  }
+}
+
+// the far-cell deflection at (qx, qy) against the sums: the larger difference
+__device__ inline double flip_diff(double cx, double cy, double qx, double qy, double mass,
+                                   double q0, double q1, double q2, const double* pm,
+                                   const double* bs) {
+  const double dx = cx - qx, dy = cy - qy;
+  const double r2 = dx * dx + dy * dy, inv = 1 / r2;
+  const double mi = mass * inv / kPI;
+  double fx = -mi * dx, fy = -mi * dy;
+  const double i2p = inv * inv / kPI;
+  fx -= (q0 * dx + q2 * dy) * i2p;
+  fy -= (q1 * dy + q2 * dx) * i2p;
+  const double t = 4 * (q0 * dx * dx + q1 * dy * dy + 2 * q2 * dx * dy) * inv * i2p;
+  fx += t * dx;
+  fy += t * dy;
+  return fmax(hypot(fx - pm[0], fy - pm[1]), hypot(fx - bs[0], fy - bs[1]));
+}
+
+__device__ inline double flip_radius(double r2crit, double box) {
+  return sqrt(fmax(r2crit, 5.83 * (box * box)));
+}
+
+// one wavefront per cell of at most kBigCell particles
+__global__ void k_flip_error(int64_t nnodes, const int32_t* __restrict__ st,
+                             const int32_t* __restrict__ cnt, const double* __restrict__ p1,
+                             const double* __restrict__ p2, const double* __restrict__ xs,
+                             const double* __restrict__ ys, const float* __restrict__ ms,
+                             const float* __restrict__ hs, const double* __restrict__ com,
+                             const double* __restrict__ mass, const double* __restrict__ quad,
+                             const double* __restrict__ r2crit, double* __restrict__ err) {
+  const int lane = threadIdx.x & 63;
+  const int64_t node = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  if (node >= nnodes) return;
+  const int32_t s = st[node], c = cnt[node];
+  if (c > kBigCell) return;   // k_flip_error_big
+  if (c == 0 || r2crit[node] >= 1e99) {
+    if (lane == 0) err[node] = 0;
+    return;
+  }
+  const double cx = com[2 * node], cy = com[2 * node + 1];
+  const double R = flip_radius(r2crit[node], p2[2 * node] - p1[2 * node]);
+  double e = 0;
+  for (int d = 0; d < kFlipDirs; ++d) {
+    double qx, qy, pm[2], bs[2];
+    flip_point(d, cx, cy, R, &qx, &qy);
+    flip_sums(s, c, lane, 64, xs, ys, ms, hs, qx, qy, pm, bs);
+    for (int k = 0; k < 2; ++k) {
+      pm[k] = wave_sum(pm[k]);
+      bs[k] = wave_sum(bs[k]);
+    }
+    e = fmax(e, flip_diff(cx, cy, qx, qy, mass[node], quad[3 * node], quad[3 * node + 1],
+                          quad[3 * node + 2], pm, bs));
+  }
+  if (lane == 0) err[node] = e;
+}
+
+// cells of more than kBigCell particles: one 1024-thread block per cell
+__global__ __launch_bounds__(1024) void k_flip_error_big(
+    const int32_t* __restrict__ list, const int32_t* __restrict__ st,
+    const int32_t* __restrict__ cnt, const double* __restrict__ p1,
+    const double* __restrict__ p2, const double* __restrict__ xs, const double* __restrict__ ys,
+    const float* __restrict__ ms, const float* __restrict__ hs, const double* __restrict__ com,
+    const double* __restrict__ mass, const double* __restrict__ quad,
+    const double* __restrict__ r2crit, double* __restrict__ err) {
+  __shared__ double sh[16];
+  const int64_t node = list[blockIdx.x];
+  const int32_t s = st[node], c = cnt[node];
+  if (r2crit[node] >= 1e99) {
+    if (threadIdx.x == 0) err[node] = 0;
+    return;
+  }
+  const double cx = com[2 * node], cy = com[2 * node + 1];
+  const double R = flip_radius(r2crit[node], p2[2 * node] - p1[2 * node]);
+  double e = 0;
+  for (int d = 0; d < kFlipDirs; ++d) {
+    double qx, qy, pm[2], bs[2];
+    flip_point(d, cx, cy, R, &qx, &qy);
+    flip_sums(s, c, threadIdx.x, blockDim.x, xs, ys, ms, hs, qx, qy, pm, bs);
+    for (int k = 0; k < 2; ++k) {
+      pm[k] = block_sum(pm[k], sh);
+      bs[k] = block_sum(bs[k], sh);
+    }
+    e = fmax(e, flip_diff(cx, cy, qx, qy, mass[node], quad[3 * node], quad[3 * node + 1],
+                          quad[3 * node + 2], pm, bs));
+  }
+  if (threadIdx.x == 0) err[node] = e;
+}
+
+// Mass-sheet correction of a tree whose particle masses have both signs.
+// The walk adds inv_area M_c (c - x) per far cell (c its |m|-weighted centre
+// of mass) and inv_area m (x_p - x) per leaf particle
+// (include/quadTree.h:861-888, 1035-1044).  Summed over any partition of the
+// particles this is inv_area (S - M x) + inv_area sum over the far cells of
+// (M_c c - D_c), with D_c = sum of m x_p over the cell and S that sum over the
+// tree; the second term vanishes when all masses share a sign.  Per cell:
+// pi inv_area (M_c c - D_c) as the FP32 walk adds it (it sums pi alpha).
+__global__ void k_sheet_delta(int64_t nnodes, const int32_t* __restrict__ st,
+                              const int32_t* __restrict__ cnt, const double* __restrict__ xs,
+                              const double* __restrict__ ys, const float* __restrict__ ms,
+                              const double* __restrict__ com, const double* __restrict__ mass,
+                              double inv_area, float2* __restrict__ delta,
+                              double* __restrict__ dip_root) {
+  const int lane = threadIdx.x & 63;
+  const int64_t node = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  if (node >= nnodes) return;
+  const int32_t s = st[node], c = cnt[node];
+  double dx = 0, dy = 0;
+  for (int32_t k = lane; k < c; k += 64) {
+    const double w = ms[s + k];
+    dx += w * xs[s + k];
+    dy += w * ys[s + k];
+  }
+  dx = wave_sum(dx);
+  dy = wave_sum(dy);
+  if (lane == 0) {
+    const double f = kPI * inv_area;
+    delta[node] = make_float2((float)(f * (mass[node] * com[2 * node] - dx)),
+                              (float)(f * (mass[node] * com[2 * node + 1] - dy)));
+    if (node == 0) {
+      dip_root[0] = dx;
+      dip_root[1] = dy;
+    }
+  }
+}
+
 // Per particle: its leaf, and whether it joins an internal cell's big list
 // (root: h > box/2; other internal cells: box/2 < h <= box;
 // include/quadTree.h:403-431) or its leaf's (h < box, :379-396).
@@ -361,7 +531,8 @@ __global__ void k_pack_nodes(int64_t nnodes, const double* __restrict__ com,
                              const double* __restrict__ p2, const int32_t* __restrict__ child,
                              const int32_t* __restrict__ st, const int32_t* __restrict__ cnt,
                              const int32_t* __restrict__ bst, const int32_t* __restrict__ bcnt,
-                             NodeD* __restrict__ nd, NodeF* __restrict__ nf) {
+                             const double* __restrict__ flip, NodeD* __restrict__ nd,
+                             NodeF* __restrict__ nf) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= nnodes) return;
   NodeD a;
@@ -392,14 +563,14 @@ __global__ void k_pack_nodes(int64_t nnodes, const double* __restrict__ com,
   NodeF f;
   f.cx = a.cx;
   f.cy = a.cy;
-  f.r2crit = a.r2crit;
   f.near2 = a.near2;
+  f.T = fmax(a.r2crit, a.near2);
   f.mass = (float)a.mass;
   // 4 q: the FP32 walk's quadrupole term needs 4 (q . e) and (q . e) / 4
   f.q0 = 4.f * (float)a.q0;
   f.q1 = 4.f * (float)a.q1;
   f.q2 = 4.f * (float)a.q2;
-  f.box = (float)box;
+  f.flip = (float)flip[i];
   f.child = a.child;
   f.start = a.start;
   f.count_cmask = a.count | (cm << 28);
@@ -475,6 +646,14 @@ template <typename T>
 int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_host,
                 const float* h_host, size_t n, slsim_plane pl) {
   hipStream_t st = ctx->stream;
+  {
+    bool pos = false, neg = false;
+    for (size_t i = 0; i < n; ++i) {
+      pos |= m_host[i] > 0;
+      neg |= m_host[i] < 0;
+    }
+    pl->mixed_sign = pos && neg;
+  }
   hipEvent_t ev0, ev1;
   SLSIM_HIP_CHECK(hipEventCreate(&ev0));
   SLSIM_HIP_CHECK(hipEventCreate(&ev1));
@@ -660,8 +839,10 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
   if ((rc = tmp.get(&com, 2 * nnodes)) || (rc = tmp.get(&mass, nnodes)) ||
       (rc = tmp.get(&quad, 3 * nnodes)) || (rc = tmp.get(&r2c, nnodes)))
     return rc;
+  std::vector<int32_t> big_cells;
   {
-    std::vector<int32_t> hcnt(nnodes), big;
+    std::vector<int32_t> hcnt(nnodes);
+    std::vector<int32_t>& big = big_cells;
     SLSIM_HIP_CHECK(hipMemcpyAsync(hcnt.data(), gcnt, 4 * nnodes, hipMemcpyDeviceToHost, st));
     SLSIM_HIP_CHECK(hipStreamSynchronize(st));
     for (int64_t i = 0; i < nnodes; ++i)
@@ -729,9 +910,33 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
   if ((rc = ctx->alloc(reinterpret_cast<void**>(&pl->partsD), sizeof(PartD) * n))) return rc;
   if ((rc = ctx->alloc(reinterpret_cast<void**>(&pl->npart), 4 * nnodes))) return rc;
   SLSIM_HIP_CHECK(hipMemcpyAsync(pl->npart, gcnt, 4 * nnodes, hipMemcpyDeviceToDevice, st));
+  double* flip;
+  if ((rc = tmp.get(&flip, nnodes))) return rc;
+  k_flip_error<<<nblocks(64 * (size_t)nnodes), kBlock, 0, st>>>(
+      nnodes, gst, gcnt, gp1, gp2, xs, ys, ms, hs, com, mass, quad, r2c, flip);
+  if (!big_cells.empty()) {
+    int32_t* dbig;
+    if ((rc = tmp.get(&dbig, big_cells.size()))) return rc;
+    SLSIM_HIP_CHECK(hipMemcpyAsync(dbig, big_cells.data(), 4 * big_cells.size(),
+                                   hipMemcpyHostToDevice, st));
+    k_flip_error_big<<<(unsigned)big_cells.size(), 1024, 0, st>>>(
+        dbig, gst, gcnt, gp1, gp2, xs, ys, ms, hs, com, mass, quad, r2c, flip);
+    SLSIM_HIP_CHECK(hipStreamSynchronize(st));   // `big_cells` is pageable host memory
+  }
   k_pack_nodes<<<nblocks(nnodes), kBlock, 0, st>>>(nnodes, com, mass, quad, r2c, gp1, gp2, gchild,
-                                                   gst, gcnt, bst, bcnt, pl->nodesD, pl->nodesF);
+                                                   gst, gcnt, bst, bcnt, flip, pl->nodesD,
+                                                   pl->nodesF);
   k_pack_parts<<<nblocks(n), kBlock, 0, st>>>(n, xs, ys, ms, hs, leaf_of, gp1, gp2, pl->partsD);
+  // masses of both signs: the sheet's partition-dependent term per cell
+  double* dip = nullptr;
+  if (pl->mixed_sign && pl->inv_area != 0.0) {
+    if ((rc = ctx->alloc(reinterpret_cast<void**>(&pl->sheetd), sizeof(float2) * (nnodes + 3))))
+      return rc;
+    SLSIM_HIP_CHECK(hipMemsetAsync(pl->sheetd + nnodes, 0, 3 * sizeof(float2), st));
+    if ((rc = tmp.get(&dip, 2))) return rc;
+    k_sheet_delta<<<nblocks(64 * (size_t)nnodes), kBlock, 0, st>>>(
+        nnodes, gst, gcnt, xs, ys, ms, com, mass, pl->inv_area, pl->sheetd, dip);
+  }
   SLSIM_HIP_CHECK(hipEventRecord(ev1, st));
   SLSIM_HIP_CHECK(hipStreamSynchronize(st));
   SLSIM_HIP_CHECK(hipGetLastError());
@@ -741,6 +946,9 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
     pl->mass = root.mass;
     pl->com[0] = root.cx;
     pl->com[1] = root.cy;
+    pl->dip[0] = root.mass * root.cx;
+    pl->dip[1] = root.mass * root.cy;
+    if (dip) SLSIM_HIP_CHECK(hipMemcpy(pl->dip, dip, 16, hipMemcpyDeviceToHost));
   }
   float ms_build = 0;
   SLSIM_HIP_CHECK(hipEventElapsedTime(&ms_build, ev0, ev1));
@@ -763,9 +971,11 @@ int export_nodes(slsim_plane pl, double* dbl, int64_t* ints) {
   hipStream_t st = ctx->stream;
   const int64_t nn = pl->nnodes;
   std::vector<NodeD> nd(nn);
+  std::vector<NodeF> nf(nn);
   std::vector<double> p1(2 * nn), p2(2 * nn);
   std::vector<int32_t> nb(nn), np(nn);
   SLSIM_HIP_CHECK(hipMemcpyAsync(nd.data(), pl->nodesD, sizeof(NodeD) * nn, hipMemcpyDeviceToHost, st));
+  SLSIM_HIP_CHECK(hipMemcpyAsync(nf.data(), pl->nodesF, sizeof(NodeF) * nn, hipMemcpyDeviceToHost, st));
   SLSIM_HIP_CHECK(hipMemcpyAsync(p1.data(), pl->box_p1, 16 * nn, hipMemcpyDeviceToHost, st));
   SLSIM_HIP_CHECK(hipMemcpyAsync(p2.data(), pl->box_p2, 16 * nn, hipMemcpyDeviceToHost, st));
   SLSIM_HIP_CHECK(hipMemcpyAsync(nb.data(), pl->nbig, 4 * nn, hipMemcpyDeviceToHost, st));
@@ -781,7 +991,7 @@ int export_nodes(slsim_plane pl, double* dbl, int64_t* ints) {
     double* d = dbl + 13 * k;
     d[0] = p1[2 * i]; d[1] = p1[2 * i + 1]; d[2] = p2[2 * i]; d[3] = p2[2 * i + 1];
     d[4] = a.cx; d[5] = a.cy; d[6] = a.mass; d[7] = a.q0; d[8] = a.q1; d[9] = a.q2;
-    d[10] = a.r2crit; d[11] = 0; d[12] = 0;
+    d[10] = a.r2crit; d[11] = nf[i].flip; d[12] = 0;
     int64_t* q = ints + 6 * k;
     q[0] = lev; q[1] = np[i]; q[2] = nb[i]; q[3] = a.child < 0 ? 1 : 0; q[4] = 0; q[5] = 0;
     ++k;

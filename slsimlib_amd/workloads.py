@@ -99,3 +99,50 @@ def particles_3d(seed_, n, n_halos=20, conc=8.0, rmax=1.0, dtype=np.float64):
         parts.append(centre + v * r[:, None])
     p = np.concatenate(parts)[rng.permutation(n)]
     return tuple(np.ascontiguousarray(p[:, j], dtype=dtype) for j in range(3))
+
+
+def c4_specs(n_real, seed0=4000, n_lo=1e4, n_hi=1e6, sides=(64, 128, 256, 512, 1024),
+             max_planes=12):
+    """Batched form: n_real independent realizations, each with 1..max_planes
+    planes evenly in D_C over [0.1, min(2, zs - 0.1)], log-uniform particle
+    counts in [n_lo, n_hi] per plane (NFW-field generator, float32
+    positions), a grid side from `sides` and zs uniform in [1, 4].  Returns
+    one dict per realization (planes' z and n, side, zs)."""
+    rng = np.random.default_rng(seed0)
+    specs = []
+    for r in range(n_real):
+        k = int(rng.integers(1, max_planes + 1))
+        zs = float(rng.uniform(1.0, 4.0))
+        side = int(rng.choice(list(sides)))
+        zl = planes_evenly_in_Dc(0.1, min(2.0, zs - 0.1), k)
+        ns = [int(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi)))) for _ in range(k)]
+        specs.append({"realization": r, "z": zl, "n": ns, "side": side, "zs": zs})
+    return specs
+
+
+def build_c4_lens(ctx, spec, theta=0.1, mass_scale=1.0, keep_particles=False):
+    """The lens of one batched-form realization (config 4 seeds)."""
+    lens = Lens(ctx, spec["zs"], OM, OL, H)
+    parts = []
+    for j, (z, n) in enumerate(zip(spec["z"], spec["n"])):
+        x, y, m, h = field_plane(4, j, z, n, soft=5e-4, realization=spec["realization"],
+                                 dtype=np.float32, mass_scale=mass_scale)
+        lens.moveinMainHalo(LensHaloParticles(ctx, x, y, m, h, z, theta=theta))
+        if keep_particles:
+            parts.append((x, y, m, h, np.float32(z)))
+    return lens, parts
+
+
+def c5_stream(n_requests, n_models, seed=5999, sides=(128, 256, 512, 1024, 2048),
+              probs=(0.4, 0.3, 0.15, 0.1, 0.05)):
+    """Serving form's request mix: per request a lens id in [0, n_models),
+    zs uniform in [1, 4], a centre uniform in the 5' field (radians) and a
+    grid side from `sides` with probabilities `probs`; the grid resolution is
+    fixed at 300"/2048 per pixel.  Returns (lens ids, zs, centres (n, 2),
+    sides, resolution in radians)."""
+    rng = np.random.default_rng(seed)
+    side = rng.choice(list(sides), size=n_requests, p=list(probs))
+    lid = rng.integers(0, n_models, n_requests)
+    zs = rng.uniform(1.0, 4.0, n_requests)
+    cen = (rng.random((n_requests, 2)) - 0.5) * 300 * arcsecTOradians
+    return lid, zs, cen, side.astype(np.int64), 300.0 / 2048 * arcsecTOradians

@@ -8,8 +8,8 @@
 // written as raw float64 to argv[1] for the parity test:
 //   [n_particles, rotated particles (x, y, z) ..., centre of mass (3),
 //    densest point (3), GridMap rays of the rotated halo (8 per ray),
-//    ReInitialize'd rays (8 per ray), uniform kappa map (Nx Ny values)]
-// The uniform map is also written as FITS to argv[2] + ".kappa.fits".
+//    ReInitialize'd rays (8 per ray), uniform alpha1 map (Nx Ny values)]
+// The uniform map is also written as FITS to argv[2] + ".alpha1.fits".
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -146,10 +146,10 @@ int main(int argc, char** argv) {
   }
   const double c2[2] = {2e-6, -3e-6};
   const size_t Nx = 20, Ny = 13;
-  PixelMap<double> u = rect.writePixelMapUniform<double>(c2, Nx, Ny, LensingVariable::KAPPA);
+  PixelMap<double> u = rect.writePixelMapUniform<double>(c2, Nx, Ny, LensingVariable::ALPHA1);
   if (u.getNx() != Nx || u.getNy() != Ny) return 17;
   for (size_t i = 0; i < u.size(); ++i) out.push_back(u[i]);
-  rect.writeFitsUniform<double>(c2, Nx, Ny, LensingVariable::KAPPA, std::string(argv[2]));
+  rect.writeFitsUniform<double>(c2, Nx, Ny, LensingVariable::ALPHA1, std::string(argv[2]));
   rect.writeFitsUniform<float>(LensingVariable::INVMAG, std::string(argv[2]) + ".invmag_grid.fits");
 
   FILE* f = std::fopen(argv[1], "wb");

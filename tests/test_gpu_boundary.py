@@ -63,17 +63,17 @@ def test_cpp_dropin_boundary_program(ctx, tmp_path):
     yr, Ar = L2.shoot(rp)
     assert np.abs(re[:, 2:4] - yr).max() <= 1e-12 * np.abs(yr).max()
     assert np.abs(re[:, 4:8] - Ar).max() <= 1e-8
-    # the uniform kappa map: 20 x 13 pixels of range / 19 around (2e-6, -3e-6)
+    # the uniform alpha1 map: 20 x 13 pixels of range / 19 around (2e-6, -3e-6)
     nx, ny = 20, 13
     umap = v[at:at + nx * ny].reshape(ny, nx)
     assert at + nx * ny == v.size
     A = re[:, 4:8]
-    kappa = 1 - 0.5 * (A[:, 0] + A[:, 3])
-    ref = F.pixelmap_uniform(re[:, :2], kappa, (2e-6, -3e-6), nx, ny, rg / (nx - 1))
+    alpha1 = re[:, 0] - re[:, 2]
+    ref = F.pixelmap_uniform(re[:, :2], alpha1, (2e-6, -3e-6), nx, ny, rg / (nx - 1))
     assert np.array_equal(umap, ref)
     assert np.count_nonzero(umap) > nx * ny // 2
     # its FITS file (tagged name) holds the same pixels
-    cards, img = F.fits_parse(open(base + ".kappa.fits", "rb").read())
+    cards, img = F.fits_parse(open(base + ".alpha1.fits", "rb").read())
     assert np.array_equal(img.astype(np.float64), umap)
     cards, img = F.fits_parse(open(base + ".invmag_grid.fits", "rb").read())
     invmag = (A[:, 0] * A[:, 3] - A[:, 1] * A[:, 2]).astype(np.float32)

@@ -1,0 +1,201 @@
+"""The three forms of the ray shooter at reduced size, on their own
+generators: the precision form (20 float32-position planes, opening angle
+0.2), the batched form (realizations of 1-12 planes, grids of uneven size,
+per-realization source redshifts) and the serving form (resident lenses,
+a Poisson stream of GridMap requests), each checked against the oracle on
+ray subsamples and, for the FP32-compensated walk, against the FP64 walk at
+every ray.
+
+Tolerances (per ray, relative |d v| / max(|v_ref|, 1e-3 rms |v_ref|)):
+  FP64 path vs oracle:  alpha <= 1e-9 (same algorithm, same precision)
+  FP32C vs FP64 path:   alpha <= 1e-5 (the precision form's bound)
+"""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+import slsimlib_amd as S
+from slsimlib_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FP32C_ALPHA_TOL = 1e-5
+ORACLE_ALPHA_TOL = 1e-9
+
+
+def rel_err(d, ref):
+    nr = np.linalg.norm(ref, axis=1)
+    floor = 1e-3 * np.sqrt(np.mean(nr ** 2))
+    return np.linalg.norm(d, axis=1) / np.maximum(nr, floor if floor > 0 else 1e-300)
+
+
+@pytest.fixture(scope="module")
+def c3_small(ctx):
+    """20 planes of 2^16 float32-position particles with the particle mass of
+    the 2^22-particle precision form (halo masses scaled by 1/64), theta 0.2."""
+    lens, parts, zl = W.build_field_lens(ctx, config=3, nplanes=20, n=1 << 16, zs=3.0,
+                                         theta=0.2, dtype=np.float32, keep_particles=True,
+                                         mass_scale=1.0 / 64)
+    L = O.Lens(3.0)
+    for (x, y, m, h, z) in parts:
+        L.moveinMainHalo(O.Tree(x, y, m, h, theta=0.2), z)
+    return {"lens": lens, "oracle": L}
+
+
+def test_precision_form_fp32c_within_bound_at_every_ray(c3_small):
+    N, rg = 512, W.grid_range(300.0)
+    g64 = S.GridMap(c3_small["lens"], N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    g32 = S.GridMap(c3_small["lens"], N, (0.0, 0.0), rg, S.SLSIM_FP32C)
+    x = g64.positions()
+    a64, a32 = x - g64.y(), x - g32.y()
+    e = rel_err(a32 - a64, a64)
+    guarded = g32.stats["n_guarded"]
+    print("precision form 512^2: alpha max %.3e p99.9 %.3e, guarded %d rays"
+          % (e.max(), np.quantile(e, 0.999), guarded))
+    assert e.max() <= FP32C_ALPHA_TOL
+    # the guard shoots few rays again: the FP32 walk does the work
+    assert 0 < guarded < 0.05 * N * N
+    # the guarded rays are the FP64 path's, bit for bit
+    same = np.all(g32.y() == g64.y(), axis=1)
+    assert same.sum() >= guarded
+
+
+def test_precision_form_fp64_matches_oracle(c3_small):
+    N, rg = 512, W.grid_range(300.0)
+    g64 = S.GridMap(c3_small["lens"], N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    sub = np.arange(0, N * N, 97)
+    x = g64.positions()[sub]
+    yr, Ar = c3_small["oracle"].shoot(x)
+    y, A = g64.y()[sub], g64.A()[sub]
+    assert rel_err((x - y) - (x - yr), x - yr).max() <= ORACLE_ALPHA_TOL
+    assert np.abs(A - Ar).max() <= 1e-6
+
+
+def test_precision_form_guard_same_for_ray_lists(c3_small):
+    """The guard's decisions depend on a ray alone (and the call's rms), so
+    the rays of a GridMap shot as a list in another order give the same
+    results bit for bit."""
+    N, rg = 128, W.grid_range(300.0)
+    g32 = S.GridMap(c3_small["lens"], N, (0.0, 0.0), rg, S.SLSIM_FP32C)
+    x = g32.positions()
+    perm = np.random.default_rng(9).permutation(N * N)
+    st = {}
+    y, A = c3_small["lens"].rayshooterInternal(x[perm], precision=S.SLSIM_FP32C, stats=st)
+    assert np.array_equal(y, g32.y()[perm]) and np.array_equal(A, g32.A()[perm])
+    assert st["n_guarded"] == g32.stats["n_guarded"]
+
+
+# --------------------------------------------------------------- batched form
+@pytest.fixture(scope="module")
+def c4_small(ctx):
+    """10 realizations of the batched-form generator with 1e3-1e4 particles
+    per plane (particle mass kept: halo masses scaled by 1/100) and sides
+    32-256."""
+    specs = W.c4_specs(10, seed0=4100, n_lo=1e3, n_hi=1e4, sides=(32, 64, 128, 256))
+    out = []
+    for sp in specs:
+        lens, parts = W.build_c4_lens(ctx, sp, mass_scale=1.0 / 100, keep_particles=True)
+        L = O.Lens(sp["zs"])
+        for (x, y, m, h, z) in parts:
+            L.moveinMainHalo(O.Tree(x, y, m, h), z)
+        out.append((sp, lens, L))
+    return out
+
+
+def test_batched_form_matches_oracle(ctx, c4_small):
+    rg = W.grid_range(300.0)
+    reqs = [(lens, sp["side"], (0.0, 0.0), rg, sp["zs"]) for sp, lens, _ in c4_small]
+    b64 = S.Batch(ctx, reqs, S.SLSIM_FP64)
+    b64.run()
+    b32 = S.Batch(ctx, reqs, S.SLSIM_FP32C)
+    b32.set_schedule(tile_log2=2, queue=1, order=2, graph=1)
+    b32.run()
+    b32.run()
+    for i, (sp, lens, L) in enumerate(c4_small):
+        N = sp["side"]
+        x = O.gridmap_positions(N, (0.0, 0.0), rg)
+        y64, A64 = b64.output(i)
+        y32, A32 = b32.output(i)
+        sub = np.arange(0, N * N, max(1, N * N // 512))
+        yr, Ar = L.shoot(x[sub])
+        assert rel_err(y64[sub] - yr, x[sub] - yr).max() <= ORACLE_ALPHA_TOL, i
+        assert np.abs(A64[sub] - Ar).max() <= 1e-6, i
+        assert rel_err(y32 - y64, x - y64).max() <= FP32C_ALPHA_TOL, i
+        # each realization in the batch equals its lone GridMap
+        g = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP32C, zs=sp["zs"])
+        assert np.array_equal(g.y(), y32) and np.array_equal(g.A(), A32), i
+
+
+# --------------------------------------------------------------- serving form
+@pytest.fixture(scope="module")
+def c5_small(ctx):
+    """6 resident 10-plane lenses of 2^14 particles (particle mass of the
+    2^20-particle models: halo masses scaled by 1/64)."""
+    out = []
+    for r in range(6):
+        lens, parts, _ = W.build_field_lens(ctx, config=5, nplanes=10, n=1 << 14, zs=3.0,
+                                            keep_particles=True, mass_scale=1.0 / 64,
+                                            realization=r)
+        out.append((lens, parts))
+    return out
+
+
+def test_serving_form_stream_matches_oracle_and_lone_shoots(ctx, c5_small):
+    """A Poisson stream of the serving mix (sides scaled down by 8) through
+    the server: every result equals the lone GridMap of its request bit for
+    bit (alone and in the mix), FP32C within the bound of FP64, and FP64
+    within 1e-9 of the oracle on a ray subsample."""
+    n_req = 40
+    lid, zs, cen, side, res = W.c5_stream(n_req, len(c5_small), seed=5901)
+    side = side // 8
+    srv = S.Server(ctx, hbm_budget=4 << 30)
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(0.004, n_req))
+    out = {}
+    lock = threading.Lock()
+
+    def collect(k, rid, N):
+        r = srv.wait(rid, N)
+        with lock:
+            out[k] = r
+
+    th = []
+    t0 = time.perf_counter()
+    for k in range(n_req):
+        d = t0 + arrivals[k] - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        N = int(side[k])
+        prec = S.SLSIM_FP32C if k % 3 else S.SLSIM_FP64
+        rid = srv.submit(c5_small[lid[k]][0], N, cen[k], res * (N - 1), float(zs[k]), prec)
+        t = threading.Thread(target=collect, args=(k, rid, N))
+        t.start()
+        th.append((t, prec))
+    for t, _ in th:
+        t.join()
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == n_req and st["rejected"] == 0
+    for k in range(n_req):
+        lens, parts = c5_small[lid[k]]
+        N = int(side[k])
+        prec = th[k][1]
+        y, A, info = out[k]
+        assert info["status"] == 0
+        g = S.GridMap(lens, N, cen[k], res * (N - 1), prec, zs=float(zs[k]))
+        assert np.array_equal(g.y(), y) and np.array_equal(g.A(), A), k
+        x = g.positions()
+        if prec == S.SLSIM_FP32C:
+            g64 = S.GridMap(lens, N, cen[k], res * (N - 1), S.SLSIM_FP64, zs=float(zs[k]))
+            assert rel_err(y - g64.y(), x - g64.y()).max() <= FP32C_ALPHA_TOL, k
+        elif k % 6 == 0:
+            # the request's source redshift per ray (the lens's own source is 3)
+            L = O.Lens(3.0)
+            for (px, py, pm, ph, z) in parts:
+                L.moveinMainHalo(O.Tree(px, py, pm, ph), z)
+            sub = np.arange(0, N * N, max(1, N * N // 256))
+            yr, Ar = L.shoot(x[sub], zs=np.full(sub.size, float(zs[k])))
+            assert rel_err(y[sub] - yr, x[sub] - yr).max() <= ORACLE_ALPHA_TOL, k

@@ -165,6 +165,29 @@ def test_plane_force_mass_sheet(ctx):
     assert np.allclose(f64[:, 2] - f0[:, 2], -ia * m.astype(np.float64).sum(), rtol=1e-9)
 
 
+def test_plane_force_mixed_sign_masses_with_sheet(ctx):
+    """Masses of both signs (a compensated field: every tenth particle carries
+    a negative mass) with inv_area != 0: cells whose masses nearly cancel
+    have centres of mass far outside them, so the sheet's per-cell
+    correction uses the signed dipole.  FP64 against the oracle, FP32C
+    against FP64."""
+    x, y, m, h = W.c1_particles()
+    m = m.copy()
+    m[::10] *= np.float32(-0.8)
+    span = max(x.max() - x.min(), y.max() - y.min())
+    ia = 1.0 / (0.25 * span * span)
+    dev = S.LensHaloParticles(ctx, x, y, m, h, 0.5, inv_area=ia)
+    rays = _c1_rays(128)
+    f64 = dev.force(rays, S.SLSIM_FP64)
+    o = O.Tree(x, y, m, h, inv_area=ia).force(rays)
+    assert rel_err(f64[:, :2] - o[:, :2], o[:, :2]).max() < 1e-11
+    assert rel_err(f64[:, 2:] - o[:, 2:], o[:, 2:]).max() < 1e-10
+    f32 = dev.force(rays, S.SLSIM_FP32C)
+    assert rel_err(f32[:, :2] - f64[:, :2], f64[:, :2]).max() <= FP32C_ALPHA_TOL
+    assert rel_err(f32[:, 2:3] - f64[:, 2:3], f64[:, 2:3]).max() <= KG_TOL
+    assert rel_err(f32[:, 3:] - f64[:, 3:], f64[:, 3:]).max() <= KG_TOL
+
+
 def test_direct_sum_matches_oracle(ctx):
     x, y, m, h = W.c1_particles()
     dev = S.LensHaloParticles(ctx, x, y, m, h, 0.5)
