@@ -237,6 +237,23 @@ int slsim_shoot_device(slsim_lens lens, size_t n, const double* x_dev, const dou
                        double* y_dev, double* A_dev, int precision, slsim_shoot_stats* stats);
 
 /*
+ * No counterpart in the reference.  slsim_shoot_device for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid hipMalloc):
+ * the same results bit for bit, with every device buffer of the call taken
+ * from one fixed 128 MiB array per device (held from the first context on
+ * the device until the process ends) instead of the context's allocations.
+ * Limits: n <= 131072 rays per call (SLSIM_ERR_ARG beyond); at most 256
+ * planes; the lens's planes must already be on the device (one call of an
+ * allocating entry point after the last change to the lens), else
+ * SLSIM_ERR_ARG.  Every call on a device shares that device's array, which
+ * keeps the last call's contents: the library runs such calls on one device
+ * one at a time; callers must not touch the array's memory themselves.
+ */
+int slsim_shoot_device_noalloc(slsim_lens lens, size_t n, const double* x_dev,
+                               const double* zs_dev, double* y_dev, double* A_dev,
+                               int precision, slsim_shoot_stats* stats);
+
+/*
  * GridMap(LensHndl, N, center, range) (TreeCode_link/gridmap.cpp:93-120): the
  * N x N ray grid x_i = center + range*((i%N, i/N)/(N-1) - 0.5) is generated on
  * the device and shot; y_dev (2 N^2) and A_dev (4 N^2) are device buffers in

@@ -158,6 +158,9 @@ SIGNATURES = {
                             P(c_double), c_int, P(slsim_shoot_stats)]),
     "slsim_shoot_device": (c_int, [slsim_lens, c_size_t, P(c_double), P(c_double), P(c_double),
                                    P(c_double), c_int, P(slsim_shoot_stats)]),
+    "slsim_shoot_device_noalloc": (c_int, [slsim_lens, c_size_t, P(c_double), P(c_double),
+                                           P(c_double), P(c_double), c_int,
+                                           P(slsim_shoot_stats)]),
     "slsim_gridmap_shoot": (c_int, [slsim_lens, c_size_t, c_double, c_double, c_double, c_double,
                                     P(c_double), P(c_double), c_int, P(slsim_shoot_stats)]),
     "slsim_gridmap_count": (c_int, [slsim_lens, c_size_t, c_double, c_double, c_double, c_double,

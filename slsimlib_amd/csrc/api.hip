@@ -17,6 +17,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -56,6 +57,68 @@ double now_ms() {
 }  // namespace slsim
 
 using namespace slsim;
+
+// ------------------------------------------------- no-allocation entry point
+// slsim_shoot_device_noalloc takes every device buffer of a call from one
+// fixed array per device, so that callers that cannot allocate (serving
+// loops and harnesses that forbid hipMalloc) can shoot.  Its address on each
+// device is looked up once, when the first context on that device is
+// created; host code uses that address, never the array's name.
+//
+// Size of one call of n <= 2^17 rays (all of it from the array): 128 B of ray
+// state per slot over n + 63 slots (16.8 MB), Morton keys and indices 24 n B
+// and the radix sort's temporary (about 12 n B); for the precision guard its
+// lists (28 n B) and outputs (48 n B), and its own FP64 shoot of at most n
+// rays over at most max(n + 63, 64 x 4096) slots (33.6 MB) with 36 n B of
+// keys and sort space: under 80 MB in all, 128 MiB held per device.
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+constexpr size_t kNoAllocMaxRays = (size_t)1 << 17;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_setup_mu;
+char* g_noalloc_ptr[kNoAllocMaxDevices];            // the array's address per device
+std::mutex g_noalloc_call_mu[kNoAllocMaxDevices];   // calls on one device in turn
+
+// With `device` current: look the array's address up once for the device.
+int noalloc_setup(int device) {
+  if (device < 0 || device >= kNoAllocMaxDevices) return SLSIM_OK;  // the entry point refuses
+  std::lock_guard<std::mutex> lk(g_noalloc_setup_mu);
+  if (g_noalloc_ptr[device]) return SLSIM_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return SLSIM_ERR_HIP;
+  }
+  g_noalloc_ptr[device] = static_cast<char*>(p);
+  return SLSIM_OK;
+}
+
+// Bump allocation from the fixed array (256-byte aligned pieces); running
+// out is an error, never an allocation.
+struct FixedScratch {
+  static constexpr bool kMayAllocate = false;
+  char* base;
+  size_t cap, off = 0;
+  FixedScratch(char* b, size_t c) : base(b), cap(c) {}
+  template <typename T>
+  int get(T** p, size_t count) {
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    const size_t at = (off + 255) & ~(size_t)255;
+    if (at > cap || bytes > cap - at) {
+      set_error("slsim_shoot_device_noalloc: the call needs more than its fixed scratch array");
+      return SLSIM_ERR_BUDGET;
+    }
+    *p = reinterpret_cast<T*>(base + at);
+    off = at + bytes;
+    return SLSIM_OK;
+  }
+};
+
+}  // namespace
 
 // ------------------------------------------------------------------ context
 int slsim_ctx_s::alloc(void** p, size_t bytes) {
@@ -177,6 +240,8 @@ int slsim_ctx_create(int device, size_t hbm_budget_bytes, slsim_ctx* out) {
   SLSIM_HIP_CHECK(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return arg_error("no such device");
   SLSIM_HIP_CHECK(hipSetDevice(device));
+  int rc = noalloc_setup(device);
+  if (rc) return rc;
   auto c = std::make_unique<slsim_ctx_s>();
   c->device = device;
   size_t fr = 0, tot = 0;
@@ -824,7 +889,7 @@ int shoot_array(slsim_lens L, size_t n, const double* x_dev, const double* zs_de
   }
   uint32_t* err;
   if ((rc = s.get(&err, 1))) return rc;
-  SLSIM_HIP_CHECK(hipMemsetAsync(err, 0, 4, c->stream));
+  launch_zero_u32(c->stream, err, 1);
   // Morton order of the rays: each wavefront gets a compact bundle
   const int nblk = (int)std::min<int64_t>(kMinmaxBlocks, (n + 255) / 256);
   double* part;
@@ -913,8 +978,8 @@ int guard_refine(slsim_lens L, const RayState& st, int64_t nslots, const GuardMa
   uint32_t* count;
   if ((rc = s.get(&acc, 1)) || (rc = s.get(&count, 1)) || (rc = s.get(&part, kGuardBlocks)))
     return rc;
-  SLSIM_HIP_CHECK(hipMemsetAsync(acc, 0, 8, c->stream));
-  SLSIM_HIP_CHECK(hipMemsetAsync(count, 0, 4, c->stream));
+  launch_zero_u32(c->stream, reinterpret_cast<uint32_t*>(acc), 2);
+  launch_zero_u32(c->stream, count, 1);
   launch_guard_sumsq(c->stream, st, nslots, m, part, acc);
   const uint32_t cap = (uint32_t)nrays;
   uint32_t* idx;
@@ -1248,6 +1313,24 @@ int slsim_shoot_device(slsim_lens L, size_t n, const double* x_dev, const double
   int rc = L->upload_trees();
   if (rc) return rc;
   Scratch s(L->ctx);
+  return shoot_array(L, n, x_dev, zs_dev, y_dev, A_dev, precision, stats, s);
+}
+
+int slsim_shoot_device_noalloc(slsim_lens L, size_t n, const double* x_dev, const double* zs_dev,
+                               double* y_dev, double* A_dev, int precision,
+                               slsim_shoot_stats* stats) {
+  if (!L) return arg_error("null lens");
+  if (n > kNoAllocMaxRays)
+    return arg_error("slsim_shoot_device_noalloc: more than 131072 rays in one call");
+  const int dev = L->ctx->device;
+  if (dev >= kNoAllocMaxDevices || !g_noalloc_ptr[dev])
+    return arg_error("slsim_shoot_device_noalloc: no scratch array on this device");
+  if (L->trees_dirty)
+    return arg_error("slsim_shoot_device_noalloc: the lens's planes are not on the device "
+                     "(shoot once with an allocating entry point after changing the lens)");
+  SLSIM_HIP_CHECK(hipSetDevice(dev));
+  std::lock_guard<std::mutex> lk(g_noalloc_call_mu[dev]);
+  FixedScratch s(g_noalloc_ptr[dev], kNoAllocScratchBytes);
   return shoot_array(L, n, x_dev, zs_dev, y_dev, A_dev, precision, stats, s);
 }
 
