@@ -7,6 +7,7 @@ this package fails loudly if it is missing (there is no CPU fallback).
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -49,8 +50,16 @@ class Context:
 
     def __init__(self, device=0, hbm_budget=0):
         self.h = _abi.slsim_ctx()
+        self._children = []    # weak references to the objects made on it, oldest first
         _check(lib.slsim_ctx_create(int(device), ctypes.c_size_t(int(hbm_budget)),
                                     ctypes.byref(self.h)))
+
+    def _adopt(self, obj):
+        """Registers an object that holds memory or handles of this context,
+        so that close() releases it first (newest first)."""
+        if len(self._children) >= 256:
+            self._children = [r for r in self._children if r() is not None]
+        self._children.append(weakref.ref(obj))
 
     def memory(self):
         used, bud = ctypes.c_size_t(), ctypes.c_size_t()
@@ -80,7 +89,17 @@ class Context:
         _check(lib.slsim_memcpy_d2d(self.h, dst, src, ctypes.c_size_t(nbytes)))
 
     def close(self):
+        """Closes every live object made on the context (newest first), then
+        the context; objects still referenced afterwards are inert."""
         if self.h:
+            for r in reversed(self._children):
+                o = r()
+                if o is not None:
+                    try:
+                        o.close()
+                    except Exception:
+                        pass
+            self._children = []
             lib.slsim_ctx_destroy(self.h)
             self.h = None
 
@@ -101,6 +120,7 @@ class LensHaloParticles:
 
     def __init__(self, ctx, x, y, m, h, z, theta=0.1, bucket=20, inv_area=0.0, max_range=-1.0):
         self.ctx = ctx
+        ctx._adopt(self)
         self.z = np.float32(z)
         self.owned_by_lens = False
         m = np.ascontiguousarray(m, dtype=np.float32)
@@ -210,7 +230,7 @@ class LensHaloParticles:
         return out
 
     def close(self):
-        if self.h and not self.owned_by_lens:
+        if self.h and not self.owned_by_lens and self.ctx.h:
             lib.slsim_plane_destroy(self.h)
         self.h = None
 
@@ -246,6 +266,7 @@ class LensHaloHalos:
 
     def __init__(self, ctx, x, y, mass, rsize, concentration, z, inv_area=0.0, theta=0.1):
         self.ctx = ctx
+        ctx._adopt(self)
         self.z = np.float32(z)
         self.owned_by_lens = False
         xs = np.ascontiguousarray(x, dtype=np.float64)
@@ -266,6 +287,7 @@ class LensHaloHalos:
         (param = concentration) or SLSIM_HALO_POWERLAW (param = beta)."""
         self = cls.__new__(cls)
         self.ctx = ctx
+        ctx._adopt(self)
         self.z = np.float32(z)
         self.owned_by_lens = False
         self.h = _halos_plane(ctx, x, y, mass, rsize, kinds, params,
@@ -292,6 +314,7 @@ class LensPlaneTree:
     def __init__(self, ctx, z, x, y, mass, rsize, concentration, sigma_background=0.0,
                  inv_screening_scale=0.0):
         self.ctx = ctx
+        ctx._adopt(self)
         self.z = float(z)
         self.owned_by_lens = False
         xs = np.ascontiguousarray(x, dtype=np.float64)
@@ -312,6 +335,7 @@ class LensPlaneTree:
         param = concentration, SLSIM_HALO_POWERLAW: param = beta)."""
         self = cls.__new__(cls)
         self.ctx = ctx
+        ctx._adopt(self)
         self.z = float(z)
         self.owned_by_lens = False
         scr = float(inv_screening_scale) != 0.0
@@ -331,6 +355,7 @@ class Lens:
 
     def __init__(self, ctx, zsource, Om=0.3, OL=0.7, h=0.7):
         self.ctx = ctx
+        ctx._adopt(self)
         self.cosmo = (Om, OL, h)
         self.h = _abi.slsim_lens()
         self.halos = []
@@ -442,7 +467,8 @@ class Lens:
 
     def close(self):
         if self.h:
-            lib.slsim_lens_destroy(self.h)
+            if self.ctx.h:
+                lib.slsim_lens_destroy(self.h)
             for hl in self.halos:
                 hl.h = None
             self.h = None
@@ -512,6 +538,7 @@ class GridMap:
     def __init__(self, lens, N, center, range_, precision=SLSIM_FP64, zs=0.0, shoot=True):
         self.lens = lens
         self.ctx = lens.ctx
+        self.ctx._adopt(self)
         self.N = int(N)
         self.center = (float(center[0]), float(center[1]))
         self.range = float(range_)
@@ -535,6 +562,7 @@ class GridMap:
         a ray list and the outputs stay in HBM."""
         self = cls.__new__(cls)
         self.lens, self.ctx = lens, lens.ctx
+        self.ctx._adopt(self)
         self.N = int(Nx)
         self.center = (float(center[0]), float(center[1]))
         self.range, self.range_y = float(range_x), float(range_y)
@@ -717,8 +745,9 @@ class GridMap:
 
     def close(self):
         if self.y_dev:
-            self.ctx.free(self.y_dev)
-            self.ctx.free(self.A_dev)
+            if self.ctx.h:
+                self.ctx.free(self.y_dev)
+                self.ctx.free(self.A_dev)
             self.y_dev = self.A_dev = None
 
     def __del__(self):
@@ -736,6 +765,7 @@ class Batch:
 
     def __init__(self, ctx, requests, precision=SLSIM_FP64):
         self.ctx = ctx
+        ctx._adopt(self)
         self.requests = list(requests)
         n = len(self.requests)
         lenses = (_abi.slsim_lens * n)(*[r[0].h for r in self.requests])
@@ -784,7 +814,8 @@ class Batch:
 
     def close(self):
         if self.h:
-            lib.slsim_batch_destroy(self.h)
+            if self.ctx.h:
+                lib.slsim_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -799,6 +830,7 @@ class Server:
 
     def __init__(self, ctx, hbm_budget=0):
         self.ctx = ctx
+        ctx._adopt(self)
         self.h = _abi.slsim_server()
         _check(lib.slsim_server_create(ctx.h, ctypes.c_size_t(int(hbm_budget)),
                                        ctypes.byref(self.h)))
@@ -860,7 +892,8 @@ class Server:
 
     def close(self):
         if self.h:
-            lib.slsim_server_destroy(self.h)
+            if self.ctx.h:
+                lib.slsim_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -853,8 +853,8 @@ int check_err(slsim_ctx c, uint32_t* err) {
 
 template <typename ScratchT>
 int guard_refine(slsim_lens L, const RayState& st, int64_t nslots, const GuardMap& m,
-                 int64_t nrays, const double* zs_dev, double* y_dev, double* A_dev, ScratchT& s,
-                 int32_t* nguard);
+                 int64_t nrays, const double* zs_dev, double zs_uniform, double* y_dev,
+                 double* A_dev, ScratchT& s, int32_t* nguard);
 
 // rayshooterInternal on device buffers (FullRange/internal_rayshooter_multi.cpp:
 // 53-251): rays Morton-ordered into 64-ray bundles, one launch per plane, the
@@ -865,7 +865,7 @@ int guard_refine(slsim_lens L, const RayState& st, int64_t nslots, const GuardMa
 template <typename ScratchT>
 int shoot_array(slsim_lens L, size_t n, const double* x_dev, const double* zs_dev,
                 double* y_dev, double* A_dev, int precision, slsim_shoot_stats* stats,
-                ScratchT& s, int lanes = kWave) {
+                ScratchT& s, int lanes = kWave, double zs_uniform = 0.0) {
   const double t0 = now_ms();
   if (precision < SLSIM_FP64 || precision > SLSIM_DIRECT64) return arg_error("bad precision");
   if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -919,7 +919,7 @@ int shoot_array(slsim_lens L, size_t n, const double* x_dev, const double* zs_de
                                                      30, c->stream));
   int32_t jmax = 0;
   double aal = 0, bbl = 0;
-  if (nplanes > 0 && !zs_dev) uniform_source(L, 0.0, &jmax, &aal, &bbl);
+  if (nplanes > 0 && !zs_dev) uniform_source(L, zs_uniform, &jmax, &aal, &bbl);
   double *zpl, *dl;
   if ((rc = s.get(&zpl, L->plane_redshifts.size())) || (rc = s.get(&dl, L->Dl.size()))) return rc;
   SLSIM_HIP_CHECK(hipMemcpyAsync(zpl, L->plane_redshifts.data(), 8 * L->plane_redshifts.size(),
@@ -956,7 +956,9 @@ int shoot_array(slsim_lens L, size_t n, const double* x_dev, const double* zs_de
     m.zs = zs_dev;
     m.n = (int64_t)n;
     int32_t ng = 0;
-    if ((rc = guard_refine(L, st, nslots, m, (int64_t)n, zs_dev, y_dev, A_dev, s, &ng))) return rc;
+    if ((rc = guard_refine(L, st, nslots, m, (int64_t)n, zs_dev, zs_uniform, y_dev, A_dev, s,
+                           &ng)))
+      return rc;
     if (stats) stats->n_guarded = ng;
   }
   if (stats) stats->total_ms = now_ms() - t0;
@@ -970,8 +972,8 @@ int shoot_array(slsim_lens L, size_t n, const double* x_dev, const double* zs_de
 // replaced.
 template <typename ScratchT>
 int guard_refine(slsim_lens L, const RayState& st, int64_t nslots, const GuardMap& m,
-                 int64_t nrays, const double* zs_dev, double* y_dev, double* A_dev, ScratchT& s,
-                 int32_t* nguard) {
+                 int64_t nrays, const double* zs_dev, double zs_uniform, double* y_dev,
+                 double* A_dev, ScratchT& s, int32_t* nguard) {
   slsim_ctx c = L->ctx;
   int rc;
   double *acc, *part;
@@ -998,7 +1000,8 @@ int guard_refine(slsim_lens L, const RayState& st, int64_t nslots, const GuardMa
   // its few rays need (a full bundle of scattered rays walks their union)
   int lanes = 1;
   while (lanes < kWave && (int64_t)h > (int64_t)lanes * kGuardWaves) lanes *= 2;
-  if ((rc = shoot_array(L, h, xl, zl, yt, At, SLSIM_FP64, nullptr, s, lanes))) return rc;
+  if ((rc = shoot_array(L, h, xl, zl, yt, At, SLSIM_FP64, nullptr, s, lanes, zs_uniform)))
+    return rc;
   launch_guard_scatter(c->stream, h, idx, yt, At, y_dev, A_dev);
   SLSIM_HIP_CHECK(hipGetLastError());
   SLSIM_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1060,7 +1063,7 @@ int gridmap_run(slsim_lens L, size_t N, double c0, double c1, double range, doub
     m.c1 = c1;
     m.range = range;
     int32_t ng = 0;
-    if ((rc = guard_refine(L, st, nslots, m, (int64_t)N * (int64_t)N, nullptr, y_dev, A_dev, s,
+    if ((rc = guard_refine(L, st, nslots, m, (int64_t)N * (int64_t)N, nullptr, zs, y_dev, A_dev, s,
                            &ng)))
       return rc;
     if (stats) stats->n_guarded = ng;
@@ -1096,10 +1099,10 @@ int gridmap_run(slsim_lens L, size_t N, double c0, double c1, double range, doub
 
 namespace slsim {
 int guard_refine_ctx(slsim_lens L, const RayState& st, int64_t nslots, const GuardMap& m,
-                     int64_t nrays, const double* zs_dev, double* y_dev, double* A_dev,
-                     int32_t* nguard) {
+                     int64_t nrays, const double* zs_dev, double zs_uniform, double* y_dev,
+                     double* A_dev, int32_t* nguard) {
   Scratch s(L->ctx);
-  return guard_refine(L, st, nslots, m, nrays, zs_dev, y_dev, A_dev, s, nguard);
+  return guard_refine(L, st, nslots, m, nrays, zs_dev, zs_uniform, y_dev, A_dev, s, nguard);
 }
 }  // namespace slsim
 

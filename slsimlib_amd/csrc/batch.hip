@@ -289,8 +289,8 @@ int slsim_batch_s::run(slsim_shoot_stats* stats) {
       m.c1 = g.c1;
       m.range = g.range;
       int32_t ng = 0;
-      if ((rc = guard_refine_ctx(reqs[r]->lens, g.st, g.nslots, m, g.N * g.N, nullptr, g.y, g.A,
-                                 &ng)))
+      if ((rc = guard_refine_ctx(reqs[r]->lens, g.st, g.nslots, m, g.N * g.N, nullptr,
+                                 reqs[r]->zs, g.y, g.A, &ng)))
         return rc;
       guarded += ng;
     }
@@ -717,7 +717,7 @@ void slsim_server_s::loop() {
       }
       int32_t ng = 0;
       rc = guard_refine_ctx(r->lens, r->st, r->nslots, m, r->nrays > 0 ? r->nrays : r->N * r->N,
-                            r->nrays > 0 ? r->zsd : nullptr, r->y, r->A, &ng);
+                            r->nrays > 0 ? r->zsd : nullptr, r->zs, r->y, r->A, &ng);
     }
     const double t = now_s();
     lk.lock();

@@ -467,10 +467,12 @@ int build_plane(slsim_ctx ctx, const T* x, const T* y, const float* m, const flo
                 slsim_plane pl);
 int export_nodes(slsim_plane pl, double* dbl, int64_t* ints);
 // the precision form's guard over one finished shoot of L (api.hip): flagged
-// and small-|alpha| rays shot again in FP64 on the context stream
+// and small-|alpha| rays shot again in FP64 on the context stream, to the
+// per-ray source redshifts zs_dev or else to zs_uniform (<= 0: the lens's
+// source)
 int guard_refine_ctx(slsim_lens L, const RayState& st, int64_t nslots, const GuardMap& m,
-                     int64_t nrays, const double* zs_dev, double* y_dev, double* A_dev,
-                     int32_t* nguard);
+                     int64_t nrays, const double* zs_dev, double zs_uniform, double* y_dev,
+                     double* A_dev, int32_t* nguard);
 // lens helpers (api.hip)
 void uniform_source(const slsim_lens_s* L, double zs, int32_t* jmax, double* aal, double* bbl);
 void make_steps(const slsim_lens_s* L, const RayState& st, int64_t nslots, uint32_t* counters,

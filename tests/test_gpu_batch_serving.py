@@ -231,3 +231,25 @@ def test_server_ray_requests_equal_rayshooter(ctx, lenses):
         y, A, _ = out[("g", k)]
         assert np.array_equal(y, ys) and np.array_equal(A, As)
     srv.close()
+
+
+def test_context_close_releases_what_was_made_on_it():
+    """Closing a context first closes the live objects made on it (server,
+    batch, grid maps, lens and its planes, newest first); afterwards they are
+    inert, and closing or dropping them again is harmless."""
+    c = S.Context(0)
+    lens, _, _ = W.build_field_lens(c, config=4, nplanes=2, n=1 << 12, mass_scale=1 / 256)
+    loose = S.LensHaloParticles(c, *W.c1_particles(4096), 0.5)
+    g = S.GridMap(lens, 16, (0.0, 0.0), W.grid_range(300.0), S.SLSIM_FP32C)
+    b = S.Batch(c, [(lens, 8, (0.0, 0.0), W.grid_range(300.0), 0.0)], S.SLSIM_FP64)
+    b.run()
+    srv = S.Server(c)
+    rid = srv.submit(lens, 8, (0.0, 0.0), W.grid_range(300.0), 0.0, S.SLSIM_FP64)
+    srv.wait(rid, 8)
+    c.close()
+    for o in (srv, b, lens, loose):
+        assert o.h is None
+    assert g.y_dev is None
+    for o in (srv, b, g, lens, loose):
+        o.close()
+    del srv, b, g, lens, loose

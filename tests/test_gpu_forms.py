@@ -63,6 +63,22 @@ def test_precision_form_fp32c_within_bound_at_every_ray(c3_small):
     assert same.sum() >= guarded
 
 
+def test_precision_form_guard_uses_the_grid_source_redshift(c3_small):
+    """A GridMap to a source redshift other than the lens's (z = 1.3, between
+    lens planes): the guard's FP64 re-shoot goes to that source too, so the
+    guarded rays equal the FP64 GridMap's bit for bit and every ray is
+    within the bound."""
+    N, rg = 384, W.grid_range(300.0)
+    g64 = S.GridMap(c3_small["lens"], N, (0.0, 0.0), rg, S.SLSIM_FP64, zs=1.3)
+    g32 = S.GridMap(c3_small["lens"], N, (0.0, 0.0), rg, S.SLSIM_FP32C, zs=1.3)
+    x = g64.positions()
+    a64, a32 = x - g64.y(), x - g32.y()
+    guarded = g32.stats["n_guarded"]
+    assert guarded > 0
+    assert rel_err(a32 - a64, a64).max() <= FP32C_ALPHA_TOL
+    assert np.all(g32.y() == g64.y(), axis=1).sum() >= guarded
+
+
 def test_precision_form_fp64_matches_oracle(c3_small):
     N, rg = 512, W.grid_range(300.0)
     g64 = S.GridMap(c3_small["lens"], N, (0.0, 0.0), rg, S.SLSIM_FP64)
