@@ -811,15 +811,18 @@ namespace {
 // Runs the plane steps [0, jmax) of one job on the context stream.
 int run_steps(slsim_ctx c, const StepJob* djobs, const StepJob* steps, int jmax,
               uint32_t nbundles, int precision, bool count, uint32_t* err,
-              slsim_shoot_stats* stats) {
+              slsim_shoot_stats* stats, uint32_t lone_rays = 0) {
   const bool timed = stats != nullptr;
   std::vector<hipEvent_t> ev(timed ? 2 * (size_t)jmax : 0);
   if (timed)
     for (auto& e : ev) SLSIM_HIP_CHECK(hipEventCreate(&e));
   for (int j = 0; j < jmax; ++j) {
     if (timed) SLSIM_HIP_CHECK(hipEventRecord(ev[2 * j], c->stream));
-    launch_plane_step(precision, count, nbundles, c->stream, djobs + j, nullptr, err,
-                      steps[j].nhtrees > 0 ? steps[j].hkind : 0);
+    if (lone_rays > 0)
+      launch_plane_step_lone(c->stream, djobs + j, lone_rays, err);
+    else
+      launch_plane_step(precision, count, nbundles, c->stream, djobs + j, nullptr, err,
+                        steps[j].nhtrees > 0 ? steps[j].hkind : 0);
     if (timed) SLSIM_HIP_CHECK(hipEventRecord(ev[2 * j + 1], c->stream));
   }
   SLSIM_HIP_CHECK(hipGetLastError());
@@ -877,6 +880,10 @@ int shoot_array(slsim_lens L, size_t n, const double* x_dev, const double* zs_de
   const int nplanes = (int)L->plane_redshifts.size() - 1;
   if (nplanes > kMaxPlanesFixed && !ScratchT::kMayAllocate)
     return arg_error("too many lens planes for the no-allocation entry point");
+  // lanes == 0: one wavefront per ray (the lone-ray FP64 walk) where every
+  // plane has particle trees only, else full bundles; rays dense in the state
+  const bool lone_asked = lanes == 0;
+  if (lone_asked) lanes = kWave;
   if (lanes < 1 || lanes > kWave) return arg_error("bad rays per bundle");
   const int64_t nbundles = ((int64_t)n + lanes - 1) / lanes;
   const int64_t nslots = nbundles * kWave;
@@ -938,18 +945,20 @@ int shoot_array(slsim_lens L, size_t n, const double* x_dev, const double* zs_de
       steps = steps_more.data();
     }
     make_steps_into(L, st, nslots, nullptr, steps);
+    bool lone = lone_asked && precision == SLSIM_FP64;
+    for (int j = 0; j < steps_to_run && lone; ++j) lone = steps[j].nhtrees == 0;
     StepJob* djobs;
     if ((rc = s.get(&djobs, (size_t)nplanes))) return rc;
     SLSIM_HIP_CHECK(hipMemcpyAsync(djobs, steps, sizeof(StepJob) * (size_t)nplanes,
                                    hipMemcpyHostToDevice, c->stream));
     rc = run_steps(c, djobs, steps, steps_to_run, (uint32_t)nbundles, precision, false, err,
-                   stats);
+                   stats, lone ? (uint32_t)n : 0u);
     if (rc) return rc;
   }
   launch_finalize_array(c->stream, st, nslots, (int64_t)n, perm, y_dev, A_dev, lanes);
   SLSIM_HIP_CHECK(hipGetLastError());
   if ((rc = check_err(c, err))) return rc;
-  if (precision == SLSIM_FP32C && steps_to_run > 0 && lanes == kWave) {
+  if (precision == SLSIM_FP32C && steps_to_run > 0 && lanes == kWave && !lone_asked) {
     GuardMap m{};
     m.perm = perm;
     m.x = x_dev;
@@ -996,10 +1005,12 @@ int guard_refine(slsim_lens L, const RayState& st, int64_t nslots, const GuardMa
   if (h == 0) return SLSIM_OK;
   double *yt, *At;
   if ((rc = s.get(&yt, 2 * (size_t)h)) || (rc = s.get(&At, 4 * (size_t)h))) return rc;
-  // few scattered rays: fewer per bundle, so that each wavefront walks what
-  // its few rays need (a full bundle of scattered rays walks their union)
+  // few scattered rays: one wavefront per ray (the lone-ray walk), else
+  // fewer rays per bundle, so that each wavefront walks what its few rays
+  // need (a full bundle of scattered rays walks their union)
   int lanes = 1;
   while (lanes < kWave && (int64_t)h > (int64_t)lanes * kGuardWaves) lanes *= 2;
+  if ((int64_t)h <= kGuardLoneRays) lanes = 0;
   if ((rc = shoot_array(L, h, xl, zl, yt, At, SLSIM_FP64, nullptr, s, lanes, zs_uniform)))
     return rc;
   launch_guard_scatter(c->stream, h, idx, yt, At, y_dev, A_dev);

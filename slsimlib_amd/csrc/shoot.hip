@@ -1202,6 +1202,190 @@ void k_plane_step_queue32(const StepJob* __restrict__ jobs, const WorkItem* __re
   plane_step_queue<SLSIM_FP32C, false>(jobs, items, nitems, counter, err);
 }
 
+// ------------------------------------------------------- lone-ray FP64 walk
+// One plane step of one ray per wavefront, for a few scattered rays (the
+// precision form's guard), where a bundle walk would leave 63 lanes idle and
+// wait on one lane's stack: the lanes take the ray's cells 64 at a time from
+// an LDS stack (the last pushed first), a lane that opens an internal cell
+// pushes its children, and the particles of the opened leaves and of the big
+// lists the ray is near are then taken 64 at a time.  The cells, the opening
+// test and the terms are the FP64 walk's (include/quadTree.h:990-1128), so
+// the result is the bundle walk's up to the order of the FP64 sums; the
+// lanes' sums are added in a fixed order.
+namespace {
+
+// a pop of up to 64 cells pushes at most 256: 64 + 192 per level of depth
+constexpr int kLoneStack = 64 + 192 * 34;
+
+__device__ inline int wave_prefix(int v, int& total) {
+  const int lane = lane_id();
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  total = __shfl(x, 63, 64);
+  return x - v;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ void lone_walk64(const TreeDesc& t, double rx, double ry, Acc64& a, int32_t* s_cell,
+                            int32_t* s_pst, int32_t* s_pcnt, int32_t* s_pkind, int32_t* s_pre,
+                            uint32_t* __restrict__ err) {
+  const int lane = lane_id();
+  const NodeD* __restrict__ nodes = t.nodesD;
+  const PartD* __restrict__ parts = t.partsD;
+  const PartD* __restrict__ bigs = t.bigD;
+  const double ia = t.inv_area;
+  if (lane == 0) s_cell[0] = 0;
+  int sp = 1;
+  __syncthreads();
+  while (sp > 0) {
+    const int c = sp < kWave ? sp : kWave;
+    sp -= c;
+    const int cell = lane < c ? s_cell[sp + lane] : -1;
+    __syncthreads();
+    int child = 0, cmask = 0, pst = 0, pcnt = 0, pkind = -1;
+    if (cell >= 0) {
+      const NodeD nd = nodes[cell];
+      const double dx = nd.cx - rx, dy = nd.cy - ry;
+      const double r2 = dx * dx + dy * dy;
+      const bool nearc = r2 < nd.near2;
+      if (!(r2 < nd.r2crit || nearc)) {
+        // far cell: monopole + quadrupole (include/quadTree.h:1089-1123)
+        const double inv = rcp64(r2);
+        const double mi = nd.mass * inv * kInvPI;
+        double tmp = nd.mass * ia - mi;
+        a.ax += tmp * dx;
+        a.ay += tmp * dy;
+        tmp = -2.0 * mi * inv;
+        a.g0 += 0.5 * (dx * dx - dy * dy) * tmp;
+        a.g1 += dx * dy * tmp;
+        const double i2p = inv * inv * kInvPI;
+        a.ax -= (nd.q0 * dx + nd.q2 * dy) * i2p;
+        a.ay -= (nd.q1 * dy + nd.q2 * dx) * i2p;
+        tmp = 4 * (nd.q0 * dx * dx + nd.q1 * dy * dy + 2 * nd.q2 * dx * dy) * inv * i2p;
+        a.ax += tmp * dx;
+        a.ay += tmp * dy;
+        a.k -= ia * nd.mass;
+      } else if (nd.child < 0) {
+        pst = nd.start;
+        pcnt = nd.count;
+        pkind = nearc ? 1 : 0;
+      } else {
+        child = nd.child;
+        cmask = nd.cmask;
+        if (nd.count > 0 && nearc) {
+          pst = nd.start;
+          pcnt = nd.count;
+          pkind = 2;
+        }
+      }
+    }
+    // children onto the stack, in lane order
+    int nch_total = 0;
+    int at = sp + wave_prefix(__builtin_popcount(cmask), nch_total);
+    if (sp + nch_total > kLoneStack) {
+      if (lane == 0) atomicOr(err, 1u);
+      return;
+    }
+    for (int k = 0; k < 4; ++k)
+      if ((cmask >> k) & 1) s_cell[at++] = child + k;
+    sp += nch_total;
+    // particle lists: the opened leaves and the near big lists of this pop
+    int nref = 0, np = 0;
+    const bool has = pkind >= 0 && pcnt > 0;
+    const int ri = wave_prefix(has ? 1 : 0, nref);
+    const int pre = wave_prefix(has ? pcnt : 0, np);
+    if (has) {
+      s_pst[ri] = pst;
+      s_pcnt[ri] = pcnt;
+      s_pkind[ri] = pkind;
+      s_pre[ri] = pre;
+    }
+    __syncthreads();
+    for (int q = lane; q < np; q += kWave) {
+      int lo = 0, hi = nref - 1;   // the last list whose prefix is <= q
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pre[mid] <= q) lo = mid;
+        else hi = mid - 1;
+      }
+      const int idx = s_pst[lo] + (q - s_pre[lo]);
+      const int kind = s_pkind[lo];
+      if (kind == 2) {
+        // an internal cell's big particle: its B-spline correction
+        const PartD p = bigs[idx];
+        const double d0 = p.x - rx, d1 = p.y - ry;
+        double rr = d0 * d0 + d1 * d1;
+        if (rr < 1e-20) rr = 1e-20;
+        const double size = p.h;
+        if (rr < 4 * size * size)
+          bspline64(d0, d1, sqrt(rr), (double)p.m, size, a.ax, a.ay, a.k, a.g0, a.g1);
+      } else {
+        // a leaf particle as a point mass (include/quadTree.h:1014-1048),
+        // and its B-spline correction when it is big and the leaf near
+        const PartD p = parts[idx];
+        const double d0 = p.x - rx, d1 = p.y - ry;
+        double rr = d0 * d0 + d1 * d1;
+        if (rr < 1e-20) rr = 1e-20;
+        const double m = p.m;
+        const double inv = rcp64(rr);
+        const double pref = m * inv * kInvPI;
+        a.ax += (ia * m - pref) * d0;
+        a.ay += (ia * m - pref) * d1;
+        const double tmp = -2.0 * pref * inv;
+        a.g0 += 0.5 * (d0 * d0 - d1 * d1) * tmp;
+        a.g1 += d0 * d1 * tmp;
+        a.k -= m * ia;
+        if (kind == 1 && !signbit(p.h)) {
+          const double size = p.h;
+          if (rr < 4 * size * size)
+            bspline64(d0, d1, sqrt(rr), (double)p.m, size, a.ax, a.ay, a.k, a.g0, a.g1);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+// jobs[0]'s plane step for the rays at slots 0 .. nrays-1 (dense state), one
+// wavefront (block) per ray; planes of particle trees only.
+__global__ __launch_bounds__(kWave) void k_plane_step_lone(const StepJob* __restrict__ jobs,
+                                                          uint32_t nrays,
+                                                          uint32_t* __restrict__ err) {
+  __shared__ int32_t s_cell[kLoneStack];
+  __shared__ int32_t s_pst[kWave], s_pcnt[kWave], s_pkind[kWave], s_pre[kWave];
+  const uint32_t r = xcd_block(blockIdx.x, gridDim.x);
+  if (r >= nrays) return;
+  const StepJob jb = as_const(jobs)[0];
+  const int slot = (int)r;
+  const RayState st = state_view(jb.sbase, jb.nslots);
+  if (jb.j >= st.jmax[slot]) return;
+  const double rx = phys_coord(st.th0[slot], jb.Dphys), ry = phys_coord(st.th1[slot], jb.Dphys);
+  double alpha0 = 0, alpha1 = 0, kappa = 0, gamma0 = 0, gamma1 = 0;
+  const int ntrees = uni(jb.ntrees);
+  for (int h = 0; h < ntrees; ++h) {
+    const TreeDesc t = as_const(jb.trees)[h];
+    Acc64 a;
+    lone_walk64(t, rx, ry, a, s_cell, s_pst, s_pcnt, s_pkind, s_pre, err);
+    alpha0 += wave_sum(a.ax);
+    alpha1 += wave_sum(a.ay);
+    kappa += wave_sum(a.k);
+    gamma0 += wave_sum(a.g0);
+    gamma1 += wave_sum(a.g1);
+  }
+  if (lane_id() == 0) recursion_step(jb, slot, alpha0, alpha1, kappa, gamma0, gamma1);
+}
+
 __global__ void k_zero_u32(uint32_t* __restrict__ p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0u;
@@ -1616,6 +1800,10 @@ void launch_plane_step(int precision, bool count, uint32_t nitems, hipStream_t s
   if (halos == 2) launch_step_hk<2>(precision, count, nb, st, jobs, items, nitems, err);
   else if (halos == 1) launch_step_hk<1>(precision, count, nb, st, jobs, items, nitems, err);
   else launch_step_hk<0>(precision, count, nb, st, jobs, items, nitems, err);
+}
+
+void launch_plane_step_lone(hipStream_t st, const StepJob* job, uint32_t nrays, uint32_t* err) {
+  if (nrays > 0) k_plane_step_lone<<<nrays, kWave, 0, st>>>(job, nrays, err);
 }
 
 void launch_zero_u32(hipStream_t st, uint32_t* p, int n) {

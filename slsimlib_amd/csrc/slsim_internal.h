@@ -211,6 +211,8 @@ constexpr double kGuardSmallAlpha = 0.01;
 // the guard's FP64 shoot puts fewer than 64 rays in a bundle while that
 // still leaves at most this many bundles (about the waves the GPU holds)
 constexpr int64_t kGuardWaves = 4096;
+// up to this many re-shot rays: one wavefront per ray
+constexpr int64_t kGuardLoneRays = 8192;
 
 // A call's slots for the guard kernels: x == null: the tiles of an N x N
 // GridMap; else slot s < n holds ray perm[s] (perm == null: ray s) of x,
@@ -419,6 +421,9 @@ struct FlatLCDM;
 // that walk them)
 void launch_plane_step(int precision, bool count, uint32_t nitems, hipStream_t st,
                        const StepJob* jobs, const WorkItem* items, uint32_t* err, int halos);
+// one plane step (the job at `job`) of the rays at slots 0 .. nrays-1, one
+// wavefront per ray (FP64, planes of particle trees only)
+void launch_plane_step_lone(hipStream_t st, const StepJob* job, uint32_t nrays, uint32_t* err);
 void launch_zero_u32(hipStream_t st, uint32_t* p, int n);
 void launch_plane_step_queue(int precision, unsigned nblocks, uint32_t nitems, hipStream_t st,
                              const StepJob* jobs, const WorkItem* items, uint32_t* counter,
