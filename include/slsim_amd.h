@@ -407,7 +407,9 @@ typedef struct {
                              2: requests on the largest planes first                      */
   int32_t graph;          /* 1: replay the whole launch sequence as a captured HIP graph  */
   int32_t blocks_per_cu;  /* persistent grid: blocks per compute unit (queue = 1)         */
-  int32_t pad;
+  int32_t fuse;           /* 1: one launch in which each wavefront takes its bundle
+                             through all of its plane steps (no launch boundary between
+                             steps; order and queue apply to the bundles)              */
 } slsim_schedule;
 
 /* n independent GridMap requests (lens_i, N_i, center_i (2 per request),
@@ -422,9 +424,10 @@ int slsim_batch_get_schedule(slsim_batch b, slsim_schedule* s);
 int slsim_batch_run(slsim_batch b, slsim_shoot_stats* stats);
 /* Device pointers of request i's y (2 N^2) and A (4 N^2), grid order. */
 int slsim_batch_output(slsim_batch b, int i, double** y_dev, double** A_dev);
-/* Autotuner: times schedules (greedy over tile, queue, order, queue grid
- * size; graphs on) and keeps the fastest.  base_ms: the fixed schedule
- * {8x8, one wave per item, request order, no graph}; best_ms: the chosen. */
+/* Autotuner: times schedules (greedy over fused steps, tile, queue, order,
+ * queue grid size; graphs on) and keeps the fastest.  base_ms: the fixed
+ * schedule {8x8, one launch per step, one wave per item, request order, no
+ * graph}; best_ms: the chosen. */
 int slsim_batch_autotune(slsim_batch b, int repeats, slsim_schedule* best, double* base_ms,
                          double* best_ms);
 int slsim_batch_destroy(slsim_batch b);

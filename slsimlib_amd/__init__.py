@@ -780,8 +780,8 @@ class Batch:
                                       _ptr(zs, ctypes.c_double), int(precision),
                                       ctypes.byref(self.h)))
 
-    def set_schedule(self, tile_log2=3, queue=0, order=0, graph=0, blocks_per_cu=4):
-        s = _abi.slsim_schedule(tile_log2, queue, order, graph, blocks_per_cu, 0)
+    def set_schedule(self, tile_log2=3, queue=0, order=0, graph=0, blocks_per_cu=4, fuse=0):
+        s = _abi.slsim_schedule(tile_log2, queue, order, graph, blocks_per_cu, fuse)
         _check(lib.slsim_batch_set_schedule(self.h, ctypes.byref(s)))
 
     def schedule(self):

@@ -81,7 +81,7 @@ class slsim_shoot_stats(ctypes.Structure):
 class slsim_schedule(ctypes.Structure):
     _fields_ = [("tile_log2", ctypes.c_int32), ("queue", ctypes.c_int32),
                 ("order", ctypes.c_int32), ("graph", ctypes.c_int32),
-                ("blocks_per_cu", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("blocks_per_cu", ctypes.c_int32), ("fuse", ctypes.c_int32)]
 
 
 class slsim_request_info(ctypes.Structure):

@@ -88,6 +88,8 @@ struct slsim_batch_s {
   int64_t total_slots = 0;
   std::vector<GridReq> greq;
   DevBlock d_greq, d_jobs, d_items, d_ctr, d_err;
+  uint32_t nfused = 0;        // fused schedule: bundles (d_items holds FusedItem)
+  char fused_halos = 0;
   std::vector<size_t> job_off, item_off;
   std::vector<uint32_t> nitems;
   std::vector<char> step_halos;   // per step: the largest StepJob::hkind of its jobs
@@ -101,6 +103,7 @@ struct slsim_batch_s {
     graph = nullptr;
   }
   int prepare();
+  int prepare_fused(const std::vector<std::vector<StepJob>>& steps, const std::vector<int>& ord);
   int enqueue();
   int run(slsim_shoot_stats* stats);
   ~slsim_batch_s() { drop_graph(); }
@@ -151,6 +154,13 @@ int slsim_batch_s::prepare() {
     make_steps(q.lens, g.st, nslots, nullptr, steps[r]);
     max_steps = std::max(max_steps, (int)q.jmax);
   }
+  std::vector<int> ord(reqs.size());
+  for (size_t r = 0; r < ord.size(); ++r) ord[r] = (int)r;
+  if (sched.order == 2)
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+      return reqs[a]->nplane_max * greq[a].nslots > reqs[b]->nplane_max * greq[b].nslots;
+    });
+  if (sched.fuse) return prepare_fused(steps, ord);
   // per step: jobs of the requests with a plane left, and their work items
   std::vector<StepJob> all_jobs;
   std::vector<WorkItem> all_items;
@@ -158,12 +168,6 @@ int slsim_batch_s::prepare() {
   item_off.assign(max_steps + 1, 0);
   nitems.assign(max_steps, 0);
   step_halos.assign(max_steps, 0);
-  std::vector<int> ord(reqs.size());
-  for (size_t r = 0; r < ord.size(); ++r) ord[r] = (int)r;
-  if (sched.order == 2)
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
-      return reqs[a]->nplane_max * greq[a].nslots > reqs[b]->nplane_max * greq[b].nslots;
-    });
   for (int j = 0; j < max_steps; ++j) {
     job_off[j] = all_jobs.size();
     item_off[j] = all_items.size();
@@ -209,8 +213,61 @@ int slsim_batch_s::prepare() {
   return SLSIM_OK;
 }
 
-// The launch sequence of one batch run (init, one launch per plane step,
-// finalize); captured into a graph when the schedule asks for it.
+// The fused schedule's lists: each request's plane steps contiguous, and one
+// item per bundle that runs them in order (requests in `ord`; order 1
+// interleaves the requests bundle by bundle).
+int slsim_batch_s::prepare_fused(const std::vector<std::vector<StepJob>>& steps,
+                                 const std::vector<int>& ord) {
+  std::vector<StepJob> all_jobs;
+  std::vector<uint32_t> first(reqs.size(), 0);
+  fused_halos = 0;
+  for (int r : ord) {
+    first[r] = (uint32_t)all_jobs.size();
+    for (int j = 0; j < reqs[r]->jmax; ++j) {
+      all_jobs.push_back(steps[r][j]);
+      if (steps[r][j].nhtrees > 0)
+        fused_halos = (char)std::max<int>(fused_halos, steps[r][j].hkind);
+    }
+  }
+  std::vector<FusedItem> items;
+  auto add = [&](int r, int64_t b) {
+    items.push_back({first[r], (uint32_t)b, (uint32_t)reqs[r]->jmax, 0u});
+  };
+  if (sched.order == 1) {
+    int64_t maxb = 0;
+    for (int r : ord) maxb = std::max(maxb, greq[r].nslots / kWave);
+    for (int64_t b = 0; b < maxb; ++b)
+      for (int r : ord)
+        if (reqs[r]->jmax > 0 && b < greq[r].nslots / kWave) add(r, b);
+  } else {
+    for (int r : ord)
+      if (reqs[r]->jmax > 0)
+        for (int64_t b = 0; b < greq[r].nslots / kWave; ++b) add(r, b);
+  }
+  nfused = (uint32_t)items.size();
+  int rc;
+  if ((rc = d_greq.get(ctx, sizeof(GridReq) * std::max<size_t>(greq.size(), 1))) ||
+      (rc = d_jobs.get(ctx, sizeof(StepJob) * std::max<size_t>(all_jobs.size(), 1))) ||
+      (rc = d_items.get(ctx, sizeof(FusedItem) * std::max<size_t>(items.size(), 1))) ||
+      (rc = d_ctr.get(ctx, 64)) || (rc = d_err.get(ctx, 64)))
+    return rc;
+  hipStream_t st = ctx->stream;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(d_greq.p, greq.data(), sizeof(GridReq) * greq.size(),
+                                 hipMemcpyHostToDevice, st));
+  if (!all_jobs.empty())
+    SLSIM_HIP_CHECK(hipMemcpyAsync(d_jobs.p, all_jobs.data(), sizeof(StepJob) * all_jobs.size(),
+                                   hipMemcpyHostToDevice, st));
+  if (!items.empty())
+    SLSIM_HIP_CHECK(hipMemcpyAsync(d_items.p, items.data(), sizeof(FusedItem) * items.size(),
+                                   hipMemcpyHostToDevice, st));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(st));
+  prepared = true;
+  return SLSIM_OK;
+}
+
+// The launch sequence of one batch run (init, one launch per plane step or
+// the fused launch, finalize); captured into a graph when the schedule asks
+// for it.
 int slsim_batch_s::enqueue() {
   hipStream_t st = ctx->stream;
   uint32_t* err = static_cast<uint32_t*>(d_err.p);
@@ -218,7 +275,12 @@ int slsim_batch_s::enqueue() {
   launch_init_batch(st, static_cast<GridReq*>(d_greq.p), (int)greq.size(), total_slots);
   const StepJob* jobs = static_cast<StepJob*>(d_jobs.p);
   const WorkItem* items = static_cast<WorkItem*>(d_items.p);
-  for (int j = 0; j < max_steps; ++j) {
+  if (sched.fuse)
+    launch_plane_steps_fused(
+        precision, sched.queue ? (unsigned)(kNumCU * std::max(1, sched.blocks_per_cu)) : 0u,
+        nfused, st, jobs, static_cast<FusedItem*>(d_items.p), static_cast<uint32_t*>(d_ctr.p),
+        err, fused_halos);
+  for (int j = 0; j < (sched.fuse ? 0 : max_steps); ++j) {
     if (nitems[j] == 0) continue;
     if (sched.queue)
       launch_plane_step_queue(precision, (unsigned)(kNumCU * std::max(1, sched.blocks_per_cu)),
@@ -340,6 +402,7 @@ int slsim_batch_create(slsim_ctx ctx, int n, const slsim_lens* lenses, const int
 int slsim_batch_set_schedule(slsim_batch b, const slsim_schedule* s) {
   if (!b || !s) return err_arg("null argument");
   if (s->tile_log2 < 1 || s->tile_log2 > 5) return err_arg("tile_log2 must be in 1..5");
+  if (s->fuse != 0 && s->fuse != 1) return err_arg("fuse must be 0 or 1");
   b->sched = *s;
   b->prepared = false;
   b->drop_graph();
@@ -390,19 +453,21 @@ int slsim_batch_autotune(slsim_batch b, int repeats, slsim_schedule* best, doubl
   cur.graph = 1;
   double tc;
   if ((rc = time_of(cur, &tc))) return rc;
-  // greedy passes over the knobs
+  // greedy passes over the knobs: fused steps, tile, queue, order, queue grid
+  const int fuses[] = {0, 1};
   const int tiles[] = {3, 2, 4};
   const int queues[] = {0, 1};
   const int orders[] = {0, 1, 2};
   const int bpcs[] = {2, 4, 8};
-  for (int knob = 0; knob < 4; ++knob) {
-    const int* vals = knob == 0 ? tiles : knob == 1 ? queues : knob == 2 ? orders : bpcs;
-    const int nv = knob == 0 ? 3 : knob == 1 ? 2 : 3;
-    if (knob == 3 && !cur.queue) continue;
+  for (int knob = 0; knob < 5; ++knob) {
+    const int* vals = knob == 0 ? fuses : knob == 1 ? tiles : knob == 2 ? queues
+                    : knob == 3 ? orders : bpcs;
+    const int nv = (knob == 0 || knob == 2) ? 2 : 3;
+    if (knob == 4 && !cur.queue) continue;
     for (int v = 0; v < nv; ++v) {
       slsim_schedule s = cur;
-      int* f = knob == 0 ? &s.tile_log2 : knob == 1 ? &s.queue : knob == 2 ? &s.order
-                                                                            : &s.blocks_per_cu;
+      int* f = knob == 0 ? &s.fuse : knob == 1 ? &s.tile_log2 : knob == 2 ? &s.queue
+             : knob == 3 ? &s.order : &s.blocks_per_cu;
       if (*f == vals[v]) continue;
       *f = vals[v];
       double t;

@@ -1171,6 +1171,53 @@ __device__ __forceinline__ void plane_step_queue(const StepJob* __restrict__ job
   }
 }
 
+// Fused steps: each wavefront takes a bundle through all of its plane steps
+// (the state it writes at one step is its own input at the next), so that
+// bundles of different requests and planes overlap without a launch
+// boundary between steps.  counter == null: one wavefront per item.
+template <int MODE, int HK>
+__device__ __forceinline__ void plane_steps_fused(const StepJob* __restrict__ jobs,
+                                                  const FusedItem* __restrict__ items,
+                                                  uint32_t nitems, uint32_t* __restrict__ counter,
+                                                  uint32_t* __restrict__ err) {
+  __shared__ int32_t s_node[kWavesPerBlock][kStackDepth];
+  __shared__ uint64_t s_mask[kWavesPerBlock][kStackDepth];
+  __shared__ uint64_t s_near[kWavesPerBlock][kNearQueue * kWave];
+  const int w = uni((int)(threadIdx.x >> 6));
+  const int lane = lane_id();
+  uint32_t idx = xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (uint32_t)w;
+  for (;;) {
+    if (counter) {
+      uint32_t k = 0;
+      if (lane == 0) k = atomicAdd(counter, 1u);
+      idx = (uint32_t)uni((int)k);
+    }
+    if (idx >= nitems) break;
+    const uint32_t job0 = as_const(items)[idx].job0;
+    const uint32_t bundle = as_const(items)[idx].bundle;
+    const uint32_t nj = as_const(items)[idx].njobs;
+    for (uint32_t j = 0; j < nj; ++j)
+      process_item<MODE, false, HK>(jobs, WorkItem{job0 + j, bundle}, s_node[w], s_mask[w],
+                                    s_near[w], err);
+    if (!counter) break;
+  }
+}
+
+template <int MODE, int HK>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_steps_fused(
+    const StepJob* __restrict__ jobs, const FusedItem* __restrict__ items, uint32_t nitems,
+    uint32_t* __restrict__ counter, uint32_t* __restrict__ err) {
+  plane_steps_fused<MODE, HK>(jobs, items, nitems, counter, err);
+}
+
+// the FP32C walk at 8 waves per SIMD, as k_plane_step32
+__global__ __launch_bounds__(kWave * kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(8)))
+void k_plane_steps_fused32(const StepJob* __restrict__ jobs, const FusedItem* __restrict__ items,
+                           uint32_t nitems, uint32_t* __restrict__ counter,
+                           uint32_t* __restrict__ err) {
+  plane_steps_fused<SLSIM_FP32C, 0>(jobs, items, nitems, counter, err);
+}
+
 template <int MODE, bool COUNT, int HK>
 __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_plane_step(
     const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items, uint32_t nitems,
@@ -1800,6 +1847,38 @@ void launch_plane_step(int precision, bool count, uint32_t nitems, hipStream_t s
   if (halos == 2) launch_step_hk<2>(precision, count, nb, st, jobs, items, nitems, err);
   else if (halos == 1) launch_step_hk<1>(precision, count, nb, st, jobs, items, nitems, err);
   else launch_step_hk<0>(precision, count, nb, st, jobs, items, nitems, err);
+}
+
+template <int HK>
+void launch_fused_hk(int precision, unsigned nb, uint32_t nitems, hipStream_t st,
+                     const StepJob* jobs, const FusedItem* items, uint32_t* counter,
+                     uint32_t* err) {
+  const dim3 blk(kWave * kWavesPerBlock);
+  if (precision == SLSIM_FP32C && HK == 0)
+    k_plane_steps_fused32<<<nb, blk, 0, st>>>(jobs, items, nitems, counter, err);
+  else if (precision == SLSIM_FP32C)
+    k_plane_steps_fused<SLSIM_FP32C, HK><<<nb, blk, 0, st>>>(jobs, items, nitems, counter, err);
+  else if (precision == SLSIM_DIRECT64)
+    k_plane_steps_fused<SLSIM_DIRECT64, HK><<<nb, blk, 0, st>>>(jobs, items, nitems, counter,
+                                                                  err);
+  else
+    k_plane_steps_fused<SLSIM_FP64, HK><<<nb, blk, 0, st>>>(jobs, items, nitems, counter, err);
+}
+
+void launch_plane_steps_fused(int precision, unsigned nblocks, uint32_t nitems, hipStream_t st,
+                              const StepJob* jobs, const FusedItem* items, uint32_t* counter,
+                              uint32_t* err, int halos) {
+  if (nitems == 0) return;
+  unsigned nb = nblocks;
+  if (nb == 0) {
+    nb = blocks_for(nitems, kWavesPerBlock);
+    counter = nullptr;
+  } else {
+    launch_zero_u32(st, counter, 1);
+  }
+  if (halos == 2) launch_fused_hk<2>(precision, nb, nitems, st, jobs, items, counter, err);
+  else if (halos == 1) launch_fused_hk<1>(precision, nb, nitems, st, jobs, items, counter, err);
+  else launch_fused_hk<0>(precision, nb, nitems, st, jobs, items, counter, err);
 }
 
 void launch_plane_step_lone(hipStream_t st, const StepJob* job, uint32_t nrays, uint32_t* err) {

@@ -210,7 +210,7 @@ constexpr double kGuardSafety = 2.0;
 constexpr double kGuardSmallAlpha = 0.01;
 // the guard's FP64 shoot puts fewer than 64 rays in a bundle while that
 // still leaves at most this many bundles (about the waves the GPU holds)
-constexpr int64_t kGuardWaves = 4096;
+constexpr int64_t kGuardWaves = 8192;
 // up to this many re-shot rays: one wavefront per ray
 constexpr int64_t kGuardLoneRays = 8192;
 
@@ -230,6 +230,15 @@ struct GuardMap {
 struct WorkItem {
   uint32_t job;
   uint32_t bundle;
+};
+
+// A bundle taken through all of its plane steps by one wavefront: jobs
+// job0 .. job0 + njobs - 1 in order (a batch's fused schedule).
+struct FusedItem {
+  uint32_t job0;
+  uint32_t bundle;
+  uint32_t njobs;
+  uint32_t pad;
 };
 
 // One GridMap request of a batch: its grid, source and state.
@@ -424,6 +433,12 @@ void launch_plane_step(int precision, bool count, uint32_t nitems, hipStream_t s
 // one plane step (the job at `job`) of the rays at slots 0 .. nrays-1, one
 // wavefront per ray (FP64, planes of particle trees only)
 void launch_plane_step_lone(hipStream_t st, const StepJob* job, uint32_t nrays, uint32_t* err);
+// every plane step of the listed bundles in one launch, each wavefront taking
+// its bundles through their steps (nblocks == 0: one wavefront per item,
+// else a persistent grid of nblocks blocks on a work queue at `counter`)
+void launch_plane_steps_fused(int precision, unsigned nblocks, uint32_t nitems, hipStream_t st,
+                              const StepJob* jobs, const FusedItem* items, uint32_t* counter,
+                              uint32_t* err, int halos);
 void launch_zero_u32(hipStream_t st, uint32_t* p, int n);
 void launch_plane_step_queue(int precision, unsigned nblocks, uint32_t nitems, hipStream_t st,
                              const StepJob* jobs, const WorkItem* items, uint32_t* counter,

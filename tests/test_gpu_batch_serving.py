@@ -47,7 +47,9 @@ def test_batch_equals_single_gridmaps(ctx, lenses, prec):
     ref = [_single(r, prec) for r in reqs]
     b = S.Batch(ctx, reqs, prec)
     for sched in ({}, {"tile_log2": 2, "queue": 1, "order": 1, "graph": 1},
-                  {"tile_log2": 4, "queue": 0, "order": 2, "graph": 1}):
+                  {"tile_log2": 4, "queue": 0, "order": 2, "graph": 1},
+                  {"fuse": 1}, {"fuse": 1, "queue": 1, "order": 1, "graph": 1},
+                  {"fuse": 1, "tile_log2": 2, "order": 2, "graph": 1}):
         b.set_schedule(**sched)
         for _ in range(2):            # graph replays too
             b.run()

@@ -127,7 +127,7 @@ def test_batched_form_matches_oracle(ctx, c4_small):
     b64 = S.Batch(ctx, reqs, S.SLSIM_FP64)
     b64.run()
     b32 = S.Batch(ctx, reqs, S.SLSIM_FP32C)
-    b32.set_schedule(tile_log2=2, queue=1, order=2, graph=1)
+    b32.set_schedule(tile_log2=2, queue=1, order=2, graph=1, fuse=1)
     b32.run()
     b32.run()
     for i, (sp, lens, L) in enumerate(c4_small):

@@ -126,7 +126,8 @@ def test_batch_and_server_with_halo_planes(ctx):
             ref.append((g.y(), g.A()))
             g.close()
         b = S.Batch(ctx, reqs, prec)
-        for sched in ({}, {"tile_log2": 2, "queue": 1, "order": 1, "graph": 1}):
+        for sched in ({}, {"tile_log2": 2, "queue": 1, "order": 1, "graph": 1},
+                      {"fuse": 1, "queue": 1, "graph": 1}):
             b.set_schedule(**sched)
             b.run()
             for i in range(len(reqs)):
