@@ -207,6 +207,7 @@ TreeDesc slsim_plane_s::desc() const {
   t.nodesF = nodesF;
   t.partsD = partsD;
   t.bigD = bigD;
+  t.partsF = partsF;
   t.inv_area = inv_area;
   t.mass = mass;
   t.comx = com[0];
@@ -284,9 +285,8 @@ int slsim_ctx_synchronize(slsim_ctx ctx) {
 // ------------------------------------------------------------------ planes
 static int plane_free_device(slsim_plane p) {
   slsim_ctx c = p->ctx;
-  void* ptrs[] = {p->nodesD, p->nodesF, p->partsD, p->bigD, p->box_p1, p->box_p2,
-                  p->nbig,   p->npart,  p->hnodes, p->hbig,  p->halos,  p->nfw,
-                  p->sheetd};
+  void* ptrs[] = {p->nodesD, p->nodesF, p->partsD, p->bigD,  p->partsF, p->box_p1, p->box_p2,
+                  p->nbig,   p->npart,  p->hnodes, p->hbig, p->halos,  p->nfw,    p->sheetd};
   for (void* q : ptrs)
     if (q) c->release(q);
   return SLSIM_OK;

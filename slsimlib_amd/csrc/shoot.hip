@@ -700,9 +700,11 @@ __device__ __forceinline__ void guard_open(cptr<NodeF> nd, double r2, double T, 
   if (mk) risk += lane_in(mk) ? fabsf(nd->flip) : 0.f;
 }
 
-// |r^2 - near2| within the band, from the FP32 r^2 (its rounding allowed for)
-__device__ inline bool near_marginal(float r2f, float nef, double D) {
-  return fabsf(r2f - nef) < sqrtf(nef * (float)D) + nef * (1.0f / 4194304.0f);
+// |r^2 - near2| within the band sqrt(near2) 2 dx, from the FP32 r^2 (its
+// rounding allowed for).  sD: 2 dx (1 + 2^-20), the margin covering the
+// hardware square root's error.
+__device__ inline bool near_marginal(float r2f, float nef, float sD) {
+  return fabsf(r2f - nef) < fmaf(__builtin_amdgcn_sqrtf(nef), sD, nef * (1.0f / 4194304.0f));
 }
 
 // The FP32 sums are updated with the same instructions on every path: a
@@ -730,34 +732,52 @@ __device__ __forceinline__ void far32(const NodeF& nd, bool far, float ex, float
   a.pg1 = fmaf(w * ex, ey, a.pg1);
 }
 
-// Particles of an opened leaf; particle - ray offsets formed in FP64.
+// Particles of an opened leaf.  particle - ray = (particle - leaf centre,
+// FP32 from the build) + (centre - ray as the FP32 pair ex + exl of the
+// walk's FP64 offset), so that no FP64 arithmetic runs per particle; the
+// offsets are good to about 2^-24 of the leaf size.  A particle that may lie
+// within 2h (the near field, whose B-spline terms are sensitive to the
+// offset) takes its offset again from the FP64 positions, as the FP64 walk
+// forms it, before the near-field rules apply.
 template <bool GUARD>
-__device__ __forceinline__ void leaf32(int start, int count, bool open, bool nearc, double rx,
-                                       double ry, cptr<PartD> parts, Acc32& a, float r2f,
-                                       float nef, double D, float& risk) {
+__device__ __forceinline__ void leaf32(int start, int count, bool open, bool nearc, float ex,
+                                       float ey, float exl, float eyl, double rx, double ry,
+                                       cptr<PartF> parts, cptr<PartD> partsd, Acc32& a,
+                                       float r2f, float nef, float sD, float& risk) {
 #pragma unroll 2
   for (int k = 0; k < count; ++k) {
-    const PartD p = parts[start + k];
-    const float d0 = (float)(p.x - rx), d1 = (float)(p.y - ry);
-    const float d1s = d1 * d1;
-    const float rr = fmaxf(fmaf(d0, d0, d1s), 1e-20f);
-    const float hs = fabsf(p.h);
-    const float inv = open ? rcp32(rr) : 0.f;       // 0: no term for this lane
+    const PartF p = parts[start + k];
+    float d0 = (p.lx + ex) + exl, d1 = (p.ly + ey) + eyl;
+    float d1s = d1 * d1;
+    float rr = fmaxf(fmaf(d0, d0, d1s), 1e-20f);
+    float inv = open ? rcp32(rr) : 0.f;             // 0: no term for this lane
     float s = p.m * inv, w = s * inv;              // point mass
-    if (open && rr < 4.f * hs * hs) {
-      // a big particle within 2h: its correction hangs on the near test
-      if (GUARD && !signbit(p.h) && near_marginal(r2f, nef, D)) risk = __builtin_inff();
-      if (nearc && !signbit(p.h)) {
-        // smoothed: the B-spline correction folded into the point mass
-        float Mr2, sigpi;
-        bspline_fused32(rr, inv, hs, Mr2, sigpi);
-        s = p.m * Mr2;
-        const float sm = p.m * sigpi;
-        w = (s - sm) * inv;
-        a.kappa(sm);
-      } else if (a.push((uint32_t)(start + k))) {
-        s = 0.f;    // deferred to FP64
-        w = 0.f;
+    if (open && rr < fabsf(p.hh4)) {
+      const PartD q = partsd[start + k];
+      d0 = (float)(q.x - rx);
+      d1 = (float)(q.y - ry);
+      d1s = d1 * d1;
+      rr = fmaxf(fmaf(d0, d0, d1s), 1e-20f);
+      inv = rcp32(rr);
+      s = p.m * inv;
+      w = s * inv;
+      const float hs = fabsf(q.h);
+      if (rr < 4.f * hs * hs) {
+        // a big particle within 2h: its correction hangs on the near test
+        const bool listed = !signbit(q.h);
+        if (GUARD && listed && near_marginal(r2f, nef, sD)) risk = __builtin_inff();
+        if (nearc && listed) {
+          // smoothed: the B-spline correction folded into the point mass
+          float Mr2, sigpi;
+          bspline_fused32(rr, inv, hs, Mr2, sigpi);
+          s = p.m * Mr2;
+          const float sm = p.m * sigpi;
+          w = (s - sm) * inv;
+          a.kappa(sm);
+        } else if (a.push((uint32_t)(start + k))) {
+          s = 0.f;    // deferred to FP64
+          w = 0.f;
+        }
       }
     }
     a.pax = fmaf(-s, d0, a.pax);
@@ -795,12 +815,14 @@ __device__ __forceinline__ void bigs32(int start, int count, bool nearc, double 
 
 // Walk of the trees of one plane, FP32 arithmetic with compensated
 // accumulation.  One loop serves every tree: when the stack is empty the next
-// tree starts with its root as the only child of a virtual entry (the cell
-// arrays carry three zero cells of padding for the loads of its siblings).
+// tree starts with its root as the only child of a virtual entry.  The
+// children of a popped entry are taken last to first, each in full (far
+// term, leaf particles or big list, push when opened), so that child 0's
+// subtree is walked next; empty children cost a scalar branch.
 template <bool GUARD>
 __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx, double ry,
                        bool active, Acc32& a, int32_t* s_node, uint64_t* s_mask,
-                       uint64_t* s_near, uint32_t* err, double D, float& risk) {
+                       uint64_t* s_near, uint32_t* err, double D, float sD, float& risk) {
   const int lane = lane_id();
   a.q = s_near;
   a.nq = 0;
@@ -808,7 +830,8 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
   if (amask == 0) return;
   int h = 0, sp = 0;
   cptr<NodeF> nodes = as_const(as_const(trees)->nodesF);
-  cptr<PartD> parts = as_const(as_const(trees)->partsD);
+  cptr<PartF> parts = as_const(as_const(trees)->partsF);
+  cptr<PartD> partsd = as_const(as_const(trees)->partsD);
   cptr<PartD> bigs = as_const(as_const(trees)->bigD);
   cptr<float2> sheetd = as_const(as_const(trees)->sheetd);
   a.tree = 0;
@@ -818,85 +841,61 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
     const bool lv = lane_in(live);
     const int base = entry & 0x0fffffff;
     const int cm = (entry >> 28) & 0xf;
-    // the four children: opening tests (their loads issued together, lane
-    // masks in scalar registers), then far terms; leaves and big-particle
-    // lists after, in one loop each
-    float exk[4], eyk[4], r2k[4];
-    uint64_t om[4], nm[4];
+    int lists = 0;                // opened internal children with a big-particle list
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const double cx = nodes[base + k].cx, cy = nodes[base + k].cy;
-      const double T = nodes[base + k].T, ne = nodes[base + k].near2;
-      const double dx = cx - rx, dy = cy - ry;
+    for (int k = 3; k >= 0; --k) {
+      if (!((cm >> k) & 1)) continue;
+      const cptr<NodeF> nd = nodes + base + k;
+      const double dx = nd->cx - rx, dy = nd->cy - ry;
       const double r2 = dx * dx + dy * dy;
-      const uint64_t lk = ((cm >> k) & 1) ? live : 0ull;
-      nm[k] = lanes_lt(r2, ne) & lk;
-      om[k] = lanes_lt(r2, T) & lk;   // r^2 < r2crit or r^2 < near2
-      exk[k] = (float)dx;
-      eyk[k] = (float)dy;
-      r2k[k] = (float)r2;
-      if (GUARD && lk) guard_open(nodes + base + k, r2, T, lk, D, risk);
-    }
-    int32_t pe[4];
-    int leaves = 0, lists = 0;    // children (bits) with particles or a big list to take
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      pe[k] = 0;
-      if ((cm >> k) & 1) {
-        const NodeF nd = nodes[base + k];
-        const uint64_t fm = live & ~om[k];
-        if (fm) {
-          far32(nd, lane_in(fm), exk[k], eyk[k], r2k[k], a);
-          if (sheetd) {
-            // masses of both signs: the far cell's share of the mass sheet
-            const float2 dl = sheetd[base + k];
-            const float sel = lane_in(fm) ? 1.f : 0.f;
-            a.pax = fmaf(sel, dl.x, a.pax);
-            a.pay = fmaf(sel, dl.y, a.pay);
-          }
+      const double T = nd->T;
+      const uint64_t om = lanes_lt(r2, T) & live;   // r^2 < r2crit or r^2 < near2
+      if (GUARD) guard_open(nd, r2, T, live, D, risk);
+      const float ex = (float)dx, ey = (float)dy;
+      const uint64_t fm = live & ~om;
+      if (fm) {
+        far32(*nd, lane_in(fm), ex, ey, (float)r2, a);
+        if (sheetd) {
+          // masses of both signs: the far cell's share of the mass sheet
+          const float2 dl = sheetd[base + k];
+          const float sel = lane_in(fm) ? 1.f : 0.f;
+          a.pax = fmaf(sel, dl.x, a.pax);
+          a.pay = fmaf(sel, dl.y, a.pay);
         }
-        if (om[k]) {
-          if (nd.child < 0) {
-            leaves |= 1 << k;
-          } else {
-            pe[k] = nd.child | (nd.count_cmask & 0xf0000000);
-            if ((nd.count_cmask & 0x0fffffff) > 0 && nm[k]) lists |= 1 << k;
-            if (GUARD && (nd.count_cmask & 0x0fffffff) > 0) {
-              // a big list: its corrections hang on the near test
-              const uint64_t lk = ((cm >> k) & 1) ? live : 0ull;
-              const bool mg = lane_in(lk) && near_marginal(r2k[k], (float)nd.near2, D);
-              if (mg) risk = __builtin_inff();
-            }
+      }
+      if (om) {
+        const int32_t child = nd->child;
+        const int32_t cc = nd->count_cmask;
+        const int cnt = cc & 0x0fffffff;
+        if (child < 0) {
+          const uint64_t nm = lanes_lt(r2, nd->near2) & om;
+          const float exl = (float)(dx - (double)ex), eyl = (float)(dy - (double)ey);
+          leaf32<GUARD>(nd->start, cnt, lane_in(om), lane_in(nm), ex, ey, exl, eyl, rx, ry,
+                        parts, partsd, a, (float)r2, (float)nd->near2, sD, risk);
+        } else {
+          if (cnt > 0) {
+            // a big list: its corrections hang on the near test
+            if (GUARD && lane_in(live) && near_marginal((float)r2, (float)nd->near2, sD))
+              risk = __builtin_inff();
+            lists |= 1 << k;
           }
+          push_entry(s_node, s_mask, sp, lane, child | (cc & (int32_t)0xf0000000), om, err);
         }
       }
     }
+    // big-particle lists of the opened internal children, for the lanes
+    // within their near radius (r^2 < near2 <= T: such lanes opened the cell)
 #pragma nounroll
     for (int k = 0; k < 4; ++k) {
-      if ((leaves >> k) & 1) {
-        const int st = nodes[base + k].start;
-        const int cnt = nodes[base + k].count_cmask & 0x0fffffff;
-        const uint64_t o = k == 0 ? om[0] : k == 1 ? om[1] : k == 2 ? om[2] : om[3];
-        const uint64_t n = k == 0 ? nm[0] : k == 1 ? nm[1] : k == 2 ? nm[2] : nm[3];
-        leaf32<GUARD>(st, cnt, lane_in(o), lane_in(n), rx, ry, parts, a, r2k[k],
-                      (float)nodes[base + k].near2, D, risk);
-      }
-    }
-#pragma nounroll
-    for (int k = 0; k < 4; ++k) {
-      if ((lists >> k) & 1) {
-        const int st = nodes[base + k].start;
-        const int cnt = nodes[base + k].count_cmask & 0x0fffffff;
-        const uint64_t n = k == 0 ? nm[0] : k == 1 ? nm[1] : k == 2 ? nm[2] : nm[3];
-        bigs32(st, cnt, lane_in(n), rx, ry, bigs, a);
-      }
+      if (!((lists >> k) & 1)) continue;
+      const cptr<NodeF> nd = nodes + base + k;
+      const double dx = nd->cx - rx, dy = nd->cy - ry;
+      const uint64_t nm = lanes_lt(dx * dx + dy * dy, nd->near2) & live;
+      if (nm) bigs32(nd->start, nd->count_cmask & 0x0fffffff, lane_in(nm), rx, ry, bigs, a);
     }
     // one compensated add per entry, by the lanes the entry holds: a lane's
     // sums do not depend on which cells its bundle mates open
     if (lv) a.flush();
-#pragma unroll
-    for (int k = 3; k >= 0; --k)
-      if (pe[k]) push_entry(s_node, s_mask, sp, lane, pe[k], om[k], err);
     if (sp > 0) {
       --sp;
       __builtin_amdgcn_wave_barrier();
@@ -907,7 +906,8 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
       if (++h >= ntrees) break;
       const cptr<TreeDesc> td = as_const(trees) + h;
       nodes = as_const(td->nodesF);
-      parts = as_const(td->partsD);
+      parts = as_const(td->partsF);
+      partsd = as_const(td->partsD);
       bigs = as_const(td->bigD);
       sheetd = as_const(td->sheetd);
       a.tree = (uint32_t)h;
@@ -1080,7 +1080,8 @@ __device__ __forceinline__ void process_item(const StepJob* __restrict__ jobs, W
     float risk = 0.f;
     // (a plane may hold halo-of-halos trees only; walk32 starts at trees[0])
     if (HK == 0 || ntrees > 0)
-      walk32<true>(jb.trees, ntrees, rx, ry, active, a, s_node, s_mask, s_near, err, D, risk);
+      walk32<true>(jb.trees, ntrees, rx, ry, active, a, s_node, s_mask, s_near, err, D,
+                   (float)dx2 * (1.f + 1.f / 1048576.f), risk);
     plane_risk = risk;
     double f[5];
     finish32(a, f);
@@ -1743,7 +1744,8 @@ __global__ __launch_bounds__(kWave * kWavesPerBlock) void k_force(
   if (F32) {
     Acc32 a;
     float unused = 0.f;   // no recursion here: no guard
-    walk32<false>(tree, 1, rx, ry, active, a, s_node[w], s_mask[w], s_near[w], err, 0.0, unused);
+    walk32<false>(tree, 1, rx, ry, active, a, s_node[w], s_mask[w], s_near[w], err, 0.0, 0.f,
+                  unused);
     finish32(a, r);
     sheet32(t, rx, ry, r);
   } else {

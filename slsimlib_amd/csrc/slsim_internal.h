@@ -72,6 +72,18 @@ struct PartD {
 };
 static_assert(sizeof(PartD) == 24, "PartD layout");
 
+// Particle of a leaf for the FP32 walk: its offset from the leaf's centre of
+// mass (FP32; the walk adds the FP32 offset of the centre from the ray), its
+// mass, and 4 h^2 (1 + 2^-10) with the leaf's big-particle flag in the sign
+// bit as PartD carries it (set = not listed).  Same order and indices as
+// PartD.
+struct PartF {
+  float lx, ly;
+  float m;
+  float hh4;
+};
+static_assert(sizeof(PartF) == 16, "PartF layout");
+
 // Halo-of-halos planes (LensHaloHalos<LensHaloNFW>, include/particle_halo.h:
 // 1092-1227): cells of the TreeQuadHalos quadtree (include/quadTreeHalos.h),
 // children of a cell contiguous (child0..child3 in the reference's order).
@@ -122,6 +134,7 @@ struct TreeDesc {
   const NodeF* nodesF;
   const PartD* partsD;
   const PartD* bigD;
+  const PartF* partsF;
   double inv_area;
   double mass, comx, comy;   // total mass and centre of mass (the mass sheet's terms)
   double dipx, dipy;         // sum of m x over the particles
@@ -344,6 +357,7 @@ struct slsim_plane_s {
   slsim::NodeF* nodesF = nullptr;
   slsim::PartD* partsD = nullptr;
   slsim::PartD* bigD = nullptr;
+  slsim::PartF* partsF = nullptr;
   // halo-of-halos plane (kind == kTreeHalos)
   int32_t kind = 0;
   slsim::HNode* hnodes = nullptr;

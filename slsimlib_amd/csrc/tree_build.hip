@@ -580,18 +580,30 @@ __global__ void k_pack_nodes(int64_t nnodes, const double* __restrict__ com,
 __global__ void k_pack_parts(size_t n, const double* __restrict__ xs, const double* __restrict__ ys,
                              const float* __restrict__ ms, const float* __restrict__ hs,
                              const int32_t* __restrict__ leaf_of, const double* __restrict__ p1,
-                             const double* __restrict__ p2, PartD* __restrict__ pd) {
+                             const double* __restrict__ p2, const double* __restrict__ com,
+                             PartD* __restrict__ pd, PartF* __restrict__ pf) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int32_t l = leaf_of[i];
   const double box = p2[2 * l] - p1[2 * l];
+  const bool listed = (double)hs[i] < box;
   PartD p;
   p.x = xs[i];
   p.y = ys[i];
   p.m = ms[i];
   // the sign bit of h carries the leaf's big-particle flag, h < box
-  p.h = ((double)hs[i] < box) ? hs[i] : -hs[i];
+  p.h = listed ? hs[i] : -hs[i];
   pd[i] = p;
+  // the FP32 walk's copy: offset from the leaf's centre of mass, 4 h^2
+  PartF f;
+  f.lx = (float)(p.x - com[2 * l]);
+  f.ly = (float)(p.y - com[2 * l + 1]);
+  f.m = ms[i];
+  // (with a margin of 2^-10 for the FP32 walk's offsets: the walk takes
+  // the particles below it to the FP64 offsets and the exact 4 h^2 test)
+  const float hh4 = 4.f * hs[i] * hs[i] * (1.f + 1.f / 1024.f);
+  f.hh4 = listed ? hh4 : -hh4;
+  pf[i] = f;
 }
 
 __global__ void k_pack_big(uint32_t nb, const uint64_t* __restrict__ pairs,
@@ -908,6 +920,7 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
     return rc;
   SLSIM_HIP_CHECK(hipMemsetAsync(pl->nodesF + nnodes, 0, 3 * sizeof(NodeF), st));
   if ((rc = ctx->alloc(reinterpret_cast<void**>(&pl->partsD), sizeof(PartD) * n))) return rc;
+  if ((rc = ctx->alloc(reinterpret_cast<void**>(&pl->partsF), sizeof(PartF) * n))) return rc;
   if ((rc = ctx->alloc(reinterpret_cast<void**>(&pl->npart), 4 * nnodes))) return rc;
   SLSIM_HIP_CHECK(hipMemcpyAsync(pl->npart, gcnt, 4 * nnodes, hipMemcpyDeviceToDevice, st));
   double* flip;
@@ -926,7 +939,8 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
   k_pack_nodes<<<nblocks(nnodes), kBlock, 0, st>>>(nnodes, com, mass, quad, r2c, gp1, gp2, gchild,
                                                    gst, gcnt, bst, bcnt, flip, pl->nodesD,
                                                    pl->nodesF);
-  k_pack_parts<<<nblocks(n), kBlock, 0, st>>>(n, xs, ys, ms, hs, leaf_of, gp1, gp2, pl->partsD);
+  k_pack_parts<<<nblocks(n), kBlock, 0, st>>>(n, xs, ys, ms, hs, leaf_of, gp1, gp2, com,
+                                               pl->partsD, pl->partsF);
   // masses of both signs: the sheet's partition-dependent term per cell
   double* dip = nullptr;
   if (pl->mixed_sign && pl->inv_area != 0.0) {
@@ -956,7 +970,7 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
   (void)hipEventDestroy(ev0);
   (void)hipEventDestroy(ev1);
   pl->bytes = nnodes * (sizeof(NodeD) + sizeof(NodeF) + 16 + 16 + 4 + 4) + 3 * sizeof(NodeF) +
-              n * sizeof(PartD) + hpairs * sizeof(PartD);
+              n * (sizeof(PartD) + sizeof(PartF)) + hpairs * sizeof(PartD);
   return SLSIM_OK;
 }
 
