@@ -89,6 +89,9 @@ struct Point_2d {
   double operator[](int i) const { return x[i]; }
   Point_2d operator-(const Point_2d& p) const { return Point_2d(x[0] - p.x[0], x[1] - p.x[1]); }
   Point_2d operator+(const Point_2d& p) const { return Point_2d(x[0] + p.x[0], x[1] + p.x[1]); }
+  Point_2d& operator+=(const Point_2d& p) { x[0] += p.x[0]; x[1] += p.x[1]; return *this; }
+  Point_2d& operator/=(PosType v) { x[0] /= v; x[1] /= v; return *this; }
+  Point_2d operator*(PosType v) const { return Point_2d(x[0] * v, x[1] * v); }
   double length() const { return std::sqrt(x[0] * x[0] + x[1] * x[1]); }
   double x[2];
 };
@@ -1613,3 +1616,277 @@ struct GridMap {
   void* yd = nullptr;
   void* Ad = nullptr;
 };
+
+// --------------------------------------------------------- adaptive grid
+// Grid (TreeCode_link/grid_maintenance.cpp, include/grid_maintenance.h:24-240):
+// the points stay in device memory (slsim_agrid); this object keeps host
+// Point copies, image points linked to their source points, in blocks: the
+// initial grid, then one block per refinement, as the reference's point
+// arrays are.  Refinements shoot with the lens the grid was made with.  The
+// reference's kd-trees (i_tree, s_tree) and the finders that walk them are
+// not built.
+struct Grid {
+  Grid(LensHndl lens, unsigned long N1d, const double center[2], double range)
+      : lens_(lens) {
+    if (N1d < 1) throw std::runtime_error("GridMap size is < 1!");
+    if (range <= 0) throw std::runtime_error("GridMap range is <= 0!");
+    slsim_cpp::check(slsim_agrid_create(lens->handle(), N1d, center, range, range, 1,
+                                        slsim_cpp::precision(), &g_));
+    take_initial();
+  }
+  Grid(LensHndl lens, unsigned long Nx, const PosType center[2], PosType rangeX, PosType rangeY)
+      : lens_(lens) {
+    if (Nx <= 0 || rangeX <= 0 || rangeY <= 0) throw std::runtime_error("");
+    slsim_cpp::check(slsim_agrid_create(lens->handle(), Nx, center, rangeX, rangeY, 0,
+                                        slsim_cpp::precision(), &g_));
+    take_initial();
+  }
+  Grid(Grid&& o) noexcept
+      : lens_(o.lens_), g_(o.g_), blocks_(std::move(o.blocks_)), by_id_(std::move(o.by_id_)) {
+    o.g_ = nullptr;
+  }
+  Grid(const Grid&) = delete;
+  Grid& operator=(const Grid&) = delete;
+  ~Grid() {
+    if (g_) slsim_agrid_destroy(g_);
+  }
+
+  // grid_maintenance.cpp:148-158: a new unrefined grid over the same box
+  Grid ReInitialize(LensHndl lens) {
+    const slsim_agrid_info i = info();
+    const Point_2d c = getInitCenter();
+    return Grid(lens, (unsigned long)i.ngrid_init, c.x, i.image_p2[0] - i.image_p1[0],
+                i.image_p2[1] - i.image_p1[1]);
+  }
+  // grid_maintenance.cpp:543-679
+  Point* RefineLeaf(LensHndl lens, Point* point) {
+    check_lens(lens);
+    int64_t first = 0, count = 0;
+    slsim_cpp::check(slsim_agrid_refine_one(g_, (int64_t)point->id, &first, &count));
+    refresh({(int64_t)point->id});
+    return take(first, count);
+  }
+  // grid_maintenance.cpp:681-982: all the cells' new points shot together
+  Point* RefineLeaves(LensHndl lens, std::vector<Point*>& points) {
+    check_lens(lens);
+    if (points.empty()) return nullptr;
+    std::vector<int64_t> ids(points.size());
+    for (size_t k = 0; k < points.size(); ++k) ids[k] = (int64_t)points[k]->id;
+    int64_t first = 0, count = 0;
+    slsim_cpp::check(slsim_agrid_refine(g_, ids.size(), ids.data(), &first, &count));
+    refresh(ids);
+    return take(first, count);
+  }
+  unsigned long getNumberOfPoints() const { return (unsigned long)info().npoints; }
+  // grid_maintenance.cpp:368-378
+  PosType EinsteinArea() const {
+    double a = 0;
+    slsim_cpp::check(slsim_agrid_einstein_area(g_, &a));
+    return a;
+  }
+  int getInitNgrid() { return (int)info().ngrid_init; }
+  int getNgrid_block() { return (int)info().ngrid_block; }
+  double getInitRange() {
+    const slsim_agrid_info i = info();
+    return i.image_p2[0] - i.image_p1[0];
+  }
+  // image_finder.cpp:826-831
+  Point_2d getInitCenter() {
+    const slsim_agrid_info i = info();
+    return Point_2d((i.image_p2[0] + i.image_p1[0]) / 2, (i.image_p2[1] + i.image_p1[1]) / 2);
+  }
+  // grid_maintenance.cpp:984-1005
+  void ClearAllMarks() {
+    for (Point* p : by_id_) {
+      p->in_image = p->image->in_image = NO;
+      p->surface_brightness = p->image->surface_brightness = 0;
+    }
+  }
+  // the image point of id i (initial grid first, then the refinements' points)
+  Point* point(unsigned long i) { return by_id_.at(i); }
+  slsim_agrid handle() const { return g_; }
+
+ private:
+  struct Block {
+    std::unique_ptr<Point[]> i, s;
+  };
+  slsim_agrid_info info() const {
+    slsim_agrid_info i;
+    slsim_cpp::check(slsim_agrid_get_info(g_, &i));
+    return i;
+  }
+  void check_lens(LensHndl lens) const {
+    if (lens != lens_)
+      throw std::invalid_argument("Grid: refinements shoot with the lens the grid was made with");
+  }
+  void take_initial() {
+    const slsim_agrid_info i = info();
+    take(0, i.npoints);
+  }
+  // host copies of points first .. first+count-1 as a new block
+  Point* take(int64_t first, int64_t count) {
+    if (count <= 0) return nullptr;
+    const size_t n = (size_t)count;
+    std::vector<double> x(2 * n), y(2 * n), A(4 * n), gs(n);
+    slsim_cpp::check(slsim_agrid_read(g_, first, count, x.data(), y.data(), A.data(), gs.data()));
+    Block b{std::unique_ptr<Point[]>(new Point[n]), std::unique_ptr<Point[]>(new Point[n])};
+    for (size_t k = 0; k < n; ++k) {
+      Point& p = b.i[k];
+      Point& q = b.s[k];
+      p.x[0] = x[2 * k];
+      p.x[1] = x[2 * k + 1];
+      q.x[0] = y[2 * k];
+      q.x[1] = y[2 * k + 1];
+      for (int c = 0; c < 4; ++c) p.A.a[c] = q.A.a[c] = A[4 * k + c];
+      p.gridsize = q.gridsize = gs[k];
+      p.id = q.id = (unsigned long)(first + (int64_t)k);
+      p.image = &q;
+      q.image = &p;
+      p.head = q.head = n;
+    }
+    Point* out = b.i.get();
+    for (size_t k = 0; k < n; ++k) by_id_.push_back(&b.i[k]);
+    blocks_.push_back(std::move(b));
+    return out;
+  }
+  // the refined cells' sizes, as the device left them
+  void refresh(const std::vector<int64_t>& ids) {
+    for (int64_t id : ids) {
+      double g = 0;
+      slsim_cpp::check(slsim_agrid_read(g_, id, 1, nullptr, nullptr, nullptr, &g));
+      by_id_.at((size_t)id)->gridsize = by_id_.at((size_t)id)->image->gridsize = g;
+    }
+  }
+  LensHndl lens_;
+  slsim_agrid g_ = nullptr;
+  std::vector<Block> blocks_;
+  std::vector<Point*> by_id_;
+};
+typedef Grid* GridHndl;
+
+// ------------------------------------------------------- curves of a GridMap
+namespace Utilities {
+// TreeCode_link/curve_routines.cpp:3519-3592: the outer boundary of the
+// pixels near the closed curve v, moved onto the curve (device passes)
+inline std::vector<Point_2d> TightestHull(const std::vector<Point_2d>& v) {
+  std::vector<double> p(2 * v.size());
+  for (size_t i = 0; i < v.size(); ++i) {
+    p[2 * i] = v[i][0];
+    p[2 * i + 1] = v[i][1];
+  }
+  slsim_curves h = nullptr;
+  slsim_cpp::check(slsim_tightest_hull(slsim_cpp::context(), p.data(), v.size(), &h));
+  std::vector<std::vector<Point_2d>> out;
+  std::vector<int32_t> t, hb;
+  int replaced;
+  slsim_cpp::take_curves(h, out, 0, t, hb, replaced);
+  return out.empty() ? std::vector<Point_2d>() : out[0];
+}
+// TreeCode_link/curve_routines.cpp:1709-1736: winding number about x and
+// the absolute area of the closed curve
+inline int windings(const Point_2d& x, const std::vector<Point_2d>& point, PosType* area) {
+  std::vector<double> p(2 * point.size());
+  for (size_t i = 0; i < point.size(); ++i) {
+    p[2 * i] = point[i][0];
+    p[2 * i + 1] = point[i][1];
+  }
+  int32_t wn = 0;
+  slsim_cpp::check(slsim_windings(x.x, p.data(), point.size(), &wn, area));
+  return wn;
+}
+// curve_routines.cpp:1738-1765: the same over the image positions of rays
+inline int windings(Point_2d& x, std::vector<RAY>& point, PosType* area) {
+  std::vector<Point_2d> p(point.size());
+  for (size_t i = 0; i < point.size(); ++i) p[i] = point[i].x;
+  return windings(x, p, area);
+}
+}  // namespace Utilities
+
+// include/grid_maintenance.h:245-330 (the data members)
+struct CriticalCurve {
+  std::vector<RAY> critcurve;
+  std::vector<Point_2d> caustic_curve_outline;
+  std::vector<Point_2d> caustic_curve_intersecting;
+  std::vector<Point_2d> ellipse_curve;
+  PosType z_source = 0.0;
+  CritType type = CritType::ND;
+  int caustic_intersections = -1;
+  Point_2d critical_center;
+  Point_2d caustic_center;
+  PosType critical_area = 0.0;
+  PosType caustic_area = 0.0;
+  PosType contour_ell = 0.0;
+  PosType ellipse_area = 0.0;
+  bool touches_edge = false;
+  bool inCausticCurve(Point_2d& x) { return Utilities::inCurve(x, caustic_curve_outline); }
+  bool inCriticalCurve(Point_2d& x) {
+    std::vector<Point_2d> c(critcurve.size());
+    for (size_t i = 0; i < c.size(); ++i) c[i] = critcurve[i].x;
+    return Utilities::inCurve(x, c);
+  }
+};
+
+namespace ImageFinding {
+// TreeCode_link/find_crit.cpp:1036-1096: the GridMap's critical curves as
+// rays to the lens's source, their centres and areas, their caustics (the
+// critical points shot through the lens in one batch per curve) and the
+// caustics' outlines (TightestHull) and areas
+inline void find_crit(Lens& lens, GridMap& gridmap, std::vector<CriticalCurve>& critcurves,
+                      bool verbose = false) {
+  (void)verbose;
+  const double z_source = lens.getSourceZ();
+  std::vector<std::vector<Point_2d>> crits;
+  std::vector<bool> hits_boundary;
+  std::vector<CritType> crit_type;
+  gridmap.find_crit(crits, hits_boundary, crit_type);
+  const size_t Ncrit = crits.size();
+  critcurves.resize(Ncrit);
+  for (size_t i = 0; i < Ncrit; ++i) {
+    CriticalCurve& cc = critcurves[i];
+    cc.type = crit_type[i];
+    cc.touches_edge = i < hits_boundary.size() ? (bool)hits_boundary[i] : false;
+    cc.z_source = z_source;
+    const size_t Npoints = crits[i].size();
+    cc.critcurve.resize(Npoints);
+    cc.critical_center = Point_2d(0, 0);
+    for (size_t j = 0; j < Npoints; ++j) {
+      cc.critcurve[j].x = crits[i][j];
+      cc.critical_center += crits[i][j];
+      cc.critcurve[j].z = z_source;
+    }
+    cc.critical_center /= (PosType)Npoints;
+    Utilities::windings(cc.critical_center, cc.critcurve, &cc.critical_area);
+    lens.rayshooterInternal(cc.critcurve.size(), cc.critcurve.data());
+    cc.caustic_center = Point_2d(0, 0);
+    cc.caustic_curve_intersecting.resize(Npoints);
+    for (size_t j = 0; j < Npoints; ++j) {
+      cc.caustic_curve_intersecting[j] = cc.critcurve[j].y;
+      cc.caustic_center += cc.critcurve[j].y;
+    }
+    cc.caustic_center /= (PosType)Npoints;
+    cc.caustic_curve_outline = Utilities::TightestHull(cc.caustic_curve_intersecting);
+    Utilities::windings(cc.caustic_center, cc.caustic_curve_outline, &cc.caustic_area);
+    cc.caustic_intersections = -1;
+  }
+}
+
+// find_crit.cpp:1098-1119: the GridMap's magnification contours as rays
+// shot to the lens's source, one batch per contour
+inline void find_magnification_contour(Lens& lens, GridMap& gridmap, double invmag,
+                                       std::vector<std::vector<RAY>>& contour,
+                                       std::vector<bool>& hits_boundary) {
+  std::vector<std::vector<Point_2d>> image_plane_contour;
+  gridmap.find_magnification_contour(image_plane_contour, hits_boundary, invmag);
+  const size_t n = image_plane_contour.size();
+  contour.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    const size_t m = image_plane_contour[i].size();
+    contour[i].resize(m);
+    for (size_t j = 0; j < m; ++j) {
+      contour[i][j].x = image_plane_contour[i][j];
+      contour[i][j].z = lens.getSourceZ();
+    }
+    lens.rayshooterInternal(m, contour[i].data());
+  }
+}
+}  // namespace ImageFinding

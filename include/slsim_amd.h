@@ -388,12 +388,69 @@ int slsim_grid_find_magnification_contour(slsim_ctx ctx, const slsim_grid* grid,
 int slsim_grid_find_boundaries_of_caustics(slsim_ctx ctx, const slsim_grid* grid,
                                            const double* y_dev, int32_t* multiplicity,
                                            slsim_curves* out);
+/* Utilities::TightestHull(v) (TreeCode_link/curve_routines.cpp:3519-3592)
+ * of the closed curve points (2 n doubles): the outer boundary of the pixels
+ * (1/100 of the curve's narrower extent) within 1.58 pixels of the curve,
+ * each boundary point moved to the nearest point of the curve; n <= 3 gives
+ * the points, a curve of zero width its box corners.  The distance passes run
+ * on the device.  *out holds one curve. */
+int slsim_tightest_hull(slsim_ctx ctx, const double* points, size_t n, slsim_curves* out);
+/* Utilities::windings(x, points, &area) (TreeCode_link/curve_routines.cpp:
+ * 1709-1736), on the host: the winding number of the closed curve (2 n
+ * doubles) about x and its absolute area. */
+int slsim_windings(const double* x, const double* points, size_t n, int32_t* winding,
+                   double* area);
 int slsim_curves_get_info(slsim_curves curves, slsim_curves_info* info);
 /* offsets: n_curves + 1 (points of curve c are offsets[c] .. offsets[c+1]),
  * points: 2 n_points, types and hits_boundary: n_curves each. */
 int slsim_curves_read(slsim_curves curves, int64_t* offsets, double* points, int32_t* types,
                       int32_t* hits_boundary);
 int slsim_curves_destroy(slsim_curves curves);
+
+/* ------------------------------------------------ adaptive grid (Grid) */
+/* Grid (TreeCode_link/grid_maintenance.cpp:22-134, include/grid_maintenance.h:
+ * 24-240): an image-plane grid refined cell by cell.  Its points (image and
+ * source positions, magnification matrix, cell size) live in device memory in
+ * the reference's order: the initial Nx x Ny grid, then each refinement's new
+ * points appended.  Point ids are indices into that order. */
+typedef struct slsim_agrid_s* slsim_agrid;
+typedef struct {
+  int64_t npoints;
+  int64_t ngrid_init, ngrid_init2, ngrid_block;
+  double image_p1[2], image_p2[2];     /* the image points' root box */
+  double source_p1[2], source_p2[2];   /* the source points' root box (with buffer) */
+} slsim_agrid_info;
+/* Grid(lens, N1d, center, range) (square != 0: grid_maintenance.cpp:22-59,
+ * points by Utilities::PositionFromIndex) or Grid(lens, Nx, center, rangeX,
+ * rangeY) (square == 0: :67-129, Ny = (int)(Nx rangeY / rangeX)): the points
+ * shot to the lens's source at `precision`.  The grid uses the lens until
+ * slsim_agrid_destroy. */
+int slsim_agrid_create(slsim_lens lens, size_t Nx, const double* center, double range_x,
+                       double range_y, int square, int precision, slsim_agrid* out);
+int slsim_agrid_destroy(slsim_agrid g);
+int slsim_agrid_get_info(slsim_agrid g, slsim_agrid_info* info);
+/* Host copies of points first .. first+count-1 (x, y: 2 a point; A: 4;
+ * gridsize: 1); any output may be NULL. */
+int slsim_agrid_read(slsim_agrid g, int64_t first, int64_t count, double* x, double* y,
+                     double* A, double* gridsize);
+/* The device arrays (valid until the next refinement): no copy. */
+int slsim_agrid_device(slsim_agrid g, double** x_dev, double** y_dev, double** A_dev,
+                       double** gridsize_dev);
+/* Grid::RefineLeaves(lens, points) (grid_maintenance.cpp:681-982): every
+ * listed cell (distinct ids) split into its eight sub-cell points, those
+ * outside the initial image box or below 1e-10 of their position dropped,
+ * the rest shot in one batch and those whose source position leaves the
+ * source box dropped, in the reference's order; the parents' cell sizes / 3
+ * (restored when none of their points stayed).  New points are ids
+ * *first .. *first + *count - 1. */
+int slsim_agrid_refine(slsim_agrid g, size_t n, const int64_t* ids, int64_t* first,
+                       int64_t* count);
+/* Grid::RefineLeaf(lens, point) (grid_maintenance.cpp:543-679): one cell,
+ * the reference's swap order for the points it drops. */
+int slsim_agrid_refine_one(slsim_agrid g, int64_t id, int64_t* first, int64_t* count);
+/* Grid::EinsteinArea (grid_maintenance.cpp:368-378): the sum of gridsize^2
+ * over the points with det A < 0 (a device reduction). */
+int slsim_agrid_einstein_area(slsim_agrid g, double* area);
 
 /* --------------------------------------------------- batched GridMaps */
 /* Batching and serving have no counterpart in the reference, which shoots

@@ -86,6 +86,8 @@ _sigs = {
                                 P(ctypes.c_int64), P(c_sz)]),
     "orc_convex_hull": (c_sz, [P(c_d), c_sz, P(ctypes.c_int64)]),
     "orc_in_curve": (None, [P(c_d), c_sz, P(c_d), c_sz, P(ctypes.c_int32)]),
+    "orc_tightest_hull": (c_sz, [P(c_d), c_sz, c_sz, P(c_d)]),
+    "orc_windings": (c_int, [P(c_d), P(c_d), c_sz, P(c_d)]),
 }
 for _n, (_r, _a) in _sigs.items():
     getattr(_lib, _n).restype = _r
@@ -483,6 +485,243 @@ def find_boundaries_of_caustics(N, x, y, A, ny=None):
     _, mult, _ = find_images(N, x, y, A, x, cap=64 * N * ny, ny=ny)
     c, h, _, _ = _crit(2, N, x, y, A, mult, 0.0, 0.0, ny)
     return c, h, mult
+
+
+def tightest_hull(points):
+    """Utilities::TightestHull(v) (TreeCode_link/curve_routines.cpp:3519-3592)."""
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    cap = 4096
+    while True:
+        out = np.zeros((cap, 2))
+        n = _lib.orc_tightest_hull(_p(p), len(p), cap, _p(out))
+        if n <= cap:
+            return out[:n].copy()
+        cap = n
+
+
+def windings(x, points):
+    """Utilities::windings(x, points, &area) (curve_routines.cpp:1709-1736):
+    (winding number, absolute area)."""
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    q = np.ascontiguousarray(x, dtype=np.float64).reshape(2)
+    area = c_d()
+    wn = _lib.orc_windings(_p(q), _p(p), len(p), ctypes.byref(area))
+    return wn, area.value
+
+
+def image_finding_find_crit(L, N, x, y, A, range_, ny=None):
+    """ImageFinding::find_crit(lens, gridmap, critcurves)
+    (TreeCode_link/find_crit.cpp:1036-1096) over a shot grid (x, y, A) and the
+    oracle lens L: per curve a dict with the reference's fields (critcurve
+    points and their rays, centres, areas, caustic points and outline)."""
+    zs = float(L.tables()["z"][-1])
+    curves, hits, types, replaced = find_crit(N, x, y, A, range_, ny)
+    out = []
+    for i, c in enumerate(curves):
+        n = len(c)
+        center = np.zeros(2)
+        for j in range(n):
+            center += c[j]
+        center = center / n
+        cy, cA = L.shoot(c, zs=np.full(n, zs))
+        ccen = np.zeros(2)
+        for j in range(n):
+            ccen += cy[j]
+        ccen = ccen / n
+        outline = tightest_hull(cy)
+        out.append({"type": int(types[i]), "touches_edge": bool(hits[i]) if replaced else False,
+                    "z_source": zs, "critcurve": c, "critcurve_y": cy, "critcurve_A": cA,
+                    "critical_center": center, "critical_area": windings(center, c)[1],
+                    "caustic_curve_intersecting": cy, "caustic_center": ccen,
+                    "caustic_curve_outline": outline,
+                    "caustic_area": windings(ccen, outline)[1]})
+    return out
+
+
+# ------------------------------------------------------------ Grid
+def _position_from_index(i, N, range_, center):
+    """Utilities::PositionFromIndex (TreeCode_link/utilities.cpp:149-158)."""
+    if N == 1:
+        return [center[0], center[1]]
+    return [center[0] + range_ * (1.0 * (i % N) / (N - 1) - 0.5),
+            center[1] + range_ * (1.0 * (i // N) / (N - 1) - 0.5)]
+
+
+def _xygridpoints(range_, center, N, remove_center):
+    """Grid::xygridpoints (TreeCode_link/image_finder.cpp:790-824): positions
+    and gridsize; the centre point left out when remove_center and N odd."""
+    pts = []
+    if remove_center and N % 2 == 1:
+        j = 0
+        for i in range(N * N):
+            if (2 * (i // N)) // (N - 1) == 1 and i % N == N // 2 + 1:
+                j = 1
+            p = _position_from_index(i, N, range_, center)
+            if i - j < len(pts):
+                pts[i - j] = p
+            else:
+                pts.append(p)
+        pts = pts[:N * N - 1]
+    else:
+        pts = [_position_from_index(i, N, range_, center) for i in range(N * N)]
+    return pts, range_ / (N - 1)
+
+
+def _inbox(x, p1, p2):
+    return (x[0] >= p1[0]) and (x[0] <= p2[0]) and (x[1] >= p1[1]) and (x[1] <= p2[1])
+
+
+class Grid:
+    """Grid (TreeCode_link/grid_maintenance.cpp) restated over the oracle
+    lens: the point lists in the reference's order (x, y, A, gridsize), the
+    two trees' root boxes, RefineLeaves (:681-982), RefineLeaf (:543-679)
+    and EinsteinArea (:368-378).  The trees themselves are left out: only
+    their root boxes enter the refinement."""
+
+    def __init__(self, L, N, center, range_, range_y=None):
+        self.L = L
+        self.Ngrid_block = 3
+        if range_y is None:                        # grid_maintenance.cpp:22-59
+            pts, gs = _xygridpoints(range_, center, N, 0)
+            self.x = [list(p) for p in pts]
+            self.gs = [gs] * len(pts)
+            buffer = range_
+        else:                                      # :67-129
+            axisratio = range_y / range_
+            N2 = int(N * axisratio)
+            self.x = [None] * (N * N2)
+            self.gs = [0.0] * (N * N2)
+            for ii in range(N):
+                for jj in range(N2):
+                    i = ii + jj * N
+                    self.x[i] = [center[0] + range_ * ii / (N - 1) - 0.5 * range_,
+                                 center[1] + range_ * jj / (N - 1) - 0.5 * range_y]
+                    self.gs[i] = range_ / (N - 1)
+            buffer = max(range_, range_y)
+        y, A = L.shoot(np.array(self.x))
+        self.y = [list(v) for v in y]
+        self.A = [list(v) for v in A]
+        # TreeStruct root boxes (tree_maintenance.cpp:96-124)
+        xs = np.array(self.x)
+        self.ip1, self.ip2 = list(xs.min(0)), list(xs.max(0))
+        ys = np.array(self.y)
+        self.sp1 = [ys[:, 0].min() - buffer, ys[:, 1].min() - buffer]
+        self.sp2 = [ys[:, 0].max() + buffer, ys[:, 1].max() + buffer]
+
+    def npoints(self):
+        return len(self.x)
+
+    def refine_leaves(self, ids):
+        """The new points' ids (appended in the reference's order)."""
+        nb = self.Ngrid_block
+        ncell = nb * nb - 1
+        i_points, i_gs = [], []
+        Nadded = 0
+        Nout_tot = 0
+        addedtocell = [0] * len(ids)
+        for ii, q in enumerate(ids):
+            if self.gs[q] < 10.0 ** -15 * max(abs(self.x[q][0]), abs(self.x[q][1])):
+                raise RuntimeError("Cell size is too small for double precision")
+            pts, g = _xygridpoints(self.gs[q] * (nb - 1) / nb, self.x[q], nb, 1)
+            i_points[Nadded:] = pts
+            i_gs[Nadded:] = [g] * len(pts)
+            self.gs[q] /= nb
+            # points outside the initial image grid, taken out by shifting
+            Nout = 0
+            for kk in range(ncell):
+                k = Nadded + kk - Nout
+                p = i_points[k]
+                if (not _inbox(p, self.ip1, self.ip2)
+                        or i_gs[k] < 1.0e-10 * max(abs(p[0]), abs(p[1]))):
+                    del i_points[k]
+                    del i_gs[k]
+                    Nout += 1
+            Nout_tot += Nout
+            Nadded += ncell - Nout
+            addedtocell[ii] = ncell - Nout
+            if addedtocell[ii] == 0:
+                self.gs[q] *= nb
+        if Nadded == 0:
+            return []
+        y, A = self.L.shoot(np.array(i_points))
+        s_points = [list(v) for v in y]
+        s_A = [list(v) for v in A]
+        # points outside the initial source grid, taken out by shifting
+        kk = 0
+        Nout = 0
+        for ii, q in enumerate(ids):
+            if addedtocell[ii] == 0:
+                continue
+            Noutcell = 0
+            for j in range(addedtocell[ii]):
+                k = kk - Nout
+                if not _inbox(s_points[k], self.sp1, self.sp2):
+                    del i_points[k], i_gs[k], s_points[k], s_A[k]
+                    Nout += 1
+                    Noutcell += 1
+                kk += 1
+            addedtocell[ii] -= Noutcell
+            if addedtocell[ii] == 0:
+                self.gs[q] *= nb
+        first = len(self.x)
+        self.x += i_points
+        self.y += s_points
+        self.A += s_A
+        self.gs += i_gs
+        return list(range(first, len(self.x)))
+
+    def refine_leaf(self, q):
+        nb = self.Ngrid_block
+        ncell = nb * nb - 1
+        if not self.gs[q] > 10.0 ** -15:
+            raise RuntimeError("cell too small")
+        i_points, g = _xygridpoints(self.gs[q] * (nb - 1) / nb, self.x[q], nb, 1)
+        i_gs = [g] * ncell
+        self.gs[q] /= nb
+        Nout = 0
+        xq = self.x[q]
+        if (xq[0] == self.ip1[0] or xq[0] == self.ip2[0] or xq[1] == self.ip1[1]
+                or xq[1] == self.ip2[1]):
+            for kk in range(ncell):
+                a, b = kk - Nout, ncell - 1 - Nout
+                if not _inbox(i_points[a], self.ip1, self.ip2):
+                    i_points[a], i_points[b] = i_points[b], i_points[a]
+                    i_gs[a], i_gs[b] = i_gs[b], i_gs[a]
+                    Nout += 1
+        Ntemp = ncell - Nout
+        i_points, i_gs = i_points[:Ntemp], i_gs[:Ntemp]
+        if Ntemp > 0:
+            y, A = self.L.shoot(np.array(i_points))
+            s_points, s_A = [list(v) for v in y], [list(v) for v in A]
+        else:
+            s_points, s_A = [], []
+        Nout = 0
+        for kk in range(Ntemp):
+            a, b = kk - Nout, Ntemp - 1 - Nout
+            if not _inbox(s_points[a], self.sp1, self.sp2):
+                for arr in (i_points, i_gs, s_points, s_A):
+                    arr[a], arr[b] = arr[b], arr[a]
+                Nout += 1
+        n = Ntemp - Nout
+        if n <= 0:
+            self.gs[q] *= nb
+            return []
+        first = len(self.x)
+        self.x += i_points[:n]
+        self.y += s_points[:n]
+        self.A += s_A[:n]
+        self.gs += i_gs[:n]
+        return list(range(first, len(self.x)))
+
+    def einstein_area(self):
+        total = 0.0
+        for a, g in zip(self.A, self.gs):
+            if a[0] * a[3] - a[1] * a[2] < 0:
+                total += g * g
+        return total
+
+    def arrays(self):
+        return (np.array(self.x), np.array(self.y), np.array(self.A), np.array(self.gs))
 
 
 # ------------------------------------------------ Lens::ResetSourcePlane

@@ -12,6 +12,10 @@
 //   GridMap::find_magnification_contour  TreeCode_link/gridmap.cpp:493-522
 //   GridMap::find_crit                   TreeCode_link/gridmap.cpp:525-647
 //   GridMap::find_boundaries_of_caustics include/gridmap.h:278-296
+//   distance_to_segment (both overloads) include/concave_hull.h:967-1001
+//   Utilities::TightestHull              TreeCode_link/curve_routines.cpp:3519-3592
+//   Utilities::windings (Point_2d)       TreeCode_link/curve_routines.cpp:1709-1736
+//   isLeft (float result)                include/Tree.h:865-867
 // The marching-squares walk is kept as the reference writes it, one branch per
 // cell type (pixel k = 1, k+1 = 10, k+nx = 100, k+nx+1 = 1000), so that it
 // checks the product's table-driven walk independently.
@@ -362,9 +366,151 @@ size_t emit(const Out& o, size_t cap_curves, size_t cap_points, int64_t* off, do
   return o.curves.size();
 }
 
+// ------------------------------------------------------------ tightest hull
+P2 sub(const P2& a, const P2& b) { return {{a[0] - b[0], a[1] - b[1]}}; }
+P2 add(const P2& a, const P2& b) { return {{a[0] + b[0], a[1] + b[1]}}; }
+P2 scale(const P2& a, double s) { return {{a[0] * s, a[1] * s}}; }
+double dot(const P2& a, const P2& b) { return a[0] * b[0] + a[1] * b[1]; }
+double length(const P2& a) { return std::sqrt(a[0] * a[0] + a[1] * a[1]); }
+double length_sqr(const P2& a) { return a[0] * a[0] + a[1] * a[1]; }
+
+double distance_to_segment(const P2& P, const P2& S1, const P2& S2) {
+  const P2 D = sub(S2, S1);
+  const double s = dot(sub(P, S1), D) / length_sqr(D);
+  if (std::isnan(s) || s <= 0) {
+    return length(sub(P, S1));
+  } else if (s >= 1) {
+    return length(sub(P, S2));
+  } else {
+    return length(sub(add(S1, scale(D, s)), P));
+  }
+}
+
+double distance_to_segment(const P2& P, const P2& S1, const P2& S2, P2& closest_point) {
+  const P2 D = sub(S2, S1);
+  const double s = dot(sub(P, S1), D) / length_sqr(D);
+  if (std::isnan(s) || s <= 0) {
+    closest_point = S1;
+  } else if (s >= 1) {
+    closest_point = S2;
+  } else {
+    closest_point = add(S1, scale(D, s));
+  }
+  return length(sub(closest_point, P));
+}
+
+std::vector<P2> tightest_hull(const std::vector<P2>& v) {
+  if (v.size() <= 3) return v;
+  P2 ll = v[0], ur = v[0];
+  for (const P2& p : v) {
+    ll.v[0] = p[0] < ll[0] ? p[0] : ll[0];
+    ll.v[1] = p[1] < ll[1] ? p[1] : ll[1];
+    ur.v[0] = p[0] > ur[0] ? p[0] : ur[0];
+    ur.v[1] = p[1] > ur[1] ? p[1] : ur[1];
+  }
+  std::vector<P2> output;
+  const double w = ur[0] - ll[0], h = ur[1] - ll[1];
+  const double resolution = (w < h ? w : h) / 100;
+  if (resolution == 0) {
+    output.push_back(ll);
+    output.push_back(ur);
+    return output;
+  }
+  const double R = resolution * std::sqrt(2.5);
+  ll.v[0] -= 2 * R;
+  ll.v[1] -= 2 * R;
+  ur.v[0] += 2 * R;
+  ur.v[1] += 2 * R;
+  const long nx = (ur[0] - ll[0]) / resolution;
+  const long ny = (ur[1] - ll[1]) / resolution;
+  const long nv = (long)v.size();
+  long count = 0;
+  P2 p = ll;
+  std::vector<bool> bitmap(nx * ny, false);
+  for (long j = 0; j < ny; ++j, p.v[1] += resolution) {
+    p.v[0] = ll[0];
+    for (long i = 0; i < nx; ++i, p.v[0] += resolution) {
+      const long m = i + nx * j;
+      for (long k = 0; k < nv; ++k) {
+        if (R > distance_to_segment(p, v[k], v[(k + 1) % nv])) {
+          bitmap[m] = true;
+          ++count;
+          break;
+        }
+      }
+    }
+  }
+  if (count > 0) {
+    std::vector<std::vector<P2>> envelopes;
+    std::vector<bool> hits_edge;
+    find_boundaries(bitmap, nx, envelopes, hits_edge, false);
+    for (P2& q : envelopes[0]) q = add(ll, scale(q, resolution));
+    const size_t n = envelopes[0].size();
+    output.resize(n);
+    P2 q, closest_point;
+    for (size_t i = 0; i < n; ++i) {
+      double min_d = distance_to_segment(envelopes[0][i], v[0], v[1], closest_point);
+      for (long k = 1; k < nv; ++k) {
+        const double tmp_d = distance_to_segment(envelopes[0][i], v[k], v[(k + 1) % nv], q);
+        if (tmp_d < min_d) {
+          min_d = tmp_d;
+          closest_point = q;
+        }
+      }
+      output[i] = closest_point;
+    }
+  }
+  return output;
+}
+
+float is_left(const P2& p0, const P2& p1, const P2& x) {
+  return (p1[0] - p0[0]) * (x[1] - p0[1]) - (x[0] - p0[0]) * (p1[1] - p0[1]);
+}
+
+int windings(const P2& x, const std::vector<P2>& point, double* area) {
+  int wn = 0;
+  const size_t Npoints = point.size();
+  *area = 0.0;
+  if (Npoints < 3) return 0;
+  for (size_t i = 0; i < Npoints; ++i) {
+    const size_t k = i < Npoints - 1 ? i + 1 : 0;
+    *area += (point[i][0] + point[k][0]) * (point[i][1] - point[k][1]);
+    if (point[i][1] <= x[1]) {
+      if (point[k][1] > x[1])
+        if (is_left(point[i], point[k], x) > 0) ++wn;
+    } else {
+      if (point[k][1] <= x[1])
+        if (is_left(point[i], point[k], x) < 0) --wn;
+    }
+  }
+  *area = std::fabs(*area) * 0.5;
+  return wn;
+}
+
 }  // namespace
 
 extern "C" {
+
+// Utilities::TightestHull of the closed curve p (2 n doubles): the hull's
+// point count (written to out only when it fits cap)
+size_t orc_tightest_hull(const double* p, size_t n, size_t cap, double* out) {
+  std::vector<P2> v(n);
+  for (size_t i = 0; i < n; ++i) v[i] = {{p[2 * i], p[2 * i + 1]}};
+  const std::vector<P2> h = tightest_hull(v);
+  if (h.size() <= cap)
+    for (size_t i = 0; i < h.size(); ++i) {
+      out[2 * i] = h[i][0];
+      out[2 * i + 1] = h[i][1];
+    }
+  return h.size();
+}
+
+// Utilities::windings(x, points, &area): the winding number
+int orc_windings(const double* x, const double* p, size_t n, double* area) {
+  std::vector<P2> v(n);
+  for (size_t i = 0; i < n; ++i) v[i] = {{p[2 * i], p[2 * i + 1]}};
+  return windings({{x[0], x[1]}}, v, area);
+}
 
 // mode 0: find_crit; 1: find_magnification_contour(invmag); 2:
 // find_boundaries_of_caustics from the multiplicities `mult` of the grid

@@ -359,6 +359,7 @@ class Lens:
         self.cosmo = (Om, OL, h)
         self.h = _abi.slsim_lens()
         self.halos = []
+        self._implant = None
         _check(lib.slsim_lens_create(ctx.h, float(zsource), float(Om), float(OL), float(h),
                                      ctypes.byref(self.h)))
 
@@ -413,10 +414,17 @@ class Lens:
         zi = ctypes.c_double()
         _check(lib.slsim_lens_reset_source_plane(self.h, float(z), int(bool(nearest)),
                                                  ctypes.byref(j), ctypes.byref(zi)))
+        self._implant = zi.value
         return j.value, zi.value
 
     def RevertSourcePlane(self):
         _check(lib.slsim_lens_reset_source(self.h, -1.0))
+        self._implant = None
+
+    def getSourceZ(self):
+        """Lens::getSourceZ (include/lens.h:107): the implanted source's
+        redshift after ResetSourcePlane, else the lens's source plane."""
+        return self._implant if self._implant is not None else float(self.tables()["z"][-1])
 
     def tables(self):
         n = ctypes.c_int()
@@ -497,6 +505,102 @@ def _read_curves(h):
         lib.slsim_curves_destroy(h)
 
 
+def tightest_hull(ctx, points):
+    """Utilities::TightestHull(v) (TreeCode_link/curve_routines.cpp:3519-3592)
+    of a closed curve of (n, 2) points, the distance passes on ctx's device:
+    (m, 2) points on the curve tracing its outer boundary."""
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    h = _abi.slsim_curves()
+    _check(lib.slsim_tightest_hull(ctx.h, _ptr(p, ctypes.c_double), len(p), ctypes.byref(h)))
+    c, _, _, _ = _read_curves(h)
+    return c[0] if c else np.zeros((0, 2))
+
+
+def windings(x, points):
+    """Utilities::windings(x, points, &area) (TreeCode_link/curve_routines.cpp:
+    1709-1736): (winding number about x, absolute area) of a closed curve."""
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    q = np.ascontiguousarray(x, dtype=np.float64).reshape(2)
+    wn = ctypes.c_int32()
+    area = ctypes.c_double()
+    _check(lib.slsim_windings(_ptr(q, ctypes.c_double), _ptr(p, ctypes.c_double), len(p),
+                              ctypes.byref(wn), ctypes.byref(area)))
+    return wn.value, area.value
+
+
+class CriticalCurve:
+    """ImageFinding::CriticalCurve (include/grid_maintenance.h:245-330): the
+    critical curve as rays (x, y, A at z_source), its centre and area, the
+    caustic points, centre, outline (TightestHull) and area."""
+
+    def __init__(self):
+        self.critcurve_x = np.zeros((0, 2))
+        self.critcurve_y = np.zeros((0, 2))
+        self.critcurve_A = np.zeros((0, 4))
+        self.z_source = 0.0
+        self.type = SLSIM_CRIT_ND
+        self.touches_edge = False
+        self.critical_center = np.zeros(2)
+        self.critical_area = 0.0
+        self.caustic_curve_intersecting = np.zeros((0, 2))
+        self.caustic_center = np.zeros(2)
+        self.caustic_curve_outline = np.zeros((0, 2))
+        self.caustic_area = 0.0
+        self.caustic_intersections = -1
+
+
+class ImageFinding:
+    """The GridMap-based ImageFinding routines (TreeCode_link/find_crit.cpp)."""
+
+    @staticmethod
+    def find_crit(lens, gridmap, precision=SLSIM_FP64):
+        """ImageFinding::find_crit(lens, gridmap, critcurves)
+        (find_crit.cpp:1036-1096): the grid's critical curves (device
+        classification), each one's points shot to the lens's source in one
+        batch, centres as point means, areas by windings, caustic outlines by
+        tightest_hull."""
+        zs = lens.getSourceZ()
+        curves, hits, types, replaced = gridmap.find_crit()
+        out = []
+        for i, c in enumerate(curves):
+            cc = CriticalCurve()
+            cc.type = int(types[i])
+            cc.touches_edge = bool(hits[i]) if replaced else False
+            cc.z_source = zs
+            n = len(c)
+            cc.critcurve_x = c
+            center = np.zeros(2)
+            for j in range(n):            # the reference's running sum
+                center += c[j]
+            cc.critical_center = center / n
+            cc.critical_area = windings(cc.critical_center, c)[1]
+            y, A = lens.rayshooterInternal(c, zs=np.full(n, zs), precision=precision)
+            cc.critcurve_y, cc.critcurve_A = y, A
+            cc.caustic_curve_intersecting = y.copy()
+            center = np.zeros(2)
+            for j in range(n):
+                center += y[j]
+            cc.caustic_center = center / n
+            cc.caustic_curve_outline = tightest_hull(lens.ctx, y)
+            cc.caustic_area = windings(cc.caustic_center, cc.caustic_curve_outline)[1]
+            out.append(cc)
+        return out
+
+    @staticmethod
+    def find_magnification_contour(lens, gridmap, invmag, precision=SLSIM_FP64):
+        """ImageFinding::find_magnification_contour(lens, gridmap, invmag,
+        contour, hits_boundary) (find_crit.cpp:1098-1119): each contour of
+        det A = invmag shot to the lens's source: a list of (x, y, A) and
+        hits_boundary."""
+        curves, hits, _, _ = gridmap.find_magnification_contour(invmag)
+        zs = lens.getSourceZ()
+        out = []
+        for c in curves:
+            y, A = lens.rayshooterInternal(c, zs=np.full(len(c), zs), precision=precision)
+            out.append((c, y, A))
+        return out, hits
+
+
 def find_boundaries(bitmap):
     """Utilities::find_boundaries(bitmap, nx, points, hits_edge)
     (include/concave_hull.h:200-494) of a (ny, nx) bitmap: (curves in pixel
@@ -529,6 +633,102 @@ def in_curve(poly, points):
     _check(lib.slsim_in_curve(_ptr(h, ctypes.c_double), len(h), _ptr(q, ctypes.c_double), len(q),
                               _ptr(out, ctypes.c_int32)))
     return out[:len(q)].astype(bool)
+
+
+class Grid:
+    """Grid(lens, N1d, center, range) (TreeCode_link/grid_maintenance.cpp:
+    22-59), or Grid.rect(lens, Nx, center, rangeX, rangeY) (:67-129): an
+    image-plane grid refined cell by cell.  Its points (image and source
+    positions, magnification matrix, cell size) stay in device memory in the
+    reference's order; point ids index that order.  The reference's kd-trees
+    of the points are not built."""
+
+    def __init__(self, lens, N, center, range_, precision=SLSIM_FP64, range_y=None):
+        self.lens = lens
+        self.ctx = lens.ctx
+        self.ctx._adopt(self)
+        self.h = _abi.slsim_agrid()
+        c = np.ascontiguousarray(center, dtype=np.float64).reshape(2)
+        square = range_y is None
+        ry = float(range_) if square else float(range_y)
+        _check(lib.slsim_agrid_create(lens.h, int(N), _ptr(c, ctypes.c_double), float(range_), ry,
+                                      int(square), int(precision), ctypes.byref(self.h)))
+
+    @classmethod
+    def rect(cls, lens, Nx, center, range_x, range_y, precision=SLSIM_FP64):
+        return cls(lens, Nx, center, range_x, precision, range_y=range_y)
+
+    def info(self):
+        i = _abi.slsim_agrid_info()
+        _check(lib.slsim_agrid_get_info(self.h, ctypes.byref(i)))
+        return {"npoints": i.npoints, "ngrid_init": i.ngrid_init, "ngrid_init2": i.ngrid_init2,
+                "ngrid_block": i.ngrid_block, "image_p1": np.array(i.image_p1[:]),
+                "image_p2": np.array(i.image_p2[:]), "source_p1": np.array(i.source_p1[:]),
+                "source_p2": np.array(i.source_p2[:])}
+
+    def getNumberOfPoints(self):
+        return self.info()["npoints"]
+
+    def getInitNgrid(self):
+        return self.info()["ngrid_init"]
+
+    def getNgrid_block(self):
+        return self.info()["ngrid_block"]
+
+    def getInitRange(self):
+        """include/grid_maintenance.h:85: the image box's x width."""
+        i = self.info()
+        return i["image_p2"][0] - i["image_p1"][0]
+
+    def getInitCenter(self):
+        """TreeCode_link/image_finder.cpp:826-831: the image box's centre."""
+        i = self.info()
+        return (i["image_p2"] + i["image_p1"]) / 2
+
+    def points(self, first=0, count=None):
+        """(x (n, 2), y (n, 2), A (n, 4), gridsize (n,)) of points first ..."""
+        if count is None:
+            count = self.getNumberOfPoints() - first
+        x, y = np.zeros((count, 2)), np.zeros((count, 2))
+        A, gs = np.zeros((count, 4)), np.zeros(count)
+        _check(lib.slsim_agrid_read(self.h, int(first), int(count), _ptr(x, ctypes.c_double),
+                                    _ptr(y, ctypes.c_double), _ptr(A, ctypes.c_double),
+                                    _ptr(gs, ctypes.c_double)))
+        return x, y, A, gs
+
+    def RefineLeaves(self, ids):
+        """Grid::RefineLeaves (grid_maintenance.cpp:681-982): the ids of the
+        new points."""
+        ids = np.ascontiguousarray(ids, dtype=np.int64).reshape(-1)
+        first, count = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib.slsim_agrid_refine(self.h, ids.size, _ptr(ids, ctypes.c_int64),
+                                      ctypes.byref(first), ctypes.byref(count)))
+        return np.arange(first.value, first.value + count.value)
+
+    def RefineLeaf(self, i):
+        """Grid::RefineLeaf (grid_maintenance.cpp:543-679)."""
+        first, count = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib.slsim_agrid_refine_one(self.h, int(i), ctypes.byref(first),
+                                          ctypes.byref(count)))
+        return np.arange(first.value, first.value + count.value)
+
+    def EinsteinArea(self):
+        """Grid::EinsteinArea (grid_maintenance.cpp:368-378)."""
+        a = ctypes.c_double()
+        _check(lib.slsim_agrid_einstein_area(self.h, ctypes.byref(a)))
+        return a.value
+
+    def close(self):
+        if self.h:
+            if self.ctx.h:
+                lib.slsim_agrid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class GridMap:

@@ -78,6 +78,16 @@ class slsim_shoot_stats(ctypes.Structure):
                 ("n_guarded", ctypes.c_int32)]
 
 
+slsim_agrid = ctypes.c_void_p
+
+
+class slsim_agrid_info(ctypes.Structure):
+    _fields_ = [("npoints", ctypes.c_int64), ("ngrid_init", ctypes.c_int64),
+                ("ngrid_init2", ctypes.c_int64), ("ngrid_block", ctypes.c_int64),
+                ("image_p1", ctypes.c_double * 2), ("image_p2", ctypes.c_double * 2),
+                ("source_p1", ctypes.c_double * 2), ("source_p2", ctypes.c_double * 2)]
+
+
 class slsim_schedule(ctypes.Structure):
     _fields_ = [("tile_log2", ctypes.c_int32), ("queue", ctypes.c_int32),
                 ("order", ctypes.c_int32), ("graph", ctypes.c_int32),
@@ -221,6 +231,20 @@ SIGNATURES = {
     "slsim_find_boundaries": (c_int, [P(ctypes.c_uint8), c_size_t, c_size_t, P(slsim_curves)]),
     "slsim_convex_hull": (c_int, [P(c_double), c_size_t, P(c_i64), P(c_size_t)]),
     "slsim_in_curve": (c_int, [P(c_double), c_size_t, P(c_double), c_size_t, P(ctypes.c_int32)]),
+    "slsim_tightest_hull": (c_int, [slsim_ctx, P(c_double), c_size_t, P(slsim_curves)]),
+    "slsim_agrid_create": (c_int, [slsim_lens, c_size_t, P(c_double), c_double, c_double, c_int,
+                                   c_int, P(slsim_agrid)]),
+    "slsim_agrid_destroy": (c_int, [slsim_agrid]),
+    "slsim_agrid_get_info": (c_int, [slsim_agrid, P(slsim_agrid_info)]),
+    "slsim_agrid_read": (c_int, [slsim_agrid, c_i64, c_i64, P(c_double), P(c_double),
+                                 P(c_double), P(c_double)]),
+    "slsim_agrid_device": (c_int, [slsim_agrid, P(c_vp), P(c_vp), P(c_vp),
+                                   P(c_vp)]),
+    "slsim_agrid_refine": (c_int, [slsim_agrid, c_size_t, P(c_i64), P(c_i64), P(c_i64)]),
+    "slsim_agrid_refine_one": (c_int, [slsim_agrid, c_i64, P(c_i64), P(c_i64)]),
+    "slsim_agrid_einstein_area": (c_int, [slsim_agrid, P(c_double)]),
+    "slsim_windings": (c_int, [P(c_double), P(c_double), c_size_t, P(ctypes.c_int32),
+                               P(c_double)]),
     "slsim_curves_get_info": (c_int, [slsim_curves, P(slsim_curves_info)]),
     "slsim_curves_read": (c_int, [slsim_curves, P(c_i64), P(c_double), P(ctypes.c_int32),
                                   P(ctypes.c_int32)]),

@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libslsim_amd.so")
 
 SOURCES = ["tree_build.hip", "shoot.hip", "api.hip", "batch.hip", "smoothing.hip", "generator.cpp",
-           "formats.cpp", "halos.hip", "images.hip", "crit.hip"]
+           "formats.cpp", "halos.hip", "images.hip", "crit.hip", "agrid.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -67,7 +67,8 @@ def build_cpp_example(verbose=False):
     the library."""
     root = os.path.dirname(HERE)
     exes = []
-    for name in ("dropin_gridmap", "dropin_formats", "dropin_halos", "dropin_boundary"):
+    for name in ("dropin_gridmap", "dropin_formats", "dropin_halos", "dropin_boundary",
+                 "dropin_grid"):
         src = os.path.join(root, "tests", "cpp", name + ".cpp")
         exe = os.path.join(root, "tests", "cpp", name)
         cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "include"), src,

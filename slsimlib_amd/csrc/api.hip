@@ -549,6 +549,30 @@ int slsim_in_curve(const double* poly, size_t n, const double* points, size_t m,
   return SLSIM_OK;
 }
 
+int slsim_tightest_hull(slsim_ctx ctx, const double* points, size_t n, slsim_curves* out) {
+  if (!ctx || !out || (n && !points)) return arg_error("null argument");
+  *out = nullptr;
+  SLSIM_HIP_CHECK(hipSetDevice(ctx->device));
+  std::unique_ptr<slsim_curves_s> r(new slsim_curves_s);
+  std::vector<double> h;
+  const int rc = tightest_hull(ctx, points, n, h);
+  if (rc) return rc;
+  r->pts.swap(h);
+  r->off = {0, (int64_t)r->pts.size() / 2};
+  r->types = {0};
+  r->hits = {0};
+  r->hits_replaced = 1;
+  *out = r.release();
+  return SLSIM_OK;
+}
+
+int slsim_windings(const double* x, const double* points, size_t n, int32_t* winding,
+                   double* area) {
+  if (!x || !winding || !area || (n && !points)) return arg_error("null argument");
+  *winding = windings_host(x, points, n, area);
+  return SLSIM_OK;
+}
+
 int slsim_curves_get_info(slsim_curves c, slsim_curves_info* info) {
   if (!c || !info) return arg_error("null argument");
   info->n_curves = c->types.size();

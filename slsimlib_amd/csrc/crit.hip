@@ -597,4 +597,186 @@ bool in_curve_host(const double* poly, size_t n, double x0, double x1) {
   return in_polygon(x0, x1, poly, (int64_t)n);
 }
 
+// ------------------------------------------------------------ tightest hull
+// Utilities::TightestHull (TreeCode_link/curve_routines.cpp:3519-3592): the
+// pixels (1/100 of the curve's narrower extent, 2 R margins) that lie within
+// R = 1.58 pixels of a segment of the closed curve, the outer boundary of
+// that bitmap (Utilities::find_boundaries with outer_only), and each of its
+// points moved to the nearest point of the curve.  Both distance passes are
+// pixels (or boundary points) times segments: one thread a pixel (boundary
+// point), the segments read through the cache by every thread in step.
+
+namespace {
+
+// distance_to_segment (include/concave_hull.h:967-1001) with the closest
+// point, Point_2d's arithmetic in the reference's order, no contraction
+__device__ inline double seg_dist(double px, double py, double ax, double ay, double bx,
+                                  double by, double* cx, double* cy) {
+#pragma clang fp contract(off)
+  const double Dx = bx - ax, Dy = by - ay;
+  const double s = ((px - ax) * Dx + (py - ay) * Dy) / (Dx * Dx + Dy * Dy);
+  double ox, oy;
+  if (isnan(s) || s <= 0) {
+    ox = ax;
+    oy = ay;
+  } else if (s >= 1) {
+    ox = bx;
+    oy = by;
+  } else {
+    ox = ax + Dx * s;
+    oy = ay + Dy * s;
+  }
+  *cx = ox;
+  *cy = oy;
+  const double ex = ox - px, ey = oy - py;
+  return sqrt(ex * ex + ey * ey);
+}
+
+// pixel (i, j) at (xs[i], ys[j]) is set when some segment (v_k, v_k+1 mod n)
+// lies within R of it (the first such segment ends the scan)
+__global__ __launch_bounds__(256) void k_hull_bitmap(const double2* __restrict__ v, int64_t nv,
+                                                     const double* __restrict__ xs,
+                                                     const double* __restrict__ ys, int64_t nx,
+                                                     int64_t ny, double R,
+                                                     uint8_t* __restrict__ bitmap) {
+  const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= nx * ny) return;
+  const double px = xs[m % nx], py = ys[m / nx];
+  uint8_t in = 0;
+  for (int64_t k = 0; k < nv; ++k) {
+    const double2 a = v[k], b = v[k + 1 < nv ? k + 1 : 0];
+    double cx, cy;
+    if (R > seg_dist(px, py, a.x, a.y, b.x, b.y, &cx, &cy)) {
+      in = 1;
+      break;
+    }
+  }
+  bitmap[m] = in;
+}
+
+// each boundary point to the closest point of the curve (the first segment
+// at the smallest distance)
+__global__ __launch_bounds__(256) void k_hull_project(const double2* __restrict__ v, int64_t nv,
+                                                      double2* __restrict__ pts, int64_t np) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double2 p = pts[i];
+  double bx, by;
+  double best = seg_dist(p.x, p.y, v[0].x, v[0].y, v[1].x, v[1].y, &bx, &by);
+  for (int64_t k = 1; k < nv; ++k) {
+    const double2 b = v[k + 1 < nv ? k + 1 : 0];
+    double cx, cy;
+    const double d = seg_dist(p.x, p.y, v[k].x, v[k].y, b.x, b.y, &cx, &cy);
+    if (d < best) {
+      best = d;
+      bx = cx;
+      by = cy;
+    }
+  }
+  pts[i] = make_double2(bx, by);
+}
+
+}  // namespace
+
+int tightest_hull(slsim_ctx c, const double* v, size_t n_, std::vector<double>& out) {
+#pragma clang fp contract(off)
+  out.clear();
+  const int64_t n = (int64_t)n_;
+  if (n <= 3) {
+    out.assign(v, v + 2 * n);
+    return SLSIM_OK;
+  }
+  double ll[2] = {v[0], v[1]}, ur[2] = {v[0], v[1]};
+  for (int64_t k = 0; k < n; ++k) {
+    ll[0] = std::min(v[2 * k], ll[0]);
+    ll[1] = std::min(v[2 * k + 1], ll[1]);
+    ur[0] = std::max(v[2 * k], ur[0]);
+    ur[1] = std::max(v[2 * k + 1], ur[1]);
+  }
+  const double resolution = std::min(ur[0] - ll[0], ur[1] - ll[1]) / 100;
+  if (resolution == 0) {
+    out = {ll[0], ll[1], ur[0], ur[1]};
+    return SLSIM_OK;
+  }
+  const double R = resolution * std::sqrt(2.5);
+  ll[0] -= 2 * R;
+  ll[1] -= 2 * R;
+  ur[0] += 2 * R;
+  ur[1] += 2 * R;
+  const int64_t nx = (int64_t)((ur[0] - ll[0]) / resolution);
+  const int64_t ny = (int64_t)((ur[1] - ll[1]) / resolution);
+  if (nx < 2 || ny < 2 || nx * ny > ((int64_t)1 << 31)) {
+    set_error("tightest_hull: bitmap size out of range");
+    return SLSIM_ERR_ARG;
+  }
+  // the pixel positions as the reference steps them (p += resolution)
+  std::vector<double> xs((size_t)nx), ys((size_t)ny);
+  double p = ll[0];
+  for (int64_t i = 0; i < nx; ++i, p += resolution) xs[(size_t)i] = p;
+  p = ll[1];
+  for (int64_t j = 0; j < ny; ++j, p += resolution) ys[(size_t)j] = p;
+  hipStream_t st = c->stream;
+  Scratch sc(c);
+  double2* dv;
+  double *dxs, *dys;
+  uint8_t* dbm;
+  int rc;
+  if ((rc = sc.get(&dv, (size_t)n)) || (rc = sc.get(&dxs, (size_t)nx)) ||
+      (rc = sc.get(&dys, (size_t)ny)) || (rc = sc.get(&dbm, (size_t)(nx * ny))))
+    return rc;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dv, v, 16 * (size_t)n, hipMemcpyHostToDevice, st));
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dxs, xs.data(), 8 * (size_t)nx, hipMemcpyHostToDevice, st));
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dys, ys.data(), 8 * (size_t)ny, hipMemcpyHostToDevice, st));
+  k_hull_bitmap<<<(unsigned)((nx * ny + 255) / 256), 256, 0, st>>>(dv, n, dxs, dys, nx, ny, R, dbm);
+  SLSIM_HIP_CHECK(hipGetLastError());
+  std::vector<uint8_t> bm((size_t)(nx * ny));
+  SLSIM_HIP_CHECK(hipMemcpyAsync(bm.data(), dbm, bm.size(), hipMemcpyDeviceToHost, st));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(st));
+  bool any = false;
+  for (uint8_t b : bm) any |= b != 0;
+  if (!any) return SLSIM_OK;
+  // the outer boundary: the walk's first curve
+  slsim_curves_s env;
+  if ((rc = find_boundaries_host(bm.data(), (size_t)nx, (size_t)ny, &env))) return rc;
+  const int64_t np = env.off.size() > 1 ? env.off[1] : 0;
+  if (np == 0) return SLSIM_OK;
+  std::vector<double> pts((size_t)(2 * np));
+  for (int64_t i = 0; i < np; ++i) {
+    pts[(size_t)(2 * i)] = ll[0] + env.pts[(size_t)(2 * i)] * resolution;
+    pts[(size_t)(2 * i + 1)] = ll[1] + env.pts[(size_t)(2 * i + 1)] * resolution;
+  }
+  double2* dp;
+  if ((rc = sc.get(&dp, (size_t)np))) return rc;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dp, pts.data(), 16 * (size_t)np, hipMemcpyHostToDevice, st));
+  k_hull_project<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(dv, n, dp, np);
+  SLSIM_HIP_CHECK(hipGetLastError());
+  SLSIM_HIP_CHECK(hipMemcpyAsync(pts.data(), dp, 16 * (size_t)np, hipMemcpyDeviceToHost, st));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(st));
+  out.swap(pts);
+  return SLSIM_OK;
+}
+
+// Utilities::windings (TreeCode_link/curve_routines.cpp:1709-1736): the
+// winding number of the closed curve about x (isLeft's value taken as a
+// float, include/Tree.h:865-867) and the absolute shoelace area
+int windings_host(const double* x, const double* pt, size_t n, double* area) {
+#pragma clang fp contract(off)
+  int wn = 0;
+  *area = 0.0;
+  if (n < 3) return 0;
+  for (size_t i = 0; i < n; ++i) {
+    const size_t k = i < n - 1 ? i + 1 : 0;
+    *area += (pt[2 * i] + pt[2 * k]) * (pt[2 * i + 1] - pt[2 * k + 1]);
+    const float left = (float)((pt[2 * k] - pt[2 * i]) * (x[1] - pt[2 * i + 1]) -
+                               (x[0] - pt[2 * i]) * (pt[2 * k + 1] - pt[2 * i + 1]));
+    if (pt[2 * i + 1] <= x[1]) {
+      if (pt[2 * k + 1] > x[1] && left > 0) ++wn;
+    } else {
+      if (pt[2 * k + 1] <= x[1] && left < 0) --wn;
+    }
+  }
+  *area = std::fabs(*area) * 0.5;
+  return wn;
+}
+
 }  // namespace slsim

@@ -749,7 +749,8 @@ __device__ __forceinline__ void leaf32(int start, int count, bool open, bool nea
     const PartF p = parts[start + k];
     float d0 = (p.lx + ex) + exl, d1 = (p.ly + ey) + eyl;
     float d1s = d1 * d1;
-    float rr = fmaxf(fmaf(d0, d0, d1s), 1e-20f);
+    // (no clamp: |hh4| >= FLT_MIN takes r^2 = 0 to the near-field branch)
+    float rr = fmaf(d0, d0, d1s);
     float inv = open ? rcp32(rr) : 0.f;             // 0: no term for this lane
     float s = p.m * inv, w = s * inv;              // point mass
     if (open && rr < fabsf(p.hh4)) {

@@ -528,5 +528,7 @@ int find_boundaries_of_caustics(slsim_ctx c, const GridGeom& g, const double* y_
 int find_boundaries_host(const uint8_t* bitmap, size_t nx, size_t ny, slsim_curves_s* out);
 std::vector<int64_t> convex_hull_host(const double* p, size_t n);
 bool in_curve_host(const double* poly, size_t n, double x0, double x1);
+int tightest_hull(slsim_ctx c, const double* v, size_t n, std::vector<double>& out);
+int windings_host(const double* x, const double* pt, size_t n, double* area);
 
 }  // namespace slsim

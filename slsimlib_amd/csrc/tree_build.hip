@@ -600,8 +600,9 @@ __global__ void k_pack_parts(size_t n, const double* __restrict__ xs, const doub
   f.ly = (float)(p.y - com[2 * l + 1]);
   f.m = ms[i];
   // (with a margin of 2^-10 for the FP32 walk's offsets: the walk takes
-  // the particles below it to the FP64 offsets and the exact 4 h^2 test)
-  const float hh4 = 4.f * hs[i] * hs[i] * (1.f + 1.f / 1024.f);
+  // the particles below it to the FP64 offsets and the exact 4 h^2 test;
+  // and at least FLT_MIN, so that a particle on the ray takes that branch)
+  const float hh4 = fmaxf(4.f * hs[i] * hs[i] * (1.f + 1.f / 1024.f), 1.17549435e-38f);
   f.hh4 = listed ? hh4 : -hh4;
   pf[i] = f;
 }

@@ -287,3 +287,80 @@ def test_rectangular_grid_positions_and_crit():
     cont, _, _, _ = O.find_magnification_contour(nx, x, A, rx, 0.2, ny=ny)
     assert len(cont) == 1
     assert np.all(np.abs(_radius(cont[0], (0.11, -0.04)) - te / 0.8) <= res)
+
+
+# ------------------------------------------------- TightestHull and windings
+def _seg_dist(p, v):
+    """Distance of each point p (m, 2) to the closed polygon v (n, 2)."""
+    a, b = v, np.roll(v, -1, axis=0)
+    d = b - a
+    t = np.clip(((p[:, None, :] - a[None]) * d[None]).sum(2) / (d * d).sum(1)[None], 0, 1)
+    q = a[None] + t[..., None] * d[None]
+    return np.sqrt(((q - p[:, None, :]) ** 2).sum(2)).min(1)
+
+
+def _shoelace(v):
+    x, y = v[:, 0], v[:, 1]
+    return 0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
+
+
+def test_windings_closed_forms():
+    sq = np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]])
+    assert O.windings((0.5, 0.5), sq) == (1, 1.0)
+    assert O.windings((0.5, 0.5), sq[::-1]) == (-1, 1.0)
+    assert O.windings((1.5, 0.5), sq)[0] == 0
+    assert O.windings((0.2, 0.2), sq[:3]) == (1, 0.5)
+    assert O.windings((0.0, 0.0), sq[:2]) == (0, 0.0)          # fewer than 3 points
+    n, r = 37, 2.5
+    ph = 2 * np.pi * np.arange(n) / n
+    poly = np.stack([r * np.cos(ph) + 1, r * np.sin(ph) - 2], 1)
+    wn, area = O.windings((1.0, -2.0), poly)
+    assert wn == 1 and area == pytest.approx(0.5 * n * r * r * np.sin(2 * np.pi / n), rel=1e-13)
+    # a curve wound twice about the point
+    wn, _ = O.windings((1.0, -2.0), np.concatenate([poly, poly]))
+    assert wn == 2
+
+
+def test_tightest_hull_small_and_degenerate():
+    p = np.array([[0.0, 0.0], [1.0, 2.0], [3.0, 1.0]])
+    assert np.array_equal(O.tightest_hull(p), p)
+    line = np.stack([np.full(7, 0.5), np.linspace(0, 1, 7)], 1)
+    assert np.array_equal(O.tightest_hull(line), [[0.5, 0.0], [0.5, 1.0]])
+
+
+@pytest.mark.parametrize("shape", ["circle", "star", "eight"])
+def test_tightest_hull_outlines(shape):
+    t = np.linspace(0, 2 * np.pi, 400, endpoint=False)
+    if shape == "circle":
+        v = np.stack([np.cos(t), np.sin(t)], 1) * 3 + [1, -1]
+        inner = v
+    elif shape == "star":
+        r = 1 + 0.45 * np.cos(5 * t)
+        v = np.stack([r * np.cos(t), r * np.sin(t)], 1)
+        inner = v
+    else:
+        v = np.stack([np.sin(t), np.sin(t) * np.cos(t)], 1)   # a figure eight
+        inner = None
+    h = O.tightest_hull(v)
+    assert len(h) > 50
+    # every outline point lies on the curve
+    span = np.ptp(v, 0).min()
+    assert _seg_dist(h, v).max() <= 1e-12 * span
+    if inner is not None:
+        # a simple curve's outline encloses its area, to about a pixel of the
+        # 1/100-extent bitmap times its perimeter
+        per = np.sqrt((np.diff(np.vstack([v, v[:1]]), axis=0) ** 2).sum(1)).sum()
+        assert abs(_shoelace(h) - _shoelace(v)) <= per * span / 100
+    else:
+        # the eight's outline goes round both lobes
+        assert (h[:, 0] > 0.5).any() and (h[:, 0] < -0.5).any()
+        assert _shoelace(h) == pytest.approx(4 / 3, rel=0.05)
+
+
+def test_product_windings_equal_oracle():
+    import slsimlib_amd as S
+    rng = np.random.default_rng(3)
+    for n in (3, 4, 17, 200):
+        poly = rng.normal(size=(n, 2))
+        for q in rng.normal(size=(40, 2)):
+            assert S.windings(q, poly) == O.windings(q, poly)

@@ -251,3 +251,76 @@ def test_rectangular_gridmap_shoot(ctx, shot):
     got = g.find_crit()
     _same(got, O.find_crit(64, x, g.y(), g.A(), rg, ny=ny))
     g.close()
+
+
+def _hull_cases():
+    rng = np.random.default_rng(21)
+    t = np.linspace(0, 2 * np.pi, 500, endpoint=False)
+    yield np.stack([np.cos(t), np.sin(t)], 1) * 2e-6 + 1e-5      # radians-sized circle
+    r = 1 + 0.45 * np.cos(5 * t)
+    yield np.stack([r * np.cos(t), r * np.sin(t)], 1)
+    yield np.stack([np.sin(t), 0.5 * np.sin(2 * t)], 1)            # a figure eight
+    # an astroid (four cusps, the caustic of an elliptical lens) with noise
+    a = np.stack([np.cos(t) ** 3, np.sin(t) ** 3], 1) * 0.3
+    yield a + rng.normal(size=a.shape) * 1e-3
+    yield rng.normal(size=(60, 2))                                  # a scribble
+    yield np.array([[0.0, 0.0], [1.0, 1.0], [2.0, 0.5]])            # three points
+    yield np.stack([np.full(9, 0.25), np.linspace(0, 1, 9)], 1)     # zero width
+
+
+def test_tightest_hull_equals_oracle(ctx):
+    for k, v in enumerate(_hull_cases()):
+        print("case: tightest hull %d (%d points)" % (k, len(v)), flush=True)
+        assert np.array_equal(S.tightest_hull(ctx, v), O.tightest_hull(v)), k
+
+
+def test_image_finding_find_crit(ctx):
+    """ImageFinding::find_crit (find_crit.cpp:1036-1096) on the 65,536-particle
+    NFW halo: the critical curves of the device grid and their rays equal the
+    oracle's composition on the same grid outputs (curve points, types and
+    centres bit for bit, caustic points to the FP64 shooter's 1e-9, the
+    outline and areas of the device's caustic points bit for bit)."""
+    lens, (px, py, pm, ph) = W.build_c1_lens(ctx, n=65536)
+    L = O.Lens(W.C1["z_source"])
+    L.moveinMainHalo(O.Tree(px, py, pm, ph), W.C1["z_lens"])
+    N, rg = 128, W.grid_range(30.0)
+    g = S.GridMap(lens, N, (0.0, 0.0), rg, S.SLSIM_FP64)
+    print("case: ImageFinding::find_crit NFW halo N=128", flush=True)
+    got = S.ImageFinding.find_crit(lens, g)
+    want = O.image_finding_find_crit(L, N, g.positions(), g.y(), g.A(), rg)
+    assert len(got) == len(want) > 0
+    assert S.SLSIM_CRIT_TANGENTIAL in [c.type for c in got]
+    for c, w in zip(got, want):
+        assert c.type == w["type"] and c.touches_edge == w["touches_edge"]
+        assert c.z_source == w["z_source"]
+        assert np.array_equal(c.critcurve_x, w["critcurve"])
+        assert np.array_equal(c.critical_center, w["critical_center"])
+        assert c.critical_area == w["critical_area"]
+        scale = np.abs(w["critcurve_y"]).max() + np.abs(w["critcurve"]).max()
+        assert np.abs(c.caustic_curve_intersecting - w["critcurve_y"]).max() <= 1e-9 * scale
+        assert np.allclose(c.critcurve_A, w["critcurve_A"], rtol=0, atol=1e-9)
+        assert np.array_equal(c.caustic_curve_outline,
+                              O.tightest_hull(c.caustic_curve_intersecting))
+        assert c.caustic_area == O.windings(c.caustic_center, c.caustic_curve_outline)[1]
+        assert c.caustic_intersections == -1
+    # the largest tangential curve of this halo is its Einstein ring: its
+    # windings area and the area of the pixels whose smaller eigenvalue is
+    # negative (the ring's inside, radial region included) agree
+    ring = max((c for c in got if c.type == S.SLSIM_CRIT_TANGENTIAL),
+               key=lambda c: c.critical_area)
+    res = rg / (N - 1)
+    A = g.A()
+    m = 0.5 * (A[:, 0] + A[:, 3])
+    det = A[:, 0] * A[:, 3] - A[:, 1] * A[:, 2]
+    inside = (m - np.sqrt(np.abs(m * m - det)) < 0).sum() * res * res
+    assert ring.critical_area == pytest.approx(inside, rel=0.15)
+    # the magnification contours as rays to the source
+    cont, hits = S.ImageFinding.find_magnification_contour(lens, g, 0.0)
+    c0, h0, _, _ = O.find_magnification_contour(N, g.positions(), g.A(), rg, 0.0)
+    assert len(cont) == len(c0) and np.array_equal(hits, h0)
+    for (x, y, Am), xw in zip(cont, c0):
+        assert np.array_equal(x, xw)
+        yw, Aw = L.shoot(xw, zs=np.full(len(xw), W.C1["z_source"]))
+        assert np.abs(y - yw).max() <= 1e-9 * (np.abs(yw).max() + np.abs(xw).max())
+    g.close()
+    lens.close()
