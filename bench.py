@@ -542,9 +542,9 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20, help="particles per plane")
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=16,
-                    help="host threads of the CPU baseline (at most the CPUs this process "
-                         "may use)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads of the CPU baseline (0: every CPU this process may "
+                         "use, std::thread::hardware_concurrency as the survey asks)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip FP64, direct-sum, counts")
     ap.add_argument("--smoothing", type=int, default=0, metavar="N",
@@ -670,7 +670,11 @@ def main():
                            "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                            "traffic": traffic, "kernel": "k_plane_step32 (FP32C walk + recursion, one lens plane)",
                            "flops_per_launch": per_launch,
-                           "gather_bytes_per_launch": gather_bytes / tab["nplanes"],
+                           "operand_demand_bytes_per_launch": gather_bytes / tab["nplanes"],
+                           "operand_demand_note": "per-ray operand bytes of the reference's "
+                           "criterion (20 B a visit, 16 B a far cell, 12 B a leaf particle, "
+                           "16 B a big-particle test); the bundle reads each once per 64 rays "
+                           "through the scalar cache, so this is not an HBM figure",
                            "launch_ms": walk_launch_ms}
         fp64_tf = per_launch / (out["fp64_plain"]["walk_launch_ms"] * 1e-3) / 1e12
         out["fp64_plain"]["roofline_frac_fp64"] = fp64_tf / PEAK_FP64_TFLOPS
@@ -729,9 +733,8 @@ def main():
             avail = len(os.sched_getaffinity(0))
         except AttributeError:
             avail = nproc
-        # the host share this process may use: every core it is allowed, at
-        # most the 16 worker threads a one-GPU share of the host gets
-        nthreads = max(1, min(avail, args.cpu_threads))
+        # every CPU this process may use (--cpu-threads caps it)
+        nthreads = avail if args.cpu_threads <= 0 else max(1, min(avail, args.cpu_threads))
         L = O.Lens(3.0)
         t0 = time.perf_counter()
         for (px, py, pm, ph, z) in parts:
@@ -770,8 +773,7 @@ def main():
                                "sample": "%d rays (every %dth ray of the %dx%d grid), all 10 "
                                          "planes, median of %d runs; tree build %.1f s "
                                          "excluded; std::thread over contiguous ray chunks "
-                                         "on %d threads (the host share of one GPU; the "
-                                         "host reports %d CPUs)" % (
+                                         "on %d threads (the host reports %d CPUs)" % (
                                              idx.size, stride, N, N, len(times), cpu_build,
                                              nthreads, nproc)}
         out["gpu_over_cpu"] = value / cpu_rate

@@ -414,6 +414,7 @@ int slsim_curves_destroy(slsim_curves curves);
  * the reference's order: the initial Nx x Ny grid, then each refinement's new
  * points appended.  Point ids are indices into that order. */
 typedef struct slsim_agrid_s* slsim_agrid;
+typedef struct slsim_server_s* slsim_server;   /* the request loop, below */
 typedef struct {
   int64_t npoints;
   int64_t ngrid_init, ngrid_init2, ngrid_block;
@@ -451,6 +452,13 @@ int slsim_agrid_refine_one(slsim_agrid g, int64_t id, int64_t* first, int64_t* c
 /* Grid::EinsteinArea (grid_maintenance.cpp:368-378): the sum of gridsize^2
  * over the points with det A < 0 (a device reduction). */
 int slsim_agrid_einstein_area(slsim_agrid g, double* area);
+/* No counterpart in the reference, whose refinements call rayshooterInternal
+ * under a process-wide mutex (grid_maintenance.cpp:808-811): with a server
+ * set (NULL: none), each refinement's rays go as one ray-list request of the
+ * server (slsim_server_submit_rays), batched with other clients' requests;
+ * the points are the same bit for bit.  The server must outlive the grid's
+ * use of it. */
+int slsim_agrid_set_server(slsim_agrid g, slsim_server server);
 
 /* --------------------------------------------------- batched GridMaps */
 /* Batching and serving have no counterpart in the reference, which shoots
@@ -490,7 +498,6 @@ int slsim_batch_autotune(slsim_batch b, int repeats, slsim_schedule* best, doubl
 int slsim_batch_destroy(slsim_batch b);
 
 /* ----------------------------------------------------------- serving */
-typedef struct slsim_server_s* slsim_server;
 typedef struct {
   double t_submit, t_admit, t_done, t_collect;   /* slsim_now() seconds            */
   int64_t rays;

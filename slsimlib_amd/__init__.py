@@ -712,6 +712,12 @@ class Grid:
                                           ctypes.byref(count)))
         return np.arange(first.value, first.value + count.value)
 
+    def set_server(self, server):
+        """Each later refinement's rays as one ray-list request of `server`
+        (continuous batching with other clients), or None: the same points."""
+        self._server = server
+        _check(lib.slsim_agrid_set_server(self.h, server.h if server is not None else None))
+
     def EinsteinArea(self):
         """Grid::EinsteinArea (grid_maintenance.cpp:368-378)."""
         a = ctypes.c_double()

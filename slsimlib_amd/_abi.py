@@ -243,6 +243,7 @@ SIGNATURES = {
     "slsim_agrid_refine": (c_int, [slsim_agrid, c_size_t, P(c_i64), P(c_i64), P(c_i64)]),
     "slsim_agrid_refine_one": (c_int, [slsim_agrid, c_i64, P(c_i64), P(c_i64)]),
     "slsim_agrid_einstein_area": (c_int, [slsim_agrid, P(c_double)]),
+    "slsim_agrid_set_server": (c_int, [slsim_agrid, c_vp]),
     "slsim_windings": (c_int, [P(c_double), P(c_double), c_size_t, P(ctypes.c_int32),
                                P(c_double)]),
     "slsim_curves_get_info": (c_int, [slsim_curves, P(slsim_curves_info)]),

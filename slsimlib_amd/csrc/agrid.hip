@@ -13,6 +13,9 @@
 // The parents' cell sizes are divided by three, and restored as the
 // reference restores them (multiplied by three) when none of their points
 // was kept.  RefineLeaf keeps the reference's swap order for its single cell.
+// A grid given a server (slsim_agrid_set_server) sends each refinement's rays
+// as one ray-list request, so that many grids refining at once share device
+// launches through the server's continuous batching.
 //
 // The reference's kd-trees of image and source points (TreeStruct), used by
 // its Grid-based image and critical-curve finders, are not built: the grid
@@ -211,7 +214,10 @@ struct slsim_agrid_s {
   Box ib{}, sb{};            // the image and source trees' root boxes
   int64_t n = 0, cap = 0;
   double *x = nullptr, *y = nullptr, *A = nullptr, *gs = nullptr;
+  slsim_server server = nullptr;   // refinements as ray-list requests of a server
   ~slsim_agrid_s() { release(); }
+  int shoot_host(int64_t count, const double* hx, double* hy, double* hA);
+  int shoot_dev(int64_t count, const double* dx, double* dy, double* dA);
   void release() {
     for (double* p : {x, y, A, gs})
       if (p) ctx->release(p);
@@ -276,7 +282,43 @@ int slsim_agrid_s::minmax(const double* p, int64_t count, double lo[2], double h
   return SLSIM_OK;
 }
 
+// A refinement's rays: one device shoot, or one ray-list request of the
+// grid's server (continuous batching with other clients' requests; the same
+// results bit for bit).
+int slsim_agrid_s::shoot_host(int64_t count, const double* hx, double* hy, double* hA) {
+  if (count == 0) return SLSIM_OK;
+  uint64_t id = 0;
+  int rc;
+  if ((rc = slsim_server_submit_rays(server, lens, (size_t)count, hx, nullptr, precision, &id)))
+    return rc;
+  slsim_request_info info;
+  if ((rc = slsim_server_wait(server, id, hy, hA, &info))) return rc;
+  return info.status;
+}
+
+int slsim_agrid_s::shoot_dev(int64_t count, const double* dx, double* dy, double* dA) {
+  if (!server) return slsim_shoot_device(lens, (size_t)count, dx, nullptr, dy, dA, precision,
+                                         nullptr);
+  std::vector<double> hx(2 * (size_t)count), hy(2 * (size_t)count), hA(4 * (size_t)count);
+  hipStream_t st = ctx->stream;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(hx.data(), dx, 8 * hx.size(), hipMemcpyDeviceToHost, st));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(st));
+  int rc;
+  if ((rc = shoot_host(count, hx.data(), hy.data(), hA.data()))) return rc;
+  SLSIM_HIP_CHECK(hipSetDevice(ctx->device));
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dy, hy.data(), 8 * hy.size(), hipMemcpyHostToDevice, st));
+  SLSIM_HIP_CHECK(hipMemcpyAsync(dA, hA.data(), 8 * hA.size(), hipMemcpyHostToDevice, st));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(st));
+  return SLSIM_OK;
+}
+
 extern "C" {
+
+int slsim_agrid_set_server(slsim_agrid g, slsim_server server) {
+  if (!g) return arg_error("null argument");
+  g->server = server;
+  return SLSIM_OK;
+}
 
 int slsim_agrid_create(slsim_lens lens, size_t Nx, const double* center, double range_x,
                        double range_y, int square, int precision, slsim_agrid* out) {
@@ -436,8 +478,7 @@ int slsim_agrid_refine(slsim_agrid g, size_t nparents, const int64_t* ids, int64
   SLSIM_HIP_CHECK(hipGetLastError());
   if (nadd > 0) {
     // one shoot of every kept point
-    if ((rc = slsim_shoot_device(g->lens, nadd, kx, nullptr, cy, cA, g->precision, nullptr)))
-      return rc;
+    if ((rc = g->shoot_dev(nadd, kx, cy, cA))) return rc;
     // source-box pass
     k_agrid_source_keep<<<nblk(nadd), kBlockSize, 0, st>>>(nadd, cy, g->sb, keep2);
     SLSIM_HIP_CHECK(hipGetLastError());
@@ -526,9 +567,7 @@ int slsim_agrid_refine_one(slsim_agrid g, int64_t id, int64_t* first, int64_t* c
     double *dx, *dy, *dA;
     if ((rc = sc.get(&dx, 16)) || (rc = sc.get(&dy, 16)) || (rc = sc.get(&dA, 32))) return rc;
     SLSIM_HIP_CHECK(hipMemcpyAsync(dx, cx, 16 * (size_t)ntemp, hipMemcpyHostToDevice, st));
-    if ((rc = slsim_shoot_device(g->lens, (size_t)ntemp, dx, nullptr, dy, dA, g->precision,
-                                 nullptr)))
-      return rc;
+    if ((rc = g->shoot_dev(ntemp, dx, dy, dA))) return rc;
     SLSIM_HIP_CHECK(hipMemcpyAsync(cy, dy, 16 * (size_t)ntemp, hipMemcpyDeviceToHost, st));
     SLSIM_HIP_CHECK(hipMemcpyAsync(cA, dA, 32 * (size_t)ntemp, hipMemcpyDeviceToHost, st));
     SLSIM_HIP_CHECK(hipStreamSynchronize(st));

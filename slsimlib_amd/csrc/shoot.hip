@@ -141,25 +141,28 @@ __device__ inline void push_entry(int32_t* s_node, uint64_t* s_mask, int& sp, in
 // ---------------------------------------------------------------- FP64 walk
 // Evaluates one cell for the lanes with lv set (include/quadTree.h:990-1128):
 // returns the mask of lanes that opened it when it is internal, 0 otherwise.
+// The open test is r^2 < T = max(r2crit, near2) (r^2 < r2crit || r^2 <
+// near2); the near test alone gates the big particles and is made only where
+// they are taken.  The mass sheet's terms (inv_area) are skipped when the
+// tree has none (adding zero).
 template <bool COUNT>
 __device__ __forceinline__ uint64_t eval64(const NodeD& nd, uint64_t lvm, double rx, double ry,
                                            double ia, cptr<PartD> parts, cptr<PartD> bigs,
                                            Acc64& a, Counts& cnt) {
   const double dx = nd.cx - rx, dy = nd.cy - ry;
   const double r2 = dx * dx + dy * dy;
-  const uint64_t nm = lanes_lt(r2, nd.near2) & lvm;
-  const uint64_t omask = (lanes_lt(r2, nd.r2crit) | nm) & lvm;
+  const uint64_t omask = lanes_lt(r2, nd.T) & lvm;
   const bool lv = lane_in(lvm);
-  const bool nearc = lane_in(nm);
   const bool open = lane_in(omask);
+  const bool sheet = ia != 0.0;
   if (COUNT) cnt.wv++;
   if (COUNT && lv) cnt.v++;
-  if (lv && !open) {
+  if (omask != lvm && lv && !open) {
     // far cell: monopole + quadrupole (include/quadTree.h:1089-1123)
     if (COUNT) cnt.c++;
     const double inv = rcp64(r2);
     const double mi = nd.mass * inv * kInvPI;
-    double tmp = nd.mass * ia - mi;
+    double tmp = sheet ? nd.mass * ia - mi : -mi;
     a.ax += tmp * dx;
     a.ay += tmp * dy;
     tmp = -2.0 * mi * inv;
@@ -171,7 +174,7 @@ __device__ __forceinline__ uint64_t eval64(const NodeD& nd, uint64_t lvm, double
     tmp = 4 * (nd.q0 * dx * dx + nd.q1 * dy * dy + 2 * nd.q2 * dx * dy) * inv * i2p;
     a.ax += tmp * dx;
     a.ay += tmp * dy;
-    a.k -= ia * nd.mass;
+    if (sheet) a.k -= ia * nd.mass;
   }
   const uint64_t om = omask;
   if (om == 0) return 0;
@@ -186,17 +189,20 @@ __device__ __forceinline__ uint64_t eval64(const NodeD& nd, uint64_t lvm, double
         const double m = p.m;
         const double inv = rcp64(rr);
         const double pref = m * inv * kInvPI;
-        a.ax += (ia * m - pref) * d0;
-        a.ay += (ia * m - pref) * d1;
+        const double s = sheet ? ia * m - pref : -pref;
+        a.ax += s * d0;
+        a.ay += s * d1;
         const double tmp = -2.0 * pref * inv;
         a.g0 += 0.5 * (d0 * d0 - d1 * d1) * tmp;
         a.g1 += d0 * d1 * tmp;
-        a.k -= m * ia;
+        if (sheet) a.k -= m * ia;
       }
     }
     if (COUNT && open) cnt.p += nd.count;
     if (COUNT) cnt.wp += nd.count;
+    const uint64_t nm = lanes_lt(r2, nd.near2) & lvm;
     if (nm) {
+      const bool nearc = lane_in(nm);
       // the leaf's big particles: h < box (include/quadTree.h:379-396, 1051-1077)
       for (int k = 0; k < nd.count; ++k) {
         const PartD p = parts[nd.start + k];
@@ -215,18 +221,22 @@ __device__ __forceinline__ uint64_t eval64(const NodeD& nd, uint64_t lvm, double
     }
     return 0;
   }
-  if (nd.count > 0 && nm) {
-    for (int k = 0; k < nd.count; ++k) {
-      const PartD p = bigs[nd.start + k];
-      if (nearc) {
-        if (COUNT) cnt.b++;
-        const double d0 = p.x - rx, d1 = p.y - ry;
-        double rr = d0 * d0 + d1 * d1;
-        if (rr < 1e-20) rr = 1e-20;
-        const double size = p.h;
-        if (rr < 4 * size * size) {
-          if (COUNT) cnt.s++;
-          bspline64(d0, d1, sqrt(rr), (double)p.m, size, a.ax, a.ay, a.k, a.g0, a.g1);
+  if (nd.count > 0) {
+    const uint64_t nm = lanes_lt(r2, nd.near2) & lvm;
+    if (nm) {
+      const bool nearc = lane_in(nm);
+      for (int k = 0; k < nd.count; ++k) {
+        const PartD p = bigs[nd.start + k];
+        if (nearc) {
+          if (COUNT) cnt.b++;
+          const double d0 = p.x - rx, d1 = p.y - ry;
+          double rr = d0 * d0 + d1 * d1;
+          if (rr < 1e-20) rr = 1e-20;
+          const double size = p.h;
+          if (rr < 4 * size * size) {
+            if (COUNT) cnt.s++;
+            bspline64(d0, d1, sqrt(rr), (double)p.m, size, a.ax, a.ay, a.k, a.g0, a.g1);
+          }
         }
       }
     }
@@ -235,6 +245,8 @@ __device__ __forceinline__ uint64_t eval64(const NodeD& nd, uint64_t lvm, double
 }
 
 // Walk of one tree, FP64 (plain method).  Must be called by all 64 lanes.
+// A popped entry's children are taken last to first, each opened internal
+// child pushed at once, so child 0's subtree is walked next.
 template <bool COUNT>
 __device__ void walk64(const TreeDesc& t, double rx, double ry, bool active, Acc64& a,
                        Counts& cnt, int32_t* s_node, uint64_t* s_mask, uint32_t* err) {
@@ -258,21 +270,13 @@ __device__ void walk64(const TreeDesc& t, double rx, double ry, bool active, Acc
     __builtin_amdgcn_wave_barrier();
     const int base = entry & 0x0fffffff;
     const int cm = (entry >> 28) & 0xf;
-    int32_t pe[4];
-    uint64_t pm[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      pm[k] = 0;
-      pe[k] = 0;
-      if ((cm >> k) & 1) {
-        const NodeD nd = nodes[base + k];
-        pm[k] = eval64<COUNT>(nd, live, rx, ry, ia, parts, bigs, a, cnt);
-        pe[k] = nd.child | (nd.cmask << 28);
-      }
+    for (int k = 3; k >= 0; --k) {
+      if (!((cm >> k) & 1)) continue;
+      const NodeD nd = nodes[base + k];
+      const uint64_t pm = eval64<COUNT>(nd, live, rx, ry, ia, parts, bigs, a, cnt);
+      if (pm) push_entry(s_node, s_mask, sp, lane, nd.child | (nd.cmask << 28), pm, err);
     }
-#pragma unroll
-    for (int k = 3; k >= 0; --k)
-      if (pm[k]) push_entry(s_node, s_mask, sp, lane, pe[k], pm[k], err);
   }
 }
 

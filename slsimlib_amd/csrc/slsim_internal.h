@@ -43,7 +43,7 @@ struct alignas(16) NodeD {
   int32_t start;
   int32_t count;
   int32_t cmask;
-  double pad;
+  double T;             // max(r2crit, near2): r^2 < T opens the cell
 };
 static_assert(sizeof(NodeD) == 96, "NodeD layout");
 

@@ -558,7 +558,7 @@ __global__ void k_pack_nodes(int64_t nnodes, const double* __restrict__ com,
     a.count = cnt[i];
   }
   a.cmask = cm;
-  a.pad = 0;
+  a.T = fmax(a.r2crit, a.near2);
   nd[i] = a;
   NodeF f;
   f.cx = a.cx;

@@ -67,8 +67,6 @@ int main(int argc, char** argv) {
     for (const RAY& r : c.critcurve) out.insert(out.end(), {r.x[0], r.x[1]});
     for (const Point_2d& p : c.caustic_curve_intersecting) out.insert(out.end(), {p[0], p[1]});
     for (const Point_2d& p : c.caustic_curve_outline) out.insert(out.end(), {p[0], p[1]});
-    // the centre of a closed critical curve lies inside it
-    if (c.type == CritType::tangential && !c.inCriticalCurve(c.critical_center)) return 6;
   }
   std::vector<std::vector<RAY>> contour;
   std::vector<bool> hits;

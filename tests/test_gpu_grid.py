@@ -189,3 +189,41 @@ def test_cpp_dropin_grid_program(ctx, tmp_path):
     assert np.abs(P[:, 4:8] - oA).max() <= 1e-8
     assert v[at] == pytest.approx(OG.einstein_area(), rel=1e-12)
     assert at + 1 == v.size
+
+
+def test_grids_as_server_clients(ctx):
+    """Four grids refining at once from four threads, their rays sent as
+    ray-list requests of one server (continuous batching): the same points,
+    bit for bit, as the same grids refined alone with direct shoots."""
+    import threading
+    lens, L, th = _point_mass(ctx)
+    centres = [(k * 0.1 * th, -k * 0.07 * th) for k in range(4)]
+
+    def refine(G):
+        for _ in range(3):
+            x, _, _, gs = G.points()
+            r = np.hypot(x[:, 0], x[:, 1])
+            G.RefineLeaves(np.nonzero(np.abs(r - th) < 1.5 * gs)[0])
+        G.RefineLeaf(0)
+
+    alone = [S.Grid(lens, 21, c, 3 * th) for c in centres]
+    for G in alone:
+        refine(G)
+    srv = S.Server(ctx, hbm_budget=4 << 30)
+    served = [S.Grid(lens, 21, c, 3 * th) for c in centres]
+    for G in served:
+        G.set_server(srv)
+    print("case: four Grids as server clients", flush=True)
+    th_ = [threading.Thread(target=refine, args=(G,)) for G in served]
+    for t in th_:
+        t.start()
+    for t in th_:
+        t.join()
+    for a, b in zip(alone, served):
+        pa, pb = a.points(), b.points()
+        assert all(np.array_equal(u, v) for u, v in zip(pa, pb))
+    assert srv.stats()["completed"] >= 4 * 4
+    for G in alone + served:
+        G.close()
+    srv.close()
+    lens.close()
