@@ -1255,6 +1255,17 @@ void k_plane_step_queue32(const StepJob* __restrict__ jobs, const WorkItem* __re
   plane_step_queue<SLSIM_FP32C, false>(jobs, items, nitems, counter, err);
 }
 
+// The FP64 walk of particle trees at 8 waves per SIMD: the 64-VGPR cap spills
+// a few FP64 sums of the per-tree loop to scratch (once per tree), and the
+// extra waves hide more of the walk's latency (10-plane 2^20-particle field,
+// 2048^2 rays: 38.7 -> 37.3 ms per plane).  Halo-of-halos planes, and the
+// work-queue and fused schedules, keep the uncapped kernels.
+__global__ __launch_bounds__(kWave * kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(8)))
+void k_plane_step64(const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items,
+                    uint32_t nitems, uint32_t* __restrict__ err) {
+  plane_step_grid<SLSIM_FP64, false, false>(jobs, items, nitems, err);
+}
+
 // ------------------------------------------------------- lone-ray FP64 walk
 // One plane step of one ray per wavefront, for a few scattered rays (the
 // precision form's guard), where a bundle walk would leave 63 lanes idle and
@@ -1843,6 +1854,8 @@ void launch_step_hk(int precision, bool count, unsigned nb, hipStream_t st, cons
     k_plane_step<SLSIM_FP32C, false, HK><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
   else if (precision == SLSIM_DIRECT64)
     k_plane_step<SLSIM_DIRECT64, false, HK><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+  else if (HK == 0)
+    k_plane_step64<<<nb, blk, 0, st>>>(jobs, items, nitems, err);
   else
     k_plane_step<SLSIM_FP64, false, HK><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
 }

@@ -9,11 +9,16 @@
 //   2. Leaves of 32 consecutive particles in key order, and a complete
 //      binary tree of bounding boxes over the leaves (heap order: node 1 is
 //      the root, the children of b are 2b and 2b+1, leaves at [L, 2L)).
-//   3. One wavefront per 64 consecutive particles.  Each lane keeps a max-heap
-//      of the K = Nsmooth+1 smallest squared distances in LDS.  An initial
-//      bound comes from the 2K particles around it in key order; the tree is
-//      then walked with one shared stack and per-lane masks, a box being taken
-//      by the lanes whose bound reaches it.
+//   3. One wavefront per query particle at a time (consecutive in key order
+//      along a wavefront's run).  A bound comes from the 2K particles around
+//      the query in key order (their K-th smallest squared distance); the
+//      lanes then test up to 64 boxes of the tree at once, pushing the
+//      children of the taken internal boxes and queueing the taken leaves,
+//      and scan the queued leaves two at a time (32 lanes a leaf), appending
+//      every squared distance within the bound to a buffer in LDS.  The K-th
+//      smallest of the buffer is found by bisection on the FP64 bit pattern
+//      (non-negative doubles order as their bits), counted with ballots.
+//      A full buffer is cut to its K smallest (the bound becomes the K-th).
 //
 // Squared distances use the reference's arithmetic: per dimension the
 // difference in the coordinate type T, squared in FP64, summed x, y, z
@@ -33,8 +38,12 @@ namespace {
 
 constexpr int kBlk = 256;
 constexpr int kLeaf = 32;           // particles per leaf
-constexpr int kKnnStack = 64;       // traversal stack entries per wavefront
-constexpr int kMaxK = 120;          // K = Nsmooth + 1 limit: the lane heaps stay under 64 KiB of LDS
+constexpr int kKnnWaves = 4;        // wavefronts per block of the query kernel
+constexpr int kKnnStack = 512;      // box stack entries per wavefront
+constexpr int kKnnLeafQ = 128;      // queued leaves per wavefront (<= 64 added a step, < 2 left)
+constexpr int kKnnBuf = 512;        // buffered squared distances per wavefront (8 slots of 64)
+constexpr int kKnnRun = 8;          // queries per wavefront
+constexpr int kMaxK = 120;          // K = Nsmooth + 1 limit (2K window: 4 slots of 64)
 
 inline unsigned nblk(size_t n, int b = kBlk) { return (unsigned)((n + b - 1) / b); }
 
@@ -155,54 +164,6 @@ __device__ inline double r2_ref(T px, T py, T pz, T qx, T qy, T qz) {
   return r2;
 }
 
-// Max-heap of this lane's K smallest values, interleaved over the wave's
-// lanes in LDS: element j of lane l at h[64 j + l].
-struct LaneHeap {
-  double* h;
-  int k, n;
-  __device__ inline double top() const { return h[0]; }
-  __device__ inline void push(double v) {   // n < k
-    int i = n++;
-    while (i > 0) {
-      const int p = (i - 1) >> 1;
-      const double pv = h[64 * p];
-      if (!(pv < v)) break;
-      h[64 * i] = pv;
-      i = p;
-    }
-    h[64 * i] = v;
-  }
-  __device__ inline void replace_top(double v) {   // n == k, v < top
-    int i = 0;
-    for (;;) {
-      const int c = 2 * i + 1;
-      if (c >= n) break;
-      double cv = h[64 * c];
-      int m = c;
-      if (c + 1 < n) {
-        const double c2 = h[64 * (c + 1)];
-        if (c2 > cv) {
-          cv = c2;
-          m = c + 1;
-        }
-      }
-      if (!(cv > v)) break;
-      h[64 * i] = cv;
-      i = m;
-    }
-    h[64 * i] = v;
-  }
-  // take v into the K smallest (bound: inclusive limit while filling)
-  __device__ inline void offer(double v, double bound) {
-    if (n < k) {
-      if (v <= bound) push(v);
-    } else if (v < top()) {
-      replace_top(v);
-    }
-  }
-  __device__ inline double limit(double bound) const { return n < k ? bound : top(); }
-};
-
 __device__ inline double box_d2(const Box& b, double qx, double qy, double qz) {
   const double q[3] = {qx, qy, qz};
   double d2 = 0;
@@ -213,70 +174,152 @@ __device__ inline double box_d2(const Box& b, double qx, double qy, double qz) {
   return d2;
 }
 
+__device__ inline uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ inline int lanes_below(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// The K-th smallest (1-based) of up to 64 S values held one per lane per
+// slot (bit patterns of non-negative doubles; absent entries ~0): the largest
+// pattern t with fewer than K values below it, built from the top bit down.
+template <int S>
+__device__ inline uint64_t kth_bits(const uint64_t (&v)[S], int slots, int K) {
+  uint64_t res = 0;
+  for (int b = 62; b >= 0; --b) {
+    const uint64_t t = res | (1ull << b);
+    int c = 0;
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+      if (s < slots) c += __builtin_popcountll(ballot64(v[s] < t));
+    if (c < K) res = t;
+  }
+  return res;
+}
+
+template <int S>
+__device__ inline void load_slots(const double* buf, int m, int lane, uint64_t (&v)[S]) {
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int j = 64 * s + lane;
+    v[s] = j < m ? (uint64_t)__double_as_longlong(buf[j]) : ~0ull;
+  }
+}
+
 template <typename T>
-__global__ __launch_bounds__(64) void k_knn(const T* __restrict__ xs, const T* __restrict__ ys,
-                                            const T* __restrict__ zs, const uint32_t* __restrict__ perm,
-                                            size_t n, int K, uint32_t L, const Box* __restrict__ box,
-                                            double slack, float* __restrict__ out,
-                                            uint32_t* __restrict__ err) {
-  extern __shared__ double s_heap[];               // 64 K doubles
-  __shared__ uint32_t s_node[kKnnStack];
-  __shared__ uint64_t s_mask[kKnnStack];
-  const int lane = (int)threadIdx.x;
-  const size_t i = blockIdx.x * (size_t)64 + lane;
-  const bool valid = i < n;
-  const size_t iq = valid ? i : n - 1;
-  const T qx = xs[iq], qy = ys[iq], qz = zs[iq];
-  LaneHeap hp{s_heap + lane, K, 0};
-  // initial bound: the K smallest over the 2K particles around it in key order
-  size_t w0 = iq > (size_t)K ? iq - K : 0;
-  size_t w1 = std::min(n, w0 + 2 * (size_t)K);
-  w0 = w1 > 2 * (size_t)K ? w1 - 2 * (size_t)K : 0;
-  for (size_t j = w0; j < w1; ++j) hp.offer(r2_ref(xs[j], ys[j], zs[j], qx, qy, qz), INFINITY);
-  const double bound = hp.top();
-  hp.n = 0;
-  // tree walk from the root
-  const double qdx = (double)qx, qdy = (double)qy, qdz = (double)qz;
-  const uint64_t vmask = __builtin_amdgcn_ballot_w64(valid);
-  int sp = 0;
-  uint32_t node = 1;
-  uint64_t live = vmask;
-  for (;;) {
-    const Box b = box[node];
-    const bool take = ((live >> lane) & 1ull) &&
-                      box_d2(b, qdx, qdy, qdz) <= hp.limit(bound) * slack;
-    const uint64_t tm = __builtin_amdgcn_ballot_w64(take);
-    if (tm) {
-      if (node >= L) {
-        const size_t s = (size_t)(node - L) * kLeaf, e = std::min(n, s + kLeaf);
-        for (size_t j = s; j < e; ++j) {
-          const T px = xs[j], py = ys[j], pz = zs[j];
-          if (take) hp.offer(r2_ref(px, py, pz, qx, qy, qz), bound);
-        }
-      } else {
-        if (sp + 2 > kKnnStack) {
-          if (lane == 0) atomicOr(err, 1u);
+__global__ __launch_bounds__(64 * kKnnWaves) void k_knn(
+    const T* __restrict__ xs, const T* __restrict__ ys, const T* __restrict__ zs,
+    const uint32_t* __restrict__ perm, size_t n, int K, uint32_t L, const Box* __restrict__ box,
+    double slack, float* __restrict__ out, uint32_t* __restrict__ err) {
+  __shared__ uint32_t s_stack_all[kKnnWaves][kKnnStack];
+  __shared__ uint32_t s_leaf_all[kKnnWaves][kKnnLeafQ];
+  __shared__ double s_buf_all[kKnnWaves][kKnnBuf];
+  const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  uint32_t* s_stack = s_stack_all[w];
+  uint32_t* s_leaf = s_leaf_all[w];
+  double* s_buf = s_buf_all[w];
+  const size_t q0 = ((size_t)blockIdx.x * kKnnWaves + w) * kKnnRun;
+  for (size_t iq = q0; iq < q0 + kKnnRun && iq < n; ++iq) {
+    const T qx = xs[iq], qy = ys[iq], qz = zs[iq];
+    const double qdx = (double)qx, qdy = (double)qy, qdz = (double)qz;
+    // bound: the K-th smallest over the 2K particles around the query in key order
+    size_t w0 = iq > (size_t)K ? iq - K : 0;
+    const size_t w1 = std::min(n, w0 + 2 * (size_t)K);
+    w0 = w1 > 2 * (size_t)K ? w1 - 2 * (size_t)K : 0;
+    double bound;
+    {
+      uint64_t v[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const size_t j = w0 + 64 * s + lane;
+        const size_t jj = j < w1 ? j : w0;
+        const double r2 = r2_ref(xs[jj], ys[jj], zs[jj], qx, qy, qz);
+        v[s] = j < w1 ? (uint64_t)__double_as_longlong(r2) : ~0ull;
+      }
+      bound = __longlong_as_double((long long)kth_bits(v, (int)((w1 - w0 + 63) / 64), K));
+    }
+    // the tree: boxes within the bound, leaves scanned two at a time
+    bool strict = false;       // after a cut the buffer holds K values <= bound
+    int sp = 1, nleaf = 0, m = 0;
+    if (lane == 0) s_stack[0] = 1;
+    bool overflow = false;
+    while (sp > 0 || nleaf > 0) {
+      if (sp > 0) {
+        const int nn = sp < 64 ? sp : 64;
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t node = lane < nn ? s_stack[sp - nn + lane] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        sp -= nn;
+        const bool take = lane < nn && box_d2(box[node], qdx, qdy, qdz) <= bound * slack;
+        const bool inner = take && node < L, leaf = take && node >= L;
+        const uint64_t bi = ballot64(inner), bl = ballot64(leaf);
+        const int ci = __builtin_popcountll(bi), cl = __builtin_popcountll(bl);
+        if (sp + 2 * ci > kKnnStack) {
+          overflow = true;
           break;
         }
-        if (lane == 0) {
-          s_node[sp] = 2 * node + 1;
-          s_mask[sp] = tm;
-          s_node[sp + 1] = 2 * node;
-          s_mask[sp + 1] = tm;
+        if (inner) {
+          const int o = sp + 2 * lanes_below(bi);
+          s_stack[o] = 2 * node + 1;
+          s_stack[o + 1] = 2 * node;
         }
-        sp += 2;
+        if (leaf) s_leaf[nleaf + lanes_below(bl)] = node - L;
+        sp += 2 * ci;
+        nleaf += cl;
+      }
+      while (nleaf >= 2 || (sp == 0 && nleaf > 0)) {
+        const int take2 = nleaf >= 2 ? 2 : 1;
+        __builtin_amdgcn_wave_barrier();
+        const int half = lane >> 5;
+        const bool has = half < take2;
+        const uint32_t lf = has ? s_leaf[nleaf - 1 - half] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        nleaf -= take2;
+        const size_t j = (size_t)lf * kLeaf + (lane & 31);
+        const bool valid = has && j < n;
+        const size_t jj = valid ? j : iq;
+        const double r2 = r2_ref(xs[jj], ys[jj], zs[jj], qx, qy, qz);
+        bool acc = valid && (strict ? r2 < bound : r2 <= bound);
+        uint64_t ba = ballot64(acc);
+        int ca = __builtin_popcountll(ba);
+        if (m + ca > kKnnBuf) {
+          // cut the buffer to its K smallest: those below the K-th, then
+          // copies of the K-th up to K values
+          uint64_t v[kKnnBuf / 64];
+          __builtin_amdgcn_wave_barrier();
+          load_slots(s_buf, m, lane, v);
+          __builtin_amdgcn_wave_barrier();
+          const uint64_t kb = kth_bits(v, (m + 63) / 64, K);
+          int base = 0;
+#pragma unroll
+          for (int s = 0; s < kKnnBuf / 64; ++s) {
+            const uint64_t b = ballot64(v[s] < kb);
+            if (v[s] < kb) s_buf[base + lanes_below(b)] = __longlong_as_double((long long)v[s]);
+            base += __builtin_popcountll(b);
+          }
+          bound = __longlong_as_double((long long)kb);
+          for (int j2 = base + lane; j2 < K; j2 += 64) s_buf[j2] = bound;
+          m = K;
+          strict = true;
+          acc = valid && r2 < bound;
+          ba = ballot64(acc);
+          ca = __builtin_popcountll(ba);
+        }
+        if (acc) s_buf[m + lanes_below(ba)] = r2;
+        m += ca;
       }
     }
-    if (sp == 0) break;
-    --sp;
+    if (overflow) {
+      if (lane == 0) atomicOr(err, 1u);
+      continue;
+    }
+    uint64_t v[kKnnBuf / 64];
     __builtin_amdgcn_wave_barrier();
-    node = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_node[sp]);
-    const uint64_t m = s_mask[sp];
-    live = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(m >> 32)) << 32) |
-           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)m);
+    load_slots(s_buf, m, lane, v);
     __builtin_amdgcn_wave_barrier();
+    const double kth = __longlong_as_double((long long)kth_bits(v, (m + 63) / 64, K));
+    if (lane == 0) out[perm[iq]] = (float)sqrt(kth);
   }
-  if (valid) out[perm[i]] = (float)sqrt(hp.top());
 }
 
 }  // namespace
@@ -356,9 +399,9 @@ int smoothing_sizes(slsim_ctx ctx, const T* x, const T* y, const T* z, size_t n,
     k_merge_boxes<<<nblk(first), kBlk, 0, st>>>(first, first, box);
   // queries
   SLSIM_HIP_CHECK(hipMemsetAsync(err, 0, 4, st));
-  const size_t lds = 64 * (size_t)K * sizeof(double);
   const double slack = sizeof(T) == 4 ? 1.0 + 1e-6 : 1.0 + 1e-12;
-  k_knn<T><<<nblk(n, 64), 64, lds, st>>>(sx, sy, sz, perm, n, K, L, box, slack, dout, err);
+  k_knn<T><<<nblk(n, kKnnWaves * kKnnRun), 64 * kKnnWaves, 0, st>>>(
+      sx, sy, sz, perm, n, K, L, box, slack, dout, err);
   SLSIM_HIP_CHECK(hipGetLastError());
   uint32_t herr = 0;
   SLSIM_HIP_CHECK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st));
