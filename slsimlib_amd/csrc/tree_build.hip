@@ -398,7 +398,7 @@ __global__ void k_flip_error(int64_t nnodes, const int32_t* __restrict__ st,
   const int64_t node = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   if (node >= nnodes) return;
   const int32_t s = st[node], c = cnt[node];
-  if (c > kBigCell) return;   // k_flip_error_big
+  if (c > kBigCell) return;   // k_flip_partial / k_flip_finish
   if (c == 0 || r2crit[node] >= 1e99) {
     if (lane == 0) err[node] = 0;
     return;
@@ -420,36 +420,89 @@ __global__ void k_flip_error(int64_t nnodes, const int32_t* __restrict__ st,
   if (lane == 0) err[node] = e;
 }
 
-// cells of more than kBigCell particles: one 1024-thread block per cell
-__global__ __launch_bounds__(1024) void k_flip_error_big(
-    const int32_t* __restrict__ list, const int32_t* __restrict__ st,
-    const int32_t* __restrict__ cnt, const double* __restrict__ p1,
-    const double* __restrict__ p2, const double* __restrict__ xs, const double* __restrict__ ys,
-    const float* __restrict__ ms, const float* __restrict__ hs, const double* __restrict__ com,
-    const double* __restrict__ mass, const double* __restrict__ quad,
-    const double* __restrict__ r2crit, double* __restrict__ err) {
+// Cells of more than kBigCell particles: the particles are cut into chunks of
+// kFlipChunk, one 1024-thread block per chunk writing its partial sums per
+// direction (pm, bs: 4 doubles) to part[chunk][direction]; k_flip_finish adds
+// a cell's chunks in order and takes the largest difference.  (A block per
+// cell left the root, with every particle of the tree, to one CU.)
+constexpr int32_t kFlipChunk = 16384;
+
+__device__ inline void flip_geometry(int64_t node, const double* __restrict__ p1,
+                                     const double* __restrict__ p2,
+                                     const double* __restrict__ com,
+                                     const double* __restrict__ r2crit, double* cx, double* cy,
+                                     double* R) {
+  *cx = com[2 * node];
+  *cy = com[2 * node + 1];
+  *R = flip_radius(r2crit[node], p2[2 * node] - p1[2 * node]);
+}
+
+__global__ __launch_bounds__(1024) void k_flip_partial(
+    const int32_t* __restrict__ chunk_cell, const int32_t* __restrict__ chunk_k0,
+    const int32_t* __restrict__ st, const int32_t* __restrict__ cnt,
+    const double* __restrict__ p1, const double* __restrict__ p2, const double* __restrict__ xs,
+    const double* __restrict__ ys, const float* __restrict__ ms, const float* __restrict__ hs,
+    const double* __restrict__ com, const double* __restrict__ r2crit,
+    double* __restrict__ part) {
   __shared__ double sh[16];
-  const int64_t node = list[blockIdx.x];
-  const int32_t s = st[node], c = cnt[node];
+  const int64_t node = chunk_cell[blockIdx.x];
+  const int32_t k0 = chunk_k0[blockIdx.x];
+  const int32_t len = min(kFlipChunk, cnt[node] - k0);
+  double* out = part + (size_t)blockIdx.x * kFlipDirs * 4;
   if (r2crit[node] >= 1e99) {
-    if (threadIdx.x == 0) err[node] = 0;
+    if (threadIdx.x < kFlipDirs * 4) out[threadIdx.x] = 0;
     return;
   }
-  const double cx = com[2 * node], cy = com[2 * node + 1];
-  const double R = flip_radius(r2crit[node], p2[2 * node] - p1[2 * node]);
-  double e = 0;
+  double cx, cy, R;
+  flip_geometry(node, p1, p2, com, r2crit, &cx, &cy, &R);
   for (int d = 0; d < kFlipDirs; ++d) {
     double qx, qy, pm[2], bs[2];
     flip_point(d, cx, cy, R, &qx, &qy);
-    flip_sums(s, c, threadIdx.x, blockDim.x, xs, ys, ms, hs, qx, qy, pm, bs);
+    flip_sums(st[node] + k0, len, threadIdx.x, blockDim.x, xs, ys, ms, hs, qx, qy, pm, bs);
     for (int k = 0; k < 2; ++k) {
       pm[k] = block_sum(pm[k], sh);
       bs[k] = block_sum(bs[k], sh);
     }
+    if (threadIdx.x == 0) {
+      out[4 * d] = pm[0];
+      out[4 * d + 1] = pm[1];
+      out[4 * d + 2] = bs[0];
+      out[4 * d + 3] = bs[1];
+    }
+  }
+}
+
+// one thread per big cell: its chunks' sums [first, first + n) in order
+__global__ void k_flip_finish(int nbig, const int32_t* __restrict__ list,
+                              const int32_t* __restrict__ first, const int32_t* __restrict__ nch,
+                              const double* __restrict__ p1, const double* __restrict__ p2,
+                              const double* __restrict__ com, const double* __restrict__ mass,
+                              const double* __restrict__ quad, const double* __restrict__ r2crit,
+                              const double* __restrict__ part, double* __restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbig) return;
+  const int64_t node = list[i];
+  if (r2crit[node] >= 1e99) {
+    err[node] = 0;
+    return;
+  }
+  double cx, cy, R;
+  flip_geometry(node, p1, p2, com, r2crit, &cx, &cy, &R);
+  double e = 0;
+  for (int d = 0; d < kFlipDirs; ++d) {
+    double qx, qy, pm[2] = {0, 0}, bs[2] = {0, 0};
+    flip_point(d, cx, cy, R, &qx, &qy);
+    for (int j = first[i]; j < first[i] + nch[i]; ++j) {
+      const double* q = part + ((size_t)j * kFlipDirs + d) * 4;
+      pm[0] += q[0];
+      pm[1] += q[1];
+      bs[0] += q[2];
+      bs[1] += q[3];
+    }
     e = fmax(e, flip_diff(cx, cy, qx, qy, mass[node], quad[3 * node], quad[3 * node + 1],
                           quad[3 * node + 2], pm, bs));
   }
-  if (threadIdx.x == 0) err[node] = e;
+  err[node] = e;
 }
 
 // Mass-sheet correction of a tree whose particle masses have both signs.
@@ -852,14 +905,17 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
   if ((rc = tmp.get(&com, 2 * nnodes)) || (rc = tmp.get(&mass, nnodes)) ||
       (rc = tmp.get(&quad, 3 * nnodes)) || (rc = tmp.get(&r2c, nnodes)))
     return rc;
-  std::vector<int32_t> big_cells;
+  std::vector<int32_t> big_cells, big_cnt;
   {
     std::vector<int32_t> hcnt(nnodes);
     std::vector<int32_t>& big = big_cells;
     SLSIM_HIP_CHECK(hipMemcpyAsync(hcnt.data(), gcnt, 4 * nnodes, hipMemcpyDeviceToHost, st));
     SLSIM_HIP_CHECK(hipStreamSynchronize(st));
     for (int64_t i = 0; i < nnodes; ++i)
-      if (hcnt[i] > kBigCell) big.push_back((int32_t)i);
+      if (hcnt[i] > kBigCell) {
+        big.push_back((int32_t)i);
+        big_cnt.push_back(hcnt[i]);
+      }
     if (!big.empty()) {
       int32_t* dbig;
       if ((rc = tmp.get(&dbig, big.size()))) return rc;
@@ -929,13 +985,34 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
   k_flip_error<<<nblocks(64 * (size_t)nnodes), kBlock, 0, st>>>(
       nnodes, gst, gcnt, gp1, gp2, xs, ys, ms, hs, com, mass, quad, r2c, flip);
   if (!big_cells.empty()) {
-    int32_t* dbig;
-    if ((rc = tmp.get(&dbig, big_cells.size()))) return rc;
-    SLSIM_HIP_CHECK(hipMemcpyAsync(dbig, big_cells.data(), 4 * big_cells.size(),
-                                   hipMemcpyHostToDevice, st));
-    k_flip_error_big<<<(unsigned)big_cells.size(), 1024, 0, st>>>(
-        dbig, gst, gcnt, gp1, gp2, xs, ys, ms, hs, com, mass, quad, r2c, flip);
-    SLSIM_HIP_CHECK(hipStreamSynchronize(st));   // `big_cells` is pageable host memory
+    // chunk table: [big cells | their first chunk | their chunk count | chunk cell | chunk k0]
+    const int nb = (int)big_cells.size();
+    std::vector<int32_t> tab(3 * (size_t)nb);
+    std::vector<int32_t> ccell, ck0;
+    for (int i = 0; i < nb; ++i) {
+      tab[i] = big_cells[i];
+      tab[nb + i] = (int32_t)ccell.size();
+      for (int32_t k0 = 0; k0 < big_cnt[i]; k0 += kFlipChunk) {
+        ccell.push_back(big_cells[i]);
+        ck0.push_back(k0);
+      }
+      tab[2 * nb + i] = (int32_t)ccell.size() - tab[nb + i];
+    }
+    const size_t nchunks = ccell.size();
+    tab.insert(tab.end(), ccell.begin(), ccell.end());
+    tab.insert(tab.end(), ck0.begin(), ck0.end());
+    int32_t* dtab;
+    double* part;
+    if ((rc = tmp.get(&dtab, tab.size())) || (rc = tmp.get(&part, nchunks * kFlipDirs * 4)))
+      return rc;
+    SLSIM_HIP_CHECK(hipMemcpyAsync(dtab, tab.data(), 4 * tab.size(), hipMemcpyHostToDevice, st));
+    k_flip_partial<<<(unsigned)nchunks, 1024, 0, st>>>(dtab + 3 * nb, dtab + 3 * nb + nchunks,
+                                                        gst, gcnt, gp1, gp2, xs, ys, ms, hs, com,
+                                                        r2c, part);
+    k_flip_finish<<<(unsigned)((nb + 63) / 64), 64, 0, st>>>(nb, dtab, dtab + nb, dtab + 2 * nb,
+                                                             gp1, gp2, com, mass, quad, r2c, part,
+                                                             flip);
+    SLSIM_HIP_CHECK(hipStreamSynchronize(st));   // `tab` is pageable host memory
   }
   k_pack_nodes<<<nblocks(nnodes), kBlock, 0, st>>>(nnodes, com, mass, quad, r2c, gp1, gp2, gchild,
                                                    gst, gcnt, bst, bcnt, flip, pl->nodesD,
