@@ -179,6 +179,35 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// A cell's moments and its critical radius (CalcMoments, include/quadTree.h:
+// 600-628): the largest corner distance from the centre of mass, over theta.
+__device__ inline void store_moments(int64_t node, double sx, double sy, double m, double q0,
+                                     double q1, double q2, const double* __restrict__ p1,
+                                     const double* __restrict__ p2, double theta,
+                                     double* __restrict__ com, double* __restrict__ mass,
+                                     double* __restrict__ quad, double* __restrict__ r2crit) {
+  com[2 * node] = sx;
+  com[2 * node + 1] = sy;
+  mass[node] = m;
+  quad[3 * node] = q0;
+  quad[3 * node + 1] = q1;
+  quad[3 * node + 2] = q2;
+  double rcom = 0;
+  const double c2[2] = {sx, sy};
+  for (int d = 0; d < 2; ++d) {
+    double a = (c2[d] - p1[2 * node + d]) * (c2[d] - p1[2 * node + d]);
+    double b = (c2[d] - p2[2 * node + d]) * (c2[d] - p2[2 * node + d]);
+    rcom += a > b ? a : b;
+  }
+  rcom = sqrt(rcom);
+  if (theta > 0.0) {
+    double r = 1.15470 * rcom / theta;
+    r2crit[node] = r * r;
+  } else {
+    r2crit[node] = 1.0e100;
+  }
+}
+
 // CalcMoments (include/quadTree.h:567-628), one wavefront per cell.
 __global__ void k_moments(int64_t nnodes, const int32_t* __restrict__ st,
                           const int32_t* __restrict__ cnt, const double* __restrict__ p1,
@@ -190,7 +219,7 @@ __global__ void k_moments(int64_t nnodes, const int32_t* __restrict__ st,
   const int64_t node = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   if (node >= nnodes) return;
   const int32_t s = st[node], c = cnt[node];
-  if (c > kBigCell) return;   // reduced by k_moments_big
+  if (c > kBigCell) return;   // k_moments_partial / k_moments_finish
   double m = 0, am = 0;
   for (int32_t k = lane; k < c; k += 64) {
     double w = ms[s + k];
@@ -220,28 +249,8 @@ __global__ void k_moments(int64_t nnodes, const int32_t* __restrict__ st,
   q0 = wave_sum(q0);
   q1 = wave_sum(q1);
   q2 = wave_sum(q2);
-  if (lane == 0) {
-    com[2 * node] = sx;
-    com[2 * node + 1] = sy;
-    mass[node] = m;
-    quad[3 * node] = q0;
-    quad[3 * node + 1] = q1;
-    quad[3 * node + 2] = q2;
-    double rcom = 0;
-    const double c2[2] = {sx, sy};
-    for (int d = 0; d < 2; ++d) {
-      double a = (c2[d] - p1[2 * node + d]) * (c2[d] - p1[2 * node + d]);
-      double b = (c2[d] - p2[2 * node + d]) * (c2[d] - p2[2 * node + d]);
-      rcom += a > b ? a : b;
-    }
-    rcom = sqrt(rcom);
-    if (theta > 0.0) {
-      double r = 1.15470 * rcom / theta;
-      r2crit[node] = r * r;
-    } else {
-      r2crit[node] = 1.0e100;
-    }
-  }
+  if (lane == 0)
+    store_moments(node, sx, sy, m, q0, q1, q2, p1, p2, theta, com, mass, quad, r2crit);
 }
 
 __device__ inline double block_sum(double v, double* sh) {
@@ -256,66 +265,90 @@ __device__ inline double block_sum(double v, double* sh) {
   return t;
 }
 
-// CalcMoments for cells of more than kBigCell particles: one 1024-thread
-// block per cell (same arithmetic as k_moments, wider reduction).
-__global__ __launch_bounds__(1024) void k_moments_big(
-    const int32_t* __restrict__ list, const int32_t* __restrict__ st,
-    const int32_t* __restrict__ cnt, const double* __restrict__ p1,
-    const double* __restrict__ p2, const double* __restrict__ xs, const double* __restrict__ ys,
-    const float* __restrict__ ms, double theta, double* __restrict__ com,
-    double* __restrict__ mass, double* __restrict__ quad, double* __restrict__ r2crit) {
+// Cells of more than kBigCell particles are cut into chunks of kChunk
+// particles, one 1024-thread block per chunk, each pass writing its partial
+// sums per chunk; a finish kernel adds a cell's chunks in order (one thread
+// per cell).  The passes follow k_moments: PASS 0 the mass and the absolute
+// mass, PASS 1 the |m|-weighted centre (needs the absolute mass), PASS 2 the
+// quadrupole (needs the centre).  (A block per cell left the root, with
+// every particle of the tree, to one CU.)
+constexpr int32_t kChunk = 16384;
+
+// The chunk table of the big cells: the cells, their first chunk, their chunk
+// count (nbig each), then per chunk its cell, its first particle (offset in
+// the cell) and its cell's index in the table (nchunks each).
+struct ChunkTable {
+  const int32_t *cell, *first, *nch;
+  const int32_t *ccell, *ck0, *cbig;
+  int nbig;
+  uint32_t nchunks;
+};
+
+__device__ inline void block_sums(double* v, int k, double* sh) {
+  for (int j = 0; j < k; ++j) v[j] = block_sum(v[j], sh);
+}
+
+template <int PASS>
+__global__ __launch_bounds__(1024) void k_moments_partial(
+    ChunkTable ct, const int32_t* __restrict__ st, const int32_t* __restrict__ cnt,
+    const double* __restrict__ xs, const double* __restrict__ ys, const float* __restrict__ ms,
+    const double* __restrict__ amb, const double* __restrict__ com, double* __restrict__ part) {
   __shared__ double sh[16];
-  const int64_t node = list[blockIdx.x];
-  const int32_t s = st[node], c = cnt[node];
-  double m = 0, am = 0;
-  for (int32_t k = threadIdx.x; k < c; k += blockDim.x) {
-    double w = ms[s + k];
-    m += w;
-    am += fabs(w);
-  }
-  m = block_sum(m, sh);
-  am = block_sum(am, sh);
-  double sx = 0, sy = 0;
-  for (int32_t k = threadIdx.x; k < c; k += blockDim.x) {
-    double w = fabs((double)ms[s + k]);
-    sx += w * xs[s + k] / am;
-    sy += w * ys[s + k] / am;
-  }
-  sx = block_sum(sx, sh);
-  sy = block_sum(sy, sh);
-  double q0 = 0, q1 = 0, q2 = 0;
-  for (int32_t k = threadIdx.x; k < c; k += blockDim.x) {
-    double dx = xs[s + k] - sx, dy = ys[s + k] - sy;
-    double r2 = dx * dx + dy * dy;
-    double w = ms[s + k];
-    q0 += (r2 - 2 * dx * dx) * w;
-    q1 += (r2 - 2 * dy * dy) * w;
-    q2 += -2 * dx * dy * w;
-  }
-  q0 = block_sum(q0, sh);
-  q1 = block_sum(q1, sh);
-  q2 = block_sum(q2, sh);
-  if (threadIdx.x == 0) {
-    com[2 * node] = sx;
-    com[2 * node + 1] = sy;
-    mass[node] = m;
-    quad[3 * node] = q0;
-    quad[3 * node + 1] = q1;
-    quad[3 * node + 2] = q2;
-    double rcom = 0;
-    const double c2[2] = {sx, sy};
-    for (int d = 0; d < 2; ++d) {
-      double a = (c2[d] - p1[2 * node + d]) * (c2[d] - p1[2 * node + d]);
-      double b = (c2[d] - p2[2 * node + d]) * (c2[d] - p2[2 * node + d]);
-      rcom += a > b ? a : b;
+  const int64_t node = ct.ccell[blockIdx.x];
+  const int32_t s = st[node] + ct.ck0[blockIdx.x];
+  const int32_t c = min(kChunk, cnt[node] - ct.ck0[blockIdx.x]);
+  double v[3] = {0, 0, 0};
+  if (PASS == 0) {
+    for (int32_t k = threadIdx.x; k < c; k += blockDim.x) {
+      double w = ms[s + k];
+      v[0] += w;
+      v[1] += fabs(w);
     }
-    rcom = sqrt(rcom);
-    if (theta > 0.0) {
-      double r = 1.15470 * rcom / theta;
-      r2crit[node] = r * r;
-    } else {
-      r2crit[node] = 1.0e100;
+    block_sums(v, 2, sh);
+  } else if (PASS == 1) {
+    const double am = amb[ct.cbig[blockIdx.x]];
+    for (int32_t k = threadIdx.x; k < c; k += blockDim.x) {
+      double w = fabs((double)ms[s + k]);
+      v[0] += w * xs[s + k] / am;
+      v[1] += w * ys[s + k] / am;
     }
+    block_sums(v, 2, sh);
+  } else {
+    const double sx = com[2 * node], sy = com[2 * node + 1];
+    for (int32_t k = threadIdx.x; k < c; k += blockDim.x) {
+      double dx = xs[s + k] - sx, dy = ys[s + k] - sy;
+      double r2 = dx * dx + dy * dy;
+      double w = ms[s + k];
+      v[0] += (r2 - 2 * dx * dx) * w;
+      v[1] += (r2 - 2 * dy * dy) * w;
+      v[2] += -2 * dx * dy * w;
+    }
+    block_sums(v, 3, sh);
+  }
+  if (threadIdx.x < 3) part[3 * (size_t)blockIdx.x + threadIdx.x] = v[threadIdx.x];
+}
+
+template <int PASS>
+__global__ void k_moments_finish(ChunkTable ct, const double* __restrict__ part,
+                                 const double* __restrict__ p1, const double* __restrict__ p2,
+                                 double theta, double* __restrict__ amb, double* __restrict__ com,
+                                 double* __restrict__ mass, double* __restrict__ quad,
+                                 double* __restrict__ r2crit) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ct.nbig) return;
+  const int64_t node = ct.cell[i];
+  double v[3] = {0, 0, 0};
+  for (int j = ct.first[i]; j < ct.first[i] + ct.nch[i]; ++j)
+    for (int k = 0; k < 3; ++k) v[k] += part[3 * (size_t)j + k];
+  if (PASS == 0) {
+    mass[node] = v[0];
+    amb[i] = v[1];
+  } else if (PASS == 1) {
+    com[2 * node] = v[0];
+    com[2 * node + 1] = v[1];
+  } else {
+    store_moments(node, com[2 * node], com[2 * node + 1], mass[node], v[0], v[1], v[2], p1, p2,
+                  theta, com, mass, quad, r2crit);
   }
 }
 
@@ -420,12 +453,10 @@ __global__ void k_flip_error(int64_t nnodes, const int32_t* __restrict__ st,
   if (lane == 0) err[node] = e;
 }
 
-// Cells of more than kBigCell particles: the particles are cut into chunks of
-// kFlipChunk, one 1024-thread block per chunk writing its partial sums per
-// direction (pm, bs: 4 doubles) to part[chunk][direction]; k_flip_finish adds
-// a cell's chunks in order and takes the largest difference.  (A block per
-// cell left the root, with every particle of the tree, to one CU.)
-constexpr int32_t kFlipChunk = 16384;
+// Cells of more than kBigCell particles, in the chunks of the moments: one
+// 1024-thread block per chunk writing its partial sums per direction (pm, bs:
+// 4 doubles) to part[chunk][direction]; k_flip_finish adds a cell's chunks in
+// order and takes the largest difference.
 
 __device__ inline void flip_geometry(int64_t node, const double* __restrict__ p1,
                                      const double* __restrict__ p2,
@@ -438,16 +469,15 @@ __device__ inline void flip_geometry(int64_t node, const double* __restrict__ p1
 }
 
 __global__ __launch_bounds__(1024) void k_flip_partial(
-    const int32_t* __restrict__ chunk_cell, const int32_t* __restrict__ chunk_k0,
-    const int32_t* __restrict__ st, const int32_t* __restrict__ cnt,
+    ChunkTable ct, const int32_t* __restrict__ st, const int32_t* __restrict__ cnt,
     const double* __restrict__ p1, const double* __restrict__ p2, const double* __restrict__ xs,
     const double* __restrict__ ys, const float* __restrict__ ms, const float* __restrict__ hs,
     const double* __restrict__ com, const double* __restrict__ r2crit,
     double* __restrict__ part) {
   __shared__ double sh[16];
-  const int64_t node = chunk_cell[blockIdx.x];
-  const int32_t k0 = chunk_k0[blockIdx.x];
-  const int32_t len = min(kFlipChunk, cnt[node] - k0);
+  const int64_t node = ct.ccell[blockIdx.x];
+  const int32_t k0 = ct.ck0[blockIdx.x];
+  const int32_t len = min(kChunk, cnt[node] - k0);
   double* out = part + (size_t)blockIdx.x * kFlipDirs * 4;
   if (r2crit[node] >= 1e99) {
     if (threadIdx.x < kFlipDirs * 4) out[threadIdx.x] = 0;
@@ -473,15 +503,13 @@ __global__ __launch_bounds__(1024) void k_flip_partial(
 }
 
 // one thread per big cell: its chunks' sums [first, first + n) in order
-__global__ void k_flip_finish(int nbig, const int32_t* __restrict__ list,
-                              const int32_t* __restrict__ first, const int32_t* __restrict__ nch,
-                              const double* __restrict__ p1, const double* __restrict__ p2,
+__global__ void k_flip_finish(ChunkTable ct, const double* __restrict__ p1, const double* __restrict__ p2,
                               const double* __restrict__ com, const double* __restrict__ mass,
                               const double* __restrict__ quad, const double* __restrict__ r2crit,
                               const double* __restrict__ part, double* __restrict__ err) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nbig) return;
-  const int64_t node = list[i];
+  if (i >= ct.nbig) return;
+  const int64_t node = ct.cell[i];
   if (r2crit[node] >= 1e99) {
     err[node] = 0;
     return;
@@ -492,7 +520,7 @@ __global__ void k_flip_finish(int nbig, const int32_t* __restrict__ list,
   for (int d = 0; d < kFlipDirs; ++d) {
     double qx, qy, pm[2] = {0, 0}, bs[2] = {0, 0};
     flip_point(d, cx, cy, R, &qx, &qy);
-    for (int j = first[i]; j < first[i] + nch[i]; ++j) {
+    for (int j = ct.first[i]; j < ct.first[i] + ct.nch[i]; ++j) {
       const double* q = part + ((size_t)j * kFlipDirs + d) * 4;
       pm[0] += q[0];
       pm[1] += q[1];
@@ -905,24 +933,54 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
   if ((rc = tmp.get(&com, 2 * nnodes)) || (rc = tmp.get(&mass, nnodes)) ||
       (rc = tmp.get(&quad, 3 * nnodes)) || (rc = tmp.get(&r2c, nnodes)))
     return rc;
-  std::vector<int32_t> big_cells, big_cnt;
+  // the chunk table of the cells of more than kBigCell particles
+  ChunkTable ct{};
   {
     std::vector<int32_t> hcnt(nnodes);
-    std::vector<int32_t>& big = big_cells;
     SLSIM_HIP_CHECK(hipMemcpyAsync(hcnt.data(), gcnt, 4 * nnodes, hipMemcpyDeviceToHost, st));
     SLSIM_HIP_CHECK(hipStreamSynchronize(st));
-    for (int64_t i = 0; i < nnodes; ++i)
-      if (hcnt[i] > kBigCell) {
-        big.push_back((int32_t)i);
-        big_cnt.push_back(hcnt[i]);
+    std::vector<int32_t> cell, first, nch, ccell, ck0, cbig;
+    for (int64_t i = 0; i < nnodes; ++i) {
+      if (hcnt[i] <= kBigCell) continue;
+      first.push_back((int32_t)ccell.size());
+      for (int32_t k0 = 0; k0 < hcnt[i]; k0 += kChunk) {
+        ccell.push_back((int32_t)i);
+        ck0.push_back(k0);
+        cbig.push_back((int32_t)cell.size());
       }
-    if (!big.empty()) {
-      int32_t* dbig;
-      if ((rc = tmp.get(&dbig, big.size()))) return rc;
-      SLSIM_HIP_CHECK(hipMemcpyAsync(dbig, big.data(), 4 * big.size(), hipMemcpyHostToDevice, st));
-      k_moments_big<<<(unsigned)big.size(), 1024, 0, st>>>(dbig, gst, gcnt, gp1, gp2, xs, ys, ms,
-                                                          pl->theta, com, mass, quad, r2c);
-      SLSIM_HIP_CHECK(hipStreamSynchronize(st));   // `big` is pageable host memory
+      nch.push_back((int32_t)ccell.size() - first.back());
+      cell.push_back((int32_t)i);
+    }
+    ct.nbig = (int)cell.size();
+    ct.nchunks = (uint32_t)ccell.size();
+    if (ct.nbig > 0) {
+      std::vector<int32_t> tab;
+      for (const auto* v : {&cell, &first, &nch, &ccell, &ck0, &cbig})
+        tab.insert(tab.end(), v->begin(), v->end());
+      int32_t* dtab;
+      double *part, *amb;
+      if ((rc = tmp.get(&dtab, tab.size())) || (rc = tmp.get(&part, 3 * (size_t)ct.nchunks)) ||
+          (rc = tmp.get(&amb, ct.nbig)))
+        return rc;
+      SLSIM_HIP_CHECK(hipMemcpyAsync(dtab, tab.data(), 4 * tab.size(), hipMemcpyHostToDevice, st));
+      const size_t nb = ct.nbig, nc = ct.nchunks;
+      ct.cell = dtab;
+      ct.first = dtab + nb;
+      ct.nch = dtab + 2 * nb;
+      ct.ccell = dtab + 3 * nb;
+      ct.ck0 = dtab + 3 * nb + nc;
+      ct.cbig = dtab + 3 * nb + 2 * nc;
+      const unsigned fb = (unsigned)((nb + 63) / 64);
+      k_moments_partial<0><<<ct.nchunks, 1024, 0, st>>>(ct, gst, gcnt, xs, ys, ms, amb, com, part);
+      k_moments_finish<0><<<fb, 64, 0, st>>>(ct, part, gp1, gp2, pl->theta, amb, com, mass, quad,
+                                             r2c);
+      k_moments_partial<1><<<ct.nchunks, 1024, 0, st>>>(ct, gst, gcnt, xs, ys, ms, amb, com, part);
+      k_moments_finish<1><<<fb, 64, 0, st>>>(ct, part, gp1, gp2, pl->theta, amb, com, mass, quad,
+                                             r2c);
+      k_moments_partial<2><<<ct.nchunks, 1024, 0, st>>>(ct, gst, gcnt, xs, ys, ms, amb, com, part);
+      k_moments_finish<2><<<fb, 64, 0, st>>>(ct, part, gp1, gp2, pl->theta, amb, com, mass, quad,
+                                             r2c);
+      SLSIM_HIP_CHECK(hipStreamSynchronize(st));   // `tab` is pageable host memory
     }
   }
   k_moments<<<nblocks(64 * (size_t)nnodes), kBlock, 0, st>>>(nnodes, gst, gcnt, gp1, gp2, xs, ys,
@@ -984,35 +1042,13 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
   if ((rc = tmp.get(&flip, nnodes))) return rc;
   k_flip_error<<<nblocks(64 * (size_t)nnodes), kBlock, 0, st>>>(
       nnodes, gst, gcnt, gp1, gp2, xs, ys, ms, hs, com, mass, quad, r2c, flip);
-  if (!big_cells.empty()) {
-    // chunk table: [big cells | their first chunk | their chunk count | chunk cell | chunk k0]
-    const int nb = (int)big_cells.size();
-    std::vector<int32_t> tab(3 * (size_t)nb);
-    std::vector<int32_t> ccell, ck0;
-    for (int i = 0; i < nb; ++i) {
-      tab[i] = big_cells[i];
-      tab[nb + i] = (int32_t)ccell.size();
-      for (int32_t k0 = 0; k0 < big_cnt[i]; k0 += kFlipChunk) {
-        ccell.push_back(big_cells[i]);
-        ck0.push_back(k0);
-      }
-      tab[2 * nb + i] = (int32_t)ccell.size() - tab[nb + i];
-    }
-    const size_t nchunks = ccell.size();
-    tab.insert(tab.end(), ccell.begin(), ccell.end());
-    tab.insert(tab.end(), ck0.begin(), ck0.end());
-    int32_t* dtab;
+  if (ct.nbig > 0) {
     double* part;
-    if ((rc = tmp.get(&dtab, tab.size())) || (rc = tmp.get(&part, nchunks * kFlipDirs * 4)))
-      return rc;
-    SLSIM_HIP_CHECK(hipMemcpyAsync(dtab, tab.data(), 4 * tab.size(), hipMemcpyHostToDevice, st));
-    k_flip_partial<<<(unsigned)nchunks, 1024, 0, st>>>(dtab + 3 * nb, dtab + 3 * nb + nchunks,
-                                                        gst, gcnt, gp1, gp2, xs, ys, ms, hs, com,
-                                                        r2c, part);
-    k_flip_finish<<<(unsigned)((nb + 63) / 64), 64, 0, st>>>(nb, dtab, dtab + nb, dtab + 2 * nb,
-                                                             gp1, gp2, com, mass, quad, r2c, part,
-                                                             flip);
-    SLSIM_HIP_CHECK(hipStreamSynchronize(st));   // `tab` is pageable host memory
+    if ((rc = tmp.get(&part, (size_t)ct.nchunks * kFlipDirs * 4))) return rc;
+    k_flip_partial<<<ct.nchunks, 1024, 0, st>>>(ct, gst, gcnt, gp1, gp2, xs, ys, ms, hs, com, r2c,
+                                                part);
+    k_flip_finish<<<(unsigned)((ct.nbig + 63) / 64), 64, 0, st>>>(ct, gp1, gp2, com, mass, quad,
+                                                                  r2c, part, flip);
   }
   k_pack_nodes<<<nblocks(nnodes), kBlock, 0, st>>>(nnodes, com, mass, quad, r2c, gp1, gp2, gchild,
                                                    gst, gcnt, bst, bcnt, flip, pl->nodesD,
