@@ -32,6 +32,9 @@ __device__ inline uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p
 __device__ inline uint64_t lanes_lt(double a, double b) {
   return __builtin_amdgcn_fcmp(a, b, 4 /* ordered less-than */);
 }
+__device__ inline uint64_t lanes_ltf(float a, float b) {
+  return __builtin_amdgcn_fcmpf(a, b, 4 /* ordered less-than */);
+}
 __device__ inline bool lane_in(uint64_t mask) { return __builtin_amdgcn_inverse_ballot_w64(mask); }
 
 __device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -743,52 +746,112 @@ __device__ __forceinline__ void far32(const NodeF& nd, bool far, float ex, float
 // within 2h (the near field, whose B-spline terms are sensitive to the
 // offset) takes its offset again from the FP64 positions, as the FP64 walk
 // forms it, before the near-field rules apply.
+
+// One leaf particle's offset and point-mass coefficients for a lane.
+struct LeafTerm {
+  float d0, d1, d1s, s, w;
+};
+
+// The near-field rules for a lane whose particle may lie within 2h: the
+// offset again from the FP64 positions, then the B-spline correction, the
+// FP64 queue or the bare point mass.
 template <bool GUARD>
-__device__ __forceinline__ void leaf32(int start, int count, bool open, bool nearc, float ex,
+__device__ __forceinline__ void leaf_near32(int idx, float m, bool nearc, double rx, double ry,
+                                            cptr<PartD> partsd, Acc32& a, float r2f, float nef,
+                                            float sD, float& risk, LeafTerm& t) {
+  const PartD q = partsd[idx];
+  t.d0 = (float)(q.x - rx);
+  t.d1 = (float)(q.y - ry);
+  t.d1s = t.d1 * t.d1;
+  const float rr = fmaxf(fmaf(t.d0, t.d0, t.d1s), 1e-20f);
+  const float inv = rcp32(rr);
+  t.s = m * inv;
+  t.w = t.s * inv;
+  const float hs = fabsf(q.h);
+  if (rr < 4.f * hs * hs) {
+    // a big particle within 2h: its correction hangs on the near test
+    const bool listed = !signbit(q.h);
+    if (GUARD && listed && near_marginal(r2f, nef, sD)) risk = __builtin_inff();
+    if (nearc && listed) {
+      // smoothed: the B-spline correction folded into the point mass
+      float Mr2, sigpi;
+      bspline_fused32(rr, inv, hs, Mr2, sigpi);
+      t.s = m * Mr2;
+      const float sm = m * sigpi;
+      t.w = (t.s - sm) * inv;
+      a.kappa(sm);
+    } else if (a.push((uint32_t)idx)) {
+      t.s = 0.f;    // deferred to FP64
+      t.w = 0.f;
+    }
+  }
+}
+
+// The point-mass term of a leaf particle; rr: the squared offset.
+__device__ __forceinline__ LeafTerm leaf_point32(const PartF& p, bool open, float ex, float ey,
+                                                 float exl, float eyl, float& rr) {
+  LeafTerm t;
+  t.d0 = (p.lx + ex) + exl;
+  t.d1 = (p.ly + ey) + eyl;
+  t.d1s = t.d1 * t.d1;
+  // (no clamp: |hh4| >= FLT_MIN takes r^2 = 0 to the near-field branch)
+  rr = fmaf(t.d0, t.d0, t.d1s);
+  const float inv = open ? rcp32(rr) : 0.f;       // 0: no term for this lane
+  t.s = p.m * inv;
+  t.w = t.s * inv;
+  return t;
+}
+
+__device__ __forceinline__ void leaf_add32(const LeafTerm& t, Acc32& a) {
+  a.pax = fmaf(-t.s, t.d0, a.pax);
+  a.pay = fmaf(-t.s, t.d1, a.pay);
+  a.pg0 = fmaf(t.w, fmaf(t.d0, t.d0, -t.d1s), a.pg0);
+  a.pg1 = fmaf(t.w * t.d0, t.d1, a.pg1);
+}
+
+// Particles of an opened leaf (om: the lanes that opened it), kLeafStep at a
+// time (one scalar load of their records): the lanes whose particle may lie
+// within 2h (4h^2 carries a margin) form masks in scalar registers, and only
+// a non-empty mask branches (uniformly, rarely) to the near-field rules, so
+// the common path carries no exec-mask changes and a quarter of the loop's
+// scalar work (10-plane 2^20-particle field: 22.5 -> 22.3 ms per plane).
+constexpr int kLeafStep = 4;
+template <bool GUARD>
+__device__ __forceinline__ void leaf32(int start, int count, uint64_t om, bool nearc, float ex,
                                        float ey, float exl, float eyl, double rx, double ry,
                                        cptr<PartF> parts, cptr<PartD> partsd, Acc32& a,
                                        float r2f, float nef, float sD, float& risk) {
-#pragma unroll 2
-  for (int k = 0; k < count; ++k) {
-    const PartF p = parts[start + k];
-    float d0 = (p.lx + ex) + exl, d1 = (p.ly + ey) + eyl;
-    float d1s = d1 * d1;
-    // (no clamp: |hh4| >= FLT_MIN takes r^2 = 0 to the near-field branch)
-    float rr = fmaf(d0, d0, d1s);
-    float inv = open ? rcp32(rr) : 0.f;             // 0: no term for this lane
-    float s = p.m * inv, w = s * inv;              // point mass
-    if (open && rr < fabsf(p.hh4)) {
-      const PartD q = partsd[start + k];
-      d0 = (float)(q.x - rx);
-      d1 = (float)(q.y - ry);
-      d1s = d1 * d1;
-      rr = fmaxf(fmaf(d0, d0, d1s), 1e-20f);
-      inv = rcp32(rr);
-      s = p.m * inv;
-      w = s * inv;
-      const float hs = fabsf(q.h);
-      if (rr < 4.f * hs * hs) {
-        // a big particle within 2h: its correction hangs on the near test
-        const bool listed = !signbit(q.h);
-        if (GUARD && listed && near_marginal(r2f, nef, sD)) risk = __builtin_inff();
-        if (nearc && listed) {
-          // smoothed: the B-spline correction folded into the point mass
-          float Mr2, sigpi;
-          bspline_fused32(rr, inv, hs, Mr2, sigpi);
-          s = p.m * Mr2;
-          const float sm = p.m * sigpi;
-          w = (s - sm) * inv;
-          a.kappa(sm);
-        } else if (a.push((uint32_t)(start + k))) {
-          s = 0.f;    // deferred to FP64
-          w = 0.f;
-        }
-      }
+  const bool open = lane_in(om);
+  int k = 0;
+  for (; k + kLeafStep <= count; k += kLeafStep) {
+    LeafTerm t[kLeafStep];
+    uint64_t n[kLeafStep], any = 0;
+#pragma unroll
+    for (int j = 0; j < kLeafStep; ++j) {
+      const PartF p = parts[start + k + j];
+      float rr;
+      t[j] = leaf_point32(p, open, ex, ey, exl, eyl, rr);
+      n[j] = lanes_ltf(rr, fabsf(p.hh4)) & om;
+      any |= n[j];
     }
-    a.pax = fmaf(-s, d0, a.pax);
-    a.pay = fmaf(-s, d1, a.pay);
-    a.pg0 = fmaf(w, fmaf(d0, d0, -d1s), a.pg0);
-    a.pg1 = fmaf(w * d0, d1, a.pg1);
+    if (any) {
+#pragma unroll
+      for (int j = 0; j < kLeafStep; ++j)
+        if (lane_in(n[j]))
+          leaf_near32<GUARD>(start + k + j, parts[start + k + j].m, nearc, rx, ry, partsd, a, r2f,
+                             nef, sD, risk, t[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kLeafStep; ++j) leaf_add32(t[j], a);
+  }
+  for (; k < count; ++k) {
+    const PartF p = parts[start + k];
+    float rr;
+    LeafTerm t = leaf_point32(p, open, ex, ey, exl, eyl, rr);
+    const uint64_t n = lanes_ltf(rr, fabsf(p.hh4)) & om;
+    if (n && lane_in(n))
+      leaf_near32<GUARD>(start + k, p.m, nearc, rx, ry, partsd, a, r2f, nef, sD, risk, t);
+    leaf_add32(t, a);
   }
 }
 
@@ -875,7 +938,7 @@ __device__ void walk32(const TreeDesc* __restrict__ trees, int ntrees, double rx
         if (child < 0) {
           const uint64_t nm = lanes_lt(r2, nd->near2) & om;
           const float exl = (float)(dx - (double)ex), eyl = (float)(dy - (double)ey);
-          leaf32<GUARD>(nd->start, cnt, lane_in(om), lane_in(nm), ex, ey, exl, eyl, rx, ry,
+          leaf32<GUARD>(nd->start, cnt, om, lane_in(nm), ex, ey, exl, eyl, rx, ry,
                         parts, partsd, a, (float)r2, (float)nd->near2, sD, risk);
         } else {
           if (cnt > 0) {
