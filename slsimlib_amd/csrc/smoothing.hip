@@ -10,11 +10,14 @@
 //      binary tree of bounding boxes over the leaves (heap order: node 1 is
 //      the root, the children of b are 2b and 2b+1, leaves at [L, 2L)).
 //   3. One wavefront per query particle at a time (consecutive in key order
-//      along a wavefront's run).  A bound comes from the 2K particles around
-//      the query in key order (their K-th smallest squared distance); the
-//      lanes then test up to 64 boxes of the tree at once, pushing the
-//      children of the taken internal boxes and queueing the taken leaves,
-//      and scan the queued leaves two at a time (32 lanes a leaf), appending
+//      along a wavefront's run).  A bound on the K-th squared distance comes
+//      from the previous query of the run (its K nearest lie within
+//      sqrt(K-th) + the step, taken when the step is at most half that), else
+//      from the 2K particles around the query in key order (their K-th
+//      smallest).  The lanes then test up to 64 boxes of the tree at once,
+//      pushing the children of the taken internal boxes and queueing the
+//      taken leaves, and scan the queued leaves two at a time (32 lanes a
+//      leaf), appending
 //      every squared distance within the bound to a buffer in LDS.  The K-th
 //      smallest of the buffer is found by bisection on the FP64 bit pattern
 //      (non-negative doubles order as their bits), counted with ballots.
@@ -219,15 +222,23 @@ __global__ __launch_bounds__(64 * kKnnWaves) void k_knn(
   uint32_t* s_leaf = s_leaf_all[w];
   double* s_buf = s_buf_all[w];
   const size_t q0 = ((size_t)blockIdx.x * kKnnWaves + w) * kKnnRun;
+  double pk = -1, px = 0, py = 0, pz = 0;   // the previous query's K-th and position
   for (size_t iq = q0; iq < q0 + kKnnRun && iq < n; ++iq) {
     const T qx = xs[iq], qy = ys[iq], qz = zs[iq];
     const double qdx = (double)qx, qdy = (double)qy, qdz = (double)qz;
-    // bound: the K-th smallest over the 2K particles around the query in key order
-    size_t w0 = iq > (size_t)K ? iq - K : 0;
-    const size_t w1 = std::min(n, w0 + 2 * (size_t)K);
-    w0 = w1 > 2 * (size_t)K ? w1 - 2 * (size_t)K : 0;
     double bound;
-    {
+    const double ddx = qdx - px, ddy = qdy - py, ddz = qdz - pz;
+    const double dd = ddx * ddx + ddy * ddy + ddz * ddz;
+    if (pk >= 0 && 4 * dd <= pk) {
+      // the previous query's K nearest lie within sqrt(pk) + |q - p| of this
+      // one (taken when the step is at most half that radius)
+      const double r = sqrt(pk) + sqrt(dd);
+      bound = r * r * slack * slack;
+    } else {
+      // the K-th smallest over the 2K particles around the query in key order
+      size_t w0 = iq > (size_t)K ? iq - K : 0;
+      const size_t w1 = std::min(n, w0 + 2 * (size_t)K);
+      w0 = w1 > 2 * (size_t)K ? w1 - 2 * (size_t)K : 0;
       uint64_t v[4];
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
@@ -319,6 +330,10 @@ __global__ __launch_bounds__(64 * kKnnWaves) void k_knn(
     __builtin_amdgcn_wave_barrier();
     const double kth = __longlong_as_double((long long)kth_bits(v, (m + 63) / 64, K));
     if (lane == 0) out[perm[iq]] = (float)sqrt(kth);
+    pk = kth;
+    px = qdx;
+    py = qdy;
+    pz = qdz;
   }
 }
 
