@@ -1029,8 +1029,9 @@ int build_plane(slsim_ctx ctx, const T* x_host, const T* y_host, const float* m_
 
   // ---- pack walk records
   if ((rc = ctx->alloc(reinterpret_cast<void**>(&pl->nodesD), sizeof(NodeD) * nnodes))) return rc;
-  // three zero cells after the last: the FP32 walk loads the root's
-  // siblings' positions when it starts a tree
+  // three zero cells after the last: a group of four child slots read from
+  // any cell stays inside the array (the FP32 walk starts a tree as a virtual
+  // entry whose first slot is the root)
   if ((rc = ctx->alloc(reinterpret_cast<void**>(&pl->nodesF), sizeof(NodeF) * (nnodes + 3))))
     return rc;
   SLSIM_HIP_CHECK(hipMemsetAsync(pl->nodesF + nnodes, 0, 3 * sizeof(NodeF), st));
