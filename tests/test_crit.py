@@ -364,3 +364,37 @@ def test_product_windings_equal_oracle():
         poly = rng.normal(size=(n, 2))
         for q in rng.normal(size=(40, 2)):
             assert S.windings(q, poly) == O.windings(q, poly)
+
+
+def test_image_finding_find_crit_point_mass():
+    """ImageFinding::find_crit over a GridMap of a point mass: the tangential
+    curve is the Einstein ring (area pi thetaE^2 to a pixel times its
+    perimeter, centre on the lens), its caustic the point y = 0 (every caustic
+    point within the curve's offset from the ring: |y| ~ 2 |r - thetaE|)."""
+    GRAV = 4.7788e-20
+    zl, zs, M = 0.5, 2.0, 1e12
+    L = O.Lens(zs)
+    L.moveinMainHalo(O.Tree(np.array([0.0]), np.array([0.0]), np.array([M], np.float32),
+                            np.array([1e-12], np.float32)), zl)
+    zl_f = float(np.float32(zl))
+    Dl, Ds = O.coorDist(0, zl_f), O.coorDist(0, zs)
+    te = np.sqrt(4 * GRAV * float(np.float32(M)) * (Ds - Dl) * (1 + zl_f) / (Dl * Ds))
+    N, rg, c = 101, 3 * te, (0.01 * te, -0.013 * te)
+    pos = O.gridmap_positions(N, c, rg)
+    y, A = L.shoot(pos)
+    out = O.image_finding_find_crit(L, N, pos, y, A, rg)
+    res = rg / (N - 1)
+    tang = [o for o in out if o["type"] == O.CRIT_TANGENTIAL]
+    assert len(tang) == 1
+    t = tang[0]
+    r = _radius(t["critcurve"])
+    assert np.all(np.abs(r - te) <= res)
+    assert abs(t["critical_area"] - np.pi * te * te) <= 2 * np.pi * te * res
+    assert np.hypot(*t["critical_center"]) <= res
+    yc = np.hypot(t["caustic_curve_intersecting"][:, 0], t["caustic_curve_intersecting"][:, 1])
+    assert np.all(yc <= 2.2 * np.abs(r - te) + 1e-6 * te)
+    assert np.hypot(*t["caustic_center"]) <= 2 * res
+    assert t["caustic_area"] <= np.pi * (2 * res) ** 2
+    # the outline is drawn from the caustic points
+    assert len(t["caustic_curve_outline"]) >= 1
+    assert t["z_source"] == zs
