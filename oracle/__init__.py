@@ -3,6 +3,12 @@ oracle/liboracle.so, built from oracle/glamer_oracle.cpp by oracle/Makefile).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, as the checker.  The product never uses it.
+
+Parity unpinned (nothing the reference ships fixes them): the tie rule of the
+nearest-plane insertion for a halo exactly halfway between two planes
+(Utilities::closest is not in the reference tree; the first plane at the
+least |dDl| is taken), and the cosmological distances' integrator (the
+closed-form flat-LCDM integral stands in for CosmoLib's, which is absent).
 """
 import ctypes
 import os

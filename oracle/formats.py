@@ -7,6 +7,10 @@ Each function cites the reference code it follows (paths relative to the
 reference root).  Arithmetic keeps the reference's types: float32 products of
 float32 operands, float32 = float32 op float64 rounded once, sequential
 float64 sums (np.cumsum, not the pairwise np.sum).
+
+Parity unpinned: the exact header-card text cfitsio would write (cfitsio is
+absent, so no file written by it exists to compare with); the keys, values
+and pixel data are pinned.
 """
 import ctypes
 import struct
