@@ -1329,6 +1329,19 @@ void k_plane_step64(const StepJob* __restrict__ jobs, const WorkItem* __restrict
   plane_step_grid<SLSIM_FP64, false, false>(jobs, items, nitems, err);
 }
 
+// Planes with NFW-only halo-of-halos trees at 8 waves per SIMD as well: the
+// cap spills some of the halo walk's registers to scratch (95 -> 64 VGPRs)
+// and still gains (10,000-halo plane, 2048^2 FP64 grid: 76.7 -> 71.6 ms; the
+// screened 1,000-halo field plane: 53.0 -> 48.2 ms).  Trees with power-law
+// members (HK 2, about 120 VGPRs), the FP32C mode and the work-queue and fused
+// schedules keep the uncapped kernels.
+template <int MODE>
+__global__ __launch_bounds__(kWave * kWavesPerBlock) __attribute__((amdgpu_waves_per_eu(8)))
+void k_plane_step_nfw(const StepJob* __restrict__ jobs, const WorkItem* __restrict__ items,
+                      uint32_t nitems, uint32_t* __restrict__ err) {
+  plane_step_grid<MODE, false, 1>(jobs, items, nitems, err);
+}
+
 // ------------------------------------------------------- lone-ray FP64 walk
 // One plane step of one ray per wavefront, for a few scattered rays (the
 // precision form's guard), where a bundle walk would leave 63 lanes idle and
@@ -1911,6 +1924,8 @@ void launch_step_hk(int precision, bool count, unsigned nb, hipStream_t st, cons
   const dim3 blk(kWave * kWavesPerBlock);
   if (count)
     k_plane_step<SLSIM_FP64, true, HK><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
+  else if (HK == 1 && precision == SLSIM_FP64)
+    k_plane_step_nfw<SLSIM_FP64><<<nb, blk, 0, st>>>(jobs, items, nitems, err);
   else if (precision == SLSIM_FP32C && HK == 0)
     k_plane_step32<<<nb, blk, 0, st>>>(jobs, items, nitems, err);
   else if (precision == SLSIM_FP32C)
