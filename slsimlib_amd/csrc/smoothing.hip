@@ -22,6 +22,8 @@
 //      smallest of the buffer is found by bisection on the FP64 bit pattern
 //      (non-negative doubles order as their bits), counted with ballots.
 //      A full buffer is cut to its K smallest (the bound becomes the K-th).
+//      A query whose box stack would overflow LDS is listed and done again
+//      by a kernel whose stack (in global memory) holds every cell.
 //
 // Squared distances use the reference's arithmetic: per dimension the
 // difference in the coordinate type T, squared in FP64, summed x, y, z
@@ -46,6 +48,7 @@ constexpr int kKnnStack = 512;      // box stack entries per wavefront
 constexpr int kKnnLeafQ = 128;      // queued leaves per wavefront (<= 64 added a step, < 2 left)
 constexpr int kKnnBuf = 512;        // buffered squared distances per wavefront (8 slots of 64)
 constexpr int kKnnRun = 8;          // queries per wavefront
+constexpr int kKnnRetryWaves = 256; // wavefronts of the global-stack retry kernel
 constexpr int kMaxK = 120;          // K = Nsmooth + 1 limit (2K window: 4 slots of 64)
 
 inline unsigned nblk(size_t n, int b = kBlk) { return (unsigned)((n + b - 1) / b); }
@@ -209,131 +212,170 @@ __device__ inline void load_slots(const double* buf, int m, int lane, uint64_t (
   }
 }
 
+// One query, by a whole wavefront: the K-th smallest squared distance from
+// particle iq, the tree walked with the box stack `stk` of `cap` entries
+// (pk >= 0: the previous query's K-th at (px, py, pz) may give the bound).
+// Returns false, having written nothing, when the stack would overflow.
+template <typename T>
+__device__ bool knn_query(const T* __restrict__ xs, const T* __restrict__ ys,
+                          const T* __restrict__ zs, size_t n, int K, uint32_t L,
+                          const Box* __restrict__ box, double slack, size_t iq, double pk,
+                          double px, double py, double pz, uint32_t* stk, int cap,
+                          uint32_t* s_leaf, double* s_buf, double* kth_out) {
+  const int lane = (int)(threadIdx.x & 63);
+  const T qx = xs[iq], qy = ys[iq], qz = zs[iq];
+  const double qdx = (double)qx, qdy = (double)qy, qdz = (double)qz;
+  double bound;
+  const double ddx = qdx - px, ddy = qdy - py, ddz = qdz - pz;
+  const double dd = ddx * ddx + ddy * ddy + ddz * ddz;
+  if (pk >= 0 && 4 * dd <= pk) {
+    // the previous query's K nearest lie within sqrt(pk) + |q - p| of this
+    // one (taken when the step is at most half that radius)
+    const double r = sqrt(pk) + sqrt(dd);
+    bound = r * r * slack * slack;
+  } else {
+    // the K-th smallest over the 2K particles around the query in key order
+    size_t w0 = iq > (size_t)K ? iq - K : 0;
+    const size_t w1 = std::min(n, w0 + 2 * (size_t)K);
+    w0 = w1 > 2 * (size_t)K ? w1 - 2 * (size_t)K : 0;
+    uint64_t v[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const size_t j = w0 + 64 * s + lane;
+      const size_t jj = j < w1 ? j : w0;
+      const double r2 = r2_ref(xs[jj], ys[jj], zs[jj], qx, qy, qz);
+      v[s] = j < w1 ? (uint64_t)__double_as_longlong(r2) : ~0ull;
+    }
+    bound = __longlong_as_double((long long)kth_bits(v, (int)((w1 - w0 + 63) / 64), K));
+  }
+  // the tree: boxes within the bound, leaves scanned two at a time
+  bool strict = false;       // after a cut the buffer holds K values <= bound
+  int sp = 1, nleaf = 0, m = 0;
+  if (lane == 0) stk[0] = 1;
+  while (sp > 0 || nleaf > 0) {
+    if (sp > 0) {
+      const int nn = sp < 64 ? sp : 64;
+      __builtin_amdgcn_wave_barrier();
+      const uint32_t node = lane < nn ? stk[sp - nn + lane] : 0u;
+      __builtin_amdgcn_wave_barrier();
+      sp -= nn;
+      const bool take = lane < nn && box_d2(box[node], qdx, qdy, qdz) <= bound * slack;
+      const bool inner = take && node < L, leaf = take && node >= L;
+      const uint64_t bi = ballot64(inner), bl = ballot64(leaf);
+      const int ci = __builtin_popcountll(bi), cl = __builtin_popcountll(bl);
+      if (sp + 2 * ci > cap) return false;
+      if (inner) {
+        const int o = sp + 2 * lanes_below(bi);
+        stk[o] = 2 * node + 1;
+        stk[o + 1] = 2 * node;
+      }
+      if (leaf) s_leaf[nleaf + lanes_below(bl)] = node - L;
+      sp += 2 * ci;
+      nleaf += cl;
+    }
+    while (nleaf >= 2 || (sp == 0 && nleaf > 0)) {
+      const int take2 = nleaf >= 2 ? 2 : 1;
+      __builtin_amdgcn_wave_barrier();
+      const int half = lane >> 5;
+      const bool has = half < take2;
+      const uint32_t lf = has ? s_leaf[nleaf - 1 - half] : 0u;
+      __builtin_amdgcn_wave_barrier();
+      nleaf -= take2;
+      const size_t j = (size_t)lf * kLeaf + (lane & 31);
+      const bool valid = has && j < n;
+      const size_t jj = valid ? j : iq;
+      const double r2 = r2_ref(xs[jj], ys[jj], zs[jj], qx, qy, qz);
+      bool acc = valid && (strict ? r2 < bound : r2 <= bound);
+      uint64_t ba = ballot64(acc);
+      int ca = __builtin_popcountll(ba);
+      if (m + ca > kKnnBuf) {
+        // cut the buffer to its K smallest: those below the K-th, then
+        // copies of the K-th up to K values
+        uint64_t v[kKnnBuf / 64];
+        __builtin_amdgcn_wave_barrier();
+        load_slots(s_buf, m, lane, v);
+        __builtin_amdgcn_wave_barrier();
+        const uint64_t kb = kth_bits(v, (m + 63) / 64, K);
+        int base = 0;
+#pragma unroll
+        for (int s = 0; s < kKnnBuf / 64; ++s) {
+          const uint64_t b = ballot64(v[s] < kb);
+          if (v[s] < kb) s_buf[base + lanes_below(b)] = __longlong_as_double((long long)v[s]);
+          base += __builtin_popcountll(b);
+        }
+        bound = __longlong_as_double((long long)kb);
+        for (int j2 = base + lane; j2 < K; j2 += 64) s_buf[j2] = bound;
+        m = K;
+        strict = true;
+        acc = valid && r2 < bound;
+        ba = ballot64(acc);
+        ca = __builtin_popcountll(ba);
+      }
+      if (acc) s_buf[m + lanes_below(ba)] = r2;
+      m += ca;
+    }
+  }
+  uint64_t v[kKnnBuf / 64];
+  __builtin_amdgcn_wave_barrier();
+  load_slots(s_buf, m, lane, v);
+  __builtin_amdgcn_wave_barrier();
+  *kth_out = __longlong_as_double((long long)kth_bits(v, (m + 63) / 64, K));
+  return true;
+}
+
+// Runs of kKnnRun consecutive queries per wavefront, the box stack in LDS; a
+// query whose stack would overflow is listed for k_knn_retry.
 template <typename T>
 __global__ __launch_bounds__(64 * kKnnWaves) void k_knn(
     const T* __restrict__ xs, const T* __restrict__ ys, const T* __restrict__ zs,
     const uint32_t* __restrict__ perm, size_t n, int K, uint32_t L, const Box* __restrict__ box,
-    double slack, float* __restrict__ out, uint32_t* __restrict__ err) {
+    double slack, float* __restrict__ out, uint32_t* __restrict__ nretry,
+    uint32_t* __restrict__ retry) {
   __shared__ uint32_t s_stack_all[kKnnWaves][kKnnStack];
   __shared__ uint32_t s_leaf_all[kKnnWaves][kKnnLeafQ];
   __shared__ double s_buf_all[kKnnWaves][kKnnBuf];
   const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-  uint32_t* s_stack = s_stack_all[w];
-  uint32_t* s_leaf = s_leaf_all[w];
-  double* s_buf = s_buf_all[w];
   const size_t q0 = ((size_t)blockIdx.x * kKnnWaves + w) * kKnnRun;
   double pk = -1, px = 0, py = 0, pz = 0;   // the previous query's K-th and position
   for (size_t iq = q0; iq < q0 + kKnnRun && iq < n; ++iq) {
-    const T qx = xs[iq], qy = ys[iq], qz = zs[iq];
-    const double qdx = (double)qx, qdy = (double)qy, qdz = (double)qz;
-    double bound;
-    const double ddx = qdx - px, ddy = qdy - py, ddz = qdz - pz;
-    const double dd = ddx * ddx + ddy * ddy + ddz * ddz;
-    if (pk >= 0 && 4 * dd <= pk) {
-      // the previous query's K nearest lie within sqrt(pk) + |q - p| of this
-      // one (taken when the step is at most half that radius)
-      const double r = sqrt(pk) + sqrt(dd);
-      bound = r * r * slack * slack;
-    } else {
-      // the K-th smallest over the 2K particles around the query in key order
-      size_t w0 = iq > (size_t)K ? iq - K : 0;
-      const size_t w1 = std::min(n, w0 + 2 * (size_t)K);
-      w0 = w1 > 2 * (size_t)K ? w1 - 2 * (size_t)K : 0;
-      uint64_t v[4];
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const size_t j = w0 + 64 * s + lane;
-        const size_t jj = j < w1 ? j : w0;
-        const double r2 = r2_ref(xs[jj], ys[jj], zs[jj], qx, qy, qz);
-        v[s] = j < w1 ? (uint64_t)__double_as_longlong(r2) : ~0ull;
-      }
-      bound = __longlong_as_double((long long)kth_bits(v, (int)((w1 - w0 + 63) / 64), K));
-    }
-    // the tree: boxes within the bound, leaves scanned two at a time
-    bool strict = false;       // after a cut the buffer holds K values <= bound
-    int sp = 1, nleaf = 0, m = 0;
-    if (lane == 0) s_stack[0] = 1;
-    bool overflow = false;
-    while (sp > 0 || nleaf > 0) {
-      if (sp > 0) {
-        const int nn = sp < 64 ? sp : 64;
-        __builtin_amdgcn_wave_barrier();
-        const uint32_t node = lane < nn ? s_stack[sp - nn + lane] : 0u;
-        __builtin_amdgcn_wave_barrier();
-        sp -= nn;
-        const bool take = lane < nn && box_d2(box[node], qdx, qdy, qdz) <= bound * slack;
-        const bool inner = take && node < L, leaf = take && node >= L;
-        const uint64_t bi = ballot64(inner), bl = ballot64(leaf);
-        const int ci = __builtin_popcountll(bi), cl = __builtin_popcountll(bl);
-        if (sp + 2 * ci > kKnnStack) {
-          overflow = true;
-          break;
-        }
-        if (inner) {
-          const int o = sp + 2 * lanes_below(bi);
-          s_stack[o] = 2 * node + 1;
-          s_stack[o + 1] = 2 * node;
-        }
-        if (leaf) s_leaf[nleaf + lanes_below(bl)] = node - L;
-        sp += 2 * ci;
-        nleaf += cl;
-      }
-      while (nleaf >= 2 || (sp == 0 && nleaf > 0)) {
-        const int take2 = nleaf >= 2 ? 2 : 1;
-        __builtin_amdgcn_wave_barrier();
-        const int half = lane >> 5;
-        const bool has = half < take2;
-        const uint32_t lf = has ? s_leaf[nleaf - 1 - half] : 0u;
-        __builtin_amdgcn_wave_barrier();
-        nleaf -= take2;
-        const size_t j = (size_t)lf * kLeaf + (lane & 31);
-        const bool valid = has && j < n;
-        const size_t jj = valid ? j : iq;
-        const double r2 = r2_ref(xs[jj], ys[jj], zs[jj], qx, qy, qz);
-        bool acc = valid && (strict ? r2 < bound : r2 <= bound);
-        uint64_t ba = ballot64(acc);
-        int ca = __builtin_popcountll(ba);
-        if (m + ca > kKnnBuf) {
-          // cut the buffer to its K smallest: those below the K-th, then
-          // copies of the K-th up to K values
-          uint64_t v[kKnnBuf / 64];
-          __builtin_amdgcn_wave_barrier();
-          load_slots(s_buf, m, lane, v);
-          __builtin_amdgcn_wave_barrier();
-          const uint64_t kb = kth_bits(v, (m + 63) / 64, K);
-          int base = 0;
-#pragma unroll
-          for (int s = 0; s < kKnnBuf / 64; ++s) {
-            const uint64_t b = ballot64(v[s] < kb);
-            if (v[s] < kb) s_buf[base + lanes_below(b)] = __longlong_as_double((long long)v[s]);
-            base += __builtin_popcountll(b);
-          }
-          bound = __longlong_as_double((long long)kb);
-          for (int j2 = base + lane; j2 < K; j2 += 64) s_buf[j2] = bound;
-          m = K;
-          strict = true;
-          acc = valid && r2 < bound;
-          ba = ballot64(acc);
-          ca = __builtin_popcountll(ba);
-        }
-        if (acc) s_buf[m + lanes_below(ba)] = r2;
-        m += ca;
-      }
-    }
-    if (overflow) {
-      if (lane == 0) atomicOr(err, 1u);
+    double kth;
+    if (!knn_query(xs, ys, zs, n, K, L, box, slack, iq, pk, px, py, pz, s_stack_all[w],
+                   kKnnStack, s_leaf_all[w], s_buf_all[w], &kth)) {
+      if (lane == 0) retry[atomicAdd(nretry, 1u)] = (uint32_t)iq;
+      pk = -1;
       continue;
     }
-    uint64_t v[kKnnBuf / 64];
-    __builtin_amdgcn_wave_barrier();
-    load_slots(s_buf, m, lane, v);
-    __builtin_amdgcn_wave_barrier();
-    const double kth = __longlong_as_double((long long)kth_bits(v, (m + 63) / 64, K));
     if (lane == 0) out[perm[iq]] = (float)sqrt(kth);
     pk = kth;
-    px = qdx;
-    py = qdy;
-    pz = qdz;
+    px = (double)xs[iq];
+    py = (double)ys[iq];
+    pz = (double)zs[iq];
+  }
+}
+
+// The listed queries again, one wavefront each in turn over a fixed grid,
+// with a box stack in global memory of `cap` entries per wavefront: every
+// cell of the tree at most once, so it cannot overflow.
+template <typename T>
+__global__ __launch_bounds__(64) void k_knn_retry(
+    const T* __restrict__ xs, const T* __restrict__ ys, const T* __restrict__ zs,
+    const uint32_t* __restrict__ perm, size_t n, int K, uint32_t L, const Box* __restrict__ box,
+    double slack, float* __restrict__ out, const uint32_t* __restrict__ nretry,
+    const uint32_t* __restrict__ retry, uint32_t* __restrict__ gstack, int cap,
+    uint32_t* __restrict__ err) {
+  __shared__ uint32_t s_leaf[kKnnLeafQ];
+  __shared__ double s_buf[kKnnBuf];
+  uint32_t* stk = gstack + (size_t)blockIdx.x * cap;
+  const uint32_t nr = *nretry;
+  for (uint32_t i = blockIdx.x; i < nr; i += gridDim.x) {
+    const size_t iq = retry[i];
+    double kth;
+    if (!knn_query(xs, ys, zs, n, K, L, box, slack, iq, -1.0, 0.0, 0.0, 0.0, stk, cap, s_leaf,
+                   s_buf, &kth)) {
+      if (threadIdx.x == 0) atomicOr(err, 1u);
+      continue;
+    }
+    if (threadIdx.x == 0) out[perm[iq]] = (float)sqrt(kth);
   }
 }
 
@@ -415,9 +457,25 @@ int smoothing_sizes(slsim_ctx ctx, const T* x, const T* y, const T* z, size_t n,
   // queries
   SLSIM_HIP_CHECK(hipMemsetAsync(err, 0, 4, st));
   const double slack = sizeof(T) == 4 ? 1.0 + 1e-6 : 1.0 + 1e-12;
+  uint32_t *nretry, *retry;
+  if ((rc = get((void**)&nretry, 4)) || (rc = get((void**)&retry, n * 4))) return rc;
+  SLSIM_HIP_CHECK(hipMemsetAsync(nretry, 0, 4, st));
   k_knn<T><<<nblk(n, kKnnWaves * kKnnRun), 64 * kKnnWaves, 0, st>>>(
-      sx, sy, sz, perm, n, K, L, box, slack, dout, err);
+      sx, sy, sz, perm, n, K, L, box, slack, dout, nretry, retry);
   SLSIM_HIP_CHECK(hipGetLastError());
+  uint32_t hretry = 0;
+  SLSIM_HIP_CHECK(hipMemcpyAsync(&hretry, nretry, 4, hipMemcpyDeviceToHost, st));
+  SLSIM_HIP_CHECK(hipStreamSynchronize(st));
+  if (hretry > 0) {
+    // the queries whose LDS stack would overflow: a stack of every cell
+    const int cap = (int)(2 * (size_t)L);
+    const unsigned nw = std::min<uint32_t>(hretry, kKnnRetryWaves);
+    uint32_t* gstack;
+    if ((rc = get((void**)&gstack, (size_t)nw * cap * 4))) return rc;
+    k_knn_retry<T><<<nw, 64, 0, st>>>(sx, sy, sz, perm, n, K, L, box, slack, dout, nretry, retry,
+                                     gstack, cap, err);
+    SLSIM_HIP_CHECK(hipGetLastError());
+  }
   uint32_t herr = 0;
   SLSIM_HIP_CHECK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st));
   SLSIM_HIP_CHECK(hipMemcpyAsync(out_host, dout, n * 4, hipMemcpyDeviceToHost, st));

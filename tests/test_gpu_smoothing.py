@@ -62,3 +62,18 @@ def test_smoothing_errors(ctx):
     assert e.value.code == 1
     with pytest.raises(S.SlsimError):
         S.LensHaloParticles.calculate_smoothing(ctx, 120, *np.random.default_rng(6).normal(size=(3, 500)))
+
+
+def test_smoothing_dense_cluster_with_outliers(ctx):
+    """A dense cluster with sparse outliers: an outlier's neighbour sphere
+    takes in the whole cluster's subtree, so its box stack outgrows the
+    in-LDS one (about 520 entries at this size) and the query is done again
+    with a stack in global memory.  Outliers and cluster particles against
+    the brute-force definition."""
+    rng = np.random.default_rng(11)
+    nc, no = 400000, 300
+    x, y, z = np.concatenate([rng.normal(0, 0.01, (3, nc)), rng.uniform(-1, 1, (3, no))], 1)
+    g = S.LensHaloParticles.calculate_smoothing(ctx, 64, x, y, z)
+    which = np.concatenate([np.arange(nc, nc + no), rng.choice(nc, 200, replace=False)])
+    o = O.smoothing_sizes(x, y, z, 64, brute=True, which=which)
+    assert np.array_equal(g[which], o)
