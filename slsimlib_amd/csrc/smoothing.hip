@@ -48,7 +48,7 @@ constexpr int kKnnStack = 512;      // box stack entries per wavefront
 constexpr int kKnnLeafQ = 128;      // queued leaves per wavefront (<= 64 added a step, < 2 left)
 constexpr int kKnnBuf = 512;        // buffered squared distances per wavefront (8 slots of 64)
 constexpr int kKnnRun = 8;          // queries per wavefront
-constexpr int kKnnRetryWaves = 256; // wavefronts of the global-stack retry kernel
+constexpr int kKnnRetryWaves = 8;   // wavefronts of the global-stack retry kernel
 constexpr int kMaxK = 120;          // K = Nsmooth + 1 limit (2K window: 4 slots of 64)
 
 inline unsigned nblk(size_t n, int b = kBlk) { return (unsigned)((n + b - 1) / b); }
@@ -409,9 +409,10 @@ int smoothing_sizes(slsim_ctx ctx, const T* x, const T* y, const T* z, size_t n,
   if ((rc = get((void**)&dx, n * sizeof(T))) || (rc = get((void**)&dy, n * sizeof(T))) ||
       (rc = get((void**)&dz, n * sizeof(T))) || (rc = get((void**)&sx, n * sizeof(T))) ||
       (rc = get((void**)&sy, n * sizeof(T))) || (rc = get((void**)&sz, n * sizeof(T))) ||
-      (rc = get((void**)&keys, n * 8)) || (rc = get((void**)&keys_s, n * 8)) ||
+      (rc = get((void**)&keys, std::max<size_t>(n, 64) * 8)) ||
+      (rc = get((void**)&keys_s, std::max<size_t>(n, 64) * 8)) ||
       (rc = get((void**)&idx, n * 4)) || (rc = get((void**)&perm, n * 4)) ||
-      (rc = get((void**)&err, 4)) || (rc = get((void**)&dout, n * 4)))
+      (rc = get((void**)&err, 8)) || (rc = get((void**)&dout, n * 4)))
     return rc;
   SLSIM_HIP_CHECK(hipMemcpyAsync(dx, x, n * sizeof(T), hipMemcpyHostToDevice, st));
   SLSIM_HIP_CHECK(hipMemcpyAsync(dy, y, n * sizeof(T), hipMemcpyHostToDevice, st));
@@ -455,27 +456,22 @@ int smoothing_sizes(slsim_ctx ctx, const T* x, const T* y, const T* z, size_t n,
   for (uint32_t first = L / 2; first >= 1; first /= 2)
     k_merge_boxes<<<nblk(first), kBlk, 0, st>>>(first, first, box);
   // queries
-  SLSIM_HIP_CHECK(hipMemsetAsync(err, 0, 4, st));
+  SLSIM_HIP_CHECK(hipMemsetAsync(err, 0, 8, st));   // the error flag, the retry count
   const double slack = sizeof(T) == 4 ? 1.0 + 1e-6 : 1.0 + 1e-12;
-  uint32_t *nretry, *retry;
-  if ((rc = get((void**)&nretry, 4)) || (rc = get((void**)&retry, n * 4))) return rc;
-  SLSIM_HIP_CHECK(hipMemsetAsync(nretry, 0, 4, st));
+  // the retry list and the retry kernel's stacks reuse the key arrays (dead
+  // once the particles are in key order): n and 2 kKnnRetryWaves L <= n + 32
+  // entries of 2 max(n, 64)
+  uint32_t* nretry = err + 1;
+  uint32_t* retry = reinterpret_cast<uint32_t*>(keys);
+  uint32_t* gstack = reinterpret_cast<uint32_t*>(keys_s);
   k_knn<T><<<nblk(n, kKnnWaves * kKnnRun), 64 * kKnnWaves, 0, st>>>(
       sx, sy, sz, perm, n, K, L, box, slack, dout, nretry, retry);
   SLSIM_HIP_CHECK(hipGetLastError());
-  uint32_t hretry = 0;
-  SLSIM_HIP_CHECK(hipMemcpyAsync(&hretry, nretry, 4, hipMemcpyDeviceToHost, st));
-  SLSIM_HIP_CHECK(hipStreamSynchronize(st));
-  if (hretry > 0) {
-    // the queries whose LDS stack would overflow: a stack of every cell
-    const int cap = (int)(2 * (size_t)L);
-    const unsigned nw = std::min<uint32_t>(hretry, kKnnRetryWaves);
-    uint32_t* gstack;
-    if ((rc = get((void**)&gstack, (size_t)nw * cap * 4))) return rc;
-    k_knn_retry<T><<<nw, 64, 0, st>>>(sx, sy, sz, perm, n, K, L, box, slack, dout, nretry, retry,
-                                     gstack, cap, err);
-    SLSIM_HIP_CHECK(hipGetLastError());
-  }
+  // the queries whose LDS stack would overflow (usually none: the kernel
+  // reads the count and exits), with a stack of every cell
+  k_knn_retry<T><<<kKnnRetryWaves, 64, 0, st>>>(sx, sy, sz, perm, n, K, L, box, slack, dout,
+                                               nretry, retry, gstack, (int)(2 * L), err);
+  SLSIM_HIP_CHECK(hipGetLastError());
   uint32_t herr = 0;
   SLSIM_HIP_CHECK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st));
   SLSIM_HIP_CHECK(hipMemcpyAsync(out_host, dout, n * 4, hipMemcpyDeviceToHost, st));
